@@ -1,0 +1,167 @@
+/* vistio_hip.h — C ABI of the MI355X (gfx950) Projective Dynamics step.
+ *
+ * Plain C types only (no torch, no HIP types). One context = one HIP stream
+ * on one device; a context is not re-entrant (the Rust side holds it behind
+ * `&mut self`). All host buffers are caller-owned; the context owns device
+ * memory. No exception or panic crosses this boundary: every entry point
+ * returns a vh_status and vh_last_error() gives the message.
+ *
+ * Reference interfaces each entry point replaces (file:line in the Vistio
+ * tree; the Rust-side binding is shown in INTEGRATION.md):
+ *   SolverStrategy            crates/vistio-solver/src/strategy.rs:42-69
+ *   ProjectiveDynamicsSolver  crates/vistio-solver/src/pd_solver.rs:53-476
+ *   GpuBackend                crates/vistio-gpu/src/backend.rs:19-46
+ *   VistioError               crates/vistio-types/src/error.rs:9-44
+ */
+#ifndef VISTIO_HIP_H
+#define VISTIO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. Mapping to VistioError (crates/vistio-types/src/error.rs):
+ *   VH_E_INVALID_CONFIG, VH_E_NOT_INITIALIZED, VH_E_NOT_SPD, VH_E_UNSUPPORTED
+ *                         -> VistioError::InvalidConfig
+ *   VH_E_INVALID_MESH     -> VistioError::InvalidMesh
+ *   VH_E_GPU, VH_E_INTERNAL -> VistioError::Gpu
+ *   VH_E_INVALID_ARG      -> VistioError::InvalidConfig (null pointer / bad size)
+ *   VH_E_NONFINITE        -> VistioError::InvalidConfig (NaN/Inf in the state) */
+typedef enum {
+  VH_OK = 0,
+  VH_E_INVALID_CONFIG = 1,
+  VH_E_NOT_INITIALIZED = 2, /* pd_solver.rs:264-268 "Solver not initialized" */
+  VH_E_NOT_SPD = 3,         /* pd_solver.rs:135-137 "Cholesky factorization failed" */
+  VH_E_UNSUPPORTED = 4,     /* unknown ConstitutiveModel::name() / bending kind */
+  VH_E_INVALID_MESH = 5,    /* state.rs:92-105 length mismatch, bad indices */
+  VH_E_GPU = 6,             /* any HIP runtime error */
+  VH_E_INTERNAL = 7,
+  VH_E_INVALID_ARG = 8,
+  VH_E_NONFINITE = 9
+} vh_status;
+
+/* Membrane projection, keyed by ConstitutiveModel::name():
+ *   VH_MODEL_ARAP            ElementData::project (element.rs:127-144), used by init()
+ *   VH_MODEL_COROTATIONAL    "co_rotational" (crates/vistio-material/src/corotational.rs:51-74)
+ * Other models ("isotropic_linear", "orthotropic_linear",
+ * "anisotropic_corotational") return VH_E_UNSUPPORTED. */
+enum { VH_MODEL_ARAP = 0, VH_MODEL_COROTATIONAL = 1 };
+/* Bending: dihedral springs (crates/vistio-solver/src/bending.rs). */
+enum { VH_BENDING_NONE = 0, VH_BENDING_DIHEDRAL = 1 };
+/* Precision of the local step and the triangular solves. State (positions,
+ * velocities) is always held in FP64 on the device.
+ *   VH_PREC_F64     FP64 local step, FP64 solve.
+ *   VH_PREC_F32_IR  FP32 local step (delta form), FP32 solve, FP64 residual
+ *                   r = b - A x with iterative refinement. */
+enum { VH_PREC_F64 = 0, VH_PREC_F32_IR = 1 };
+
+/* SolverConfig (crates/vistio-solver/src/config.rs:10-129) fields on the
+ * path, plus the material values init_with_material derives
+ * (element.rs:106 -> k_membrane, bending.rs:160 -> k_bending,
+ * pd_solver.rs:1221 -> density) and the HIP-specific knobs. */
+typedef struct {
+  int32_t model;            /* VH_MODEL_* */
+  int32_t bending;          /* VH_BENDING_* */
+  double k_membrane;        /* element weight = k_membrane * rest area */
+  double k_bending;         /* hinge weight = k_bending * rest edge length */
+  double density;           /* kg/m^2 */
+  double dt_matrix;         /* dt in A = M/dt^2 + ... (the reference fixes 1/60) */
+  double gravity[3];
+  double damping;           /* v *= 1 - damping */
+  double rayleigh;          /* free vertices: v *= 1/(1 + rayleigh*dt) */
+  int32_t max_iterations;
+  double tolerance;         /* early exit when the relative update < tolerance */
+  int32_t has_ground;
+  double ground_height;
+  int32_t precision;        /* VH_PREC_* */
+  int32_t ir_sweeps;        /* VH_PREC_F32_IR: refinement sweeps per PD iteration (1..3) */
+  int32_t ir_final_sweeps;  /* VH_PREC_F32_IR: sweeps on the last iteration (1..3) */
+  int32_t leaf_size;        /* nested-dissection leaf size; 0 = default (16) */
+} vh_pd_params;
+
+/* StepResult, crates/vistio-solver/src/strategy.rs:15-24. */
+typedef struct {
+  uint32_t iterations;
+  double final_residual;
+  int32_t converged;
+  double wall_time; /* seconds, host clock around the step(s) */
+} vh_step_result;
+
+/* SimulationState SoA buffers (crates/vistio-solver/src/state.rs:23-64). */
+typedef struct {
+  float* pos_x; float* pos_y; float* pos_z;
+  float* vel_x; float* vel_y; float* vel_z;
+  float* prev_x; float* prev_y; float* prev_z;
+  float* pred_x; float* pred_y; float* pred_z;
+  const float* inv_mass; /* pinned vertices have inv_mass == 0 */
+} vh_state_f32;
+
+typedef struct {
+  uint32_t n, t, h;
+  uint32_t supernodes, levels;
+  int64_t nnz_a;      /* stored entries of A (full symmetric) */
+  int64_t nnz_factor; /* entries of all solve blocks M_P (= supernodal L) */
+  int32_t max_front;  /* max (w + r) over supernodes */
+  double setup_seconds;
+} vh_pd_info;
+
+typedef struct vh_ctx vh_ctx;
+
+/* Context lifetime. */
+vh_status vh_create(int32_t device, vh_ctx** out);
+vh_status vh_destroy(vh_ctx* ctx);
+const char* vh_last_error(const vh_ctx* ctx);
+
+/* ProjectiveDynamicsSolver::init / init_with_material (pd_solver.rs:106-143,
+ * 221-257): rest data, lumped masses, system matrix, factorization, device
+ * upload. hinge holds 4h indices (v0, v1, wing_a, wing_b) of the interior
+ * edges (crates/vistio-mesh/src/topology.rs:36-51). lumped_mass_out (n, may
+ * be NULL) receives lumped_masses() (pd_solver.rs:95-97). The device state is
+ * initialised from px/py/pz with zero velocity (state.rs:119-139). */
+vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz,
+                     uint32_t t, const uint32_t* tri, uint32_t h, const uint32_t* hinge,
+                     const uint8_t* pinned, const vh_pd_params* params, float* lumped_mass_out);
+
+/* SolverStrategy::step (pd_solver.rs:259-476) on host SoA state: uploads
+ * pos/vel and inv_mass, runs one step on the device, downloads
+ * pos/vel/prev/pred. */
+vh_status vh_pd_step(vh_ctx* ctx, double dt, vh_state_f32* state, vh_step_result* out);
+
+/* Resident mode: the state stays on the device between calls. */
+vh_status vh_pd_step_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_result* out);
+/* which: 0 = pos, 1 = vel, 2 = prev, 3 = pred. Arrays are SoA x[n], y[n], z[n]
+ * in the caller's vertex order. */
+vh_status vh_state_set_f64(vh_ctx* ctx, int32_t which, const double* x, const double* y, const double* z);
+vh_status vh_state_get_f64(vh_ctx* ctx, int32_t which, double* x, double* y, double* z);
+/* Pinning flags from SimulationState::inv_mass (inv_mass[i] == 0 -> pinned). */
+vh_status vh_state_set_inv_mass(vh_ctx* ctx, const float* inv_mass);
+
+/* Parity / inspection entry points. */
+/* r = b - A x at the current device state (pred as set): the local step and
+ * RHS assembly (pd_solver.rs:296-389, assembly.rs:157-240) in residual form.
+ * out: SoA 3n, caller's vertex order. dt as in step. */
+vh_status vh_pd_residual(vh_ctx* ctx, double dt, double* out);
+/* x = A^{-1} b with the device solve (FaerSolver::solve x3,
+ * crates/vistio-math/src/faer_solver.rs:95-129), SoA 3n in the caller's order.
+ * In VH_PREC_F32_IR mode runs `sweeps` refinement sweeps (>= 1). */
+vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps);
+vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out);
+vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out);
+
+/* Mesh helpers (crates/vistio-mesh): quad_grid (generators.rs:28-100) and the
+ * interior edges of Topology::build (topology.rs:58-120, :158-168) in a
+ * canonical (v0, v1)-sorted order. */
+vh_status vh_mesh_quad_grid(uint32_t cols, uint32_t rows, float width, float height,
+                            float* px, float* py, float* pz, uint32_t* indices);
+/* Writes up to cap interior edges (6 u32 each: v0, v1, wing_a, wing_b, tri_a,
+ * tri_b) and their count. Call with cap = 0 to query the count. */
+vh_status vh_mesh_interior_edges(uint32_t n, uint32_t t, const uint32_t* indices, uint32_t* out,
+                                 uint32_t cap, uint32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISTIO_HIP_H */

@@ -1,0 +1,102 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle."""
+import numpy as np
+import pytest
+
+from conftest import DT, gpu_solver, max_vertex_err, oracle_solver, sheet
+
+pytestmark = pytest.mark.gpu
+
+
+def _perturbed_state(px, py, pz, seed, amp=2e-3):
+    rng = np.random.default_rng(seed)
+    x = np.stack([px, py, pz]).astype(np.float64)
+    x = x + amp * rng.standard_normal(x.shape) * np.array([[1.0], [1.0], [0.5]])
+    pred = x + 1e-3 * rng.standard_normal(x.shape)
+    return x, pred
+
+
+@pytest.mark.parametrize("c", [8, 20, 64])
+def test_solve_f64_matches_oracle(c):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin)
+    b = np.random.default_rng(c).standard_normal((3, px.size))
+    xo = np.stack([o.solve(b[k]) for k in range(3)])
+    xg = g.solve(b)
+    rel = np.abs(xg - xo).max() / np.abs(xo).max()
+    print(f"c={c} solve f64 rel err {rel:.3e}", flush=True)
+    assert rel < 1e-11
+    A = o.matrix()
+    res = max(np.abs(A @ xg[k] - b[k]).max() for k in range(3)) / np.abs(b).max()
+    assert res < 1e-11
+
+
+@pytest.mark.parametrize("c", [8, 20])
+def test_solve_f32_ir_converges(c):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin, precision="f32_ir")
+    b = np.random.default_rng(c).standard_normal((3, px.size))
+    xo = np.stack([o.solve(b[k]) for k in range(3)])
+    errs = []
+    for sweeps in (1, 2, 3):
+        xg = g.solve(b, sweeps=sweeps)
+        errs.append(np.abs(xg - xo).max() / np.abs(xo).max())
+    print(f"c={c} f32 IR rel err by sweeps {errs}", flush=True)
+    assert errs[0] < 1e-3
+    assert errs[2] < 1e-10
+    assert errs[1] < errs[0] and errs[2] < errs[1]
+
+
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-11), ("f32_ir", 1e-6)])
+def test_residual_matches_oracle(precision, tol):
+    px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin, precision=precision)
+    x, pred = _perturbed_state(px, py, pz, 7)
+    o.set(0, x); o.set(3, pred)
+    g.set_state(0, x); g.set_state(3, pred)
+    b, _, _ = o.local_rhs()
+    A = o.matrix()
+    r_o = np.stack([b[k] - A @ x[k] for k in range(3)])
+    r_g = g.residual(DT)
+    scale = np.abs(b).max()
+    rel = np.abs(r_g - r_o).max() / scale
+    print(f"{precision} residual rel err {rel:.3e} (|b| {scale:.3e})", flush=True)
+    assert rel < tol
+
+
+@pytest.mark.parametrize("c,nsteps", [(20, 120)])
+def test_step_f64_matches_oracle(c, nsteps):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin)
+    o.step(DT, nsteps)
+    r = g.step_resident(DT, nsteps)
+    err = max_vertex_err(g.positions(), o.positions())
+    print(f"c={c} f64 steps={nsteps} max vertex err {err:.3e} m, iters {r.iterations}", flush=True)
+    assert r.iterations == 10
+    assert err < 1e-9
+
+
+def test_step_f64_rotated_sheet_matches_oracle():
+    px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin)
+    o.step(DT, 60)
+    g.step_resident(DT, 60)
+    err = max_vertex_err(g.positions(), o.positions())
+    print(f"rotated sheet f64 60 steps max vertex err {err:.3e} m", flush=True)
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("c", [20])
+def test_step_f32_ir_within_1e6(c):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g = gpu_solver(px, py, pz, idx, hg, pin, precision="f32_ir", ir_sweeps=2, ir_final_sweeps=3)
+    o.step(DT, 120)
+    g.step_resident(DT, 120)
+    err = max_vertex_err(g.positions(), o.positions())
+    print(f"c={c} f32_ir 120 steps max vertex err {err:.3e} m", flush=True)
+    assert err < 1e-6

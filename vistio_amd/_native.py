@@ -1,0 +1,102 @@
+"""ctypes binding of the C ABI in include/vistio_hip.h (libvistio_hip.so).
+
+The library is built in-tree (``__graft_entry__.build()`` or
+``make -C vistio_amd/csrc``). Loading fails loudly when it is missing: there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvistio_hip.so")
+
+VH_OK = 0
+STATUS_NAMES = {
+    0: "VH_OK", 1: "VH_E_INVALID_CONFIG", 2: "VH_E_NOT_INITIALIZED", 3: "VH_E_NOT_SPD",
+    4: "VH_E_UNSUPPORTED", 5: "VH_E_INVALID_MESH", 6: "VH_E_GPU", 7: "VH_E_INTERNAL",
+    8: "VH_E_INVALID_ARG", 9: "VH_E_NONFINITE",
+}
+MODEL_ARAP, MODEL_COROTATIONAL = 0, 1
+BENDING_NONE, BENDING_DIHEDRAL = 0, 1
+PREC_F64, PREC_F32_IR = 0, 1
+
+# Every symbol include/vistio_hip.h declares.
+EXPORTS = (
+    "vh_create", "vh_destroy", "vh_last_error", "vh_pd_init", "vh_pd_step", "vh_pd_step_resident",
+    "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
+    "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges",
+)
+
+
+class PdParams(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32), ("bending", C.c_int32),
+        ("k_membrane", C.c_double), ("k_bending", C.c_double), ("density", C.c_double),
+        ("dt_matrix", C.c_double), ("gravity", C.c_double * 3), ("damping", C.c_double),
+        ("rayleigh", C.c_double), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
+        ("has_ground", C.c_int32), ("ground_height", C.c_double), ("precision", C.c_int32),
+        ("ir_sweeps", C.c_int32), ("ir_final_sweeps", C.c_int32), ("leaf_size", C.c_int32),
+    ]
+
+
+class StepResultC(C.Structure):
+    _fields_ = [("iterations", C.c_uint32), ("final_residual", C.c_double),
+                ("converged", C.c_int32), ("wall_time", C.c_double)]
+
+
+class StateF32(C.Structure):
+    _fields_ = [(nm, C.POINTER(C.c_float)) for nm in (
+        "pos_x", "pos_y", "pos_z", "vel_x", "vel_y", "vel_z",
+        "prev_x", "prev_y", "prev_z", "pred_x", "pred_y", "pred_z")] + [("inv_mass", C.POINTER(C.c_float))]
+
+
+class PdInfo(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("t", C.c_uint32), ("h", C.c_uint32), ("supernodes", C.c_uint32),
+                ("levels", C.c_uint32), ("nnz_a", C.c_int64), ("nnz_factor", C.c_int64),
+                ("max_front", C.c_int32), ("setup_seconds", C.c_double)]
+
+
+_lib = None
+
+
+def load():
+    """Load libvistio_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                          "or `make -C vistio_amd/csrc` (the HIP path has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    dp, fp, up = C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_uint32)
+    sig = {
+        "vh_create": ([C.c_int32, C.POINTER(vp)], C.c_int),
+        "vh_destroy": ([vp], C.c_int),
+        "vh_last_error": ([vp], C.c_char_p),
+        "vh_pd_init": ([vp, C.c_uint32, fp, fp, fp, C.c_uint32, up, C.c_uint32, up, C.POINTER(C.c_uint8),
+                        C.POINTER(PdParams), fp], C.c_int),
+        "vh_pd_step": ([vp, C.c_double, C.POINTER(StateF32), C.POINTER(StepResultC)], C.c_int),
+        "vh_pd_step_resident": ([vp, C.c_double, C.c_uint32, C.POINTER(StepResultC)], C.c_int),
+        "vh_state_set_f64": ([vp, C.c_int32, dp, dp, dp], C.c_int),
+        "vh_state_get_f64": ([vp, C.c_int32, dp, dp, dp], C.c_int),
+        "vh_state_set_inv_mass": ([vp, fp], C.c_int),
+        "vh_pd_residual": ([vp, C.c_double, dp], C.c_int),
+        "vh_pd_solve": ([vp, dp, dp, C.c_int32], C.c_int),
+        "vh_pd_info_get": ([vp, C.POINTER(PdInfo)], C.c_int),
+        "vh_pd_lumped_masses": ([vp, fp], C.c_int),
+        "vh_mesh_quad_grid": ([C.c_uint32, C.c_uint32, C.c_float, C.c_float, fp, fp, fp, up], C.c_int),
+        "vh_mesh_interior_edges": ([C.c_uint32, C.c_uint32, up, up, C.c_uint32, up], C.c_int),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,606 @@
+// HIP kernels for the gfx950 Projective Dynamics step.
+//
+// Data layout (all per-vertex arrays in the nested-dissection order chosen
+// at set-up, so the solve reads contiguous runs):
+//   x, v, prev, pred, r : double [n][3]   (state is FP64 in both precisions)
+//   ctri                : T [t][3][3]     membrane contribution per triangle corner
+//   chinge              : T [h][3]        w_b * s^T(target - x) per hinge
+//   y, d                : T [n][3]        forward result / solution of A d = r
+//   U                   : T [sum r][3]    forward update vectors per supernode
+//   M                   : T               per-supernode blocks [L^{-1}; L_IP L^{-1}]
+//
+// The local step is written in residual form: each element emits
+// w * G^T (F_target - F) (membrane) or w_b * s (s^T target - s^T x) (bending),
+// so the gather produces r = b - A x directly (assembly.rs:157-240 minus A x).
+// In exact arithmetic this equals the reference's rhs minus A x; x + A^{-1} r
+// is the reference's solve result (pd_solver.rs:381-400).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace vhk {
+
+struct SnDev {
+  int col0, w, r, rows_off, u_off, map_off, child_off, nchild;
+  long long m_off;
+};
+
+struct StepDev {
+  int done;         // set when the relative update fell below the tolerance
+  int iterations;   // PD iterations run in the current step
+  double residual;  // final_residual of the current step
+  unsigned counter; // arrival counter of the commit kernel's last-block reduce
+  int pad;
+};
+
+__device__ __forceinline__ double3 ld3(const double* p, int i) {
+  return make_double3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+}
+__device__ __forceinline__ double3 operator+(double3 a, double3 b) { return make_double3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ double3 operator-(double3 a, double3 b) { return make_double3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ double3 operator*(double3 a, double s) { return make_double3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double3 ddiv(double3 a, double s) { return make_double3(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ double ddot(double3 a, double3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ double3 dcross(double3 a, double3 b) {
+  return make_double3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double dlen(double3 a) { return sqrt(ddot(a, a)); }
+
+__device__ __forceinline__ float3 f3(double3 a) { return make_float3((float)a.x, (float)a.y, (float)a.z); }
+__device__ __forceinline__ float3 operator+(float3 a, float3 b) { return make_float3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 operator-(float3 a, float3 b) { return make_float3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 operator*(float3 a, float s) { return make_float3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float fdot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float3 fcross(float3 a, float3 b) {
+  return make_float3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float flen(float3 a) { return sqrtf(fdot(a, a)); }
+
+// ---------------------------------------------------------------------------
+// Membrane projection, FP64, following the reference formulas:
+// deformation_gradient (decomposition.rs:147-155), polar_decomposition_3x2
+// (decomposition.rs:76-138), clamp_stretch (decomposition.rs:50-66),
+// CoRotationalModel::project (corotational.rs:51-72), ARAP target = R
+// (element.rs:127-144), degenerate Dm^{-1} -> unchanged positions
+// (element.rs:192-202). Returns D = F_target - F.
+// ---------------------------------------------------------------------------
+__device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, const double* dm, int corot,
+                                          double3& D0, double3& D1) {
+  const double3 s0v = p1 - p0, s1v = p2 - p0;
+  const double3 F0 = s0v * dm[0] + s1v * dm[1];
+  const double3 F1 = s0v * dm[2] + s1v * dm[3];
+  D0 = make_double3(0, 0, 0);
+  D1 = make_double3(0, 0, 0);
+  const double det = dm[0] * dm[3] - dm[1] * dm[2];
+  if (!(fabs(det) > 1e-12)) return;
+  const double a = ddot(F0, F0), b = ddot(F0, F1), d = ddot(F1, F1);
+  const double ht = 0.5 * (a + d), hd = 0.5 * (a - d);
+  const double disc = sqrt(hd * hd + b * b);
+  const double l0 = fmax(ht + disc, 0.0), l1 = fmax(ht - disc, 0.0);
+  const double s0 = sqrt(l0), s1 = sqrt(l1);
+  const double eps = 1e-10;
+  double3 R0, R1;
+  double v0x, v0y, v1x, v1y;
+  if (s0 < eps) {
+    R0 = make_double3(1, 0, 0); R1 = make_double3(0, 1, 0);
+    v0x = 1; v0y = 0; v1x = 0; v1y = 1;
+    if (corot) {  // stretch 0, singular values 0 -> clamped target 0
+      D0 = make_double3(-F0.x, -F0.y, -F0.z);
+      D1 = make_double3(-F1.x, -F1.y, -F1.z);
+    } else {
+      D0 = R0 - F0; D1 = R1 - F1;
+    }
+    return;
+  }
+  if (fabs(b) > eps) {
+    const double ex = l0 - d, ey = b;
+    const double rl = 1.0 / sqrt(ex * ex + ey * ey);
+    v0x = ex * rl; v0y = ey * rl;
+    v1x = -v0y; v1y = v0x;
+  } else if (a >= d) {
+    v0x = 1; v0y = 0; v1x = 0; v1y = 1;
+  } else {
+    v0x = 0; v0y = 1; v1x = 1; v1y = 0;
+  }
+  const double is0 = 1.0 / s0, is1 = s1 > eps ? 1.0 / s1 : 0.0;
+  const double si00 = v0x * v0x * is0 + v1x * v1x * is1;
+  const double si01 = v0x * v0y * is0 + v1x * v1y * is1;
+  const double si11 = v0y * v0y * is0 + v1y * v1y * is1;
+  R0 = F0 * si00 + F1 * si01;
+  R1 = F0 * si01 + F1 * si11;
+  double3 T0 = R0, T1 = R1;
+  if (corot) {
+    const double c0 = fmin(s0, 1.0), c1 = fmin(s1, 1.0);
+    const double t00 = v0x * v0x * c0 + v1x * v1x * c1;
+    const double t01 = v0x * v0y * c0 + v1x * v1y * c1;
+    const double t11 = v0y * v0y * c0 + v1y * v1y * c1;
+    T0 = R0 * t00 + R1 * t01;
+    T1 = R0 * t01 + R1 * t11;
+  }
+  D0 = T0 - F0;
+  D1 = T1 - F1;
+}
+
+// Membrane projection, FP32 residual form. F and E = F^T F - I are formed in
+// FP64 from FP64 positions; the eigen-decomposition runs in FP32 on E, so
+// sigma - 1 = mu / (1 + sigma) keeps the strain's relative accuracy, and
+//   D = F_target - F = F V diag(d_i / sigma_i) V^T,
+//   d_i = min(sigma_i, 1) - sigma_i (co-rotational) or 1 - sigma_i (ARAP),
+// which equals R S_clamped - F (resp. R - F) in exact arithmetic. The
+// eigenvector uses whichever of (mu0 - e_d, b) and (b, mu0 - e_a) avoids
+// cancellation (both are the reference's (lambda0 - d, b) direction).
+// Degenerate singular values fall back to the FP64 path.
+__device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, const double* dm, int corot,
+                                          float3& D0, float3& D1) {
+  const double3 s0v = p1 - p0, s1v = p2 - p0;
+  const double3 F0 = s0v * dm[0] + s1v * dm[1];
+  const double3 F1 = s0v * dm[2] + s1v * dm[3];
+  D0 = make_float3(0, 0, 0);
+  D1 = make_float3(0, 0, 0);
+  const double det = dm[0] * dm[3] - dm[1] * dm[2];
+  if (!(fabs(det) > 1e-12)) return;
+  const float ea = (float)(ddot(F0, F0) - 1.0);
+  const float eb = (float)ddot(F0, F1);
+  const float ed = (float)(ddot(F1, F1) - 1.0);
+  const float hd = 0.5f * (ea - ed), ht = 0.5f * (ea + ed);
+  const float disc = sqrtf(hd * hd + eb * eb);
+  const float mu0 = ht + disc, mu1 = ht - disc;
+  const float s0 = sqrtf(fmaxf(1.0f + mu0, 0.0f)), s1 = sqrtf(fmaxf(1.0f + mu1, 0.0f));
+  // Small sigma_1 (strong compression): 1 + mu1 loses its relative accuracy
+  // in FP32, so such elements take the FP64 path.
+  if (s0 < 1e-10f || s1 < 0.25f) {
+    double3 E0, E1;
+    membrane_delta_f64(p0, p1, p2, dm, corot, E0, E1);
+    D0 = f3(E0); D1 = f3(E1);
+    return;
+  }
+  float v0x, v0y;
+  if (fabsf(eb) > 1e-10f) {
+    float ex, ey;
+    if (hd >= 0.0f) { ex = hd + disc; ey = eb; } else { ex = eb; ey = disc - hd; }
+    const float rl = 1.0f / sqrtf(ex * ex + ey * ey);
+    v0x = ex * rl; v0y = ey * rl;
+  } else if (ea >= ed) {
+    v0x = 1.0f; v0y = 0.0f;
+  } else {
+    v0x = 0.0f; v0y = 1.0f;
+  }
+  const float v1x = -v0y, v1y = v0x;  // sign of v1 is irrelevant in V diag V^T
+  const float m0 = mu0 / (1.0f + s0), m1 = mu1 / (1.0f + s1);  // sigma - 1
+  const float d0 = corot ? -fmaxf(m0, 0.0f) : -m0;
+  const float d1 = corot ? -fmaxf(m1, 0.0f) : -m1;
+  const float k0 = d0 / s0, k1 = d1 / s1;
+  const float K00 = v0x * v0x * k0 + v1x * v1x * k1;
+  const float K01 = v0x * v0y * k0 + v1x * v1y * k1;
+  const float K11 = v0y * v0y * k0 + v1y * v1y * k1;
+  const float3 f0 = f3(F0), f1 = f3(F1);
+  D0 = f0 * K00 + f1 * K01;
+  D1 = f0 * K01 + f1 * K11;
+}
+
+// Dihedral bending, FP64, following compute_dihedral_angle
+// (bending.rs:56-91), BendingData::project (bending.rs:182-227) and
+// rotate_around_axis (bending.rs:231-239). Returns s^T(target - x) =
+// (t_a - x_a) + (t_b - x_b) (the edge vertices are not moved).
+__device__ inline double3 bending_delta_f64(double3 p0, double3 p1, double3 pa, double3 pb, double rest) {
+  const double3 zero = make_double3(0, 0, 0);
+  const double3 edge = p1 - p0;
+  const double el = dlen(edge);
+  double cur = M_PI;
+  double3 axis = zero;
+  if (el >= 1e-10) {
+    axis = ddiv(edge, el);
+    const double3 ta = pa - p0, tb = pb - p0;
+    double3 qa = ta - axis * ddot(ta, axis);
+    double3 qb = tb - axis * ddot(tb, axis);
+    const double la = dlen(qa), lb = dlen(qb);
+    if (!(la < 1e-10 || lb < 1e-10)) {
+      qa = ddiv(qa, la); qb = ddiv(qb, lb);
+      const double c = fmin(1.0, fmax(-1.0, ddot(qa, qb)));
+      cur = acos(c);
+    }
+  }
+  const double diff = cur - rest;
+  if (fabs(diff) < 1e-8) return zero;
+  if (el < 1e-10) return zero;
+  const double3 ta = pa - p0, tb = pb - p0;
+  const double3 qa = ta - axis * ddot(ta, axis);
+  const double3 qb = tb - axis * ddot(tb, axis);
+  const double sign = ddot(dcross(qa, qb), axis) >= 0.0 ? 1.0 : -1.0;
+  const double half = diff * 0.5 * sign;
+  const double3 mid = (p0 + p1) * 0.5;
+  double sa, ca;
+  sincos(half, &sa, &ca);
+  const double3 va = pa - mid, vb = pb - mid;
+  const double3 ra = va * ca + dcross(axis, va) * sa + axis * ddot(axis, va) * (1.0 - ca);
+  const double3 rb = vb * ca + dcross(axis, vb) * (-sa) + axis * ddot(axis, vb) * (1.0 - ca);
+  return ((mid + ra) - pa) + ((mid + rb) - pb);
+}
+
+// Dihedral bending, FP32 residual form: vectors relative to v0 (formed in
+// FP64, rounded once), angle by atan2(|qa x qb|, qa . qb) (same value as the
+// reference's acos of the normalised dot, without its loss near pi), and the
+// displacement of each wing as sin(h) (axis x v) - 2 sin^2(h/2) v_perp.
+__device__ inline float3 bending_delta_f32(double3 p0, double3 p1, double3 pa, double3 pb, float rest) {
+  const float3 zero = make_float3(0, 0, 0);
+  const float3 e = f3(p1 - p0), a = f3(pa - p0), b = f3(pb - p0);
+  const float el = flen(e);
+  if (!(el >= 1e-10f)) return zero;
+  const float3 axis = e * (1.0f / el);
+  const float3 qa = a - axis * fdot(a, axis);
+  const float3 qb = b - axis * fdot(b, axis);
+  const float3 cr = fcross(qa, qb);
+  float cur = (float)M_PI;
+  if (!(flen(qa) < 1e-10f || flen(qb) < 1e-10f)) cur = atan2f(flen(cr), fdot(qa, qb));
+  const float diff = cur - rest;
+  if (fabsf(diff) < 1e-8f) return zero;
+  const float sign = fdot(cr, axis) >= 0.0f ? 1.0f : -1.0f;
+  const float half = diff * 0.5f * sign;
+  const float sh = sinf(half);
+  const float q = sinf(0.5f * half);
+  const float omc = 2.0f * q * q;  // 1 - cos(half)
+  const float3 m = e * 0.5f;
+  const float3 va = a - m, vb = b - m;
+  const float3 da = fcross(axis, va) * sh - (va - axis * fdot(axis, va)) * omc;
+  const float3 db = fcross(axis, vb) * (-sh) - (vb - axis * fdot(axis, vb)) * omc;
+  return da + db;
+}
+
+template <class T> struct Prec;
+template <> struct Prec<double> { static constexpr bool f32 = false; };
+template <> struct Prec<float> { static constexpr bool f32 = true; };
+
+// Local step: one thread per triangle (blocks [0, tb)) or hinge (the rest).
+// pd_solver.rs:300-367 (the loops) and the per-element RHS terms of
+// assembly.rs:176-235.
+template <class T>
+__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, const double* __restrict__ x,
+                                               const int* __restrict__ tri_idx, const double* __restrict__ tri_dm,
+                                               const T* __restrict__ tri_w, const int* __restrict__ hinge_idx,
+                                               const T* __restrict__ hinge_rest, const T* __restrict__ hinge_w,
+                                               T* __restrict__ ctri, T* __restrict__ chinge, const StepDev* st) {
+  if (st && st->done) return;
+  if ((int)blockIdx.x < tb) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= t) return;
+    const int i0 = tri_idx[3 * e], i1 = tri_idx[3 * e + 1], i2 = tri_idx[3 * e + 2];
+    const double3 p0 = ld3(x, i0), p1 = ld3(x, i1), p2 = ld3(x, i2);
+    double dm[4];
+    for (int k = 0; k < 4; ++k) dm[k] = tri_dm[4 * e + k];
+    const T w = tri_w[e];
+    const T a = (T)dm[0], b = (T)dm[1], c = (T)dm[2], d = (T)dm[3];
+    const T g[3][2] = {{-(a + b), -(c + d)}, {a, c}, {b, d}};
+    T Dx0, Dy0, Dz0, Dx1, Dy1, Dz1;
+    if (Prec<T>::f32) {
+      float3 D0, D1;
+      membrane_delta_f32(p0, p1, p2, dm, corot, D0, D1);
+      Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
+    } else {
+      double3 D0, D1;
+      membrane_delta_f64(p0, p1, p2, dm, corot, D0, D1);
+      Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
+    }
+    T* o = ctri + 9 * (size_t)e;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      o[3 * k + 0] = w * (g[k][0] * Dx0 + g[k][1] * Dx1);
+      o[3 * k + 1] = w * (g[k][0] * Dy0 + g[k][1] * Dy1);
+      o[3 * k + 2] = w * (g[k][0] * Dz0 + g[k][1] * Dz1);
+    }
+  } else {
+    const int e = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
+    if (e >= h) return;
+    const int* q = hinge_idx + 4 * (size_t)e;
+    const double3 p0 = ld3(x, q[0]), p1 = ld3(x, q[1]), pa = ld3(x, q[2]), pb = ld3(x, q[3]);
+    const T w = hinge_w[e];
+    T cx, cy, cz;
+    if (Prec<T>::f32) {
+      const float3 c = bending_delta_f32(p0, p1, pa, pb, (float)hinge_rest[e]);
+      cx = c.x; cy = c.y; cz = c.z;
+    } else {
+      const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)hinge_rest[e]);
+      cx = c.x; cy = c.y; cz = c.z;
+    }
+    T* o = chinge + 3 * (size_t)e;
+    o[0] = w * cx; o[1] = w * cy; o[2] = w * cz;
+  }
+}
+
+// Vertex gather (no atomics): r_i = m_i (pred_i / dt^2 - x_i / dt_A^2)
+//   + sum of the vertex's triangle-corner slots + s_k * hinge slots.
+// Slot code c >= 0: triangle corner c = 3e + k; c < 0: hinge slot
+// (-c - 1) = 4e + k with stencil sign s = (-1, -1, +1, +1)[k] (assembly.rs:210).
+template <class T>
+__global__ void __launch_bounds__(256) k_gather(int n, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
+                                                const T* __restrict__ ctri, const T* __restrict__ chinge,
+                                                const double* __restrict__ mass, const double* __restrict__ pred,
+                                                const double* __restrict__ x, double inv_dt2, double inv_dtm2,
+                                                double* __restrict__ r, const StepDev* st) {
+  if (st && st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T ax = 0, ay = 0, az = 0;
+  const int pe = inc_ptr[i + 1];
+  for (int p = inc_ptr[i]; p < pe; ++p) {
+    const int c = inc[p];
+    if (c >= 0) {
+      const T* s = ctri + 3 * (size_t)c;
+      ax += s[0]; ay += s[1]; az += s[2];
+    } else {
+      const int sl = -c - 1;
+      const T* s = chinge + 3 * (size_t)(sl >> 2);
+      const T sg = (sl & 3) < 2 ? T(-1) : T(1);
+      ax += sg * s[0]; ay += sg * s[1]; az += sg * s[2];
+    }
+  }
+  const double m = mass[i];
+  r[3 * i + 0] = m * (inv_dt2 * pred[3 * i + 0] - inv_dtm2 * x[3 * i + 0]) + (double)ax;
+  r[3 * i + 1] = m * (inv_dt2 * pred[3 * i + 1] - inv_dtm2 * x[3 * i + 1]) + (double)ay;
+  r[3 * i + 2] = m * (inv_dt2 * pred[3 * i + 2] - inv_dtm2 * x[3 * i + 2]) + (double)az;
+}
+
+template <class T>
+__device__ __forceinline__ T group_sum(T v, int g) {
+  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, g);
+  return v;
+}
+
+// Forward sweep of one level, L y = r. Task = rows [b, e) of supernode P.
+// front[0:w+r) = extend-add of the children's update vectors (children in a
+// fixed order, each map injective: no atomics); t = r_P - front[0:w);
+//   rows i < w : y_P[i]     = sum_k Linv[i][k] t[k]
+//   rows w + q : U_P[q]     = sum_k W[q][k] t[k] + front[w + q]
+// with W = L_IP L_PP^{-1}, so U_P is P's whole update to its ancestors.
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd(const int4* __restrict__ tasks, const SnDev* __restrict__ sn,
+                                             const int* __restrict__ emap, const int* __restrict__ children,
+                                             const T* __restrict__ M, const double* __restrict__ rhs,
+                                             T* __restrict__ y, T* __restrict__ U, const StepDev* st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* front = reinterpret_cast<T*>(smem_raw);
+  const int4 task = tasks[blockIdx.x];
+  const SnDev P = sn[task.x];
+  const int w = P.w, m = P.w + P.r;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < 3 * m; k += blockDim.x) front[k] = T(0);
+  __syncthreads();
+  for (int c = 0; c < P.nchild; ++c) {
+    const SnDev C = sn[children[P.child_off + c]];
+    for (int q = tid; q < C.r; q += blockDim.x) {
+      const int pos = emap[C.map_off + q];
+      const T* u = U + 3 * (size_t)(C.u_off + q);
+      front[3 * pos + 0] += u[0];
+      front[3 * pos + 1] += u[1];
+      front[3 * pos + 2] += u[2];
+    }
+    __syncthreads();
+  }
+  for (int j = tid; j < w; j += blockDim.x) {
+    const double* b = rhs + 3 * (size_t)(P.col0 + j);
+    front[3 * j + 0] = (T)b[0] - front[3 * j + 0];
+    front[3 * j + 1] = (T)b[1] - front[3 * j + 1];
+    front[3 * j + 2] = (T)b[2] - front[3 * j + 2];
+  }
+  __syncthreads();
+  const int g = task.w;
+  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  const T* Mp = M + P.m_off;
+  const long long tri = (long long)w * (w + 1) / 2;
+  for (int row = task.y + grp; row < task.z; row += ngrp) {
+    const int len = row < w ? row + 1 : w;
+    const T* Mr = Mp + (row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w);
+    T sx = 0, sy = 0, sz = 0;
+    for (int k = lane; k < len; k += g) {
+      const T mv = Mr[k];
+      sx += mv * front[3 * k + 0];
+      sy += mv * front[3 * k + 1];
+      sz += mv * front[3 * k + 2];
+    }
+    sx = group_sum(sx, g); sy = group_sum(sy, g); sz = group_sum(sz, g);
+    if (lane == 0) {
+      if (row < w) {
+        T* o = y + 3 * (size_t)(P.col0 + row);
+        o[0] = sx; o[1] = sy; o[2] = sz;
+      } else {
+        T* o = U + 3 * (size_t)(P.u_off + row - w);
+        o[0] = sx + front[3 * row + 0];
+        o[1] = sy + front[3 * row + 1];
+        o[2] = sz + front[3 * row + 2];
+      }
+    }
+  }
+}
+
+// Backward sweep of one level, L^T d = y. Task = columns [b, e) of P, C =
+// task.w columns per pass, S = 256 / C row slices:
+//   d_P[j] = sum_{k >= j} Linv[k][j] y_P[k] - sum_q W[q][j] d[I_P[q]].
+// Lanes of a wave read consecutive columns of one stored row (coalesced);
+// the S slice partials are summed in a fixed order.
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd(const int4* __restrict__ tasks, const SnDev* __restrict__ sn,
+                                             const int* __restrict__ rowidx, const T* __restrict__ M,
+                                             const T* __restrict__ y, T* __restrict__ d, const StepDev* st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int4 task = tasks[blockIdx.x];
+  const SnDev P = sn[task.x];
+  const int w = P.w, m = P.w + P.r;
+  const int C = task.w, S = blockDim.x / C;
+  T* v = reinterpret_cast<T*>(smem_raw);
+  T* part = v + 3 * m;
+  const int tid = threadIdx.x;
+  for (int k = tid; k < m; k += blockDim.x) {
+    if (k < w) {
+      const T* s = y + 3 * (size_t)(P.col0 + k);
+      v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
+    } else {
+      const T* s = d + 3 * (size_t)rowidx[P.rows_off + k - w];
+      v[3 * k + 0] = -s[0]; v[3 * k + 1] = -s[1]; v[3 * k + 2] = -s[2];
+    }
+  }
+  __syncthreads();
+  const int jl = tid % C, sl = tid / C;
+  const int j = task.y + jl;
+  const T* Mp = M + P.m_off;
+  const long long tri = (long long)w * (w + 1) / 2;
+  T sx = 0, sy = 0, sz = 0;
+  if (j < task.z) {
+    for (int k = task.y + sl; k < w; k += S) {
+      if (k >= j) {
+        const T mv = Mp[(long long)k * (k + 1) / 2 + j];
+        sx += mv * v[3 * k + 0]; sy += mv * v[3 * k + 1]; sz += mv * v[3 * k + 2];
+      }
+    }
+    for (int q = sl; q < P.r; q += S) {
+      const T mv = Mp[tri + (long long)q * w + j];
+      const int k = w + q;
+      sx += mv * v[3 * k + 0]; sy += mv * v[3 * k + 1]; sz += mv * v[3 * k + 2];
+    }
+  }
+  part[3 * tid + 0] = sx; part[3 * tid + 1] = sy; part[3 * tid + 2] = sz;
+  __syncthreads();
+  if (sl == 0 && j < task.z) {
+    T ax = 0, ay = 0, az = 0;
+    for (int s = 0; s < S; ++s) {
+      const int u = s * C + jl;
+      ax += part[3 * u + 0]; ay += part[3 * u + 1]; az += part[3 * u + 2];
+    }
+    T* o = d + 3 * (size_t)(P.col0 + j);
+    o[0] = ax; o[1] = ay; o[2] = az;
+  }
+}
+
+// Iterative refinement (FP32 solve, FP64 residual): acc += d; if update_r,
+// r -= A d (A in FP64 CSR, permuted order). Rows only read d, so the in-place
+// update of r is race-free.
+__global__ void __launch_bounds__(256) k_ir_update(int n, const int* __restrict__ Ap, const int* __restrict__ Ai,
+                                                   const double* __restrict__ Ax, const float* __restrict__ d,
+                                                   double* __restrict__ r, double* __restrict__ acc, int first,
+                                                   int update_r, const StepDev* st) {
+  if (st && st->done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int c = 0; c < 3; ++c) {
+    const double dv = (double)d[3 * i + c];
+    acc[3 * i + c] = first ? dv : acc[3 * i + c] + dv;
+  }
+  if (update_r) {
+    double sx = 0, sy = 0, sz = 0;
+    for (int p = Ap[i]; p < Ap[i + 1]; ++p) {
+      const double a = Ax[p];
+      const float* dv = d + 3 * (size_t)Ai[p];
+      sx += a * (double)dv[0]; sy += a * (double)dv[1]; sz += a * (double)dv[2];
+    }
+    r[3 * i + 0] -= sx; r[3 * i + 1] -= sy; r[3 * i + 2] -= sz;
+  }
+}
+
+// Prediction, state.rs:145-169 and pd_solver.rs:274-283: prev = x;
+// pred = x + dt v + dt^2 g (pinned: pred = x; ground clamp); x = pred.
+__global__ void __launch_bounds__(256) k_predict(int n, double* __restrict__ x, const double* __restrict__ v,
+                                                 double* __restrict__ prev, double* __restrict__ pred,
+                                                 const uint8_t* __restrict__ pinned, double dt, double gx, double gy,
+                                                 double gz, int has_ground, double ground, StepDev* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) { st->done = 0; st->iterations = 0; st->residual = DBL_MAX; st->counter = 0; }
+  if (i >= n) return;
+  const double dt2 = dt * dt;
+  double p[3];
+  for (int c = 0; c < 3; ++c) { p[c] = x[3 * i + c]; prev[3 * i + c] = p[c]; }
+  if (!pinned[i]) {
+    p[0] = p[0] + dt * v[3 * i + 0] + dt2 * gx;
+    p[1] = p[1] + dt * v[3 * i + 1] + dt2 * gy;
+    p[2] = p[2] + dt * v[3 * i + 2] + dt2 * gz;
+    if (has_ground && p[1] < ground) p[1] = ground;
+  }
+  for (int c = 0; c < 3; ++c) { pred[3 * i + c] = p[c]; x[3 * i + c] = p[c]; }
+}
+
+// Commit one PD iteration, pd_solver.rs:402-444: diff = |delta|^2 and norm =
+// |x|^2 summed in FP64, x += delta, pinned -> prev, ground clamp. The last
+// block to arrive reduces the per-block partials in a fixed order and sets
+// final_residual / iterations / done.
+template <class D>
+__global__ void __launch_bounds__(256) k_commit(int n, double* __restrict__ x, const D* __restrict__ delta,
+                                                const double* __restrict__ prev, const uint8_t* __restrict__ pinned,
+                                                int has_ground, double ground, double tol, int iter,
+                                                double* __restrict__ partials, StepDev* st) {
+  if (st->done) return;
+  __shared__ double sd[256], sn[256];
+  __shared__ bool last;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double df = 0, nm = 0;
+  if (i < n) {
+    double p[3];
+    for (int c = 0; c < 3; ++c) {
+      const double o = x[3 * i + c];
+      const double dl = (double)delta[3 * i + c];
+      df += dl * dl;
+      nm += o * o;
+      p[c] = o + dl;
+    }
+    if (pinned[i]) { for (int c = 0; c < 3; ++c) p[c] = prev[3 * i + c]; }
+    else if (has_ground && p[1] < ground) p[1] = ground;
+    for (int c = 0; c < 3; ++c) x[3 * i + c] = p[c];
+  }
+  sd[threadIdx.x] = df; sn[threadIdx.x] = nm;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sd[0];
+    partials[2 * blockIdx.x + 1] = sn[0];
+    __threadfence();
+    const unsigned ticket = atomicAdd(&st->counter, 1u);
+    last = ticket == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  double a = 0, b = 0;
+  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
+    a += __builtin_nontemporal_load(&partials[2 * k]);
+    b += __builtin_nontemporal_load(&partials[2 * k + 1]);
+  }
+  sd[threadIdx.x] = a; sn[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double res = sn[0] > 1e-12 ? sqrt(sd[0] / sn[0]) : sqrt(sd[0]);
+    st->residual = res;
+    st->iterations = iter + 1;
+    st->counter = 0;
+    if (res < tol) st->done = 1;
+  }
+}
+
+// Finalize, pd_solver.rs:447-466 and state.rs:195-240.
+__global__ void __launch_bounds__(256) k_finalize(int n, const double* __restrict__ x, double* __restrict__ prev,
+                                                  double* __restrict__ v, const uint8_t* __restrict__ pinned,
+                                                  double inv_dt, int has_ground, double ground, double fdamp,
+                                                  double frayl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double vel[3];
+  for (int c = 0; c < 3; ++c) vel[c] = (x[3 * i + c] - prev[3 * i + c]) * inv_dt;
+  if (has_ground && !pinned[i] && x[3 * i + 1] <= ground) {
+    if (vel[1] < 0.0) vel[1] = 0.0;
+    prev[3 * i + 1] = ground;
+    vel[0] *= 0.5; vel[2] *= 0.5;
+  }
+  for (int c = 0; c < 3; ++c) vel[c] *= fdamp;
+  if (frayl > 0.0 && !pinned[i])
+    for (int c = 0; c < 3; ++c) vel[c] *= frayl;
+  for (int c = 0; c < 3; ++c) v[3 * i + c] = vel[c];
+}
+
+}  // namespace vhk

@@ -1,0 +1,509 @@
+// Host-side set-up for the HIP PD path. See vh_setup.hpp for the contract.
+//
+// The rest-state formulas restate the reference's (cited per function) in
+// f64. The factorization is this library's own: nested dissection by
+// recursive coordinate bisection, one supernode per dissection-tree node, a
+// multifrontal dense LL^T per supernode, and the partitioned-inverse blocks
+// M_P = [L_PP^{-1}; L_IP L_PP^{-1}] that turn both triangular sweeps of the
+// GPU solve into independent GEMVs per supernode.
+#include "vh_setup.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "vistio_hip.h"
+
+namespace vh {
+namespace {
+
+struct D3 {
+  double x, y, z;
+};
+inline D3 sub(D3 a, D3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline D3 scale(D3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline D3 divs(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double len(D3 a) { return std::sqrt(dot(a, a)); }
+
+// compute_dm_inv — crates/vistio-solver/src/element.rs:228-263.
+void dm_inverse(D3 e1, D3 e2, double out[4]) {
+  out[0] = 1; out[1] = 0; out[2] = 0; out[3] = 1;
+  const double l1 = len(e1);
+  if (l1 < 1e-10) return;
+  const D3 u = divs(e1, l1);
+  const D3 nrm = cross(e1, e2);
+  if (len(nrm) < 1e-10) return;
+  const D3 vv = cross(nrm, u);
+  const D3 v = scale(vv, 1.0 / len(vv));
+  const double dm00 = dot(e1, u), dm10 = dot(e1, v), dm01 = dot(e2, u), dm11 = dot(e2, v);
+  const double det = dm00 * dm11 - dm01 * dm10;
+  if (std::fabs(det) < 1e-10) return;
+  const double inv = 1.0 / det;
+  out[0] = dm11 * inv;
+  out[1] = -dm10 * inv;
+  out[2] = -dm01 * inv;
+  out[3] = dm00 * inv;
+}
+
+// compute_dihedral_angle — crates/vistio-solver/src/bending.rs:56-91.
+double dihedral(D3 p0, D3 p1, D3 pa, D3 pb) {
+  const D3 edge = sub(p1, p0);
+  const double el = len(edge);
+  if (el < 1e-10) return M_PI;
+  const D3 e = divs(edge, el);
+  const D3 ta = sub(pa, p0), tb = sub(pb, p0);
+  D3 qa = sub(ta, scale(e, dot(ta, e)));
+  D3 qb = sub(tb, scale(e, dot(tb, e)));
+  const double la = len(qa), lb = len(qb);
+  if (la < 1e-10 || lb < 1e-10) return M_PI;
+  qa = divs(qa, la);
+  qb = divs(qb, lb);
+  double c = dot(qa, qb);
+  c = std::min(1.0, std::max(-1.0, c));
+  return std::acos(c);
+}
+
+// Nested dissection by recursive coordinate bisection. The vertex separator
+// is the one-sided boundary of the median cut on the side where it is
+// smaller. Nodes are appended in postorder (left, right, separator).
+struct NdNode {
+  std::vector<int> verts;
+  int left = -1, right = -1;
+};
+
+struct NdBuilder {
+  const std::vector<int32_t>& Ap;
+  const std::vector<int32_t>& Ai;
+  const double* X;
+  const double* Y;
+  const double* Z;
+  int leaf;
+  std::vector<int> mark;
+  int stamp = 0;
+  std::vector<NdNode> nodes;
+
+  int build(std::vector<int>& verts) {
+    if ((int)verts.size() <= leaf) {
+      NdNode nd;
+      nd.verts = verts;
+      nodes.push_back(std::move(nd));
+      return (int)nodes.size() - 1;
+    }
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v : verts) {
+      const double c[3] = {X[v], Y[v], Z[v]};
+      for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], c[k]); hi[k] = std::max(hi[k], c[k]); }
+    }
+    int ax = 0;
+    for (int k = 1; k < 3; ++k) if (hi[k] - lo[k] > hi[ax] - lo[ax]) ax = k;
+    const double* C = ax == 0 ? X : (ax == 1 ? Y : Z);
+    std::vector<int> s(verts);
+    std::sort(s.begin(), s.end(), [&](int a, int b) { return C[a] < C[b] || (C[a] == C[b] && a < b); });
+    const size_t half = s.size() / 2;
+    const int L = ++stamp, R = ++stamp, S = ++stamp;
+    for (size_t i = 0; i < s.size(); ++i) mark[s[i]] = i < half ? L : R;
+    std::vector<int> bl, br;
+    for (int v : s) {
+      const int other = mark[v] == L ? R : L;
+      for (int p = Ap[v]; p < Ap[v + 1]; ++p)
+        if (mark[Ai[p]] == other) { (mark[v] == L ? bl : br).push_back(v); break; }
+    }
+    std::vector<int>& sep = bl.size() <= br.size() ? bl : br;
+    for (int v : sep) mark[v] = S;
+    std::vector<int> left, right;
+    for (int v : s) {
+      if (mark[v] == L) left.push_back(v);
+      else if (mark[v] == R) right.push_back(v);
+    }
+    if (left.empty() || right.empty()) {  // cannot split: keep as one dense leaf
+      NdNode nd;
+      nd.verts = verts;
+      nodes.push_back(std::move(nd));
+      return (int)nodes.size() - 1;
+    }
+    std::vector<int> sepv(sep);
+    const int a = build(left);
+    const int b = build(right);
+    NdNode nd;
+    nd.verts = std::move(sepv);
+    nd.left = a;
+    nd.right = b;
+    nodes.push_back(std::move(nd));
+    return (int)nodes.size() - 1;
+  }
+};
+
+// Dense lower Cholesky in place on the leading w x w block of F (ld = m).
+bool dense_chol(double* F, int m, int w) {
+  for (int j = 0; j < w; ++j) {
+    double* Fj = F + (size_t)j * m;
+    double d = Fj[j];
+    for (int k = 0; k < j; ++k) d -= Fj[k] * Fj[k];
+    if (!(d > 0.0)) return false;
+    const double ljj = std::sqrt(d);
+    Fj[j] = ljj;
+    const double inv = 1.0 / ljj;
+    for (int i = j + 1; i < w; ++i) {
+      double* Fi = F + (size_t)i * m;
+      double s = Fi[j];
+      for (int k = 0; k < j; ++k) s -= Fi[k] * Fj[k];
+      Fi[j] = s * inv;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+bool build_setup(const SetupInput& in, Setup& S) {
+  const int n = in.n, t = in.t, h = in.h;
+  S.n = n; S.t = t; S.h = h;
+  if (n <= 0 || t < 0 || h < 0 || !in.px || !in.py || !in.pz || !in.pinned || (t > 0 && !in.tri) ||
+      (h > 0 && !in.hinge)) {
+    S.status = VH_E_INVALID_MESH; S.err = "empty mesh or null input array"; return false;
+  }
+  for (int64_t k = 0; k < 3 * (int64_t)t; ++k)
+    if (in.tri[k] >= (uint32_t)n) { S.status = VH_E_INVALID_MESH; S.err = "triangle index out of range"; return false; }
+  for (int64_t k = 0; k < 4 * (int64_t)h; ++k)
+    if (in.hinge[k] >= (uint32_t)n) { S.status = VH_E_INVALID_MESH; S.err = "hinge index out of range"; return false; }
+  if (!(in.dt_matrix > 0)) { S.status = VH_E_INVALID_CONFIG; S.err = "dt_matrix must be > 0"; return false; }
+
+  std::vector<double> X(n), Y(n), Z(n);
+  for (int i = 0; i < n; ++i) { X[i] = in.px[i]; Y[i] = in.py[i]; Z[i] = in.pz[i]; }
+  auto P = [&](int i) { return D3{X[i], Y[i], Z[i]}; };
+
+  // Rest data, element.rs:54-91 (weight = k * area, element.rs:77).
+  std::vector<double> dm(4 * (size_t)t), area(t), tw(t);
+  for (int e = 0; e < t; ++e) {
+    const int i0 = in.tri[3 * e], i1 = in.tri[3 * e + 1], i2 = in.tri[3 * e + 2];
+    const D3 e1 = sub(P(i1), P(i0)), e2 = sub(P(i2), P(i0));
+    area[e] = 0.5 * len(cross(e1, e2));
+    dm_inverse(e1, e2, &dm[4 * (size_t)e]);
+    tw[e] = in.k_membrane * area[e];
+  }
+  // Lumped masses, pd_solver.rs:1219-1242.
+  std::vector<double> mass(n, 0.0);
+  for (int e = 0; e < t; ++e) {
+    const double third = (area[e] * in.density) / 3.0;
+    for (int k = 0; k < 3; ++k) mass[in.tri[3 * e + k]] += third;
+  }
+  for (int i = 0; i < n; ++i) {
+    if (in.pinned[i]) mass[i] = 1e8;
+    else if (mass[i] < 1e-8) mass[i] = 1e-8;
+  }
+  S.mass_orig = mass;
+  // Bending rest data, bending.rs:98-148 (snap to pi / 0 within 1e-3).
+  std::vector<double> hrest(h), hw(h);
+  for (int e = 0; e < h; ++e) {
+    const uint32_t* q = in.hinge + 4 * (size_t)e;
+    double a = dihedral(P(q[0]), P(q[1]), P(q[2]), P(q[3]));
+    if (std::fabs(a - M_PI) < 1e-3) a = M_PI;
+    else if (std::fabs(a) < 1e-3) a = 0.0;
+    hrest[e] = a;
+    hw[e] = in.k_bending * len(sub(P(q[1]), P(q[0])));
+  }
+
+  // System matrix A = M/h^2 + sum w g g^T + sum w_b s s^T, assembly.rs:44-134,
+  // with the same |value| > 1e-12 filter on each triplet.
+  struct Trip { int32_t r, c; double v; };
+  std::vector<Trip> trip;
+  trip.reserve((size_t)n + 9 * (size_t)t + 16 * (size_t)h);
+  const double inv_dt2 = 1.0 / (in.dt_matrix * in.dt_matrix);
+  for (int i = 0; i < n; ++i) trip.push_back({i, i, mass[i] * inv_dt2});
+  for (int e = 0; e < t; ++e) {
+    const double* d = &dm[4 * (size_t)e];
+    const double g[3][2] = {{-(d[0] + d[1]), -(d[2] + d[3])}, {d[0], d[2]}, {d[1], d[3]}};
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const double v = g[a][0] * g[b][0] + g[a][1] * g[b][1];
+        if (std::fabs(v) > 1e-12) trip.push_back({(int32_t)in.tri[3 * e + a], (int32_t)in.tri[3 * e + b], tw[e] * v});
+      }
+  }
+  const double st[4] = {-1, -1, 1, 1};
+  for (int e = 0; e < h; ++e)
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) {
+        const double v = hw[e] * st[a] * st[b];
+        if (std::fabs(v) > 1e-12)
+          trip.push_back({(int32_t)in.hinge[4 * e + a], (int32_t)in.hinge[4 * e + b], v});
+      }
+  std::sort(trip.begin(), trip.end(), [](const Trip& a, const Trip& b) { return a.r < b.r || (a.r == b.r && a.c < b.c); });
+  std::vector<int32_t> Ap(n + 1, 0), Ai;
+  std::vector<double> Ax;
+  for (size_t k = 0; k < trip.size();) {
+    size_t j = k;
+    double s = 0;
+    while (j < trip.size() && trip[j].r == trip[k].r && trip[j].c == trip[k].c) s += trip[j++].v;
+    Ai.push_back(trip[k].c); Ax.push_back(s); Ap[trip[k].r + 1]++;
+    k = j;
+  }
+  for (int i = 0; i < n; ++i) Ap[i + 1] += Ap[i];
+  trip.clear(); trip.shrink_to_fit();
+
+  // Nested dissection.
+  NdBuilder nb{Ap, Ai, X.data(), Y.data(), Z.data(), std::max(1, in.leaf_size), std::vector<int>(n, 0)};
+  {
+    std::vector<int> all(n);
+    std::iota(all.begin(), all.end(), 0);
+    nb.build(all);
+  }
+  const int ns = (int)nb.nodes.size();
+  S.perm.assign(n, 0); S.iperm.assign(n, 0);
+  S.sn.assign(ns, Supernode());
+  std::vector<int> sub_lo(ns);
+  {
+    int col = 0;
+    for (int s = 0; s < ns; ++s) {  // nodes are already in postorder
+      const NdNode& nd = nb.nodes[s];
+      sub_lo[s] = nd.left >= 0 ? sub_lo[nd.left] : col;
+      S.sn[s].col0 = col;
+      S.sn[s].w = (int)nd.verts.size();
+      for (int v : nd.verts) { S.perm[col] = v; S.iperm[v] = col; ++col; }
+      if (nd.left >= 0) { S.sn[nd.left].parent = s; S.sn[nd.right].parent = s; }
+    }
+  }
+  for (int s = ns - 1; s >= 0; --s) S.sn[s].depth = S.sn[s].parent < 0 ? 0 : S.sn[S.sn[s].parent].depth + 1;
+  S.max_depth = 0;
+  for (const auto& x : S.sn) S.max_depth = std::max(S.max_depth, x.depth);
+  S.children.clear();
+  for (int s = 0; s < ns; ++s) {
+    const NdNode& nd = nb.nodes[s];
+    S.sn[s].child_off = (int)S.children.size();
+    S.sn[s].nchild = nd.left >= 0 ? 2 : 0;
+    if (nd.left >= 0) { S.children.push_back(nd.left); S.children.push_back(nd.right); }
+  }
+
+  // Permuted matrix.
+  S.Ap.assign(n + 1, 0);
+  S.Ai.resize(Ai.size()); S.Ax.resize(Ax.size());
+  {
+    std::vector<std::pair<int32_t, double>> row;
+    int64_t w = 0;
+    for (int i = 0; i < n; ++i) {
+      const int o = S.perm[i];
+      row.clear();
+      for (int p = Ap[o]; p < Ap[o + 1]; ++p) row.push_back({S.iperm[Ai[p]], Ax[p]});
+      std::sort(row.begin(), row.end());
+      for (auto& e : row) { S.Ai[w] = e.first; S.Ax[w] = e.second; ++w; }
+      S.Ap[i + 1] = (int32_t)w;
+    }
+  }
+
+  // Row structure I_P = vertices outside P's subtree adjacent to it.
+  std::vector<int> mark(n, -1);
+  S.rowidx.clear();
+  for (int s = 0; s < ns; ++s) {
+    Supernode& x = S.sn[s];
+    const int lo = sub_lo[s], hi = x.col0 + x.w;
+    std::vector<int> rows;
+    for (int i = lo; i < hi; ++i)
+      for (int p = S.Ap[i]; p < S.Ap[i + 1]; ++p) {
+        const int j = S.Ai[p];
+        if (j >= hi && mark[j] != s) { mark[j] = s; rows.push_back(j); }
+      }
+    std::sort(rows.begin(), rows.end());
+    x.r = (int)rows.size();
+    x.rows_off = (int)S.rowidx.size();
+    S.rowidx.insert(S.rowidx.end(), rows.begin(), rows.end());
+  }
+  // Offsets for M, update vectors and the extend-add maps.
+  int64_t moff = 0;
+  int uoff = 0;
+  S.max_front = 0;
+  for (int s = 0; s < ns; ++s) {
+    Supernode& x = S.sn[s];
+    x.m_off = moff;
+    moff += (int64_t)x.w * (x.w + 1) / 2 + (int64_t)x.r * x.w;
+    x.u_off = uoff;
+    uoff += x.r;
+    S.max_front = std::max(S.max_front, x.w + x.r);
+  }
+  S.m_total = moff;
+  S.nnz_factor = moff;
+  S.u_total = uoff;
+  S.M.assign((size_t)moff, 0.0);
+  S.emap.assign((size_t)uoff, 0);
+  std::vector<int> loc(n, -1);
+  for (int s = 0; s < ns; ++s) {
+    const Supernode& x = S.sn[s];
+    if (x.parent < 0) continue;
+    const Supernode& p = S.sn[x.parent];
+    for (int q = 0; q < p.r; ++q) loc[S.rowidx[p.rows_off + q]] = p.w + q;
+    for (int q = 0; q < x.r; ++q) {
+      const int v = S.rowidx[x.rows_off + q];
+      int pos;
+      if (v >= p.col0 && v < p.col0 + p.w) pos = v - p.col0;
+      else pos = loc[v];
+      if (pos < 0) { S.status = VH_E_INTERNAL; S.err = "extend-add map: row not in parent front"; return false; }
+      S.emap[x.u_off + q] = pos;
+    }
+    for (int q = 0; q < p.r; ++q) loc[S.rowidx[p.rows_off + q]] = -1;
+    S.sn[s].map_off = x.u_off;
+  }
+
+  // Multifrontal numeric LL^T (f64) and the partitioned-inverse blocks.
+  std::vector<std::vector<double>> upd(ns);
+  std::vector<double> F, Linv;
+  for (int s = 0; s < ns; ++s) {
+    const Supernode& x = S.sn[s];
+    const int w = x.w, r = x.r, m = w + r;
+    F.assign((size_t)m * m, 0.0);
+    for (int q = 0; q < r; ++q) loc[S.rowidx[x.rows_off + q]] = w + q;
+    for (int jj = 0; jj < w; ++jj) {
+      const int J = x.col0 + jj;
+      for (int p = S.Ap[J]; p < S.Ap[J + 1]; ++p) {
+        const int i = S.Ai[p];
+        if (i < J) continue;
+        const int pos = i < x.col0 + w ? i - x.col0 : loc[i];
+        if (pos < 0) { S.status = VH_E_INTERNAL; S.err = "matrix entry outside front"; return false; }
+        F[(size_t)pos * m + jj] += S.Ax[p];
+      }
+    }
+    for (int q = 0; q < r; ++q) loc[S.rowidx[x.rows_off + q]] = -1;
+    for (int c = 0; c < x.nchild; ++c) {
+      const int ch = S.children[x.child_off + c];
+      const Supernode& y = S.sn[ch];
+      const std::vector<double>& U = upd[ch];
+      const int* mp = &S.emap[y.u_off];
+      for (int a = 0; a < y.r; ++a)
+        for (int b = 0; b <= a; ++b) F[(size_t)mp[a] * m + mp[b]] += U[(size_t)a * y.r + b];
+      upd[ch].clear(); upd[ch].shrink_to_fit();
+    }
+    if (!dense_chol(F.data(), m, w)) {
+      S.status = VH_E_NOT_SPD; S.err = "Cholesky factorization failed: matrix is not positive definite"; return false;
+    }
+    // L_IP = F_IP L_PP^{-T}
+    for (int q = 0; q < r; ++q) {
+      double* Fq = &F[(size_t)(w + q) * m];
+      for (int j = 0; j < w; ++j) {
+        const double* Lj = &F[(size_t)j * m];
+        double v = Fq[j];
+        for (int k = 0; k < j; ++k) v -= Fq[k] * Lj[k];
+        Fq[j] = v / Lj[j];
+      }
+    }
+    // Update matrix U = F_II - L_IP L_IP^T (lower)
+    if (x.parent >= 0 && r > 0) {
+      std::vector<double>& U = upd[s];
+      U.assign((size_t)r * r, 0.0);
+      for (int a = 0; a < r; ++a) {
+        const double* La = &F[(size_t)(w + a) * m];
+        for (int b = 0; b <= a; ++b) {
+          const double* Lb = &F[(size_t)(w + b) * m];
+          double v = F[(size_t)(w + a) * m + w + b];
+          for (int k = 0; k < w; ++k) v -= La[k] * Lb[k];
+          U[(size_t)a * r + b] = v;
+        }
+      }
+    }
+    // Linv = L_PP^{-1} (lower), column by column.
+    Linv.assign((size_t)w * w, 0.0);
+    for (int j = 0; j < w; ++j) {
+      Linv[(size_t)j * w + j] = 1.0 / F[(size_t)j * m + j];
+      for (int i = j + 1; i < w; ++i) {
+        const double* Li = &F[(size_t)i * m];
+        double v = 0;
+        for (int k = j; k < i; ++k) v -= Li[k] * Linv[(size_t)k * w + j];
+        Linv[(size_t)i * w + j] = v / Li[i];
+      }
+    }
+    double* Mp = &S.M[(size_t)x.m_off];
+    for (int i = 0; i < w; ++i)
+      for (int j = 0; j <= i; ++j) Mp[(size_t)i * (i + 1) / 2 + j] = Linv[(size_t)i * w + j];
+    double* Wp = Mp + (size_t)w * (w + 1) / 2;
+    for (int q = 0; q < r; ++q) {
+      const double* Lq = &F[(size_t)(w + q) * m];
+      for (int j = 0; j < w; ++j) {
+        double v = 0;
+        for (int k = j; k < w; ++k) v += Lq[k] * Linv[(size_t)k * w + j];
+        Wp[(size_t)q * w + j] = v;
+      }
+    }
+  }
+
+  // Elements with permuted vertex labels, sorted by their smallest label for
+  // locality of the position gathers.
+  {
+    std::vector<int> ord(t);
+    std::iota(ord.begin(), ord.end(), 0);
+    auto tmin = [&](int e) {
+      return std::min(S.iperm[in.tri[3 * e]], std::min(S.iperm[in.tri[3 * e + 1]], S.iperm[in.tri[3 * e + 2]]));
+    };
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return tmin(a) < tmin(b); });
+    S.tri_idx.resize(3 * (size_t)t); S.tri_dm.resize(4 * (size_t)t); S.tri_w.resize(t);
+    for (int k = 0; k < t; ++k) {
+      const int e = ord[k];
+      for (int c = 0; c < 3; ++c) S.tri_idx[3 * k + c] = S.iperm[in.tri[3 * e + c]];
+      for (int c = 0; c < 4; ++c) S.tri_dm[4 * k + c] = dm[4 * (size_t)e + c];
+      S.tri_w[k] = tw[e];
+    }
+    std::vector<int> hord(h);
+    std::iota(hord.begin(), hord.end(), 0);
+    auto hmin = [&](int e) {
+      int m = S.iperm[in.hinge[4 * e]];
+      for (int c = 1; c < 4; ++c) m = std::min(m, S.iperm[in.hinge[4 * e + c]]);
+      return m;
+    };
+    std::stable_sort(hord.begin(), hord.end(), [&](int a, int b) { return hmin(a) < hmin(b); });
+    S.hinge_idx.resize(4 * (size_t)h); S.hinge_rest.resize(h); S.hinge_w.resize(h);
+    for (int k = 0; k < h; ++k) {
+      const int e = hord[k];
+      for (int c = 0; c < 4; ++c) S.hinge_idx[4 * k + c] = S.iperm[in.hinge[4 * e + c]];
+      S.hinge_rest[k] = hrest[e];
+      S.hinge_w[k] = hw[e];
+    }
+  }
+  S.mass.resize(n); S.pinned.resize(n);
+  for (int i = 0; i < n; ++i) { S.mass[i] = mass[S.perm[i]]; S.pinned[i] = in.pinned[S.perm[i]] ? 1 : 0; }
+  // Vertex -> slot incidence: triangle slot 3e+k as e>=0 code 3e+k, hinge
+  // slot (e, k) as -(4e+k)-1. Triangle slots first, each group ascending.
+  {
+    std::vector<int32_t> cnt(n + 1, 0);
+    for (int k = 0; k < 3 * t; ++k) cnt[S.tri_idx[k] + 1]++;
+    for (int k = 0; k < 4 * h; ++k) cnt[S.hinge_idx[k] + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    S.inc_ptr = cnt;
+    S.inc.assign(cnt[n], 0);
+    std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
+    for (int k = 0; k < 3 * t; ++k) S.inc[cur[S.tri_idx[k]]++] = k;
+    for (int k = 0; k < 4 * h; ++k) S.inc[cur[S.hinge_idx[k]]++] = -k - 1;
+  }
+
+  // Level task lists. Forward: chunks of rows with at most ~4096 entries;
+  // backward: chunks of up to 64 columns.
+  auto pow2ceil = [](int v) { int p = 1; while (p < v) p <<= 1; return p; };
+  S.fwd_tasks.clear(); S.bwd_tasks.clear();
+  S.fwd_level_off.assign(S.max_depth + 2, 0);
+  S.bwd_level_off.assign(S.max_depth + 2, 0);
+  for (int lvl = 0; lvl <= S.max_depth; ++lvl) {  // forward runs deepest level first
+    const int depth = S.max_depth - lvl;
+    S.fwd_level_off[lvl] = (int)S.fwd_tasks.size();
+    for (int s = 0; s < ns; ++s) {
+      const Supernode& x = S.sn[s];
+      if (x.depth != depth) continue;  // w == 0 nodes still pass child updates up
+      const int g = std::min(64, std::max(4, pow2ceil(x.w) / 4));
+      const int m = x.w + x.r;
+      int rows_per = std::max(256 / g, 4096 / std::max(1, x.w));
+      rows_per = std::max(1, rows_per);
+      for (int b = 0; b < m; b += rows_per) S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g});
+    }
+  }
+  S.fwd_level_off[S.max_depth + 1] = (int)S.fwd_tasks.size();
+  for (int lvl = 0; lvl <= S.max_depth; ++lvl) {  // backward runs the root level first
+    S.bwd_level_off[lvl] = (int)S.bwd_tasks.size();
+    for (int s = 0; s < ns; ++s) {
+      const Supernode& x = S.sn[s];
+      if (x.depth != lvl || x.w == 0) continue;
+      const int c = std::min(64, pow2ceil(x.w));
+      for (int b = 0; b < x.w; b += c) S.bwd_tasks.push_back({s, b, std::min(x.w, b + c), c});
+    }
+  }
+  S.bwd_level_off[S.max_depth + 1] = (int)S.bwd_tasks.size();
+  S.status = VH_OK;
+  return true;
+}
+
+}  // namespace vh

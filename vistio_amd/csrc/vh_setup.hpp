@@ -1,0 +1,91 @@
+// Host-side set-up of the PD system for the HIP path: rest data, lumped
+// masses, the constant system matrix, a nested-dissection supernodal LL^T
+// factorization and the per-supernode blocks the GPU solve applies.
+//
+// Reference counterparts (file:line in the Vistio tree):
+//   ElementData::from_mesh / compute_dm_inv   crates/vistio-solver/src/element.rs:54-91, 228-263
+//   compute_lumped_masses                     crates/vistio-solver/src/pd_solver.rs:1219-1242
+//   BendingData::from_topology                crates/vistio-solver/src/bending.rs:98-148
+//   assemble_system_matrix                    crates/vistio-solver/src/assembly.rs:44-134
+//   FaerSolver::factorize                     crates/vistio-math/src/faer_solver.rs:67-93
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vh {
+
+struct SetupInput {
+  int n = 0, t = 0, h = 0;
+  const float* px = nullptr;
+  const float* py = nullptr;
+  const float* pz = nullptr;
+  const uint32_t* tri = nullptr;    // 3t vertex indices
+  const uint32_t* hinge = nullptr;  // 4h: v0, v1, wing_a, wing_b
+  const uint8_t* pinned = nullptr;  // n flags
+  double k_membrane = 1e3;          // weight = k * area
+  double k_bending = 1e-4;          // weight = k * edge length
+  double density = 0.15;            // kg/m^2
+  double dt_matrix = 1.0 / 60.0;
+  int leaf_size = 16;               // nested-dissection leaf size
+};
+
+// One supernode = one node of the nested-dissection tree: a contiguous run of
+// w columns (in the permuted order) with r off-diagonal rows I_P (all in
+// ancestors). Its solve block is the (w + r) x w lower trapezoid
+//   M_P = [ L_PP^{-1} ; L_IP L_PP^{-1} ]
+// stored row-major: rows 0..w-1 packed lower-triangular (row i has i+1
+// entries), rows w..w+r-1 dense (w entries each).
+struct Supernode {
+  int col0 = 0, w = 0, r = 0;
+  int64_t m_off = 0;   // offset of M_P in the value array
+  int rows_off = 0;    // offset of I_P in rowidx
+  int u_off = 0;       // offset (in rows) of this node's forward update vector
+  int map_off = 0;     // offset of this node's r-entry map into its parent's front
+  int parent = -1;
+  int child_off = 0, nchild = 0;
+  int depth = 0;
+};
+
+// Forward task: rows [b, e) of supernode sn. Backward task: columns [b, e).
+struct Task {
+  int sn, b, e, g;  // g: lanes per row (forward) / row slices (backward)
+};
+
+struct Setup {
+  int n = 0, t = 0, h = 0;
+  std::vector<int> perm, iperm;  // perm[new] = old vertex id
+  // elements, vertex labels already permuted
+  std::vector<int32_t> tri_idx;      // 3t
+  std::vector<double> tri_dm;        // 4t (dm_inv, column-major as the reference)
+  std::vector<double> tri_w;         // t  (k * area)
+  std::vector<int32_t> hinge_idx;    // 4h
+  std::vector<double> hinge_rest;    // h
+  std::vector<double> hinge_w;       // h
+  std::vector<double> mass;          // n, permuted (pinned -> 1e8)
+  std::vector<double> mass_orig;     // n, original vertex order (lumped_masses())
+  std::vector<uint8_t> pinned;       // n, permuted
+  std::vector<int32_t> inc_ptr, inc; // vertex -> element slot incidence (see kernels)
+  // system matrix in the permuted order, full symmetric CSR
+  std::vector<int32_t> Ap, Ai;
+  std::vector<double> Ax;
+  // supernodal structure
+  std::vector<Supernode> sn;         // postorder: children before parents
+  std::vector<int32_t> rowidx, children, emap;
+  std::vector<double> M;
+  int64_t m_total = 0;
+  int u_total = 0;
+  int max_front = 0;                 // max (w + r)
+  int max_depth = 0;
+  std::vector<Task> fwd_tasks, bwd_tasks;
+  std::vector<int> fwd_level_off, bwd_level_off;  // tasks of level l: [off[l], off[l+1])
+  int64_t nnz_factor = 0;            // sum of w(w+1)/2 + r*w
+  std::string err;
+  int status = 0;                    // 0 ok, else a vh_status code
+};
+
+// Builds everything. Returns false and sets err/status on failure.
+bool build_setup(const SetupInput& in, Setup& out);
+
+}  // namespace vh

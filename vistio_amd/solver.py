@@ -1,0 +1,325 @@
+"""Host-side mirror of Vistio's solver interface over the HIP C ABI.
+
+Mirrors, with the same names, argument meaning and error behaviour:
+  SolverStrategy (init / step / name)      crates/vistio-solver/src/strategy.rs:42-69
+  ProjectiveDynamicsSolver                 crates/vistio-solver/src/pd_solver.rs:53-476
+    init()               -> ARAP, k = stretch_weight*1000, 200 g/m^2, k_b = bending_weight*100 (:221-257)
+    init_with_material() -> model by ConstitutiveModel::name(), k = avg_stretch*10,
+                            k_b = avg_bending*100, density from properties (:106-143)
+    lumped_masses()      (:95-97)
+  SolverConfig                             crates/vistio-solver/src/config.rs:10-166
+  SimulationState                          crates/vistio-solver/src/state.rs:23-140
+  StepResult                               crates/vistio-solver/src/strategy.rs:15-24
+  VistioError                              crates/vistio-types/src/error.rs:9-44
+  quad_grid / Topology                     crates/vistio-mesh/src/generators.rs:28-100, topology.rs:58-120
+
+All numerical work runs in libvistio_hip.so on the GPU; this module only
+moves arrays across the C ABI.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as nat
+
+
+class VistioError(Exception):
+    """Base error (VistioError)."""
+
+
+class InvalidConfig(VistioError):
+    pass
+
+
+class InvalidMesh(VistioError):
+    pass
+
+
+class GpuError(VistioError):
+    pass
+
+
+_STATUS_TO_ERROR = {1: InvalidConfig, 2: InvalidConfig, 3: InvalidConfig, 4: InvalidConfig, 5: InvalidMesh,
+                    6: GpuError, 7: GpuError, 8: InvalidConfig, 9: InvalidConfig}
+
+
+def _check(status: int, ctx=None, what: str = ""):
+    if status == nat.VH_OK:
+        return
+    msg = ""
+    if ctx:
+        raw = nat.load().vh_last_error(ctx)
+        msg = raw.decode() if raw else ""
+    cls = _STATUS_TO_ERROR.get(status, VistioError)
+    raise cls(f"{what}: {nat.STATUS_NAMES.get(status, status)}: {msg}".strip(": "))
+
+
+@dataclass
+class SolverConfig:
+    """SolverConfig fields used by the PD step (config.rs:10-166 defaults)."""
+    max_iterations: int = 10
+    tolerance: float = 1e-6
+    gravity: Sequence[float] = (0.0, -9.81, 0.0)
+    damping: float = 0.01
+    stretch_weight: float = 1.0
+    bending_weight: float = 0.5
+    rayleigh_mass_damping: float = 2.0
+    # HIP-specific knobs (no reference counterpart)
+    precision: str = "f64"          # "f64" or "f32_ir"
+    ir_sweeps: int = 1
+    ir_final_sweeps: int = 3
+    leaf_size: int = 16
+
+
+@dataclass
+class FabricProperties:
+    """The FabricProperties fields init_with_material reads (properties.rs:23-98)."""
+    density: float = 150.0                  # g/m^2
+    stretch_stiffness_warp: float = 100.0
+    stretch_stiffness_weft: float = 100.0
+    bending_stiffness_warp: float = 1e-6
+    bending_stiffness_weft: float = 1e-6
+
+    def avg_stretch_stiffness(self) -> float:
+        return (self.stretch_stiffness_warp + self.stretch_stiffness_weft) / 2.0
+
+    def avg_bending_stiffness(self) -> float:
+        return (self.bending_stiffness_warp + self.bending_stiffness_weft) / 2.0
+
+    def is_anisotropic(self) -> bool:
+        ratio = self.stretch_stiffness_warp / self.stretch_stiffness_weft if self.stretch_stiffness_weft > 1e-8 else 1.0
+        return abs(ratio - 1.0) > 0.05
+
+
+@dataclass
+class TriangleMesh:
+    pos_x: np.ndarray
+    pos_y: np.ndarray
+    pos_z: np.ndarray
+    indices: np.ndarray
+
+    def vertex_count(self) -> int:
+        return int(self.pos_x.size)
+
+    def triangle_count(self) -> int:
+        return int(self.indices.size // 3)
+
+
+def quad_grid(cols: int, rows: int, width: float, height: float) -> TriangleMesh:
+    n = (cols + 1) * (rows + 1)
+    px, py, pz = (np.empty(n, dtype=np.float32) for _ in range(3))
+    idx = np.empty(6 * cols * rows, dtype=np.uint32)
+    _check(nat.load().vh_mesh_quad_grid(cols, rows, width, height, nat.ptr(px, C.c_float), nat.ptr(py, C.c_float),
+                                        nat.ptr(pz, C.c_float), nat.ptr(idx, C.c_uint32)), what="quad_grid")
+    return TriangleMesh(px, py, pz, idx)
+
+
+@dataclass
+class Topology:
+    """Interior edges (v0, v1, wing_a, wing_b, tri_a, tri_b), sorted by (v0, v1)."""
+    interior_edges: np.ndarray
+
+    @staticmethod
+    def build(mesh: TriangleMesh) -> "Topology":
+        L = nat.load()
+        n, t = mesh.vertex_count(), mesh.triangle_count()
+        idx = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
+        cnt = C.c_uint32(0)
+        _check(L.vh_mesh_interior_edges(n, t, nat.ptr(idx, C.c_uint32), None, 0, C.byref(cnt)), what="topology")
+        out = np.empty((cnt.value, 6), dtype=np.uint32)
+        _check(L.vh_mesh_interior_edges(n, t, nat.ptr(idx, C.c_uint32), nat.ptr(out, C.c_uint32), cnt.value,
+                                        C.byref(cnt)), what="topology")
+        return Topology(out)
+
+
+@dataclass
+class SimulationState:
+    vertex_count: int
+    pos_x: np.ndarray
+    pos_y: np.ndarray
+    pos_z: np.ndarray
+    vel_x: np.ndarray
+    vel_y: np.ndarray
+    vel_z: np.ndarray
+    prev_x: np.ndarray
+    prev_y: np.ndarray
+    prev_z: np.ndarray
+    pred_x: np.ndarray
+    pred_y: np.ndarray
+    pred_z: np.ndarray
+    mass: np.ndarray
+    inv_mass: np.ndarray
+
+    @staticmethod
+    def from_mesh_with_masses(mesh: TriangleMesh, masses, pinned) -> "SimulationState":
+        n = mesh.vertex_count()
+        pinned = np.asarray(pinned)
+        masses = np.asarray(masses, dtype=np.float32)
+        if pinned.size != n:
+            raise InvalidMesh(f"Pinned array length ({pinned.size}) != vertex count ({n})")
+        if masses.size != n:
+            raise InvalidMesh(f"Mass array length ({masses.size}) != vertex count ({n})")
+        mass = masses.copy()
+        inv = np.zeros(n, dtype=np.float32)
+        free = ~pinned.astype(bool)
+        mass[~free] = np.finfo(np.float32).max
+        ok = free & (mass > 1e-12)
+        inv[ok] = (1.0 / mass[ok]).astype(np.float32)
+        z = lambda: np.zeros(n, dtype=np.float32)
+        return SimulationState(n, mesh.pos_x.copy(), mesh.pos_y.copy(), mesh.pos_z.copy(), z(), z(), z(),
+                               mesh.pos_x.copy(), mesh.pos_y.copy(), mesh.pos_z.copy(), z(), z(), z(), mass, inv)
+
+
+@dataclass
+class StepResult:
+    iterations: int
+    final_residual: float
+    converged: bool
+    wall_time: float
+
+
+_MODEL_BY_NAME = {"arap": nat.MODEL_ARAP, "co_rotational": nat.MODEL_COROTATIONAL}
+
+
+class HipPdSolver:
+    """ProjectiveDynamicsSolver on the GPU (a SolverStrategy implementor)."""
+
+    def __init__(self, device: int = 0):
+        self._L = nat.load()
+        self._ctx = C.c_void_p()
+        _check(self._L.vh_create(device, C.byref(self._ctx)), what="vh_create")
+        self._n = 0
+        self._initialized = False
+        self._mass = None
+        self.config: Optional[SolverConfig] = None
+
+    def close(self):
+        if self._ctx:
+            self._L.vh_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def name(self) -> str:
+        return "ProjectiveDynamics"
+
+    # -- init -------------------------------------------------------------
+    def _init(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: int,
+              k_membrane: float, k_bending: float, density_kgm2: float):
+        n = mesh.vertex_count()
+        pin = np.ascontiguousarray(np.asarray(pinned).astype(np.uint8))
+        if pin.size != n:
+            raise InvalidMesh(f"Pinned array length ({pin.size}) != vertex count ({n})")
+        if config.precision not in ("f64", "f32_ir"):
+            raise InvalidConfig(f"unknown precision {config.precision!r}")
+        p = nat.PdParams()
+        p.model = model
+        p.bending = nat.BENDING_DIHEDRAL
+        p.k_membrane, p.k_bending, p.density = k_membrane, k_bending, density_kgm2
+        p.dt_matrix = 1.0 / 60.0  # pd_solver.rs:130,180,244
+        for k in range(3):
+            p.gravity[k] = float(config.gravity[k])
+        p.damping, p.rayleigh = float(config.damping), float(config.rayleigh_mass_damping)
+        p.max_iterations, p.tolerance = int(config.max_iterations), float(config.tolerance)
+        p.has_ground, p.ground_height = 0, 0.0
+        p.precision = nat.PREC_F64 if config.precision == "f64" else nat.PREC_F32_IR
+        p.ir_sweeps, p.ir_final_sweeps, p.leaf_size = int(config.ir_sweeps), int(config.ir_final_sweeps), int(config.leaf_size)
+        px, py, pz = (np.ascontiguousarray(a, dtype=np.float32) for a in (mesh.pos_x, mesh.pos_y, mesh.pos_z))
+        tri = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
+        hg = np.ascontiguousarray(topology.interior_edges[:, :4], dtype=np.uint32)
+        self._mass = np.empty(n, dtype=np.float32)
+        self._params = p
+        _check(self._L.vh_pd_init(self._ctx, n, nat.ptr(px, C.c_float), nat.ptr(py, C.c_float), nat.ptr(pz, C.c_float),
+                                  tri.size // 3, nat.ptr(tri, C.c_uint32), hg.shape[0], nat.ptr(hg, C.c_uint32),
+                                  nat.ptr(pin, C.c_uint8), C.byref(p), nat.ptr(self._mass, C.c_float)),
+               self._ctx, "init")
+        self._n = n
+        self.config = config
+        self._initialized = True
+
+    def init(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned):
+        """SolverStrategy::init (pd_solver.rs:221-257): ARAP, 200 g/m^2."""
+        self._init(mesh, topology, config, pinned, nat.MODEL_ARAP, float(np.float32(config.stretch_weight) * np.float32(1000.0)),
+                   float(np.float32(config.bending_weight) * np.float32(100.0)), 0.2)
+
+    def init_with_material(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig,
+                           properties: FabricProperties, model: str, pinned):
+        """init_with_material (pd_solver.rs:106-143). model = ConstitutiveModel::name()."""
+        if model not in _MODEL_BY_NAME:
+            raise InvalidConfig(f"unsupported constitutive model {model!r}")
+        self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], properties.avg_stretch_stiffness() * 10.0,
+                   properties.avg_bending_stiffness() * 100.0, properties.density / 1000.0)
+
+    def init_raw(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: str,
+                 k_membrane: float, k_bending: float, density_kgm2: float):
+        """Init with explicit membrane/bending weights and areal density (kg/m^2)."""
+        if model not in _MODEL_BY_NAME:
+            raise InvalidConfig(f"unsupported constitutive model {model!r}")
+        self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], k_membrane, k_bending, density_kgm2)
+
+    def lumped_masses(self) -> np.ndarray:
+        if not self._initialized:
+            raise InvalidConfig("Solver not initialized. Call init() first.")
+        return self._mass
+
+    # -- stepping -----------------------------------------------------------
+    def step(self, state: SimulationState, dt: float) -> StepResult:
+        """SolverStrategy::step on host state (uploads, steps on the GPU, downloads)."""
+        if not self._initialized:
+            raise InvalidConfig("Solver not initialized. Call init() first.")
+        n = self._n
+        arrs = [state.pos_x, state.pos_y, state.pos_z, state.vel_x, state.vel_y, state.vel_z,
+                state.prev_x, state.prev_y, state.prev_z, state.pred_x, state.pred_y, state.pred_z, state.inv_mass]
+        for a in arrs:
+            if a.dtype != np.float32 or a.size != n or not a.flags.c_contiguous:
+                raise InvalidMesh("state arrays must be contiguous float32 of the mesh's vertex count")
+        s = nat.StateF32(*[nat.ptr(a, C.c_float) for a in arrs])
+        r = nat.StepResultC()
+        _check(self._L.vh_pd_step(self._ctx, float(dt), C.byref(s), C.byref(r)), self._ctx, "step")
+        return StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time)
+
+    def step_resident(self, dt: float, nsteps: int = 1) -> StepResult:
+        if not self._initialized:
+            raise InvalidConfig("Solver not initialized. Call init() first.")
+        r = nat.StepResultC()
+        _check(self._L.vh_pd_step_resident(self._ctx, float(dt), int(nsteps), C.byref(r)), self._ctx, "step_resident")
+        return StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time)
+
+    def set_state(self, which: int, xyz) -> None:
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(3, self._n))
+        _check(self._L.vh_state_set_f64(self._ctx, which, nat.ptr(a[0], C.c_double), nat.ptr(a[1], C.c_double),
+                                        nat.ptr(a[2], C.c_double)), self._ctx, "set_state")
+
+    def get_state(self, which: int = 0) -> np.ndarray:
+        a = np.empty((3, self._n))
+        _check(self._L.vh_state_get_f64(self._ctx, which, nat.ptr(a[0], C.c_double), nat.ptr(a[1], C.c_double),
+                                        nat.ptr(a[2], C.c_double)), self._ctx, "get_state")
+        return a
+
+    def positions(self) -> np.ndarray:
+        return self.get_state(0)
+
+    def residual(self, dt: float) -> np.ndarray:
+        out = np.empty(3 * self._n)
+        _check(self._L.vh_pd_residual(self._ctx, float(dt), nat.ptr(out, C.c_double)), self._ctx, "residual")
+        return out.reshape(3, self._n)
+
+    def solve(self, b, sweeps: int = 3) -> np.ndarray:
+        bb = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(3 * self._n))
+        x = np.empty(3 * self._n)
+        _check(self._L.vh_pd_solve(self._ctx, nat.ptr(bb, C.c_double), nat.ptr(x, C.c_double), int(sweeps)),
+               self._ctx, "solve")
+        return x.reshape(3, self._n)
+
+    def info(self) -> dict:
+        i = nat.PdInfo()
+        _check(self._L.vh_pd_info_get(self._ctx, C.byref(i)), self._ctx, "info")
+        return {k: getattr(i, k) for k, _ in nat.PdInfo._fields_}
