@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""PD steps/s of the 256x256 hanging sheet on one MI355X: FP32 local step +
+FP32 supernodal solves with FP64-residual refinement, next to the FP64 GPU
+path and the CPU restatement of the reference step.
+
+Workload: quad_grid(256, 256, 1 m, 1 m) in the z = 0
+plane (66,049 vertices, 131,072 triangles, 196,096 hinges), row 0 pinned,
+gravity -9.81 m/s^2 along y, 0.15 kg/m^2, co-rotational membrane weight
+1e3 N/m, dihedral bending weight 1e-4 N m, Rayleigh 0.01, dt = 1/60 s,
+10 PD iterations per step (tolerance 0), synthetic mesh (no dataset).
+
+Timing: W warm-up steps, the state is reset to rest, then exactly K steps
+are timed from a device synchronisation to the next (state resident in
+HBM). The error is the max per-vertex distance between the FP32-refined and
+the FP64 GPU positions after those K steps from rest. The FP64 GPU path is
+itself held to <= 1e-9 m against the FP64 CPU oracle by the GPU tests.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "PD steps/s at 256² sheet, FP32-refined vs FP64 GPU, at ≤1e-6 m err"
+HBM_PEAK_GBS = 8000.0
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def make_sheet(cells):
+    import vistio_amd as va
+    mesh = va.quad_grid(cells, cells, 1.0, 1.0)
+    topo = va.Topology.build(mesh)
+    pinned = np.zeros(mesh.vertex_count(), dtype=np.uint8)
+    pinned[: cells + 1] = 1
+    return mesh, topo, pinned
+
+
+def make_solver(mesh, topo, pinned, precision, ir_sweeps, ir_final_sweeps):
+    import vistio_amd as va
+    cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                          rayleigh_mass_damping=0.01, precision=precision, ir_sweeps=ir_sweeps,
+                          ir_final_sweeps=ir_final_sweeps)
+    s = va.HipPdSolver(0)
+    t0 = time.perf_counter()
+    s.init_raw(mesh, topo, cfg, pinned, "co_rotational", 1e3, 1e-4, 0.15)
+    return s, time.perf_counter() - t0
+
+
+def reset(s, mesh):
+    rest = np.stack([mesh.pos_x, mesh.pos_y, mesh.pos_z]).astype(np.float64)
+    s.set_state(0, rest)
+    s.set_state(2, rest)
+    s.set_state(1, np.zeros_like(rest))
+
+
+def run_path(mesh, topo, pinned, precision, steps, warmup, dt, ir_sweeps=1, ir_final_sweeps=3):
+    s, setup = make_solver(mesh, topo, pinned, precision, ir_sweeps, ir_final_sweeps)
+    if warmup:
+        s.step_resident(dt, warmup)
+    reset(s, mesh)
+    s.step_resident(dt, 0)  # device synchronisation before the timed region
+    t0 = time.perf_counter()
+    r = s.step_resident(dt, steps)  # ends with a device synchronisation
+    wall = time.perf_counter() - t0
+    x = s.positions()
+    prof = s.profile(dt, 10)
+    info = s.info()
+    s.close()
+    return dict(wall=wall, x=x, prof=prof, info=info, setup=setup, iterations=r.iterations,
+                final_residual=r.final_residual)
+
+
+def roofline_of(prof):
+    """Dominant kernels: the band launches of the supernodal solve."""
+    us = sum(prof["fwd_us"]) + sum(prof["bwd_us"])
+    by = sum(prof["fwd_bytes"]) + sum(prof["bwd_bytes"])
+    launches = 2 * prof["bands"]
+    achieved = by / (us * 1e-6) / 1e9
+    return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_fwd_band/k_bwd_band (supernodal sweeps)",
+                launch_avg_us=round(us / launches, 3), bytes_per_launch=round(by / launches))
+
+
+def cpu_baseline(cells, sample_steps, dt):
+    """The oracle in the reference's precision split (f32 state, f64 solve,
+    three single-RHS solves per iteration), one thread, on a bounded sample."""
+    from oracle import mesh as omesh
+    from oracle import ref
+    ref.lib()
+    px, py, pz, idx = omesh.quad_grid(cells, cells, 1.0, 1.0)
+    hg = omesh.hinges4(idx, px.size)
+    pin = omesh.pinned_top_row(cells, cells)
+    o = ref.Oracle(ref.F32_REF, px, py, pz, idx, hg, pin, ref.params())
+    t0 = time.perf_counter()
+    o.step(dt, sample_steps)
+    wall = time.perf_counter() - t0
+    return dict(value=round(sample_steps / wall, 4), unit="steps/s", cores=1, kind="port",
+                sample=f"{sample_steps} steps of the {cells}x{cells} hanging sheet (10 PD iterations each, "
+                       f"f32 state + f64 LL^T with three single-RHS solves, as the reference), one thread, "
+                       f"{wall:.1f} s; CPU: {cpu_model()} ({os.cpu_count()} logical CPUs on the host)")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--ir-sweeps", type=int, default=1)
+    ap.add_argument("--ir-final-sweeps", type=int, default=2)
+    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: the PD step of one sheet runs on a single MI355X; "
+              f"--gpus {args.gpus} (WORLD_SIZE {world}) is not supported", file=sys.stderr)
+        sys.exit(2)
+    dt = 1.0 / 60.0
+    mesh, topo, pinned = make_sheet(args.cells)
+    f64 = run_path(mesh, topo, pinned, "f64", args.steps, args.warmup, dt)
+    f32 = run_path(mesh, topo, pinned, "f32_ir", args.steps, args.warmup, dt, args.ir_sweeps, args.ir_final_sweeps)
+    err = float(np.sqrt(((f32["x"] - f64["x"]) ** 2).sum(0)).max())
+    value = args.steps / f32["wall"]
+    v64 = args.steps / f64["wall"]
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * f32["wall"] / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"hanging_sheet {args.cells}x{args.cells} cells ({mesh.vertex_count()} verts, "
+                        f"{mesh.triangle_count()} tris), co-rotational membrane 1e3 N/m + dihedral bending "
+                        f"1e-4 N m, 0.15 kg/m^2, Rayleigh 0.01, dt 1/60 s, 10 PD iterations/step; FP32 local "
+                        f"step + FP32 supernodal solves, FP64 residual refinement "
+                        f"({args.ir_sweeps} sweep/iteration, {args.ir_final_sweeps} on the last), FP64 state",
+            "cells": args.cells,
+            "pd_iterations_per_step": 10,
+        },
+        "roofline": roofline_of(f32["prof"]),
+        "extra": {
+            "pd_iterations_per_s": round(10 * value, 1),
+            "max_vertex_err_vs_fp64_gpu_m": err,
+            "fp64_gpu_steps_per_s": round(v64, 3),
+            "fp32_ir_over_fp64_gpu": round(value / v64, 3),
+            "fp64_roofline": roofline_of(f64["prof"]),
+            "local_step": {
+                "fp32_us": round(f32["prof"]["local_us"], 3),
+                "fp32_gbs": round(f32["prof"]["local_bytes"] / (f32["prof"]["local_us"] * 1e-6) / 1e9, 1),
+                "fp64_us": round(f64["prof"]["local_us"], 3),
+                "fp64_gbs": round(f64["prof"]["local_bytes"] / (f64["prof"]["local_us"] * 1e-6) / 1e9, 1),
+            },
+            "iteration_us": {"fp32_ir": round(f32["prof"]["iteration_us"], 2),
+                             "fp64": round(f64["prof"]["iteration_us"], 2)},
+            "solve_bands": f32["info"]["levels"],
+            "factor_entries": f32["info"]["nnz_factor"],
+            "setup_s": {"fp32_ir": round(f32["setup"], 2), "fp64": round(f64["setup"], 2)},
+            "final_residual": f32["final_residual"],
+        },
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(args.cells, args.cpu_sample_steps, dt)
+        out["cpu_baseline"] = cb
+        out["extra"]["fp32_ir_over_cpu"] = round(value / cb["value"], 1)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,13 @@ typedef struct {
   int32_t ir_sweeps;        /* VH_PREC_F32_IR: refinement sweeps per PD iteration (1..3) */
   int32_t ir_final_sweeps;  /* VH_PREC_F32_IR: sweeps on the last iteration (1..3) */
   int32_t leaf_size;        /* nested-dissection leaf size; 0 = default (16) */
+  /* Solve schedule: consecutive dissection levels are merged into bands, one
+   * dense block per band node, chosen to minimise
+   * 2*(band_launch_us + block bytes / band_gbps) per band. 0 = defaults
+   * (6 levels max, 6 us, 4000 GB/s); max_band_levels = 1 disables merging. */
+  int32_t max_band_levels;
+  double band_launch_us;
+  double band_gbps;
 } vh_pd_params;
 
 /* StepResult, crates/vistio-solver/src/strategy.rs:15-24. */
@@ -148,6 +155,24 @@ vh_status vh_pd_residual(vh_ctx* ctx, double dt, double* out);
  * In VH_PREC_F32_IR mode runs `sweeps` refinement sweeps (>= 1). */
 vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps);
 vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out);
+
+/* Per-kernel timing of PD iterations, measured with HIP events on the
+ * context's stream around every launch (no graph). Bands: forward runs
+ * the deepest band first, backward the root band first. bytes = algorithmic
+ * bytes one launch must move (block entries, element records, vectors). */
+#define VH_MAX_BANDS 16
+typedef struct {
+  uint32_t bands;
+  uint32_t iterations;      /* PD iterations timed */
+  double local_us, local_bytes;
+  double fwd_us[VH_MAX_BANDS], fwd_bytes[VH_MAX_BANDS];
+  double bwd_us[VH_MAX_BANDS], bwd_bytes[VH_MAX_BANDS];
+  double ir_us, ir_bytes;   /* refinement update (FP32 mode), per launch */
+  double iteration_us;      /* one PD iteration, all launches */
+} vh_profile;
+/* Times `iterations` PD iterations from the current device state (the state
+ * advances like the loop of one step; call after a step). */
+vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile* out);
 vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out);
 
 /* Mesh helpers (crates/vistio-mesh): quad_grid (generators.rs:28-100) and the

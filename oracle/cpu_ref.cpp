@@ -452,13 +452,15 @@ struct OracleBase {
   virtual void set(int which, const double* in) = 0;
   virtual void masses(double* out) const = 0;
   virtual void local_rhs(double* rhs_out, double* tri_targets, double* hinge_targets) = 0;
+  virtual void elements(double* dm, double* area, double* weight) const = 0;
+  virtual void hinges(double* rest, double* weight) const = 0;
 };
 
 template <class R> struct Oracle : OracleBase {
   struct Tri { int i[3]; R dm_inv[4]; R area; R weight; };
   struct Hinge { int v[4]; R rest; R weight; };
   std::vector<Tri> tris;
-  std::vector<Hinge> hinges;
+  std::vector<Hinge> hinges_;
   std::vector<R> mass;  // solver lumped masses (pinned -> 1e8)
   std::vector<R> px, py, pz, vx, vy, vz, qx, qy, qz, rx, ry, rz;  // pos, vel, prev, pred
   std::vector<R> inv_mass;
@@ -499,10 +501,10 @@ template <class R> struct Oracle : OracleBase {
       else if (mass[i] < R(1e-8)) mass[i] = R(1e-8);
     }
     // BendingData::from_topology — bending.rs:98-148
-    hinges.resize(h);
+    hinges_.resize(h);
     const R PI = R(M_PI);
     for (int e = 0; e < h; ++e) {
-      Hinge& H = hinges[e];
+      Hinge& H = hinges_[e];
       for (int k = 0; k < 4; ++k) H.v[k] = int(hg[4 * e + k]);
       V3<R> q[4];
       for (int k = 0; k < 4; ++k) q[k] = V3<R>(px[H.v[k]], py[H.v[k]], pz[H.v[k]]);
@@ -536,7 +538,7 @@ template <class R> struct Oracle : OracleBase {
         }
     }
     const R st[4] = {-1, -1, 1, 1};
-    for (const Hinge& H : hinges)
+    for (const Hinge& H : hinges_)
       for (int ia = 0; ia < 4; ++ia)
         for (int ib = 0; ib < 4; ++ib) {
           const R val = H.weight * st[ia] * st[ib];
@@ -573,7 +575,7 @@ template <class R> struct Oracle : OracleBase {
       membrane_project(p[0], p[1], p[2], T.dm_inv, model, &tt[size_t(3) * e]);
     }
     for (int e = 0; e < h; ++e) {
-      const Hinge& H = hinges[e];
+      const Hinge& H = hinges_[e];
       V3<R> q[4];
       for (int k = 0; k < 4; ++k) q[k] = V3<R>(px[H.v[k]], py[H.v[k]], pz[H.v[k]]);
       bending_project(q[0], q[1], q[2], q[3], H.rest, &ht[size_t(4) * e]);
@@ -611,7 +613,7 @@ template <class R> struct Oracle : OracleBase {
       }
       const R st[4] = {-1, -1, 1, 1};
       for (int e = 0; e < h; ++e) {
-        const Hinge& H = hinges[e];
+        const Hinge& H = hinges_[e];
         R tg[4];
         for (int k = 0; k < 4; ++k) {
           const V3<R>& v = ht[size_t(4) * e + k];
@@ -712,6 +714,16 @@ template <class R> struct Oracle : OracleBase {
       for (int i = 0; i < n; ++i) (*s[which][a])[i] = R(in[size_t(a) * n + i]);
   }
   void masses(double* out) const override { for (int i = 0; i < n; ++i) out[i] = double(mass[i]); }
+  void elements(double* dm, double* area, double* weight) const override {
+    for (int e = 0; e < t; ++e) {
+      for (int k = 0; k < 4; ++k) dm[4 * e + k] = double(tris[e].dm_inv[k]);
+      area[e] = double(tris[e].area);
+      weight[e] = double(tris[e].weight);
+    }
+  }
+  void hinges(double* rest, double* weight) const override {
+    for (int e = 0; e < h; ++e) { rest[e] = double(hinges_[e].rest); weight[e] = double(hinges_[e].weight); }
+  }
 };
 
 }  // namespace
@@ -757,6 +769,10 @@ void or_solve(void* o, const double* b, double* x) {
 // Local step at the current positions with pred as set: RHS (3n, SoA) plus the
 // per-element targets (3t x xyz and 4h x xyz, AoS) of ElementData::project*
 // and BendingData::project. Uses dt_matrix as dt.
+void or_elements(void* o, double* dm, double* area, double* weight) {
+  static_cast<OracleBase*>(o)->elements(dm, area, weight);
+}
+void or_hinges(void* o, double* rest, double* weight) { static_cast<OracleBase*>(o)->hinges(rest, weight); }
 void or_local_rhs(void* o, double* rhs, double* tri_targets, double* hinge_targets) {
   static_cast<OracleBase*>(o)->local_rhs(rhs, tri_targets, hinge_targets);
 }

@@ -71,6 +71,8 @@ def lib():
         L.or_factor_nnz.argtypes = [vp]
         L.or_solve.argtypes = [vp, dp, dp]
         L.or_local_rhs.argtypes = [vp, dp, dp, dp]
+        L.or_elements.argtypes = [vp, dp, dp, dp]
+        L.or_hinges.argtypes = [vp, dp, dp]
         L.or_polar.argtypes = [C.c_int, dp, dp]
         L.or_target_f.argtypes = [C.c_int, C.c_int, dp, dp]
         L.or_dm_inv.argtypes = [C.c_int, dp, dp, dp]
@@ -170,6 +172,16 @@ class Oracle:
 
     def solve_time(self) -> float:
         return float(lib().or_solve_time(self._h))
+
+    def elements(self):
+        dm, area, w = np.empty(4 * self.t), np.empty(self.t), np.empty(self.t)
+        lib().or_elements(self._h, _p(dm, C.c_double), _p(area, C.c_double), _p(w, C.c_double))
+        return dm.reshape(self.t, 4), area, w
+
+    def hinges(self):
+        rest, w = np.empty(self.h), np.empty(self.h)
+        lib().or_hinges(self._h, _p(rest, C.c_double), _p(w, C.c_double))
+        return rest, w
 
     def local_rhs(self):
         rhs = np.empty(3 * self.n)

@@ -21,19 +21,23 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
   std::vector<double> r(3 * (size_t)n), y(3 * (size_t)n), d(3 * (size_t)n), U(3 * (size_t)S.u_total + 3);
   for (int k = 0; k < n; ++k)
     for (int c = 0; c < 3; ++c) r[3 * k + c] = b[(size_t)c * n + S.perm[k]];
-  for (int l = 0; l <= S.max_depth; ++l)
-    for (int q = S.fwd_level_off[l]; q < S.fwd_level_off[l + 1]; ++q) {
-      const vh::Task& T = S.fwd_tasks[q];
+  // Replay the band schedule task by task (forward bands deepest first,
+  // backward bands root first), with the push-slot data flow of the GPU
+  // kernels: children write update rows into the parent's front slots,
+  // backward tasks push -d into the descendants' v slots.
+  const int nb = S.max_depth + 1;
+  const int nfwd = (int)S.fwd_tasks.size();
+  std::vector<double> F(3 * (size_t)S.u_total + 3, 0.0), V(3 * (size_t)S.u_total + 3, 0.0);
+  for (int l = 0; l < nb; ++l)
+    for (int t = S.fwd_level_off[l]; t < S.fwd_level_off[l + 1]; ++t) {
+      const vh::Task& T = S.ptasks[t];
       const vh::Supernode& P = S.sn[T.sn];
+      if (P.depth != S.max_depth - l) return -10;
       const int w = P.w, m = P.w + P.r;
       std::vector<double> front(3 * (size_t)m, 0.0);
-      for (int c = 0; c < P.nchild; ++c) {
-        const vh::Supernode& C = S.sn[S.children[P.child_off + c]];
-        for (int qq = 0; qq < C.r; ++qq) {
-          const int pos = S.emap[C.map_off + qq];
-          for (int k = 0; k < 3; ++k) front[3 * pos + k] += U[3 * (size_t)(C.u_off + qq) + k];
-        }
-      }
+      for (int j = 0; j < m; ++j)
+        for (int q = S.fslot_ptr[P.fm_off + j]; q < S.fslot_ptr[P.fm_off + j + 1]; ++q)
+          for (int k = 0; k < 3; ++k) front[3 * j + k] += F[3 * (size_t)q + k];
       for (int j = 0; j < w; ++j)
         for (int k = 0; k < 3; ++k) front[3 * j + k] = r[3 * (size_t)(P.col0 + j) + k] - front[3 * j + k];
       const double* Mp = &S.M[P.m_off];
@@ -41,33 +45,35 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       for (int row = T.b; row < T.e; ++row) {
         const int len = row < w ? row + 1 : w;
         const double* Mr = Mp + (row < w ? (long long)row * (row + 1) / 2 : tri_n + (long long)(row - w) * w);
-        double s[3] = {0, 0, 0};
+        double s3[3] = {0, 0, 0};
         for (int kk = 0; kk < len; ++kk)
-          for (int k = 0; k < 3; ++k) s[k] += Mr[kk] * front[3 * kk + k];
+          for (int k = 0; k < 3; ++k) s3[k] += Mr[kk] * front[3 * kk + k];
         for (int k = 0; k < 3; ++k) {
-          if (row < w) y[3 * (size_t)(P.col0 + row) + k] = s[k];
-          else U[3 * (size_t)(P.u_off + row - w) + k] = s[k] + front[3 * row + k];
+          if (row < w) y[3 * (size_t)(P.col0 + row) + k] = s3[k];
+          else F[3 * (size_t)S.uslot[P.u_off + row - w] + k] = s3[k] + front[3 * row + k];
         }
       }
     }
-  for (int l = 0; l <= S.max_depth; ++l)
-    for (int q = S.bwd_level_off[l]; q < S.bwd_level_off[l + 1]; ++q) {
-      const vh::Task& T = S.bwd_tasks[q];
+  for (int l = 0; l < nb; ++l)
+    for (int t = S.bwd_level_off[l]; t < S.bwd_level_off[l + 1]; ++t) {
+      const vh::Task& T = S.ptasks[nfwd + t];
       const vh::Supernode& P = S.sn[T.sn];
+      if (P.depth != l) return -11;
       const int w = P.w, m = P.w + P.r;
       std::vector<double> v(3 * (size_t)m);
-      for (int kk = 0; kk < m; ++kk)
+      for (int kk = T.b; kk < m; ++kk)
         for (int k = 0; k < 3; ++k)
-          v[3 * kk + k] = kk < w ? y[3 * (size_t)(P.col0 + kk) + k] : -d[3 * (size_t)S.rowidx[P.rows_off + kk - w] + k];
-      const double* Mp = &S.M[P.m_off];
-      const long long tri_n = (long long)w * (w + 1) / 2;
+          v[3 * kk + k] = kk < w ? y[3 * (size_t)(P.col0 + kk) + k] : V[3 * (size_t)(P.u_off + kk - w) + k];
+      const double* Tp = &S.Mt[P.m_off];
       for (int j = T.b; j < T.e; ++j) {
-        double s[3] = {0, 0, 0};
-        for (int kk = j; kk < w; ++kk)
-          for (int k = 0; k < 3; ++k) s[k] += Mp[(long long)kk * (kk + 1) / 2 + j] * v[3 * kk + k];
-        for (int qq = 0; qq < P.r; ++qq)
-          for (int k = 0; k < 3; ++k) s[k] += Mp[tri_n + (long long)qq * w + j] * v[3 * (w + qq) + k];
-        for (int k = 0; k < 3; ++k) d[3 * (size_t)(P.col0 + j) + k] = s[k];
+        const double* row = Tp + (long long)j * m - (long long)j * (j - 1) / 2;
+        double s3[3] = {0, 0, 0};
+        for (int i = 0; i < m - j; ++i)
+          for (int k = 0; k < 3; ++k) s3[k] += row[i] * v[3 * (j + i) + k];
+        const int col = P.col0 + j;
+        for (int k = 0; k < 3; ++k) d[3 * (size_t)col + k] = s3[k];
+        for (int q = S.dpush_ptr[col]; q < S.dpush_ptr[col + 1]; ++q)
+          for (int k = 0; k < 3; ++k) V[3 * (size_t)S.dpush_idx[q] + k] = -s3[k];
       }
     }
   for (int k = 0; k < n; ++k)
@@ -76,6 +82,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
     stats[0] = (long long)S.sn.size(); stats[1] = S.max_depth + 1; stats[2] = S.nnz_factor;
     stats[3] = (long long)S.Ai.size(); stats[4] = S.max_front;
     stats[5] = (long long)S.fwd_tasks.size(); stats[6] = (long long)S.bwd_tasks.size();
+    stats[7] = (long long)S.band_start.size();
   }
   return 0;
 }

@@ -100,3 +100,33 @@ def test_step_f32_ir_within_1e6(c):
     err = max_vertex_err(g.positions(), o.positions())
     print(f"c={c} f32_ir 120 steps max vertex err {err:.3e} m", flush=True)
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("max_band_levels", [1, 2, 0])
+def test_band_schedules_solve_to_oracle(max_band_levels):
+    """One band per dissection level (1), pairs (2) and the cost-model choice
+    (0) are different block structures of the same factor: all must solve to
+    the oracle's precision."""
+    px, py, pz, idx, hg, pin = sheet(64)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    b = np.random.default_rng(11).standard_normal((3, px.size))
+    xo = np.stack([o.solve(b[k]) for k in range(3)])
+    g = gpu_solver(px, py, pz, idx, hg, pin, max_band_levels=max_band_levels)
+    xg = g.solve(b)
+    rel = np.abs(xg - xo).max() / np.abs(xo).max()
+    print(f"max_band_levels={max_band_levels} levels={g.info()['levels']} rel err {rel:.3e}", flush=True)
+    assert rel < 1e-11
+
+
+def test_tolerance_early_exit_matches_oracle():
+    """tolerance > 0: the device-side convergence flag stops the iteration
+    at the same count as the oracle (pd_solver.rs:442-444)."""
+    px, py, pz, idx, hg, pin = sheet(20)
+    o = oracle_solver(px, py, pz, idx, hg, pin, tolerance=1e-6, max_iterations=30)
+    g = gpu_solver(px, py, pz, idx, hg, pin, tolerance=1e-6, max_iterations=30)
+    for step in range(5):
+        ro = o.step(DT, 1)
+        rg = g.step_resident(DT, 1)
+        assert rg.iterations == ro.iterations, (step, rg.iterations, ro.iterations)
+        assert abs(rg.final_residual - ro.final_residual) <= 1e-6 * ro.final_residual + 1e-15
+    assert max_vertex_err(g.positions(), o.positions()) < 1e-9

@@ -1,0 +1,328 @@
+"""Pins the CPU oracle (oracle/cpu_ref.cpp) before it is trusted as the checker.
+
+Each group names what pins it:
+  * the reference's own known-answer tests (file:line in the Vistio tree);
+  * closed forms and invariants the mathematics fixes;
+  * an independent formulation of the whole step (dense matrices, numpy SVD
+    polar, matrix-exponential rotations, numpy dense solve) on tiny meshes;
+  * scipy's sparse LU for the oracle's own Cholesky.
+"""
+import numpy as np
+import pytest
+import scipy.linalg as sla
+import scipy.sparse.linalg as spla
+
+from conftest import DT, PHYS, oracle_solver, sheet
+from oracle import mesh, ref
+
+F64, F32 = ref.F64, ref.F32_REF
+
+
+# --- polar / co-rotational: crates/vistio-math/tests/math_tests.rs:41-67 and
+# crates/vistio-material/tests/material_tests.rs:124-170 -------------------
+
+@pytest.mark.parametrize("mode", [F64, F32])
+def test_polar_identity(mode):  # math_tests.rs:41-51
+    p = ref.polar([1, 0, 0, 0, 1, 0], mode)
+    assert np.allclose(p["rot"], [[1, 0, 0], [0, 1, 0]], atol=1e-6)
+    assert np.allclose(p["stretch"][[0, 3]], [1, 1], atol=1e-6)
+
+
+@pytest.mark.parametrize("mode", [F64, F32])
+def test_polar_scaled(mode):  # math_tests.rs:53-59
+    p = ref.polar([2, 0, 0, 0, 2, 0], mode)
+    assert abs(p["stretch"][0] - 2) < 1e-6 and abs(p["stretch"][3] - 2) < 1e-6
+
+
+def test_polar_degenerate_is_finite():  # math_tests.rs:61-67
+    p = ref.polar([0, 0, 0, 0, 0, 0])
+    assert np.all(np.isfinite(p["rot"])) and np.all(np.isfinite(p["stretch"]))
+
+
+def test_polar_matches_numpy_svd():
+    """F = R S with R = U V^T, S = V diag(s) V^T from numpy's SVD (independent)."""
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        F = rng.standard_normal((3, 2)) + np.eye(3, 2)
+        U, s, Vt = np.linalg.svd(F, full_matrices=False)
+        R = U @ Vt
+        S = Vt.T @ np.diag(s) @ Vt
+        p = ref.polar(np.concatenate([F[:, 0], F[:, 1]]))
+        Ro = p["rot"].T  # columns
+        So = p["stretch"].reshape(2, 2)
+        assert np.allclose(Ro, R, atol=1e-10)
+        assert np.allclose(So, S, atol=1e-10)
+        assert np.allclose(np.sort(p["s"])[::-1], s, atol=1e-10)
+
+
+def _tf(F, model=1, mode=F64):
+    return ref.target_f(np.concatenate([F[:, 0], F[:, 1]]), model, mode).T
+
+
+def test_corotational_known_answers():
+    I = np.eye(3, 2)
+    assert np.allclose(_tf(I), I, atol=1e-12)                           # material_tests.rs:124-131 (identity)
+    rot90 = np.array([[0, -1], [1, 0], [0, 0]], float)                  # :152-158 (pure rotation, zero energy)
+    assert np.allclose(_tf(rot90), rot90, atol=1e-12)
+    comp = 0.5 * I                                                      # :232-244 (compression, zero energy)
+    assert np.allclose(_tf(comp), comp, atol=1e-12)
+    assert np.allclose(_tf(2.0 * I), I, atol=1e-12)                     # :246-256 (stretch penalised -> rest)
+    mixed = np.diag([0.5, 1.5])                                         # :259-280 (mixed)
+    F = np.vstack([mixed, [0, 0]])
+    assert np.allclose(_tf(F), np.vstack([np.diag([0.5, 1.0]), [0, 0]]), atol=1e-12)
+    assert np.allclose(_tf(2.0 * I, model=0), I, atol=1e-12)            # ARAP target is R
+
+
+def test_corotational_target_matches_svd_clamp():
+    """R S_clamped = U diag(min(s, 1)) V^T (independent closed form)."""
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        F = np.eye(3, 2) + 0.3 * rng.standard_normal((3, 2))
+        U, s, Vt = np.linalg.svd(F, full_matrices=False)
+        assert np.allclose(_tf(F), U @ np.diag(np.minimum(s, 1.0)) @ Vt, atol=1e-10)
+        assert np.allclose(_tf(F, model=0), U @ Vt, atol=1e-10)
+
+
+# --- rest data: element.rs:228-263, solver_tests.rs:279-293 ---------------
+
+def test_dm_inv_right_triangle_closed_form():
+    h = 0.25
+    d = ref.dm_inv([h, 0, 0], [0, h, 0])
+    assert np.allclose(d, [1 / h, 0, 0, 1 / h], atol=1e-12)
+    d = ref.dm_inv([0, -h, 0], [h, 0, 0])  # e1 along -y, e2 along x: Dm = [[h, 0], [0, h]]
+    assert np.allclose(d, [1 / h, 0, 0, 1 / h], atol=1e-12)
+    assert np.allclose(ref.dm_inv([0, 0, 0], [1, 0, 0]), [1, 0, 0, 1])  # degenerate fallback
+
+
+def test_element_weight_is_stiffness_times_area_and_masses():
+    px, py, pz, idx, hg, pin = sheet(2)
+    o = oracle_solver(px, py, pz, idx, hg, pin, k_membrane=42.0)
+    dm, area, w = o.elements()
+    assert np.allclose(w, 42.0 * area, rtol=1e-14)                     # solver_tests.rs:279-293
+    assert np.allclose(area, 0.5 * 0.5 * 0.5)                          # right isosceles, legs 0.5
+    m = o.masses()
+    free = pin == 0
+    # lumped masses: sum of free masses + pinned share = total area * density
+    tri = idx.reshape(-1, 3)
+    exp = np.zeros(px.size)
+    for e, t in enumerate(tri):
+        exp[t] += area[e] * 0.15 / 3
+    assert np.allclose(m[free], exp[free], rtol=1e-14)
+    assert np.all(m[~free] == 1e8)                                     # pd_solver.rs:1234-1235
+
+
+@pytest.mark.parametrize("mode", [F64, F32])
+def test_flat_rest_angles_exactly_pi(mode):  # solver_tests.rs:703-718
+    px, py, pz, idx, hg, pin = sheet(5)
+    o = oracle_solver(px, py, pz, idx, hg, pin, mode=mode)
+    rest, w = o.hinges()
+    pi = np.pi if mode == F64 else float(np.float32(np.pi))
+    assert np.all(rest == pi)
+
+
+def test_dihedral_angles():
+    v0, v1 = [0, 0, 0], [1, 0, 0]
+    assert abs(ref.dihedral(np.array([v0, v1, [0.5, 1, 0], [0.5, -1, 0]])) - np.pi) < 1e-12  # solver_tests.rs:515-527
+    for th in np.linspace(0.2, 3.0, 8):
+        wb = [0.5, np.cos(th), np.sin(th)]
+        assert abs(ref.dihedral(np.array([v0, v1, [0.5, 1, 0], wb])) - th) < 1e-12
+
+
+def test_bending_projection_restores_rest_angle_and_keeps_edge():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        p = rng.standard_normal((4, 3))
+        rest = rng.uniform(0.3, 3.0)
+        q = ref.bend_project(p, rest)
+        assert np.array_equal(q[0], p[0]) and np.array_equal(q[1], p[1])  # solver_tests.rs:529-550
+        assert abs(ref.dihedral(q) - rest) < 1e-9                          # rotation reaches rest angle
+        mid = 0.5 * (p[0] + p[1])
+        for k in (2, 3):  # rigid rotation about the edge: distance to the edge midpoint kept
+            assert abs(np.linalg.norm(q[k] - mid) - np.linalg.norm(p[k] - mid)) < 1e-12
+
+
+# --- mesh: mesh_tests.rs:86-194, bench_tests.rs:13-22 ------------------------
+
+def test_grid_counts_and_coordinates():
+    px, py, pz, idx = mesh.quad_grid(20, 20, 1.0, 1.0)
+    assert px.size == 441 and idx.size // 3 == 800
+    assert mesh.pinned_top_row(20, 20).sum() == 21
+    assert px[0] == -0.5 and py[0] == 0.5 and px[20] == 0.5 and py[-1] == -0.5
+    e = mesh.interior_edges(idx, px.size)
+    # interior edges of a c x c grid: 3c^2 - 2c (all edges minus the 4c boundary ones)
+    assert e.shape[0] == 3 * 400 - 2 * 20 == 1160
+    assert np.all(e[:, 0] < e[:, 1]) and np.all(e[:, 4] < e[:, 5])
+    tri = idx.reshape(-1, 3)
+    for v0, v1, wa, wb, ta, tb in e[:50]:
+        assert set(tri[ta]) == {v0, v1, wa} and set(tri[tb]) == {v0, v1, wb}
+
+
+def test_grid_256_sizes():
+    px, py, pz, idx = mesh.quad_grid(256, 256, 1.0, 1.0)
+    assert px.size == 66049 and idx.size // 3 == 131072
+    assert mesh.interior_edges(idx, px.size).shape[0] == 196096
+
+
+# --- system matrix and Cholesky: assembly.rs:44-134, math_tests.rs:114-250 --
+
+@pytest.mark.parametrize("c", [4, 20])
+def test_matrix_invariants(c):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    A = o.matrix()
+    assert abs(A - A.T).max() < 1e-12 * abs(A).max()
+    assert np.all(A.diagonal() > 0)
+    one = np.ones(px.size)
+    # membrane gradients and the bending stencil sum to zero: A 1 = M / h^2 1
+    assert np.allclose(A @ one, o.masses() / DT**2, rtol=1e-9, atol=1e-9)
+    if c == 20:
+        assert A.nnz == 4517  # nnz(A) for configs[0], matches the reference's triplet pattern
+
+
+@pytest.mark.parametrize("c", [4, 20, 64])
+def test_cholesky_solve_matches_scipy(c):
+    px, py, pz, idx, hg, pin = sheet(c)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    A = o.matrix().tocsc()
+    b = np.random.default_rng(c).standard_normal(px.size)
+    x = o.solve(b)
+    xs = spla.spsolve(A, b)
+    assert np.abs(x - xs).max() <= 1e-10 * np.abs(xs).max()
+
+
+def test_cholesky_solve_3x3_known_answer():
+    """A 3x3 SPD system from math_tests.rs:114-140 style: solve and compare with a dense solve."""
+    px, py, pz, idx, hg, pin = sheet(1)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    A = o.matrix().toarray()
+    b = np.array([1.0, -2.0, 0.5, 3.0])
+    assert np.allclose(o.solve(b), sla.solve(A, b), rtol=1e-12)
+
+
+# --- the whole step against an independent dense formulation ----------------
+
+def _dense_pd_step(x, v, tri, hinges, dm, w_t, rest, w_h, mass, pinned, ph, dt, corot=True):
+    """Independent restatement of pd_solver.rs:259-476 with dense matrices:
+    G_e (2x3) from Dm^{-1}, polar by numpy SVD, wing rotations by scipy expm."""
+    n = x.shape[1]
+    g = np.asarray(ph["gravity"])[:, None]
+    free = ~pinned
+    prev = x.copy()
+    pred = np.where(free, x + dt * v + dt * dt * g, x)
+    A = np.diag(mass / ph["dt_matrix"] ** 2)
+    Gs = []
+    for e, t in enumerate(tri):
+        a, b, c, d = dm[e]
+        G = np.array([[-(a + b), a, b], [-(c + d), c, d]])  # F = X_e G^T
+        Gs.append(G)
+        A[np.ix_(t, t)] += w_t[e] * G.T @ G
+    s = np.array([-1.0, -1.0, 1.0, 1.0])
+    for e, q in enumerate(hinges):
+        A[np.ix_(q, q)] += w_h[e] * np.outer(s, s)
+    pos = pred.copy()
+    for _ in range(ph["max_iterations"]):
+        rhs = (mass / dt**2)[None, :] * pred
+        for e, t in enumerate(tri):
+            F = pos[:, t] @ Gs[e].T
+            U, sv, Vt = np.linalg.svd(F, full_matrices=False)
+            Ft = U @ np.diag(np.minimum(sv, 1.0) if corot else np.ones(2)) @ Vt
+            rhs[:, t] += w_t[e] * Ft @ Gs[e]
+        for e, q in enumerate(hinges):
+            p0, p1, pa, pb = (pos[:, k] for k in q)
+            ax = (p1 - p0) / np.linalg.norm(p1 - p0)
+            qa = (pa - p0) - ax * np.dot(pa - p0, ax)
+            qb = (pb - p0) - ax * np.dot(pb - p0, ax)
+            cur = np.arctan2(np.linalg.norm(np.cross(qa, qb)), np.dot(qa, qb))
+            diff = cur - rest[e]
+            tgt = np.stack([p0, p1, pa, pb], 1)
+            if abs(diff) >= 1e-8:
+                sign = 1.0 if np.dot(np.cross(qa, qb), ax) >= 0 else -1.0
+                K = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+                mid = 0.5 * (p0 + p1)
+                h = 0.5 * diff * sign
+                tgt[:, 2] = mid + sla.expm(h * K) @ (pa - mid)
+                tgt[:, 3] = mid + sla.expm(-h * K) @ (pb - mid)
+            rhs[:, q] += w_h[e] * np.outer(tgt @ s, s)
+        pos = np.linalg.solve(A, rhs.T).T
+        pos[:, pinned] = prev[:, pinned]
+    vel = (pos - prev) / dt * (1.0 - ph["damping"])
+    if ph["rayleigh"] > 0:
+        vel[:, free] *= 1.0 / (1.0 + ph["rayleigh"] * dt)
+    return pos, vel
+
+
+@pytest.mark.parametrize("corot", [True, False])
+def test_step_matches_dense_restatement(corot):
+    """Two oracle steps (f64) vs the dense restatement on a perturbed 4x4 sheet
+    (rotated into the XZ plane so membrane and bending are both active)."""
+    px, py, pz, idx, hg, pin = sheet(4, rotate_xz=True)
+    ph = dict(PHYS, model=1 if corot else 0, k_bending=5.0, max_iterations=4)
+    o = oracle_solver(px, py, pz, idx, hg, pin, **{k: ph[k] for k in ("model", "k_bending", "max_iterations")})
+    rng = np.random.default_rng(5)
+    x0 = np.stack([px, py, pz]).astype(np.float64) + 0.02 * rng.standard_normal((3, px.size)) * (pin == 0)
+    v0 = 0.1 * rng.standard_normal((3, px.size)) * (pin == 0)
+    o.set(0, x0); o.set(1, v0)
+    dm, _, w_t = o.elements()
+    rest, w_h = o.hinges()
+    tri = idx.reshape(-1, 3).astype(int)
+    x, v = x0.copy(), v0.copy()
+    for _ in range(2):
+        x, v = _dense_pd_step(x, v, tri, hg.astype(int), dm, w_t, rest, w_h, o.masses(), pin.astype(bool), ph, DT,
+                              corot)
+    o.step(DT, 2)
+    assert np.abs(o.positions() - x).max() < 1e-11
+    assert np.abs(o.get(1) - v).max() < 1e-8
+
+
+# --- step invariants (solver_tests.rs:418-479, 720-766) --------------------
+
+def test_step_invariants_hanging_sheet():
+    px, py, pz, idx, hg, pin = sheet(20)
+    # exactly mirror-symmetric x coordinates (the f32 grid is symmetric only to 1 ulp)
+    px = np.float32(np.round(px.astype(np.float64) * 20) / 20)
+    assert np.array_equal(px.reshape(21, 21), -px.reshape(21, 21)[:, ::-1])
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    x0 = o.positions().copy()
+    o.step(DT, 30)
+    x = o.positions()
+    piv = pin.astype(bool)
+    assert np.array_equal(x[:, piv], x0[:, piv])                     # pinned vertices stay (:418-450)
+    assert x[1, ~piv].mean() < x0[1, ~piv].mean()                    # centroid falls (:452-479)
+    assert np.abs(x[2]).max() < 1e-12                                # in-plane curtain stays in z = 0
+    # mirror symmetry x -> -x of the checkerboard grid (c even)
+    grid = x.reshape(3, 21, 21)
+    assert np.abs(grid[0] + grid[0][:, ::-1]).max() < 1e-12
+    assert np.abs(grid[1] - grid[1][:, ::-1]).max() < 1e-12
+
+
+def test_rayleigh_reduces_kinetic_energy():  # solver_tests.rs:720-766
+    px, py, pz, idx, hg, pin = sheet(6)
+    ke = []
+    for ray in (0.0, 5.0):
+        o = oracle_solver(px, py, pz, idx, hg, pin, rayleigh=ray)
+        o.step(DT, 20)
+        v = o.get(1)
+        m = o.masses()
+        ke.append(0.5 * float((m[pin == 0] * (v[:, pin == 0] ** 2).sum(0)).sum()))
+    assert ke[1] < ke[0]
+
+
+def test_predict_closed_form():  # solver_tests.rs:39-63 (pred = pos + dt^2 g at rest)
+    px, py, pz, idx, hg, pin = sheet(2)
+    o = oracle_solver(px, py, pz, idx, hg, pin, max_iterations=0)
+    x0 = o.positions().copy()
+    o.step(DT, 1)
+    pred = o.get(3)
+    free = pin == 0
+    assert np.allclose(pred[1, free], x0[1, free] - 9.81 * DT * DT, rtol=0, atol=1e-15)
+    assert np.array_equal(pred[:, ~free], x0[:, ~free])
+
+
+def test_f32_reference_mode_tracks_f64():
+    """The reference's own precision split stays near the f64 re-evaluation on configs[0]."""
+    px, py, pz, idx, hg, pin = sheet(20)
+    a = oracle_solver(px, py, pz, idx, hg, pin, mode=F64)
+    b = oracle_solver(px, py, pz, idx, hg, pin, mode=F32)
+    a.step(DT, 20); b.step(DT, 20)
+    err = np.sqrt(((a.positions() - b.positions()) ** 2).sum(0)).max()
+    assert 0 < err < 1e-5

@@ -26,7 +26,7 @@ PREC_F64, PREC_F32_IR = 0, 1
 EXPORTS = (
     "vh_create", "vh_destroy", "vh_last_error", "vh_pd_init", "vh_pd_step", "vh_pd_step_resident",
     "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
-    "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges",
+    "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
 )
 
 
@@ -38,6 +38,7 @@ class PdParams(C.Structure):
         ("rayleigh", C.c_double), ("max_iterations", C.c_int32), ("tolerance", C.c_double),
         ("has_ground", C.c_int32), ("ground_height", C.c_double), ("precision", C.c_int32),
         ("ir_sweeps", C.c_int32), ("ir_final_sweeps", C.c_int32), ("leaf_size", C.c_int32),
+        ("max_band_levels", C.c_int32), ("band_launch_us", C.c_double), ("band_gbps", C.c_double),
     ]
 
 
@@ -56,6 +57,17 @@ class PdInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("t", C.c_uint32), ("h", C.c_uint32), ("supernodes", C.c_uint32),
                 ("levels", C.c_uint32), ("nnz_a", C.c_int64), ("nnz_factor", C.c_int64),
                 ("max_front", C.c_int32), ("setup_seconds", C.c_double)]
+
+
+VH_MAX_BANDS = 16
+
+
+class Profile(C.Structure):
+    _fields_ = [("bands", C.c_uint32), ("iterations", C.c_uint32), ("local_us", C.c_double),
+                ("local_bytes", C.c_double), ("fwd_us", C.c_double * VH_MAX_BANDS),
+                ("fwd_bytes", C.c_double * VH_MAX_BANDS), ("bwd_us", C.c_double * VH_MAX_BANDS),
+                ("bwd_bytes", C.c_double * VH_MAX_BANDS), ("ir_us", C.c_double), ("ir_bytes", C.c_double),
+                ("iteration_us", C.c_double)]
 
 
 _lib = None
@@ -87,6 +99,7 @@ def load():
         "vh_pd_solve": ([vp, dp, dp, C.c_int32], C.c_int),
         "vh_pd_info_get": ([vp, C.POINTER(PdInfo)], C.c_int),
         "vh_pd_lumped_masses": ([vp, fp], C.c_int),
+        "vh_pd_profile": ([vp, C.c_double, C.c_uint32, C.POINTER(Profile)], C.c_int),
         "vh_mesh_quad_grid": ([C.c_uint32, C.c_uint32, C.c_float, C.c_float, fp, fp, fp, up], C.c_int),
         "vh_mesh_interior_edges": ([C.c_uint32, C.c_uint32, up, up, C.c_uint32, up], C.c_int),
     }

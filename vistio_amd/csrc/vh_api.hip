@@ -1,17 +1,19 @@
 // C ABI of the gfx950 Projective Dynamics step (declared in include/vistio_hip.h).
 //
 // One vh_ctx owns one HIP stream and all device buffers of one solver
-// instance. The step runs entirely on the device:
-//   predict -> { local -> gather -> solve (forward levels, backward levels)
-//                [-> refinement sweeps] -> commit } x max_iterations -> finalize
+// instance. A step runs entirely on the device:
+//   predict -> { local -> forward bands (first one also forms r) ->
+//                backward bands (last one also commits x) [-> refinement] }
+//              x max_iterations -> residual -> finalize
 // following ProjectiveDynamicsSolver::step (crates/vistio-solver/src/pd_solver.rs:259-476).
+// The whole step is captured once into a HIP graph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -19,21 +21,6 @@
 #include "vh_kernels.hpp"
 #include "vh_setup.hpp"
 #include "vistio_hip.h"
-
-namespace {
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct Graph {
-  hipGraphExec_t exec = nullptr;
-  double dt = 0;
-  bool valid = false;
-};
-
-}  // namespace
 
 struct vh_ctx {
   int device = 0;
@@ -45,28 +32,30 @@ struct vh_ctx {
   double setup_seconds = 0;
   int prec = VH_PREC_F64;
   size_t tsz = 8;  // bytes of the compute type
-  // device buffers
+  // state, FP64 [n][3]
   double *x = nullptr, *v = nullptr, *prev = nullptr, *pred = nullptr, *r = nullptr, *acc = nullptr;
   double* mass = nullptr;
   uint8_t* pinned = nullptr;
-  int* tri_idx = nullptr;
+  // elements
+  int *tri_idx = nullptr, *tri_slot = nullptr, *hinge_idx = nullptr, *hinge_slot = nullptr;
   double* tri_dm = nullptr;
-  void* tri_w = nullptr;
-  int* hinge_idx = nullptr;
-  void *hinge_rest = nullptr, *hinge_w = nullptr;
-  void *ctri = nullptr, *chinge = nullptr;
-  int *inc_ptr = nullptr, *inc = nullptr;
-  vhk::SnDev* sn = nullptr;
-  int4 *fwd_tasks = nullptr, *bwd_tasks = nullptr;
-  int *rowidx = nullptr, *emap = nullptr, *children = nullptr;
-  void *M = nullptr, *y = nullptr, *d = nullptr, *U = nullptr;
+  void *tri_w = nullptr, *hinge_rest = nullptr, *hinge_w = nullptr;
+  void* cslot = nullptr;  // RHS contribution slots, T [3t + 4h][3]
+  int* inc_ptr = nullptr;
+  // solve
+  vhk::TaskDev* tasks = nullptr;  // forward tasks then backward tasks
+  int *fslot_ptr = nullptr, *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
+  void *M = nullptr, *Mt = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
   double* Ax = nullptr;
   double* partials = nullptr;
+  int resid_blocks = 0;  // blocks writing residual partials
   vhk::StepDev* st = nullptr;
   std::vector<void*> allocs;
-  size_t fwd_lds = 0, bwd_lds = 0;
   std::vector<double> hbuf;  // host staging (3n)
+  hipGraphExec_t step_graph = nullptr;
+  double graph_dt = 0;
+  bool use_graph = true;
 };
 
 namespace {
@@ -125,7 +114,14 @@ vh_status upload_t(vh_ctx* ctx, void** p, const std::vector<double>& h) {
   return VH_OK;
 }
 
+void drop_graph(vh_ctx* ctx) {
+  if (ctx->step_graph) (void)hipGraphExecDestroy(ctx->step_graph);
+  ctx->step_graph = nullptr;
+  ctx->graph_dt = 0;
+}
+
 void free_all(vh_ctx* ctx) {
+  drop_graph(ctx);
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->initialized = false;
@@ -136,47 +132,78 @@ inline int nblk(int n) { return (n + 255) / 256; }
 // ---- kernel sequences -----------------------------------------------------
 
 template <class T>
-void enqueue_solve(vh_ctx* c, const vhk::StepDev* st) {
+vhk::RhsSrc<T> rhs_src(vh_ctx* c, double dt) {
+  return vhk::RhsSrc<T>{c->inc_ptr, (const T*)c->cslot, c->mass, c->pred, c->x, 1.0 / (dt * dt),
+                        1.0 / (c->P.dt_matrix * c->P.dt_matrix)};
+}
+
+vhk::CommitArgs commit_args(vh_ctx* c, bool on, bool want_resid) {
+  return vhk::CommitArgs{on ? 1 : 0, c->P.has_ground, c->P.ground_height, c->x, c->prev, c->pinned, c->partials,
+                         want_resid ? 1 : 0};
+}
+
+// One solve of A d = r (forward bands deepest first, backward bands root
+// first). from_slots: the first forward band forms r from the RHS slots.
+// commit: the last backward band adds d to x (FP64 mode).
+template <class T>
+void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool want_resid, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
-  for (int l = 0; l <= S.max_depth; ++l) {
+  const int nb = S.max_depth + 1;
+  const int nfwd = (int)S.fwd_tasks.size();
+  const int nupper = (int)S.upper_cols.size();
+  vhk::BandArgs<T> a{};
+  a.fslot_ptr = c->fslot_ptr; a.uslot = c->uslot; a.dpush_ptr = c->dpush_ptr; a.dpush_idx = c->dpush_idx;
+  a.src = rhs_src<T>(c, dt); a.r = c->r; a.upper_cols = c->upper_cols; a.nupper = nupper;
+  for (int l = 0; l < nb; ++l) {
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
-    if (e > b)
-      hipLaunchKernelGGL(vhk::k_fwd<T>, dim3(e - b), dim3(256), c->fwd_lds, c->stream, c->fwd_tasks + b, c->sn,
-                         c->emap, c->children, (const T*)c->M, c->r, (T*)c->y, (T*)c->U, st);
+    const bool first = l == 0 && from_slots;
+    a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = first ? 1 : 0;
+    const int grid = (e - b) + (first ? nblk(nupper) : 0);
+    const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * c->tsz;
+    if (grid > 0)
+      hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M, (T*)c->y,
+                         (T*)c->F, st);
   }
-  for (int l = 0; l <= S.max_depth; ++l) {
+  a.rhs_from_slots = 0;
+  for (int l = 0; l < nb; ++l) {
     const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
-    if (e > b)
-      hipLaunchKernelGGL(vhk::k_bwd<T>, dim3(e - b), dim3(256), c->bwd_lds, c->stream, c->bwd_tasks + b, c->sn,
-                         c->rowidx, (const T*)c->M, (const T*)c->y, (T*)c->d, st);
+    const bool last = l == nb - 1;
+    a.tasks = c->tasks + nfwd + b; a.ntasks = e - b;
+    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid);
+    const int grid = (e - b) + (last && commit ? nblk(nupper) : 0);
+    const size_t lds = 3 * (size_t)S.band_front[l] * c->tsz;
+    if (grid > 0)
+      hipLaunchKernelGGL(vhk::k_bwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
+                         (const T*)c->y, (T*)c->V, (T*)c->d, st);
   }
 }
 
 template <class T>
-void enqueue_local_gather(vh_ctx* c, double dt, const vhk::StepDev* st) {
+void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
   const int tb = nblk(S.t), hb = nblk(S.h);
   if (tb + hb > 0)
     hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb,
                        c->P.model == VH_MODEL_COROTATIONAL ? 1 : 0, c->x, c->tri_idx, c->tri_dm, (const T*)c->tri_w,
-                       c->hinge_idx, (const T*)c->hinge_rest, (const T*)c->hinge_w, (T*)c->ctri, (T*)c->chinge, st);
-  const double inv_dt2 = 1.0 / (dt * dt);
-  const double inv_dtm2 = 1.0 / (c->P.dt_matrix * c->P.dt_matrix);
-  hipLaunchKernelGGL(vhk::k_gather<T>, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->inc_ptr, c->inc,
-                     (const T*)c->ctri, (const T*)c->chinge, c->mass, c->pred, c->x, inv_dt2, inv_dtm2, c->r, st);
+                       c->tri_slot, c->hinge_idx, (const T*)c->hinge_rest, (const T*)c->hinge_w, c->hinge_slot,
+                       (T*)c->cslot, st);
 }
 
-// One solve of A * delta = r in the context's precision; the solution is in
-// d (FP64 mode) or acc (FP32 refinement mode). r is consumed.
-void enqueue_correction(vh_ctx* c, int sweeps, const vhk::StepDev* st) {
+// One correction of the iterate: x += A^{-1} (b - A x). FP64: one solve whose
+// last band commits. FP32: `sweeps` solves with FP64 residual updates, the
+// last update commits the accumulated correction.
+void enqueue_correction(vh_ctx* c, double dt, int sweeps, bool from_slots, bool commit, bool want_resid,
+                        const vhk::StepDev* st) {
   if (c->prec == VH_PREC_F64) {
-    enqueue_solve<double>(c, st);
+    enqueue_solve<double>(c, dt, from_slots, commit, want_resid, st);
     return;
   }
   for (int s = 0; s < sweeps; ++s) {
-    enqueue_solve<float>(c, st);
+    const bool last = s + 1 == sweeps;
+    enqueue_solve<float>(c, dt, from_slots && s == 0, false, false, st);
+    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid);
     hipLaunchKernelGGL(vhk::k_ir_update, dim3(nblk(c->S.n)), dim3(256), 0, c->stream, c->S.n, c->Ap, c->Ai, c->Ax,
-                       (const float*)c->d, c->r, c->acc, s == 0 ? 1 : 0, s + 1 < sweeps ? 1 : 0, st);
+                       (const float*)c->d, c->r, c->acc, s == 0 ? 1 : 0, last ? 0 : 1, cc, st);
   }
 }
 
@@ -187,21 +214,51 @@ void enqueue_step(vh_ctx* c, double dt) {
                      c->pinned, dt, P.gravity[0], P.gravity[1], P.gravity[2], P.has_ground, P.ground_height, c->st);
   for (int it = 0; it < P.max_iterations; ++it) {
     const bool last = it + 1 == P.max_iterations;
-    if (c->prec == VH_PREC_F64) enqueue_local_gather<double>(c, dt, c->st);
-    else enqueue_local_gather<float>(c, dt, c->st);
-    enqueue_correction(c, last ? P.ir_final_sweeps : P.ir_sweeps, c->st);
-    if (c->prec == VH_PREC_F64)
-      hipLaunchKernelGGL(vhk::k_commit<double>, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->x,
-                         (const double*)c->d, c->prev, c->pinned, P.has_ground, P.ground_height, P.tolerance, it,
-                         c->partials, c->st);
-    else
-      hipLaunchKernelGGL(vhk::k_commit<double>, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->x,
-                         (const double*)c->acc, c->prev, c->pinned, P.has_ground, P.ground_height, P.tolerance, it,
-                         c->partials, c->st);
+    // With tolerance 0 the loop never exits early, so only the last
+    // iteration's residual (StepResult.final_residual) is formed.
+    const bool want_resid = last || P.tolerance > 0;
+    if (c->prec == VH_PREC_F64) enqueue_local<double>(c, c->st);
+    else enqueue_local<float>(c, c->st);
+    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, c->st);
+    if (want_resid)
+      hipLaunchKernelGGL(vhk::k_resid_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, c->resid_blocks,
+                         P.tolerance, it, c->st);
   }
   const double frayl = P.rayleigh > 0 ? 1.0 / (1.0 + P.rayleigh * dt) : 0.0;
   hipLaunchKernelGGL(vhk::k_finalize, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->x, c->prev, c->v, c->pinned,
                      1.0 / dt, P.has_ground, P.ground_height, 1.0 - P.damping, frayl);
+}
+
+// Enqueues one step: replays the captured graph (captured on first use for
+// this dt), or launches the kernels directly when graphs are disabled.
+vh_status launch_step(vh_ctx* ctx, double dt) {
+  if (!ctx->use_graph) {
+    enqueue_step(ctx, dt);
+    VH_HIP(hipGetLastError());
+    return VH_OK;
+  }
+  if (!ctx->step_graph || ctx->graph_dt != dt) {
+    drop_graph(ctx);
+    hipGraph_t g = nullptr;
+    VH_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(ctx, dt);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    if (le != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(ctx, VH_E_GPU, std::string("capture: ") + hipGetErrorString(le));
+    }
+    if (ce != hipSuccess) return fail(ctx, VH_E_GPU, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&ctx->step_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+      ctx->step_graph = nullptr;
+      return fail(ctx, VH_E_GPU, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    ctx->graph_dt = dt;
+  }
+  VH_HIP(hipGraphLaunch(ctx->step_graph, ctx->stream));
+  return VH_OK;
 }
 
 // Permuted AoS [n][3] <-> caller SoA (x[n], y[n], z[n]).
@@ -255,6 +312,7 @@ vh_status vh_create(int32_t device, vh_ctx** out) {
   vh_ctx* ctx = new (std::nothrow) vh_ctx();
   if (!ctx) return VH_E_INTERNAL;
   ctx->device = device;
+  if (const char* g = getenv("VISTIO_HIP_GRAPH")) ctx->use_graph = atoi(g) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
     delete ctx;
     return VH_E_GPU;
@@ -304,6 +362,10 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge; in.pinned = pinned;
   in.k_membrane = P.k_membrane; in.k_bending = P.k_bending; in.density = P.density; in.dt_matrix = P.dt_matrix;
   in.leaf_size = P.leaf_size > 0 ? P.leaf_size : 16;
+  in.value_bytes = (int)ctx->tsz;
+  if (P.max_band_levels > 0) in.max_band_levels = P.max_band_levels;
+  if (P.band_launch_us > 0) in.band_launch_us = P.band_launch_us;
+  if (P.band_gbps > 0) in.band_gbps = P.band_gbps;
   ctx->S = vh::Setup();
   if (!vh::build_setup(in, ctx->S)) return fail(ctx, (vh_status)ctx->S.status, ctx->S.err);
   vh::Setup& S = ctx->S;
@@ -312,10 +374,10 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
     for (int i = 0; i < S.n; ++i) lumped_mass_out[i] = (float)S.mass_orig[i];
 
   vh_status s = VH_OK;
-#define UP(expr)                 \
-  do {                           \
-    s = (expr);                  \
-    if (s != VH_OK) return s;    \
+#define UP(expr)              \
+  do {                        \
+    s = (expr);               \
+    if (s != VH_OK) return s; \
   } while (0)
   const size_t n3 = 3 * (size_t)S.n;
   UP(dalloc(ctx, &ctx->x, n3)); UP(dalloc(ctx, &ctx->v, n3));
@@ -324,54 +386,59 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   UP(upload(ctx, &ctx->mass, S.mass));
   UP(upload(ctx, &ctx->pinned, S.pinned));
   UP(upload(ctx, &ctx->tri_idx, S.tri_idx));
+  UP(upload(ctx, &ctx->tri_slot, S.tri_slot));
   UP(upload(ctx, &ctx->tri_dm, S.tri_dm));
   UP(upload_t(ctx, &ctx->tri_w, S.tri_w));
   UP(upload(ctx, &ctx->hinge_idx, S.hinge_idx));
+  UP(upload(ctx, &ctx->hinge_slot, S.hinge_slot));
   UP(upload_t(ctx, &ctx->hinge_rest, S.hinge_rest));
   UP(upload_t(ctx, &ctx->hinge_w, S.hinge_w));
-  UP(dalloc_bytes(ctx, &ctx->ctri, 9 * (size_t)S.t * ctx->tsz));
-  UP(dalloc_bytes(ctx, &ctx->chinge, 3 * (size_t)S.h * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->cslot, 3 * (size_t)std::max(1, S.inc_ptr[S.n]) * ctx->tsz));
   UP(upload(ctx, &ctx->inc_ptr, S.inc_ptr));
-  UP(upload(ctx, &ctx->inc, S.inc));
-  std::vector<vhk::SnDev> snd(S.sn.size());
-  for (size_t k = 0; k < S.sn.size(); ++k) {
-    const vh::Supernode& q = S.sn[k];
-    snd[k] = vhk::SnDev{q.col0, q.w, q.r, q.rows_off, q.u_off, q.map_off, q.child_off, q.nchild, (long long)q.m_off};
+  std::vector<vhk::TaskDev> td(S.ptasks.size());
+  for (size_t k = 0; k < td.size(); ++k) {
+    const vh::Task& T = S.ptasks[k];
+    const vh::Supernode& q = S.sn[T.sn];
+    td[k].q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
+    td[k].q1 = make_int4(T.b, T.e, T.g, q.u_off);
+    td[k].q2 = make_int4(q.fm_off, 0, 0, 0);
   }
-  UP(upload(ctx, &ctx->sn, snd));
-  std::vector<int4> ft(S.fwd_tasks.size()), bt(S.bwd_tasks.size());
-  for (size_t k = 0; k < ft.size(); ++k) ft[k] = make_int4(S.fwd_tasks[k].sn, S.fwd_tasks[k].b, S.fwd_tasks[k].e, S.fwd_tasks[k].g);
-  for (size_t k = 0; k < bt.size(); ++k) bt[k] = make_int4(S.bwd_tasks[k].sn, S.bwd_tasks[k].b, S.bwd_tasks[k].e, S.bwd_tasks[k].g);
-  UP(upload(ctx, &ctx->fwd_tasks, ft));
-  UP(upload(ctx, &ctx->bwd_tasks, bt));
-  UP(upload(ctx, &ctx->rowidx, S.rowidx));
-  UP(upload(ctx, &ctx->emap, S.emap));
-  UP(upload(ctx, &ctx->children, S.children));
+  UP(upload(ctx, &ctx->tasks, td));
+  UP(upload(ctx, &ctx->fslot_ptr, S.fslot_ptr));
+  UP(upload(ctx, &ctx->uslot, S.uslot));
+  UP(upload(ctx, &ctx->dpush_ptr, S.dpush_ptr));
+  UP(upload(ctx, &ctx->dpush_idx, S.dpush_idx));
+  UP(upload(ctx, &ctx->upper_cols, S.upper_cols));
   UP(upload_t(ctx, &ctx->M, S.M));
+  UP(upload_t(ctx, &ctx->Mt, S.Mt));
   UP(dalloc_bytes(ctx, &ctx->y, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->d, n3 * ctx->tsz));
-  UP(dalloc_bytes(ctx, &ctx->U, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->F, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->V, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
   if (ctx->prec == VH_PREC_F32_IR) {
     UP(upload(ctx, &ctx->Ap, S.Ap));
     UP(upload(ctx, &ctx->Ai, S.Ai));
     UP(upload(ctx, &ctx->Ax, S.Ax));
   }
-  UP(dalloc(ctx, &ctx->partials, 2 * (size_t)nblk(S.n)));
+  // Residual partials come from the last backward band (FP64) or the last
+  // refinement update (FP32).
+  const int nb = S.max_depth + 1;
+  const int last_bwd = S.bwd_level_off[nb] - S.bwd_level_off[nb - 1];
+  ctx->resid_blocks = ctx->prec == VH_PREC_F64 ? last_bwd + nblk((int)S.upper_cols.size()) : nblk(S.n);
+  UP(dalloc(ctx, &ctx->partials, 2 * (size_t)std::max(1, ctx->resid_blocks)));
   UP(dalloc(ctx, &ctx->st, 1));
 #undef UP
   VH_HIP(hipMemset(ctx->st, 0, sizeof(vhk::StepDev)));
-  ctx->fwd_lds = 3 * (size_t)S.max_front * ctx->tsz;
-  ctx->bwd_lds = (3 * (size_t)S.max_front + 3 * 256) * ctx->tsz;
-  const size_t lds_max = 160 * 1024;
-  if (ctx->bwd_lds > lds_max)
-    return fail(ctx, VH_E_UNSUPPORTED, "supernode front exceeds the 160 KB LDS of one workgroup");
-  if (ctx->fwd_lds > 64 * 1024) {
-    if (ctx->prec == VH_PREC_F64) VH_HIP(hipFuncSetAttribute((const void*)vhk::k_fwd<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->fwd_lds));
-    else VH_HIP(hipFuncSetAttribute((const void*)vhk::k_fwd<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->fwd_lds));
-  }
-  if (ctx->bwd_lds > 64 * 1024) {
-    if (ctx->prec == VH_PREC_F64) VH_HIP(hipFuncSetAttribute((const void*)vhk::k_bwd<double>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->bwd_lds));
-    else VH_HIP(hipFuncSetAttribute((const void*)vhk::k_bwd<float>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)ctx->bwd_lds));
+  VH_HIP(hipMemset(ctx->F, 0, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  VH_HIP(hipMemset(ctx->V, 0, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  size_t lds = 0;
+  for (int f : S.band_front) lds = std::max(lds, 3 * (size_t)f * ctx->tsz);
+  if (lds > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "supernode front exceeds the 160 KB LDS of one workgroup");
+  if (lds > 64 * 1024) {
+    const void* kf = ctx->prec == VH_PREC_F64 ? (const void*)vhk::k_fwd_band<double> : (const void*)vhk::k_fwd_band<float>;
+    const void* kb = ctx->prec == VH_PREC_F64 ? (const void*)vhk::k_bwd_band<double> : (const void*)vhk::k_bwd_band<float>;
+    VH_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    VH_HIP(hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   // Initial state (state.rs:119-139): pos = prev = mesh positions, vel = 0.
   std::vector<double> h3(n3);
@@ -430,8 +497,10 @@ vh_status vh_pd_step_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_r
   if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
   VH_HIP(hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
-  for (uint32_t s = 0; s < nsteps; ++s) enqueue_step(ctx, dt);
-  VH_HIP(hipGetLastError());
+  for (uint32_t s = 0; s < nsteps; ++s) {
+    const vh_status ls = launch_step(ctx, dt);
+    if (ls != VH_OK) return ls;
+  }
   vh_status st = read_step_result(ctx, out);
   if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
@@ -440,6 +509,7 @@ vh_status vh_pd_step_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_r
 vh_status vh_pd_step(vh_ctx* ctx, double dt, vh_state_f32* s, vh_step_result* out) {
   if (!ctx) return VH_E_INVALID_ARG;
   if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
   if (!s || !s->pos_x || !s->pos_y || !s->pos_z || !s->vel_x || !s->vel_y || !s->vel_z || !s->prev_x ||
       !s->prev_y || !s->prev_z || !s->pred_x || !s->pred_y || !s->pred_z)
     return fail(ctx, VH_E_INVALID_ARG, "null state array");
@@ -459,8 +529,10 @@ vh_status vh_pd_step(vh_ctx* ctx, double dt, vh_state_f32* s, vh_step_result* ou
   }
   VH_HIP(hipMemcpyAsync(ctx->x, hp.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(hipMemcpyAsync(ctx->v, hv.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  enqueue_step(ctx, dt);
-  VH_HIP(hipGetLastError());
+  {
+    const vh_status ls = launch_step(ctx, dt);
+    if (ls != VH_OK) return ls;
+  }
   std::vector<double> hq(n3), hr(n3);
   VH_HIP(hipMemcpyAsync(hp.data(), ctx->x, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(hv.data(), ctx->v, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -484,14 +556,21 @@ vh_status vh_pd_residual(vh_ctx* ctx, double dt, double* out) {
   if (!out) return fail(ctx, VH_E_INVALID_ARG, "out is null");
   if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
   VH_HIP(hipSetDevice(ctx->device));
-  if (ctx->prec == VH_PREC_F64) enqueue_local_gather<double>(ctx, dt, nullptr);
-  else enqueue_local_gather<float>(ctx, dt, nullptr);
+  const int n = ctx->S.n;
+  if (ctx->prec == VH_PREC_F64) {
+    enqueue_local<double>(ctx, nullptr);
+    hipLaunchKernelGGL(vhk::k_gather<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, rhs_src<double>(ctx, dt),
+                       ctx->r);
+  } else {
+    enqueue_local<float>(ctx, nullptr);
+    hipLaunchKernelGGL(vhk::k_gather<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, rhs_src<float>(ctx, dt),
+                       ctx->r);
+  }
   VH_HIP(hipGetLastError());
-  const size_t n3 = 3 * (size_t)ctx->S.n;
+  const size_t n3 = 3 * (size_t)n;
   ctx->hbuf.resize(n3);
   VH_HIP(hipMemcpyAsync(ctx->hbuf.data(), ctx->r, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
-  const int n = ctx->S.n;
   from_dev_order(ctx->S, ctx->hbuf, out, out + n, out + 2 * (size_t)n);
   return VH_OK;
 }
@@ -506,7 +585,7 @@ vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps) {
   const size_t n3 = 3 * (size_t)n;
   to_dev_order(ctx->S, b, b + n, b + 2 * (size_t)n, ctx->hbuf);
   VH_HIP(hipMemcpyAsync(ctx->r, ctx->hbuf.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  enqueue_correction(ctx, sweeps, nullptr);
+  enqueue_correction(ctx, ctx->P.dt_matrix, sweeps, false, false, false, nullptr);
   VH_HIP(hipGetLastError());
   if (ctx->prec == VH_PREC_F64) {
     VH_HIP(hipMemcpyAsync(ctx->hbuf.data(), ctx->d, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -529,6 +608,113 @@ vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out) {
   out->nnz_factor = S.nnz_factor;
   out->max_front = S.max_front;
   out->setup_seconds = ctx->setup_seconds;
+  return VH_OK;
+}
+
+vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile* out) {
+  if (!ctx || !out) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!(dt > 0) || iterations == 0) return fail(ctx, VH_E_INVALID_ARG, "dt > 0 and iterations > 0 required");
+  const vh::Setup& S = ctx->S;
+  const int nb = S.max_depth + 1;
+  if (nb > VH_MAX_BANDS) return fail(ctx, VH_E_UNSUPPORTED, "more bands than vh_profile holds");
+  VH_HIP(hipSetDevice(ctx->device));
+  std::memset(out, 0, sizeof(*out));
+  out->bands = (uint32_t)nb;
+  out->iterations = iterations;
+  const double ts = (double)ctx->tsz;
+  // algorithmic bytes per launch
+  const double n = S.n, t = S.t, h = S.h;
+  out->local_bytes = n * 24 + t * (12 + 32 + ts + 12 + 9 * ts) + h * (16 + 2 * ts + 16 + 12 * ts);
+  std::vector<double> band_entries(nb, 0), band_rows(nb, 0), band_cols(nb, 0);
+  for (const vh::Supernode& q : S.sn) {
+    band_entries[q.depth] += (double)q.w * (q.w + 1) / 2 + (double)q.r * q.w;
+    band_rows[q.depth] += q.w + q.r;
+    band_cols[q.depth] += q.w;
+  }
+  for (int l = 0; l < nb; ++l) {
+    const int fd = S.max_depth - l;  // forward launch l runs band fd
+    // block entries + front inputs (r or RHS slots, child slots) + outputs
+    out->fwd_bytes[l] = band_entries[fd] * ts + band_rows[fd] * 3 * ts + band_cols[fd] * 24 +
+                        (l == 0 ? (double)S.inc_ptr[S.n] * 3 * ts + n * 56 : 0.0) + band_rows[fd] * 3 * ts;
+    out->bwd_bytes[l] = band_entries[l] * ts + band_rows[l] * 3 * ts + band_cols[l] * 3 * ts * 2 +
+                        (l == nb - 1 && ctx->prec == VH_PREC_F64 ? n * 48 : 0.0);
+  }
+  out->ir_bytes = ctx->prec == VH_PREC_F32_IR ? (double)S.Ai.size() * 12 + n * (12 + 48 + 48) : 0.0;
+  // events: one after every launch
+  const int per_iter = 1 + 2 * nb + (ctx->prec == VH_PREC_F32_IR ? 1 : 0);
+  std::vector<hipEvent_t> ev((size_t)iterations * per_iter + 1);
+  for (auto& e : ev) VH_HIP(hipEventCreate(&e));
+  const bool f64 = ctx->prec == VH_PREC_F64;
+  size_t k = 0;
+  VH_HIP(hipEventRecord(ev[k++], ctx->stream));
+  const vhk::StepDev* st = nullptr;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    if (f64) enqueue_local<double>(ctx, st); else enqueue_local<float>(ctx, st);
+    VH_HIP(hipEventRecord(ev[k++], ctx->stream));
+    // the same launches as one correction, with an event after each
+    const int nfwd = (int)S.fwd_tasks.size();
+    const int nupper = (int)S.upper_cols.size();
+    auto run = [&](auto zero) -> vh_status {
+      using T = decltype(zero);
+      vhk::BandArgs<T> a{};
+      a.fslot_ptr = ctx->fslot_ptr; a.uslot = ctx->uslot; a.dpush_ptr = ctx->dpush_ptr; a.dpush_idx = ctx->dpush_idx;
+      a.src = rhs_src<T>(ctx, dt); a.r = ctx->r; a.upper_cols = ctx->upper_cols; a.nupper = nupper;
+      for (int l = 0; l < nb; ++l) {
+        const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
+        a.tasks = ctx->tasks + b; a.ntasks = e - b; a.rhs_from_slots = l == 0 ? 1 : 0;
+        const int grid = (e - b) + (l == 0 ? nblk(nupper) : 0);
+        const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * ctx->tsz;
+        if (grid > 0)
+          hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, ctx->stream, a, (const T*)ctx->M,
+                             (T*)ctx->y, (T*)ctx->F, st);
+        VH_HIP(hipEventRecord(ev[k++], ctx->stream));
+      }
+      a.rhs_from_slots = 0;
+      for (int l = 0; l < nb; ++l) {
+        const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
+        const bool last = l == nb - 1;
+        a.tasks = ctx->tasks + nfwd + b; a.ntasks = e - b;
+        const bool commit = last && f64;
+        const vhk::CommitArgs cc = commit_args(ctx, commit, false);
+        const int grid = (e - b) + (commit ? nblk(nupper) : 0);
+        const size_t lds = 3 * (size_t)S.band_front[l] * ctx->tsz;
+        if (grid > 0)
+          hipLaunchKernelGGL(vhk::k_bwd_band<T>, dim3(grid), dim3(256), lds, ctx->stream, a, cc, (const T*)ctx->Mt,
+                             (const T*)ctx->y, (T*)ctx->V, (T*)ctx->d, st);
+        VH_HIP(hipEventRecord(ev[k++], ctx->stream));
+      }
+      return VH_OK;
+    };
+    vh_status rs = f64 ? run(0.0) : run(0.0f);
+    if (rs != VH_OK) return rs;
+    if (!f64) {
+      const vhk::CommitArgs cc = commit_args(ctx, true, false);
+      hipLaunchKernelGGL(vhk::k_ir_update, dim3(nblk(S.n)), dim3(256), 0, ctx->stream, S.n, ctx->Ap, ctx->Ai, ctx->Ax,
+                         (const float*)ctx->d, ctx->r, ctx->acc, 1, 0, cc, st);
+      VH_HIP(hipEventRecord(ev[k++], ctx->stream));
+    }
+  }
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  auto span = [&](size_t a, size_t b) -> double {
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
+    return 1000.0 * ms;
+  };
+  for (uint32_t it = 0; it < iterations; ++it) {
+    const size_t base = (size_t)it * per_iter;
+    size_t q = base;
+    out->local_us += span(q, q + 1); ++q;
+    for (int l = 0; l < nb; ++l) { out->fwd_us[l] += span(q, q + 1); ++q; }
+    for (int l = 0; l < nb; ++l) { out->bwd_us[l] += span(q, q + 1); ++q; }
+    if (!f64) { out->ir_us += span(q, q + 1); ++q; }
+    out->iteration_us += span(base, base + per_iter);
+  }
+  const double inv = 1.0 / iterations;
+  out->local_us *= inv; out->ir_us *= inv; out->iteration_us *= inv;
+  for (int l = 0; l < nb; ++l) { out->fwd_us[l] *= inv; out->bwd_us[l] *= inv; }
+  for (auto& e : ev) (void)hipEventDestroy(e);
   return VH_OK;
 }
 

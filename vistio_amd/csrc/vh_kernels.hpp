@@ -26,6 +26,7 @@ namespace vhk {
 struct SnDev {
   int col0, w, r, rows_off, u_off, map_off, child_off, nchild;
   long long m_off;
+  int fm_off, pad;
 };
 
 struct StepDev {
@@ -255,13 +256,17 @@ template <> struct Prec<float> { static constexpr bool f32 = true; };
 
 // Local step: one thread per triangle (blocks [0, tb)) or hinge (the rest).
 // pd_solver.rs:300-367 (the loops) and the per-element RHS terms of
-// assembly.rs:176-235.
+// assembly.rs:176-235. Each element corner's contribution goes to its own
+// slot of the vertex-sorted slot array (a hinge's contribution enters with
+// the stencil sign s = (-1, -1, +1, +1)[k], assembly.rs:210), so the RHS
+// gather is a contiguous segmented sum with no atomics.
 template <class T>
 __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, const double* __restrict__ x,
                                                const int* __restrict__ tri_idx, const double* __restrict__ tri_dm,
-                                               const T* __restrict__ tri_w, const int* __restrict__ hinge_idx,
-                                               const T* __restrict__ hinge_rest, const T* __restrict__ hinge_w,
-                                               T* __restrict__ ctri, T* __restrict__ chinge, const StepDev* st) {
+                                               const T* __restrict__ tri_w, const int* __restrict__ tri_slot,
+                                               const int* __restrict__ hinge_idx, const T* __restrict__ hinge_rest,
+                                               const T* __restrict__ hinge_w, const int* __restrict__ hinge_slot,
+                                               T* __restrict__ cslot, const StepDev* st) {
   if (st && st->done) return;
   if ((int)blockIdx.x < tb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -283,12 +288,12 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, 
       membrane_delta_f64(p0, p1, p2, dm, corot, D0, D1);
       Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
     }
-    T* o = ctri + 9 * (size_t)e;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      o[3 * k + 0] = w * (g[k][0] * Dx0 + g[k][1] * Dx1);
-      o[3 * k + 1] = w * (g[k][0] * Dy0 + g[k][1] * Dy1);
-      o[3 * k + 2] = w * (g[k][0] * Dz0 + g[k][1] * Dz1);
+      T* o = cslot + 3 * (size_t)tri_slot[3 * e + k];
+      o[0] = w * (g[k][0] * Dx0 + g[k][1] * Dx1);
+      o[1] = w * (g[k][0] * Dy0 + g[k][1] * Dy1);
+      o[2] = w * (g[k][0] * Dz0 + g[k][1] * Dz1);
     }
   } else {
     const int e = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
@@ -304,42 +309,51 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, 
       const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)hinge_rest[e]);
       cx = c.x; cy = c.y; cz = c.z;
     }
-    T* o = chinge + 3 * (size_t)e;
-    o[0] = w * cx; o[1] = w * cy; o[2] = w * cz;
+    cx *= w; cy *= w; cz *= w;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      T* o = cslot + 3 * (size_t)hinge_slot[4 * e + k];
+      const T sg = k < 2 ? T(-1) : T(1);
+      o[0] = sg * cx; o[1] = sg * cy; o[2] = sg * cz;
+    }
   }
 }
 
-// Vertex gather (no atomics): r_i = m_i (pred_i / dt^2 - x_i / dt_A^2)
-//   + sum of the vertex's triangle-corner slots + s_k * hinge slots.
-// Slot code c >= 0: triangle corner c = 3e + k; c < 0: hinge slot
-// (-c - 1) = 4e + k with stencil sign s = (-1, -1, +1, +1)[k] (assembly.rs:210).
+// Inputs of the RHS r = b - A x at one vertex: the inertia term
+// m_i (pred_i / dt^2 - x_i / dt_A^2) and the vertex's contiguous slots.
 template <class T>
-__global__ void __launch_bounds__(256) k_gather(int n, const int* __restrict__ inc_ptr, const int* __restrict__ inc,
-                                                const T* __restrict__ ctri, const T* __restrict__ chinge,
-                                                const double* __restrict__ mass, const double* __restrict__ pred,
-                                                const double* __restrict__ x, double inv_dt2, double inv_dtm2,
-                                                double* __restrict__ r, const StepDev* st) {
-  if (st && st->done) return;
+struct RhsSrc {
+  const int* inc_ptr;
+  const T* cslot;
+  const double* mass;
+  const double* pred;
+  const double* x;
+  double inv_dt2, inv_dtm2;
+};
+
+template <class T>
+__device__ __forceinline__ double3 rhs_from_slots(const RhsSrc<T>& s, int i) {
+  T ax = 0, ay = 0, az = 0;
+  const int pe = s.inc_ptr[i + 1];
+  int p = s.inc_ptr[i];
+#pragma unroll 4
+  for (; p < pe; ++p) {
+    const T* c = s.cslot + 3 * (size_t)p;
+    ax += c[0]; ay += c[1]; az += c[2];
+  }
+  const double m = s.mass[i];
+  return make_double3(m * (s.inv_dt2 * s.pred[3 * i + 0] - s.inv_dtm2 * s.x[3 * i + 0]) + (double)ax,
+                      m * (s.inv_dt2 * s.pred[3 * i + 1] - s.inv_dtm2 * s.x[3 * i + 1]) + (double)ay,
+                      m * (s.inv_dt2 * s.pred[3 * i + 2] - s.inv_dtm2 * s.x[3 * i + 2]) + (double)az);
+}
+
+// Stand-alone RHS gather (residual inspection entry point).
+template <class T>
+__global__ void __launch_bounds__(256) k_gather(int n, RhsSrc<T> s, double* __restrict__ r) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  T ax = 0, ay = 0, az = 0;
-  const int pe = inc_ptr[i + 1];
-  for (int p = inc_ptr[i]; p < pe; ++p) {
-    const int c = inc[p];
-    if (c >= 0) {
-      const T* s = ctri + 3 * (size_t)c;
-      ax += s[0]; ay += s[1]; az += s[2];
-    } else {
-      const int sl = -c - 1;
-      const T* s = chinge + 3 * (size_t)(sl >> 2);
-      const T sg = (sl & 3) < 2 ? T(-1) : T(1);
-      ax += sg * s[0]; ay += sg * s[1]; az += sg * s[2];
-    }
-  }
-  const double m = mass[i];
-  r[3 * i + 0] = m * (inv_dt2 * pred[3 * i + 0] - inv_dtm2 * x[3 * i + 0]) + (double)ax;
-  r[3 * i + 1] = m * (inv_dt2 * pred[3 * i + 1] - inv_dtm2 * x[3 * i + 1]) + (double)ay;
-  r[3 * i + 2] = m * (inv_dt2 * pred[3 * i + 2] - inv_dtm2 * x[3 * i + 2]) + (double)az;
+  const double3 v = rhs_from_slots(s, i);
+  r[3 * i + 0] = v.x; r[3 * i + 1] = v.y; r[3 * i + 2] = v.z;
 }
 
 template <class T>
@@ -348,65 +362,156 @@ __device__ __forceinline__ T group_sum(T v, int g) {
   return v;
 }
 
-// Forward sweep of one level, L y = r. Task = rows [b, e) of supernode P.
-// front[0:w+r) = extend-add of the children's update vectors (children in a
-// fixed order, each map injective: no atomics); t = r_P - front[0:w);
-//   rows i < w : y_P[i]     = sum_k Linv[i][k] t[k]
-//   rows w + q : U_P[q]     = sum_k W[q][k] t[k] + front[w + q]
-// with W = L_IP L_PP^{-1}, so U_P is P's whole update to its ancestors.
+// Dot product of one stored row (len entries, contiguous) with an LDS vector
+// of xyz triples, split over the g lanes of a group; 8 independent loads in
+// flight per lane, partial sums reduced across the group.
 template <class T>
-__global__ void __launch_bounds__(256) k_fwd(const int4* __restrict__ tasks, const SnDev* __restrict__ sn,
-                                             const int* __restrict__ emap, const int* __restrict__ children,
-                                             const T* __restrict__ M, const double* __restrict__ rhs,
-                                             T* __restrict__ y, T* __restrict__ U, const StepDev* st) {
-  if (st && st->done) return;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* front = reinterpret_cast<T*>(smem_raw);
-  const int4 task = tasks[blockIdx.x];
-  const SnDev P = sn[task.x];
-  const int w = P.w, m = P.w + P.r;
-  const int tid = threadIdx.x;
-  for (int k = tid; k < 3 * m; k += blockDim.x) front[k] = T(0);
-  __syncthreads();
-  for (int c = 0; c < P.nchild; ++c) {
-    const SnDev C = sn[children[P.child_off + c]];
-    for (int q = tid; q < C.r; q += blockDim.x) {
-      const int pos = emap[C.map_off + q];
-      const T* u = U + 3 * (size_t)(C.u_off + q);
-      front[3 * pos + 0] += u[0];
-      front[3 * pos + 1] += u[1];
-      front[3 * pos + 2] += u[2];
+__device__ __forceinline__ void row_dot(const T* __restrict__ row, int len, const T* __restrict__ vec, int lane, int g,
+                                        T& sx, T& sy, T& sz) {
+  T ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+  int k = lane;
+  for (; k + 7 * g < len; k += 8 * g) {
+    T mv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) mv[u] = __builtin_nontemporal_load(row + k + u * g);
+#pragma unroll
+    for (int u = 0; u < 8; u += 2) {
+      const T* v0 = vec + 3 * (k + u * g);
+      const T* v1 = vec + 3 * (k + (u + 1) * g);
+      ax += mv[u] * v0[0]; ay += mv[u] * v0[1]; az += mv[u] * v0[2];
+      bx += mv[u + 1] * v1[0]; by += mv[u + 1] * v1[1]; bz += mv[u + 1] * v1[2];
     }
+  }
+  for (; k < len; k += g) {
+    const T m0 = __builtin_nontemporal_load(row + k);
+    const T* v0 = vec + 3 * k;
+    ax += m0 * v0[0]; ay += m0 * v0[1]; az += m0 * v0[2];
+  }
+  sx = group_sum(ax + bx, g);
+  sy = group_sum(ay + by, g);
+  sz = group_sum(az + bz, g);
+}
+
+// Solve task descriptor: everything a task needs in one 48-byte record.
+//   q0 = (col0, w, r, m_off)   supernode columns, off-diagonal rows, block offset
+//   q1 = (b, e, g, u_off)      row range, lanes per row, update-row / v-slot base
+//   q2 = (fm_off, 0, 0, 0)     front slot pointer base
+struct TaskDev {
+  int4 q0, q1, q2;
+};
+
+// Commit of one PD iteration for one vertex, pd_solver.rs:402-438: x +=
+// delta, pinned -> prev, ground clamp; returns |delta|^2 and |x_old|^2.
+struct CommitArgs {
+  int on;           // commit in this launch
+  int has_ground;
+  double ground;
+  double* x;
+  const double* prev;
+  const uint8_t* pinned;
+  double* partials;  // per-block (diff, norm), when want_resid
+  int want_resid;
+};
+
+__device__ __forceinline__ void commit_vertex(const CommitArgs& c, int i, double dx, double dy, double dz,
+                                              double& df, double& nm) {
+  const double ox = c.x[3 * i + 0], oy = c.x[3 * i + 1], oz = c.x[3 * i + 2];
+  df += dx * dx + dy * dy + dz * dz;
+  nm += ox * ox + oy * oy + oz * oz;
+  double px = ox + dx, py = oy + dy, pz = oz + dz;
+  if (c.pinned[i]) { px = c.prev[3 * i + 0]; py = c.prev[3 * i + 1]; pz = c.prev[3 * i + 2]; }
+  else if (c.has_ground && py < c.ground) py = c.ground;
+  c.x[3 * i + 0] = px; c.x[3 * i + 1] = py; c.x[3 * i + 2] = pz;
+}
+
+// Block sum of (df, nm) into partials[blockIdx.x] (fixed order).
+__device__ inline void block_partials(double df, double nm, double* partials) {
+  __shared__ double sd[256], sn[256];
+  sd[threadIdx.x] = df; sn[threadIdx.x] = nm;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
     __syncthreads();
   }
-  for (int j = tid; j < w; j += blockDim.x) {
-    const double* b = rhs + 3 * (size_t)(P.col0 + j);
-    front[3 * j + 0] = (T)b[0] - front[3 * j + 0];
-    front[3 * j + 1] = (T)b[1] - front[3 * j + 1];
-    front[3 * j + 2] = (T)b[2] - front[3 * j + 2];
+  if (threadIdx.x == 0) { partials[2 * blockIdx.x] = sd[0]; partials[2 * blockIdx.x + 1] = sn[0]; }
+}
+
+template <class T>
+struct BandArgs {
+  const TaskDev* tasks;
+  int ntasks;
+  const int* fslot_ptr;
+  const int* uslot;
+  const int* dpush_ptr;
+  const int* dpush_idx;
+  // RHS: from the slots (first forward band of the first sweep, which also
+  // stores r) or from r
+  int rhs_from_slots;
+  RhsSrc<T> src;
+  double* r;
+  // extra blocks after the tasks: RHS (forward) or commit (backward) of the
+  // columns outside the deepest band
+  const int* upper_cols;
+  int nupper;
+};
+
+// Forward task (L y = r): rows [b, e) of supernode P's block
+//   M_P = [L_PP^{-1}; L_IP L_PP^{-1}] (row-major, row i < w has i+1 entries).
+// front[j] = r_P[j] - (sum of the slots the children pushed for position j),
+// pass-through front[w + q] = slot sum only; then
+//   rows i < w : y_P[i]            = sum_k M[i][k] t[k]
+//   rows w + q : slot(U_P[q])      = sum_k M[w+q][k] t[k] + front[w + q]
+// so each update row lands in its slot of the parent's front. Only the front
+// entries the task's rows read are assembled.
+template <class T>
+__device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
+                         T* __restrict__ F, T* front) {
+  const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
+  const int* fp = a.fslot_ptr + td.q2.x;
+  const int tid = threadIdx.x;
+  const int tlen = e <= w ? e : w;
+  const bool store_r = a.rhs_from_slots && e >= w;
+  for (int j = tid; j < tlen; j += blockDim.x) {
+    T ax = 0, ay = 0, az = 0;
+    for (int q = fp[j]; q < fp[j + 1]; ++q) {
+      const T* u = F + 3 * (size_t)q;
+      ax += u[0]; ay += u[1]; az += u[2];
+    }
+    double3 rb;
+    if (a.rhs_from_slots) {
+      rb = rhs_from_slots(a.src, col0 + j);
+      if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z; }
+    } else {
+      rb = ld3(a.r, col0 + j);
+    }
+    front[3 * j + 0] = (T)rb.x - ax;
+    front[3 * j + 1] = (T)rb.y - ay;
+    front[3 * j + 2] = (T)rb.z - az;
+  }
+  for (int j = (b > w ? b : w) + tid; j < e; j += blockDim.x) {
+    T ax = 0, ay = 0, az = 0;
+    for (int q = fp[j]; q < fp[j + 1]; ++q) {
+      const T* u = F + 3 * (size_t)q;
+      ax += u[0]; ay += u[1]; az += u[2];
+    }
+    front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
   }
   __syncthreads();
-  const int g = task.w;
   const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
-  const T* Mp = M + P.m_off;
+  const T* Mp = M + moff;
   const long long tri = (long long)w * (w + 1) / 2;
-  for (int row = task.y + grp; row < task.z; row += ngrp) {
+  for (int row = b + grp; row < e; row += ngrp) {
     const int len = row < w ? row + 1 : w;
     const T* Mr = Mp + (row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w);
-    T sx = 0, sy = 0, sz = 0;
-    for (int k = lane; k < len; k += g) {
-      const T mv = Mr[k];
-      sx += mv * front[3 * k + 0];
-      sy += mv * front[3 * k + 1];
-      sz += mv * front[3 * k + 2];
-    }
-    sx = group_sum(sx, g); sy = group_sum(sy, g); sz = group_sum(sz, g);
+    T sx, sy, sz;
+    row_dot(Mr, len, front, lane, g, sx, sy, sz);
     if (lane == 0) {
       if (row < w) {
-        T* o = y + 3 * (size_t)(P.col0 + row);
+        T* o = y + 3 * (size_t)(col0 + row);
         o[0] = sx; o[1] = sy; o[2] = sz;
       } else {
-        T* o = U + 3 * (size_t)(P.u_off + row - w);
+        T* o = F + 3 * (size_t)a.uslot[uoff + row - w];
         o[0] = sx + front[3 * row + 0];
         o[1] = sy + front[3 * row + 1];
         o[2] = sz + front[3 * row + 2];
@@ -415,88 +520,153 @@ __global__ void __launch_bounds__(256) k_fwd(const int4* __restrict__ tasks, con
   }
 }
 
-// Backward sweep of one level, L^T d = y. Task = columns [b, e) of P, C =
-// task.w columns per pass, S = 256 / C row slices:
-//   d_P[j] = sum_{k >= j} Linv[k][j] y_P[k] - sum_q W[q][j] d[I_P[q]].
-// Lanes of a wave read consecutive columns of one stored row (coalesced);
-// the S slice partials are summed in a fixed order.
+// Backward task (L^T d = y): columns [b, e) of P, read as rows of the
+// transposed copy Mt (row j = column j of M_P, m - j entries). With
+// v = [y_P ; V_P] where V_P[q] = -d[I_P[q]] was pushed by the owner of that
+// column,
+//   d_P[j] = sum_{k >= j} Linv[k][j] y_P[k] - sum_q W[q][j] d[I_P[q]]
+//          = sum_i Mt[j][i] v[j + i].
+// Each d_P[j] is stored, pushed (negated) to the v slots of the descendants
+// that read it, and, when committing, added to x.
 template <class T>
-__global__ void __launch_bounds__(256) k_bwd(const int4* __restrict__ tasks, const SnDev* __restrict__ sn,
-                                             const int* __restrict__ rowidx, const T* __restrict__ M,
-                                             const T* __restrict__ y, T* __restrict__ d, const StepDev* st) {
-  if (st && st->done) return;
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int4 task = tasks[blockIdx.x];
-  const SnDev P = sn[task.x];
-  const int w = P.w, m = P.w + P.r;
-  const int C = task.w, S = blockDim.x / C;
-  T* v = reinterpret_cast<T*>(smem_raw);
-  T* part = v + 3 * m;
+__device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
+                         const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, double& df, double& nm) {
+  const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
+  const int m = w + r;
   const int tid = threadIdx.x;
-  for (int k = tid; k < m; k += blockDim.x) {
-    if (k < w) {
-      const T* s = y + 3 * (size_t)(P.col0 + k);
-      v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
-    } else {
-      const T* s = d + 3 * (size_t)rowidx[P.rows_off + k - w];
-      v[3 * k + 0] = -s[0]; v[3 * k + 1] = -s[1]; v[3 * k + 2] = -s[2];
-    }
+  for (int k = b + tid; k < m; k += blockDim.x) {
+    const T* s = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
+    v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
   }
   __syncthreads();
-  const int jl = tid % C, sl = tid / C;
-  const int j = task.y + jl;
-  const T* Mp = M + P.m_off;
-  const long long tri = (long long)w * (w + 1) / 2;
-  T sx = 0, sy = 0, sz = 0;
-  if (j < task.z) {
-    for (int k = task.y + sl; k < w; k += S) {
-      if (k >= j) {
-        const T mv = Mp[(long long)k * (k + 1) / 2 + j];
-        sx += mv * v[3 * k + 0]; sy += mv * v[3 * k + 1]; sz += mv * v[3 * k + 2];
+  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  const T* Tp = Mt + moff;
+  for (int j = b + grp; j < e; j += ngrp) {
+    const T* row = Tp + (long long)j * m - (long long)j * (j - 1) / 2;
+    T sx, sy, sz;
+    row_dot(row, m - j, v + 3 * j, lane, g, sx, sy, sz);
+    if (lane == 0) {
+      const int col = col0 + j;
+      T* o = d + 3 * (size_t)col;
+      o[0] = sx; o[1] = sy; o[2] = sz;
+      for (int q = a.dpush_ptr[col]; q < a.dpush_ptr[col + 1]; ++q) {
+        T* p = V + 3 * (size_t)a.dpush_idx[q];
+        p[0] = -sx; p[1] = -sy; p[2] = -sz;
       }
+      if (c.on) commit_vertex(c, col, (double)sx, (double)sy, (double)sz, df, nm);
     }
-    for (int q = sl; q < P.r; q += S) {
-      const T mv = Mp[tri + (long long)q * w + j];
-      const int k = w + q;
-      sx += mv * v[3 * k + 0]; sy += mv * v[3 * k + 1]; sz += mv * v[3 * k + 2];
-    }
-  }
-  part[3 * tid + 0] = sx; part[3 * tid + 1] = sy; part[3 * tid + 2] = sz;
-  __syncthreads();
-  if (sl == 0 && j < task.z) {
-    T ax = 0, ay = 0, az = 0;
-    for (int s = 0; s < S; ++s) {
-      const int u = s * C + jl;
-      ax += part[3 * u + 0]; ay += part[3 * u + 1]; az += part[3 * u + 2];
-    }
-    T* o = d + 3 * (size_t)(P.col0 + j);
-    o[0] = ax; o[1] = ay; o[2] = az;
   }
 }
 
-// Iterative refinement (FP32 solve, FP64 residual): acc += d; if update_r,
-// r -= A d (A in FP64 CSR, permuted order). Rows only read d, so the in-place
-// update of r is race-free.
+// One band of the forward sweep per launch. Blocks [0, ntasks) run tasks;
+// in the first band of a sweep that starts from the slots, the remaining
+// blocks compute (and store) r for the columns outside the deepest band.
+template <class T>
+__global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
+                                                  T* __restrict__ F, const StepDev* st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  if ((int)blockIdx.x < a.ntasks) {
+    fwd_task<T>(a.tasks[blockIdx.x], a, M, y, F, reinterpret_cast<T*>(smem_raw));
+    return;
+  }
+  const int k = ((int)blockIdx.x - a.ntasks) * blockDim.x + threadIdx.x;
+  if (k >= a.nupper) return;
+  const int col = a.upper_cols[k];
+  const double3 rb = rhs_from_slots(a.src, col);
+  double* o = a.r + 3 * (size_t)col;
+  o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
+}
+
+// One band of the backward sweep per launch. In the last band of an FP64
+// solve the tasks also commit their columns and the extra blocks commit the
+// columns outside the deepest band (solved by earlier launches).
+template <class T>
+__global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Mt,
+                                                  const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
+                                                  const StepDev* st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double df = 0, nm = 0;
+  if ((int)blockIdx.x < a.ntasks) {
+    bwd_task<T>(a.tasks[blockIdx.x], a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+  } else {
+    const int k = ((int)blockIdx.x - a.ntasks) * blockDim.x + threadIdx.x;
+    if (c.on && k < a.nupper) {
+      const int col = a.upper_cols[k];
+      const T* s = d + 3 * (size_t)col;
+      commit_vertex(c, col, (double)s[0], (double)s[1], (double)s[2], df, nm);
+    }
+  }
+  if (c.on && c.want_resid) {
+    __syncthreads();
+    block_partials(df, nm, c.partials);
+  }
+}
+
+// Iterative refinement (FP32 solve, FP64 residual): delta += d. Between
+// sweeps r -= A d (A in FP64 CSR, permuted order; rows only read d, so the
+// in-place update of r is race-free). After the last sweep the iteration is
+// committed with the accumulated delta.
 __global__ void __launch_bounds__(256) k_ir_update(int n, const int* __restrict__ Ap, const int* __restrict__ Ai,
                                                    const double* __restrict__ Ax, const float* __restrict__ d,
                                                    double* __restrict__ r, double* __restrict__ acc, int first,
-                                                   int update_r, const StepDev* st) {
+                                                   int update_r, CommitArgs c, const StepDev* st) {
   if (st && st->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  for (int c = 0; c < 3; ++c) {
-    const double dv = (double)d[3 * i + c];
-    acc[3 * i + c] = first ? dv : acc[3 * i + c] + dv;
-  }
-  if (update_r) {
-    double sx = 0, sy = 0, sz = 0;
-    for (int p = Ap[i]; p < Ap[i + 1]; ++p) {
-      const double a = Ax[p];
-      const float* dv = d + 3 * (size_t)Ai[p];
-      sx += a * (double)dv[0]; sy += a * (double)dv[1]; sz += a * (double)dv[2];
+  double df = 0, nm = 0;
+  if (i < n) {
+    double a3[3];
+    for (int k = 0; k < 3; ++k) {
+      const double dv = (double)d[3 * i + k];
+      a3[k] = first ? dv : acc[3 * i + k] + dv;
     }
-    r[3 * i + 0] -= sx; r[3 * i + 1] -= sy; r[3 * i + 2] -= sz;
+    if (c.on) {
+      commit_vertex(c, i, a3[0], a3[1], a3[2], df, nm);
+    } else {
+      for (int k = 0; k < 3; ++k) acc[3 * i + k] = a3[k];
+    }
+    if (update_r) {
+      double sx = 0, sy = 0, sz = 0;
+      for (int p = Ap[i]; p < Ap[i + 1]; ++p) {
+        const double av = Ax[p];
+        const float* dv = d + 3 * (size_t)Ai[p];
+        sx += av * (double)dv[0]; sy += av * (double)dv[1]; sz += av * (double)dv[2];
+      }
+      r[3 * i + 0] -= sx; r[3 * i + 1] -= sy; r[3 * i + 2] -= sz;
+    }
   }
+  if (c.on && c.want_resid) block_partials(df, nm, c.partials);
+}
+
+// Final residual of a PD iteration (pd_solver.rs:402-421, 440-444): sums the
+// per-block (|delta|^2, |x|^2) partials in a fixed order.
+__global__ void __launch_bounds__(256) k_resid_reduce(const double* __restrict__ partials, int count, double tol,
+                                                      int iter, StepDev* st) {
+  if (st->done) return;
+  __shared__ double sd[256], sn[256];
+  double a = 0, b = 0;
+  for (int k = threadIdx.x; k < count; k += blockDim.x) { a += partials[2 * k]; b += partials[2 * k + 1]; }
+  sd[threadIdx.x] = a; sn[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double res = sn[0] > 1e-12 ? sqrt(sd[0] / sn[0]) : sqrt(sd[0]);
+    st->residual = res;
+    st->iterations = iter + 1;
+    if (res < tol) st->done = 1;
+  }
+}
+
+// Iteration counter when no residual is formed (tolerance 0, not the last
+// iteration): keeps StepResult.iterations exact.
+__global__ void k_count_iter(int iter, StepDev* st) {
+  if (st->done) return;
+  st->iterations = iter + 1;
 }
 
 // Prediction, state.rs:145-169 and pd_solver.rs:274-283: prev = x;
@@ -518,69 +688,6 @@ __global__ void __launch_bounds__(256) k_predict(int n, double* __restrict__ x, 
     if (has_ground && p[1] < ground) p[1] = ground;
   }
   for (int c = 0; c < 3; ++c) { pred[3 * i + c] = p[c]; x[3 * i + c] = p[c]; }
-}
-
-// Commit one PD iteration, pd_solver.rs:402-444: diff = |delta|^2 and norm =
-// |x|^2 summed in FP64, x += delta, pinned -> prev, ground clamp. The last
-// block to arrive reduces the per-block partials in a fixed order and sets
-// final_residual / iterations / done.
-template <class D>
-__global__ void __launch_bounds__(256) k_commit(int n, double* __restrict__ x, const D* __restrict__ delta,
-                                                const double* __restrict__ prev, const uint8_t* __restrict__ pinned,
-                                                int has_ground, double ground, double tol, int iter,
-                                                double* __restrict__ partials, StepDev* st) {
-  if (st->done) return;
-  __shared__ double sd[256], sn[256];
-  __shared__ bool last;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double df = 0, nm = 0;
-  if (i < n) {
-    double p[3];
-    for (int c = 0; c < 3; ++c) {
-      const double o = x[3 * i + c];
-      const double dl = (double)delta[3 * i + c];
-      df += dl * dl;
-      nm += o * o;
-      p[c] = o + dl;
-    }
-    if (pinned[i]) { for (int c = 0; c < 3; ++c) p[c] = prev[3 * i + c]; }
-    else if (has_ground && p[1] < ground) p[1] = ground;
-    for (int c = 0; c < 3; ++c) x[3 * i + c] = p[c];
-  }
-  sd[threadIdx.x] = df; sn[threadIdx.x] = nm;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    partials[2 * blockIdx.x] = sd[0];
-    partials[2 * blockIdx.x + 1] = sn[0];
-    __threadfence();
-    const unsigned ticket = atomicAdd(&st->counter, 1u);
-    last = ticket == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double a = 0, b = 0;
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += blockDim.x) {
-    a += __builtin_nontemporal_load(&partials[2 * k]);
-    b += __builtin_nontemporal_load(&partials[2 * k + 1]);
-  }
-  sd[threadIdx.x] = a; sn[threadIdx.x] = b;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) { sd[threadIdx.x] += sd[threadIdx.x + s]; sn[threadIdx.x] += sn[threadIdx.x + s]; }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const double res = sn[0] > 1e-12 ? sqrt(sd[0] / sn[0]) : sqrt(sd[0]);
-    st->residual = res;
-    st->iterations = iter + 1;
-    st->counter = 0;
-    if (res < tol) st->done = 1;
-  }
 }
 
 // Finalize, pd_solver.rs:447-466 and state.rs:195-240.

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <numeric>
 
 #include "vistio_hip.h"
@@ -250,31 +251,129 @@ bool build_setup(const SetupInput& in, Setup& S) {
     std::iota(all.begin(), all.end(), 0);
     nb.build(all);
   }
-  const int ns = (int)nb.nodes.size();
-  S.perm.assign(n, 0); S.iperm.assign(n, 0);
-  S.sn.assign(ns, Supernode());
-  std::vector<int> sub_lo(ns);
+  // Fine dissection tree: parents, depths, sizes, and each subtree's
+  // boundary size r (vertices outside the subtree adjacent to it, all of
+  // them in ancestor separators).
+  const int nf = (int)nb.nodes.size();
+  std::vector<int> fpar(nf, -1), fdep(nf, 0), fw(nf), flo(nf), fhi(nf), fr(nf, 0);
+  std::vector<int> fperm(n), fiperm(n);
   {
     int col = 0;
-    for (int s = 0; s < ns; ++s) {  // nodes are already in postorder
-      const NdNode& nd = nb.nodes[s];
-      sub_lo[s] = nd.left >= 0 ? sub_lo[nd.left] : col;
-      S.sn[s].col0 = col;
-      S.sn[s].w = (int)nd.verts.size();
-      for (int v : nd.verts) { S.perm[col] = v; S.iperm[v] = col; ++col; }
-      if (nd.left >= 0) { S.sn[nd.left].parent = s; S.sn[nd.right].parent = s; }
+    for (int x = 0; x < nf; ++x) {  // nodes are in postorder
+      const NdNode& nd = nb.nodes[x];
+      fw[x] = (int)nd.verts.size();
+      flo[x] = nd.left >= 0 ? flo[nd.left] : col;
+      for (int v : nd.verts) { fperm[col] = v; fiperm[v] = col; ++col; }
+      fhi[x] = col;
+      if (nd.left >= 0) { fpar[nd.left] = x; fpar[nd.right] = x; }
     }
   }
-  for (int s = ns - 1; s >= 0; --s) S.sn[s].depth = S.sn[s].parent < 0 ? 0 : S.sn[S.sn[s].parent].depth + 1;
-  S.max_depth = 0;
-  for (const auto& x : S.sn) S.max_depth = std::max(S.max_depth, x.depth);
-  S.children.clear();
-  for (int s = 0; s < ns; ++s) {
-    const NdNode& nd = nb.nodes[s];
-    S.sn[s].child_off = (int)S.children.size();
-    S.sn[s].nchild = nd.left >= 0 ? 2 : 0;
-    if (nd.left >= 0) { S.children.push_back(nd.left); S.children.push_back(nd.right); }
+  int dmax = 0;
+  for (int x = nf - 1; x >= 0; --x) { fdep[x] = fpar[x] < 0 ? 0 : fdep[fpar[x]] + 1; dmax = std::max(dmax, fdep[x]); }
+  {
+    std::vector<int> mk(n, -1);
+    for (int x = 0; x < nf; ++x)
+      for (int c = flo[x]; c < fhi[x]; ++c) {
+        const int v = fperm[c];
+        for (int p = Ap[v]; p < Ap[v + 1]; ++p) {
+          const int u = Ai[p];
+          if (fiperm[u] >= fhi[x] && mk[u] != x) { mk[u] = x; fr[x]++; }
+        }
+      }
   }
+  // W[x][b]: columns of x's subtree nodes with depth <= b.
+  const int L = dmax + 1;
+  std::vector<int64_t> Wb((size_t)nf * L, 0);
+  for (int x = 0; x < nf; ++x) {
+    const NdNode& nd = nb.nodes[x];
+    for (int b = fdep[x]; b < L; ++b) {
+      int64_t w = fw[x];
+      if (nd.left >= 0 && b > fdep[x]) w += Wb[(size_t)nd.left * L + b] + Wb[(size_t)nd.right * L + b];
+      Wb[(size_t)x * L + b] = w;
+    }
+  }
+  // Bands by dynamic programming over levels.
+  const int64_t front_cap = (int64_t)(150 * 1024) / (3 * std::max(4, in.value_bytes));
+  const int maxb = std::max(1, in.max_band_levels);
+  std::vector<std::vector<int>> at_depth(L);
+  for (int x = 0; x < nf; ++x) at_depth[fdep[x]].push_back(x);
+  auto band_cost = [&](int a, int b) -> double {
+    double entries = 0;
+    for (int x : at_depth[a]) {
+      const double W = (double)Wb[(size_t)x * L + b];
+      if (W + fr[x] > front_cap) return 1e300;
+      entries += W * (W + 1) / 2 + (double)fr[x] * W;
+    }
+    return 2.0 * (in.band_launch_us * 1e-6 + entries * in.value_bytes / (in.band_gbps * 1e9));
+  };
+  std::vector<double> best(L + 1, 1e300);
+  std::vector<int> from(L + 1, 0);
+  best[0] = 0;
+  for (int e = 1; e <= L; ++e)
+    for (int a = std::max(0, e - maxb); a < e; ++a) {
+      const double c = best[a] + band_cost(a, e - 1);
+      if (c < best[e]) { best[e] = c; from[e] = a; }
+    }
+  if (!(best[L] < 1e299)) { S.status = VH_E_UNSUPPORTED; S.err = "a separator front exceeds the LDS budget"; return false; }
+  std::vector<int> bands;
+  for (int e = L; e > 0; e = from[e]) bands.push_back(from[e]);
+  std::reverse(bands.begin(), bands.end());
+  std::vector<int> band_of_depth(L, 0), band_end(bands.size());
+  for (size_t k = 0; k < bands.size(); ++k) {
+    band_end[k] = k + 1 < bands.size() ? bands[k + 1] - 1 : L - 1;
+    for (int d = bands[k]; d <= band_end[k]; ++d) band_of_depth[d] = (int)k;
+  }
+  S.band_start = bands;
+  S.nd_levels = L;
+
+  // Supernodes: each node at a band's first level with its descendants inside
+  // the band. Columns in postorder of supernodes; within one, the members'
+  // columns in dissection postorder.
+  S.perm.assign(n, 0); S.iperm.assign(n, 0);
+  S.sn.clear();
+  S.children.clear();
+  std::vector<int> sub_lo;
+  {
+    int col = 0;
+    std::vector<std::vector<int>> kids;  // children per supernode, filled as they complete
+    std::function<int(int)> visit = [&](int root) -> int {
+      const int bend = band_end[band_of_depth[fdep[root]]];
+      std::vector<int> members, stack{root};
+      while (!stack.empty()) {
+        const int x = stack.back(); stack.pop_back();
+        members.push_back(x);
+        const NdNode& nd = nb.nodes[x];
+        if (nd.left >= 0 && fdep[x] < bend) { stack.push_back(nd.left); stack.push_back(nd.right); }
+      }
+      std::sort(members.begin(), members.end());  // fine node ids are postorder
+      std::vector<int> ch;
+      const int lo = col;
+      for (int x : members) {
+        const NdNode& nd = nb.nodes[x];
+        if (nd.left >= 0 && fdep[x] == bend) { ch.push_back(visit(nd.left)); ch.push_back(visit(nd.right)); }
+      }
+      Supernode sn;
+      sn.col0 = col;
+      for (int x : members)
+        for (int v : nb.nodes[x].verts) { S.perm[col] = v; S.iperm[v] = col; ++col; }
+      sn.w = col - sn.col0;
+      sn.depth = band_of_depth[fdep[root]];
+      const int id = (int)S.sn.size();
+      S.sn.push_back(sn);
+      sub_lo.push_back(lo);
+      kids.push_back(ch);
+      for (int c : ch) S.sn[c].parent = id;
+      return id;
+    };
+    visit(nf - 1);
+    for (size_t s = 0; s < S.sn.size(); ++s) {
+      S.sn[s].child_off = (int)S.children.size();
+      S.sn[s].nchild = (int)kids[s].size();
+      S.children.insert(S.children.end(), kids[s].begin(), kids[s].end());
+    }
+  }
+  const int ns = (int)S.sn.size();
+  S.max_depth = (int)bands.size() - 1;
 
   // Permuted matrix.
   S.Ap.assign(n + 1, 0);
@@ -325,6 +424,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
   S.nnz_factor = moff;
   S.u_total = uoff;
   S.M.assign((size_t)moff, 0.0);
+  S.Mt.assign((size_t)moff, 0.0);
   S.emap.assign((size_t)uoff, 0);
   std::vector<int> loc(n, -1);
   for (int s = 0; s < ns; ++s) {
@@ -342,6 +442,39 @@ bool build_setup(const SetupInput& in, Setup& S) {
     }
     for (int q = 0; q < p.r; ++q) loc[S.rowidx[p.rows_off + q]] = -1;
     S.sn[s].map_off = x.u_off;
+  }
+
+  // Forward push slots (inverse of the extend-add maps) and backward push
+  // lists.
+  {
+    S.fslot_ptr.clear();
+    S.uslot.assign((size_t)S.u_total, 0);
+    std::vector<std::vector<int32_t>> lists;
+    int32_t slot = 0;
+    for (int s = 0; s < ns; ++s) {
+      Supernode& x = S.sn[s];
+      const int m = x.w + x.r;
+      lists.assign(m, {});
+      for (int c = 0; c < x.nchild; ++c) {
+        const Supernode& y = S.sn[S.children[x.child_off + c]];
+        for (int q = 0; q < y.r; ++q) lists[S.emap[y.u_off + q]].push_back(y.u_off + q);
+      }
+      x.fm_off = (int)S.fslot_ptr.size();
+      for (int j = 0; j < m; ++j) {
+        S.fslot_ptr.push_back(slot);
+        for (int32_t u : lists[j]) S.uslot[u] = slot++;
+      }
+      S.fslot_ptr.push_back(slot);
+    }
+    std::vector<int32_t> cnt(n + 1, 0);
+    for (int s = 0; s < ns; ++s)
+      for (int q = 0; q < S.sn[s].r; ++q) cnt[S.rowidx[S.sn[s].rows_off + q] + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    S.dpush_ptr = cnt;
+    S.dpush_idx.assign((size_t)cnt[n], 0);
+    std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
+    for (int s = 0; s < ns; ++s)
+      for (int q = 0; q < S.sn[s].r; ++q) S.dpush_idx[cur[S.rowidx[S.sn[s].rows_off + q]]++] = S.sn[s].u_off + q;
   }
 
   // Multifrontal numeric LL^T (f64) and the partitioned-inverse blocks.
@@ -422,6 +555,14 @@ bool build_setup(const SetupInput& in, Setup& S) {
         Wp[(size_t)q * w + j] = v;
       }
     }
+    // Transposed copy: row j holds column j of M_P, i.e. Linv[j..w-1][j]
+    // followed by W[0..r-1][j]; entry i multiplies front position j + i.
+    double* Tp = &S.Mt[(size_t)x.m_off];
+    for (int j = 0; j < w; ++j) {
+      double* Tj = Tp + (size_t)j * m - (size_t)j * (j - 1) / 2;
+      for (int k = j; k < w; ++k) Tj[k - j] = Mp[(size_t)k * (k + 1) / 2 + j];
+      for (int q = 0; q < r; ++q) Tj[w - j + q] = Wp[(size_t)q * w + j];
+    }
   }
 
   // Elements with permuted vertex labels, sorted by their smallest label for
@@ -468,27 +609,45 @@ bool build_setup(const SetupInput& in, Setup& S) {
     S.inc_ptr = cnt;
     S.inc.assign(cnt[n], 0);
     std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
-    for (int k = 0; k < 3 * t; ++k) S.inc[cur[S.tri_idx[k]]++] = k;
-    for (int k = 0; k < 4 * h; ++k) S.inc[cur[S.hinge_idx[k]]++] = -k - 1;
+    S.tri_slot.assign(3 * (size_t)t, 0);
+    S.hinge_slot.assign(4 * (size_t)h, 0);
+    for (int k = 0; k < 3 * t; ++k) { S.tri_slot[k] = cur[S.tri_idx[k]]; S.inc[cur[S.tri_idx[k]]++] = k; }
+    for (int k = 0; k < 4 * h; ++k) { S.hinge_slot[k] = cur[S.hinge_idx[k]]; S.inc[cur[S.hinge_idx[k]]++] = -k - 1; }
   }
+  S.upper_cols.clear();
+  S.band_front.assign(S.max_depth + 1, 0);
+  for (int s = 0; s < ns; ++s) {
+    const Supernode& x = S.sn[s];
+    S.band_front[x.depth] = std::max(S.band_front[x.depth], x.w + x.r);
+    if (x.depth < S.max_depth)
+      for (int j = 0; j < x.w; ++j) S.upper_cols.push_back(x.col0 + j);
+  }
+  std::sort(S.upper_cols.begin(), S.upper_cols.end());
+  if (S.m_total >= (int64_t)1 << 31) { S.status = VH_E_UNSUPPORTED; S.err = "solve blocks exceed 2^31 entries"; return false; }
 
-  // Level task lists. Forward: chunks of rows with at most ~4096 entries;
-  // backward: chunks of up to 64 columns.
+  // Task lists. A task is a run of rows of one stored form with about
+  // kChunk entries, so wide supernodes spread over many workgroups. g lanes
+  // cooperate on one row (g ~ row length / 4, 4..64).
+  const int kChunk = 4096;
   auto pow2ceil = [](int v) { int p = 1; while (p < v) p <<= 1; return p; };
+  auto lanes_for = [&](int len) { return std::min(64, std::max(4, pow2ceil(std::max(1, len)) / 4)); };
   S.fwd_tasks.clear(); S.bwd_tasks.clear();
   S.fwd_level_off.assign(S.max_depth + 2, 0);
   S.bwd_level_off.assign(S.max_depth + 2, 0);
-  for (int lvl = 0; lvl <= S.max_depth; ++lvl) {  // forward runs deepest level first
+  std::vector<int> nfc(ns, 0), nbc(ns, 0);
+  for (int lvl = 0; lvl <= S.max_depth; ++lvl) {  // forward runs the deepest level first
     const int depth = S.max_depth - lvl;
     S.fwd_level_off[lvl] = (int)S.fwd_tasks.size();
     for (int s = 0; s < ns; ++s) {
       const Supernode& x = S.sn[s];
       if (x.depth != depth) continue;  // w == 0 nodes still pass child updates up
-      const int g = std::min(64, std::max(4, pow2ceil(x.w) / 4));
+      const int g = lanes_for(x.w);
       const int m = x.w + x.r;
-      int rows_per = std::max(256 / g, 4096 / std::max(1, x.w));
-      rows_per = std::max(1, rows_per);
-      for (int b = 0; b < m; b += rows_per) S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g});
+      const int rows_per = std::max(256 / g, kChunk / std::max(1, x.w));
+      for (int b = 0; b < m; b += rows_per) {
+        S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g, 0});
+        nfc[s]++;
+      }
     }
   }
   S.fwd_level_off[S.max_depth + 1] = (int)S.fwd_tasks.size();
@@ -496,12 +655,29 @@ bool build_setup(const SetupInput& in, Setup& S) {
     S.bwd_level_off[lvl] = (int)S.bwd_tasks.size();
     for (int s = 0; s < ns; ++s) {
       const Supernode& x = S.sn[s];
-      if (x.depth != lvl || x.w == 0) continue;
-      const int c = std::min(64, pow2ceil(x.w));
-      for (int b = 0; b < x.w; b += c) S.bwd_tasks.push_back({s, b, std::min(x.w, b + c), c});
+      if (x.depth != lvl) continue;
+      const int m = x.w + x.r;
+      const int g = lanes_for(m);
+      const int rows_per = std::max(256 / g, kChunk / std::max(1, m));
+      for (int b = 0; b < x.w; b += rows_per) {
+        S.bwd_tasks.push_back({s, b, std::min(x.w, b + rows_per), g, 1});
+        nbc[s]++;
+      }
+      if (x.w == 0) {  // empty separator: one empty task keeps the children's wait chain intact
+        S.bwd_tasks.push_back({s, 0, 0, g, 1});
+        nbc[s]++;
+      }
     }
   }
   S.bwd_level_off[S.max_depth + 1] = (int)S.bwd_tasks.size();
+  S.ptasks = S.fwd_tasks;
+  S.ptasks.insert(S.ptasks.end(), S.bwd_tasks.begin(), S.bwd_tasks.end());
+  S.need_f.assign(ns, 0); S.need_b.assign(ns, 0); S.parent.assign(ns, -1);
+  for (int s = 0; s < ns; ++s) {
+    S.parent[s] = S.sn[s].parent;
+    for (int c = 0; c < S.sn[s].nchild; ++c) S.need_f[s] += nfc[S.children[S.sn[s].child_off + c]];
+    S.need_b[s] = S.sn[s].parent >= 0 ? nbc[S.sn[s].parent] : nfc[s];
+  }
   S.status = VH_OK;
   return true;
 }

@@ -29,6 +29,16 @@ struct SetupInput {
   double density = 0.15;            // kg/m^2
   double dt_matrix = 1.0 / 60.0;
   int leaf_size = 16;               // nested-dissection leaf size
+  // Level banding of the dissection tree. Each band of consecutive levels
+  // becomes one level of supernodes (a node plus its descendants inside the
+  // band form one dense block), trading block entries for fewer dependent
+  // launches. Bands are chosen by dynamic programming on the cost
+  //   2 * (band_launch_us + entries * value_bytes / band_gbps)
+  // per band (forward and backward sweep).
+  int value_bytes = 8;              // 8 = FP64 blocks, 4 = FP32 blocks
+  double band_launch_us = 6.0;
+  double band_gbps = 4000.0;
+  int max_band_levels = 6;          // 1 = one band per dissection level
 };
 
 // One supernode = one node of the nested-dissection tree: a contiguous run of
@@ -46,11 +56,14 @@ struct Supernode {
   int parent = -1;
   int child_off = 0, nchild = 0;
   int depth = 0;
+  int fm_off = 0;      // offset of this node's front gather map (w + r + 1 entries) in fmap_ptr
 };
 
-// Forward task: rows [b, e) of supernode sn. Backward task: columns [b, e).
+// A solve task: rows [b, e) of one of supernode sn's two stored forms with g
+// lanes per row. kind 0 = forward (rows of M_P), 1 = backward (rows of M_P^T,
+// i.e. columns of M_P).
 struct Task {
-  int sn, b, e, g;  // g: lanes per row (forward) / row slices (backward)
+  int sn, b, e, g, kind;
 };
 
 struct Setup {
@@ -73,13 +86,37 @@ struct Setup {
   // supernodal structure
   std::vector<Supernode> sn;         // postorder: children before parents
   std::vector<int32_t> rowidx, children, emap;
-  std::vector<double> M;
+  // Forward push slots. Every update-vector row (child c, row q) owns one
+  // slot uslot[c.u_off + q] in the parent's slot array; the slots feeding
+  // front position j of supernode P are [fslot_ptr[P.fm_off + j],
+  // fslot_ptr[P.fm_off + j + 1]), in child order.
+  std::vector<int32_t> fslot_ptr, uslot;
+  // Backward push slots: supernode c's off-diagonal rows read -d[I_c[q]]
+  // from slot c.u_off + q; column k's value is pushed to
+  // dpush_idx[dpush_ptr[k] .. dpush_ptr[k + 1]).
+  std::vector<int32_t> dpush_ptr, dpush_idx;
+  // RHS slots: element corner (e, k) adds into slot tri_slot[3e + k]
+  // (hinges: hinge_slot[4e + k]); vertex i sums slots [inc_ptr[i], inc_ptr[i+1]).
+  std::vector<int32_t> tri_slot, hinge_slot;
+  std::vector<int32_t> upper_cols;   // columns outside the deepest band
+  std::vector<int32_t> band_front;   // max (w + r) per band
+  std::vector<double> M;             // M_P row-major trapezoids (forward sweep)
+  std::vector<double> Mt;            // M_P^T: column j of M_P stored contiguously (backward sweep)
   int64_t m_total = 0;
   int u_total = 0;
   int max_front = 0;                 // max (w + r)
-  int max_depth = 0;
+  int max_depth = 0;                 // bands - 1
+  int nd_levels = 0;                 // levels of the fine dissection tree
+  std::vector<int> band_start;       // first dissection level of each band
   std::vector<Task> fwd_tasks, bwd_tasks;
   std::vector<int> fwd_level_off, bwd_level_off;  // tasks of level l: [off[l], off[l+1])
+  // Dependency-driven schedule of both sweeps in one launch: all forward
+  // tasks (deepest level first) then all backward tasks (root first), so
+  // every task depends only on tasks listed before it.
+  std::vector<Task> ptasks;
+  std::vector<int32_t> need_f;       // forward chunks of P's children (wait count of P's forward)
+  std::vector<int32_t> need_b;       // backward chunks of P's parent, or forward chunks of the root
+  std::vector<int32_t> parent;
   int64_t nnz_factor = 0;            // sum of w(w+1)/2 + r*w
   std::string err;
   int status = 0;                    // 0 ok, else a vh_status code

@@ -73,6 +73,9 @@ class SolverConfig:
     ir_sweeps: int = 1
     ir_final_sweeps: int = 3
     leaf_size: int = 16
+    max_band_levels: int = 0        # 0 = default; 1 = one solve level per dissection level
+    band_launch_us: float = 0.0     # 0 = default
+    band_gbps: float = 0.0          # 0 = default
 
 
 @dataclass
@@ -232,6 +235,8 @@ class HipPdSolver:
         p.has_ground, p.ground_height = 0, 0.0
         p.precision = nat.PREC_F64 if config.precision == "f64" else nat.PREC_F32_IR
         p.ir_sweeps, p.ir_final_sweeps, p.leaf_size = int(config.ir_sweeps), int(config.ir_final_sweeps), int(config.leaf_size)
+        p.max_band_levels = int(config.max_band_levels)
+        p.band_launch_us, p.band_gbps = float(config.band_launch_us), float(config.band_gbps)
         px, py, pz = (np.ascontiguousarray(a, dtype=np.float32) for a in (mesh.pos_x, mesh.pos_y, mesh.pos_z))
         tri = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
         hg = np.ascontiguousarray(topology.interior_edges[:, :4], dtype=np.uint32)
@@ -318,6 +323,17 @@ class HipPdSolver:
         _check(self._L.vh_pd_solve(self._ctx, nat.ptr(bb, C.c_double), nat.ptr(x, C.c_double), int(sweeps)),
                self._ctx, "solve")
         return x.reshape(3, self._n)
+
+    def profile(self, dt: float, iterations: int = 10) -> dict:
+        """Per-launch kernel times (HIP events on the solver's stream) and
+        algorithmic bytes for `iterations` PD iterations."""
+        p = nat.Profile()
+        _check(self._L.vh_pd_profile(self._ctx, float(dt), int(iterations), C.byref(p)), self._ctx, "profile")
+        nb = p.bands
+        return dict(bands=nb, iterations=p.iterations, local_us=p.local_us, local_bytes=p.local_bytes,
+                    fwd_us=list(p.fwd_us[:nb]), fwd_bytes=list(p.fwd_bytes[:nb]), bwd_us=list(p.bwd_us[:nb]),
+                    bwd_bytes=list(p.bwd_bytes[:nb]), ir_us=p.ir_us, ir_bytes=p.ir_bytes,
+                    iteration_us=p.iteration_us)
 
     def info(self) -> dict:
         i = nat.PdInfo()
