@@ -126,7 +126,7 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--ir-sweeps", type=int, default=1)
     ap.add_argument("--ir-final-sweeps", type=int, default=2)
-    ap.add_argument("--cpu-sample-steps", type=int, default=8)
+    ap.add_argument("--cpu-sample-steps", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))

@@ -34,7 +34,7 @@ PHYS = dict(model=1, k_membrane=1e3, k_bending=1e-4, density=0.15, dt_matrix=DT,
 
 
 def gpu_solver(px, py, pz, idx, hg, pin, precision="f64", ir_sweeps=1, ir_final_sweeps=3, leaf=16, max_band_levels=0,
-               **over):
+               task_entries=0, **over):
     """HIP solver with the same physics as the oracle's params(**PHYS)."""
     import vistio_amd as va
     ph = dict(PHYS)
@@ -44,7 +44,7 @@ def gpu_solver(px, py, pz, idx, hg, pin, precision="f64", ir_sweeps=1, ir_final_
     cfg = va.SolverConfig(max_iterations=ph["max_iterations"], tolerance=ph["tolerance"], gravity=ph["gravity"],
                           damping=ph["damping"], rayleigh_mass_damping=ph["rayleigh"], precision=precision,
                           ir_sweeps=ir_sweeps, ir_final_sweeps=ir_final_sweeps, leaf_size=leaf,
-                          max_band_levels=max_band_levels)
+                          max_band_levels=max_band_levels, task_entries=task_entries)
     s = va.HipPdSolver()
     s.init_raw(m, topo, cfg, pin, "co_rotational" if ph["model"] == 1 else "arap", ph["k_membrane"],
                ph["k_bending"], ph["density"])

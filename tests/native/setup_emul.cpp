@@ -5,6 +5,7 @@
 // schedule can be checked against an independent solve without a GPU.
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "vh_setup.hpp"
@@ -27,7 +28,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
   // backward tasks push -d into the descendants' v slots.
   const int nb = S.max_depth + 1;
   const int nfwd = (int)S.fwd_tasks.size();
-  std::vector<double> F(3 * (size_t)S.u_total + 3, 0.0), V(3 * (size_t)S.u_total + 3, 0.0);
+  std::vector<double> F(3 * (size_t)S.fslot_total + 3, 0.0), V(3 * (size_t)S.u_total + 3, 0.0);
   for (int l = 0; l < nb; ++l)
     for (int t = S.fwd_level_off[l]; t < S.fwd_level_off[l + 1]; ++t) {
       const vh::Task& T = S.ptasks[t];
@@ -36,8 +37,8 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       const int w = P.w, m = P.w + P.r;
       std::vector<double> front(3 * (size_t)m, 0.0);
       for (int j = 0; j < m; ++j)
-        for (int q = S.fslot_ptr[P.fm_off + j]; q < S.fslot_ptr[P.fm_off + j + 1]; ++q)
-          for (int k = 0; k < 3; ++k) front[3 * j + k] += F[3 * (size_t)q + k];
+        for (int q = 0; q < P.fslot_k; ++q)
+          for (int k = 0; k < 3; ++k) front[3 * j + k] += F[3 * ((size_t)P.fm_off + (size_t)j * P.fslot_k + q) + k];
       for (int j = 0; j < w; ++j)
         for (int k = 0; k < 3; ++k) front[3 * j + k] = r[3 * (size_t)(P.col0 + j) + k] - front[3 * j + k];
       const double* Mp = &S.M[P.m_off];
@@ -86,3 +87,4 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
   }
   return 0;
 }
+

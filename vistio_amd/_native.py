@@ -39,6 +39,7 @@ class PdParams(C.Structure):
         ("has_ground", C.c_int32), ("ground_height", C.c_double), ("precision", C.c_int32),
         ("ir_sweeps", C.c_int32), ("ir_final_sweeps", C.c_int32), ("leaf_size", C.c_int32),
         ("max_band_levels", C.c_int32), ("band_launch_us", C.c_double), ("band_gbps", C.c_double),
+        ("task_entries", C.c_int32),
     ]
 
 

@@ -44,7 +44,7 @@ struct vh_ctx {
   int* inc_ptr = nullptr;
   // solve
   vhk::TaskDev* tasks = nullptr;  // forward tasks then backward tasks
-  int *fslot_ptr = nullptr, *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
+  int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
   void *M = nullptr, *Mt = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
   double* Ax = nullptr;
@@ -152,13 +152,13 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool want
   const int nfwd = (int)S.fwd_tasks.size();
   const int nupper = (int)S.upper_cols.size();
   vhk::BandArgs<T> a{};
-  a.fslot_ptr = c->fslot_ptr; a.uslot = c->uslot; a.dpush_ptr = c->dpush_ptr; a.dpush_idx = c->dpush_idx;
+  a.uslot = c->uslot; a.dpush_ptr = c->dpush_ptr; a.dpush_idx = c->dpush_idx;
   a.src = rhs_src<T>(c, dt); a.r = c->r; a.upper_cols = c->upper_cols; a.nupper = nupper;
   for (int l = 0; l < nb; ++l) {
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
     a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = first ? 1 : 0;
-    const int grid = (e - b) + (first ? nblk(nupper) : 0);
+    const int grid = (e - b) + (first ? nblk(4 * nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * c->tsz;
     if (grid > 0)
       hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M, (T*)c->y,
@@ -366,6 +366,7 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   if (P.max_band_levels > 0) in.max_band_levels = P.max_band_levels;
   if (P.band_launch_us > 0) in.band_launch_us = P.band_launch_us;
   if (P.band_gbps > 0) in.band_gbps = P.band_gbps;
+  if (P.task_entries > 0) in.task_entries = P.task_entries;
   ctx->S = vh::Setup();
   if (!vh::build_setup(in, ctx->S)) return fail(ctx, (vh_status)ctx->S.status, ctx->S.err);
   vh::Setup& S = ctx->S;
@@ -401,10 +402,9 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
     const vh::Supernode& q = S.sn[T.sn];
     td[k].q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
     td[k].q1 = make_int4(T.b, T.e, T.g, q.u_off);
-    td[k].q2 = make_int4(q.fm_off, 0, 0, 0);
+    td[k].q2 = make_int4(q.fm_off, q.fslot_k, 0, 0);
   }
   UP(upload(ctx, &ctx->tasks, td));
-  UP(upload(ctx, &ctx->fslot_ptr, S.fslot_ptr));
   UP(upload(ctx, &ctx->uslot, S.uslot));
   UP(upload(ctx, &ctx->dpush_ptr, S.dpush_ptr));
   UP(upload(ctx, &ctx->dpush_idx, S.dpush_idx));
@@ -413,7 +413,7 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   UP(upload_t(ctx, &ctx->Mt, S.Mt));
   UP(dalloc_bytes(ctx, &ctx->y, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->d, n3 * ctx->tsz));
-  UP(dalloc_bytes(ctx, &ctx->F, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->F, 3 * (size_t)std::max<int64_t>(1, S.fslot_total) * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->V, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
   if (ctx->prec == VH_PREC_F32_IR) {
     UP(upload(ctx, &ctx->Ap, S.Ap));
@@ -429,7 +429,7 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   UP(dalloc(ctx, &ctx->st, 1));
 #undef UP
   VH_HIP(hipMemset(ctx->st, 0, sizeof(vhk::StepDev)));
-  VH_HIP(hipMemset(ctx->F, 0, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
+  VH_HIP(hipMemset(ctx->F, 0, 3 * (size_t)std::max<int64_t>(1, S.fslot_total) * ctx->tsz));
   VH_HIP(hipMemset(ctx->V, 0, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
   size_t lds = 0;
   for (int f : S.band_front) lds = std::max(lds, 3 * (size_t)f * ctx->tsz);
@@ -658,12 +658,12 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
     auto run = [&](auto zero) -> vh_status {
       using T = decltype(zero);
       vhk::BandArgs<T> a{};
-      a.fslot_ptr = ctx->fslot_ptr; a.uslot = ctx->uslot; a.dpush_ptr = ctx->dpush_ptr; a.dpush_idx = ctx->dpush_idx;
+      a.uslot = ctx->uslot; a.dpush_ptr = ctx->dpush_ptr; a.dpush_idx = ctx->dpush_idx;
       a.src = rhs_src<T>(ctx, dt); a.r = ctx->r; a.upper_cols = ctx->upper_cols; a.nupper = nupper;
       for (int l = 0; l < nb; ++l) {
         const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
         a.tasks = ctx->tasks + b; a.ntasks = e - b; a.rhs_from_slots = l == 0 ? 1 : 0;
-        const int grid = (e - b) + (l == 0 ? nblk(nupper) : 0);
+        const int grid = (e - b) + (l == 0 ? nblk(4 * nupper) : 0);
         const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * ctx->tsz;
         if (grid > 0)
           hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, ctx->stream, a, (const T*)ctx->M,

@@ -332,6 +332,14 @@ struct RhsSrc {
 };
 
 template <class T>
+__device__ __forceinline__ double3 inertia(const RhsSrc<T>& s, int i) {
+  const double m = s.mass[i];
+  return make_double3(m * (s.inv_dt2 * s.pred[3 * i + 0] - s.inv_dtm2 * s.x[3 * i + 0]),
+                      m * (s.inv_dt2 * s.pred[3 * i + 1] - s.inv_dtm2 * s.x[3 * i + 1]),
+                      m * (s.inv_dt2 * s.pred[3 * i + 2] - s.inv_dtm2 * s.x[3 * i + 2]));
+}
+
+template <class T>
 __device__ __forceinline__ double3 rhs_from_slots(const RhsSrc<T>& s, int i) {
   T ax = 0, ay = 0, az = 0;
   const int pe = s.inc_ptr[i + 1];
@@ -341,10 +349,29 @@ __device__ __forceinline__ double3 rhs_from_slots(const RhsSrc<T>& s, int i) {
     const T* c = s.cslot + 3 * (size_t)p;
     ax += c[0]; ay += c[1]; az += c[2];
   }
-  const double m = s.mass[i];
-  return make_double3(m * (s.inv_dt2 * s.pred[3 * i + 0] - s.inv_dtm2 * s.x[3 * i + 0]) + (double)ax,
-                      m * (s.inv_dt2 * s.pred[3 * i + 1] - s.inv_dtm2 * s.x[3 * i + 1]) + (double)ay,
-                      m * (s.inv_dt2 * s.pred[3 * i + 2] - s.inv_dtm2 * s.x[3 * i + 2]) + (double)az);
+  const double3 in = inertia(s, i);
+  return make_double3(in.x + (double)ax, in.y + (double)ay, in.z + (double)az);
+}
+
+template <class T>
+__device__ __forceinline__ T group_sum(T v, int g) {
+  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, g);
+  return v;
+}
+
+// The slot sum of vertex i split over the L lanes of a group (all lanes of
+// the group call it with the same i; lane 0 gets the total).
+template <class T>
+__device__ __forceinline__ double3 rhs_from_slots_group(const RhsSrc<T>& s, int i, int lane, int L) {
+  T ax = 0, ay = 0, az = 0;
+  const int pe = s.inc_ptr[i + 1];
+  for (int p = s.inc_ptr[i] + lane; p < pe; p += L) {
+    const T* c = s.cslot + 3 * (size_t)p;
+    ax += c[0]; ay += c[1]; az += c[2];
+  }
+  ax = group_sum(ax, L); ay = group_sum(ay, L); az = group_sum(az, L);
+  const double3 in = inertia(s, i);
+  return make_double3(in.x + (double)ax, in.y + (double)ay, in.z + (double)az);
 }
 
 // Stand-alone RHS gather (residual inspection entry point).
@@ -354,12 +381,6 @@ __global__ void __launch_bounds__(256) k_gather(int n, RhsSrc<T> s, double* __re
   if (i >= n) return;
   const double3 v = rhs_from_slots(s, i);
   r[3 * i + 0] = v.x; r[3 * i + 1] = v.y; r[3 * i + 2] = v.z;
-}
-
-template <class T>
-__device__ __forceinline__ T group_sum(T v, int g) {
-  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, g);
-  return v;
 }
 
 // Dot product of one stored row (len entries, contiguous) with an LDS vector
@@ -395,7 +416,7 @@ __device__ __forceinline__ void row_dot(const T* __restrict__ row, int len, cons
 // Solve task descriptor: everything a task needs in one 48-byte record.
 //   q0 = (col0, w, r, m_off)   supernode columns, off-diagonal rows, block offset
 //   q1 = (b, e, g, u_off)      row range, lanes per row, update-row / v-slot base
-//   q2 = (fm_off, 0, 0, 0)     front slot pointer base
+//   q2 = (fm_off, K, 0, 0)     first front slot, slots per front position
 struct TaskDev {
   int4 q0, q1, q2;
 };
@@ -440,7 +461,6 @@ template <class T>
 struct BandArgs {
   const TaskDev* tasks;
   int ntasks;
-  const int* fslot_ptr;
   const int* uslot;
   const int* dpush_ptr;
   const int* dpush_idx;
@@ -463,61 +483,152 @@ struct BandArgs {
 //   rows w + q : slot(U_P[q])      = sum_k M[w+q][k] t[k] + front[w + q]
 // so each update row lands in its slot of the parent's front. Only the front
 // entries the task's rows read are assembled.
+// Rows of a task are prefetched kPrefRows at a time per lane group, up to
+// kPrefLoads entries per lane each (rows longer than kPrefLoads * g finish
+// with plain loads).
+constexpr int kPrefRows = 2;
+constexpr int kPrefLoads = 8;
+
+template <class T>
+__device__ __forceinline__ void prefetch_row(T (&mv)[kPrefLoads], const T* __restrict__ row, int len, int lane,
+                                             int g) {
+#pragma unroll
+  for (int i = 0; i < kPrefLoads; ++i) {
+    const int k = lane + i * g;
+    mv[i] = k < len ? __builtin_nontemporal_load(row + k) : T(0);
+  }
+}
+
+// row_dot with the first kPrefLoads * g entries already in registers.
+template <class T>
+__device__ __forceinline__ void row_dot_pref(const T (&mv)[kPrefLoads], const T* __restrict__ row, int len,
+                                             const T* __restrict__ vec, int lane, int g, T& sx, T& sy, T& sz) {
+  T ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
+#pragma unroll
+  for (int i = 0; i < kPrefLoads; i += 2) {
+    const int k0 = lane + i * g, k1 = k0 + g;
+    if (k0 < len) { const T* v0 = vec + 3 * k0; ax += mv[i] * v0[0]; ay += mv[i] * v0[1]; az += mv[i] * v0[2]; }
+    if (k1 < len) { const T* v1 = vec + 3 * k1; bx += mv[i + 1] * v1[0]; by += mv[i + 1] * v1[1]; bz += mv[i + 1] * v1[2]; }
+  }
+  for (int k = lane + kPrefLoads * g; k < len; k += g) {
+    const T m0 = __builtin_nontemporal_load(row + k);
+    const T* v0 = vec + 3 * k;
+    ax += m0 * v0[0]; ay += m0 * v0[1]; az += m0 * v0[2];
+  }
+  sx = group_sum(ax + bx, g);
+  sy = group_sum(ay + by, g);
+  sz = group_sum(az + bz, g);
+}
+
+__device__ __forceinline__ long long fwd_row_off(int row, int w, long long tri) {
+  return row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w;
+}
+
+template <class T>
+__device__ __forceinline__ void fwd_store(const BandArgs<T>& a, int row, int w, int col0, int uoff, T sx, T sy, T sz,
+                                          const T* front, T* __restrict__ y, T* __restrict__ F) {
+  if (row < w) {
+    T* o = y + 3 * (size_t)(col0 + row);
+    o[0] = sx; o[1] = sy; o[2] = sz;
+  } else {
+    T* o = F + 3 * (size_t)a.uslot[uoff + row - w];
+    o[0] = sx + front[3 * row + 0];
+    o[1] = sy + front[3 * row + 1];
+    o[2] = sz + front[3 * row + 2];
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void front_sum(const T* __restrict__ f, int K, T& ax, T& ay, T& az) {
+  ax = 0; ay = 0; az = 0;
+  if (K > 0) { ax += f[0]; ay += f[1]; az += f[2]; }
+  if (K > 1) { ax += f[3]; ay += f[4]; az += f[5]; }
+  if (K > 2) { ax += f[6]; ay += f[7]; az += f[8]; }
+  for (int k = 3; k < K; ++k) { ax += f[3 * k]; ay += f[3 * k + 1]; az += f[3 * k + 2]; }
+}
+
 template <class T>
 __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
                          T* __restrict__ F, T* front) {
   const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
-  const int* fp = a.fslot_ptr + td.q2.x;
+  const int K = td.q2.y;
+  const T* Fp = F + 3 * (size_t)td.q2.x;
   const int tid = threadIdx.x;
   const int tlen = e <= w ? e : w;
-  const bool store_r = a.rhs_from_slots && e >= w;
-  for (int j = tid; j < tlen; j += blockDim.x) {
-    T ax = 0, ay = 0, az = 0;
-    for (int q = fp[j]; q < fp[j + 1]; ++q) {
-      const T* u = F + 3 * (size_t)q;
-      ax += u[0]; ay += u[1]; az += u[2];
-    }
-    double3 rb;
-    if (a.rhs_from_slots) {
-      rb = rhs_from_slots(a.src, col0 + j);
-      if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z; }
-    } else {
-      rb = ld3(a.r, col0 + j);
-    }
-    front[3 * j + 0] = (T)rb.x - ax;
-    front[3 * j + 1] = (T)rb.y - ay;
-    front[3 * j + 2] = (T)rb.z - az;
-  }
-  for (int j = (b > w ? b : w) + tid; j < e; j += blockDim.x) {
-    T ax = 0, ay = 0, az = 0;
-    for (int q = fp[j]; q < fp[j + 1]; ++q) {
-      const T* u = F + 3 * (size_t)q;
-      ax += u[0]; ay += u[1]; az += u[2];
-    }
-    front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
-  }
-  __syncthreads();
+  // Block rows first: their addresses depend only on the task, so these
+  // loads are in flight while the front is assembled.
   const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
   const T* Mp = M + moff;
   const long long tri = (long long)w * (w + 1) / 2;
-  for (int row = b + grp; row < e; row += ngrp) {
-    const int len = row < w ? row + 1 : w;
-    const T* Mr = Mp + (row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w);
-    T sx, sy, sz;
-    row_dot(Mr, len, front, lane, g, sx, sy, sz);
-    if (lane == 0) {
-      if (row < w) {
-        T* o = y + 3 * (size_t)(col0 + row);
-        o[0] = sx; o[1] = sy; o[2] = sz;
-      } else {
-        T* o = F + 3 * (size_t)a.uslot[uoff + row - w];
-        o[0] = sx + front[3 * row + 0];
-        o[1] = sy + front[3 * row + 1];
-        o[2] = sz + front[3 * row + 2];
+  T mv[kPrefRows][kPrefLoads];
+#pragma unroll
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int row = b + grp + p * ngrp;
+    const int len = row < e ? (row < w ? row + 1 : w) : 0;
+    prefetch_row(mv[p], Mp + fwd_row_off(row < e ? row : b, w, tri), len, lane, g);
+  }
+  if (a.rhs_from_slots) {
+    // lane groups of L threads per column sum the column's RHS slots
+    int L = 1;
+    while (L < 8 && (int)blockDim.x / (2 * L) >= tlen) L *= 2;
+    const int lane = tid & (L - 1), grp = tid / L, ngrp = blockDim.x / L;
+    const bool store_r = e >= w;
+    for (int j = grp; j < tlen; j += ngrp) {
+      const double3 rb = rhs_from_slots_group(a.src, col0 + j, lane, L);
+      if (lane == 0) {
+        if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z; }
+        T ax, ay, az;
+        front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+        front[3 * j + 0] = (T)rb.x - ax;
+        front[3 * j + 1] = (T)rb.y - ay;
+        front[3 * j + 2] = (T)rb.z - az;
       }
     }
+  } else {
+#pragma unroll 4
+    for (int j = tid; j < tlen; j += blockDim.x) {
+      T ax, ay, az;
+      front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+      const double* rb = a.r + 3 * (size_t)(col0 + j);
+      front[3 * j + 0] = (T)rb[0] - ax;
+      front[3 * j + 1] = (T)rb[1] - ay;
+      front[3 * j + 2] = (T)rb[2] - az;
+    }
   }
+#pragma unroll 4
+  for (int j = (b > w ? b : w) + tid; j < e; j += blockDim.x) {
+    T ax, ay, az;
+    front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+    front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
+  }
+  __syncthreads();
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int row = b + grp + p * ngrp;
+    if (row >= e) break;  // uniform within the group
+    const int len = row < w ? row + 1 : w;
+    T sx, sy, sz;
+    row_dot_pref(mv[p], Mp + fwd_row_off(row, w, tri), len, front, lane, g, sx, sy, sz);
+    if (lane == 0) fwd_store(a, row, w, col0, uoff, sx, sy, sz, front, y, F);
+  }
+  for (int row = b + grp + kPrefRows * ngrp; row < e; row += ngrp) {
+    const int len = row < w ? row + 1 : w;
+    T sx, sy, sz;
+    row_dot(Mp + fwd_row_off(row, w, tri), len, front, lane, g, sx, sy, sz);
+    if (lane == 0) fwd_store(a, row, w, col0, uoff, sx, sy, sz, front, y, F);
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void bwd_store(const BandArgs<T>& a, const CommitArgs& c, int col, T sx, T sy, T sz,
+                                          T* __restrict__ V, T* __restrict__ d, double& df, double& nm) {
+  T* o = d + 3 * (size_t)col;
+  o[0] = sx; o[1] = sy; o[2] = sz;
+  for (int q = a.dpush_ptr[col]; q < a.dpush_ptr[col + 1]; ++q) {
+    T* p = V + 3 * (size_t)a.dpush_idx[q];
+    p[0] = -sx; p[1] = -sy; p[2] = -sz;
+  }
+  if (c.on) commit_vertex(c, col, (double)sx, (double)sy, (double)sz, df, nm);
 }
 
 // Backward task (L^T d = y): columns [b, e) of P, read as rows of the
@@ -535,27 +646,32 @@ __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArg
   const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
   const int m = w + r;
   const int tid = threadIdx.x;
+  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  const T* Tp = Mt + moff;
+  T mv[kPrefRows][kPrefLoads];
+#pragma unroll
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int j = b + grp + p * ngrp;
+    const int jj = j < e ? j : b;
+    prefetch_row(mv[p], Tp + (long long)jj * m - (long long)jj * (jj - 1) / 2, j < e ? m - j : 0, lane, g);
+  }
+#pragma unroll 4
   for (int k = b + tid; k < m; k += blockDim.x) {
     const T* s = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
     v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
   }
   __syncthreads();
-  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
-  const T* Tp = Mt + moff;
-  for (int j = b + grp; j < e; j += ngrp) {
-    const T* row = Tp + (long long)j * m - (long long)j * (j - 1) / 2;
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int j = b + grp + p * ngrp;
+    if (j >= e) break;  // uniform within the group
     T sx, sy, sz;
-    row_dot(row, m - j, v + 3 * j, lane, g, sx, sy, sz);
-    if (lane == 0) {
-      const int col = col0 + j;
-      T* o = d + 3 * (size_t)col;
-      o[0] = sx; o[1] = sy; o[2] = sz;
-      for (int q = a.dpush_ptr[col]; q < a.dpush_ptr[col + 1]; ++q) {
-        T* p = V + 3 * (size_t)a.dpush_idx[q];
-        p[0] = -sx; p[1] = -sy; p[2] = -sz;
-      }
-      if (c.on) commit_vertex(c, col, (double)sx, (double)sy, (double)sz, df, nm);
-    }
+    row_dot_pref(mv[p], Tp + (long long)j * m - (long long)j * (j - 1) / 2, m - j, v + 3 * j, lane, g, sx, sy, sz);
+    if (lane == 0) bwd_store(a, c, col0 + j, sx, sy, sz, V, d, df, nm);
+  }
+  for (int j = b + grp + kPrefRows * ngrp; j < e; j += ngrp) {
+    T sx, sy, sz;
+    row_dot(Tp + (long long)j * m - (long long)j * (j - 1) / 2, m - j, v + 3 * j, lane, g, sx, sy, sz);
+    if (lane == 0) bwd_store(a, c, col0 + j, sx, sy, sz, V, d, df, nm);
   }
 }
 
@@ -571,12 +687,16 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __rest
     fwd_task<T>(a.tasks[blockIdx.x], a, M, y, F, reinterpret_cast<T*>(smem_raw));
     return;
   }
-  const int k = ((int)blockIdx.x - a.ntasks) * blockDim.x + threadIdx.x;
-  if (k >= a.nupper) return;
+  // 4 lanes per column: blockDim/4 columns per extra block
+  const int k = ((int)blockIdx.x - a.ntasks) * (blockDim.x / 4) + (int)threadIdx.x / 4;
+  const int lane = threadIdx.x & 3;
+  if (k >= a.nupper) return;  // uniform within each 4-lane group
   const int col = a.upper_cols[k];
-  const double3 rb = rhs_from_slots(a.src, col);
-  double* o = a.r + 3 * (size_t)col;
-  o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
+  const double3 rb = rhs_from_slots_group(a.src, col, lane, 4);
+  if (lane == 0) {
+    double* o = a.r + 3 * (size_t)col;
+    o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
+  }
 }
 
 // One band of the backward sweep per launch. In the last band of an FP64

@@ -447,10 +447,9 @@ bool build_setup(const SetupInput& in, Setup& S) {
   // Forward push slots (inverse of the extend-add maps) and backward push
   // lists.
   {
-    S.fslot_ptr.clear();
     S.uslot.assign((size_t)S.u_total, 0);
     std::vector<std::vector<int32_t>> lists;
-    int32_t slot = 0;
+    int64_t slot = 0;
     for (int s = 0; s < ns; ++s) {
       Supernode& x = S.sn[s];
       const int m = x.w + x.r;
@@ -459,13 +458,16 @@ bool build_setup(const SetupInput& in, Setup& S) {
         const Supernode& y = S.sn[S.children[x.child_off + c]];
         for (int q = 0; q < y.r; ++q) lists[S.emap[y.u_off + q]].push_back(y.u_off + q);
       }
-      x.fm_off = (int)S.fslot_ptr.size();
-      for (int j = 0; j < m; ++j) {
-        S.fslot_ptr.push_back(slot);
-        for (int32_t u : lists[j]) S.uslot[u] = slot++;
-      }
-      S.fslot_ptr.push_back(slot);
+      int k = 0;
+      for (int j = 0; j < m; ++j) k = std::max(k, (int)lists[j].size());
+      x.fm_off = (int)slot;
+      x.fslot_k = k;
+      for (int j = 0; j < m; ++j)
+        for (size_t q = 0; q < lists[j].size(); ++q) S.uslot[lists[j][q]] = (int32_t)(slot + (int64_t)j * k + q);
+      slot += (int64_t)m * k;
+      if (slot >= ((int64_t)1 << 31)) { S.status = VH_E_UNSUPPORTED; S.err = "front slots exceed 2^31"; return false; }
     }
+    S.fslot_total = slot;
     std::vector<int32_t> cnt(n + 1, 0);
     for (int s = 0; s < ns; ++s)
       for (int q = 0; q < S.sn[s].r; ++q) cnt[S.rowidx[S.sn[s].rows_off + q] + 1]++;
@@ -625,12 +627,19 @@ bool build_setup(const SetupInput& in, Setup& S) {
   std::sort(S.upper_cols.begin(), S.upper_cols.end());
   if (S.m_total >= (int64_t)1 << 31) { S.status = VH_E_UNSUPPORTED; S.err = "solve blocks exceed 2^31 entries"; return false; }
 
-  // Task lists. A task is a run of rows of one stored form with about
-  // kChunk entries, so wide supernodes spread over many workgroups. g lanes
-  // cooperate on one row (g ~ row length / 4, 4..64).
-  const int kChunk = 4096;
+  // Task lists. A task is a run of rows of one stored form; g lanes share a
+  // row (g ~ row length / 8, 4..64, so a row is usually 8 loads per lane)
+  // and each lane group takes two rows, whose loads are issued together
+  // (kernels' kPrefRows x kPrefLoads). Rows longer than 8g take one row per
+  // group. task_entries caps short-row tasks.
+  const int kChunk = std::max(256, in.task_entries);
   auto pow2ceil = [](int v) { int p = 1; while (p < v) p <<= 1; return p; };
-  auto lanes_for = [&](int len) { return std::min(64, std::max(4, pow2ceil(std::max(1, len)) / 4)); };
+  auto lanes_for = [&](int len) { return std::min(64, std::max(4, pow2ceil((std::max(1, len) + 7) / 8))); };
+  auto rows_for = [&](int len, int g) {
+    const int ngrp = 256 / g;
+    if (len > 8 * g) return ngrp;
+    return std::max(ngrp, std::min(2 * ngrp, kChunk / std::max(1, len)));
+  };
   S.fwd_tasks.clear(); S.bwd_tasks.clear();
   S.fwd_level_off.assign(S.max_depth + 2, 0);
   S.bwd_level_off.assign(S.max_depth + 2, 0);
@@ -643,7 +652,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
       if (x.depth != depth) continue;  // w == 0 nodes still pass child updates up
       const int g = lanes_for(x.w);
       const int m = x.w + x.r;
-      const int rows_per = std::max(256 / g, kChunk / std::max(1, x.w));
+      const int rows_per = rows_for(x.w, g);
       for (int b = 0; b < m; b += rows_per) {
         S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g, 0});
         nfc[s]++;
@@ -658,7 +667,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
       if (x.depth != lvl) continue;
       const int m = x.w + x.r;
       const int g = lanes_for(m);
-      const int rows_per = std::max(256 / g, kChunk / std::max(1, m));
+      const int rows_per = rows_for(m, g);
       for (int b = 0; b < x.w; b += rows_per) {
         S.bwd_tasks.push_back({s, b, std::min(x.w, b + rows_per), g, 1});
         nbc[s]++;

@@ -38,7 +38,8 @@ struct SetupInput {
   int value_bytes = 8;              // 8 = FP64 blocks, 4 = FP32 blocks
   double band_launch_us = 6.0;
   double band_gbps = 4000.0;
-  int max_band_levels = 6;          // 1 = one band per dissection level
+  int max_band_levels = 3;          // 1 = one band per dissection level
+  int task_entries = 1024;          // block entries per solve task (rows per task follow)
 };
 
 // One supernode = one node of the nested-dissection tree: a contiguous run of
@@ -56,7 +57,8 @@ struct Supernode {
   int parent = -1;
   int child_off = 0, nchild = 0;
   int depth = 0;
-  int fm_off = 0;      // offset of this node's front gather map (w + r + 1 entries) in fmap_ptr
+  int fm_off = 0;      // first forward slot of this node's front (fslot_k slots per position)
+  int fslot_k = 0;     // slots per front position: most child update rows feeding one position
 };
 
 // A solve task: rows [b, e) of one of supernode sn's two stored forms with g
@@ -87,10 +89,11 @@ struct Setup {
   std::vector<Supernode> sn;         // postorder: children before parents
   std::vector<int32_t> rowidx, children, emap;
   // Forward push slots. Every update-vector row (child c, row q) owns one
-  // slot uslot[c.u_off + q] in the parent's slot array; the slots feeding
-  // front position j of supernode P are [fslot_ptr[P.fm_off + j],
-  // fslot_ptr[P.fm_off + j + 1]), in child order.
-  std::vector<int32_t> fslot_ptr, uslot;
+  // slot uslot[c.u_off + q] of the parent's front: position j of supernode P
+  // owns slots P.fm_off + j*P.fslot_k + k, k < P.fslot_k, filled in child
+  // order; unused slots stay zero. fslot_total = number of slots.
+  std::vector<int32_t> uslot;
+  int64_t fslot_total = 0;
   // Backward push slots: supernode c's off-diagonal rows read -d[I_c[q]]
   // from slot c.u_off + q; column k's value is pushed to
   // dpush_idx[dpush_ptr[k] .. dpush_ptr[k + 1]).

@@ -76,6 +76,7 @@ class SolverConfig:
     max_band_levels: int = 0        # 0 = default; 1 = one solve level per dissection level
     band_launch_us: float = 0.0     # 0 = default
     band_gbps: float = 0.0          # 0 = default
+    task_entries: int = 0           # 0 = default (1024 block entries per solve task)
 
 
 @dataclass
@@ -237,6 +238,7 @@ class HipPdSolver:
         p.ir_sweeps, p.ir_final_sweeps, p.leaf_size = int(config.ir_sweeps), int(config.ir_final_sweeps), int(config.leaf_size)
         p.max_band_levels = int(config.max_band_levels)
         p.band_launch_us, p.band_gbps = float(config.band_launch_us), float(config.band_gbps)
+        p.task_entries = int(config.task_entries)
         px, py, pz = (np.ascontiguousarray(a, dtype=np.float32) for a in (mesh.pos_x, mesh.pos_y, mesh.pos_z))
         tri = np.ascontiguousarray(mesh.indices, dtype=np.uint32)
         hg = np.ascontiguousarray(topology.interior_edges[:, :4], dtype=np.uint32)
