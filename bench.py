@@ -88,15 +88,17 @@ def run_path(mesh, topo, pinned, precision, steps, warmup, dt, ir_sweeps=1, ir_f
 
 
 def roofline_of(prof):
-    """Dominant kernels: the band launches of the supernodal solve."""
+    """Dominant kernels: the band launches of the supernodal solve (forward
+    bands, the root band, backward bands; 2 * bands - 1 launches a sweep)."""
     us = sum(prof["fwd_us"]) + sum(prof["bwd_us"])
     by = sum(prof["fwd_bytes"]) + sum(prof["bwd_bytes"])
-    launches = 2 * prof["bands"]
+    launches = prof["sweeps"] * (2 * prof["bands"] - 1)
     achieved = by / (us * 1e-6) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                kernel="k_fwd_band/k_bwd_band (supernodal sweeps)",
-                launch_avg_us=round(us / launches, 3), bytes_per_launch=round(by / launches))
+                kernel="k_fwd_band/k_root_band/k_bwd_band (supernodal sweeps)",
+                launch_avg_us=round(us / launches, 3), bytes_per_launch=round(by / launches),
+                launches_per_iteration=prof["launches"])
 
 
 def cpu_baseline(cells, sample_steps, dt):

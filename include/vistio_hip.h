@@ -158,17 +158,25 @@ vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps);
 vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out);
 
 /* Per-kernel timing of PD iterations, measured with HIP events on the
- * context's stream around every launch (no graph). Bands: forward runs
- * the deepest band first, backward the root band first. bytes = algorithmic
- * bytes one launch must move (block entries, element records, vectors). */
+ * context's stream around every launch (no graph). An iteration is the
+ * local step plus `sweeps` solves (ir_sweeps in FP32 mode, 1 in FP64), as in
+ * every iteration of a step but the last. A solve launches the forward
+ * bands deepest first (fwd_*[0 .. bands-2]), the root band, whose forward
+ * and backward sweeps are one launch (fwd_*[bands-1]), then the backward
+ * bands below the root, shallowest first (bwd_*[1 .. bands-1]; bwd_*[0] is
+ * 0). Times and bytes are per iteration, summed over its sweeps; bytes are
+ * the algorithmic bytes the launches must move (block entries, element
+ * records, vectors). */
 #define VH_MAX_BANDS 16
 typedef struct {
   uint32_t bands;
   uint32_t iterations;      /* PD iterations timed */
+  uint32_t sweeps;          /* solves per iteration */
+  uint32_t launches;        /* kernel launches per iteration */
   double local_us, local_bytes;
   double fwd_us[VH_MAX_BANDS], fwd_bytes[VH_MAX_BANDS];
   double bwd_us[VH_MAX_BANDS], bwd_bytes[VH_MAX_BANDS];
-  double ir_us, ir_bytes;   /* refinement update (FP32 mode), per launch */
+  double ir_us, ir_bytes;   /* refinement updates between sweeps (FP32 mode) */
   double iteration_us;      /* one PD iteration, all launches */
 } vh_profile;
 /* Times `iterations` PD iterations from the current device state (the state

@@ -1,8 +1,9 @@
 // Test-only CPU model of the HIP solve schedule (tests/ infrastructure).
 // Runs the product's host set-up (vistio_amd/csrc/vh_setup.cpp) and replays
-// the forward/backward task lists exactly as k_fwd / k_bwd do, sequentially
-// in double, so the set-up's supernode blocks, extend-add maps and task
-// schedule can be checked against an independent solve without a GPU.
+// the forward, root and backward task lists as k_fwd_band / k_root_band /
+// k_bwd_band do, sequentially in double, so the set-up's supernode blocks,
+// push slots, root inverse and task schedule can be checked against an
+// independent solve without a GPU.
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
@@ -12,11 +13,13 @@
 
 extern "C" int emul_solve(int n, const float* px, const float* py, const float* pz, int t, const uint32_t* tri,
                           int h, const uint32_t* hinge, const uint8_t* pinned, double k_mem, double k_bend,
-                          double density, double dt_matrix, int leaf, const double* b, double* x, long long* stats) {
+                          double density, double dt_matrix, int leaf, int max_band_levels, const double* b,
+                          double* x, long long* stats) {
   vh::SetupInput in;
   in.n = n; in.t = t; in.h = h; in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge;
   in.pinned = pinned; in.k_membrane = k_mem; in.k_bending = k_bend; in.density = density;
   in.dt_matrix = dt_matrix; in.leaf_size = leaf;
+  if (max_band_levels > 0) in.max_band_levels = max_band_levels;
   vh::Setup S;
   if (!vh::build_setup(in, S)) return S.status ? S.status : -1;
   std::vector<double> r(3 * (size_t)n), y(3 * (size_t)n), d(3 * (size_t)n), U(3 * (size_t)S.u_total + 3);
@@ -34,6 +37,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       const vh::Task& T = S.ptasks[t];
       const vh::Supernode& P = S.sn[T.sn];
       if (P.depth != S.max_depth - l) return -10;
+      if ((T.kind == 2) != (l == nb - 1) || (T.kind == 2) != (P.parent < 0)) return -12;
       const int w = P.w, m = P.w + P.r;
       std::vector<double> front(3 * (size_t)m, 0.0);
       for (int j = 0; j < m; ++j)
@@ -41,6 +45,20 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
           for (int k = 0; k < 3; ++k) front[3 * j + k] += F[3 * ((size_t)P.fm_off + (size_t)j * P.fslot_k + q) + k];
       for (int j = 0; j < w; ++j)
         for (int k = 0; k < 3; ++k) front[3 * j + k] = r[3 * (size_t)(P.col0 + j) + k] - front[3 * j + k];
+      if (T.kind == 2) {  // root: d = Sroot front, pushed to the descendants
+        if (P.r != 0 || S.Sroot.size() != (size_t)w * w) return -13;
+        for (int row = T.b; row < T.e; ++row) {
+          const double* Sr = &S.Sroot[(size_t)row * w];
+          double s3[3] = {0, 0, 0};
+          for (int kk = 0; kk < w; ++kk)
+            for (int k = 0; k < 3; ++k) s3[k] += Sr[kk] * front[3 * kk + k];
+          const int col = P.col0 + row;
+          for (int k = 0; k < 3; ++k) d[3 * (size_t)col + k] = s3[k];
+          for (int q = S.dpush_ptr[col]; q < S.dpush_ptr[col + 1]; ++q)
+            for (int k = 0; k < 3; ++k) V[3 * (size_t)S.dpush_idx[q] + k] = -s3[k];
+        }
+        continue;
+      }
       const double* Mp = &S.M[P.m_off];
       const long long tri_n = (long long)w * (w + 1) / 2;
       for (int row = T.b; row < T.e; ++row) {
@@ -59,7 +77,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
     for (int t = S.bwd_level_off[l]; t < S.bwd_level_off[l + 1]; ++t) {
       const vh::Task& T = S.ptasks[nfwd + t];
       const vh::Supernode& P = S.sn[T.sn];
-      if (P.depth != l) return -11;
+      if (P.depth != l || l == 0 || T.kind != 1) return -11;
       const int w = P.w, m = P.w + P.r;
       std::vector<double> v(3 * (size_t)m);
       for (int kk = T.b; kk < m; ++kk)

@@ -64,7 +64,8 @@ VH_MAX_BANDS = 16
 
 
 class Profile(C.Structure):
-    _fields_ = [("bands", C.c_uint32), ("iterations", C.c_uint32), ("local_us", C.c_double),
+    _fields_ = [("bands", C.c_uint32), ("iterations", C.c_uint32), ("sweeps", C.c_uint32),
+                ("launches", C.c_uint32), ("local_us", C.c_double),
                 ("local_bytes", C.c_double), ("fwd_us", C.c_double * VH_MAX_BANDS),
                 ("fwd_bytes", C.c_double * VH_MAX_BANDS), ("bwd_us", C.c_double * VH_MAX_BANDS),
                 ("bwd_bytes", C.c_double * VH_MAX_BANDS), ("ir_us", C.c_double), ("ir_bytes", C.c_double),

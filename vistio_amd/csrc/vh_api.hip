@@ -45,7 +45,7 @@ struct vh_ctx {
   // solve
   vhk::TaskDev* tasks = nullptr;  // forward tasks then backward tasks
   int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
-  void *M = nullptr, *Mt = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
+  void *M = nullptr, *Mt = nullptr, *Sroot = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
   double* Ax = nullptr;
   double* partials = nullptr;
@@ -137,16 +137,28 @@ vhk::RhsSrc<T> rhs_src(vh_ctx* c, double dt) {
                         1.0 / (c->P.dt_matrix * c->P.dt_matrix)};
 }
 
-vhk::CommitArgs commit_args(vh_ctx* c, bool on, bool want_resid) {
-  return vhk::CommitArgs{on ? 1 : 0, c->P.has_ground, c->P.ground_height, c->x, c->prev, c->pinned, c->partials,
-                         want_resid ? 1 : 0};
+vhk::CommitArgs commit_args(vh_ctx* c, bool on, bool want_resid, bool add_acc) {
+  return vhk::CommitArgs{on ? 1 : 0,   c->P.has_ground, c->P.ground_height, c->x, c->prev, c->pinned, c->partials,
+                         want_resid ? 1 : 0, c->acc, add_acc ? 1 : 0};
 }
 
-// One solve of A d = r (forward bands deepest first, backward bands root
-// first). from_slots: the first forward band forms r from the RHS slots.
-// commit: the last backward band adds d to x (FP64 mode).
+// Records an event after each launch when profiling (ev == nullptr: off).
+struct Marks {
+  hipEvent_t* ev = nullptr;
+  size_t k = 0;
+  void mark(hipStream_t s) {
+    if (ev) (void)hipEventRecord(ev[k++], s);
+  }
+};
+
+// One solve of A d = r: forward bands deepest first, then the root band
+// (forward and backward in one launch through its dense inverse), then the
+// backward bands below the root. from_slots: the first launch forms r from
+// the RHS slots. commit: the last launch adds d (plus acc when add_acc) to x.
+// Launch count: 2 * bands - 1.
 template <class T>
-void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool want_resid, const vhk::StepDev* st) {
+void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_acc, bool want_resid,
+                   const vhk::StepDev* st, Marks& mk) {
   const vh::Setup& S = c->S;
   const int nb = S.max_depth + 1;
   const int nfwd = (int)S.fwd_tasks.size();
@@ -154,7 +166,7 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool want
   vhk::BandArgs<T> a{};
   a.uslot = c->uslot; a.dpush_ptr = c->dpush_ptr; a.dpush_idx = c->dpush_idx;
   a.src = rhs_src<T>(c, dt); a.r = c->r; a.upper_cols = c->upper_cols; a.nupper = nupper;
-  for (int l = 0; l < nb; ++l) {
+  for (int l = 0; l + 1 < nb; ++l) {
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
     a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = first ? 1 : 0;
@@ -163,18 +175,32 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool want
     if (grid > 0)
       hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M, (T*)c->y,
                          (T*)c->F, st);
+    mk.mark(c->stream);
+  }
+  {
+    // The root band: upper_cols is empty when it is the only band.
+    const int b = S.fwd_level_off[nb - 1], e = S.fwd_level_off[nb];
+    const bool only = nb == 1;
+    a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = only && from_slots ? 1 : 0;
+    const vhk::CommitArgs cc = commit_args(c, only && commit, want_resid, add_acc);
+    const size_t lds = 3 * (size_t)S.band_front[0] * c->tsz;
+    if (e > b)
+      hipLaunchKernelGGL(vhk::k_root_band<T>, dim3(e - b), dim3(256), lds, c->stream, a, cc, (const T*)c->Sroot,
+                         (T*)c->F, (T*)c->V, (T*)c->d, st);
+    mk.mark(c->stream);
   }
   a.rhs_from_slots = 0;
-  for (int l = 0; l < nb; ++l) {
+  for (int l = 1; l < nb; ++l) {
     const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
     const bool last = l == nb - 1;
     a.tasks = c->tasks + nfwd + b; a.ntasks = e - b;
-    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid);
+    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid, add_acc);
     const int grid = (e - b) + (last && commit ? nblk(nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[l] * c->tsz;
     if (grid > 0)
       hipLaunchKernelGGL(vhk::k_bwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
                          (const T*)c->y, (T*)c->V, (T*)c->d, st);
+    mk.mark(c->stream);
   }
 }
 
@@ -190,20 +216,24 @@ void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
 }
 
 // One correction of the iterate: x += A^{-1} (b - A x). FP64: one solve whose
-// last band commits. FP32: `sweeps` solves with FP64 residual updates, the
-// last update commits the accumulated correction.
+// last launch commits. FP32: `sweeps` solves; between them the correction is
+// accumulated in acc and r updated with the FP64 residual, and the last
+// solve's last launch commits acc + d. Without commit (a standalone solve)
+// every sweep accumulates into acc.
 void enqueue_correction(vh_ctx* c, double dt, int sweeps, bool from_slots, bool commit, bool want_resid,
-                        const vhk::StepDev* st) {
+                        const vhk::StepDev* st, Marks& mk) {
   if (c->prec == VH_PREC_F64) {
-    enqueue_solve<double>(c, dt, from_slots, commit, want_resid, st);
+    enqueue_solve<double>(c, dt, from_slots, commit, false, want_resid, st, mk);
     return;
   }
   for (int s = 0; s < sweeps; ++s) {
     const bool last = s + 1 == sweeps;
-    enqueue_solve<float>(c, dt, from_slots && s == 0, false, false, st);
-    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid);
-    hipLaunchKernelGGL(vhk::k_ir_update, dim3(nblk(c->S.n)), dim3(256), 0, c->stream, c->S.n, c->Ap, c->Ai, c->Ax,
-                       (const float*)c->d, c->r, c->acc, s == 0 ? 1 : 0, last ? 0 : 1, cc, st);
+    enqueue_solve<float>(c, dt, from_slots && s == 0, last && commit, s > 0, last && want_resid, st, mk);
+    if (!last || !commit) {
+      hipLaunchKernelGGL(vhk::k_ir_update, dim3(nblk(c->S.n)), dim3(256), 0, c->stream, c->S.n, c->Ap, c->Ai, c->Ax,
+                         (const float*)c->d, c->r, c->acc, s == 0 ? 1 : 0, last ? 0 : 1, st);
+      mk.mark(c->stream);
+    }
   }
 }
 
@@ -219,7 +249,8 @@ void enqueue_step(vh_ctx* c, double dt) {
     const bool want_resid = last || P.tolerance > 0;
     if (c->prec == VH_PREC_F64) enqueue_local<double>(c, c->st);
     else enqueue_local<float>(c, c->st);
-    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, c->st);
+    Marks none;
+    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, c->st, none);
     if (want_resid)
       hipLaunchKernelGGL(vhk::k_resid_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, c->resid_blocks,
                          P.tolerance, it, c->st);
@@ -411,6 +442,7 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   UP(upload(ctx, &ctx->upper_cols, S.upper_cols));
   UP(upload_t(ctx, &ctx->M, S.M));
   UP(upload_t(ctx, &ctx->Mt, S.Mt));
+  UP(upload_t(ctx, &ctx->Sroot, S.Sroot));
   UP(dalloc_bytes(ctx, &ctx->y, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->d, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->F, 3 * (size_t)std::max<int64_t>(1, S.fslot_total) * ctx->tsz));
@@ -420,11 +452,11 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
     UP(upload(ctx, &ctx->Ai, S.Ai));
     UP(upload(ctx, &ctx->Ax, S.Ax));
   }
-  // Residual partials come from the last backward band (FP64) or the last
-  // refinement update (FP32).
+  // Residual partials come from the committing launch: the last backward
+  // band, or the root band when it is the only band.
   const int nb = S.max_depth + 1;
-  const int last_bwd = S.bwd_level_off[nb] - S.bwd_level_off[nb - 1];
-  ctx->resid_blocks = ctx->prec == VH_PREC_F64 ? last_bwd + nblk((int)S.upper_cols.size()) : nblk(S.n);
+  ctx->resid_blocks = nb == 1 ? S.fwd_level_off[1] - S.fwd_level_off[0]
+                              : S.bwd_level_off[nb] - S.bwd_level_off[nb - 1] + nblk((int)S.upper_cols.size());
   UP(dalloc(ctx, &ctx->partials, 2 * (size_t)std::max(1, ctx->resid_blocks)));
   UP(dalloc(ctx, &ctx->st, 1));
 #undef UP
@@ -435,10 +467,13 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   for (int f : S.band_front) lds = std::max(lds, 3 * (size_t)f * ctx->tsz);
   if (lds > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "supernode front exceeds the 160 KB LDS of one workgroup");
   if (lds > 64 * 1024) {
-    const void* kf = ctx->prec == VH_PREC_F64 ? (const void*)vhk::k_fwd_band<double> : (const void*)vhk::k_fwd_band<float>;
-    const void* kb = ctx->prec == VH_PREC_F64 ? (const void*)vhk::k_bwd_band<double> : (const void*)vhk::k_bwd_band<float>;
+    const bool f64 = ctx->prec == VH_PREC_F64;
+    const void* kf = f64 ? (const void*)vhk::k_fwd_band<double> : (const void*)vhk::k_fwd_band<float>;
+    const void* kb = f64 ? (const void*)vhk::k_bwd_band<double> : (const void*)vhk::k_bwd_band<float>;
+    const void* kr = f64 ? (const void*)vhk::k_root_band<double> : (const void*)vhk::k_root_band<float>;
     VH_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     VH_HIP(hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    VH_HIP(hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   // Initial state (state.rs:119-139): pos = prev = mesh positions, vel = 0.
   std::vector<double> h3(n3);
@@ -585,7 +620,8 @@ vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps) {
   const size_t n3 = 3 * (size_t)n;
   to_dev_order(ctx->S, b, b + n, b + 2 * (size_t)n, ctx->hbuf);
   VH_HIP(hipMemcpyAsync(ctx->r, ctx->hbuf.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  enqueue_correction(ctx, ctx->P.dt_matrix, sweeps, false, false, false, nullptr);
+  Marks none;
+  enqueue_correction(ctx, ctx->P.dt_matrix, sweeps, false, false, false, nullptr, none);
   VH_HIP(hipGetLastError());
   if (ctx->prec == VH_PREC_F64) {
     VH_HIP(hipMemcpyAsync(ctx->hbuf.data(), ctx->d, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -620,83 +656,54 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
   if (nb > VH_MAX_BANDS) return fail(ctx, VH_E_UNSUPPORTED, "more bands than vh_profile holds");
   VH_HIP(hipSetDevice(ctx->device));
   std::memset(out, 0, sizeof(*out));
+  const bool f64 = ctx->prec == VH_PREC_F64;
+  const int sweeps = f64 ? 1 : std::max(1, ctx->P.ir_sweeps);
+  const int solve_launches = 2 * nb - 1;
+  const int per_iter = 1 + sweeps * solve_launches + (sweeps - 1);
   out->bands = (uint32_t)nb;
   out->iterations = iterations;
+  out->sweeps = (uint32_t)sweeps;
+  out->launches = (uint32_t)per_iter;
   const double ts = (double)ctx->tsz;
-  // algorithmic bytes per launch
+  // Algorithmic bytes per launch.
   const double n = S.n, t = S.t, h = S.h;
   out->local_bytes = n * 24 + t * (12 + 32 + ts + 12 + 9 * ts) + h * (16 + 2 * ts + 16 + 12 * ts);
   std::vector<double> band_entries(nb, 0), band_rows(nb, 0), band_cols(nb, 0);
   for (const vh::Supernode& q : S.sn) {
-    band_entries[q.depth] += (double)q.w * (q.w + 1) / 2 + (double)q.r * q.w;
+    band_entries[q.depth] += q.parent < 0 ? (double)q.w * q.w : (double)q.w * (q.w + 1) / 2 + (double)q.r * q.w;
     band_rows[q.depth] += q.w + q.r;
     band_cols[q.depth] += q.w;
   }
-  for (int l = 0; l < nb; ++l) {
+  const double slots_in = (double)S.inc_ptr[S.n] * 3 * ts + n * 56;  // RHS slots, mass, pred, x -> r
+  std::vector<double> fb(nb, 0), bb(nb, 0);
+  for (int l = 0; l + 1 < nb; ++l) {
     const int fd = S.max_depth - l;  // forward launch l runs band fd
-    // block entries + front inputs (r or RHS slots, child slots) + outputs
-    out->fwd_bytes[l] = band_entries[fd] * ts + band_rows[fd] * 3 * ts + band_cols[fd] * 24 +
-                        (l == 0 ? (double)S.inc_ptr[S.n] * 3 * ts + n * 56 : 0.0) + band_rows[fd] * 3 * ts;
-    out->bwd_bytes[l] = band_entries[l] * ts + band_rows[l] * 3 * ts + band_cols[l] * 3 * ts * 2 +
-                        (l == nb - 1 && ctx->prec == VH_PREC_F64 ? n * 48 : 0.0);
+    // block entries + front inputs (r or RHS slots, child slots) + update rows out
+    fb[l] = band_entries[fd] * ts + band_rows[fd] * 3 * ts + band_cols[fd] * 24 + (l == 0 ? slots_in : 0.0) +
+            band_rows[fd] * 3 * ts;
   }
-  out->ir_bytes = ctx->prec == VH_PREC_F32_IR ? (double)S.Ai.size() * 12 + n * (12 + 48 + 48) : 0.0;
-  // events: one after every launch
-  const int per_iter = 1 + 2 * nb + (ctx->prec == VH_PREC_F32_IR ? 1 : 0);
+  // root: dense inverse + front inputs + d and its pushes (+ commit when alone)
+  fb[nb - 1] = band_entries[0] * ts + band_rows[0] * 3 * ts + band_cols[0] * 24 + band_cols[0] * 3 * ts * 2 +
+               (nb == 1 ? slots_in + n * 48 : 0.0);
+  for (int l = 1; l < nb; ++l)
+    bb[l] = band_entries[l] * ts + band_rows[l] * 3 * ts + band_cols[l] * 3 * ts * 2 + (l == nb - 1 ? n * 48 : 0.0);
+  const double ir_bytes = f64 ? 0.0 : (double)S.Ai.size() * 12 + n * (12 + 48 + 48);
   std::vector<hipEvent_t> ev((size_t)iterations * per_iter + 1);
   for (auto& e : ev) VH_HIP(hipEventCreate(&e));
-  const bool f64 = ctx->prec == VH_PREC_F64;
-  size_t k = 0;
-  VH_HIP(hipEventRecord(ev[k++], ctx->stream));
-  const vhk::StepDev* st = nullptr;
+  Marks mk;
+  mk.ev = ev.data();
+  mk.mark(ctx->stream);
   for (uint32_t it = 0; it < iterations; ++it) {
-    if (f64) enqueue_local<double>(ctx, st); else enqueue_local<float>(ctx, st);
-    VH_HIP(hipEventRecord(ev[k++], ctx->stream));
-    // the same launches as one correction, with an event after each
-    const int nfwd = (int)S.fwd_tasks.size();
-    const int nupper = (int)S.upper_cols.size();
-    auto run = [&](auto zero) -> vh_status {
-      using T = decltype(zero);
-      vhk::BandArgs<T> a{};
-      a.uslot = ctx->uslot; a.dpush_ptr = ctx->dpush_ptr; a.dpush_idx = ctx->dpush_idx;
-      a.src = rhs_src<T>(ctx, dt); a.r = ctx->r; a.upper_cols = ctx->upper_cols; a.nupper = nupper;
-      for (int l = 0; l < nb; ++l) {
-        const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
-        a.tasks = ctx->tasks + b; a.ntasks = e - b; a.rhs_from_slots = l == 0 ? 1 : 0;
-        const int grid = (e - b) + (l == 0 ? nblk(4 * nupper) : 0);
-        const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * ctx->tsz;
-        if (grid > 0)
-          hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, ctx->stream, a, (const T*)ctx->M,
-                             (T*)ctx->y, (T*)ctx->F, st);
-        VH_HIP(hipEventRecord(ev[k++], ctx->stream));
-      }
-      a.rhs_from_slots = 0;
-      for (int l = 0; l < nb; ++l) {
-        const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
-        const bool last = l == nb - 1;
-        a.tasks = ctx->tasks + nfwd + b; a.ntasks = e - b;
-        const bool commit = last && f64;
-        const vhk::CommitArgs cc = commit_args(ctx, commit, false);
-        const int grid = (e - b) + (commit ? nblk(nupper) : 0);
-        const size_t lds = 3 * (size_t)S.band_front[l] * ctx->tsz;
-        if (grid > 0)
-          hipLaunchKernelGGL(vhk::k_bwd_band<T>, dim3(grid), dim3(256), lds, ctx->stream, a, cc, (const T*)ctx->Mt,
-                             (const T*)ctx->y, (T*)ctx->V, (T*)ctx->d, st);
-        VH_HIP(hipEventRecord(ev[k++], ctx->stream));
-      }
-      return VH_OK;
-    };
-    vh_status rs = f64 ? run(0.0) : run(0.0f);
-    if (rs != VH_OK) return rs;
-    if (!f64) {
-      const vhk::CommitArgs cc = commit_args(ctx, true, false);
-      hipLaunchKernelGGL(vhk::k_ir_update, dim3(nblk(S.n)), dim3(256), 0, ctx->stream, S.n, ctx->Ap, ctx->Ai, ctx->Ax,
-                         (const float*)ctx->d, ctx->r, ctx->acc, 1, 0, cc, st);
-      VH_HIP(hipEventRecord(ev[k++], ctx->stream));
-    }
+    if (f64) enqueue_local<double>(ctx, nullptr); else enqueue_local<float>(ctx, nullptr);
+    mk.mark(ctx->stream);
+    enqueue_correction(ctx, dt, sweeps, true, true, false, nullptr, mk);
   }
   VH_HIP(hipGetLastError());
   VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (mk.k != ev.size()) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    return fail(ctx, VH_E_INTERNAL, "profile: launch count mismatch");
+  }
   auto span = [&](size_t a, size_t b) -> double {
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ev[a], ev[b]);
@@ -706,14 +713,20 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
     const size_t base = (size_t)it * per_iter;
     size_t q = base;
     out->local_us += span(q, q + 1); ++q;
-    for (int l = 0; l < nb; ++l) { out->fwd_us[l] += span(q, q + 1); ++q; }
-    for (int l = 0; l < nb; ++l) { out->bwd_us[l] += span(q, q + 1); ++q; }
-    if (!f64) { out->ir_us += span(q, q + 1); ++q; }
+    for (int s = 0; s < sweeps; ++s) {
+      for (int l = 0; l < nb; ++l) { out->fwd_us[l] += span(q, q + 1); ++q; }
+      for (int l = 1; l < nb; ++l) { out->bwd_us[l] += span(q, q + 1); ++q; }
+      if (s + 1 < sweeps) { out->ir_us += span(q, q + 1); ++q; }
+    }
     out->iteration_us += span(base, base + per_iter);
   }
   const double inv = 1.0 / iterations;
   out->local_us *= inv; out->ir_us *= inv; out->iteration_us *= inv;
-  for (int l = 0; l < nb; ++l) { out->fwd_us[l] *= inv; out->bwd_us[l] *= inv; }
+  for (int l = 0; l < nb; ++l) {
+    out->fwd_us[l] *= inv; out->bwd_us[l] *= inv;
+    out->fwd_bytes[l] = fb[l] * sweeps; out->bwd_bytes[l] = bb[l] * sweeps;
+  }
+  out->ir_bytes = ir_bytes * (sweeps - 1);
   for (auto& e : ev) (void)hipEventDestroy(e);
   return VH_OK;
 }

@@ -432,6 +432,8 @@ struct CommitArgs {
   const uint8_t* pinned;
   double* partials;  // per-block (diff, norm), when want_resid
   int want_resid;
+  const double* acc;  // earlier refinement sweeps' correction, added when add_acc
+  int add_acc;
 };
 
 __device__ __forceinline__ void commit_vertex(const CommitArgs& c, int i, double dx, double dy, double dz,
@@ -619,6 +621,15 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
   }
 }
 
+// Commit of column col with this sweep's d (plus earlier sweeps' acc).
+__device__ __forceinline__ void commit_col(const CommitArgs& c, int col, double dx, double dy, double dz, double& df,
+                                           double& nm) {
+  if (c.add_acc) {
+    dx += c.acc[3 * col + 0]; dy += c.acc[3 * col + 1]; dz += c.acc[3 * col + 2];
+  }
+  commit_vertex(c, col, dx, dy, dz, df, nm);
+}
+
 template <class T>
 __device__ __forceinline__ void bwd_store(const BandArgs<T>& a, const CommitArgs& c, int col, T sx, T sy, T sz,
                                           T* __restrict__ V, T* __restrict__ d, double& df, double& nm) {
@@ -628,7 +639,7 @@ __device__ __forceinline__ void bwd_store(const BandArgs<T>& a, const CommitArgs
     T* p = V + 3 * (size_t)a.dpush_idx[q];
     p[0] = -sx; p[1] = -sy; p[2] = -sz;
   }
-  if (c.on) commit_vertex(c, col, (double)sx, (double)sy, (double)sz, df, nm);
+  if (c.on) commit_col(c, col, (double)sx, (double)sy, (double)sz, df, nm);
 }
 
 // Backward task (L^T d = y): columns [b, e) of P, read as rows of the
@@ -716,7 +727,7 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
     if (c.on && k < a.nupper) {
       const int col = a.upper_cols[k];
       const T* s = d + 3 * (size_t)col;
-      commit_vertex(c, col, (double)s[0], (double)s[1], (double)s[2], df, nm);
+      commit_col(c, col, (double)s[0], (double)s[1], (double)s[2], df, nm);
     }
   }
   if (c.on && c.want_resid) {
@@ -725,39 +736,101 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
   }
 }
 
-// Iterative refinement (FP32 solve, FP64 residual): delta += d. Between
-// sweeps r -= A d (A in FP64 CSR, permuted order; rows only read d, so the
-// in-place update of r is race-free). After the last sweep the iteration is
-// committed with the accumulated delta.
+// Root task: the root supernode R has no off-diagonal rows, so its forward
+// and backward sweeps collapse into d_R = (L_RR L_RR^T)^{-1} t_R with the
+// dense symmetric inverse Sroot (row-major w x w); rows [b, e) of it. Same
+// front assembly as a forward task, same stores as a backward task.
+template <class T>
+__device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot,
+                          T* __restrict__ F, T* __restrict__ V, T* __restrict__ d, T* front, double& df, double& nm) {
+  const int col0 = td.q0.x, w = td.q0.y;
+  const int b = td.q1.x, e = td.q1.y, g = td.q1.z;
+  const int K = td.q2.y;
+  const T* Fp = F + 3 * (size_t)td.q2.x;
+  const int tid = threadIdx.x;
+  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  T mv[kPrefRows][kPrefLoads];
+#pragma unroll
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int row = b + grp + p * ngrp;
+    prefetch_row(mv[p], Sroot + (size_t)(row < e ? row : b) * w, row < e ? w : 0, lane, g);
+  }
+  if (a.rhs_from_slots) {
+    int L = 1;
+    while (L < 8 && (int)blockDim.x / (2 * L) >= w) L *= 2;
+    const int ln = tid & (L - 1), gp = tid / L, ng = blockDim.x / L;
+    for (int j = gp; j < w; j += ng) {
+      const double3 rb = rhs_from_slots_group(a.src, col0 + j, ln, L);
+      if (ln == 0) {
+        double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
+        T ax, ay, az;
+        front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+        front[3 * j + 0] = (T)rb.x - ax; front[3 * j + 1] = (T)rb.y - ay; front[3 * j + 2] = (T)rb.z - az;
+      }
+    }
+  } else {
+#pragma unroll 4
+    for (int j = tid; j < w; j += blockDim.x) {
+      T ax, ay, az;
+      front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+      const double* rb = a.r + 3 * (size_t)(col0 + j);
+      front[3 * j + 0] = (T)rb[0] - ax; front[3 * j + 1] = (T)rb[1] - ay; front[3 * j + 2] = (T)rb[2] - az;
+    }
+  }
+  __syncthreads();
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int row = b + grp + p * ngrp;
+    if (row >= e) break;  // uniform within the group
+    T sx, sy, sz;
+    row_dot_pref(mv[p], Sroot + (size_t)row * w, w, front, lane, g, sx, sy, sz);
+    if (lane == 0) bwd_store(a, c, col0 + row, sx, sy, sz, V, d, df, nm);
+  }
+  for (int row = b + grp + kPrefRows * ngrp; row < e; row += ngrp) {
+    T sx, sy, sz;
+    row_dot(Sroot + (size_t)row * w, w, front, lane, g, sx, sy, sz);
+    if (lane == 0) bwd_store(a, c, col0 + row, sx, sy, sz, V, d, df, nm);
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_root_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Sroot,
+                                                   T* __restrict__ F, T* __restrict__ V, T* __restrict__ d,
+                                                   const StepDev* st) {
+  if (st && st->done) return;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double df = 0, nm = 0;
+  if ((int)blockIdx.x < a.ntasks)
+    root_task<T>(a.tasks[blockIdx.x], a, c, Sroot, F, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+  if (c.on && c.want_resid) {
+    __syncthreads();
+    block_partials(df, nm, c.partials);
+  }
+}
+
+// Iterative refinement between FP32 sweeps (FP64 residual): acc += d and,
+// when update_r, r -= A d (A in FP64 CSR, permuted order; rows only read d,
+// so the in-place update of r is race-free). The last sweep's d is
+// committed together with acc by the last backward band.
 __global__ void __launch_bounds__(256) k_ir_update(int n, const int* __restrict__ Ap, const int* __restrict__ Ai,
                                                    const double* __restrict__ Ax, const float* __restrict__ d,
                                                    double* __restrict__ r, double* __restrict__ acc, int first,
-                                                   int update_r, CommitArgs c, const StepDev* st) {
+                                                   int update_r, const StepDev* st) {
   if (st && st->done) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  double df = 0, nm = 0;
-  if (i < n) {
-    double a3[3];
-    for (int k = 0; k < 3; ++k) {
-      const double dv = (double)d[3 * i + k];
-      a3[k] = first ? dv : acc[3 * i + k] + dv;
-    }
-    if (c.on) {
-      commit_vertex(c, i, a3[0], a3[1], a3[2], df, nm);
-    } else {
-      for (int k = 0; k < 3; ++k) acc[3 * i + k] = a3[k];
-    }
-    if (update_r) {
-      double sx = 0, sy = 0, sz = 0;
-      for (int p = Ap[i]; p < Ap[i + 1]; ++p) {
-        const double av = Ax[p];
-        const float* dv = d + 3 * (size_t)Ai[p];
-        sx += av * (double)dv[0]; sy += av * (double)dv[1]; sz += av * (double)dv[2];
-      }
-      r[3 * i + 0] -= sx; r[3 * i + 1] -= sy; r[3 * i + 2] -= sz;
-    }
+  if (i >= n) return;
+  for (int k = 0; k < 3; ++k) {
+    const double dv = (double)d[3 * i + k];
+    acc[3 * i + k] = first ? dv : acc[3 * i + k] + dv;
   }
-  if (c.on && c.want_resid) block_partials(df, nm, c.partials);
+  if (update_r) {
+    double sx = 0, sy = 0, sz = 0;
+    for (int p = Ap[i]; p < Ap[i + 1]; ++p) {
+      const double av = Ax[p];
+      const float* dv = d + 3 * (size_t)Ai[p];
+      sx += av * (double)dv[0]; sy += av * (double)dv[1]; sz += av * (double)dv[2];
+    }
+    r[3 * i + 0] -= sx; r[3 * i + 1] -= sy; r[3 * i + 2] -= sz;
+  }
 }
 
 // Final residual of a PD iteration (pd_solver.rs:402-421, 440-444): sums the
@@ -780,13 +853,6 @@ __global__ void __launch_bounds__(256) k_resid_reduce(const double* __restrict__
     st->iterations = iter + 1;
     if (res < tol) st->done = 1;
   }
-}
-
-// Iteration counter when no residual is formed (tolerance 0, not the last
-// iteration): keeps StepResult.iterations exact.
-__global__ void k_count_iter(int iter, StepDev* st) {
-  if (st->done) return;
-  st->iterations = iter + 1;
 }
 
 // Prediction, state.rs:145-169 and pd_solver.rs:274-283: prev = x;

@@ -297,14 +297,16 @@ bool build_setup(const SetupInput& in, Setup& S) {
   const int maxb = std::max(1, in.max_band_levels);
   std::vector<std::vector<int>> at_depth(L);
   for (int x = 0; x < nf; ++x) at_depth[fdep[x]].push_back(x);
+  // A band costs two launches (forward, backward) over its blocks; the root
+  // band costs one launch over its dense w x w inverse.
   auto band_cost = [&](int a, int b) -> double {
     double entries = 0;
     for (int x : at_depth[a]) {
       const double W = (double)Wb[(size_t)x * L + b];
       if (W + fr[x] > front_cap) return 1e300;
-      entries += W * (W + 1) / 2 + (double)fr[x] * W;
+      entries += a == 0 ? W * W : W * (W + 1) / 2 + (double)fr[x] * W;
     }
-    return 2.0 * (in.band_launch_us * 1e-6 + entries * in.value_bytes / (in.band_gbps * 1e9));
+    return (a == 0 ? 1.0 : 2.0) * (in.band_launch_us * 1e-6 + entries * in.value_bytes / (in.band_gbps * 1e9));
   };
   std::vector<double> best(L + 1, 1e300);
   std::vector<int> from(L + 1, 0);
@@ -565,6 +567,21 @@ bool build_setup(const SetupInput& in, Setup& S) {
       for (int k = j; k < w; ++k) Tj[k - j] = Mp[(size_t)k * (k + 1) / 2 + j];
       for (int q = 0; q < r; ++q) Tj[w - j + q] = Wp[(size_t)q * w + j];
     }
+    if (x.parent < 0) {  // root: full inverse of L_RR L_RR^T = Linv^T Linv
+      if (r != 0) { S.status = VH_E_INTERNAL; S.err = "root supernode has off-diagonal rows"; return false; }
+      S.Sroot.assign((size_t)w * w, 0.0);
+      double* Sp = S.Sroot.data();
+      for (int k = 0; k < w; ++k) {
+        const double* Lk = &Linv[(size_t)k * w];
+        for (int i = 0; i <= k; ++i) {
+          const double a = Lk[i];
+          double* Si = Sp + (size_t)i * w;
+          for (int j = i; j <= k; ++j) Si[j] += a * Lk[j];
+        }
+      }
+      for (int i = 0; i < w; ++i)
+        for (int j = 0; j < i; ++j) Sp[(size_t)i * w + j] = Sp[(size_t)j * w + i];
+    }
   }
 
   // Elements with permuted vertex labels, sorted by their smallest label for
@@ -653,8 +670,9 @@ bool build_setup(const SetupInput& in, Setup& S) {
       const int g = lanes_for(x.w);
       const int m = x.w + x.r;
       const int rows_per = rows_for(x.w, g);
+      const int kind = x.parent < 0 ? 2 : 0;
       for (int b = 0; b < m; b += rows_per) {
-        S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g, 0});
+        S.fwd_tasks.push_back({s, b, std::min(m, b + rows_per), g, kind});
         nfc[s]++;
       }
     }
@@ -664,7 +682,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
     S.bwd_level_off[lvl] = (int)S.bwd_tasks.size();
     for (int s = 0; s < ns; ++s) {
       const Supernode& x = S.sn[s];
-      if (x.depth != lvl) continue;
+      if (x.depth != lvl || x.parent < 0) continue;  // the root is finished by its root tasks
       const int m = x.w + x.r;
       const int g = lanes_for(m);
       const int rows_per = rows_for(m, g);

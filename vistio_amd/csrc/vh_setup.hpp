@@ -61,9 +61,11 @@ struct Supernode {
   int fslot_k = 0;     // slots per front position: most child update rows feeding one position
 };
 
-// A solve task: rows [b, e) of one of supernode sn's two stored forms with g
+// A solve task: rows [b, e) of one of supernode sn's stored forms with g
 // lanes per row. kind 0 = forward (rows of M_P), 1 = backward (rows of M_P^T,
-// i.e. columns of M_P).
+// i.e. columns of M_P), 2 = root (rows of Sroot). Forward tasks cover every
+// band but the root's; the root band's forward level holds the root tasks;
+// the backward list has no root level.
 struct Task {
   int sn, b, e, g, kind;
 };
@@ -105,6 +107,9 @@ struct Setup {
   std::vector<int32_t> band_front;   // max (w + r) per band
   std::vector<double> M;             // M_P row-major trapezoids (forward sweep)
   std::vector<double> Mt;            // M_P^T: column j of M_P stored contiguously (backward sweep)
+  // The root supernode (no off-diagonal rows) is solved in one pass:
+  // d_R = (L_RR L_RR^T)^{-1} t_R with this dense w_R x w_R row-major inverse.
+  std::vector<double> Sroot;
   int64_t m_total = 0;
   int u_total = 0;
   int max_front = 0;                 // max (w + r)

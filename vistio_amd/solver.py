@@ -327,12 +327,13 @@ class HipPdSolver:
         return x.reshape(3, self._n)
 
     def profile(self, dt: float, iterations: int = 10) -> dict:
-        """Per-launch kernel times (HIP events on the solver's stream) and
-        algorithmic bytes for `iterations` PD iterations."""
+        """Kernel times (HIP events on the solver's stream) and algorithmic
+        bytes per PD iteration, averaged over `iterations` iterations."""
         p = nat.Profile()
         _check(self._L.vh_pd_profile(self._ctx, float(dt), int(iterations), C.byref(p)), self._ctx, "profile")
         nb = p.bands
-        return dict(bands=nb, iterations=p.iterations, local_us=p.local_us, local_bytes=p.local_bytes,
+        return dict(bands=nb, iterations=p.iterations, sweeps=p.sweeps, launches=p.launches,
+                    local_us=p.local_us, local_bytes=p.local_bytes,
                     fwd_us=list(p.fwd_us[:nb]), fwd_bytes=list(p.fwd_bytes[:nb]), bwd_us=list(p.bwd_us[:nb]),
                     bwd_bytes=list(p.bwd_bytes[:nb]), ir_us=p.ir_us, ir_bytes=p.ir_bytes,
                     iteration_us=p.iteration_us)
