@@ -169,37 +169,59 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
   for (int l = 0; l + 1 < nb; ++l) {
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
-    a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = first ? 1 : 0;
+    a.tasks = c->tasks + b; a.ntasks = e - b;
     const int grid = (e - b) + (first ? nblk(4 * nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * c->tsz;
-    if (grid > 0)
-      hipLaunchKernelGGL(vhk::k_fwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M, (T*)c->y,
-                         (T*)c->F, st);
+    if (grid > 0) {
+      if (first)
+        hipLaunchKernelGGL((vhk::k_fwd_band<T, true>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
+                           (T*)c->y, (T*)c->F, st);
+      else
+        hipLaunchKernelGGL((vhk::k_fwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
+                           (T*)c->y, (T*)c->F, st);
+    }
     mk.mark(c->stream);
   }
   {
     // The root band: upper_cols is empty when it is the only band.
     const int b = S.fwd_level_off[nb - 1], e = S.fwd_level_off[nb];
     const bool only = nb == 1;
-    a.tasks = c->tasks + b; a.ntasks = e - b; a.rhs_from_slots = only && from_slots ? 1 : 0;
-    const vhk::CommitArgs cc = commit_args(c, only && commit, want_resid, add_acc);
+    const bool slots = only && from_slots, cm = only && commit;
+    a.tasks = c->tasks + b; a.ntasks = e - b;
+    const vhk::CommitArgs cc = commit_args(c, cm, want_resid, add_acc);
     const size_t lds = 3 * (size_t)S.band_front[0] * c->tsz;
-    if (e > b)
-      hipLaunchKernelGGL(vhk::k_root_band<T>, dim3(e - b), dim3(256), lds, c->stream, a, cc, (const T*)c->Sroot,
-                         (T*)c->F, (T*)c->V, (T*)c->d, st);
+    const T* Sr = (const T*)c->Sroot;
+    if (e > b) {
+      if (slots && cm)
+        hipLaunchKernelGGL((vhk::k_root_band<T, true, true>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+                           (T*)c->F, (T*)c->V, (T*)c->d, st);
+      else if (slots)
+        hipLaunchKernelGGL((vhk::k_root_band<T, true, false>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+                           (T*)c->F, (T*)c->V, (T*)c->d, st);
+      else if (cm)
+        hipLaunchKernelGGL((vhk::k_root_band<T, false, true>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+                           (T*)c->F, (T*)c->V, (T*)c->d, st);
+      else
+        hipLaunchKernelGGL((vhk::k_root_band<T, false, false>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+                           (T*)c->F, (T*)c->V, (T*)c->d, st);
+    }
     mk.mark(c->stream);
   }
-  a.rhs_from_slots = 0;
   for (int l = 1; l < nb; ++l) {
     const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
-    const bool last = l == nb - 1;
+    const bool cm = l == nb - 1 && commit;
     a.tasks = c->tasks + nfwd + b; a.ntasks = e - b;
-    const vhk::CommitArgs cc = commit_args(c, last && commit, want_resid, add_acc);
-    const int grid = (e - b) + (last && commit ? nblk(nupper) : 0);
+    const vhk::CommitArgs cc = commit_args(c, cm, want_resid, add_acc);
+    const int grid = (e - b) + (cm ? nblk(nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[l] * c->tsz;
-    if (grid > 0)
-      hipLaunchKernelGGL(vhk::k_bwd_band<T>, dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
-                         (const T*)c->y, (T*)c->V, (T*)c->d, st);
+    if (grid > 0) {
+      if (cm)
+        hipLaunchKernelGGL((vhk::k_bwd_band<T, true>), dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
+                           (const T*)c->y, (T*)c->V, (T*)c->d, st);
+      else
+        hipLaunchKernelGGL((vhk::k_bwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
+                           (const T*)c->y, (T*)c->V, (T*)c->d, st);
+    }
     mk.mark(c->stream);
   }
 }
@@ -242,15 +264,17 @@ void enqueue_step(vh_ctx* c, double dt) {
   const vh_pd_params& P = c->P;
   hipLaunchKernelGGL(vhk::k_predict, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->x, c->v, c->prev, c->pred,
                      c->pinned, dt, P.gravity[0], P.gravity[1], P.gravity[2], P.has_ground, P.ground_height, c->st);
+  // With tolerance 0 the loop never exits early: only the last iteration's
+  // residual (StepResult.final_residual) is formed, and the kernels do not
+  // read the early-exit flag.
+  const vhk::StepDev* stk = P.tolerance > 0 ? c->st : nullptr;
   for (int it = 0; it < P.max_iterations; ++it) {
     const bool last = it + 1 == P.max_iterations;
-    // With tolerance 0 the loop never exits early, so only the last
-    // iteration's residual (StepResult.final_residual) is formed.
     const bool want_resid = last || P.tolerance > 0;
-    if (c->prec == VH_PREC_F64) enqueue_local<double>(c, c->st);
-    else enqueue_local<float>(c, c->st);
+    if (c->prec == VH_PREC_F64) enqueue_local<double>(c, stk);
+    else enqueue_local<float>(c, stk);
     Marks none;
-    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, c->st, none);
+    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, stk, none);
     if (want_resid)
       hipLaunchKernelGGL(vhk::k_resid_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, c->resid_blocks,
                          P.tolerance, it, c->st);
@@ -468,12 +492,16 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   if (lds > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "supernode front exceeds the 160 KB LDS of one workgroup");
   if (lds > 64 * 1024) {
     const bool f64 = ctx->prec == VH_PREC_F64;
-    const void* kf = f64 ? (const void*)vhk::k_fwd_band<double> : (const void*)vhk::k_fwd_band<float>;
-    const void* kb = f64 ? (const void*)vhk::k_bwd_band<double> : (const void*)vhk::k_bwd_band<float>;
-    const void* kr = f64 ? (const void*)vhk::k_root_band<double> : (const void*)vhk::k_root_band<float>;
-    VH_HIP(hipFuncSetAttribute(kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    VH_HIP(hipFuncSetAttribute(kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    VH_HIP(hipFuncSetAttribute(kr, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const void* ks[] = {
+        f64 ? (const void*)vhk::k_fwd_band<double, false> : (const void*)vhk::k_fwd_band<float, false>,
+        f64 ? (const void*)vhk::k_fwd_band<double, true> : (const void*)vhk::k_fwd_band<float, true>,
+        f64 ? (const void*)vhk::k_bwd_band<double, false> : (const void*)vhk::k_bwd_band<float, false>,
+        f64 ? (const void*)vhk::k_bwd_band<double, true> : (const void*)vhk::k_bwd_band<float, true>,
+        f64 ? (const void*)vhk::k_root_band<double, false, false> : (const void*)vhk::k_root_band<float, false, false>,
+        f64 ? (const void*)vhk::k_root_band<double, false, true> : (const void*)vhk::k_root_band<float, false, true>,
+        f64 ? (const void*)vhk::k_root_band<double, true, false> : (const void*)vhk::k_root_band<float, true, false>,
+        f64 ? (const void*)vhk::k_root_band<double, true, true> : (const void*)vhk::k_root_band<float, true, true>};
+    for (const void* k : ks) VH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   // Initial state (state.rs:119-139): pos = prev = mesh positions, vel = 0.
   std::vector<double> h3(n3);

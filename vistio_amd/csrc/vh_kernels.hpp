@@ -20,6 +20,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 namespace vhk {
 
@@ -353,23 +354,53 @@ __device__ __forceinline__ double3 rhs_from_slots(const RhsSrc<T>& s, int i) {
   return make_double3(in.x + (double)ax, in.y + (double)ay, in.z + (double)az);
 }
 
-template <class T>
-__device__ __forceinline__ T group_sum(T v, int g) {
-  for (int o = g >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, g);
-  return v;
+// Sum over the G lanes of a lane group (xor butterfly, every lane gets the
+// total); the three components are interleaved so their shuffles overlap.
+template <int G, class T>
+__device__ __forceinline__ void sum3(T& x, T& y, T& z) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) {
+    const T a = __shfl_xor(x, o, G), b = __shfl_xor(y, o, G), c = __shfl_xor(z, o, G);
+    x += a; y += b; z += c;
+  }
+}
+
+// Calls f(std::integral_constant<int, G>) for the runtime lane count g, one
+// of the powers of two in [lo, 64].
+template <int lo, class F>
+__device__ __forceinline__ void with_lanes(int g, F&& f) {
+  switch (g) {
+    case 1: if constexpr (lo <= 1) f(std::integral_constant<int, 1>{}); break;
+    case 2: if constexpr (lo <= 2) f(std::integral_constant<int, 2>{}); break;
+    case 4: if constexpr (lo <= 4) f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    case 32: f(std::integral_constant<int, 32>{}); break;
+    case 64: f(std::integral_constant<int, 64>{}); break;
+    default: break;
+  }
 }
 
 // The slot sum of vertex i split over the L lanes of a group (all lanes of
-// the group call it with the same i; lane 0 gets the total).
-template <class T>
-__device__ __forceinline__ double3 rhs_from_slots_group(const RhsSrc<T>& s, int i, int lane, int L) {
+// the group call it with the same i and get the total). Four slot loads per
+// lane are issued together (clamped addresses, masked sums).
+template <int L, class T>
+__device__ __forceinline__ double3 rhs_group(const RhsSrc<T>& s, int i, int lane) {
   T ax = 0, ay = 0, az = 0;
-  const int pe = s.inc_ptr[i + 1];
-  for (int p = s.inc_ptr[i] + lane; p < pe; p += L) {
-    const T* c = s.cslot + 3 * (size_t)p;
-    ax += c[0]; ay += c[1]; az += c[2];
+  const int p0 = s.inc_ptr[i], pe = s.inc_ptr[i + 1];
+  for (int p = p0 + lane; p < pe; p += 4 * L) {
+    T v[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = p + u * L;
+      const T* c = s.cslot + 3 * (size_t)(q < pe ? q : p);
+      v[u][0] = c[0]; v[u][1] = c[1]; v[u][2] = c[2];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (p + u * L < pe) { ax += v[u][0]; ay += v[u][1]; az += v[u][2]; }
   }
-  ax = group_sum(ax, L); ay = group_sum(ay, L); az = group_sum(az, L);
+  sum3<L>(ax, ay, az);
   const double3 in = inertia(s, i);
   return make_double3(in.x + (double)ax, in.y + (double)ay, in.z + (double)az);
 }
@@ -383,34 +414,49 @@ __global__ void __launch_bounds__(256) k_gather(int n, RhsSrc<T> s, double* __re
   r[3 * i + 0] = v.x; r[3 * i + 1] = v.y; r[3 * i + 2] = v.z;
 }
 
-// Dot product of one stored row (len entries, contiguous) with an LDS vector
-// of xyz triples, split over the g lanes of a group; 8 independent loads in
-// flight per lane, partial sums reduced across the group.
-template <class T>
-__device__ __forceinline__ void row_dot(const T* __restrict__ row, int len, const T* __restrict__ vec, int lane, int g,
-                                        T& sx, T& sy, T& sz) {
-  T ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-  int k = lane;
-  for (; k + 7 * g < len; k += 8 * g) {
-    T mv[8];
+// Row kernels of the solve: a stored row (len entries, contiguous) times an
+// LDS vector of xyz triples, split over the G lanes of a group; lane l owns
+// entries l, l + G, l + 2G, ...
+
+// Loads entries lane + i G (i < N) of a row into registers, zero past len.
+// Every lane loads (clamped address), so the N loads are issued together.
+template <int G, int N, class T>
+__device__ __forceinline__ void load_row(T (&mv)[N], const T* __restrict__ row, int len, int lane) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) mv[u] = __builtin_nontemporal_load(row + k + u * g);
+  for (int i = 0; i < N; ++i) {
+    const int k = lane + i * G;
+    const T v = __builtin_nontemporal_load(row + (k < len ? k : 0));
+    mv[i] = k < len ? v : T(0);
+  }
+}
+
+// acc += sum_i mv[i] vec[lane + i G] for a row of len >= 1 entries (the
+// index is clamped past len, where mv is zero).
+template <int G, int N, class T>
+__device__ __forceinline__ void fma_row(const T (&mv)[N], const T* __restrict__ vec, int len, int lane, T& ax, T& ay,
+                                        T& az) {
 #pragma unroll
-    for (int u = 0; u < 8; u += 2) {
-      const T* v0 = vec + 3 * (k + u * g);
-      const T* v1 = vec + 3 * (k + (u + 1) * g);
-      ax += mv[u] * v0[0]; ay += mv[u] * v0[1]; az += mv[u] * v0[2];
-      bx += mv[u + 1] * v1[0]; by += mv[u + 1] * v1[1]; bz += mv[u + 1] * v1[2];
-    }
+  for (int i = 0; i < N; ++i) {
+    const int k0 = lane + i * G;
+    const T* v = vec + 3 * (k0 < len ? k0 : len - 1);
+    ax += mv[i] * v[0]; ay += mv[i] * v[1]; az += mv[i] * v[2];
   }
-  for (; k < len; k += g) {
-    const T m0 = __builtin_nontemporal_load(row + k);
-    const T* v0 = vec + 3 * k;
-    ax += m0 * v0[0]; ay += m0 * v0[1]; az += m0 * v0[2];
+}
+
+// Dot product of a row (len >= 1) whose first N G entries are in mv; longer
+// rows continue 8 loads per lane at a time. Every lane gets the sums.
+template <int G, int N, class T>
+__device__ __forceinline__ void dot_row(const T (&mv)[N], const T* __restrict__ row, int len,
+                                        const T* __restrict__ vec, int lane, T& sx, T& sy, T& sz) {
+  T ax = 0, ay = 0, az = 0;
+  fma_row<G, N>(mv, vec, len, lane, ax, ay, az);
+  for (int base = N * G; base < len; base += 8 * G) {
+    T m2[8];
+    load_row<G, 8>(m2, row + base, len - base, lane);
+    fma_row<G, 8>(m2, vec + 3 * base, len - base, lane, ax, ay, az);
   }
-  sx = group_sum(ax + bx, g);
-  sy = group_sum(ay + by, g);
-  sz = group_sum(az + bz, g);
+  sum3<G>(ax, ay, az);
+  sx = ax; sy = ay; sz = az;
 }
 
 // Solve task descriptor: everything a task needs in one 48-byte record.
@@ -466,9 +512,8 @@ struct BandArgs {
   const int* uslot;
   const int* dpush_ptr;
   const int* dpush_idx;
-  // RHS: from the slots (first forward band of the first sweep, which also
-  // stores r) or from r
-  int rhs_from_slots;
+  // RHS sources: the slots (read by the first launch of a sweep that starts
+  // from them, template flag S, which also stores r) or r
   RhsSrc<T> src;
   double* r;
   // extra blocks after the tasks: RHS (forward) or commit (backward) of the
@@ -485,58 +530,25 @@ struct BandArgs {
 //   rows w + q : slot(U_P[q])      = sum_k M[w+q][k] t[k] + front[w + q]
 // so each update row lands in its slot of the parent's front. Only the front
 // entries the task's rows read are assembled.
-// Rows of a task are prefetched kPrefRows at a time per lane group, up to
-// kPrefLoads entries per lane each (rows longer than kPrefLoads * g finish
-// with plain loads).
+// Each lane group of G lanes takes rows b + grp, b + grp + ngrp, ...; the
+// first kPrefRows of them (kPrefLoads entries per lane each) and their
+// output slots are loaded before the front is assembled.
 constexpr int kPrefRows = 2;
 constexpr int kPrefLoads = 8;
-
-template <class T>
-__device__ __forceinline__ void prefetch_row(T (&mv)[kPrefLoads], const T* __restrict__ row, int len, int lane,
-                                             int g) {
-#pragma unroll
-  for (int i = 0; i < kPrefLoads; ++i) {
-    const int k = lane + i * g;
-    mv[i] = k < len ? __builtin_nontemporal_load(row + k) : T(0);
-  }
-}
-
-// row_dot with the first kPrefLoads * g entries already in registers.
-template <class T>
-__device__ __forceinline__ void row_dot_pref(const T (&mv)[kPrefLoads], const T* __restrict__ row, int len,
-                                             const T* __restrict__ vec, int lane, int g, T& sx, T& sy, T& sz) {
-  T ax = 0, ay = 0, az = 0, bx = 0, by = 0, bz = 0;
-#pragma unroll
-  for (int i = 0; i < kPrefLoads; i += 2) {
-    const int k0 = lane + i * g, k1 = k0 + g;
-    if (k0 < len) { const T* v0 = vec + 3 * k0; ax += mv[i] * v0[0]; ay += mv[i] * v0[1]; az += mv[i] * v0[2]; }
-    if (k1 < len) { const T* v1 = vec + 3 * k1; bx += mv[i + 1] * v1[0]; by += mv[i + 1] * v1[1]; bz += mv[i + 1] * v1[2]; }
-  }
-  for (int k = lane + kPrefLoads * g; k < len; k += g) {
-    const T m0 = __builtin_nontemporal_load(row + k);
-    const T* v0 = vec + 3 * k;
-    ax += m0 * v0[0]; ay += m0 * v0[1]; az += m0 * v0[2];
-  }
-  sx = group_sum(ax + bx, g);
-  sy = group_sum(ay + by, g);
-  sz = group_sum(az + bz, g);
-}
 
 __device__ __forceinline__ long long fwd_row_off(int row, int w, long long tri) {
   return row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w;
 }
 
+// Output of forward row `row`: y (rows < w) or the update slot us of the
+// parent's front (rows >= w); lane k < 3 stores component k.
 template <class T>
-__device__ __forceinline__ void fwd_store(const BandArgs<T>& a, int row, int w, int col0, int uoff, T sx, T sy, T sz,
-                                          const T* front, T* __restrict__ y, T* __restrict__ F) {
-  if (row < w) {
-    T* o = y + 3 * (size_t)(col0 + row);
-    o[0] = sx; o[1] = sy; o[2] = sz;
-  } else {
-    T* o = F + 3 * (size_t)a.uslot[uoff + row - w];
-    o[0] = sx + front[3 * row + 0];
-    o[1] = sy + front[3 * row + 1];
-    o[2] = sz + front[3 * row + 2];
+__device__ __forceinline__ void fwd_store(int row, int w, int col0, int us, T sx, T sy, T sz, const T* front,
+                                          T* __restrict__ y, T* __restrict__ F, int lane) {
+  if (lane < 3) {
+    const T s = lane == 0 ? sx : (lane == 1 ? sy : sz);
+    if (row < w) y[3 * (size_t)(col0 + row) + lane] = s;
+    else F[3 * (size_t)us + lane] = s + front[3 * row + lane];
   }
 }
 
@@ -549,47 +561,35 @@ __device__ __forceinline__ void front_sum(const T* __restrict__ f, int K, T& ax,
   for (int k = 3; k < K; ++k) { ax += f[3 * k]; ay += f[3 * k + 1]; az += f[3 * k + 2]; }
 }
 
-template <class T>
-__device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
-                         T* __restrict__ F, T* front) {
-  const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
-  const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
-  const int K = td.q2.y;
-  const T* Fp = F + 3 * (size_t)td.q2.x;
+// Front entries [0, n) of a task: r (or, in the first launch of a sweep
+// that starts from the RHS slots, the slot sums, also stored to r when
+// store_r) minus the K child slots of each position.
+template <bool S, class T>
+__device__ __forceinline__ void assemble_front(const BandArgs<T>& a, int col0, int n, const T* __restrict__ Fp, int K,
+                                               bool store_r, T* front) {
   const int tid = threadIdx.x;
-  const int tlen = e <= w ? e : w;
-  // Block rows first: their addresses depend only on the task, so these
-  // loads are in flight while the front is assembled.
-  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
-  const T* Mp = M + moff;
-  const long long tri = (long long)w * (w + 1) / 2;
-  T mv[kPrefRows][kPrefLoads];
-#pragma unroll
-  for (int p = 0; p < kPrefRows; ++p) {
-    const int row = b + grp + p * ngrp;
-    const int len = row < e ? (row < w ? row + 1 : w) : 0;
-    prefetch_row(mv[p], Mp + fwd_row_off(row < e ? row : b, w, tri), len, lane, g);
-  }
-  if (a.rhs_from_slots) {
-    // lane groups of L threads per column sum the column's RHS slots
+  if constexpr (S) {
+    // L lanes per column sum the column's RHS slots
     int L = 1;
-    while (L < 8 && (int)blockDim.x / (2 * L) >= tlen) L *= 2;
-    const int lane = tid & (L - 1), grp = tid / L, ngrp = blockDim.x / L;
-    const bool store_r = e >= w;
-    for (int j = grp; j < tlen; j += ngrp) {
-      const double3 rb = rhs_from_slots_group(a.src, col0 + j, lane, L);
-      if (lane == 0) {
-        if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z; }
-        T ax, ay, az;
-        front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
-        front[3 * j + 0] = (T)rb.x - ax;
-        front[3 * j + 1] = (T)rb.y - ay;
-        front[3 * j + 2] = (T)rb.z - az;
+    while (L < 8 && (int)blockDim.x / (2 * L) >= n) L *= 2;
+    with_lanes<1>(L, [&](auto LC) {
+      constexpr int Lc = decltype(LC)::value;
+      const int ln = tid & (Lc - 1), gp = tid / Lc, ng = blockDim.x / Lc;
+      for (int j = gp; j < n; j += ng) {
+        const double3 rb = rhs_group<Lc>(a.src, col0 + j, ln);
+        if (ln == 0) {
+          if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z; }
+          T ax, ay, az;
+          front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+          front[3 * j + 0] = (T)rb.x - ax;
+          front[3 * j + 1] = (T)rb.y - ay;
+          front[3 * j + 2] = (T)rb.z - az;
+        }
       }
-    }
+    });
   } else {
 #pragma unroll 4
-    for (int j = tid; j < tlen; j += blockDim.x) {
+    for (int j = tid; j < n; j += blockDim.x) {
       T ax, ay, az;
       front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
       const double* rb = a.r + 3 * (size_t)(col0 + j);
@@ -598,6 +598,32 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
       front[3 * j + 2] = (T)rb[2] - az;
     }
   }
+}
+
+template <int G, bool S, class T>
+__device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
+                         T* __restrict__ F, T* front) {
+  const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
+  const int K = td.q2.y;
+  const T* Fp = F + 3 * (size_t)td.q2.x;
+  const int tid = threadIdx.x;
+  const int tlen = e <= w ? e : w;
+  const int lane = tid & (G - 1), grp = tid / G, ngrp = blockDim.x / G;
+  const T* Mp = M + moff;
+  const long long tri = (long long)w * (w + 1) / 2;
+  // Block rows and output slots first: their addresses depend only on the
+  // task, so these loads are in flight while the front is assembled.
+  T mv[kPrefRows][kPrefLoads];
+  int us[kPrefRows];
+#pragma unroll
+  for (int p = 0; p < kPrefRows; ++p) {
+    const int row = b + grp + p * ngrp;
+    const int len = row < e ? (row < w ? row + 1 : w) : 0;
+    load_row<G, kPrefLoads>(mv[p], Mp + fwd_row_off(row < e ? row : b, w, tri), len, lane);
+    us[p] = row < e && row >= w ? a.uslot[uoff + row - w] : 0;
+  }
+  assemble_front<S>(a, col0, tlen, Fp, K, e >= w, front);
 #pragma unroll 4
   for (int j = (b > w ? b : w) + tid; j < e; j += blockDim.x) {
     T ax, ay, az;
@@ -605,19 +631,24 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
     front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
   }
   __syncthreads();
+#pragma unroll
   for (int p = 0; p < kPrefRows; ++p) {
     const int row = b + grp + p * ngrp;
-    if (row >= e) break;  // uniform within the group
-    const int len = row < w ? row + 1 : w;
-    T sx, sy, sz;
-    row_dot_pref(mv[p], Mp + fwd_row_off(row, w, tri), len, front, lane, g, sx, sy, sz);
-    if (lane == 0) fwd_store(a, row, w, col0, uoff, sx, sy, sz, front, y, F);
+    if (row < e) {  // uniform within the group
+      const int len = row < w ? row + 1 : w;
+      T sx, sy, sz;
+      dot_row<G, kPrefLoads>(mv[p], Mp + fwd_row_off(row, w, tri), len, front, lane, sx, sy, sz);
+      fwd_store(row, w, col0, us[p], sx, sy, sz, front, y, F, lane);
+    }
   }
   for (int row = b + grp + kPrefRows * ngrp; row < e; row += ngrp) {
     const int len = row < w ? row + 1 : w;
+    const int u = row >= w ? a.uslot[uoff + row - w] : 0;
+    T m1[kPrefLoads];
+    load_row<G, kPrefLoads>(m1, Mp + fwd_row_off(row, w, tri), len, lane);
     T sx, sy, sz;
-    row_dot(Mp + fwd_row_off(row, w, tri), len, front, lane, g, sx, sy, sz);
-    if (lane == 0) fwd_store(a, row, w, col0, uoff, sx, sy, sz, front, y, F);
+    dot_row<G, kPrefLoads>(m1, Mp + fwd_row_off(row, w, tri), len, front, lane, sx, sy, sz);
+    fwd_store(row, w, col0, u, sx, sy, sz, front, y, F, lane);
   }
 }
 
@@ -630,16 +661,51 @@ __device__ __forceinline__ void commit_col(const CommitArgs& c, int col, double 
   commit_vertex(c, col, dx, dy, dz, df, nm);
 }
 
-template <class T>
-__device__ __forceinline__ void bwd_store(const BandArgs<T>& a, const CommitArgs& c, int col, T sx, T sy, T sz,
-                                          T* __restrict__ V, T* __restrict__ d, double& df, double& nm) {
-  T* o = d + 3 * (size_t)col;
-  o[0] = sx; o[1] = sy; o[2] = sz;
-  for (int q = a.dpush_ptr[col]; q < a.dpush_ptr[col + 1]; ++q) {
-    T* p = V + 3 * (size_t)a.dpush_idx[q];
-    p[0] = -sx; p[1] = -sy; p[2] = -sz;
+// Output metadata of one solved column, loaded together with the block rows
+// so that the stores after the dot product wait on no further load: the
+// range of its pushes and, when committing (C), component `lane` (< 3) of
+// x, prev and acc.
+struct ColPre {
+  int q0, q1, pin;
+  double xo, po, ao;
+};
+
+template <bool C, class T>
+__device__ __forceinline__ void col_pre(const BandArgs<T>& a, const CommitArgs& c, int col, int lane, ColPre& p) {
+  p.q0 = a.dpush_ptr[col];
+  p.q1 = a.dpush_ptr[col + 1];
+  p.pin = 0; p.xo = 0; p.po = 0; p.ao = 0;
+  if (C && lane < 3) {
+    p.xo = c.x[3 * (size_t)col + lane];
+    p.po = c.prev[3 * (size_t)col + lane];
+    if (c.add_acc) p.ao = c.acc[3 * (size_t)col + lane];
+    p.pin = c.pinned[col];
   }
-  if (c.on) commit_col(c, col, (double)sx, (double)sy, (double)sz, df, nm);
+}
+
+// Stores of one solved column (every lane of the group holds sx, sy, sz):
+// d, the negated pushes to the descendants' v slots spread over the g lanes,
+// and, when committing, x += d (+ acc) with the pinned / ground rules of
+// commit_vertex, component k by lane k.
+template <bool C, class T>
+__device__ __forceinline__ void col_store(const BandArgs<T>& a, const CommitArgs& c, int col, T sx, T sy, T sz,
+                                          const ColPre& p, int lane, int g, T* __restrict__ V, T* __restrict__ d,
+                                          double& df, double& nm) {
+  const T s = lane == 0 ? sx : (lane == 1 ? sy : sz);
+  if (lane < 3) d[3 * (size_t)col + lane] = s;
+  for (int q = p.q0 + lane; q < p.q1; q += g) {
+    T* o = V + 3 * (size_t)a.dpush_idx[q];
+    o[0] = -sx; o[1] = -sy; o[2] = -sz;
+  }
+  if (C && lane < 3) {
+    const double dv = (double)s + p.ao;
+    df += dv * dv;
+    nm += p.xo * p.xo;
+    double v = p.xo + dv;
+    if (p.pin) v = p.po;
+    else if (lane == 1 && c.has_ground && v < c.ground) v = c.ground;
+    c.x[3 * (size_t)col + lane] = v;
+  }
 }
 
 // Backward task (L^T d = y): columns [b, e) of P, read as rows of the
@@ -650,21 +716,24 @@ __device__ __forceinline__ void bwd_store(const BandArgs<T>& a, const CommitArgs
 //          = sum_i Mt[j][i] v[j + i].
 // Each d_P[j] is stored, pushed (negated) to the v slots of the descendants
 // that read it, and, when committing, added to x.
-template <class T>
+template <int G, bool C, class T>
 __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
                          const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, double& df, double& nm) {
   const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
-  const int b = td.q1.x, e = td.q1.y, g = td.q1.z, uoff = td.q1.w;
+  const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int m = w + r;
   const int tid = threadIdx.x;
-  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  const int lane = tid & (G - 1), grp = tid / G, ngrp = blockDim.x / G;
   const T* Tp = Mt + moff;
+  auto row_ptr = [&](int j) { return Tp + (long long)j * m - (long long)j * (j - 1) / 2; };
   T mv[kPrefRows][kPrefLoads];
+  ColPre cp[kPrefRows];
 #pragma unroll
   for (int p = 0; p < kPrefRows; ++p) {
     const int j = b + grp + p * ngrp;
     const int jj = j < e ? j : b;
-    prefetch_row(mv[p], Tp + (long long)jj * m - (long long)jj * (jj - 1) / 2, j < e ? m - j : 0, lane, g);
+    load_row<G, kPrefLoads>(mv[p], row_ptr(jj), j < e ? m - j : 0, lane);
+    col_pre<C>(a, c, col0 + jj, lane, cp[p]);
   }
 #pragma unroll 4
   for (int k = b + tid; k < m; k += blockDim.x) {
@@ -672,38 +741,48 @@ __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArg
     v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
   }
   __syncthreads();
+#pragma unroll
   for (int p = 0; p < kPrefRows; ++p) {
     const int j = b + grp + p * ngrp;
-    if (j >= e) break;  // uniform within the group
-    T sx, sy, sz;
-    row_dot_pref(mv[p], Tp + (long long)j * m - (long long)j * (j - 1) / 2, m - j, v + 3 * j, lane, g, sx, sy, sz);
-    if (lane == 0) bwd_store(a, c, col0 + j, sx, sy, sz, V, d, df, nm);
+    if (j < e) {  // uniform within the group
+      T sx, sy, sz;
+      dot_row<G, kPrefLoads>(mv[p], row_ptr(j), m - j, v + 3 * j, lane, sx, sy, sz);
+      col_store<C>(a, c, col0 + j, sx, sy, sz, cp[p], lane, G, V, d, df, nm);
+    }
   }
   for (int j = b + grp + kPrefRows * ngrp; j < e; j += ngrp) {
+    ColPre q;
+    col_pre<C>(a, c, col0 + j, lane, q);
+    T m1[kPrefLoads];
+    load_row<G, kPrefLoads>(m1, row_ptr(j), m - j, lane);
     T sx, sy, sz;
-    row_dot(Tp + (long long)j * m - (long long)j * (j - 1) / 2, m - j, v + 3 * j, lane, g, sx, sy, sz);
-    if (lane == 0) bwd_store(a, c, col0 + j, sx, sy, sz, V, d, df, nm);
+    dot_row<G, kPrefLoads>(m1, row_ptr(j), m - j, v + 3 * j, lane, sx, sy, sz);
+    col_store<C>(a, c, col0 + j, sx, sy, sz, q, lane, G, V, d, df, nm);
   }
 }
 
 // One band of the forward sweep per launch. Blocks [0, ntasks) run tasks;
 // in the first band of a sweep that starts from the slots, the remaining
 // blocks compute (and store) r for the columns outside the deepest band.
-template <class T>
+template <class T, bool S>
 __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
                                                   T* __restrict__ F, const StepDev* st) {
-  if (st && st->done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   if ((int)blockIdx.x < a.ntasks) {
-    fwd_task<T>(a.tasks[blockIdx.x], a, M, y, F, reinterpret_cast<T*>(smem_raw));
+    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
+    if (st && st->done) return;
+    with_lanes<4>(td.q1.z, [&](auto GC) {
+      fwd_task<decltype(GC)::value, S, T>(td, a, M, y, F, reinterpret_cast<T*>(smem_raw));
+    });
     return;
   }
+  if (!S || (st && st->done)) return;
   // 4 lanes per column: blockDim/4 columns per extra block
   const int k = ((int)blockIdx.x - a.ntasks) * (blockDim.x / 4) + (int)threadIdx.x / 4;
   const int lane = threadIdx.x & 3;
   if (k >= a.nupper) return;  // uniform within each 4-lane group
   const int col = a.upper_cols[k];
-  const double3 rb = rhs_from_slots_group(a.src, col, lane, 4);
+  const double3 rb = rhs_group<4>(a.src, col, lane);
   if (lane == 0) {
     double* o = a.r + 3 * (size_t)col;
     o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
@@ -713,24 +792,28 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __rest
 // One band of the backward sweep per launch. In the last band of an FP64
 // solve the tasks also commit their columns and the extra blocks commit the
 // columns outside the deepest band (solved by earlier launches).
-template <class T>
+template <class T, bool C>
 __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Mt,
                                                   const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
                                                   const StepDev* st) {
-  if (st && st->done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double df = 0, nm = 0;
   if ((int)blockIdx.x < a.ntasks) {
-    bwd_task<T>(a.tasks[blockIdx.x], a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
+    if (st && st->done) return;
+    with_lanes<4>(td.q1.z, [&](auto GC) {
+      bwd_task<decltype(GC)::value, C, T>(td, a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+    });
   } else {
+    if (st && st->done) return;
     const int k = ((int)blockIdx.x - a.ntasks) * blockDim.x + threadIdx.x;
-    if (c.on && k < a.nupper) {
+    if (C && k < a.nupper) {
       const int col = a.upper_cols[k];
       const T* s = d + 3 * (size_t)col;
       commit_col(c, col, (double)s[0], (double)s[1], (double)s[2], df, nm);
     }
   }
-  if (c.on && c.want_resid) {
+  if (C && c.want_resid) {
     __syncthreads();
     block_partials(df, nm, c.partials);
   }
@@ -740,68 +823,60 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
 // and backward sweeps collapse into d_R = (L_RR L_RR^T)^{-1} t_R with the
 // dense symmetric inverse Sroot (row-major w x w); rows [b, e) of it. Same
 // front assembly as a forward task, same stores as a backward task.
-template <class T>
+template <int G, bool S, bool C, class T>
 __device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot,
                           T* __restrict__ F, T* __restrict__ V, T* __restrict__ d, T* front, double& df, double& nm) {
   const int col0 = td.q0.x, w = td.q0.y;
-  const int b = td.q1.x, e = td.q1.y, g = td.q1.z;
+  const int b = td.q1.x, e = td.q1.y;
   const int K = td.q2.y;
   const T* Fp = F + 3 * (size_t)td.q2.x;
   const int tid = threadIdx.x;
-  const int lane = tid & (g - 1), grp = tid / g, ngrp = blockDim.x / g;
+  const int lane = tid & (G - 1), grp = tid / G, ngrp = blockDim.x / G;
   T mv[kPrefRows][kPrefLoads];
+  ColPre cp[kPrefRows];
 #pragma unroll
   for (int p = 0; p < kPrefRows; ++p) {
     const int row = b + grp + p * ngrp;
-    prefetch_row(mv[p], Sroot + (size_t)(row < e ? row : b) * w, row < e ? w : 0, lane, g);
+    const int rr = row < e ? row : b;
+    load_row<G, kPrefLoads>(mv[p], Sroot + (size_t)rr * w, row < e ? w : 0, lane);
+    col_pre<C>(a, c, col0 + rr, lane, cp[p]);
   }
-  if (a.rhs_from_slots) {
-    int L = 1;
-    while (L < 8 && (int)blockDim.x / (2 * L) >= w) L *= 2;
-    const int ln = tid & (L - 1), gp = tid / L, ng = blockDim.x / L;
-    for (int j = gp; j < w; j += ng) {
-      const double3 rb = rhs_from_slots_group(a.src, col0 + j, ln, L);
-      if (ln == 0) {
-        double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
-        T ax, ay, az;
-        front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
-        front[3 * j + 0] = (T)rb.x - ax; front[3 * j + 1] = (T)rb.y - ay; front[3 * j + 2] = (T)rb.z - az;
-      }
-    }
-  } else {
-#pragma unroll 4
-    for (int j = tid; j < w; j += blockDim.x) {
-      T ax, ay, az;
-      front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
-      const double* rb = a.r + 3 * (size_t)(col0 + j);
-      front[3 * j + 0] = (T)rb[0] - ax; front[3 * j + 1] = (T)rb[1] - ay; front[3 * j + 2] = (T)rb[2] - az;
-    }
-  }
+  assemble_front<S>(a, col0, w, Fp, K, true, front);
   __syncthreads();
+#pragma unroll
   for (int p = 0; p < kPrefRows; ++p) {
     const int row = b + grp + p * ngrp;
-    if (row >= e) break;  // uniform within the group
-    T sx, sy, sz;
-    row_dot_pref(mv[p], Sroot + (size_t)row * w, w, front, lane, g, sx, sy, sz);
-    if (lane == 0) bwd_store(a, c, col0 + row, sx, sy, sz, V, d, df, nm);
+    if (row < e) {  // uniform within the group
+      T sx, sy, sz;
+      dot_row<G, kPrefLoads>(mv[p], Sroot + (size_t)row * w, w, front, lane, sx, sy, sz);
+      col_store<C>(a, c, col0 + row, sx, sy, sz, cp[p], lane, G, V, d, df, nm);
+    }
   }
   for (int row = b + grp + kPrefRows * ngrp; row < e; row += ngrp) {
+    ColPre q;
+    col_pre<C>(a, c, col0 + row, lane, q);
+    T m1[kPrefLoads];
+    load_row<G, kPrefLoads>(m1, Sroot + (size_t)row * w, w, lane);
     T sx, sy, sz;
-    row_dot(Sroot + (size_t)row * w, w, front, lane, g, sx, sy, sz);
-    if (lane == 0) bwd_store(a, c, col0 + row, sx, sy, sz, V, d, df, nm);
+    dot_row<G, kPrefLoads>(m1, Sroot + (size_t)row * w, w, front, lane, sx, sy, sz);
+    col_store<C>(a, c, col0 + row, sx, sy, sz, q, lane, G, V, d, df, nm);
   }
 }
 
-template <class T>
+template <class T, bool S, bool C>
 __global__ void __launch_bounds__(256) k_root_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Sroot,
                                                    T* __restrict__ F, T* __restrict__ V, T* __restrict__ d,
                                                    const StepDev* st) {
-  if (st && st->done) return;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double df = 0, nm = 0;
-  if ((int)blockIdx.x < a.ntasks)
-    root_task<T>(a.tasks[blockIdx.x], a, c, Sroot, F, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
-  if (c.on && c.want_resid) {
+  if ((int)blockIdx.x < a.ntasks) {
+    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
+    if (st && st->done) return;
+    with_lanes<4>(td.q1.z, [&](auto GC) {
+      root_task<decltype(GC)::value, S, C, T>(td, a, c, Sroot, F, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+    });
+  }
+  if (C && c.want_resid) {
     __syncthreads();
     block_partials(df, nm, c.partials);
   }
