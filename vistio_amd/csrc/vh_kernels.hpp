@@ -531,9 +531,11 @@ struct BandArgs {
 // so each update row lands in its slot of the parent's front. Only the front
 // entries the task's rows read are assembled.
 // Each lane group of G lanes takes rows b + grp, b + grp + ngrp, ...; the
-// first kPrefRows of them (kPrefLoads entries per lane each) and their
-// output slots are loaded before the front is assembled.
-constexpr int kPrefRows = 2;
+// first pref_rows<T>() of them (kPrefLoads entries per lane each) and
+// their output slots are loaded before the front is assembled. Two rows:
+// four cost more in occupancy than they save in front re-reads.
+template <class T>
+constexpr int pref_rows() { return 2; }
 constexpr int kPrefLoads = 8;
 
 __device__ __forceinline__ long long fwd_row_off(int row, int w, long long tri) {
@@ -603,6 +605,7 @@ __device__ __forceinline__ void assemble_front(const BandArgs<T>& a, int col0, i
 template <int G, bool S, class T>
 __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
                          T* __restrict__ F, T* front) {
+  constexpr int kPrefRows = pref_rows<T>();
   const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int K = td.q2.y;
@@ -719,6 +722,7 @@ __device__ __forceinline__ void col_store(const BandArgs<T>& a, const CommitArgs
 template <int G, bool C, class T>
 __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
                          const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, double& df, double& nm) {
+  constexpr int kPrefRows = pref_rows<T>();
   const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int m = w + r;
@@ -826,6 +830,7 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
 template <int G, bool S, bool C, class T>
 __device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot,
                           T* __restrict__ F, T* __restrict__ V, T* __restrict__ d, T* front, double& df, double& nm) {
+  constexpr int kPrefRows = pref_rows<T>();
   const int col0 = td.q0.x, w = td.q0.y;
   const int b = td.q1.x, e = td.q1.y;
   const int K = td.q2.y;

@@ -646,16 +646,17 @@ bool build_setup(const SetupInput& in, Setup& S) {
 
   // Task lists. A task is a run of rows of one stored form; g lanes share a
   // row (g ~ row length / 8, 4..64, so a row is usually 8 loads per lane)
-  // and each lane group takes two rows, whose loads are issued together
-  // (kernels' kPrefRows x kPrefLoads). Rows longer than 8g take one row per
-  // group. task_entries caps short-row tasks.
+  // and each lane group takes up to pr = 2 rows, whose loads are issued
+  // together before the task's front is assembled (the kernels'
+  // pref_rows). A task holds about task_entries block entries: at least one
+  // row per group, at most pr.
   const int kChunk = std::max(256, in.task_entries);
+  const int pr = 2;
   auto pow2ceil = [](int v) { int p = 1; while (p < v) p <<= 1; return p; };
   auto lanes_for = [&](int len) { return std::min(64, std::max(4, pow2ceil((std::max(1, len) + 7) / 8))); };
   auto rows_for = [&](int len, int g) {
     const int ngrp = 256 / g;
-    if (len > 8 * g) return ngrp;
-    return std::max(ngrp, std::min(2 * ngrp, kChunk / std::max(1, len)));
+    return std::max(ngrp, std::min(pr * ngrp, kChunk / std::max(1, len)));
   };
   S.fwd_tasks.clear(); S.bwd_tasks.clear();
   S.fwd_level_off.assign(S.max_depth + 2, 0);
