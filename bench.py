@@ -87,7 +87,7 @@ def run_path(mesh, topo, pinned, precision, steps, warmup, dt, ir_sweeps=1, ir_f
                 final_residual=r.final_residual)
 
 
-def roofline_of(prof):
+def roofline_of(prof, traffic=None):
     """Dominant kernels: the band launches of the supernodal solve (forward
     bands, the root band, backward bands; 2 * bands - 1 launches a sweep)."""
     us = sum(prof["fwd_us"]) + sum(prof["bwd_us"])
@@ -95,10 +95,37 @@ def roofline_of(prof):
     launches = prof["sweeps"] * (2 * prof["bands"] - 1)
     achieved = by / (us * 1e-6) / 1e9
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_fwd_band/k_root_band/k_bwd_band (supernodal sweeps)",
                 launch_avg_us=round(us / launches, 3), bytes_per_launch=round(by / launches),
                 launches_per_iteration=prof["launches"])
+
+
+def copy_probe(mb):
+    """A kernel with known HBM traffic (reads mb MB, writes mb MB), run three
+    times, so that counter totals can be scaled to bytes."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.add(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    del a, b
+
+
+def traffic_from_profiles(prec):
+    """HBM bytes per band launch measured with rocprofv3 FETCH_SIZE and
+    WRITE_SIZE (profiles/band_kernels_256_traffic.md), or None."""
+    path = os.path.join(ROOT, "profiles", "band_kernels_256_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch_{prec}:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(cells, sample_steps, dt):
@@ -130,6 +157,9 @@ def main():
     ap.add_argument("--ir-final-sweeps", type=int, default=2)
     ap.add_argument("--cpu-sample-steps", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first run a plain copy-like kernel (b = a + 1, float32) over this many MB, "
+                         "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -137,6 +167,8 @@ def main():
               f"--gpus {args.gpus} (WORLD_SIZE {world}) is not supported", file=sys.stderr)
         sys.exit(2)
     dt = 1.0 / 60.0
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
     mesh, topo, pinned = make_sheet(args.cells)
     f64 = run_path(mesh, topo, pinned, "f64", args.steps, args.warmup, dt)
     f32 = run_path(mesh, topo, pinned, "f32_ir", args.steps, args.warmup, dt, args.ir_sweeps, args.ir_final_sweeps)
@@ -165,13 +197,13 @@ def main():
             "cells": args.cells,
             "pd_iterations_per_step": 10,
         },
-        "roofline": roofline_of(f32["prof"]),
+        "roofline": roofline_of(f32["prof"], traffic_from_profiles("fp32") if args.cells == 256 else None),
         "extra": {
             "pd_iterations_per_s": round(10 * value, 1),
             "max_vertex_err_vs_fp64_gpu_m": err,
             "fp64_gpu_steps_per_s": round(v64, 3),
             "fp32_ir_over_fp64_gpu": round(value / v64, 3),
-            "fp64_roofline": roofline_of(f64["prof"]),
+            "fp64_roofline": roofline_of(f64["prof"], traffic_from_profiles("fp64") if args.cells == 256 else None),
             "local_step": {
                 "fp32_us": round(f32["prof"]["local_us"], 3),
                 "fp32_gbs": round(f32["prof"]["local_bytes"] / (f32["prof"]["local_us"] * 1e-6) / 1e9, 1),
