@@ -40,6 +40,8 @@
 #include <string>
 #include <vector>
 
+#include "contact_ref.hpp"
+
 namespace {
 
 // ---------------------------------------------------------------------------
@@ -435,6 +437,41 @@ struct or_params {
 
 struct or_step_result { int iterations; double final_residual; int converged; double wall_time; };
 
+// Contact settings of ProjectiveDynamicsSolver::step_with_ipc
+// (crates/vistio-solver/src/config.rs:47-120) and of the collision pipeline
+// the runner builds (crates/vistio-bench/src/runner.rs:60-95).
+struct or_ipc_params {
+  int has_ground;          // ground plane y = ground_height (also the state's ground)
+  double ground_height;
+  int has_sphere;          // analytic sphere collider
+  double sphere_center[3];
+  double sphere_radius;
+  int self_collision;      // triangle-pair VT/EE barrier + VT CCD
+  int exclusion_depth;     // n-ring exclusion (runner: 1)
+  double d_hat;            // barrier_d_hat (squared distance)
+  double kappa;            // barrier stiffness (runner: estimate_initial_kappa)
+  double thickness;        // broad-phase margin and self-contact thickness (runner: 1.5 d_hat)
+  int al_max_iterations;
+  double al_mu_initial, al_mu_growth, al_tolerance;
+  int compliant_contact;
+  double compliant_d_hat_scale;
+  double friction_coefficient, contact_damping;
+  int adaptive_contact_damping;
+  double contact_damping_max, contact_velocity_threshold;
+};
+
+// Diagnostics of one IPC step (no counterpart in the reference's StepResult).
+struct or_ipc_stats {
+  int al_iterations;        // outer iterations run
+  int al_converged;
+  double mu;                // penalty after the step
+  double max_violation;     // of the last detection
+  long long active;         // active contacts of the last detection
+  long long self_contacts;
+  long long pairs;          // candidate triangle pairs of the last detection
+  double last_alpha;        // step fraction of the last PD iteration
+};
+
 }  // extern "C"
 
 namespace {
@@ -454,6 +491,9 @@ struct OracleBase {
   virtual void local_rhs(double* rhs_out, double* tri_targets, double* hinge_targets) = 0;
   virtual void elements(double* dm, double* area, double* weight) const = 0;
   virtual void hinges(double* rest, double* weight) const = 0;
+  virtual int step_ipc(double dt, const or_ipc_params& ip, or_step_result* res, or_ipc_stats* st) = 0;
+  virtual void get_lambda(double* out) const = 0;
+  virtual void set_lambda(const double* in) = 0;
 };
 
 template <class R> struct Oracle : OracleBase {
@@ -703,6 +743,212 @@ template <class R> struct Oracle : OracleBase {
     return 0;
   }
 
+  // ---- IPC contact step --------------------------------------------------
+  std::vector<R> lx, ly, lz;          // SimulationState.al_lambda (state.rs:55-63)
+  std::vector<uint32_t> tri_flat;
+  std::vector<int> xptr, xidx;        // exclusion sets
+  int xdepth = -1;
+
+  void ensure_contact_data(int depth) {
+    if (lx.size() != size_t(n)) { lx.assign(n, 0); ly.assign(n, 0); lz.assign(n, 0); }
+    if (tri_flat.empty()) {
+      tri_flat.resize(3 * size_t(t));
+      for (int e = 0; e < t; ++e)
+        for (int k = 0; k < 3; ++k) tri_flat[3 * e + k] = uint32_t(tris[e].i[k]);
+    }
+    if (xdepth != depth) { oc::exclusion_sets(n, t, tri_flat.data(), depth, xptr, xidx); xdepth = depth; }
+  }
+
+  // Broad phase of SelfCollisionSystem::detect_ipc_contacts: every pair of
+  // f32 triangle boxes (margin = thickness) that overlap.
+  std::vector<std::pair<int, int>> pairs_at(const std::vector<R>& X, const std::vector<R>& Y,
+                                            const std::vector<R>& Z, double margin) const {
+    std::vector<float> fx(X.begin(), X.end()), fy(Y.begin(), Y.end()), fz(Z.begin(), Z.end());
+    std::vector<oc::Box32> box(t);
+    for (int e = 0; e < t; ++e) box[e] = oc::tri_box(fx.data(), fy.data(), fz.data(), tris[e].i, float(margin));
+    return oc::tri_pairs(box);
+  }
+
+  // ProjectiveDynamicsSolver::step_with_ipc — pd_solver.rs:549-1157, with
+  // CollisionPipeline as the handler (runner.rs:17-34).
+  int step_ipc(double dt_d, const or_ipc_params& ip, or_step_result* res, or_ipc_stats* st) override {
+    const auto t0 = std::chrono::steady_clock::now();
+    ensure_contact_data(ip.exclusion_depth);
+    const R dt = R(dt_d);
+    oc::IpcScene sc;
+    sc.has_ground = ip.has_ground != 0; sc.ground = ip.ground_height;
+    sc.has_sphere = ip.has_sphere != 0;
+    for (int k = 0; k < 3; ++k) sc.center[k] = ip.sphere_center[k];
+    sc.radius = ip.sphere_radius;
+    sc.self_collision = ip.self_collision != 0;
+    sc.thickness = ip.thickness;
+    const bool gnd = sc.has_ground;  // runner.rs:118-122 sets state.ground_height
+    const R gh = R(ip.ground_height);
+    qx = px; qy = py; qz = pz;  // save_previous
+    const R dh = R(ip.compliant_contact ? ip.d_hat * ip.compliant_d_hat_scale : ip.d_hat);
+    const R kappa = R(ip.kappa);
+    {  // CFL-like velocity clamp, pd_solver.rs:572-595
+      const R max_vel = R(0.5) * std::sqrt(dh) / dt;
+      for (int i = 0; i < n; ++i) {
+        if (inv_mass[i] == R(0)) continue;
+        const R v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
+        if (v2 > max_vel * max_vel) {
+          const R sc2 = max_vel / std::sqrt(v2);
+          vx[i] *= sc2; vy[i] *= sc2; vz[i] *= sc2;
+        }
+      }
+    }
+    const R dt2 = dt * dt;  // predict, state.rs:145-169
+    for (int i = 0; i < n; ++i) {
+      if (inv_mass[i] == R(0)) { rx[i] = px[i]; ry[i] = py[i]; rz[i] = pz[i]; continue; }
+      rx[i] = px[i] + dt * vx[i] + dt2 * grav[0];
+      ry[i] = py[i] + dt * vy[i] + dt2 * grav[1];
+      rz[i] = pz[i] + dt * vz[i] + dt2 * grav[2];
+      if (gnd && ry[i] < gh) ry[i] = gh;
+    }
+    px = rx; py = ry; pz = rz;
+    R mu = R(ip.al_mu_initial);
+    int total_iterations = 0;
+    double final_residual = std::numeric_limits<double>::max();
+    bool al_converged = false;
+    std::vector<uint8_t> ever(n, 0);
+    std::vector<R> fnx(n, 0), fny(n, 0), fnz(n, 0);
+    oc::Forces<R> bf, uf;
+    uf.reset(n);
+    std::vector<std::pair<int, int>> pairs;
+    std::vector<R> rhs[3];
+    std::vector<double> bd(n), xd(n), work(n);
+    std::vector<R> sol[3] = {std::vector<R>(n), std::vector<R>(n), std::vector<R>(n)};
+    int al_done = 0;
+    R alpha = 1;
+    auto detect_at = [&](oc::Forces<R>& F) {
+      if (sc.self_collision) pairs = pairs_at(px, py, pz, ip.thickness);
+      else pairs.clear();
+      oc::detect<R>(n, px.data(), py.data(), pz.data(), t, tri_flat.data(), pairs, xptr, xidx, sc, dh, kappa, F);
+      for (int i = 0; i < n; ++i)
+        if (F.in_contact[i]) { ever[i] = 1; fnx[i] = F.nx[i]; fny[i] = F.ny[i]; fnz[i] = F.nz[i]; }
+    };
+    for (int al = 0; al < ip.al_max_iterations; ++al) {
+      al_done = al + 1;
+      detect_at(bf);
+      for (int i = 0; i < n; ++i)
+        if (!bf.in_contact[i]) { lx[i] = 0; ly[i] = 0; lz[i] = 0; }
+      for (int iter = 0; iter < P.max_iterations; ++iter) {
+        local_and_rhs(dt, rhs, nullptr, nullptr);
+        const std::vector<R>* G[3] = {&bf.gx, &bf.gy, &bf.gz};
+        const std::vector<R>* L[3] = {&lx, &ly, &lz};
+        for (int ax = 0; ax < 3; ++ax)  // assemble_barrier_rhs, assembly.rs:272-282
+          for (int i = 0; i < n; ++i) rhs[ax][i] -= mu * (*G[ax])[i] + (*L[ax])[i];
+        for (int ax = 0; ax < 3; ++ax) {
+          for (int i = 0; i < n; ++i) bd[i] = double(rhs[ax][i]);
+          chol.solve(bd.data(), xd.data(), work);
+          for (int i = 0; i < n; ++i) sol[ax][i] = R(xd[i]);
+        }
+        double diff = 0, norm = 0;  // pd_solver.rs:873-891
+        for (int i = 0; i < n; ++i) {
+          const double dx = double(sol[0][i] - px[i]), dy = double(sol[1][i] - py[i]), dz = double(sol[2][i] - pz[i]);
+          diff += dx * dx + dy * dy + dz * dz;
+          norm += double(px[i]) * double(px[i]) + double(py[i]) * double(py[i]) + double(pz[i]) * double(pz[i]);
+        }
+        final_residual = norm > 1e-12 ? std::sqrt(diff / norm) : std::sqrt(diff);
+        // CCD-limited step with the pairs of the last detection, pd_solver.rs:897-906
+        R max_alpha = oc::ccd_step<R>(n, px.data(), py.data(), pz.data(), sol[0].data(), sol[1].data(), sol[2].data(),
+                                      tri_flat.data(), pairs, sc);
+        if (max_alpha < R(1)) max_alpha *= R(0.8);
+        max_alpha = std::min(max_alpha, R(1));
+        alpha = max_alpha;  // backtracking, pd_solver.rs:909-930
+        double gdd = 0;
+        for (int i = 0; i < n; ++i)
+          if (inv_mass[i] > R(0))
+            gdd += double(bf.gx[i] * (sol[0][i] - px[i]) + bf.gy[i] * (sol[1][i] - py[i]) + bf.gz[i] * (sol[2][i] - pz[i]));
+        for (int bt = 0; bt < 4; ++bt) {
+          if (alpha < R(0.1)) break;
+          if (gdd * double(mu) > 1e-4 * diff * 100.0) alpha *= R(0.5);
+          else break;
+        }
+        for (int i = 0; i < n; ++i)  // pd_solver.rs:934-950
+          if (inv_mass[i] > R(0)) {
+            px[i] += alpha * (sol[0][i] - px[i]);
+            py[i] += alpha * (sol[1][i] - py[i]);
+            pz[i] += alpha * (sol[2][i] - pz[i]);
+          }
+        total_iterations++;
+        if (final_residual < P.tolerance) break;
+      }
+      detect_at(uf);  // pd_solver.rs:966-978
+      if (uf.max_violation < R(ip.al_tolerance)) { al_converged = true; break; }
+      for (int i = 0; i < n; ++i)  // pd_solver.rs:1006-1015
+        if (uf.in_contact[i]) { lx[i] += mu * uf.gx[i]; ly[i] += mu * uf.gy[i]; lz[i] += mu * uf.gz[i]; }
+      mu *= R(ip.al_mu_growth);
+    }
+    const R inv_dt = R(1) / dt;  // update_velocities
+    for (int i = 0; i < n; ++i) {
+      vx[i] = (px[i] - qx[i]) * inv_dt; vy[i] = (py[i] - qy[i]) * inv_dt; vz[i] = (pz[i] - qz[i]) * inv_dt;
+    }
+    const R muf = R(ip.friction_coefficient), cd = R(ip.contact_damping);
+    for (int i = 0; i < n; ++i) {  // velocity filter, pd_solver.rs:1036-1111
+      if (inv_mass[i] == R(0) || !ever[i]) continue;
+      const R nx = fnx[i], ny = fny[i], nz = fnz[i];
+      if (nx * nx + ny * ny + nz * nz < R(0.5)) continue;
+      const R vn = vx[i] * nx + vy[i] * ny + vz[i] * nz;
+      vx[i] -= vn * nx; vy[i] -= vn * ny; vz[i] -= vn * nz;
+      const R vnm = std::fabs(vn);
+      const R tx = vx[i], ty = vy[i], tz = vz[i];
+      const R vt = std::sqrt(tx * tx + ty * ty + tz * tz);
+      if (vt > R(1e-8)) {
+        const R fi = muf * vnm;
+        if (fi >= vt) { vx[i] = 0; vy[i] = 0; vz[i] = 0; }
+        else { const R s2 = R(1) - fi / vt; vx[i] = tx * s2; vy[i] = ty * s2; vz[i] = tz * s2; }
+      }
+      R damp_i = cd;
+      if (ip.adaptive_contact_damping) {
+        const R v = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+        const R tt = std::min(std::max(R(1) - v / std::max(R(ip.contact_velocity_threshold), R(1e-6)), R(0)), R(1));
+        damp_i = cd + tt * (R(ip.contact_damping_max) - cd);
+      }
+      vx[i] *= R(1) - damp_i; vy[i] *= R(1) - damp_i; vz[i] *= R(1) - damp_i;
+    }
+    if (uf.active > 0) {  // pd_solver.rs:1115-1117
+      const R f = R(1) - R(0.05);
+      for (int i = 0; i < n; ++i) { vx[i] *= f; vy[i] *= f; vz[i] *= f; }
+    }
+    if (gnd)  // enforce_ground_velocities, state.rs:195-213
+      for (int i = 0; i < n; ++i)
+        if (inv_mass[i] > R(0) && py[i] <= gh) {
+          if (vy[i] < R(0)) vy[i] = R(0);
+          qy[i] = gh;
+          vx[i] *= R(0.5); vz[i] *= R(0.5);
+        }
+    const R fdamp = R(1) - damp;
+    for (int i = 0; i < n; ++i) { vx[i] *= fdamp; vy[i] *= fdamp; vz[i] *= fdamp; }
+    if (rayl > R(0)) {
+      const R f = R(1) / (R(1) + rayl * dt);
+      for (int i = 0; i < n; ++i)
+        if (inv_mass[i] > R(0)) { vx[i] *= f; vy[i] *= f; vz[i] *= f; }
+    }
+    res->iterations = total_iterations;
+    res->final_residual = final_residual;
+    res->converged = (al_converged || final_residual < P.tolerance) ? 1 : 0;
+    res->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (st) {
+      st->al_iterations = al_done; st->al_converged = al_converged ? 1 : 0; st->mu = double(mu);
+      st->max_violation = double(uf.max_violation); st->active = uf.active; st->self_contacts = uf.self_contacts;
+      st->pairs = (long long)pairs.size(); st->last_alpha = double(alpha);
+    }
+    return 0;
+  }
+  void get_lambda(double* out) const override {
+    for (int i = 0; i < n; ++i) {
+      out[i] = lx.empty() ? 0.0 : double(lx[i]);
+      out[n + i] = ly.empty() ? 0.0 : double(ly[i]);
+      out[2 * size_t(n) + i] = lz.empty() ? 0.0 : double(lz[i]);
+    }
+  }
+  void set_lambda(const double* in) override {
+    lx.assign(n, 0); ly.assign(n, 0); lz.assign(n, 0);
+    for (int i = 0; i < n; ++i) { lx[i] = R(in[i]); ly[i] = R(in[n + i]); lz[i] = R(in[2 * size_t(n) + i]); }
+  }
+
   void get(int which, double* out) const override {
     const std::vector<R>* s[4][3] = {{&px, &py, &pz}, {&vx, &vy, &vz}, {&qx, &qy, &qz}, {&rx, &ry, &rz}};
     for (int a = 0; a < 3; ++a)
@@ -846,6 +1092,122 @@ void or_membrane_project(int mode, int model, const double* p, const double* dm,
     for (int k = 0; k < 3; ++k) { out[3 * k] = double(o[k].x); out[3 * k + 1] = double(o[k].y); out[3 * k + 2] = double(o[k].z); }
   };
   if (mode == 0) run(0.0); else run(0.0f);
+}
+
+
+// ---- IPC contact step and contact primitives --------------------------------
+int or_step_ipc(void* o, double dt, int nsteps, const or_ipc_params* ip, or_step_result* res, or_ipc_stats* st) {
+  for (int k = 0; k < nsteps; ++k)
+    if (static_cast<OracleBase*>(o)->step_ipc(dt, *ip, res, st)) return 1;
+  return 0;
+}
+void or_get_lambda(void* o, double* out) { static_cast<OracleBase*>(o)->get_lambda(out); }
+void or_set_lambda(void* o, const double* in) { static_cast<OracleBase*>(o)->set_lambda(in); }
+
+double oc_barrier_energy(double d, double dh) { return oc::barrier_energy(d, dh); }
+double oc_barrier_gradient(double d, double dh) { return oc::barrier_gradient(d, dh); }
+double oc_barrier_hessian(double d, double dh) { return oc::barrier_hessian(d, dh); }
+double oc_scaled_barrier_gradient(double d, double dh, double k) { return oc::scaled_barrier_gradient(d, dh, k); }
+double oc_barrier_energy_thick(double d, double dh, double th) { return oc::barrier_energy_thick(d, dh, th); }
+double oc_barrier_gradient_thick(double d, double dh, double th) { return oc::barrier_gradient_thick(d, dh, th); }
+double oc_estimate_kappa(double c, double m, double g, double dh) { return oc::estimate_kappa(c, m, g, dh); }
+
+static oc::P3<double> p3(const double* v) { return oc::P3<double>(v[0], v[1], v[2]); }
+// p = [p, a, b, c] (12 doubles); returns d^2, type
+double oc_pt_dist2(const double* p, int* type) { return oc::pt_dist2(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), type); }
+void oc_pt_grad(const double* p, double* g) {
+  oc::P3<double> G[4];
+  oc::pt_grad(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), G);
+  for (int k = 0; k < 4; ++k) { g[3 * k] = G[k].x; g[3 * k + 1] = G[k].y; g[3 * k + 2] = G[k].z; }
+}
+double oc_ee_dist2(const double* p, int* type) { return oc::ee_dist2(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), type); }
+void oc_ee_grad(const double* p, double* g) {
+  oc::P3<double> G[4];
+  oc::ee_grad(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), G);
+  for (int k = 0; k < 4; ++k) { g[3 * k] = G[k].x; g[3 * k + 1] = G[k].y; g[3 * k + 2] = G[k].z; }
+}
+int oc_cubic_roots(double a, double b, double c, double d, double* out) { return oc::cubic_roots(a, b, c, d, out); }
+// p = [p0, p1, a0, a1, b0, b1, c0, c1] (24 doubles)
+int oc_vt_ccd(const double* p, double* toi) {
+  return oc::vt_ccd(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), p3(p + 12), p3(p + 15), p3(p + 18), p3(p + 21), toi) ? 1 : 0;
+}
+int oc_ee_ccd(const double* p, double* toi) {
+  return oc::ee_ccd(p3(p), p3(p + 3), p3(p + 6), p3(p + 9), p3(p + 12), p3(p + 15), p3(p + 18), p3(p + 21), toi) ? 1 : 0;
+}
+// Candidate triangle pairs (ta < tb, sorted) of f32 positions with margin;
+// brute = 1 tests all pairs. Returns the count; writes up to cap pairs.
+long long oc_tri_pairs(int n, const float* x, const float* y, const float* z, int t, const uint32_t* tri, float margin,
+                       int brute, int* out, long long cap) {
+  (void)n;
+  std::vector<oc::Box32> box(t);
+  for (int e = 0; e < t; ++e) {
+    const int v[3] = {int(tri[3 * e]), int(tri[3 * e + 1]), int(tri[3 * e + 2])};
+    box[e] = oc::tri_box(x, y, z, v, margin);
+  }
+  std::vector<std::pair<int, int>> pr;
+  if (brute) {
+    for (int a = 0; a < t; ++a)
+      for (int b = a + 1; b < t; ++b)
+        if (oc::overlaps(box[a], box[b])) pr.push_back({a, b});
+  } else {
+    pr = oc::tri_pairs(box);
+  }
+  for (long long k = 0; k < (long long)pr.size() && k < cap; ++k) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
+  return (long long)pr.size();
+}
+static oc::IpcScene scene_of(const or_ipc_params* ip) {
+  oc::IpcScene sc;
+  sc.has_ground = ip->has_ground != 0; sc.ground = ip->ground_height;
+  sc.has_sphere = ip->has_sphere != 0;
+  for (int k = 0; k < 3; ++k) sc.center[k] = ip->sphere_center[k];
+  sc.radius = ip->sphere_radius;
+  sc.self_collision = ip->self_collision != 0;
+  sc.thickness = ip->thickness;
+  return sc;
+}
+// CollisionPipeline::detect_ipc_contacts at positions (SoA, 3n doubles),
+// d_hat and kappa as given. out_grad / out_normal: 3n SoA; in_contact: n.
+// stats = [max_violation, active, self_contacts, pairs].
+void oc_detect(int n, const double* pos, int t, const uint32_t* tri, const or_ipc_params* ip, double dh, double kappa,
+               double* out_grad, double* out_normal, uint8_t* in_contact, double* stats) {
+  const double *x = pos, *y = pos + n, *z = pos + 2 * size_t(n);
+  std::vector<int> xp, xi;
+  oc::exclusion_sets(n, t, tri, ip->exclusion_depth, xp, xi);
+  std::vector<std::pair<int, int>> pairs;
+  oc::IpcScene sc = scene_of(ip);
+  if (sc.self_collision) {
+    std::vector<float> fx(x, x + n), fy(y, y + n), fz(z, z + n);
+    std::vector<oc::Box32> box(t);
+    for (int e = 0; e < t; ++e) {
+      const int v[3] = {int(tri[3 * e]), int(tri[3 * e + 1]), int(tri[3 * e + 2])};
+      box[e] = oc::tri_box(fx.data(), fy.data(), fz.data(), v, float(ip->thickness));
+    }
+    pairs = oc::tri_pairs(box);
+  }
+  oc::Forces<double> F;
+  oc::detect<double>(n, x, y, z, t, tri, pairs, xp, xi, sc, dh, kappa, F);
+  for (int i = 0; i < n; ++i) {
+    out_grad[i] = F.gx[i]; out_grad[n + i] = F.gy[i]; out_grad[2 * size_t(n) + i] = F.gz[i];
+    out_normal[i] = F.nx[i]; out_normal[n + i] = F.ny[i]; out_normal[2 * size_t(n) + i] = F.nz[i];
+    in_contact[i] = F.in_contact[i];
+  }
+  stats[0] = F.max_violation; stats[1] = double(F.active); stats[2] = double(F.self_contacts); stats[3] = double(pairs.size());
+}
+// CollisionPipeline::compute_ccd_step from p0 to p1 (SoA 3n), with the
+// candidate pairs of p0 (margin = thickness).
+double oc_ccd_step(int n, const double* p0, const double* p1, int t, const uint32_t* tri, const or_ipc_params* ip) {
+  oc::IpcScene sc = scene_of(ip);
+  std::vector<std::pair<int, int>> pairs;
+  if (sc.self_collision) {
+    std::vector<float> fx(p0, p0 + n), fy(p0 + n, p0 + 2 * size_t(n)), fz(p0 + 2 * size_t(n), p0 + 3 * size_t(n));
+    std::vector<oc::Box32> box(t);
+    for (int e = 0; e < t; ++e) {
+      const int v[3] = {int(tri[3 * e]), int(tri[3 * e + 1]), int(tri[3 * e + 2])};
+      box[e] = oc::tri_box(fx.data(), fy.data(), fz.data(), v, float(ip->thickness));
+    }
+    pairs = oc::tri_pairs(box);
+  }
+  return oc::ccd_step<double>(n, p0, p0 + n, p0 + 2 * size_t(n), p1, p1 + n, p1 + 2 * size_t(n), tri, pairs, sc);
 }
 
 }  // extern "C"

@@ -42,6 +42,55 @@ class OrStepResult(C.Structure):
                 ("converged", C.c_int), ("wall_time", C.c_double)]
 
 
+class OrIpcParams(C.Structure):
+    _fields_ = [
+        ("has_ground", C.c_int), ("ground_height", C.c_double),
+        ("has_sphere", C.c_int), ("sphere_center", C.c_double * 3), ("sphere_radius", C.c_double),
+        ("self_collision", C.c_int), ("exclusion_depth", C.c_int),
+        ("d_hat", C.c_double), ("kappa", C.c_double), ("thickness", C.c_double),
+        ("al_max_iterations", C.c_int), ("al_mu_initial", C.c_double), ("al_mu_growth", C.c_double),
+        ("al_tolerance", C.c_double),
+        ("compliant_contact", C.c_int), ("compliant_d_hat_scale", C.c_double),
+        ("friction_coefficient", C.c_double), ("contact_damping", C.c_double),
+        ("adaptive_contact_damping", C.c_int), ("contact_damping_max", C.c_double),
+        ("contact_velocity_threshold", C.c_double),
+    ]
+
+
+class OrIpcStats(C.Structure):
+    _fields_ = [("al_iterations", C.c_int), ("al_converged", C.c_int), ("mu", C.c_double),
+                ("max_violation", C.c_double), ("active", C.c_longlong), ("self_contacts", C.c_longlong),
+                ("pairs", C.c_longlong), ("last_alpha", C.c_double)]
+
+
+def ipc_params(ground=None, sphere=None, self_collision=False, exclusion_depth=1, d_hat=1e-3, kappa=1e4,
+               thickness=None, al_max_iterations=5, al_mu_initial=1.0, al_mu_growth=10.0, al_tolerance=1e-4,
+               compliant_contact=False, compliant_d_hat_scale=2.0, friction_coefficient=0.4, contact_damping=0.3,
+               adaptive_contact_damping=True, contact_damping_max=0.5,
+               contact_velocity_threshold=0.1) -> OrIpcParams:
+    """Contact settings; defaults follow SolverConfig::default (config.rs:113-133)
+    except compliant_contact (off: literal d_hat) and kappa (given)."""
+    q = OrIpcParams()
+    q.has_ground = 0 if ground is None else 1
+    q.ground_height = 0.0 if ground is None else float(ground)
+    q.has_sphere = 0 if sphere is None else 1
+    if sphere is not None:
+        (cx, cy, cz), r = sphere
+        q.sphere_center[0], q.sphere_center[1], q.sphere_center[2] = cx, cy, cz
+        q.sphere_radius = r
+    q.self_collision = 1 if self_collision else 0
+    q.exclusion_depth = exclusion_depth
+    q.d_hat, q.kappa = d_hat, kappa
+    q.thickness = 1.5 * d_hat if thickness is None else thickness
+    q.al_max_iterations, q.al_mu_initial, q.al_mu_growth, q.al_tolerance = (
+        al_max_iterations, al_mu_initial, al_mu_growth, al_tolerance)
+    q.compliant_contact, q.compliant_d_hat_scale = int(compliant_contact), compliant_d_hat_scale
+    q.friction_coefficient, q.contact_damping = friction_coefficient, contact_damping
+    q.adaptive_contact_damping = int(adaptive_contact_damping)
+    q.contact_damping_max, q.contact_velocity_threshold = contact_damping_max, contact_velocity_threshold
+    return q
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
     return _LIB_PATH
@@ -80,6 +129,40 @@ def lib():
         L.or_dihedral.argtypes = [C.c_int, dp]
         L.or_bend_project.argtypes = [C.c_int, dp, C.c_double, dp]
         L.or_membrane_project.argtypes = [C.c_int, C.c_int, dp, dp, dp]
+        ip = C.POINTER(OrIpcParams)
+        L.or_step_ipc.argtypes = [vp, C.c_double, C.c_int, ip, C.POINTER(OrStepResult), C.POINTER(OrIpcStats)]
+        L.or_get_lambda.argtypes = [vp, dp]
+        L.or_set_lambda.argtypes = [vp, dp]
+        for nm in ("oc_barrier_energy", "oc_barrier_gradient", "oc_barrier_hessian", "oc_barrier_energy_thick",
+                   "oc_barrier_gradient_thick"):
+            getattr(L, nm).restype = C.c_double
+            getattr(L, nm).argtypes = [C.c_double, C.c_double]
+        L.oc_barrier_energy_thick.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oc_barrier_gradient_thick.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oc_scaled_barrier_gradient.restype = C.c_double
+        L.oc_scaled_barrier_gradient.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.oc_estimate_kappa.restype = C.c_double
+        L.oc_estimate_kappa.argtypes = [C.c_double] * 4
+        ipp = C.POINTER(C.c_int)
+        L.oc_pt_dist2.restype = C.c_double
+        L.oc_pt_dist2.argtypes = [dp, ipp]
+        L.oc_ee_dist2.restype = C.c_double
+        L.oc_ee_dist2.argtypes = [dp, ipp]
+        L.oc_pt_grad.argtypes = [dp, dp]
+        L.oc_ee_grad.argtypes = [dp, dp]
+        L.oc_cubic_roots.restype = C.c_int
+        L.oc_cubic_roots.argtypes = [C.c_double] * 4 + [dp]
+        L.oc_vt_ccd.restype = C.c_int
+        L.oc_vt_ccd.argtypes = [dp, dp]
+        L.oc_ee_ccd.restype = C.c_int
+        L.oc_ee_ccd.argtypes = [dp, dp]
+        L.oc_tri_pairs.restype = C.c_longlong
+        L.oc_tri_pairs.argtypes = [C.c_int, fp, fp, fp, C.c_int, up, C.c_float, C.c_int, C.POINTER(C.c_int),
+                                   C.c_longlong]
+        L.oc_detect.argtypes = [C.c_int, dp, C.c_int, up, ip, C.c_double, C.c_double, dp, dp,
+                                C.POINTER(C.c_uint8), dp]
+        L.oc_ccd_step.restype = C.c_double
+        L.oc_ccd_step.argtypes = [C.c_int, dp, dp, C.c_int, up, ip]
         _lib = L
     return _lib
 
@@ -183,6 +266,22 @@ class Oracle:
         lib().or_hinges(self._h, _p(rest, C.c_double), _p(w, C.c_double))
         return rest, w
 
+    def step_ipc(self, ipc: OrIpcParams, dt=1.0 / 60.0, nsteps=1):
+        """step_with_ipc (pd_solver.rs:549-1157) with the collision pipeline
+        as the handler; returns (StepResult, stats)."""
+        r, st = OrStepResult(), OrIpcStats()
+        lib().or_step_ipc(self._h, dt, nsteps, C.byref(ipc), C.byref(r), C.byref(st))
+        return r, st
+
+    def get_lambda(self) -> np.ndarray:
+        out = np.empty(3 * self.n)
+        lib().or_get_lambda(self._h, _p(out, C.c_double))
+        return out.reshape(3, self.n)
+
+    def set_lambda(self, arr):
+        a = _d(arr).reshape(-1)
+        lib().or_set_lambda(self._h, _p(a, C.c_double))
+
     def local_rhs(self):
         rhs = np.empty(3 * self.n)
         tt = np.empty(9 * self.t)
@@ -230,3 +329,121 @@ def membrane_project(p3, dm, model=1, mode=F64):
     out = np.empty(9)
     lib().or_membrane_project(mode, model, _p(p, C.c_double), _p(d, C.c_double), _p(out, C.c_double))
     return out.reshape(3, 3)
+
+
+# ---- contact primitives (contact_ref.hpp) ----------------------------------
+
+def barrier_energy(d, dh):
+    return lib().oc_barrier_energy(d, dh)
+
+
+def barrier_gradient(d, dh):
+    return lib().oc_barrier_gradient(d, dh)
+
+
+def barrier_hessian(d, dh):
+    return lib().oc_barrier_hessian(d, dh)
+
+
+def scaled_barrier_gradient(d, dh, kappa):
+    return lib().oc_scaled_barrier_gradient(d, dh, kappa)
+
+
+def barrier_energy_thick(d, dh, th):
+    return lib().oc_barrier_energy_thick(d, dh, th)
+
+
+def barrier_gradient_thick(d, dh, th):
+    return lib().oc_barrier_gradient_thick(d, dh, th)
+
+
+def estimate_kappa(configured, avg_mass, gravity, dh):
+    return lib().oc_estimate_kappa(configured, avg_mass, gravity, dh)
+
+
+PT_FACE, PT_EDGE0, PT_EDGE1, PT_EDGE2, PT_VERT0, PT_VERT1, PT_VERT2, EE_INTERIOR, DEGENERATE = range(9)
+
+
+def pt_dist2(p, a, b, c):
+    v = _d(np.concatenate([p, a, b, c]))
+    ty = C.c_int(0)
+    d = lib().oc_pt_dist2(_p(v, C.c_double), C.byref(ty))
+    return d, ty.value
+
+
+def pt_grad(p, a, b, c):
+    v = _d(np.concatenate([p, a, b, c]))
+    g = np.empty(12)
+    lib().oc_pt_grad(_p(v, C.c_double), _p(g, C.c_double))
+    return g.reshape(4, 3)
+
+
+def ee_dist2(a0, a1, b0, b1):
+    v = _d(np.concatenate([a0, a1, b0, b1]))
+    ty = C.c_int(0)
+    d = lib().oc_ee_dist2(_p(v, C.c_double), C.byref(ty))
+    return d, ty.value
+
+
+def ee_grad(a0, a1, b0, b1):
+    v = _d(np.concatenate([a0, a1, b0, b1]))
+    g = np.empty(12)
+    lib().oc_ee_grad(_p(v, C.c_double), _p(g, C.c_double))
+    return g.reshape(4, 3)
+
+
+def cubic_roots(a, b, c, d):
+    out = np.empty(3)
+    k = lib().oc_cubic_roots(a, b, c, d, _p(out, C.c_double))
+    return out[:k].copy()
+
+
+def vt_ccd(p0, p1, a0, a1, b0, b1, c0, c1):
+    v = _d(np.concatenate([p0, p1, a0, a1, b0, b1, c0, c1]))
+    t = C.c_double(0)
+    hit = lib().oc_vt_ccd(_p(v, C.c_double), C.byref(t))
+    return t.value if hit else None
+
+
+def ee_ccd(a0, a1, b0, b1, c0, c1, d0, d1):
+    v = _d(np.concatenate([a0, a1, b0, b1, c0, c1, d0, d1]))
+    t = C.c_double(0)
+    hit = lib().oc_ee_ccd(_p(v, C.c_double), C.byref(t))
+    return t.value if hit else None
+
+
+def tri_pairs(px, py, pz, indices, margin, brute=False):
+    """Every overlapping pair of f32 triangle boxes (ta < tb), sorted, as (k, 2) int32."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float32) for a in (px, py, pz))
+    tri = np.ascontiguousarray(indices, dtype=np.uint32)
+    t = tri.size // 3
+    L = lib()
+    cnt = L.oc_tri_pairs(x.size, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), t, _p(tri, C.c_uint32),
+                         margin, int(brute), None, 0)
+    out = np.empty((max(cnt, 1), 2), dtype=np.int32)
+    L.oc_tri_pairs(x.size, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), t, _p(tri, C.c_uint32),
+                   margin, int(brute), out.ctypes.data_as(C.POINTER(C.c_int)), cnt)
+    return out[:cnt]
+
+
+def detect(pos, indices, ipc: OrIpcParams, d_hat, kappa):
+    """CollisionPipeline::detect_ipc_contacts at SoA positions (3, n)."""
+    pos = _d(pos).reshape(3, -1)
+    n = pos.shape[1]
+    tri = np.ascontiguousarray(indices, dtype=np.uint32)
+    g, nrm = np.empty(3 * n), np.empty(3 * n)
+    ic = np.empty(n, dtype=np.uint8)
+    st = np.empty(4)
+    lib().oc_detect(n, _p(pos, C.c_double), tri.size // 3, _p(tri, C.c_uint32), C.byref(ipc), d_hat, kappa,
+                    _p(g, C.c_double), _p(nrm, C.c_double), ic.ctypes.data_as(C.POINTER(C.c_uint8)),
+                    _p(st, C.c_double))
+    return dict(grad=g.reshape(3, n), normal=nrm.reshape(3, n), in_contact=ic, max_violation=st[0],
+                active=int(st[1]), self_contacts=int(st[2]), pairs=int(st[3]))
+
+
+def ccd_step(p0, p1, indices, ipc: OrIpcParams):
+    """CollisionPipeline::compute_ccd_step from p0 to p1 (SoA (3, n))."""
+    p0, p1 = _d(p0).reshape(3, -1), _d(p1).reshape(3, -1)
+    tri = np.ascontiguousarray(indices, dtype=np.uint32)
+    return lib().oc_ccd_step(p0.shape[1], _p(p0, C.c_double), _p(p1, C.c_double), tri.size // 3,
+                             _p(tri, C.c_uint32), C.byref(ipc))
