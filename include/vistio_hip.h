@@ -184,6 +184,76 @@ typedef struct {
 vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile* out);
 vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out);
 
+/* ---- IPC contact (augmented-Lagrangian step) --------------------------------
+ * Settings of ProjectiveDynamicsSolver::step_with_ipc
+ * (crates/vistio-solver/src/config.rs:47-120) and of the CollisionPipeline the
+ * benchmark runner builds (crates/vistio-bench/src/runner.rs:60-95, 118-122):
+ * ground plane (also the state's ground height), analytic sphere, mesh
+ * self-collision with n-ring exclusion. kappa is the barrier stiffness the
+ * handler uses (the runner: estimate_initial_kappa, barrier.rs:102-121). */
+typedef struct {
+  int32_t has_ground;
+  double ground_height;
+  int32_t has_sphere;
+  double sphere_center[3];
+  double sphere_radius;
+  int32_t self_collision;
+  int32_t exclusion_depth;        /* TopologyExclusion depth (runner: 1) */
+  double d_hat;                   /* barrier_d_hat, a squared distance */
+  double kappa;
+  double thickness;               /* broad-phase margin and self-contact thickness (runner: 1.5 d_hat) */
+  int32_t al_max_iterations;
+  double al_mu_initial, al_mu_growth, al_tolerance;
+  int32_t compliant_contact;      /* effective d_hat = d_hat * compliant_d_hat_scale */
+  double compliant_d_hat_scale;
+  double friction_coefficient, contact_damping;
+  int32_t adaptive_contact_damping;
+  double contact_damping_max, contact_velocity_threshold;
+  uint32_t max_pairs;             /* candidate-pair buffer; 0 = 32 per triangle */
+} vh_ipc_params;
+
+/* Diagnostics of the last IPC step or detection. */
+typedef struct {
+  int32_t al_iterations;
+  int32_t al_converged;
+  double mu;
+  double max_violation;
+  int64_t active;                 /* active contacts (self + ground + sphere) */
+  int64_t self_contacts;
+  int64_t pairs;                  /* candidate triangle pairs */
+  double last_alpha;              /* step fraction of the last PD iteration */
+} vh_ipc_stats;
+
+/* Enables the contact step on an initialized context (after vh_pd_init). */
+vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params);
+/* ProjectiveDynamicsSolver::step_with_ipc (crates/vistio-solver/src/pd_solver.rs:549-1157)
+ * with the pipeline as IpcCollisionHandler, on caller f32 SoA state; the
+ * augmented-Lagrangian multipliers (SimulationState.al_lambda_{x,y,z},
+ * state.rs:55-63) are read and written through lambda_{x,y,z}. */
+vh_status vh_pd_step_ipc(vh_ctx* ctx, double dt, vh_state_f32* state, float* lambda_x, float* lambda_y,
+                         float* lambda_z, vh_step_result* out);
+/* The same step with the state (and multipliers) resident in HBM. */
+vh_status vh_pd_step_ipc_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_result* out,
+                                  vh_ipc_stats* stats);
+/* CollisionPipeline::detect_ipc_contacts (crates/vistio-contact/src/collision_pipeline.rs:207-351)
+ * at positions pos (3n SoA, caller order; NULL = the resident state) with the
+ * given d_hat and kappa. Outputs (caller order): grad 3n SoA, normal 3n SoA,
+ * in_contact n; any may be NULL. */
+vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kappa, double* grad, double* normal,
+                        uint8_t* in_contact, vh_ipc_stats* stats);
+/* CollisionPipeline::compute_ccd_step (collision_pipeline.rs:355-400) from p0
+ * to p1 (3n SoA each), with the candidate pairs of p0. */
+vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, double* alpha);
+/* BroadPhase::query_triangle_pairs (crates/vistio-contact/src/bvh.rs:379-392):
+ * every pair of triangle boxes (f32, inflated by margin) that overlap, as
+ * (ta < tb) in the caller's triangle numbering, sorted. Writes up to cap
+ * pairs and the total count. */
+vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* ta, uint32_t* tb, uint64_t cap,
+                       uint64_t* count);
+/* Resident multipliers (3n SoA, caller order). */
+vh_status vh_state_lambda_set_f64(vh_ctx* ctx, const double* lambda);
+vh_status vh_state_lambda_get_f64(vh_ctx* ctx, double* lambda);
+
 /* Mesh helpers (crates/vistio-mesh): quad_grid (generators.rs:28-100) and the
  * interior edges of Topology::build (topology.rs:58-120, :158-168) in a
  * canonical (v0, v1)-sorted order. */

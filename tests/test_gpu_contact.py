@@ -1,0 +1,222 @@
+"""GPU parity of the IPC contact path against the oracle (FP64 restatement).
+
+Broad phase: the candidate triangle pairs equal the oracle's all-pairs set
+exactly. Detection and CCD: FP64 agreement to rounding. The whole
+step_with_ipc: positions and velocities after several steps of a sheet
+dropped on the ground, a sheet draped on a sphere and a sheet folded onto
+itself.
+"""
+import numpy as np
+import pytest
+
+from conftest import DT, PHYS, max_vertex_err
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(cols, rows, w, h):
+    from oracle import mesh
+    px, py, pz, idx = mesh.quad_grid(cols, rows, w, h)
+    return px, py, pz, idx, mesh.hinges4(idx, px.size)
+
+
+def _solver(px, py, pz, idx, hg, pin, ipc_cfg, precision="f64", **phys):
+    import vistio_amd as va
+    ph = dict(PHYS)
+    ph.update(phys)
+    m = va.TriangleMesh(px, py, pz, idx)
+    topo = va.Topology(np.concatenate([hg, np.zeros((hg.shape[0], 2), np.uint32)], 1))
+    cfg = va.SolverConfig(max_iterations=ph["max_iterations"], tolerance=ph["tolerance"], gravity=ph["gravity"],
+                          damping=ph["damping"], rayleigh_mass_damping=ph["rayleigh"], precision=precision,
+                          ir_sweeps=2, ir_final_sweeps=3)
+    s = va.HipPdSolver()
+    s.init_raw(m, topo, cfg, pin, "co_rotational", ph["k_membrane"], ph["k_bending"], ph["density"])
+    s.init_ipc(ipc_cfg)
+    return s
+
+
+def _oracle(px, py, pz, idx, hg, pin, **phys):
+    from oracle import ref
+    ph = dict(PHYS)
+    ph.update(phys)
+    return ref.Oracle(ref.F64, px, py, pz, idx, hg, pin, ref.params(**ph))
+
+
+def _cfgs(ground=None, sphere=None, self_collision=False, d_hat=1e-3, kappa=1e3, thickness=None, al=5):
+    import vistio_amd as va
+    from oracle import ref
+    th = 1.5 * d_hat if thickness is None else thickness
+    g = va.IpcConfig(ground_height=ground, sphere=sphere, self_collision=self_collision, barrier_d_hat=d_hat,
+                     barrier_kappa=kappa, thickness=th, al_max_iterations=al, compliant_contact=False)
+    o = ref.ipc_params(ground=ground, sphere=sphere, self_collision=self_collision, d_hat=d_hat, kappa=kappa,
+                       thickness=th, al_max_iterations=al, compliant_contact=False)
+    return g, o
+
+
+def _f32_exact(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def _folded_sheet(c=8, gap=0.02, seed=0):
+    """A 2c x c strip folded in half along x = 0.5: the upper half lies gap
+    above the lower one (a connected mesh with a sharp fold), plus noise."""
+    px, py, pz, idx, hg = _mesh(2 * c, c, 1.0, 0.5)
+    x = px.astype(np.float64)
+    y = np.zeros_like(x)
+    z = py.astype(np.float64)
+    up = x > 0.0
+    x[up] = -x[up]
+    y[up] = gap
+    rng = np.random.default_rng(seed)
+    y = y + rng.normal(size=y.size) * gap * 0.05
+    pos = np.stack([x, y, z])
+    return pos.astype(np.float32), idx, hg
+
+
+def test_bvh_pairs_match_all_pairs():
+    from oracle import ref
+    for c, margin, seed in ((12, 0.01, 0), (16, 0.05, 1), (10, 0.0, 2)):
+        px, py, pz, idx, hg = _mesh(c, c, 1.0, 1.0)
+        rng = np.random.default_rng(seed)
+        pz = (rng.normal(size=px.size) * 0.05).astype(np.float32)
+        pin = np.zeros(px.size, dtype=np.uint8)
+        g, _ = _cfgs(self_collision=True)
+        s = _solver(px, py, pz, idx, hg, pin, g)
+        pos = np.stack([px, py, pz]).astype(np.float64)
+        got = s.bvh_pairs(pos, margin)
+        want = ref.tri_pairs(px, py, pz, idx, margin)
+        print(f"c={c} margin={margin}: {len(got)} pairs", flush=True)
+        assert got.shape == want.shape and np.array_equal(got.astype(np.int64), want.astype(np.int64))
+        s.close()
+    pos, idx, hg = _folded_sheet()
+    pin = np.zeros(pos.shape[1], dtype=np.uint8)
+    g, _ = _cfgs(self_collision=True)
+    s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, g)
+    got = s.bvh_pairs(pos.astype(np.float64), 0.03)
+    want = ref.tri_pairs(pos[0], pos[1], pos[2], idx, 0.03)
+    assert np.array_equal(got.astype(np.int64), want.astype(np.int64)) and len(got) > 0
+    s.close()
+
+
+def test_detect_matches_oracle():
+    from oracle import ref
+    pos, idx, hg = _folded_sheet(gap=0.015)
+    n = pos.shape[1]
+    pin = np.zeros(n, dtype=np.uint8)
+    pos64 = pos.astype(np.float64)
+    pos64[1] += 0.012  # lower layer 12 mm above the ground at y = 0
+    for case in ("self", "ground", "sphere", "all"):
+        kw = dict(self_collision=case in ("self", "all"), ground=0.0 if case in ("ground", "all") else None,
+                  sphere=((0.0, -0.2, 0.25), 0.215) if case in ("sphere", "all") else None)
+        g, o = _cfgs(d_hat=4e-4, kappa=1e3, **kw)
+        s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, g)
+        for dh, kappa in ((4e-4, 1e3), (1e-3, 37.5)):
+            got = s.detect_contacts(pos64, dh, kappa)
+            want = ref.detect(pos64, idx, o, dh, kappa)
+            print(f"{case} dh={dh}: active {got['active']} self {got['self_contacts']} pairs {got['pairs']}", flush=True)
+            assert got["active"] == want["active"] and got["self_contacts"] == want["self_contacts"]
+            assert np.array_equal(got["in_contact"], want["in_contact"])
+            scale = max(1.0, np.abs(want["grad"]).max())
+            assert np.abs(got["grad"] - want["grad"]).max() <= 1e-10 * scale
+            assert np.abs(got["normal"] - want["normal"]).max() <= 1e-9
+            assert got["max_violation"] == pytest.approx(want["max_violation"], abs=1e-15)
+        s.close()
+
+
+def test_ccd_step_matches_oracle():
+    from oracle import ref
+    pos, idx, hg = _folded_sheet(gap=0.02)
+    n = pos.shape[1]
+    pin = np.zeros(n, dtype=np.uint8)
+    p0 = pos.astype(np.float64)
+    p0[1] += 0.03
+    rng = np.random.default_rng(5)
+    moves = []
+    d = np.zeros_like(p0); d[1] = -0.05; moves.append(d)                       # everything through the ground
+    d = np.zeros_like(p0); d[1, p0[1] > 0.04] = -0.03; moves.append(d)         # upper layer through the lower
+    moves.append(rng.normal(size=p0.shape) * 0.01)
+    for case in ("self", "ground", "sphere", "all"):
+        kw = dict(self_collision=case in ("self", "all"), ground=0.0 if case in ("ground", "all") else None,
+                  sphere=((0.0, -0.25, 0.25), 0.26) if case in ("sphere", "all") else None)
+        g, o = _cfgs(**kw)
+        s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, g)
+        for d in moves:
+            got = s.compute_ccd_step(p0, p0 + d)
+            want = ref.ccd_step(p0, p0 + d, idx, o)
+            print(f"{case}: alpha gpu {got} oracle {want}", flush=True)
+            assert got == pytest.approx(want, rel=1e-12, abs=1e-15)
+        s.close()
+
+
+def _run_pair(px, py, pz, idx, hg, pin, g, o, steps, **phys):
+    s = _solver(px, py, pz, idx, hg, pin, g, **phys)
+    orc = _oracle(px, py, pz, idx, hg, pin, **phys)
+    errs = []
+    for k in range(steps):
+        r, st = s.step_ipc_resident(DT, 1)
+        ro, sto = orc.step_ipc(o, DT)
+        assert r.iterations == ro.iterations, (k, r.iterations, ro.iterations)
+        assert st["al_iterations"] == sto.al_iterations, (k, st, sto.al_iterations)
+        errs.append(max_vertex_err(s.positions(), orc.positions()))
+    v = np.abs(s.get_state(1) - orc.get(1)).max()
+    lam = np.abs(s.get_lambda() - orc.get_lambda()).max()
+    s.close()
+    return errs, v, lam, st
+
+
+def test_step_ipc_sheet_on_ground():
+    c = 8
+    px, py, pz, idx, hg = _mesh(c, c, 0.5, 0.5)
+    pz2, py2 = py.copy(), np.full_like(py, 0.03)
+    pin = np.zeros(px.size, dtype=np.uint8)
+    g, o = _cfgs(ground=0.0, d_hat=1e-3, kappa=1e2)
+    errs, v, lam, st = _run_pair(px, py2, pz2, idx, hg, pin, g, o, 20)
+    print("ground: max err per step", ["%.1e" % e for e in errs], "v", v, "lambda", lam, st, flush=True)
+    assert max(errs) <= 1e-9 and v <= 1e-7 and lam <= 1e-6
+
+
+def test_step_ipc_sphere_drape():
+    c = 12
+    px, py, pz, idx, hg = _mesh(c, c, 0.6, 0.6)
+    y = np.full_like(py, 0.33)
+    pin = np.zeros(px.size, dtype=np.uint8)
+    g, o = _cfgs(sphere=((0.0, 0.0, 0.0), 0.3), ground=-0.3, d_hat=1e-3, kappa=1e2)
+    errs, v, lam, st = _run_pair(px, y, py.copy(), idx, hg, pin, g, o, 15)
+    print("sphere: max err per step", ["%.1e" % e for e in errs], "v", v, "lambda", lam, st, flush=True)
+    assert max(errs) <= 1e-8 and v <= 1e-6
+
+
+def test_step_ipc_self_fold():
+    pos, idx, hg = _folded_sheet(c=6, gap=0.03)
+    pin = np.zeros(pos.shape[1], dtype=np.uint8)
+    y = pos[1] + np.float32(0.002)
+    g, o = _cfgs(self_collision=True, ground=0.0, d_hat=4e-4, kappa=1e2, thickness=0.01)
+    errs, v, lam, st = _run_pair(pos[0], y, pos[2], idx, hg, pin, g, o, 12)
+    print("self fold: max err per step", ["%.1e" % e for e in errs], "v", v, "lambda", lam, st, flush=True)
+    assert st["self_contacts"] >= 0
+    assert max(errs) <= 1e-8 and v <= 1e-6
+
+
+def test_step_with_ipc_host_state():
+    """The drop-in entry point on f32 host state agrees with the resident
+    step to the f32 rounding of the state; the multipliers come back."""
+    import vistio_amd as va
+    c = 6
+    px, py, pz, idx, hg = _mesh(c, c, 0.5, 0.5)
+    pz2, py2 = py.copy(), np.full_like(py, 0.02)
+    pin = np.zeros(px.size, dtype=np.uint8)
+    g, _ = _cfgs(ground=0.0, d_hat=1e-3, kappa=1e2)
+    a = _solver(px, py2, pz2, idx, hg, pin, g)
+    b = _solver(px, py2, pz2, idx, hg, pin, g)
+    m = va.TriangleMesh(px, py2, pz2, idx)
+    state = va.SimulationState.from_mesh_with_masses(m, a.lumped_masses(), pin)
+    for _ in range(6):
+        a.step_with_ipc(state, DT)
+        b.step_ipc_resident(DT, 1)
+    assert np.all(np.isfinite(state.pos_y)) and state.pos_y.min() > 0.0
+    err = np.abs(b.positions() - np.stack([state.pos_x, state.pos_y, state.pos_z])).max()
+    lam = np.abs(b.get_lambda() - np.stack([state.al_lambda_x, state.al_lambda_y, state.al_lambda_z])).max()
+    print("host vs resident:", err, lam, flush=True)
+    assert err < 1e-5 and lam <= 1e-3 * max(1.0, np.abs(b.get_lambda()).max())
+    a.close()
+    b.close()

@@ -4,8 +4,10 @@ The numerical path lives in libvistio_hip.so (vistio_amd/csrc, C ABI in
 include/vistio_hip.h); this package is the host-side mirror of the
 reference's solver interface.
 """
-from .solver import (FabricProperties, GpuError, HipPdSolver, InvalidConfig, InvalidMesh, SimulationState,
-                     SolverConfig, StepResult, Topology, TriangleMesh, VistioError, quad_grid)
+from .solver import (FabricProperties, GpuError, HipPdSolver, InvalidConfig, InvalidMesh, IpcConfig,
+                     SimulationState, SolverConfig, StepResult, Topology, TriangleMesh, VistioError,
+                     estimate_initial_kappa, quad_grid)
 
-__all__ = ["FabricProperties", "GpuError", "HipPdSolver", "InvalidConfig", "InvalidMesh", "SimulationState",
-           "SolverConfig", "StepResult", "Topology", "TriangleMesh", "VistioError", "quad_grid"]
+__all__ = ["FabricProperties", "GpuError", "HipPdSolver", "InvalidConfig", "InvalidMesh", "IpcConfig",
+           "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh", "VistioError",
+           "estimate_initial_kappa", "quad_grid"]

@@ -27,6 +27,8 @@ EXPORTS = (
     "vh_create", "vh_destroy", "vh_last_error", "vh_pd_init", "vh_pd_step", "vh_pd_step_resident",
     "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
     "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
+    "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
+    "vh_state_lambda_set_f64", "vh_state_lambda_get_f64",
 )
 
 
@@ -58,6 +60,27 @@ class PdInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("t", C.c_uint32), ("h", C.c_uint32), ("supernodes", C.c_uint32),
                 ("levels", C.c_uint32), ("nnz_a", C.c_int64), ("nnz_factor", C.c_int64),
                 ("max_front", C.c_int32), ("setup_seconds", C.c_double)]
+
+
+class IpcParams(C.Structure):
+    _fields_ = [
+        ("has_ground", C.c_int32), ("ground_height", C.c_double),
+        ("has_sphere", C.c_int32), ("sphere_center", C.c_double * 3), ("sphere_radius", C.c_double),
+        ("self_collision", C.c_int32), ("exclusion_depth", C.c_int32),
+        ("d_hat", C.c_double), ("kappa", C.c_double), ("thickness", C.c_double),
+        ("al_max_iterations", C.c_int32), ("al_mu_initial", C.c_double), ("al_mu_growth", C.c_double),
+        ("al_tolerance", C.c_double),
+        ("compliant_contact", C.c_int32), ("compliant_d_hat_scale", C.c_double),
+        ("friction_coefficient", C.c_double), ("contact_damping", C.c_double),
+        ("adaptive_contact_damping", C.c_int32), ("contact_damping_max", C.c_double),
+        ("contact_velocity_threshold", C.c_double), ("max_pairs", C.c_uint32),
+    ]
+
+
+class IpcStats(C.Structure):
+    _fields_ = [("al_iterations", C.c_int32), ("al_converged", C.c_int32), ("mu", C.c_double),
+                ("max_violation", C.c_double), ("active", C.c_int64), ("self_contacts", C.c_int64),
+                ("pairs", C.c_int64), ("last_alpha", C.c_double)]
 
 
 VH_MAX_BANDS = 16
@@ -104,6 +127,16 @@ def load():
         "vh_pd_profile": ([vp, C.c_double, C.c_uint32, C.POINTER(Profile)], C.c_int),
         "vh_mesh_quad_grid": ([C.c_uint32, C.c_uint32, C.c_float, C.c_float, fp, fp, fp, up], C.c_int),
         "vh_mesh_interior_edges": ([C.c_uint32, C.c_uint32, up, up, C.c_uint32, up], C.c_int),
+        "vh_ipc_init": ([vp, C.POINTER(IpcParams)], C.c_int),
+        "vh_pd_step_ipc": ([vp, C.c_double, C.POINTER(StateF32), fp, fp, fp, C.POINTER(StepResultC)], C.c_int),
+        "vh_pd_step_ipc_resident": ([vp, C.c_double, C.c_uint32, C.POINTER(StepResultC), C.POINTER(IpcStats)],
+                                    C.c_int),
+        "vh_ipc_detect": ([vp, dp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_uint8), C.POINTER(IpcStats)],
+                          C.c_int),
+        "vh_ipc_ccd_step": ([vp, dp, dp, dp], C.c_int),
+        "vh_bvh_pairs": ([vp, dp, C.c_double, up, up, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+        "vh_state_lambda_set_f64": ([vp, dp], C.c_int),
+        "vh_state_lambda_get_f64": ([vp, dp], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

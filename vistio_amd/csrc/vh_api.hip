@@ -18,9 +18,34 @@
 #include <string>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "vh_kernels.hpp"
+#include "vh_contact.hpp"
 #include "vh_setup.hpp"
 #include "vistio_hip.h"
+
+// Device buffers and settings of the contact step (vh_ipc_init).
+struct IpcState {
+  bool on = false;
+  vh_ipc_params P{};
+  vhc::Scene sc{};
+  double *gself = nullptr, *grad = nullptr, *nrm = nullptr, *fnrm = nullptr, *lambda = nullptr, *ext = nullptr;
+  double *partials = nullptr, *scratch_a = nullptr, *scratch_b = nullptr;
+  uint8_t *inc = nullptr, *ever = nullptr;
+  int *xptr = nullptr, *xidx = nullptr;
+  vhc::Box *box = nullptr, *node = nullptr;
+  unsigned int *code = nullptr, *code_s = nullptr;
+  int *id = nullptr, *id_s = nullptr;
+  int *left = nullptr, *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *range_hi = nullptr;
+  int* visits = nullptr;
+  int2* pairs = nullptr;
+  unsigned int cap = 0;
+  vhc::BvhDev* bd = nullptr;
+  vhc::IpcDev* ip = nullptr;
+  void* sort_tmp = nullptr;
+  size_t sort_bytes = 0;
+};
 
 struct vh_ctx {
   int device = 0;
@@ -51,6 +76,8 @@ struct vh_ctx {
   double* partials = nullptr;
   int resid_blocks = 0;  // blocks writing residual partials
   vhk::StepDev* st = nullptr;
+  const double* cur_ext = nullptr;  // RHS external term of the launches being enqueued
+  IpcState ipc;
   std::vector<void*> allocs;
   std::vector<double> hbuf;  // host staging (3n)
   hipGraphExec_t step_graph = nullptr;
@@ -125,6 +152,7 @@ void free_all(vh_ctx* ctx) {
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->initialized = false;
+  ctx->ipc = IpcState{};
 }
 
 inline int nblk(int n) { return (n + 255) / 256; }
@@ -134,7 +162,7 @@ inline int nblk(int n) { return (n + 255) / 256; }
 template <class T>
 vhk::RhsSrc<T> rhs_src(vh_ctx* c, double dt) {
   return vhk::RhsSrc<T>{c->inc_ptr, (const T*)c->cslot, c->mass, c->pred, c->x, 1.0 / (dt * dt),
-                        1.0 / (c->P.dt_matrix * c->P.dt_matrix)};
+                        1.0 / (c->P.dt_matrix * c->P.dt_matrix), c->cur_ext};
 }
 
 vhk::CommitArgs commit_args(vh_ctx* c, bool on, bool want_resid, bool add_acc) {
@@ -313,6 +341,128 @@ vh_status launch_step(vh_ctx* ctx, double dt) {
     ctx->graph_dt = dt;
   }
   VH_HIP(hipGraphLaunch(ctx->step_graph, ctx->stream));
+  return VH_OK;
+}
+
+// ---- contact step -------------------------------------------------------
+
+const int* ipc_done_flag(vh_ctx* c) {
+  return reinterpret_cast<const int*>(reinterpret_cast<const char*>(c->ipc.ip) + offsetof(vhc::IpcDev, al_done));
+}
+
+__global__ void k_ipc_clear(vhc::IpcDev* ip) {
+  if (threadIdx.x != 0) return;
+  ip->al_done = 0; ip->pd_done = 0; ip->overflow = 0; ip->toi = vhc::kNoToi;
+  ip->viol = 0; ip->active = 0; ip->self_contacts = 0;
+}
+
+// Candidate triangle pairs of positions x (boxes inflated by margin).
+void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
+  IpcState& I = c->ipc;
+  const int t = c->S.t;
+  hipLaunchKernelGGL(vhc::k_bvh_reset, dim3(1), dim3(64), 0, c->stream, I.bd, skip);
+  if (t == 0) return;
+  hipLaunchKernelGGL(vhc::k_boxes, dim3(nblk(t)), dim3(256), 0, c->stream, t, c->tri_idx, x, margin, I.box, I.bd, skip);
+  hipLaunchKernelGGL(vhc::k_morton, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.bd, I.code, I.id, skip);
+  size_t bytes = I.sort_bytes;
+  (void)rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t, 0u, 30u,
+                                  c->stream);
+  if (t < 2) return;
+  hipLaunchKernelGGL(vhc::k_zero, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t - 1, I.visits, skip);
+  hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.right,
+                     I.parent, I.leaf_parent, I.range_hi, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
+                     I.parent, I.leaf_parent, I.visits, I.node, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
+                     I.range_hi, I.node, I.pairs, I.cap, I.bd, skip);
+}
+
+// CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
+void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset_lambda, const int* skip) {
+  IpcState& I = c->ipc;
+  const int n = c->S.n;
+  hipLaunchKernelGGL(vhc::k_ipc_detect_begin, dim3(1), dim3(64), 0, c->stream, I.ip);
+  if (sc.self_collision) {
+    enqueue_pairs(c, x, (float)sc.thickness, skip);
+    hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
+    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, c->stream,
+                       I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.gself, I.ip, skip);
+  }
+  hipLaunchKernelGGL(vhc::k_vertex_contacts, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, sc, I.gself, I.grad, I.nrm,
+                     I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, skip);
+}
+
+// step_with_ipc (pd_solver.rs:549-1157) as one launch sequence: the AL and
+// PD loops have fixed trip counts and stop through device flags.
+void enqueue_ipc_step(vh_ctx* c, double dt) {
+  IpcState& I = c->ipc;
+  const vh::Setup& S = c->S;
+  const vh_pd_params& P = c->P;
+  const int n = S.n;
+  const bool f64 = c->prec == VH_PREC_F64;
+  const vh_ipc_params& Q = I.P;
+  const int* skip = ipc_done_flag(c);
+  const double max_vel = 0.5 * std::sqrt(I.sc.dh) / dt;
+  hipLaunchKernelGGL(vhc::k_ipc_begin, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->v, c->prev, c->pred,
+                     c->pinned, I.ever, dt, P.gravity[0], P.gravity[1], P.gravity[2], max_vel, Q.has_ground,
+                     Q.ground_height, Q.al_mu_initial, I.ip, c->st);
+  enqueue_detect(c, c->x, I.sc, true, skip);
+  const double* dl = f64 ? (const double*)c->d : c->acc;
+  for (int al = 0; al < Q.al_max_iterations; ++al) {
+    hipLaunchKernelGGL(vhc::k_ipc_ext, dim3(nblk(3 * n)), dim3(256), 0, c->stream, n, I.grad, I.lambda, I.ext, I.ip);
+    c->cur_ext = I.ext;
+    for (int it = 0; it < P.max_iterations; ++it) {
+      if (f64) enqueue_local<double>(c, c->st); else enqueue_local<float>(c, c->st);
+      Marks none;
+      const int sweeps = it + 1 == P.max_iterations ? P.ir_final_sweeps : P.ir_sweeps;
+      enqueue_correction(c, dt, sweeps, true, false, false, c->st, none);
+      hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
+                         I.partials, I.ip);
+      if (I.sc.self_collision)
+        hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, c->stream,
+                           I.pairs, &I.bd->npairs, I.cap, c->tri_idx, c->x, dl, I.ip);
+      if (I.sc.has_ground || I.sc.has_sphere)
+        hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
+      hipLaunchKernelGGL(vhc::k_ipc_alpha, dim3(1), dim3(256), 0, c->stream, I.partials, nblk(n), P.tolerance, I.ip);
+      hipLaunchKernelGGL(vhc::k_ipc_update, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, c->pinned, I.ip);
+      hipLaunchKernelGGL(vhc::k_ipc_pd_latch, dim3(1), dim3(64), 0, c->stream, I.ip, c->st);
+    }
+    c->cur_ext = nullptr;
+    enqueue_detect(c, c->x, I.sc, false, skip);
+    hipLaunchKernelGGL(vhc::k_ipc_al_end, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.grad, I.inc, I.lambda,
+                       Q.al_tolerance, al + 1 < Q.al_max_iterations ? 1 : 0, I.ip);
+    hipLaunchKernelGGL(vhc::k_ipc_al_commit, dim3(1), dim3(64), 0, c->stream, Q.al_tolerance, Q.al_mu_growth,
+                       al + 1 == Q.al_max_iterations ? 1 : 0, I.ip, c->st);
+  }
+  vhc::FinalArgs f{};
+  f.inv_dt = 1.0 / dt; f.mu_f = Q.friction_coefficient; f.cdamp = Q.contact_damping;
+  f.cdamp_max = Q.contact_damping_max; f.v_thresh = Q.contact_velocity_threshold;
+  f.one_minus_damp = 1.0 - P.damping; f.frayl = P.rayleigh > 0 ? 1.0 / (1.0 + P.rayleigh * dt) : 0.0;
+  f.ground = Q.ground_height; f.adaptive = Q.adaptive_contact_damping; f.has_ground = Q.has_ground;
+  hipLaunchKernelGGL(vhc::k_ipc_finalize, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->prev, c->v, c->pinned,
+                     I.ever, I.fnrm, f, I.ip);
+}
+
+// Reads the step result and the contact diagnostics (synchronizes).
+vh_status read_ipc_result(vh_ctx* ctx, vh_step_result* out, vh_ipc_stats* st) {
+  vhc::IpcDev h{};
+  unsigned int np = 0;
+  VH_HIP(hipMemcpyAsync(&h, ctx->ipc.ip, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(&np, &ctx->ipc.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (out) {
+    out->iterations = (uint32_t)h.iterations;
+    out->final_residual = h.resid;
+    out->converged = (h.converged || h.resid < ctx->P.tolerance) ? 1 : 0;
+  }
+  if (st) {
+    st->al_iterations = h.al_iters; st->al_converged = h.converged; st->mu = h.mu;
+    st->max_violation = __builtin_bit_cast(double, h.viol);
+    st->active = (int64_t)h.active; st->self_contacts = (int64_t)h.self_contacts;
+    st->pairs = ctx->ipc.sc.self_collision ? (int64_t)np : 0; st->last_alpha = h.last_alpha;
+  }
+  if (h.overflow) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  if (!(h.resid == h.resid)) return fail(ctx, VH_E_NONFINITE, "non-finite positions in the IPC step");
   return VH_OK;
 }
 
@@ -763,6 +913,289 @@ vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out) {
   if (!ctx || !out) return VH_E_INVALID_ARG;
   if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
   for (int i = 0; i < ctx->S.n; ++i) out[i] = (float)ctx->S.mass_orig[i];
+  return VH_OK;
+}
+
+
+// ---- IPC contact ------------------------------------------------------------
+
+vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!params) return fail(ctx, VH_E_INVALID_ARG, "params is null");
+  const vh_ipc_params& Q = *params;
+  if (!(Q.d_hat > 0) || !(Q.kappa >= 0) || !(Q.thickness >= 0) || Q.al_max_iterations < 1 || Q.exclusion_depth < 0 ||
+      (Q.has_sphere && !(Q.sphere_radius > 0)) || (Q.compliant_contact && !(Q.compliant_d_hat_scale > 0)))
+    return fail(ctx, VH_E_INVALID_CONFIG, "invalid contact parameters");
+  if (ctx->ipc.on) return fail(ctx, VH_E_INVALID_CONFIG, "contact already initialized on this context");
+  VH_HIP(hipSetDevice(ctx->device));
+  const vh::Setup& S = ctx->S;
+  const int n = S.n, t = S.t;
+  IpcState& I = ctx->ipc;
+  I.P = Q;
+  I.sc.has_ground = Q.has_ground; I.sc.ground = Q.ground_height;
+  I.sc.has_sphere = Q.has_sphere;
+  I.sc.cx = Q.sphere_center[0]; I.sc.cy = Q.sphere_center[1]; I.sc.cz = Q.sphere_center[2];
+  I.sc.radius = Q.sphere_radius;
+  I.sc.self_collision = Q.self_collision;
+  I.sc.thickness = Q.thickness;
+  I.sc.dh = Q.compliant_contact ? Q.d_hat * Q.compliant_d_hat_scale : Q.d_hat;
+  I.sc.kappa = Q.kappa;
+  // Exclusion sets (exclusion.rs:22-41): v plus `depth` one-ring expansions,
+  // in the device's vertex labels.
+  std::vector<std::vector<int>> ring(n);
+  for (int e = 0; e < t; ++e)
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        if (a != b) ring[S.tri_idx[3 * e + a]].push_back(S.tri_idx[3 * e + b]);
+  for (auto& r : ring) { std::sort(r.begin(), r.end()); r.erase(std::unique(r.begin(), r.end()), r.end()); }
+  std::vector<int> xptr(n + 1, 0), xidx, cur, next;
+  for (int v = 0; v < n; ++v) {
+    cur.assign(1, v);
+    for (int d = 0; d < Q.exclusion_depth; ++d) {
+      next = cur;
+      for (int u : cur) next.insert(next.end(), ring[u].begin(), ring[u].end());
+      std::sort(next.begin(), next.end());
+      next.erase(std::unique(next.begin(), next.end()), next.end());
+      cur.swap(next);
+    }
+    xidx.insert(xidx.end(), cur.begin(), cur.end());
+    xptr[v + 1] = (int)xidx.size();
+  }
+  const uint64_t cap64 = Q.max_pairs ? (uint64_t)Q.max_pairs : 32ull * (uint64_t)std::max(1, t);
+  if (cap64 > (1ull << 31)) return fail(ctx, VH_E_UNSUPPORTED, "max_pairs too large");
+  I.cap = (unsigned int)cap64;
+  const size_t n3 = 3 * (size_t)n;
+  const int tt = std::max(1, t);
+#define UP(call)                          \
+  do {                                    \
+    vh_status s_ = (call);                \
+    if (s_ != VH_OK) return s_;           \
+  } while (0)
+  UP(dalloc(ctx, &I.gself, n3)); UP(dalloc(ctx, &I.grad, n3)); UP(dalloc(ctx, &I.nrm, n3));
+  UP(dalloc(ctx, &I.fnrm, n3)); UP(dalloc(ctx, &I.lambda, n3)); UP(dalloc(ctx, &I.ext, n3));
+  UP(dalloc(ctx, &I.scratch_a, n3)); UP(dalloc(ctx, &I.scratch_b, n3));
+  UP(dalloc(ctx, &I.partials, 3 * (size_t)nblk(n)));
+  UP(dalloc(ctx, &I.inc, (size_t)n)); UP(dalloc(ctx, &I.ever, (size_t)n));
+  UP(upload(ctx, &I.xptr, xptr)); UP(upload(ctx, &I.xidx, xidx));
+  UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, (size_t)tt));
+  UP(dalloc(ctx, &I.code, (size_t)tt)); UP(dalloc(ctx, &I.code_s, (size_t)tt));
+  UP(dalloc(ctx, &I.id, (size_t)tt)); UP(dalloc(ctx, &I.id_s, (size_t)tt));
+  UP(dalloc(ctx, &I.left, (size_t)tt)); UP(dalloc(ctx, &I.right, (size_t)tt)); UP(dalloc(ctx, &I.parent, (size_t)tt));
+  UP(dalloc(ctx, &I.leaf_parent, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
+  UP(dalloc(ctx, &I.visits, (size_t)tt));
+  UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
+  UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
+#undef UP
+  size_t bytes = 0;
+  VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
+                                   ctx->stream));
+  I.sort_bytes = std::max<size_t>(bytes, 16);
+  {
+    vh_status s_ = dalloc_bytes(ctx, &I.sort_tmp, I.sort_bytes);
+    if (s_ != VH_OK) return s_;
+  }
+  VH_HIP(hipMemset(I.gself, 0, n3 * 8));
+  VH_HIP(hipMemset(I.lambda, 0, n3 * 8));
+  VH_HIP(hipMemset(I.fnrm, 0, n3 * 8));
+  VH_HIP(hipMemset(I.ever, 0, (size_t)n));
+  VH_HIP(hipMemset(I.bd, 0, sizeof(vhc::BvhDev)));
+  VH_HIP(hipMemset(I.ip, 0, sizeof(vhc::IpcDev)));
+  I.on = true;
+  return VH_OK;
+}
+
+vh_status vh_pd_step_ipc_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_result* out,
+                                  vh_ipc_stats* stats) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
+  VH_HIP(hipSetDevice(ctx->device));
+  const auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t k = 0; k < nsteps; ++k) {
+    enqueue_ipc_step(ctx, dt);
+    VH_HIP(hipGetLastError());
+  }
+  vh_status st = read_ipc_result(ctx, out, stats);
+  if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+vh_status vh_pd_step_ipc(vh_ctx* ctx, double dt, vh_state_f32* s, float* lambda_x, float* lambda_y,
+                         float* lambda_z, vh_step_result* out) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
+  if (!s || !s->pos_x || !s->pos_y || !s->pos_z || !s->vel_x || !s->vel_y || !s->vel_z || !s->prev_x ||
+      !s->prev_y || !s->prev_z || !s->pred_x || !s->pred_y || !s->pred_z || !lambda_x || !lambda_y || !lambda_z)
+    return fail(ctx, VH_E_INVALID_ARG, "null state array");
+  const auto t0 = std::chrono::steady_clock::now();
+  const vh::Setup& S = ctx->S;
+  const size_t n3 = 3 * (size_t)S.n;
+  std::vector<double> hp(n3), hv(n3), hl(n3);
+  for (int k = 0; k < S.n; ++k) {
+    const int o = S.perm[k];
+    hp[3 * k] = s->pos_x[o]; hp[3 * k + 1] = s->pos_y[o]; hp[3 * k + 2] = s->pos_z[o];
+    hv[3 * k] = s->vel_x[o]; hv[3 * k + 1] = s->vel_y[o]; hv[3 * k + 2] = s->vel_z[o];
+    hl[3 * k] = lambda_x[o]; hl[3 * k + 1] = lambda_y[o]; hl[3 * k + 2] = lambda_z[o];
+  }
+  VH_HIP(hipSetDevice(ctx->device));
+  if (s->inv_mass) {
+    vh_status e = vh_state_set_inv_mass(ctx, s->inv_mass);
+    if (e != VH_OK) return e;
+  }
+  VH_HIP(hipMemcpyAsync(ctx->x, hp.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(ctx->v, hv.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(ctx->ipc.lambda, hl.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
+  enqueue_ipc_step(ctx, dt);
+  VH_HIP(hipGetLastError());
+  std::vector<double> hq(n3), hr(n3);
+  VH_HIP(hipMemcpyAsync(hp.data(), ctx->x, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(hv.data(), ctx->v, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(hq.data(), ctx->prev, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(hr.data(), ctx->pred, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(hl.data(), ctx->ipc.lambda, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  vh_status st = read_ipc_result(ctx, out, nullptr);
+  for (int k = 0; k < S.n; ++k) {
+    const int o = S.perm[k];
+    s->pos_x[o] = (float)hp[3 * k]; s->pos_y[o] = (float)hp[3 * k + 1]; s->pos_z[o] = (float)hp[3 * k + 2];
+    s->vel_x[o] = (float)hv[3 * k]; s->vel_y[o] = (float)hv[3 * k + 1]; s->vel_z[o] = (float)hv[3 * k + 2];
+    s->prev_x[o] = (float)hq[3 * k]; s->prev_y[o] = (float)hq[3 * k + 1]; s->prev_z[o] = (float)hq[3 * k + 2];
+    s->pred_x[o] = (float)hr[3 * k]; s->pred_y[o] = (float)hr[3 * k + 1]; s->pred_z[o] = (float)hr[3 * k + 2];
+    lambda_x[o] = (float)hl[3 * k]; lambda_y[o] = (float)hl[3 * k + 1]; lambda_z[o] = (float)hl[3 * k + 2];
+  }
+  if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return st;
+}
+
+vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kappa, double* grad, double* normal,
+                        uint8_t* in_contact, vh_ipc_stats* stats) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!(d_hat > 0) || !(kappa >= 0)) return fail(ctx, VH_E_INVALID_ARG, "d_hat > 0 and kappa >= 0 required");
+  VH_HIP(hipSetDevice(ctx->device));
+  const vh::Setup& S = ctx->S;
+  const int n = S.n;
+  const double* x = ctx->x;
+  if (pos) {
+    to_dev_order(S, pos, pos + n, pos + 2 * (size_t)n, ctx->hbuf);
+    VH_HIP(hipMemcpyAsync(ctx->ipc.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice,
+                          ctx->stream));
+    x = ctx->ipc.scratch_a;
+  }
+  vhc::Scene sc = ctx->ipc.sc;
+  sc.dh = d_hat;
+  sc.kappa = kappa;
+  hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, ctx->ipc.ip);
+  enqueue_detect(ctx, x, sc, false, nullptr);
+  VH_HIP(hipGetLastError());
+  const size_t n3 = 3 * (size_t)n;
+  std::vector<double> g(n3), nr(n3);
+  std::vector<uint8_t> ic(n);
+  VH_HIP(hipMemcpyAsync(g.data(), ctx->ipc.grad, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(nr.data(), ctx->ipc.nrm, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(ic.data(), ctx->ipc.inc, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
+  vh_status st = read_ipc_result(ctx, nullptr, stats);
+  if (st != VH_OK && st != VH_E_NONFINITE) return st;
+  if (grad) from_dev_order(S, g, grad, grad + n, grad + 2 * (size_t)n);
+  if (normal) from_dev_order(S, nr, normal, normal + n, normal + 2 * (size_t)n);
+  if (in_contact)
+    for (int k = 0; k < n; ++k) in_contact[S.perm[k]] = ic[k];
+  return VH_OK;
+}
+
+vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, double* alpha) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!p0 || !p1 || !alpha) return fail(ctx, VH_E_INVALID_ARG, "null array");
+  VH_HIP(hipSetDevice(ctx->device));
+  const vh::Setup& S = ctx->S;
+  const int n = S.n;
+  IpcState& I = ctx->ipc;
+  to_dev_order(S, p0, p0 + n, p0 + 2 * (size_t)n, ctx->hbuf);
+  std::vector<double> a = ctx->hbuf;
+  to_dev_order(S, p1, p1 + n, p1 + 2 * (size_t)n, ctx->hbuf);
+  for (size_t k = 0; k < a.size(); ++k) ctx->hbuf[k] -= a[k];  // delta = p1 - p0
+  VH_HIP(hipMemcpyAsync(I.scratch_a, a.data(), a.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(I.scratch_b, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
+  if (I.sc.self_collision) {
+    enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
+    hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, ctx->stream, I.pairs,
+                       &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ip);
+  }
+  if (I.sc.has_ground || I.sc.has_sphere)
+    hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, I.scratch_a, I.scratch_b,
+                       I.sc, I.ip);
+  VH_HIP(hipGetLastError());
+  vhc::IpcDev h{};
+  unsigned int np = 0;
+  VH_HIP(hipMemcpyAsync(&h, I.ip, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (I.sc.self_collision && np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  const double toi = std::min(1.0, __builtin_bit_cast(double, h.toi));
+  *alpha = std::max(toi, 1e-6);
+  return VH_OK;
+}
+
+vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* ta, uint32_t* tb, uint64_t cap,
+                       uint64_t* count) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!pos || !count || (cap > 0 && (!ta || !tb))) return fail(ctx, VH_E_INVALID_ARG, "null array");
+  VH_HIP(hipSetDevice(ctx->device));
+  const vh::Setup& S = ctx->S;
+  const int n = S.n;
+  IpcState& I = ctx->ipc;
+  to_dev_order(S, pos, pos + n, pos + 2 * (size_t)n, ctx->hbuf);
+  VH_HIP(hipMemcpyAsync(I.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr);
+  VH_HIP(hipGetLastError());
+  unsigned int np = 0;
+  VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  std::vector<int2> hp(np);
+  if (np) VH_HIP(hipMemcpy(hp.data(), I.pairs, (size_t)np * sizeof(int2), hipMemcpyDeviceToHost));
+  std::vector<std::pair<uint32_t, uint32_t>> pr(np);
+  for (unsigned int k = 0; k < np; ++k) {
+    const uint32_t a = (uint32_t)S.tri_orig[hp[k].x], b = (uint32_t)S.tri_orig[hp[k].y];
+    pr[k] = {std::min(a, b), std::max(a, b)};
+  }
+  std::sort(pr.begin(), pr.end());
+  *count = np;
+  for (uint64_t k = 0; k < std::min<uint64_t>(cap, np); ++k) { ta[k] = pr[k].first; tb[k] = pr[k].second; }
+  return VH_OK;
+}
+
+vh_status vh_state_lambda_set_f64(vh_ctx* ctx, const double* lambda) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized || !ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!lambda) return fail(ctx, VH_E_INVALID_ARG, "null array");
+  VH_HIP(hipSetDevice(ctx->device));
+  const int n = ctx->S.n;
+  to_dev_order(ctx->S, lambda, lambda + n, lambda + 2 * (size_t)n, ctx->hbuf);
+  VH_HIP(hipMemcpyAsync(ctx->ipc.lambda, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_state_lambda_get_f64(vh_ctx* ctx, double* lambda) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized || !ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!lambda) return fail(ctx, VH_E_INVALID_ARG, "null array");
+  VH_HIP(hipSetDevice(ctx->device));
+  const int n = ctx->S.n;
+  ctx->hbuf.resize(3 * (size_t)n);
+  VH_HIP(hipMemcpyAsync(ctx->hbuf.data(), ctx->ipc.lambda, ctx->hbuf.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  from_dev_order(ctx->S, ctx->hbuf, lambda, lambda + n, lambda + 2 * (size_t)n);
   return VH_OK;
 }
 

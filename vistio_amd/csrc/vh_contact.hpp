@@ -1,0 +1,981 @@
+// gfx950 kernels of the IPC contact path: LBVH broad phase, vertex-triangle
+// and edge-edge barrier forces, continuous collision, and the per-vertex
+// stages of the augmented-Lagrangian step.
+//
+// Reference: crates/vistio-contact/src/{barrier,distance_primitives,ccd,bvh,
+// self_collision,exclusion,ipc_response,ground_plane,sphere,
+// collision_pipeline}.rs and ProjectiveDynamicsSolver::step_with_ipc
+// (crates/vistio-solver/src/pd_solver.rs:549-1157). Everything is FP64 except
+// the broad phase, whose boxes are f32 (bvh.rs:15-62), so that the candidate
+// pairs are exactly the overlapping pairs of the reference's f32 boxes.
+//
+// Broad phase: Karras' LBVH (Morton codes of box centroids, radix-sorted;
+// internal nodes from common-prefix splits; bottom-up bounds with one
+// arrival counter per node). Every leaf traverses the tree for leaves that
+// follow it in Morton order, so each overlapping pair is emitted once. The
+// reference's recursive median-split query drops the pairs whose left-subtree
+// index is larger (bvh.rs:207); here the pair set is complete.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace vhc {
+
+// ---- small FP64 vector ---------------------------------------------------
+
+struct D3 {
+  double x, y, z;
+};
+__device__ __forceinline__ D3 d3(double x, double y, double z) { return D3{x, y, z}; }
+__device__ __forceinline__ D3 operator+(D3 a, D3 b) { return d3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ D3 operator-(D3 a, D3 b) { return d3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ D3 operator-(D3 a) { return d3(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ D3 operator*(D3 a, double s) { return d3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ D3 cross(D3 a, D3 b) {
+  return d3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double len2(D3 a) { return dot(a, a); }
+__device__ __forceinline__ double clamp01(double v) { return fmin(fmax(v, 0.0), 1.0); }
+__device__ __forceinline__ D3 ld(const double* x, int i) { return d3(x[3 * i], x[3 * i + 1], x[3 * i + 2]); }
+
+// ---- barrier.rs ------------------------------------------------------------
+
+// barrier.rs:40-53: db/dd of b(d) = -(d - dh)^2 ln(d/dh), d clamped to
+// [1e-3 dh, dh].
+__device__ __forceinline__ double barrier_gradient(double d, double dh) {
+  const double dc = fmin(fmax(d, dh * 1e-3), dh);
+  if (dc >= dh) return 0.0;
+  const double diff = dc - dh;
+  return -diff * (2.0 * log(dc / dh) + diff / dc);
+}
+// barrier.rs:82-93
+__device__ __forceinline__ double scaled_barrier_gradient(double d, double dh, double kappa) {
+  const double r = kappa * barrier_gradient(d, dh);
+  return isfinite(r) ? r : copysign(1e6, r);
+}
+// barrier.rs:140-163: thickness offset d - th^2 (non-positive: d = 1e-12).
+__device__ __forceinline__ double scaled_barrier_gradient_thick(double d, double dh, double kappa, double th) {
+  const double de = d - th * th;
+  const double r = kappa * barrier_gradient(de <= 0.0 ? 1e-12 : de, dh);
+  return isfinite(r) ? r : copysign(1e6, r);
+}
+
+// ---- distance_primitives.rs ------------------------------------------------
+
+__device__ __forceinline__ double point_segment_dist2(D3 p, D3 a, D3 b) {
+  const D3 ab = b - a, ap = p - a;
+  const double l2 = len2(ab);
+  if (l2 < 1e-20) return len2(ap);
+  const double t = clamp01(dot(ap, ab) / l2);
+  return len2(p - (a + ab * t));
+}
+
+// distance_primitives.rs:36-116 (Ericson's Voronoi-region closest point).
+__device__ double pt_dist2(D3 p, D3 a, D3 b, D3 c) {
+  const D3 ab = b - a, ac = c - a, ap = p - a, bp = p - b, cp = p - c;
+  if (len2(cross(ab, ac)) < 1e-20) {
+    const double e0 = point_segment_dist2(p, a, b), e1 = point_segment_dist2(p, b, c), e2 = point_segment_dist2(p, c, a);
+    if (e0 <= e1 && e0 <= e2) return e0;
+    return e1 <= e2 ? e1 : e2;
+  }
+  const double d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= 0.0 && d2 <= 0.0) return len2(ap);
+  const double d3v = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3v >= 0.0 && d4 <= d3v) return len2(bp);
+  const double d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= 0.0 && d5 <= d6) return len2(cp);
+  const double vc = d1 * d4 - d3v * d2;
+  if (vc <= 0.0 && d1 >= 0.0 && d3v <= 0.0) return len2(p - (a + ab * (d1 / (d1 - d3v))));
+  const double va = d3v * d6 - d5 * d4;
+  if (va <= 0.0 && (d4 - d3v) >= 0.0 && (d5 - d6) >= 0.0)
+    return len2(p - (b + (c - b) * ((d4 - d3v) / ((d4 - d3v) + (d5 - d6)))));
+  const double vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) return len2(p - (a + ac * (d2 / (d2 - d6))));
+  const double den = 1.0 / (va + vb + vc);
+  return len2(p - (a + ab * (vb * den) + ac * (vc * den)));
+}
+
+// distance_primitives.rs:124-177 (gradient through the plane projection).
+__device__ void pt_grad(D3 p, D3 a, D3 b, D3 c, D3 g[4]) {
+  g[0] = g[1] = g[2] = g[3] = d3(0, 0, 0);
+  const D3 ab = b - a, ac = c - a;
+  const D3 nrm = cross(ab, ac);
+  const double a2 = len2(nrm);
+  if (a2 < 1e-20) return;
+  const D3 n = nrm * (1.0 / sqrt(a2));
+  const D3 proj = p - n * dot(p - a, n);
+  const D3 v2 = proj - a;
+  const double d00 = dot(ab, ab), d01 = dot(ab, ac), d11 = dot(ac, ac), d20 = dot(v2, ab), d21 = dot(v2, ac);
+  const double den = d00 * d11 - d01 * d01;
+  if (fabs(den) < 1e-20) return;
+  const double inv = 1.0 / den;
+  const double v = (d11 * d20 - d01 * d21) * inv, w = (d00 * d21 - d01 * d20) * inv, u = 1.0 - v - w;
+  const D3 diff = p - (a * u + b * v + c * w);
+  g[0] = diff * 2.0;
+  g[1] = -diff * (2.0 * u);
+  g[2] = -diff * (2.0 * v);
+  g[3] = -diff * (2.0 * w);
+}
+
+// distance_primitives.rs:185-246 (segment-segment, eps = 1e-10 branches).
+__device__ double ee_dist2(D3 a0, D3 a1, D3 b0, D3 b1) {
+  const D3 d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+  const double eps = 1e-10;
+  if (a < eps && e < eps) return len2(r);
+  double s, t;
+  if (a < eps) {
+    s = 0.0; t = clamp01(f / e);
+  } else {
+    const double c = dot(d1, r);
+    if (e < eps) {
+      t = 0.0; s = clamp01(-c / a);
+    } else {
+      const double b = dot(d1, d2);
+      const double den = a * e - b * b;
+      s = fabs(den) > eps ? clamp01((b * f - c * e) / den) : 0.0;
+      t = (b * s + f) / e;
+      if (t < 0.0) { t = 0.0; s = clamp01(-c / a); }
+      else if (t > 1.0) { t = 1.0; s = clamp01((b - c) / a); }
+    }
+  }
+  return len2((a0 + d1 * s) - (b0 + d2 * t));
+}
+
+// distance_primitives.rs:251-303
+__device__ void ee_grad(D3 a0, D3 a1, D3 b0, D3 b1, D3 g[4]) {
+  const D3 d1 = a1 - a0, d2 = b1 - b0, r = a0 - b0;
+  const double a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r), c = dot(d1, r), b = dot(d1, d2);
+  const double eps = 1e-10;
+  double s = 0.0, t = 0.0;
+  if (!(a < eps || e < eps)) {
+    const double den = a * e - b * b;
+    s = fabs(den) > eps ? clamp01((b * f - c * e) / den) : 0.0;
+    t = (b * s + f) / e;
+    if (t < 0.0) { t = 0.0; s = clamp01(-c / a); }
+    else if (t > 1.0) { t = 1.0; s = clamp01((b - c) / a); }
+  }
+  const D3 diff = (a0 + d1 * s) - (b0 + d2 * t);
+  g[0] = diff * (2.0 * (1.0 - s));
+  g[1] = diff * (2.0 * s);
+  g[2] = -diff * (2.0 * (1.0 - t));
+  g[3] = -diff * (2.0 * t);
+}
+
+// ---- ccd.rs ----------------------------------------------------------------
+
+// ccd.rs:352-377
+__device__ int quadratic_roots(double a, double b, double c, double out[3]) {
+  const double eps = 1e-8;
+  if (fabs(a) < eps) {
+    if (fabs(b) < eps) return 0;
+    out[0] = -c / b;
+    return 1;
+  }
+  const double disc = b * b - 4.0 * a * c;
+  if (disc < -eps) return 0;
+  if (fabs(disc) <= eps) { out[0] = -b / (2.0 * a); return 1; }
+  const double sq = sqrt(disc);
+  out[0] = (-b + sq) / (2.0 * a);
+  out[1] = (-b - sq) / (2.0 * a);
+  return 2;
+}
+
+// ccd.rs:299-349 (Cardano; trigonometric form for three real roots)
+__device__ int cubic_roots(double a, double b, double c, double d, double out[3]) {
+  const double eps = 1e-8;
+  if (fabs(a) < eps) return quadratic_roots(b, c, d, out);
+  const double p = b / a, q = c / a, r = d / a;
+  const double ad = q - p * p / 3.0;
+  const double bd = r - p * q / 3.0 + 2.0 * p * p * p / 27.0;
+  const double disc = bd * bd / 4.0 + ad * ad * ad / 27.0;
+  const double shift = -p / 3.0;
+  if (disc > eps) {
+    const double sq = sqrt(disc);
+    out[0] = cbrt(-bd / 2.0 + sq) + cbrt(-bd / 2.0 - sq) + shift;
+    return 1;
+  }
+  if (fabs(disc) <= eps) {
+    if (fabs(bd) < eps) { out[0] = shift; return 1; }
+    const double u = cbrt(bd / 2.0);
+    out[0] = -2.0 * u + shift;
+    out[1] = u + shift;
+    return 2;
+  }
+  const double rv = sqrt(-ad * ad * ad / 27.0);
+  if (fabs(rv) < 1e-12) { out[0] = shift; return 1; }
+  const double cv = fmin(fmax(-bd / (2.0 * rv), -1.0), 1.0);
+  const double th = acos(cv);
+  const double m = 2.0 * cbrt(rv);
+  const double pi = 3.14159265358979323846;
+  out[0] = m * cos(th / 3.0) + shift;
+  out[1] = m * cos((th + 2.0 * pi) / 3.0) + shift;
+  out[2] = m * cos((th + 4.0 * pi) / 3.0) + shift;
+  return 3;
+}
+
+// ccd.rs:32-129: smallest safe TOI (0.9 t, at least 1e-6) of vertex p
+// against the moving triangle (a, b, c), or 2 (no hit).
+__device__ double vt_ccd(D3 p0, D3 p1, D3 a0, D3 a1, D3 b0, D3 b1, D3 c0, D3 c1) {
+  const D3 dp = p1 - p0;
+  const D3 ar = a0 - p0, br = b0 - p0, cr = c0 - p0;
+  const D3 av = (a1 - a0) - dp, bv = (b1 - b0) - dp, cv = (c1 - c0) - dp;
+  const D3 vv = cross(bv, cv), rv = cross(br, cv), vr = cross(bv, cr), rr = cross(br, cr);
+  const double ca = dot(av, vv);
+  const double cb = dot(ar, vv) + dot(av, rv + vr);
+  const double cc = dot(ar, rv + vr) + dot(av, rr);
+  const double cd = dot(ar, rr);
+  double roots[3];
+  const int nr = cubic_roots(ca, cb, cc, cd, roots);
+  double best = 2.0;
+  for (int k = 0; k < nr; ++k) {
+    double t = roots[k];
+    if (t < -1e-6 || t > 1.0 + 1e-6) continue;
+    t = clamp01(t);
+    const D3 pa = ar + av * t, pb = br + bv * t, pc = cr + cv * t;
+    if (len2(cross(pb - pa, pc - pa)) < 1e-20) continue;
+    const D3 v0 = pb - pa, v1 = pc - pa, v2 = -pa;
+    const double d00 = dot(v0, v0), d01 = dot(v0, v1), d11 = dot(v1, v1), d20 = dot(v2, v0), d21 = dot(v2, v1);
+    const double den = d00 * d11 - d01 * d01;
+    if (fabs(den) < 1e-20) continue;
+    const double inv = 1.0 / den;
+    const double u = (d11 * d20 - d01 * d21) * inv, w = (d00 * d21 - d01 * d20) * inv, v = 1.0 - u - w;
+    if (v >= -0.01 && u >= -0.01 && w >= -0.01) best = fmin(best, fmax(t * 0.9, 1e-6));
+  }
+  return best;
+}
+
+// ---- LBVH (bvh.rs semantics: f32 boxes, closed-interval overlap) ----------
+
+struct Box {
+  float lo[3], hi[3];
+};
+
+__device__ __forceinline__ bool overlaps(const Box& a, const Box& b) {
+  return a.lo[0] <= b.hi[0] && a.hi[0] >= b.lo[0] && a.lo[1] <= b.hi[1] && a.hi[1] >= b.lo[1] &&
+         a.lo[2] <= b.hi[2] && a.hi[2] >= b.lo[2];
+}
+
+// Order-preserving float <-> uint for atomic min/max.
+__device__ __forceinline__ unsigned int f2o(float f) {
+  const unsigned int u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float o2f(unsigned int u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+struct BvhDev {
+  unsigned int scene[6];  // order-mapped lo xyz, hi xyz of the box centroids
+  unsigned int npairs;    // pairs found (may exceed the capacity)
+  unsigned int pad;
+};
+
+// bvh.rs:28-42: box of f32 positions, inflated by margin. Also folds the
+// centroid into the scene bounds.
+__global__ void __launch_bounds__(256) k_boxes(int t, const int* __restrict__ tri, const double* __restrict__ x,
+                                               float margin, Box* __restrict__ box, BvhDev* bd, const int* skip) {
+  if (skip && *skip) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  float c[3] = {0, 0, 0};
+  bool ok = e < t;
+  if (ok) {
+    Box b;
+    const int v0 = tri[3 * e], v1 = tri[3 * e + 1], v2 = tri[3 * e + 2];
+    for (int a = 0; a < 3; ++a) {
+      const float p0 = (float)x[3 * v0 + a], p1 = (float)x[3 * v1 + a], p2 = (float)x[3 * v2 + a];
+      b.lo[a] = fminf(fminf(p0, p1), p2) - margin;
+      b.hi[a] = fmaxf(fmaxf(p0, p1), p2) + margin;
+      c[a] = 0.5f * (b.lo[a] + b.hi[a]);
+    }
+    box[e] = b;
+  }
+  // block reduction of the centroid bounds, one atomic per block
+  __shared__ float smin[3][256], smax[3][256];
+  for (int a = 0; a < 3; ++a) {
+    smin[a][threadIdx.x] = ok ? c[a] : FLT_MAX;
+    smax[a][threadIdx.x] = ok ? c[a] : -FLT_MAX;
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        smin[a][threadIdx.x] = fminf(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+        smax[a][threadIdx.x] = fmaxf(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; ++a) {
+      atomicMin(&bd->scene[a], f2o(smin[a][0]));
+      atomicMax(&bd->scene[3 + a], f2o(smax[a][0]));
+    }
+}
+
+// Scene bounds to empty, pair count to 0.
+__global__ void k_bvh_reset(BvhDev* bd, const int* skip) {
+  if (threadIdx.x != 0 || (skip && *skip)) return;
+  for (int a = 0; a < 3; ++a) { bd->scene[a] = f2o(FLT_MAX); bd->scene[3 + a] = f2o(-FLT_MAX); }
+  bd->npairs = 0;
+}
+
+__device__ __forceinline__ unsigned int expand_bits(unsigned int v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+// 30-bit Morton code of each box centroid in the scene bounds.
+__global__ void __launch_bounds__(256) k_morton(int t, const Box* __restrict__ box, const BvhDev* bd,
+                                                unsigned int* __restrict__ code, int* __restrict__ id,
+                                                const int* skip) {
+  if (skip && *skip) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= t) return;
+  unsigned int q[3];
+  for (int a = 0; a < 3; ++a) {
+    const float lo = o2f(bd->scene[a]), hi = o2f(bd->scene[3 + a]);
+    const float c = 0.5f * (box[e].lo[a] + box[e].hi[a]);
+    const float ext = hi - lo;
+    float u = ext > 0.0f ? (c - lo) / ext : 0.0f;
+    u = fminf(fmaxf(u * 1024.0f, 0.0f), 1023.0f);
+    q[a] = (unsigned int)u;
+  }
+  code[e] = (expand_bits(q[0]) << 2) | (expand_bits(q[1]) << 1) | expand_bits(q[2]);
+  id[e] = e;
+}
+
+// Common-prefix length of sorted keys i and j (ties broken by index).
+__device__ __forceinline__ int delta(const unsigned int* code, int t, int i, int j) {
+  if (j < 0 || j >= t) return -1;
+  const unsigned int a = code[i], b = code[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((unsigned int)(i ^ j));
+}
+
+// Karras (2012) internal node i of t - 1: its key range and split. Children
+// >= 0 are internal nodes, < 0 are leaves (~leaf).
+__global__ void __launch_bounds__(256) k_karras(int t, const unsigned int* __restrict__ code, int* __restrict__ left,
+                                                int* __restrict__ right, int* __restrict__ parent,
+                                                int* __restrict__ leaf_parent, int* __restrict__ range_hi,
+                                                const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= t - 1) return;
+  const int d = (delta(code, t, i, i + 1) - delta(code, t, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(code, t, i, i - d);
+  int lmax = 2;
+  while (delta(code, t, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int s = lmax >> 1; s >= 1; s >>= 1)
+    if (delta(code, t, i, i + (l + s) * d) > dmin) l += s;
+  const int j = i + l * d;
+  const int dnode = delta(code, t, i, j);
+  int sp = 0;
+  for (int s = (l + 1) >> 1;; s = (s + 1) >> 1) {
+    if (delta(code, t, i, i + (sp + s) * d) > dnode) sp += s;
+    if (s == 1) break;
+  }
+  const int g = i + sp * d + (d < 0 ? d : 0);
+  const int lo = i < j ? i : j, hi = i < j ? j : i;
+  const int lc = lo == g ? ~g : g;
+  const int rc = hi == g + 1 ? ~(g + 1) : g + 1;
+  left[i] = lc;
+  right[i] = rc;
+  range_hi[i] = hi;
+  if (lc >= 0) parent[lc] = i; else leaf_parent[~lc] = i;
+  if (rc >= 0) parent[rc] = i; else leaf_parent[~rc] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+// Bottom-up node boxes: the second child to arrive at a node merges the
+// two boxes and continues upwards (release/acquire through the counter).
+__global__ void __launch_bounds__(256) k_bvh_bounds(int t, const Box* __restrict__ box, const int* __restrict__ id,
+                                                    const int* __restrict__ left, const int* __restrict__ right,
+                                                    const int* __restrict__ parent,
+                                                    const int* __restrict__ leaf_parent, int* __restrict__ visits,
+                                                    Box* __restrict__ node, const int* skip) {
+  if (skip && *skip) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t || t < 2) return;
+  int p = leaf_parent[k];
+  while (p >= 0) {
+    __threadfence();
+    if (atomicAdd(&visits[p], 1) == 0) return;  // the sibling has not arrived yet
+    __threadfence();
+    const int lc = left[p], rc = right[p];
+    const Box a = lc >= 0 ? node[lc] : box[id[~lc]];
+    const Box b = rc >= 0 ? node[rc] : box[id[~rc]];
+    Box m;
+    for (int q = 0; q < 3; ++q) { m.lo[q] = fminf(a.lo[q], b.lo[q]); m.hi[q] = fmaxf(a.hi[q], b.hi[q]); }
+    node[p] = m;
+    p = parent[p];
+  }
+}
+
+// Every leaf k collects the leaves j > k (Morton order) whose boxes overlap
+// its own: each unordered pair once. Pairs are stored as triangle indices
+// (min, max); the count may exceed cap (then the caller grows the buffer).
+__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict__ box, const int* __restrict__ id,
+                                                   const int* __restrict__ left, const int* __restrict__ right,
+                                                   const int* __restrict__ range_hi, const Box* __restrict__ node,
+                                                   int2* __restrict__ pairs, unsigned int cap, BvhDev* bd,
+                                                   const int* skip) {
+  if (skip && *skip) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t || t < 2) return;
+  const int me = id[k];
+  const Box q = box[me];
+  int stack[64];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int nd = stack[--sp];
+    const int ch[2] = {left[nd], right[nd]};
+    for (int c = 0; c < 2; ++c) {
+      const int x = ch[c];
+      if (x < 0) {
+        const int j = ~x;
+        if (j <= k) continue;
+        const int other = id[j];
+        if (overlaps(q, box[other])) {
+          const unsigned int slot = atomicAdd(&bd->npairs, 1u);
+          if (slot < cap) pairs[slot] = make_int2(me < other ? me : other, me < other ? other : me);
+        }
+      } else {
+        if (range_hi[x] <= k) continue;  // every leaf below precedes k
+        if (overlaps(q, node[x]) && sp < 64) stack[sp++] = x;
+      }
+    }
+  }
+}
+
+// ---- IPC step state --------------------------------------------------------
+
+// "No time of impact": 2.0, above every real step fraction; the minimum is
+// taken with atomicMin on the bits (non-negative doubles order like their bits).
+constexpr unsigned long long kNoToi = 0x4000000000000000ull;
+
+struct IpcDev {
+  double mu;                    // AL penalty
+  double alpha;                 // step fraction of the current PD iteration
+  double resid;                 // displacement residual of the current PD iteration
+  unsigned long long toi;       // min TOI as double bits, kNoToi when none
+  unsigned long long viol;      // max violation as double bits (non-negative)
+  unsigned long long active;    // active contacts of the last detection
+  unsigned long long self_contacts;
+  int al_done, pd_done;
+  int al_iters, iterations;
+  int converged, overflow;  // overflow: the candidate pairs exceeded the buffer (sticky)
+  double last_alpha;
+};
+
+// Per-detection inputs of the scene (collision_pipeline.rs:207-351).
+struct Scene {
+  int has_ground;
+  double ground;
+  int has_sphere;
+  double cx, cy, cz, radius;
+  int self_collision;
+  double thickness;
+  double dh, kappa;
+};
+
+__device__ __forceinline__ bool excluded(const int* __restrict__ xptr, const int* __restrict__ xidx, int i, int j) {
+  for (int q = xptr[i]; q < xptr[i + 1]; ++q)
+    if (xidx[q] == j) return true;
+  return false;
+}
+
+__device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
+  atomicAdd(g + 3 * i + 0, s * v.x);
+  atomicAdd(g + 3 * i + 1, s * v.y);
+  atomicAdd(g + 3 * i + 2, s * v.z);
+}
+
+// self_collision.rs:199-276 with ipc_response.rs:58-219: the vertex-triangle
+// (6) and edge-edge (9) tests of one candidate pair, barrier gradients added
+// to gself (FP64 atomics).
+__global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ pairs, const unsigned int* npairs_p,
+                                                       unsigned int cap, const int* __restrict__ tri,
+                                                       const double* __restrict__ x, const int* __restrict__ xptr,
+                                                       const int* __restrict__ xidx, Scene sc,
+                                                       double* __restrict__ gself, IpcDev* ip, const int* skip) {
+  if (skip && *skip) return;
+  const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
+  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0 && *npairs_p > cap) ip->overflow = 1;
+  if (k >= np) return;
+  const int2 pr = pairs[k];
+  const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
+  const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+  const double dh = sc.dh;
+  unsigned long long cnt = 0;
+  bool viol = false;
+  auto bgrad = [&](double d2) {
+    return sc.thickness > 0.0 ? scaled_barrier_gradient_thick(d2, dh, sc.kappa, sc.thickness)
+                              : scaled_barrier_gradient(d2, dh, sc.kappa);
+  };
+  for (int side = 0; side < 2; ++side) {
+    const int* V = side == 0 ? A : B;
+    const int* T = side == 0 ? B : A;
+    for (int q = 0; q < 3; ++q) {
+      const int v = V[q];
+      if (v == T[0] || v == T[1] || v == T[2]) continue;
+      if (excluded(xptr, xidx, v, T[0]) || excluded(xptr, xidx, v, T[1]) || excluded(xptr, xidx, v, T[2])) continue;
+      const D3 p = ld(x, v), a = ld(x, T[0]), b = ld(x, T[1]), c = ld(x, T[2]);
+      const double d2 = pt_dist2(p, a, b, c);
+      if (d2 < dh && d2 > 0.0) {
+        D3 g[4];
+        pt_grad(p, a, b, c, g);
+        const double bg = bgrad(d2);
+        atomic_add3(gself, v, g[0], bg);
+        atomic_add3(gself, T[0], g[1], bg);
+        atomic_add3(gself, T[1], g[2], bg);
+        atomic_add3(gself, T[2], g[3], bg);
+        ++cnt;
+        if (d2 <= 1e-12) viol = true;
+      }
+    }
+  }
+  const int EA[3][2] = {{A[0], A[1]}, {A[1], A[2]}, {A[2], A[0]}};
+  const int EB[3][2] = {{B[0], B[1]}, {B[1], B[2]}, {B[2], B[0]}};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      const int a0 = EA[i][0], a1 = EA[i][1], b0 = EB[j][0], b1 = EB[j][1];
+      if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+      if (excluded(xptr, xidx, a0, b0) && excluded(xptr, xidx, a0, b1) && excluded(xptr, xidx, a1, b0) &&
+          excluded(xptr, xidx, a1, b1))
+        continue;
+      const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
+      const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
+      if (d2 < dh && d2 > 0.0) {
+        D3 g[4];
+        ee_grad(pa0, pa1, pb0, pb1, g);
+        const double bg = bgrad(d2);
+        atomic_add3(gself, a0, g[0], bg);
+        atomic_add3(gself, a1, g[1], bg);
+        atomic_add3(gself, b0, g[2], bg);
+        atomic_add3(gself, b1, g[3], bg);
+        ++cnt;
+        if (d2 <= 1e-12) viol = true;
+      }
+    }
+  if (cnt) {
+    atomicAdd(&ip->self_contacts, cnt);
+    atomicAdd(&ip->active, cnt);
+  }
+  if (viol) atomicMax(&ip->viol, (unsigned long long)__double_as_longlong(1e-3));
+}
+
+// Per-vertex half of collision_pipeline.rs:207-351: self-contact normal from
+// the self gradient, ground and sphere barriers (ground_plane.rs:68-110,
+// sphere.rs:102-176), normal normalisation; then the solver's bookkeeping of
+// pd_solver.rs:640-664 / 966-978 (ever-in-contact, last normals) and, at the
+// start of an AL iteration (lambda_reset), the multiplier reset of
+// pd_solver.rs:656-663.
+__global__ void __launch_bounds__(256) k_vertex_contacts(int n, const double* __restrict__ x, Scene sc,
+                                                         const double* __restrict__ gself, double* __restrict__ grad,
+                                                         double* __restrict__ nrm, uint8_t* __restrict__ inc,
+                                                         uint8_t* __restrict__ ever, double* __restrict__ fnrm,
+                                                         double* __restrict__ lambda, int lambda_reset, IpcDev* ip,
+                                                         const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double gx = 0, gy = 0, gz = 0, nx = 0, ny = 0, nz = 0;
+  bool in = false;
+  unsigned long long act = 0;
+  double viol = 0;
+  if (sc.self_collision) {
+    const double sx = gself[3 * i], sy = gself[3 * i + 1], sz = gself[3 * i + 2];
+    gx += sx; gy += sy; gz += sz;
+    const double g2 = sx * sx + sy * sy + sz * sz;
+    if (g2 > 1e-12) {
+      const double gl = sqrt(g2);
+      nx += sx / gl; ny += sy / gl; nz += sz / gl;
+      in = true;
+    }
+  }
+  const double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+  const double dh = sc.dh, kappa = sc.kappa;
+  if (sc.has_ground) {
+    const double ds = py - sc.ground;
+    if (ds <= 0.0) {
+      viol = fmax(viol, -ds);
+      const double dcl = fmax(-ds, 1e-6);
+      ++act;
+      gy += scaled_barrier_gradient(dcl * dcl, dh, kappa) * 2.0 * dcl;
+    } else if (ds * ds < dh) {
+      ++act;
+      gy += scaled_barrier_gradient(ds * ds, dh, kappa) * 2.0 * ds;
+    }
+    if (ds * ds < dh || ds <= 0.0) { ny += 1.0; in = true; }
+  }
+  if (sc.has_sphere) {
+    const double dx = px - sc.cx, dy = py - sc.cy, dz = pz - sc.cz;
+    const double r = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ds = r - sc.radius;
+    if (ds <= 0.0) {
+      viol = fmax(viol, -ds);
+      const double dcl = fmax(-ds, 1e-6);
+      ++act;
+      const double bg = scaled_barrier_gradient(dcl * dcl, dh, kappa);
+      if (r > 1e-6) {
+        const double f = bg * 2.0 * dcl;
+        gx += f * (dx / r); gy += f * (dy / r); gz += f * (dz / r);
+      } else {
+        gy += bg * 2.0 * dcl;
+      }
+    } else if (ds * ds < dh) {
+      ++act;
+      const double f = scaled_barrier_gradient(ds * ds, dh, kappa) * 2.0 * ds;
+      gx += f * (dx / r); gy += f * (dy / r); gz += f * (dz / r);
+    }
+    if (ds * ds < dh || ds <= 0.0) {
+      if (r > 1e-6) { nx += dx / r; ny += dy / r; nz += dz / r; }
+      in = true;
+    }
+  }
+  if (in) {
+    const double l2 = nx * nx + ny * ny + nz * nz;
+    if (l2 > 1e-12) {
+      const double inv = 1.0 / sqrt(l2);
+      nx *= inv; ny *= inv; nz *= inv;
+    }
+  }
+  grad[3 * i] = gx; grad[3 * i + 1] = gy; grad[3 * i + 2] = gz;
+  nrm[3 * i] = nx; nrm[3 * i + 1] = ny; nrm[3 * i + 2] = nz;
+  inc[i] = in ? 1 : 0;
+  if (in) {
+    ever[i] = 1;
+    fnrm[3 * i] = nx; fnrm[3 * i + 1] = ny; fnrm[3 * i + 2] = nz;
+  } else if (lambda_reset) {
+    lambda[3 * i] = 0; lambda[3 * i + 1] = 0; lambda[3 * i + 2] = 0;
+  }
+  if (act) atomicAdd(&ip->active, act);
+  if (viol > 0) atomicMax(&ip->viol, (unsigned long long)__double_as_longlong(viol));
+}
+
+// ---- per-step kernels of step_with_ipc --------------------------------------
+
+// Start of the step (pd_solver.rs:566-609): save_previous, the CFL-like
+// velocity clamp |v| <= 0.5 sqrt(d_hat) / dt for free vertices, predict
+// (state.rs:145-169, ground clamp when the scene has a ground), x = pred.
+// Also resets the step's counters.
+__global__ void __launch_bounds__(256) k_ipc_begin(int n, double* __restrict__ x, double* __restrict__ v,
+                                                   double* __restrict__ prev, double* __restrict__ pred,
+                                                   const uint8_t* __restrict__ pinned, uint8_t* __restrict__ ever,
+                                                   double dt, double gx, double gy, double gz, double max_vel,
+                                                   int has_ground, double ground, double mu0, IpcDev* ip,
+                                                   vhk::StepDev* st) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    ip->mu = mu0; ip->alpha = 1.0; ip->resid = DBL_MAX; ip->al_done = 0; ip->pd_done = 0;
+    ip->al_iters = 1; ip->iterations = 0; ip->converged = 0; ip->last_alpha = 1.0; ip->toi = kNoToi;
+    ip->viol = 0; ip->active = 0; ip->self_contacts = 0;
+    st->done = 0; st->iterations = 0; st->residual = DBL_MAX;
+  }
+  if (i >= n) return;
+  ever[i] = 0;
+  double px = x[3 * i], py = x[3 * i + 1], pz = x[3 * i + 2];
+  prev[3 * i] = px; prev[3 * i + 1] = py; prev[3 * i + 2] = pz;
+  if (pinned[i]) {
+    pred[3 * i] = px; pred[3 * i + 1] = py; pred[3 * i + 2] = pz;
+    return;
+  }
+  double vx = v[3 * i], vy = v[3 * i + 1], vz = v[3 * i + 2];
+  const double v2 = vx * vx + vy * vy + vz * vz;
+  if (v2 > max_vel * max_vel) {
+    const double s = max_vel / sqrt(v2);
+    vx *= s; vy *= s; vz *= s;
+    v[3 * i] = vx; v[3 * i + 1] = vy; v[3 * i + 2] = vz;
+  }
+  const double dt2 = dt * dt;
+  px += dt * vx + dt2 * gx;
+  py += dt * vy + dt2 * gy;
+  pz += dt * vz + dt2 * gz;
+  if (has_ground && py < ground) py = ground;
+  pred[3 * i] = px; pred[3 * i + 1] = py; pred[3 * i + 2] = pz;
+  x[3 * i] = px; x[3 * i + 1] = py; x[3 * i + 2] = pz;
+}
+
+// Counters of the detection that follows.
+__global__ void k_ipc_detect_begin(IpcDev* ip) {
+  if (threadIdx.x != 0 || ip->al_done) return;
+  ip->viol = 0; ip->active = 0; ip->self_contacts = 0;
+}
+
+// Zero fill (instead of memset, so the sequence stays capturable).
+__global__ void __launch_bounds__(256) k_zero(int count, int* __restrict__ p, const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+__global__ void __launch_bounds__(256) k_zero_d(int count, double* __restrict__ p, const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0;
+}
+
+// assemble_barrier_rhs (assembly.rs:272-282): the per-vertex RHS term
+// ext = -(mu g + lambda), frozen for the AL iteration.
+__global__ void __launch_bounds__(256) k_ipc_ext(int n, const double* __restrict__ grad,
+                                                 const double* __restrict__ lambda, double* __restrict__ ext,
+                                                 const IpcDev* ip) {
+  if (ip->al_done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * n) return;
+  ext[i] = -(ip->mu * grad[i] + lambda[i]);
+}
+
+// Per-block partial sums of one PD iteration (pd_solver.rs:873-891, 911-922):
+// |delta|^2, |x|^2 and, over free vertices, g . delta.
+__global__ void __launch_bounds__(256) k_ipc_sums(int n, const double* __restrict__ x, const double* __restrict__ dl,
+                                                  const double* __restrict__ grad, const uint8_t* __restrict__ pinned,
+                                                  double* __restrict__ partials, const IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double a = 0, b = 0, c = 0;
+  if (i < n) {
+    const double dx = dl[3 * i], dy = dl[3 * i + 1], dz = dl[3 * i + 2];
+    const double ox = x[3 * i], oy = x[3 * i + 1], oz = x[3 * i + 2];
+    a = dx * dx + dy * dy + dz * dz;
+    b = ox * ox + oy * oy + oz * oz;
+    if (!pinned[i]) c = grad[3 * i] * dx + grad[3 * i + 1] * dy + grad[3 * i + 2] * dz;
+  }
+  __shared__ double sa[256], sb[256], sc2[256];
+  sa[threadIdx.x] = a; sb[threadIdx.x] = b; sc2[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s];
+      sc2[threadIdx.x] += sc2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[3 * blockIdx.x] = sa[0]; partials[3 * blockIdx.x + 1] = sb[0]; partials[3 * blockIdx.x + 2] = sc2[0];
+  }
+}
+
+__device__ __forceinline__ void toi_min(IpcDev* ip, double t) {
+  atomicMin(&ip->toi, (unsigned long long)__double_as_longlong(t));
+}
+
+// Self part of compute_ccd_step (ccd.rs:238-292) over the candidate pairs of
+// the last detection, from x to x + delta.
+__global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pairs, const unsigned int* npairs_p,
+                                                   unsigned int cap, const int* __restrict__ tri,
+                                                   const double* __restrict__ x, const double* __restrict__ dl,
+                                                   IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
+  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  const int2 pr = pairs[k];
+  const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
+  const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+  double best = 2.0;
+  for (int side = 0; side < 2; ++side) {
+    const int* V = side == 0 ? A : B;
+    const int* T = side == 0 ? B : A;
+    for (int q = 0; q < 3; ++q) {
+      const int v = V[q];
+      if (v == T[0] || v == T[1] || v == T[2]) continue;
+      const D3 p0 = ld(x, v), a0 = ld(x, T[0]), b0 = ld(x, T[1]), c0 = ld(x, T[2]);
+      best = fmin(best, vt_ccd(p0, p0 + ld(dl, v), a0, a0 + ld(dl, T[0]), b0, b0 + ld(dl, T[1]), c0,
+                               c0 + ld(dl, T[2])));
+    }
+  }
+  if (best <= 1.0) toi_min(ip, fmax(best, 1e-6));
+}
+
+// Ground and sphere parts of compute_ccd_step (ground_plane.rs:112-134,
+// sphere.rs:178-224), from x to x + delta.
+__global__ void __launch_bounds__(256) k_ccd_vertices(int n, const double* __restrict__ x,
+                                                      const double* __restrict__ dl, Scene sc, IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double best = 2.0;
+  const double x0 = x[3 * i], y0 = x[3 * i + 1], z0 = x[3 * i + 2];
+  const double x1 = x0 + dl[3 * i], y1 = y0 + dl[3 * i + 1], z1 = z0 + dl[3 * i + 2];
+  if (sc.has_ground && y0 > sc.ground && y1 < sc.ground) {
+    const double t = (sc.ground - y0) / (y1 - y0);
+    if (t >= 0.0 && t <= 1.0) best = fmin(best, fmax(t * 0.9, 1e-6));
+  }
+  if (sc.has_sphere) {
+    const double r2 = sc.radius * sc.radius;
+    const double ax = x0 - sc.cx, ay = y0 - sc.cy, az = z0 - sc.cz;
+    if (!(ax * ax + ay * ay + az * az < r2)) {
+      const double bx = x1 - sc.cx, by = y1 - sc.cy, bz = z1 - sc.cz;
+      const double vx = bx - ax, vy = by - ay, vz = bz - az;
+      const double a = vx * vx + vy * vy + vz * vz;
+      const double b = 2.0 * (ax * vx + ay * vy + az * vz);
+      const double c = ax * ax + ay * ay + az * az - r2;
+      if (a > 1e-8) {
+        const double disc = b * b - 4.0 * a * c;
+        if (disc >= 0.0) {
+          const double t = (-b - sqrt(disc)) / (2.0 * a);
+          if (t >= 0.0 && t <= 1.0) best = fmin(best, fmax(t * 0.9, 1e-6));
+        }
+      }
+    }
+  }
+  if (best <= 1.0) toi_min(ip, best);
+}
+
+// The step fraction of one PD iteration (pd_solver.rs:873-930): residual,
+// CCD limit with the 0.8 safety factor, and the barrier backtracking.
+__global__ void __launch_bounds__(256) k_ipc_alpha(const double* __restrict__ partials, int blocks, double tol,
+                                                   IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  __shared__ double sa[256], sb[256], sc2[256];
+  double a = 0, b = 0, c = 0;
+  for (int k = threadIdx.x; k < blocks; k += blockDim.x) {
+    a += partials[3 * k]; b += partials[3 * k + 1]; c += partials[3 * k + 2];
+  }
+  sa[threadIdx.x] = a; sb[threadIdx.x] = b; sc2[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s]; sb[threadIdx.x] += sb[threadIdx.x + s];
+      sc2[threadIdx.x] += sc2[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double diff = sa[0], norm = sb[0], gdd = sc2[0];
+  const double res = norm > 1e-12 ? sqrt(diff / norm) : sqrt(diff);
+  double alpha = fmin(1.0, __longlong_as_double((long long)ip->toi));
+  alpha = fmax(alpha, 1e-6);
+  if (alpha < 1.0) alpha *= 0.8;
+  alpha = fmin(alpha, 1.0);
+  for (int bt = 0; bt < 4; ++bt) {
+    if (alpha < 0.1) break;
+    if (gdd * ip->mu > 1e-4 * diff * 100.0) alpha *= 0.5;
+    else break;
+  }
+  ip->alpha = alpha;
+  ip->last_alpha = alpha;
+  ip->resid = res;
+  ip->iterations += 1;
+  ip->toi = kNoToi;
+  if (res < tol) ip->pd_done = 2;  // set after this iteration's update
+}
+
+// x += alpha delta for free vertices (pd_solver.rs:934-950).
+__global__ void __launch_bounds__(256) k_ipc_update(int n, double* __restrict__ x, const double* __restrict__ dl,
+                                                    const uint8_t* __restrict__ pinned, IpcDev* ip) {
+  if (ip->al_done || ip->pd_done == 1) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || pinned[i]) return;
+  const double a = ip->alpha;
+  x[3 * i] += a * dl[3 * i];
+  x[3 * i + 1] += a * dl[3 * i + 1];
+  x[3 * i + 2] += a * dl[3 * i + 2];
+}
+
+// Turns a converged PD loop (pd_done = 2 after its last update) into a stop
+// of the remaining PD iterations of this AL iteration.
+__global__ void k_ipc_pd_latch(IpcDev* ip, vhk::StepDev* st) {
+  if (threadIdx.x == 0 && ip->pd_done == 2) { ip->pd_done = 1; st->done = 1; }
+}
+
+// End of an AL iteration (pd_solver.rs:996-1021), after the detection at
+// the new positions. Unless the violation is under al_tolerance: lambda +=
+// mu g on contact vertices and, when another AL iteration follows
+// (reset), lambda = 0 off contact (the reset at the start of that
+// iteration, pd_solver.rs:656-663, which reuses this detection).
+__global__ void __launch_bounds__(256) k_ipc_al_end(int n, const double* __restrict__ grad,
+                                                    const uint8_t* __restrict__ inc, double* __restrict__ lambda,
+                                                    double al_tol, int reset, const IpcDev* ip) {
+  if (ip->al_done) return;
+  if (__longlong_as_double((long long)ip->viol) < al_tol) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (inc[i]) {
+    const double mu = ip->mu;
+    lambda[3 * i] += mu * grad[3 * i];
+    lambda[3 * i + 1] += mu * grad[3 * i + 1];
+    lambda[3 * i + 2] += mu * grad[3 * i + 2];
+  } else if (reset) {
+    lambda[3 * i] = 0; lambda[3 * i + 1] = 0; lambda[3 * i + 2] = 0;
+  }
+}
+
+// The AL decision (one thread, after k_ipc_al_end): converged -> stop;
+// else mu *= growth, and the next AL iteration starts unless this was the
+// last one. The PD-loop flags are re-armed for that iteration.
+__global__ void k_ipc_al_commit(double al_tol, double growth, int last, IpcDev* ip, vhk::StepDev* st) {
+  if (threadIdx.x != 0 || ip->al_done) return;
+  if (__longlong_as_double((long long)ip->viol) < al_tol) {
+    ip->al_done = 1; ip->converged = 1; st->done = 1;
+    return;
+  }
+  ip->mu *= growth;
+  if (last) { ip->al_done = 1; st->done = 1; return; }
+  ip->al_iters += 1;
+  ip->pd_done = 0;
+  st->done = 0;
+}
+
+// End of the step (pd_solver.rs:1026-1135): velocities, the inelastic contact
+// filter with Coulomb friction and (adaptive) contact damping for vertices
+// that were ever in contact, the 5 % global damping when the last detection
+// has contacts, ground velocities, damping and Rayleigh damping.
+struct FinalArgs {
+  double inv_dt, mu_f, cdamp, cdamp_max, v_thresh, one_minus_damp, frayl, ground;
+  int adaptive, has_ground;
+};
+__global__ void __launch_bounds__(256) k_ipc_finalize(int n, const double* __restrict__ x, double* __restrict__ prev,
+                                                      double* __restrict__ v, const uint8_t* __restrict__ pinned,
+                                                      const uint8_t* __restrict__ ever,
+                                                      const double* __restrict__ fnrm, FinalArgs f,
+                                                      const IpcDev* ip) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double vx = (x[3 * i] - prev[3 * i]) * f.inv_dt;
+  double vy = (x[3 * i + 1] - prev[3 * i + 1]) * f.inv_dt;
+  double vz = (x[3 * i + 2] - prev[3 * i + 2]) * f.inv_dt;
+  const bool free_v = !pinned[i];
+  if (free_v && ever[i]) {
+    const double nx = fnrm[3 * i], ny = fnrm[3 * i + 1], nz = fnrm[3 * i + 2];
+    if (nx * nx + ny * ny + nz * nz >= 0.5) {
+      const double vn = vx * nx + vy * ny + vz * nz;
+      vx -= vn * nx; vy -= vn * ny; vz -= vn * nz;
+      const double vnm = fabs(vn);
+      const double vt = sqrt(vx * vx + vy * vy + vz * vz);
+      if (vt > 1e-8) {
+        const double fi = f.mu_f * vnm;
+        if (fi >= vt) { vx = 0; vy = 0; vz = 0; }
+        else { const double s = 1.0 - fi / vt; vx *= s; vy *= s; vz *= s; }
+      }
+      double dmp = f.cdamp;
+      if (f.adaptive) {
+        const double vm = sqrt(vx * vx + vy * vy + vz * vz);
+        const double t = fmin(fmax(1.0 - vm / fmax(f.v_thresh, 1e-6), 0.0), 1.0);
+        dmp = f.cdamp + t * (f.cdamp_max - f.cdamp);
+      }
+      vx *= 1.0 - dmp; vy *= 1.0 - dmp; vz *= 1.0 - dmp;
+    }
+  }
+  if (ip->active > 0) { vx *= 0.95; vy *= 0.95; vz *= 0.95; }
+  if (f.has_ground && free_v && x[3 * i + 1] <= f.ground) {
+    if (vy < 0) vy = 0;
+    prev[3 * i + 1] = f.ground;
+    vx *= 0.5; vz *= 0.5;
+  }
+  vx *= f.one_minus_damp; vy *= f.one_minus_damp; vz *= f.one_minus_damp;
+  if (free_v && f.frayl > 0) { vx *= f.frayl; vy *= f.frayl; vz *= f.frayl; }
+  v[3 * i] = vx; v[3 * i + 1] = vy; v[3 * i + 2] = vz;
+}
+
+}  // namespace vhc

@@ -321,7 +321,9 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, 
 }
 
 // Inputs of the RHS r = b - A x at one vertex: the inertia term
-// m_i (pred_i / dt^2 - x_i / dt_A^2) and the vertex's contiguous slots.
+// m_i (pred_i / dt^2 - x_i / dt_A^2), an optional per-vertex external term
+// (the contact step's -(mu g + lambda), assembly.rs:272-282) and the
+// vertex's contiguous slots.
 template <class T>
 struct RhsSrc {
   const int* inc_ptr;
@@ -330,14 +332,17 @@ struct RhsSrc {
   const double* pred;
   const double* x;
   double inv_dt2, inv_dtm2;
+  const double* ext;  // [n][3] or null
 };
 
 template <class T>
 __device__ __forceinline__ double3 inertia(const RhsSrc<T>& s, int i) {
   const double m = s.mass[i];
-  return make_double3(m * (s.inv_dt2 * s.pred[3 * i + 0] - s.inv_dtm2 * s.x[3 * i + 0]),
-                      m * (s.inv_dt2 * s.pred[3 * i + 1] - s.inv_dtm2 * s.x[3 * i + 1]),
-                      m * (s.inv_dt2 * s.pred[3 * i + 2] - s.inv_dtm2 * s.x[3 * i + 2]));
+  double3 r = make_double3(m * (s.inv_dt2 * s.pred[3 * i + 0] - s.inv_dtm2 * s.x[3 * i + 0]),
+                           m * (s.inv_dt2 * s.pred[3 * i + 1] - s.inv_dtm2 * s.x[3 * i + 1]),
+                           m * (s.inv_dt2 * s.pred[3 * i + 2] - s.inv_dtm2 * s.x[3 * i + 2]));
+  if (s.ext) { r.x += s.ext[3 * i + 0]; r.y += s.ext[3 * i + 1]; r.z += s.ext[3 * i + 2]; }
+  return r;
 }
 
 template <class T>

@@ -594,6 +594,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
     };
     std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return tmin(a) < tmin(b); });
     S.tri_idx.resize(3 * (size_t)t); S.tri_dm.resize(4 * (size_t)t); S.tri_w.resize(t);
+    S.tri_orig = ord;
     for (int k = 0; k < t; ++k) {
       const int e = ord[k];
       for (int c = 0; c < 3; ++c) S.tri_idx[3 * k + c] = S.iperm[in.tri[3 * e + c]];

@@ -75,6 +75,7 @@ struct Setup {
   std::vector<int> perm, iperm;  // perm[new] = old vertex id
   // elements, vertex labels already permuted
   std::vector<int32_t> tri_idx;      // 3t
+  std::vector<int32_t> tri_orig;     // t  (device triangle k is input triangle tri_orig[k])
   std::vector<double> tri_dm;        // 4t (dm_inv, column-major as the reference)
   std::vector<double> tri_w;         // t  (k * area)
   std::vector<int32_t> hinge_idx;    // 4h

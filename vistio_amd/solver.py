@@ -157,6 +157,15 @@ class SimulationState:
     pred_z: np.ndarray
     mass: np.ndarray
     inv_mass: np.ndarray
+    # augmented-Lagrangian multipliers of the contact step (state.rs:55-63)
+    al_lambda_x: np.ndarray = None
+    al_lambda_y: np.ndarray = None
+    al_lambda_z: np.ndarray = None
+
+    def __post_init__(self):
+        for nm in ("al_lambda_x", "al_lambda_y", "al_lambda_z"):
+            if getattr(self, nm) is None:
+                setattr(self, nm, np.zeros(self.vertex_count, dtype=np.float32))
 
     @staticmethod
     def from_mesh_with_masses(mesh: TriangleMesh, masses, pinned) -> "SimulationState":
@@ -187,6 +196,45 @@ class StepResult:
 
 
 _MODEL_BY_NAME = {"arap": nat.MODEL_ARAP, "co_rotational": nat.MODEL_COROTATIONAL}
+
+
+@dataclass
+class IpcConfig:
+    """Contact settings: SolverConfig's IPC fields (config.rs:47-120, defaults
+    config.rs:113-133) plus the collision pipeline the runner builds
+    (runner.rs:60-95): ground plane, analytic sphere, self-collision."""
+    ground_height: Optional[float] = None
+    sphere: Optional[tuple] = None            # ((cx, cy, cz), radius)
+    self_collision: bool = False
+    exclusion_depth: int = 1
+    barrier_d_hat: float = 1e-3
+    barrier_kappa: float = 1e4                # the handler's kappa (see estimate_initial_kappa)
+    thickness: Optional[float] = None         # None: 1.5 * barrier_d_hat (runner.rs:60)
+    al_max_iterations: int = 10
+    al_mu_initial: float = 1.0
+    al_mu_growth: float = 10.0
+    al_tolerance: float = 1e-4
+    compliant_contact: bool = True
+    compliant_d_hat_scale: float = 2.0
+    friction_coefficient: float = 0.4
+    contact_damping: float = 0.3
+    adaptive_contact_damping: bool = True
+    contact_damping_max: float = 0.5
+    contact_velocity_threshold: float = 0.1
+    max_pairs: int = 0                        # 0: 32 candidate pairs per triangle
+
+
+def estimate_initial_kappa(configured: float, avg_mass: float, gravity_mag: float, d_hat: float) -> float:
+    """barrier.rs:102-121: kappa = m g / (2 sqrt(d_hat/2) |b'(d_hat/2)|) in
+    [1, 1e6]; a configured kappa > 0 is returned as is."""
+    if configured > 0:
+        return float(configured)
+    dm = 0.5 * d_hat
+    diff = dm - d_hat
+    g = abs(-diff * (2.0 * np.log(dm / d_hat) + diff / dm))
+    if g < 1e-12:
+        return 1e4
+    return float(min(max(avg_mass * gravity_mag / (2.0 * np.sqrt(dm) * g), 1.0), 1e6))
 
 
 class HipPdSolver:
@@ -299,6 +347,101 @@ class HipPdSolver:
         r = nat.StepResultC()
         _check(self._L.vh_pd_step_resident(self._ctx, float(dt), int(nsteps), C.byref(r)), self._ctx, "step_resident")
         return StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time)
+
+    # -- contact ------------------------------------------------------------
+    def init_ipc(self, cfg: IpcConfig) -> None:
+        """Enables step_with_ipc on this solver (after init)."""
+        if not self._initialized:
+            raise InvalidConfig("Solver not initialized. Call init() first.")
+        q = nat.IpcParams()
+        q.has_ground = 0 if cfg.ground_height is None else 1
+        q.ground_height = 0.0 if cfg.ground_height is None else float(cfg.ground_height)
+        q.has_sphere = 0 if cfg.sphere is None else 1
+        if cfg.sphere is not None:
+            (cx, cy, cz), r = cfg.sphere
+            q.sphere_center[0], q.sphere_center[1], q.sphere_center[2] = cx, cy, cz
+            q.sphere_radius = float(r)
+        q.self_collision = int(bool(cfg.self_collision))
+        q.exclusion_depth = int(cfg.exclusion_depth)
+        q.d_hat, q.kappa = float(cfg.barrier_d_hat), float(cfg.barrier_kappa)
+        q.thickness = 1.5 * cfg.barrier_d_hat if cfg.thickness is None else float(cfg.thickness)
+        q.al_max_iterations = int(cfg.al_max_iterations)
+        q.al_mu_initial, q.al_mu_growth, q.al_tolerance = cfg.al_mu_initial, cfg.al_mu_growth, cfg.al_tolerance
+        q.compliant_contact, q.compliant_d_hat_scale = int(cfg.compliant_contact), cfg.compliant_d_hat_scale
+        q.friction_coefficient, q.contact_damping = cfg.friction_coefficient, cfg.contact_damping
+        q.adaptive_contact_damping = int(cfg.adaptive_contact_damping)
+        q.contact_damping_max, q.contact_velocity_threshold = cfg.contact_damping_max, cfg.contact_velocity_threshold
+        q.max_pairs = int(cfg.max_pairs)
+        _check(self._L.vh_ipc_init(self._ctx, C.byref(q)), self._ctx, "init_ipc")
+
+    def step_with_ipc(self, state: SimulationState, dt: float) -> StepResult:
+        """ProjectiveDynamicsSolver::step_with_ipc on host state."""
+        n = self._n
+        arrs = [state.pos_x, state.pos_y, state.pos_z, state.vel_x, state.vel_y, state.vel_z,
+                state.prev_x, state.prev_y, state.prev_z, state.pred_x, state.pred_y, state.pred_z, state.inv_mass]
+        lam = [state.al_lambda_x, state.al_lambda_y, state.al_lambda_z]
+        for a in arrs + lam:
+            if a.dtype != np.float32 or a.size != n or not a.flags.c_contiguous:
+                raise InvalidMesh("state arrays must be contiguous float32 of the mesh's vertex count")
+        s = nat.StateF32(*[nat.ptr(a, C.c_float) for a in arrs])
+        r = nat.StepResultC()
+        _check(self._L.vh_pd_step_ipc(self._ctx, float(dt), C.byref(s), *[nat.ptr(a, C.c_float) for a in lam],
+                                      C.byref(r)), self._ctx, "step_with_ipc")
+        return StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time)
+
+    def step_ipc_resident(self, dt: float, nsteps: int = 1):
+        """step_with_ipc with the state in HBM; returns (StepResult, stats)."""
+        r, st = nat.StepResultC(), nat.IpcStats()
+        _check(self._L.vh_pd_step_ipc_resident(self._ctx, float(dt), int(nsteps), C.byref(r), C.byref(st)),
+               self._ctx, "step_ipc_resident")
+        return (StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time),
+                {k: getattr(st, k) for k, _ in nat.IpcStats._fields_})
+
+    def detect_contacts(self, pos, d_hat: float, kappa: float) -> dict:
+        """CollisionPipeline::detect_ipc_contacts at positions (3, n), or the
+        resident state when pos is None (IpcBarrierForces, pd_solver.rs:1160-1196)."""
+        n = self._n
+        p = None if pos is None else np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3, n))
+        g, nr = np.empty((3, n)), np.empty((3, n))
+        ic = np.empty(n, dtype=np.uint8)
+        st = nat.IpcStats()
+        _check(self._L.vh_ipc_detect(self._ctx, None if p is None else nat.ptr(p, C.c_double), float(d_hat),
+                                     float(kappa), nat.ptr(g, C.c_double), nat.ptr(nr, C.c_double),
+                                     nat.ptr(ic, C.c_uint8), C.byref(st)), self._ctx, "detect_contacts")
+        return dict(grad=g, normal=nr, in_contact=ic, max_violation=st.max_violation, active=st.active,
+                    self_contacts=st.self_contacts, pairs=st.pairs)
+
+    def compute_ccd_step(self, p0, p1) -> float:
+        """CollisionPipeline::compute_ccd_step from p0 to p1 ((3, n) each)."""
+        n = self._n
+        a = np.ascontiguousarray(np.asarray(p0, dtype=np.float64).reshape(3, n))
+        b = np.ascontiguousarray(np.asarray(p1, dtype=np.float64).reshape(3, n))
+        out = C.c_double(0)
+        _check(self._L.vh_ipc_ccd_step(self._ctx, nat.ptr(a, C.c_double), nat.ptr(b, C.c_double), C.byref(out)),
+               self._ctx, "compute_ccd_step")
+        return out.value
+
+    def bvh_pairs(self, pos, margin: float) -> np.ndarray:
+        """Every overlapping pair of triangle boxes (ta < tb), sorted, (k, 2) uint32."""
+        n = self._n
+        p = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3, n))
+        cnt = C.c_uint64(0)
+        _check(self._L.vh_bvh_pairs(self._ctx, nat.ptr(p, C.c_double), float(margin), None, None, 0, C.byref(cnt)),
+               self._ctx, "bvh_pairs")
+        k = int(cnt.value)
+        ta, tb = np.empty(max(k, 1), dtype=np.uint32), np.empty(max(k, 1), dtype=np.uint32)
+        _check(self._L.vh_bvh_pairs(self._ctx, nat.ptr(p, C.c_double), float(margin), nat.ptr(ta, C.c_uint32),
+                                    nat.ptr(tb, C.c_uint32), k, C.byref(cnt)), self._ctx, "bvh_pairs")
+        return np.stack([ta[:k], tb[:k]], 1)
+
+    def set_lambda(self, xyz) -> None:
+        a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(3, self._n))
+        _check(self._L.vh_state_lambda_set_f64(self._ctx, nat.ptr(a, C.c_double)), self._ctx, "set_lambda")
+
+    def get_lambda(self) -> np.ndarray:
+        a = np.empty((3, self._n))
+        _check(self._L.vh_state_lambda_get_f64(self._ctx, nat.ptr(a, C.c_double)), self._ctx, "get_lambda")
+        return a
 
     def set_state(self, which: int, xyz) -> None:
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(3, self._n))
