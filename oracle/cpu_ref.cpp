@@ -839,10 +839,19 @@ template <class R> struct Oracle : OracleBase {
         const std::vector<R>* L[3] = {&lx, &ly, &lz};
         for (int ax = 0; ax < 3; ++ax)  // assemble_barrier_rhs, assembly.rs:272-282
           for (int i = 0; i < n; ++i) rhs[ax][i] -= mu * (*G[ax])[i] + (*L[ax])[i];
+        // sol = A^{-1} rhs, formed as pos + A^{-1} (rhs - A pos): the same
+        // vector, without the cancellation of sol - pos in the CCD and the
+        // step below (their inputs are these small differences).
+        const std::vector<R>* Pp[3] = {&px, &py, &pz};
         for (int ax = 0; ax < 3; ++ax) {
-          for (int i = 0; i < n; ++i) bd[i] = double(rhs[ax][i]);
+          const std::vector<R>& X = *Pp[ax];
+          for (int i = 0; i < n; ++i) {
+            double ax_i = 0;
+            for (int q = Ap[i]; q < Ap[i + 1]; ++q) ax_i += Ax[q] * double(X[Ai[q]]);
+            bd[i] = double(rhs[ax][i]) - ax_i;
+          }
           chol.solve(bd.data(), xd.data(), work);
-          for (int i = 0; i < n; ++i) sol[ax][i] = R(xd[i]);
+          for (int i = 0; i < n; ++i) sol[ax][i] = X[i] + R(xd[i]);
         }
         double diff = 0, norm = 0;  // pd_solver.rs:873-891
         for (int i = 0; i < n; ++i) {

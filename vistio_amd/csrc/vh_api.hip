@@ -39,6 +39,7 @@ struct IpcState {
   int *id = nullptr, *id_s = nullptr;
   int *left = nullptr, *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *range_hi = nullptr;
   int* visits = nullptr;
+  int* orig = nullptr;  // caller's index of each device triangle
   int2* pairs = nullptr;
   unsigned int cap = 0;
   vhc::BvhDev* bd = nullptr;
@@ -374,7 +375,7 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   hipLaunchKernelGGL(vhc::k_bvh_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
                      I.parent, I.leaf_parent, I.visits, I.node, skip);
   hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
-                     I.range_hi, I.node, I.pairs, I.cap, I.bd, skip);
+                     I.range_hi, I.node, I.orig, I.pairs, I.cap, I.bd, skip);
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
@@ -984,6 +985,7 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.left, (size_t)tt)); UP(dalloc(ctx, &I.right, (size_t)tt)); UP(dalloc(ctx, &I.parent, (size_t)tt));
   UP(dalloc(ctx, &I.leaf_parent, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
   UP(dalloc(ctx, &I.visits, (size_t)tt));
+  UP(upload(ctx, &I.orig, S.tri_orig));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
 #undef UP

@@ -22,6 +22,12 @@
 #include <cfloat>
 #include <cstdint>
 
+// No a*b+c contraction in this file: the contact tests (distances, the CCD
+// cubic's coefficients and roots, barycentric inside tests) take branch
+// decisions on values with heavy cancellation, and must round like the
+// reference's separate multiplies and adds. Restored at the end of the file.
+#pragma clang fp contract(off)
+
 namespace vhc {
 
 // ---- small FP64 vector ---------------------------------------------------
@@ -420,13 +426,19 @@ __global__ void __launch_bounds__(256) k_bvh_bounds(int t, const Box* __restrict
 }
 
 // Every leaf k collects the leaves j > k (Morton order) whose boxes overlap
-// its own: each unordered pair once. Pairs are stored as triangle indices
-// (min, max); the count may exceed cap (then the caller grows the buffer).
+// its own: each unordered pair once. A pair is stored as (a, b) with the
+// caller's triangle index orig[a] < orig[b]: the reference tests vertices of
+// its first triangle against the second and edges of the first against the
+// second (self_collision.rs:199-276), and for parallel edges the closest
+// points, hence the gradient, depend on that order. The count may exceed cap.
+// Stack bound: delta() strictly grows from a node to its internal children and
+// lies in [2, 63] (30-bit keys, index ties add 32), so a root-to-leaf path
+// has at most 62 internal nodes and the stack never holds more than 63.
 __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict__ box, const int* __restrict__ id,
                                                    const int* __restrict__ left, const int* __restrict__ right,
                                                    const int* __restrict__ range_hi, const Box* __restrict__ node,
-                                                   int2* __restrict__ pairs, unsigned int cap, BvhDev* bd,
-                                                   const int* skip) {
+                                                   const int* __restrict__ orig, int2* __restrict__ pairs,
+                                                   unsigned int cap, BvhDev* bd, const int* skip) {
   if (skip && *skip) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t || t < 2) return;
@@ -446,7 +458,8 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict_
         const int other = id[j];
         if (overlaps(q, box[other])) {
           const unsigned int slot = atomicAdd(&bd->npairs, 1u);
-          if (slot < cap) pairs[slot] = make_int2(me < other ? me : other, me < other ? other : me);
+          const bool first = orig[me] < orig[other];
+          if (slot < cap) pairs[slot] = make_int2(first ? me : other, first ? other : me);
         }
       } else {
         if (range_hi[x] <= k) continue;  // every leaf below precedes k
@@ -979,3 +992,5 @@ __global__ void __launch_bounds__(256) k_ipc_finalize(int n, const double* __res
 }
 
 }  // namespace vhc
+
+#pragma clang fp contract(fast)
