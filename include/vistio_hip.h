@@ -250,6 +250,13 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
  * pairs and the total count. */
 vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* ta, uint32_t* tb, uint64_t cap,
                        uint64_t* count);
+/* Intersection-freeness check of a contact scene: the number of triangle
+ * pairs that share no vertex and properly cross (an edge of one passes
+ * through the other) at pos (3n SoA, caller order; NULL = the resident
+ * state). Candidates are the margin-0 box pairs of vh_bvh_pairs. The
+ * reference has no such function; it checks what its contact scenes
+ * promise (no interpenetration) on the GPU's own state. */
+vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count);
 /* Resident multipliers (3n SoA, caller order). */
 vh_status vh_state_lambda_set_f64(vh_ctx* ctx, const double* lambda);
 vh_status vh_state_lambda_get_f64(vh_ctx* ctx, double* lambda);

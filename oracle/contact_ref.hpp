@@ -404,6 +404,31 @@ inline std::vector<std::pair<int, int>> tri_pairs(const std::vector<Box32>& box)
   return out;
 }
 
+// ---- intersection-freeness (no reference function) -------------------------
+
+// The check behind the contact scenes' promise of no interpenetration: two
+// triangles with no shared vertex cross when an edge of one properly passes
+// through the other (signs of tetrahedron volumes; touching and coplanar
+// overlap are not counted).
+inline int orient_sign(const P3<double>& a, const P3<double>& b, const P3<double>& c, const P3<double>& d) {
+  const double v = (b - a).cross(c - a).dot(d - a);
+  return (v > 0.0) - (v < 0.0);
+}
+inline bool seg_crosses_tri(const P3<double>& p, const P3<double>& q, const P3<double>& a, const P3<double>& b,
+                            const P3<double>& c) {
+  const int s1 = orient_sign(a, b, c, p), s2 = orient_sign(a, b, c, q);
+  if (s1 * s2 >= 0) return false;
+  const int t1 = orient_sign(p, q, a, b), t2 = orient_sign(p, q, b, c), t3 = orient_sign(p, q, c, a);
+  return t1 != 0 && t1 == t2 && t2 == t3;
+}
+inline bool tris_cross(const P3<double> P[3], const P3<double> Q[3]) {
+  for (int e = 0; e < 3; ++e) {
+    if (seg_crosses_tri(P[e], P[(e + 1) % 3], Q[0], Q[1], Q[2])) return true;
+    if (seg_crosses_tri(Q[e], Q[(e + 1) % 3], P[0], P[1], P[2])) return true;
+  }
+  return false;
+}
+
 // ---- exclusion.rs ---------------------------------------------------------
 
 // exclusion.rs:22-41 with topology.rs:126-141: the set of v plus every vertex

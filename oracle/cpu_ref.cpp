@@ -1164,6 +1164,37 @@ long long oc_tri_pairs(int n, const float* x, const float* y, const float* z, in
   for (long long k = 0; k < (long long)pr.size() && k < cap; ++k) { out[2 * k] = pr[k].first; out[2 * k + 1] = pr[k].second; }
   return (long long)pr.size();
 }
+// Triangle pairs with no shared vertex that properly cross, at f64
+// positions: brute = 1 tests every pair, 0 only the margin-0 box pairs.
+long long oc_crossings(int n, const double* pos, int t, const uint32_t* tri, int brute) {
+  auto P = [&](uint32_t i) { return oc::P3<double>(pos[i], pos[n + i], pos[2 * (size_t)n + i]); };
+  std::vector<std::pair<int, int>> pr;
+  if (brute) {
+    for (int a = 0; a < t; ++a)
+      for (int b = a + 1; b < t; ++b) pr.push_back({a, b});
+  } else {
+    std::vector<float> x(n), y(n), z(n);
+    for (int i = 0; i < n; ++i) { x[i] = (float)pos[i]; y[i] = (float)pos[n + i]; z[i] = (float)pos[2 * (size_t)n + i]; }
+    std::vector<oc::Box32> box(t);
+    for (int e = 0; e < t; ++e) {
+      const int v[3] = {int(tri[3 * e]), int(tri[3 * e + 1]), int(tri[3 * e + 2])};
+      box[e] = oc::tri_box(x.data(), y.data(), z.data(), v, 0.0f);
+    }
+    pr = oc::tri_pairs(box);
+  }
+  long long hits = 0;
+  for (const auto& q : pr) {
+    const uint32_t* A = tri + 3 * (size_t)q.first;
+    const uint32_t* B = tri + 3 * (size_t)q.second;
+    bool shared = false;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) shared |= A[i] == B[j];
+    if (shared) continue;
+    const oc::P3<double> PA[3] = {P(A[0]), P(A[1]), P(A[2])}, PB[3] = {P(B[0]), P(B[1]), P(B[2])};
+    hits += oc::tris_cross(PA, PB) ? 1 : 0;
+  }
+  return hits;
+}
 static oc::IpcScene scene_of(const or_ipc_params* ip) {
   oc::IpcScene sc;
   sc.has_ground = ip->has_ground != 0; sc.ground = ip->ground_height;

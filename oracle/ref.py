@@ -159,6 +159,8 @@ def lib():
         L.oc_tri_pairs.restype = C.c_longlong
         L.oc_tri_pairs.argtypes = [C.c_int, fp, fp, fp, C.c_int, up, C.c_float, C.c_int, C.POINTER(C.c_int),
                                    C.c_longlong]
+        L.oc_crossings.restype = C.c_longlong
+        L.oc_crossings.argtypes = [C.c_int, dp, C.c_int, up, C.c_int]
         L.oc_detect.argtypes = [C.c_int, dp, C.c_int, up, ip, C.c_double, C.c_double, dp, dp,
                                 C.POINTER(C.c_uint8), dp]
         L.oc_ccd_step.restype = C.c_double
@@ -424,6 +426,14 @@ def tri_pairs(px, py, pz, indices, margin, brute=False):
     L.oc_tri_pairs(x.size, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), t, _p(tri, C.c_uint32),
                    margin, int(brute), out.ctypes.data_as(C.POINTER(C.c_int)), cnt)
     return out[:cnt]
+
+
+def crossings(pos, indices, brute=True):
+    """Triangle pairs with no shared vertex that properly cross, at SoA
+    positions (3, n); brute tests every pair, else the margin-0 box pairs."""
+    p = np.ascontiguousarray(pos, dtype=np.float64)
+    tri = np.ascontiguousarray(indices, dtype=np.uint32)
+    return int(lib().oc_crossings(p.shape[1], _p(p, C.c_double), tri.size // 3, _p(tri, C.c_uint32), int(brute)))
 
 
 def detect(pos, indices, ipc: OrIpcParams, d_hat, kappa):

@@ -223,6 +223,67 @@ def test_tri_pairs_sweep_equals_all_pairs():
     assert len(s) <= len(d)
 
 
+# ---- intersection-freeness check ---------------------------------------------
+
+def _cross_np(P, Q):
+    """Independent form: an edge of one triangle meets the plane of the other
+    strictly between its ends, at a point with positive barycentrics."""
+    def seg_hits(p, q, a, b, c):
+        n = np.cross(b - a, c - a)
+        dp, dq = n @ (p - a), n @ (q - a)
+        if dp * dq >= 0:
+            return False
+        x = p + dp / (dp - dq) * (q - p)
+        w = [np.cross(b - x, c - x) @ n, np.cross(c - x, a - x) @ n, np.cross(a - x, b - x) @ n]
+        return min(w) > 0
+    return any(seg_hits(P[e], P[(e + 1) % 3], *Q) or seg_hits(Q[e], Q[(e + 1) % 3], *P) for e in range(3))
+
+
+def test_crossings_kats_and_independent_form():
+    tri = V([0, 1, 2, 3, 4, 5], dtype=np.uint32)
+    base = np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0]])
+    def count(Q):
+        return ref.crossings(np.concatenate([base, Q]).T, tri)
+    assert count(np.array([[0.2, 0.2, -0.5], [0.3, 0.2, 0.5], [0.2, 0.4, 0.5]])) == 1   # pierces the face
+    assert count(np.array([[0.2, 0.2, 0.1], [0.3, 0.2, 0.5], [0.2, 0.4, 0.5]])) == 0    # above it
+    assert count(np.array([[2.0, 2.0, -0.5], [2.1, 2.0, 0.5], [2.0, 2.1, 0.5]])) == 0   # beside it
+    assert count(np.array([[0.2, 0.2, 0.0], [0.3, 0.2, 0.5], [0.2, 0.4, 0.5]])) == 0    # touches at a vertex
+    assert count(np.array([[0.1, 0.1, 0.0], [0.6, 0.1, 0.0], [0.1, 0.6, 0.0]])) == 0    # coplanar overlap
+    # a triangle whose edge passes through the face with its vertices outside it
+    assert count(np.array([[-0.5, 0.25, -0.3], [1.5, 0.25, 0.3], [0.5, 2.0, 0.0]])) == 1
+    # pairs sharing a vertex are skipped even when they cross
+    assert ref.crossings(np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0.3, 0.3, 0.5], [0.3, 0.3, -0.5]]).T,
+                         V([0, 1, 2, 0, 3, 4], dtype=np.uint32)) == 0
+    rng = np.random.default_rng(7)
+    hits = 0
+    for _ in range(3000):
+        P, Q = rng.random((3, 3)), rng.random((3, 3))
+        want = _cross_np(P, Q)
+        got = ref.crossings(np.concatenate([P, Q]).T, tri)
+        assert got == int(want)
+        hits += got
+    assert 300 < hits < 2700
+
+
+def test_crossings_box_pairs_equal_all_pairs():
+    """Two crumpled sheets through each other (one mesh): the box-pair path
+    counts what testing every pair counts; one sheet alone has none."""
+    for c, amp, seed in ((8, 0.02, 0), (10, 0.05, 1)):
+        px, py, pz, idx = mesh.quad_grid(c, c, 1.0, 1.0)
+        rng = np.random.default_rng(seed)
+        one = np.stack([px, py, pz]).astype(np.float64)
+        one[2] = rng.normal(size=px.size) * amp
+        two = one.copy()
+        two[0] += 0.37 / c
+        two[1] += 0.23 / c
+        two[2] = rng.normal(size=px.size) * amp
+        pos = np.concatenate([one, two], 1)
+        idx2 = np.concatenate([idx, idx + px.size]).astype(np.uint32)
+        a, b = ref.crossings(pos, idx2), ref.crossings(pos, idx2, brute=False)
+        assert a == b and a > 10
+        assert ref.crossings(one, idx) == 0
+
+
 # ---- ipc_response.rs / pipeline (ipc_response_tests.rs) ----------------------
 
 def _two_triangles(z):

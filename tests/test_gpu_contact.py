@@ -220,3 +220,86 @@ def test_step_with_ipc_host_state():
     assert err < 1e-5 and lam <= 1e-3 * max(1.0, np.abs(b.get_lambda()).max())
     a.close()
     b.close()
+
+
+def test_crossings_match_oracle():
+    """The intersection-freeness check: two crumpled sheets through each
+    other (one mesh) give the oracle's all-pairs count; one sheet gives 0."""
+    from oracle import mesh, ref
+    for c, amp, seed in ((8, 0.02, 0), (12, 0.05, 1)):
+        px, py, pz, idx = mesh.quad_grid(c, c, 1.0, 1.0)
+        rng = np.random.default_rng(seed)
+        one = np.stack([px, py, pz]).astype(np.float64)
+        one[2] = rng.normal(size=px.size) * amp
+        two = one.copy()
+        two[0] += 0.37 / c
+        two[1] += 0.23 / c
+        two[2] = rng.normal(size=px.size) * amp
+        pos = np.concatenate([one, two], 1)
+        idx2 = np.concatenate([idx, idx + px.size]).astype(np.uint32)
+        hg = mesh.hinges4(idx2, pos.shape[1])
+        pin = np.zeros(pos.shape[1], dtype=np.uint8)
+        g, _ = _cfgs(self_collision=True)
+        p32 = pos.astype(np.float32)
+        s = _solver(p32[0], p32[1], p32[2], idx2, hg, pin, g)
+        want = ref.crossings(pos, idx2)
+        got = s.count_crossings(pos)
+        print(f"c={c}: crossings gpu {got} oracle {want}", flush=True)
+        assert got == want and want > 10
+        assert s.count_crossings(np.concatenate([one, one + np.array([[0.0], [0.0], [0.5]])], 1)) == 0
+        s.close()
+
+
+def _ribbon(cols, rows):
+    """A cols x rows ribbon, 0.2 m x 2.0 m, rotated 45 degrees about its short
+    (x) axis, lowest point 0.5 m above the ground y = 0."""
+    px, py, pz, idx, hg = _mesh(cols, rows, 0.2, 2.0)
+    c = np.sqrt(0.5)
+    y = py.astype(np.float64) * c
+    z = py.astype(np.float64) * c
+    y += 0.5 - y.min()
+    return px, y.astype(np.float32), z.astype(np.float32), idx, hg
+
+
+def test_ribbon_drop_parity_and_intersection_free():
+    """A ribbon dropped at 45 degrees onto the ground folds onto itself:
+    FP64 positions follow the oracle while the contact set builds up, and the
+    FP32-refined run stays intersection-free (no crossing triangle pair) at
+    the end of every step."""
+    import vistio_amd as va
+    from oracle import ref
+    px, py, pz, idx, hg = _ribbon(4, 40)
+    n = px.size
+    pin = np.zeros(n, dtype=np.uint8)
+    dh = 5e-4
+    g = va.IpcConfig(ground_height=0.0, self_collision=True, barrier_d_hat=dh, barrier_kappa=1e4,
+                     thickness=1.5 * dh, al_max_iterations=8, compliant_contact=False)
+    o = ref.ipc_params(ground=0.0, self_collision=True, d_hat=dh, kappa=1e4, thickness=1.5 * dh,
+                       al_max_iterations=8, compliant_contact=False)
+    errs, v, lam, st = _run_pair(px, py, pz, idx, hg, pin, g, o, 75)
+    # Once the ribbon lands (step ~40) the motion is chaotic: the oracle
+    # itself, restarted from positions perturbed by 1e-13 m, drifts by 1e-5 m
+    # within 30 steps. Before that the bar is 1e-8 m; after it, the GPU may
+    # drift no more than 10x the oracle's own drift.
+    a, b = _oracle(px, py, pz, idx, hg, pin), _oracle(px, py, pz, idx, hg, pin)
+    b.set(0, b.get(0) + np.random.default_rng(0).normal(size=(3, n)) * 1e-13)
+    drift = []
+    for _ in range(75):
+        a.step_ipc(o, DT)
+        b.step_ipc(o, DT)
+        drift.append(max_vertex_err(a.positions(), b.positions()))
+    print("ribbon f64: max err every 5 steps", ["%.1e" % e for e in errs[::5]], "v", v, st, flush=True)
+    print("oracle drift from 1e-13 m:       ", ["%.1e" % e for e in drift[::5]], flush=True)
+    assert max(errs[:40]) <= 1e-8 and st["self_contacts"] > 0
+    assert max(errs) <= 10 * max(drift)
+    s = _solver(px, py, pz, idx, hg, pin, g, precision="f32_ir")
+    most = 0
+    for k in range(150):
+        r, st = s.step_ipc_resident(DT, 1)
+        most = max(most, st["self_contacts"])
+        assert s.count_crossings() == 0, k
+    x = s.positions()
+    print("ribbon f32_ir: most self contacts", most, "final", st, "y range", x[1].min(), x[1].max(), flush=True)
+    assert most > 0 and x[1].min() > -1e-3
+    assert ref.crossings(x, idx) == 0
+    s.close()

@@ -28,7 +28,7 @@ EXPORTS = (
     "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
     "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
-    "vh_state_lambda_set_f64", "vh_state_lambda_get_f64",
+    "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings",
 )
 
 
@@ -135,6 +135,7 @@ def load():
                           C.c_int),
         "vh_ipc_ccd_step": ([vp, dp, dp, dp], C.c_int),
         "vh_bvh_pairs": ([vp, dp, C.c_double, up, up, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+        "vh_ipc_crossings": ([vp, dp, C.POINTER(C.c_uint64)], C.c_int),
         "vh_state_lambda_set_f64": ([vp, dp], C.c_int),
         "vh_state_lambda_get_f64": ([vp, dp], C.c_int),
     }

@@ -1145,6 +1145,32 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   return VH_OK;
 }
 
+vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
+  if (!count) return fail(ctx, VH_E_INVALID_ARG, "null count");
+  VH_HIP(hipSetDevice(ctx->device));
+  const vh::Setup& S = ctx->S;
+  IpcState& I = ctx->ipc;
+  const double* x = ctx->x;
+  if (pos) {
+    to_dev_order(S, pos, pos + S.n, pos + 2 * (size_t)S.n, ctx->hbuf);
+    VH_HIP(hipMemcpyAsync(I.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+    x = I.scratch_a;
+  }
+  enqueue_pairs(ctx, x, 0.0f, nullptr);
+  hipLaunchKernelGGL(vhc::k_tri_crossings, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, ctx->stream,
+                     I.pairs, &I.bd->npairs, I.cap, ctx->tri_idx, x, I.bd);
+  VH_HIP(hipGetLastError());
+  vhc::BvhDev h{};
+  VH_HIP(hipMemcpyAsync(&h, I.bd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (h.npairs > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  *count = h.crossings;
+  return VH_OK;
+}
+
 vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* ta, uint32_t* tb, uint64_t cap,
                        uint64_t* count) {
   if (!ctx) return VH_E_INVALID_ARG;

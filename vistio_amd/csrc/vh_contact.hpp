@@ -278,7 +278,7 @@ __device__ __forceinline__ float o2f(unsigned int u) {
 struct BvhDev {
   unsigned int scene[6];  // order-mapped lo xyz, hi xyz of the box centroids
   unsigned int npairs;    // pairs found (may exceed the capacity)
-  unsigned int pad;
+  unsigned int crossings; // triangle pairs that cross (k_tri_crossings)
 };
 
 // bvh.rs:28-42: box of f32 positions, inflated by margin. Also folds the
@@ -327,6 +327,7 @@ __global__ void k_bvh_reset(BvhDev* bd, const int* skip) {
   if (threadIdx.x != 0 || (skip && *skip)) return;
   for (int a = 0; a < 3; ++a) { bd->scene[a] = f2o(FLT_MAX); bd->scene[3 + a] = f2o(-FLT_MAX); }
   bd->npairs = 0;
+  bd->crossings = 0;
 }
 
 __device__ __forceinline__ unsigned int expand_bits(unsigned int v) {
@@ -989,6 +990,57 @@ __global__ void __launch_bounds__(256) k_ipc_finalize(int n, const double* __res
   vx *= f.one_minus_damp; vy *= f.one_minus_damp; vz *= f.one_minus_damp;
   if (free_v && f.frayl > 0) { vx *= f.frayl; vy *= f.frayl; vz *= f.frayl; }
   v[3 * i] = vx; v[3 * i + 1] = vy; v[3 * i + 2] = vz;
+}
+
+// ---- intersection-freeness -------------------------------------------------
+
+// Sign of the volume of the tetrahedron (a, b, c, d): -1, 0 or 1.
+__device__ __forceinline__ int orient(D3 a, D3 b, D3 c, D3 d) {
+  const double v = dot(cross(b - a, c - a), d - a);
+  return (v > 0.0) - (v < 0.0);
+}
+// Segment pq properly crosses triangle abc: p and q strictly on opposite
+// sides of its plane, and the line pq passes strictly inside the triangle
+// (the three volumes along pq have one strict sign).
+__device__ __forceinline__ bool seg_crosses_tri(D3 p, D3 q, D3 a, D3 b, D3 c) {
+  const int s1 = orient(a, b, c, p), s2 = orient(a, b, c, q);
+  if (s1 * s2 >= 0) return false;  // same side, or an end on the plane
+  const int t1 = orient(p, q, a, b), t2 = orient(p, q, b, c), t3 = orient(p, q, c, a);
+  return t1 != 0 && t1 == t2 && t2 == t3;
+}
+// Two triangles with no shared vertex cross when an edge of one crosses
+// the other. Touching (zero volumes) and coplanar overlap are not counted.
+__device__ __forceinline__ bool tris_cross(const D3 P[3], const D3 Q[3]) {
+  for (int e = 0; e < 3; ++e) {
+    if (seg_crosses_tri(P[e], P[(e + 1) % 3], Q[0], Q[1], Q[2])) return true;
+    if (seg_crosses_tri(Q[e], Q[(e + 1) % 3], P[0], P[1], P[2])) return true;
+  }
+  return false;
+}
+
+// Counts the candidate pairs (boxes with margin 0) whose triangles share no
+// vertex and cross: the intersection-freeness check of a contact scene.
+__global__ void __launch_bounds__(256) k_tri_crossings(const int2* __restrict__ pairs, const unsigned int* npairs_p,
+                                                       unsigned int cap, const int* __restrict__ tri,
+                                                       const double* __restrict__ x, BvhDev* bd) {
+  const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
+  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
+  bool hit = false;
+  if (k < np) {
+    const int2 pr = pairs[k];
+    const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
+    const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    bool shared = false;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) shared |= A[i] == B[j];
+    if (!shared) {
+      const D3 P[3] = {ld(x, A[0]), ld(x, A[1]), ld(x, A[2])};
+      const D3 Q[3] = {ld(x, B[0]), ld(x, B[1]), ld(x, B[2])};
+      hit = tris_cross(P, Q);
+    }
+  }
+  const unsigned long long m = __ballot(hit);
+  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&bd->crossings, (unsigned int)__popcll(m));
 }
 
 }  // namespace vhc

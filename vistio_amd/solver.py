@@ -434,6 +434,18 @@ class HipPdSolver:
                                     nat.ptr(tb, C.c_uint32), k, C.byref(cnt)), self._ctx, "bvh_pairs")
         return np.stack([ta[:k], tb[:k]], 1)
 
+    def count_crossings(self, pos=None) -> int:
+        """Triangle pairs with no shared vertex that properly cross, at pos
+        (3 x n) or at the resident positions: 0 for an intersection-free
+        contact scene."""
+        cnt = C.c_uint64(0)
+        p = None
+        if pos is not None:
+            p = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3, self._n))
+        _check(self._L.vh_ipc_crossings(self._ctx, None if p is None else nat.ptr(p, C.c_double), C.byref(cnt)),
+               self._ctx, "count_crossings")
+        return int(cnt.value)
+
     def set_lambda(self, xyz) -> None:
         a = np.ascontiguousarray(np.asarray(xyz, dtype=np.float64).reshape(3, self._n))
         _check(self._L.vh_state_lambda_set_f64(self._ctx, nat.ptr(a, C.c_double)), self._ctx, "set_lambda")
