@@ -147,6 +147,138 @@ def cpu_baseline(cells, sample_steps, dt):
                        f"{wall:.1f} s; CPU: {cpu_model()} ({os.cpu_count()} logical CPUs on the host)")
 
 
+def contact_scene(name):
+    """The two contact workloads: the sphere drape (quad_grid(512, 512, 1.5 m,
+    1.5 m) laid flat at y = 0.8 m over an analytic sphere of radius 0.5 m at
+    the origin, ground at y = -0.3 m, d_hat 1e-3, kappa 1e4, AL <= 5, 180
+    steps) and the self-fold ribbon (quad_grid(64, 640, 0.2 m, 2.0 m) turned
+    45 degrees about its short axis, lowest point 0.5 m above the ground at
+    y = 0, self-contact with 1-ring exclusion, d_hat 5e-4, AL <= 8, 300
+    steps). Both have self-collision on and compliant contact off.
+
+    d_hat is a distance here: the barrier acts on squared distances, so it
+    gets barrier_d_hat = d_hat^2, and the broad-phase margin / self-contact
+    thickness is 1.5 d_hat (the reference runner's thickness = 1.5 *
+    barrier_d_hat, read in metres). Read as m^2 instead, 1e-3 and 5e-4 would
+    switch the self-contact barrier on between vertices two rings apart on
+    the flat sheet (cells of 2.9 and 3.1 mm vs 3.2 and 2.2 cm activation
+    radii)."""
+    import vistio_amd as va
+    if name == "sphere_drape":
+        cells, w = 512, 1.5
+        m = va.quad_grid(cells, cells, w, w)
+        px = m.pos_x.astype(np.float32)
+        pz = m.pos_y.astype(np.float32)
+        py = np.full_like(px, 0.8)
+        dh, al, steps = 1e-3, 5, 180
+        ipc = va.IpcConfig(ground_height=-0.3, sphere=((0.0, 0.0, 0.0), 0.5), self_collision=True,
+                           barrier_d_hat=dh * dh, barrier_kappa=1e4, thickness=1.5 * dh, al_max_iterations=al,
+                           compliant_contact=False)
+        desc = (f"sphere_drape {cells}x{cells} cells, 1.5 m sheet flat at y 0.8 m over a sphere r 0.5 m, "
+                f"ground -0.3 m, self-collision on (1-ring exclusion), d_hat 1e-3 m (barrier_d_hat 1e-6 m^2, "
+                f"thickness 1.5e-3 m), kappa 1e4, AL <= 5")
+    elif name == "self_fold_ribbon":
+        m = va.quad_grid(64, 640, 0.2, 2.0)
+        c = np.float32(np.sqrt(0.5))
+        px = m.pos_x.astype(np.float32)
+        y = m.pos_y.astype(np.float64) * c
+        pz = (m.pos_y.astype(np.float64) * c).astype(np.float32)
+        py = (y + 0.5 - y.min()).astype(np.float32)
+        dh, al, steps = 5e-4, 8, 300
+        ipc = va.IpcConfig(ground_height=0.0, self_collision=True, exclusion_depth=1, barrier_d_hat=dh * dh,
+                           barrier_kappa=1e4, thickness=1.5 * dh, al_max_iterations=al, compliant_contact=False)
+        desc = ("self_fold_ribbon 64x640 cells, 0.2 m x 2.0 m at 45 degrees, lowest point 0.5 m above the "
+                "ground y 0, self-contact (1-ring exclusion), d_hat 5e-4 m (barrier_d_hat 2.5e-7 m^2, "
+                "thickness 7.5e-4 m), kappa 1e4, AL <= 8")
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    mesh = va.TriangleMesh(px, py, pz, m.indices)
+    return mesh, ipc, steps, desc
+
+
+def run_contact(args):
+    """Steps/s of the IPC step (state resident in HBM). Every step is one
+    call that ends with a device synchronisation; the intersection check
+    between steps is outside the timed intervals."""
+    import vistio_amd as va
+    dt = 1.0 / 60.0
+    mesh, ipc, steps, desc = contact_scene(args.workload)
+    steps = args.steps if args.steps_given else steps
+    topo = va.Topology.build(mesh)
+    pinned = np.zeros(mesh.vertex_count(), dtype=np.uint8)
+    res = {}
+    for prec in ("f32_ir", "f64"):
+        cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                              rayleigh_mass_damping=0.01, precision=prec, ir_sweeps=args.ir_sweeps,
+                              ir_final_sweeps=args.ir_final_sweeps)
+        s = va.HipPdSolver(0)
+        t0 = time.perf_counter()
+        s.init_raw(mesh, topo, cfg, pinned, "co_rotational", 1e3, 1e-4, 0.15)
+        s.init_ipc(ipc)
+        setup = time.perf_counter() - t0
+        rest = np.stack([mesh.pos_x, mesh.pos_y, mesh.pos_z]).astype(np.float64)
+        s.step_ipc_resident(dt, max(args.warmup, 1))
+        s.set_state(0, rest); s.set_state(2, rest); s.set_state(1, np.zeros_like(rest))
+        s.set_lambda(np.zeros_like(rest))
+        wall, al, most_self, most_act, cross_steps, worst = 0.0, 0, 0, 0, 0, 0
+        check = prec == "f32_ir" or args.check_all
+        for k in range(steps):
+            t0 = time.perf_counter()
+            r, st = s.step_ipc_resident(dt, 1)
+            wall += time.perf_counter() - t0
+            al += st["al_iterations"]
+            most_self = max(most_self, st["self_contacts"])
+            most_act = max(most_act, st["active"])
+            if check and (k % args.check_every == 0 or k == steps - 1):
+                c = s.count_crossings()
+                cross_steps += c > 0
+                worst = max(worst, c)
+            if k % 30 == 29:
+                print(f"[{args.workload} {prec}] step {k + 1}/{steps}: {1000 * wall / (k + 1):.2f} ms/step, "
+                      f"AL {st['al_iterations']}, contacts {st['active']} (self {st['self_contacts']})",
+                      file=sys.stderr, flush=True)
+        x = s.positions()
+        res[prec] = dict(wall=wall, x=x, al=al / steps, most_self=most_self, most_active=most_act,
+                         crossing_steps=cross_steps if check else None, worst_crossings=worst if check else None,
+                         setup=setup, last=st)
+        s.close()
+    f32, f64 = res["f32_ir"], res["f64"]
+    value = steps / f32["wall"]
+    x = f32["x"]
+    out = {
+        "metric": f"IPC steps/s ({args.workload})",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * f32["wall"] / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": desc + f"; {mesh.vertex_count()} verts, {mesh.triangle_count()} tris; co-rotational "
+                               "1e3 N/m, bending 1e-4 N m, 0.15 kg/m^2, Rayleigh 0.01, dt 1/60 s, 10 PD "
+                               "iterations per AL iteration; FP64 contact terms and CCD, FP32-refined solve"},
+        "extra": {
+            "fp64_gpu_steps_per_s": round(steps / f64["wall"], 3),
+            "al_iterations_per_step": {"fp32_ir": round(f32["al"], 3), "fp64": round(f64["al"], 3)},
+            "most_active_contacts": f32["most_active"],
+            "most_self_contacts": f32["most_self"],
+            "crossing_check_every": args.check_every,
+            "steps_with_crossings": f32["crossing_steps"],
+            "most_crossing_pairs": f32["worst_crossings"],
+            "y_min": float(x[1].min()),
+            "fp32_ir_vs_fp64_gpu_max_vertex_m": float(np.sqrt(((f32["x"] - f64["x"]) ** 2).sum(0)).max()),
+            "setup_s": round(f32["setup"], 2),
+        },
+    }
+    if args.workload == "sphere_drape":
+        out["extra"]["min_dist_to_sphere_minus_r"] = float(np.sqrt((x ** 2).sum(0)).min() - 0.5)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,12 +292,22 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first run a plain copy-like kernel (b = a + 1, float32) over this many MB, "
                          "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    ap.add_argument("--workload", default="hanging_sheet",
+                    choices=("hanging_sheet", "sphere_drape", "self_fold_ribbon"),
+                    help="hanging_sheet is the headline metric; the contact workloads print their own line")
+    ap.add_argument("--check-every", type=int, default=1,
+                    help="contact workloads: count crossing triangle pairs every this many steps")
+    ap.add_argument("--check-all", action="store_true", help="contact workloads: also check the FP64 run")
     args = ap.parse_args()
+    args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
         print(f"bench.py measures one GPU: the PD step of one sheet runs on a single MI355X; "
               f"--gpus {args.gpus} (WORLD_SIZE {world}) is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload != "hanging_sheet":
+        run_contact(args)
+        return
     dt = 1.0 / 60.0
     if args.copy_probe_mb > 0:
         copy_probe(args.copy_probe_mb)

@@ -237,10 +237,13 @@ vh_status vh_pd_step_ipc_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_st
                                   vh_ipc_stats* stats);
 /* CollisionPipeline::detect_ipc_contacts (crates/vistio-contact/src/collision_pipeline.rs:207-351)
  * at positions pos (3n SoA, caller order; NULL = the resident state) with the
- * given d_hat and kappa. Outputs (caller order): grad 3n SoA, normal 3n SoA,
- * in_contact n; any may be NULL. */
+ * given d_hat and kappa. Outputs (caller order), the fields of
+ * IpcBarrierForces (crates/vistio-solver/src/pd_solver.rs:1160-1196): grad 3n
+ * SoA, normal 3n SoA, in_contact n, hessian_diag n (ground and sphere terms
+ * kappa |b''(d^2)| 4 d^2, collision_pipeline.rs:272-276, 301-305); any may be
+ * NULL. */
 vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kappa, double* grad, double* normal,
-                        uint8_t* in_contact, vh_ipc_stats* stats);
+                        uint8_t* in_contact, double* hessian_diag, vh_ipc_stats* stats);
 /* CollisionPipeline::compute_ccd_step (collision_pipeline.rs:355-400) from p0
  * to p1 (3n SoA each), with the candidate pairs of p0. */
 vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, double* alpha);

@@ -410,8 +410,11 @@ inline std::vector<std::pair<int, int>> tri_pairs(const std::vector<Box32>& box)
 // triangles with no shared vertex cross when an edge of one properly passes
 // through the other (signs of tetrahedron volumes; touching and coplanar
 // overlap are not counted).
+// Volumes below 1e-10 of |b-a| |c-a| |d-a| are 0 (coplanar rounding noise).
 inline int orient_sign(const P3<double>& a, const P3<double>& b, const P3<double>& c, const P3<double>& d) {
-  const double v = (b - a).cross(c - a).dot(d - a);
+  const P3<double> u = b - a, w = c - a, z = d - a;
+  const double v = u.cross(w).dot(z);
+  if (v * v <= 1e-20 * (u.len2() * w.len2() * z.len2())) return 0;
   return (v > 0.0) - (v < 0.0);
 }
 inline bool seg_crosses_tri(const P3<double>& p, const P3<double>& q, const P3<double>& a, const P3<double>& b,
@@ -477,12 +480,14 @@ struct IpcScene {
 
 template <class R> struct Forces {
   std::vector<R> gx, gy, gz, nx, ny, nz;
+  std::vector<R> hd;  // hessian_diag: ground and sphere terms (collision_pipeline.rs:272-276, 301-305)
   std::vector<uint8_t> in_contact;
   R max_violation = 0;
   long long active = 0, self_contacts = 0;
   void reset(int n) {
     gx.assign(n, 0); gy.assign(n, 0); gz.assign(n, 0);
     nx.assign(n, 0); ny.assign(n, 0); nz.assign(n, 0);
+    hd.assign(n, 0);
     in_contact.assign(n, 0);
     max_violation = 0; active = 0; self_contacts = 0;
   }
@@ -571,7 +576,11 @@ void detect(int n, const R* px, const R* py, const R* pz, int /*t*/, const uint3
         F.active++;
         F.gy[i] += scaled_barrier_gradient(ds * ds, dh, kappa) * R(2) * ds;
       }
-      if (ds * ds < dh || ds <= 0) { F.ny[i] += R(1); F.in_contact[i] = 1; }
+      if (ds * ds < dh || ds <= 0) {
+        F.ny[i] += R(1); F.in_contact[i] = 1;
+        const R de = std::max(ds, R(1e-6)), d2 = de * de;
+        F.hd[i] += std::fabs(kappa * barrier_hessian(d2, dh) * R(4) * d2);
+      }
     }
   }
   if (sc.has_sphere) {
@@ -599,6 +608,8 @@ void detect(int n, const R* px, const R* py, const R* pz, int /*t*/, const uint3
       if (ds * ds < dh || ds <= 0) {
         if (r > R(1e-6)) { F.nx[i] += dx / r; F.ny[i] += dy / r; F.nz[i] += dz / r; }
         F.in_contact[i] = 1;
+        const R de = std::max(ds, R(1e-6)), d2 = de * de;
+        F.hd[i] += std::fabs(kappa * barrier_hessian(d2, dh) * R(4) * d2);
       }
     }
   }

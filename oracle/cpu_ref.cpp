@@ -1209,7 +1209,7 @@ static oc::IpcScene scene_of(const or_ipc_params* ip) {
 // d_hat and kappa as given. out_grad / out_normal: 3n SoA; in_contact: n.
 // stats = [max_violation, active, self_contacts, pairs].
 void oc_detect(int n, const double* pos, int t, const uint32_t* tri, const or_ipc_params* ip, double dh, double kappa,
-               double* out_grad, double* out_normal, uint8_t* in_contact, double* stats) {
+               double* out_grad, double* out_normal, uint8_t* in_contact, double* out_hess, double* stats) {
   const double *x = pos, *y = pos + n, *z = pos + 2 * size_t(n);
   std::vector<int> xp, xi;
   oc::exclusion_sets(n, t, tri, ip->exclusion_depth, xp, xi);
@@ -1230,6 +1230,7 @@ void oc_detect(int n, const double* pos, int t, const uint32_t* tri, const or_ip
     out_grad[i] = F.gx[i]; out_grad[n + i] = F.gy[i]; out_grad[2 * size_t(n) + i] = F.gz[i];
     out_normal[i] = F.nx[i]; out_normal[n + i] = F.ny[i]; out_normal[2 * size_t(n) + i] = F.nz[i];
     in_contact[i] = F.in_contact[i];
+    out_hess[i] = F.hd[i];
   }
   stats[0] = F.max_violation; stats[1] = double(F.active); stats[2] = double(F.self_contacts); stats[3] = double(pairs.size());
 }

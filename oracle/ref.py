@@ -162,7 +162,7 @@ def lib():
         L.oc_crossings.restype = C.c_longlong
         L.oc_crossings.argtypes = [C.c_int, dp, C.c_int, up, C.c_int]
         L.oc_detect.argtypes = [C.c_int, dp, C.c_int, up, ip, C.c_double, C.c_double, dp, dp,
-                                C.POINTER(C.c_uint8), dp]
+                                C.POINTER(C.c_uint8), dp, dp]
         L.oc_ccd_step.restype = C.c_double
         L.oc_ccd_step.argtypes = [C.c_int, dp, dp, C.c_int, up, ip]
         _lib = L
@@ -443,12 +443,13 @@ def detect(pos, indices, ipc: OrIpcParams, d_hat, kappa):
     tri = np.ascontiguousarray(indices, dtype=np.uint32)
     g, nrm = np.empty(3 * n), np.empty(3 * n)
     ic = np.empty(n, dtype=np.uint8)
+    hd = np.empty(n)
     st = np.empty(4)
     lib().oc_detect(n, _p(pos, C.c_double), tri.size // 3, _p(tri, C.c_uint32), C.byref(ipc), d_hat, kappa,
                     _p(g, C.c_double), _p(nrm, C.c_double), ic.ctypes.data_as(C.POINTER(C.c_uint8)),
-                    _p(st, C.c_double))
-    return dict(grad=g.reshape(3, n), normal=nrm.reshape(3, n), in_contact=ic, max_violation=st[0],
-                active=int(st[1]), self_contacts=int(st[2]), pairs=int(st[3]))
+                    _p(hd, C.c_double), _p(st, C.c_double))
+    return dict(grad=g.reshape(3, n), normal=nrm.reshape(3, n), in_contact=ic, hessian_diag=hd,
+                max_violation=st[0], active=int(st[1]), self_contacts=int(st[2]), pairs=int(st[3]))
 
 
 def ccd_step(p0, p1, indices, ipc: OrIpcParams):

@@ -265,6 +265,22 @@ def test_crossings_kats_and_independent_form():
     assert 300 < hits < 2700
 
 
+def test_crossings_ignore_coplanar_rounding_noise():
+    """Two triangles of one flat sheet that share no vertex, 2 mm apart, in
+    randomly rotated planes 1 m from the origin: the volumes are rounding
+    noise with random signs, and no crossing may be reported."""
+    h = 0.2 / 64
+    A = np.array([[0.0, 0.0, 0], [0.0, h, 0], [h, h, 0]])
+    B = np.array([[-h, h, 0], [-h, 2 * h, 0], [0.0, 2 * h, 0]])
+    tri = V([0, 1, 2, 3, 4, 5], dtype=np.uint32)
+    rng = np.random.default_rng(3)
+    for _ in range(2000):
+        q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+        off = rng.normal(size=3)
+        P = np.concatenate([A, B]) @ q.T + off
+        assert ref.crossings(P.T, tri) == 0
+
+
 def test_crossings_box_pairs_equal_all_pairs():
     """Two crumpled sheets through each other (one mesh): the box-pair path
     counts what testing every pair counts; one sheet alone has none."""
@@ -330,6 +346,15 @@ def test_ground_and_sphere_closed_forms():
     assert g[1] == pytest.approx(ref.scaled_barrier_gradient(4e-4, dh, kappa) * 2 * 0.02, rel=1e-14)
     assert g[2] == 0.0 and r["in_contact"].tolist() == [1, 1, 0, 1]
     assert r["max_violation"] == pytest.approx(0.02) and r["active"] == 3
+    # hessian_diag (collision_pipeline.rs:272-276): kappa |b''(d^2)| 4 d^2 with
+    # d clamped to >= 1e-6; b'' from finite differences of b'
+    def bpp(d):
+        e = 1e-7 * d
+        return (ref.barrier_gradient(d + e, dh) - ref.barrier_gradient(d - e, dh)) / (2 * e)
+    hd = r["hessian_diag"]
+    assert hd[0] == pytest.approx(kappa * abs(bpp(1e-4)) * 4e-4, rel=1e-6)
+    assert hd[1] == pytest.approx(kappa * abs(ref.barrier_hessian(1e-12, dh)) * 4e-12, rel=1e-12)
+    assert hd[2] == 0.0 and hd[3] == hd[1]
     # sphere: vertex at radius 1.01 from the centre of a unit sphere
     pos = np.zeros((3, 2))
     pos[:, 0] = [1.01, 0, 0]

@@ -119,6 +119,8 @@ def test_detect_matches_oracle():
             scale = max(1.0, np.abs(want["grad"]).max())
             assert np.abs(got["grad"] - want["grad"]).max() <= 1e-10 * scale
             assert np.abs(got["normal"] - want["normal"]).max() <= 1e-9
+            hs = max(1.0, np.abs(want["hessian_diag"]).max())
+            assert np.abs(got["hessian_diag"] - want["hessian_diag"]).max() <= 1e-10 * hs
             assert got["max_violation"] == pytest.approx(want["max_violation"], abs=1e-15)
         s.close()
 

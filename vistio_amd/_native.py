@@ -131,7 +131,7 @@ def load():
         "vh_pd_step_ipc": ([vp, C.c_double, C.POINTER(StateF32), fp, fp, fp, C.POINTER(StepResultC)], C.c_int),
         "vh_pd_step_ipc_resident": ([vp, C.c_double, C.c_uint32, C.POINTER(StepResultC), C.POINTER(IpcStats)],
                                     C.c_int),
-        "vh_ipc_detect": ([vp, dp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_uint8), C.POINTER(IpcStats)],
+        "vh_ipc_detect": ([vp, dp, C.c_double, C.c_double, dp, dp, C.POINTER(C.c_uint8), dp, C.POINTER(IpcStats)],
                           C.c_int),
         "vh_ipc_ccd_step": ([vp, dp, dp, dp], C.c_int),
         "vh_bvh_pairs": ([vp, dp, C.c_double, up, up, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),

@@ -37,8 +37,10 @@ struct IpcState {
   vhc::Box *box = nullptr, *node = nullptr;
   unsigned int *code = nullptr, *code_s = nullptr;
   int *id = nullptr, *id_s = nullptr;
-  int *left = nullptr, *right = nullptr, *parent = nullptr, *leaf_parent = nullptr, *range_hi = nullptr;
-  int* visits = nullptr;
+  int *left = nullptr, *right = nullptr, *range_lo = nullptr, *range_hi = nullptr;
+  vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t
+  int st_levels = 0;
+  int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
   int* orig = nullptr;  // caller's index of each device triangle
   int2* pairs = nullptr;
   unsigned int cap = 0;
@@ -153,6 +155,7 @@ void free_all(vh_ctx* ctx) {
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->initialized = false;
+  if (ctx->ipc.h_done) (void)hipHostFree(ctx->ipc.h_done);
   ctx->ipc = IpcState{};
 }
 
@@ -347,6 +350,11 @@ vh_status launch_step(vh_ctx* ctx, double dt) {
 
 // ---- contact step -------------------------------------------------------
 
+// Blocks of the grid-stride kernels over candidate pairs: enough to fill the
+// chip, independent of the pair capacity (a launch whose work is skipped
+// stays cheap).
+unsigned int pair_blocks(unsigned int cap) { return std::max(1u, std::min((cap + 255) / 256, 4096u)); }
+
 const int* ipc_done_flag(vh_ctx* c) {
   return reinterpret_cast<const int*>(reinterpret_cast<const char*>(c->ipc.ip) + offsetof(vhc::IpcDev, al_done));
 }
@@ -368,34 +376,40 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   size_t bytes = I.sort_bytes;
   (void)rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t, 0u, 30u,
                                   c->stream);
+  hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.st, skip);
   if (t < 2) return;
-  hipLaunchKernelGGL(vhc::k_zero, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t - 1, I.visits, skip);
   hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.right,
-                     I.parent, I.leaf_parent, I.range_hi, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
-                     I.parent, I.leaf_parent, I.visits, I.node, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.left, I.right,
+                     I.range_lo, I.range_hi, skip);
+  for (int k = 1; k < I.st_levels; ++k)
+    hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
+                       I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
+  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.st, I.range_lo,
+                     I.range_hi, I.node, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left, I.right,
                      I.range_hi, I.node, I.orig, I.pairs, I.cap, I.bd, skip);
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
-void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset_lambda, const int* skip) {
+void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset_lambda, const int* skip,
+                    double* hess = nullptr) {
   IpcState& I = c->ipc;
   const int n = c->S.n;
   hipLaunchKernelGGL(vhc::k_ipc_detect_begin, dim3(1), dim3(64), 0, c->stream, I.ip);
   if (sc.self_collision) {
     enqueue_pairs(c, x, (float)sc.thickness, skip);
     hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
-    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
                        I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.gself, I.ip, skip);
   }
   hipLaunchKernelGGL(vhc::k_vertex_contacts, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, sc, I.gself, I.grad, I.nrm,
-                     I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, skip);
+                     I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, hess, skip);
 }
 
-// step_with_ipc (pd_solver.rs:549-1157) as one launch sequence: the AL and
-// PD loops have fixed trip counts and stop through device flags.
-void enqueue_ipc_step(vh_ctx* c, double dt) {
+// step_with_ipc (pd_solver.rs:549-1157) as one launch sequence: the PD loop
+// has a fixed trip count and stops through device flags; after each AL
+// iteration the host reads the AL flag and launches no further AL
+// iteration once it is set (the kernels of one would all be skipped).
+hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   IpcState& I = c->ipc;
   const vh::Setup& S = c->S;
   const vh_pd_params& P = c->P;
@@ -420,7 +434,7 @@ void enqueue_ipc_step(vh_ctx* c, double dt) {
       hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
                          I.partials, I.ip);
       if (I.sc.self_collision)
-        hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, c->stream,
+        hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
                            I.pairs, &I.bd->npairs, I.cap, c->tri_idx, c->x, dl, I.ip);
       if (I.sc.has_ground || I.sc.has_sphere)
         hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
@@ -434,6 +448,12 @@ void enqueue_ipc_step(vh_ctx* c, double dt) {
                        Q.al_tolerance, al + 1 < Q.al_max_iterations ? 1 : 0, I.ip);
     hipLaunchKernelGGL(vhc::k_ipc_al_commit, dim3(1), dim3(64), 0, c->stream, Q.al_tolerance, Q.al_mu_growth,
                        al + 1 == Q.al_max_iterations ? 1 : 0, I.ip, c->st);
+    if (al + 1 < Q.al_max_iterations) {
+      hipError_t e = hipMemcpyAsync(I.h_done, skip, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) return e;
+      if (*I.h_done) break;
+    }
   }
   vhc::FinalArgs f{};
   f.inv_dt = 1.0 / dt; f.mu_f = Q.friction_coefficient; f.cdamp = Q.contact_damping;
@@ -442,6 +462,7 @@ void enqueue_ipc_step(vh_ctx* c, double dt) {
   f.ground = Q.ground_height; f.adaptive = Q.adaptive_contact_damping; f.has_ground = Q.has_ground;
   hipLaunchKernelGGL(vhc::k_ipc_finalize, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->prev, c->v, c->pinned,
                      I.ever, I.fnrm, f, I.ip);
+  return hipGetLastError();
 }
 
 // Reads the step result and the contact diagnostics (synchronizes).
@@ -982,12 +1003,16 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, (size_t)tt));
   UP(dalloc(ctx, &I.code, (size_t)tt)); UP(dalloc(ctx, &I.code_s, (size_t)tt));
   UP(dalloc(ctx, &I.id, (size_t)tt)); UP(dalloc(ctx, &I.id_s, (size_t)tt));
-  UP(dalloc(ctx, &I.left, (size_t)tt)); UP(dalloc(ctx, &I.right, (size_t)tt)); UP(dalloc(ctx, &I.parent, (size_t)tt));
-  UP(dalloc(ctx, &I.leaf_parent, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
-  UP(dalloc(ctx, &I.visits, (size_t)tt));
+  UP(dalloc(ctx, &I.left, (size_t)tt)); UP(dalloc(ctx, &I.right, (size_t)tt));
+  UP(dalloc(ctx, &I.range_lo, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
+  I.st_levels = 1;
+  while ((1ll << I.st_levels) <= (long long)tt) ++I.st_levels;  // levels 0 .. floor(log2 t)
+  UP(dalloc(ctx, &I.st, (size_t)I.st_levels * (size_t)tt));
   UP(upload(ctx, &I.orig, S.tri_orig));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
+  VH_HIP(hipHostMalloc((void**)&I.h_done, sizeof(int), hipHostMallocDefault));
+  *I.h_done = 0;
 #undef UP
   size_t bytes = 0;
   VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
@@ -1016,8 +1041,7 @@ vh_status vh_pd_step_ipc_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_st
   VH_HIP(hipSetDevice(ctx->device));
   const auto t0 = std::chrono::steady_clock::now();
   for (uint32_t k = 0; k < nsteps; ++k) {
-    enqueue_ipc_step(ctx, dt);
-    VH_HIP(hipGetLastError());
+    VH_HIP(enqueue_ipc_step(ctx, dt));
   }
   vh_status st = read_ipc_result(ctx, out, stats);
   if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1051,8 +1075,7 @@ vh_status vh_pd_step_ipc(vh_ctx* ctx, double dt, vh_state_f32* s, float* lambda_
   VH_HIP(hipMemcpyAsync(ctx->x, hp.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(hipMemcpyAsync(ctx->v, hv.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(hipMemcpyAsync(ctx->ipc.lambda, hl.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  enqueue_ipc_step(ctx, dt);
-  VH_HIP(hipGetLastError());
+  VH_HIP(enqueue_ipc_step(ctx, dt));
   std::vector<double> hq(n3), hr(n3);
   VH_HIP(hipMemcpyAsync(hp.data(), ctx->x, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(hv.data(), ctx->v, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -1073,7 +1096,7 @@ vh_status vh_pd_step_ipc(vh_ctx* ctx, double dt, vh_state_f32* s, float* lambda_
 }
 
 vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kappa, double* grad, double* normal,
-                        uint8_t* in_contact, vh_ipc_stats* stats) {
+                        uint8_t* in_contact, double* hessian_diag, vh_ipc_stats* stats) {
   if (!ctx) return VH_E_INVALID_ARG;
   if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
   if (!ctx->ipc.on) return fail(ctx, VH_E_NOT_INITIALIZED, "contact not initialized: call vh_ipc_init");
@@ -1092,11 +1115,12 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   sc.dh = d_hat;
   sc.kappa = kappa;
   hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, ctx->ipc.ip);
-  enqueue_detect(ctx, x, sc, false, nullptr);
+  enqueue_detect(ctx, x, sc, false, nullptr, ctx->ipc.scratch_b);
   VH_HIP(hipGetLastError());
   const size_t n3 = 3 * (size_t)n;
-  std::vector<double> g(n3), nr(n3);
+  std::vector<double> g(n3), nr(n3), hd(n);
   std::vector<uint8_t> ic(n);
+  VH_HIP(hipMemcpyAsync(hd.data(), ctx->ipc.scratch_b, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(g.data(), ctx->ipc.grad, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(nr.data(), ctx->ipc.nrm, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(ic.data(), ctx->ipc.inc, (size_t)n, hipMemcpyDeviceToHost, ctx->stream));
@@ -1106,6 +1130,8 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   if (normal) from_dev_order(S, nr, normal, normal + n, normal + 2 * (size_t)n);
   if (in_contact)
     for (int k = 0; k < n; ++k) in_contact[S.perm[k]] = ic[k];
+  if (hessian_diag)
+    for (int k = 0; k < n; ++k) hessian_diag[S.perm[k]] = hd[k];
   return VH_OK;
 }
 
@@ -1127,7 +1153,7 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
   if (I.sc.self_collision) {
     enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
-    hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, ctx->stream, I.pairs,
+    hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream, I.pairs,
                        &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ip);
   }
   if (I.sc.has_ground || I.sc.has_sphere)
@@ -1160,7 +1186,7 @@ vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
     x = I.scratch_a;
   }
   enqueue_pairs(ctx, x, 0.0f, nullptr);
-  hipLaunchKernelGGL(vhc::k_tri_crossings, dim3(std::max(1u, (I.cap + 255) / 256)), dim3(256), 0, ctx->stream,
+  hipLaunchKernelGGL(vhc::k_tri_crossings, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream,
                      I.pairs, &I.bd->npairs, I.cap, ctx->tri_idx, x, I.bd);
   VH_HIP(hipGetLastError());
   vhc::BvhDev h{};

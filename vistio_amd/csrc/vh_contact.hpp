@@ -10,8 +10,8 @@
 // pairs are exactly the overlapping pairs of the reference's f32 boxes.
 //
 // Broad phase: Karras' LBVH (Morton codes of box centroids, radix-sorted;
-// internal nodes from common-prefix splits; bottom-up bounds with one
-// arrival counter per node). Every leaf traverses the tree for leaves that
+// internal nodes from common-prefix splits; node bounds as unions over their
+// leaf ranges from a sparse table). Every leaf traverses the tree for leaves that
 // follow it in Morton order, so each overlapping pair is emitted once. The
 // reference's recursive median-split query drops the pairs whose left-subtree
 // index is larger (bvh.rs:207); here the pair set is complete.
@@ -57,6 +57,12 @@ __device__ __forceinline__ double barrier_gradient(double d, double dh) {
   if (dc >= dh) return 0.0;
   const double diff = dc - dh;
   return -diff * (2.0 * log(dc / dh) + diff / dc);
+}
+// barrier.rs:53-63: d2b/dd2, 0 outside (0, dh).
+__device__ __forceinline__ double barrier_hessian(double d, double dh) {
+  if (d >= dh || d <= 0.0) return 0.0;
+  const double diff = d - dh, inv = 1.0 / d;
+  return -2.0 * log(d / dh) - 4.0 * diff * inv + diff * diff * inv * inv;
 }
 // barrier.rs:82-93
 __device__ __forceinline__ double scaled_barrier_gradient(double d, double dh, double kappa) {
@@ -366,12 +372,11 @@ __device__ __forceinline__ int delta(const unsigned int* code, int t, int i, int
   return 32 + __clz((unsigned int)(i ^ j));
 }
 
-// Karras (2012) internal node i of t - 1: its key range and split. Children
-// >= 0 are internal nodes, < 0 are leaves (~leaf).
+// Karras (2012) internal node i of t - 1: its key range [lo, hi] and split.
+// Children >= 0 are internal nodes, < 0 are leaves (~leaf).
 __global__ void __launch_bounds__(256) k_karras(int t, const unsigned int* __restrict__ code, int* __restrict__ left,
-                                                int* __restrict__ right, int* __restrict__ parent,
-                                                int* __restrict__ leaf_parent, int* __restrict__ range_hi,
-                                                const int* skip) {
+                                                int* __restrict__ right, int* __restrict__ range_lo,
+                                                int* __restrict__ range_hi, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= t - 1) return;
@@ -391,39 +396,62 @@ __global__ void __launch_bounds__(256) k_karras(int t, const unsigned int* __res
   }
   const int g = i + sp * d + (d < 0 ? d : 0);
   const int lo = i < j ? i : j, hi = i < j ? j : i;
-  const int lc = lo == g ? ~g : g;
-  const int rc = hi == g + 1 ? ~(g + 1) : g + 1;
-  left[i] = lc;
-  right[i] = rc;
+  left[i] = lo == g ? ~g : g;
+  right[i] = hi == g + 1 ? ~(g + 1) : g + 1;
+  range_lo[i] = lo;
   range_hi[i] = hi;
-  if (lc >= 0) parent[lc] = i; else leaf_parent[~lc] = i;
-  if (rc >= 0) parent[rc] = i; else leaf_parent[~rc] = i;
-  if (i == 0) parent[0] = -1;
 }
 
-// Bottom-up node boxes: the second child to arrive at a node merges the
-// two boxes and continues upwards (release/acquire through the counter).
-__global__ void __launch_bounds__(256) k_bvh_bounds(int t, const Box* __restrict__ box, const int* __restrict__ id,
-                                                    const int* __restrict__ left, const int* __restrict__ right,
-                                                    const int* __restrict__ parent,
-                                                    const int* __restrict__ leaf_parent, int* __restrict__ visits,
-                                                    Box* __restrict__ node, const int* skip) {
+// Node bounds without inter-thread communication. Internal node i covers
+// the sorted leaves [lo, hi], so its box is the union of those leaf boxes:
+// a sparse table st[k][j] = union of leaves j .. j + 2^k - 1 answers it with
+// two overlapping windows of length 2^floor(log2(hi - lo + 1)).
+__device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
+  Box m;
+  for (int q = 0; q < 3; ++q) { m.lo[q] = fminf(a.lo[q], b.lo[q]); m.hi[q] = fmaxf(a.hi[q], b.hi[q]); }
+  return m;
+}
+// Level 0: the leaf boxes in Morton order.
+__global__ void __launch_bounds__(256) k_st_leaves(int t, const Box* __restrict__ box, const int* __restrict__ id,
+                                                   Box* __restrict__ st0, const int* skip) {
   if (skip && *skip) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= t || t < 2) return;
-  int p = leaf_parent[k];
-  while (p >= 0) {
-    __threadfence();
-    if (atomicAdd(&visits[p], 1) == 0) return;  // the sibling has not arrived yet
-    __threadfence();
-    const int lc = left[p], rc = right[p];
-    const Box a = lc >= 0 ? node[lc] : box[id[~lc]];
-    const Box b = rc >= 0 ? node[rc] : box[id[~rc]];
-    Box m;
-    for (int q = 0; q < 3; ++q) { m.lo[q] = fminf(a.lo[q], b.lo[q]); m.hi[q] = fmaxf(a.hi[q], b.hi[q]); }
-    node[p] = m;
-    p = parent[p];
-  }
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < t) st0[j] = box[id[j]];
+}
+// Level k from level k - 1 (half = 2^(k-1)); windows reaching past t are
+// never queried.
+__global__ void __launch_bounds__(256) k_st_level(int t, int half, const Box* __restrict__ prev,
+                                                  Box* __restrict__ cur, const int* skip) {
+  if (skip && *skip) return;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= t) return;
+  const int o = j + half < t ? j + half : j;
+  cur[j] = box_union(prev[j], prev[o]);
+}
+__global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st,
+                                                     const int* __restrict__ range_lo,
+                                                     const int* __restrict__ range_hi, Box* __restrict__ node,
+                                                     const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= t - 1) return;
+  const int lo = range_lo[i], hi = range_hi[i];
+  const int k = 31 - __clz((unsigned int)(hi - lo + 1));
+  const Box* lv = st + (size_t)k * (size_t)t;
+  node[i] = box_union(lv[lo], lv[hi - (1 << k) + 1]);
+}
+
+// One slot of a global counter per calling lane, one atomic per wave: the
+// lanes active here take consecutive slots from their leader's reservation.
+__device__ __forceinline__ unsigned int wave_reserve(unsigned int* counter) {
+  const unsigned long long m = __ballot(1);
+  const unsigned int lane = __lane_id();
+  const unsigned int leader = (unsigned int)__ffsll((long long)m) - 1u;
+  const unsigned int rank = (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
+  unsigned int base = 0;
+  if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
+  base = __shfl(base, (int)leader);
+  return base + rank;
 }
 
 // Every leaf k collects the leaves j > k (Morton order) whose boxes overlap
@@ -435,7 +463,7 @@ __global__ void __launch_bounds__(256) k_bvh_bounds(int t, const Box* __restrict
 // Stack bound: delta() strictly grows from a node to its internal children and
 // lies in [2, 63] (30-bit keys, index ties add 32), so a root-to-leaf path
 // has at most 62 internal nodes and the stack never holds more than 63.
-__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict__ box, const int* __restrict__ id,
+__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict__ st0, const int* __restrict__ id,
                                                    const int* __restrict__ left, const int* __restrict__ right,
                                                    const int* __restrict__ range_hi, const Box* __restrict__ node,
                                                    const int* __restrict__ orig, int2* __restrict__ pairs,
@@ -444,7 +472,8 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict_
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t || t < 2) return;
   const int me = id[k];
-  const Box q = box[me];
+  const int ome = orig[me];
+  const Box q = st0[k];
   int stack[64];
   int sp = 0;
   stack[sp++] = 0;
@@ -456,10 +485,10 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict_
       if (x < 0) {
         const int j = ~x;
         if (j <= k) continue;
-        const int other = id[j];
-        if (overlaps(q, box[other])) {
-          const unsigned int slot = atomicAdd(&bd->npairs, 1u);
-          const bool first = orig[me] < orig[other];
+        if (overlaps(q, st0[j])) {
+          const int other = id[j];
+          const unsigned int slot = wave_reserve(&bd->npairs);
+          const bool first = ome < orig[other];
           if (slot < cap) pairs[slot] = make_int2(first ? me : other, first ? other : me);
         }
       } else {
@@ -523,69 +552,81 @@ __global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ 
                                                        double* __restrict__ gself, IpcDev* ip, const int* skip) {
   if (skip && *skip) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
-  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k == 0 && *npairs_p > cap) ip->overflow = 1;
-  if (k >= np) return;
-  const int2 pr = pairs[k];
-  const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
-  const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+  if (blockIdx.x == 0 && threadIdx.x == 0 && *npairs_p > cap) ip->overflow = 1;
   const double dh = sc.dh;
-  unsigned long long cnt = 0;
-  bool viol = false;
   auto bgrad = [&](double d2) {
     return sc.thickness > 0.0 ? scaled_barrier_gradient_thick(d2, dh, sc.kappa, sc.thickness)
                               : scaled_barrier_gradient(d2, dh, sc.kappa);
   };
-  for (int side = 0; side < 2; ++side) {
-    const int* V = side == 0 ? A : B;
-    const int* T = side == 0 ? B : A;
-    for (int q = 0; q < 3; ++q) {
-      const int v = V[q];
-      if (v == T[0] || v == T[1] || v == T[2]) continue;
-      if (excluded(xptr, xidx, v, T[0]) || excluded(xptr, xidx, v, T[1]) || excluded(xptr, xidx, v, T[2])) continue;
-      const D3 p = ld(x, v), a = ld(x, T[0]), b = ld(x, T[1]), c = ld(x, T[2]);
-      const double d2 = pt_dist2(p, a, b, c);
-      if (d2 < dh && d2 > 0.0) {
-        D3 g[4];
-        pt_grad(p, a, b, c, g);
-        const double bg = bgrad(d2);
-        atomic_add3(gself, v, g[0], bg);
-        atomic_add3(gself, T[0], g[1], bg);
-        atomic_add3(gself, T[1], g[2], bg);
-        atomic_add3(gself, T[2], g[3], bg);
-        ++cnt;
-        if (d2 <= 1e-12) viol = true;
+  unsigned long long cnt = 0;
+  bool viol = false;
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+    const int2 pr = pairs[k];
+    const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
+    const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    // Every exclusion test of the pair is between a vertex of A and one of
+    // B, and the relation is symmetric (within `depth` rings): bit 3i + j
+    // holds excluded(A[i], B[j]).
+    unsigned int ex = 0;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) ex |= (excluded(xptr, xidx, A[a], B[b]) ? 1u : 0u) << (3 * a + b);
+    for (int side = 0; side < 2; ++side) {
+      const int* V = side == 0 ? A : B;
+      const int* T = side == 0 ? B : A;
+      for (int q = 0; q < 3; ++q) {
+        const int v = V[q];
+        if (v == T[0] || v == T[1] || v == T[2]) continue;
+        const unsigned int row = side == 0 ? (ex >> (3 * q)) & 7u
+                                           : ((ex >> q) & 1u) | ((ex >> (q + 2)) & 2u) | ((ex >> (q + 4)) & 4u);
+        if (row) continue;
+        const D3 p = ld(x, v), a = ld(x, T[0]), b = ld(x, T[1]), c = ld(x, T[2]);
+        const double d2 = pt_dist2(p, a, b, c);
+        if (d2 < dh && d2 > 0.0) {
+          D3 g[4];
+          pt_grad(p, a, b, c, g);
+          const double bg = bgrad(d2);
+          atomic_add3(gself, v, g[0], bg);
+          atomic_add3(gself, T[0], g[1], bg);
+          atomic_add3(gself, T[1], g[2], bg);
+          atomic_add3(gself, T[2], g[3], bg);
+          ++cnt;
+          if (d2 <= 1e-12) viol = true;
+        }
       }
     }
-  }
-  const int EA[3][2] = {{A[0], A[1]}, {A[1], A[2]}, {A[2], A[0]}};
-  const int EB[3][2] = {{B[0], B[1]}, {B[1], B[2]}, {B[2], B[0]}};
-  for (int i = 0; i < 3; ++i)
-    for (int j = 0; j < 3; ++j) {
-      const int a0 = EA[i][0], a1 = EA[i][1], b0 = EB[j][0], b1 = EB[j][1];
-      if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-      if (excluded(xptr, xidx, a0, b0) && excluded(xptr, xidx, a0, b1) && excluded(xptr, xidx, a1, b0) &&
-          excluded(xptr, xidx, a1, b1))
-        continue;
-      const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
-      const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
-      if (d2 < dh && d2 > 0.0) {
-        D3 g[4];
-        ee_grad(pa0, pa1, pb0, pb1, g);
-        const double bg = bgrad(d2);
-        atomic_add3(gself, a0, g[0], bg);
-        atomic_add3(gself, a1, g[1], bg);
-        atomic_add3(gself, b0, g[2], bg);
-        atomic_add3(gself, b1, g[3], bg);
-        ++cnt;
-        if (d2 <= 1e-12) viol = true;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const int i1 = i == 2 ? 0 : i + 1, j1 = j == 2 ? 0 : j + 1;
+        const int a0 = A[i], a1 = A[i1], b0 = B[j], b1 = B[j1];
+        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+        const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
+                                  (1u << (3 * i1 + j1));
+        if ((ex & need) == need) continue;
+        const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
+        const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
+        if (d2 < dh && d2 > 0.0) {
+          D3 g[4];
+          ee_grad(pa0, pa1, pb0, pb1, g);
+          const double bg = bgrad(d2);
+          atomic_add3(gself, a0, g[0], bg);
+          atomic_add3(gself, a1, g[1], bg);
+          atomic_add3(gself, b0, g[2], bg);
+          atomic_add3(gself, b1, g[3], bg);
+          ++cnt;
+          if (d2 <= 1e-12) viol = true;
+        }
       }
-    }
-  if (cnt) {
-    atomicAdd(&ip->self_contacts, cnt);
-    atomicAdd(&ip->active, cnt);
   }
-  if (viol) atomicMax(&ip->viol, (unsigned long long)__double_as_longlong(1e-3));
+  // one counter update per wave
+  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  const bool any_viol = __ballot(viol) != 0;
+  if ((threadIdx.x & 63) == 0) {
+    if (cnt) {
+      atomicAdd(&ip->self_contacts, cnt);
+      atomicAdd(&ip->active, cnt);
+    }
+    if (any_viol) atomicMax(&ip->viol, (unsigned long long)__double_as_longlong(1e-3));
+  }
 }
 
 // Per-vertex half of collision_pipeline.rs:207-351: self-contact normal from
@@ -599,11 +640,11 @@ __global__ void __launch_bounds__(256) k_vertex_contacts(int n, const double* __
                                                          double* __restrict__ nrm, uint8_t* __restrict__ inc,
                                                          uint8_t* __restrict__ ever, double* __restrict__ fnrm,
                                                          double* __restrict__ lambda, int lambda_reset, IpcDev* ip,
-                                                         const int* skip) {
+                                                         double* __restrict__ hess, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double gx = 0, gy = 0, gz = 0, nx = 0, ny = 0, nz = 0;
+  double gx = 0, gy = 0, gz = 0, nx = 0, ny = 0, nz = 0, hd = 0;
   bool in = false;
   unsigned long long act = 0;
   double viol = 0;
@@ -630,7 +671,12 @@ __global__ void __launch_bounds__(256) k_vertex_contacts(int n, const double* __
       ++act;
       gy += scaled_barrier_gradient(ds * ds, dh, kappa) * 2.0 * ds;
     }
-    if (ds * ds < dh || ds <= 0.0) { ny += 1.0; in = true; }
+    if (ds * ds < dh || ds <= 0.0) {
+      ny += 1.0;
+      in = true;
+      const double de = fmax(ds, 1e-6), d2 = de * de;  // collision_pipeline.rs:272-276
+      hd += fabs(kappa * barrier_hessian(d2, dh) * 4.0 * d2);
+    }
   }
   if (sc.has_sphere) {
     const double dx = px - sc.cx, dy = py - sc.cy, dz = pz - sc.cz;
@@ -655,8 +701,11 @@ __global__ void __launch_bounds__(256) k_vertex_contacts(int n, const double* __
     if (ds * ds < dh || ds <= 0.0) {
       if (r > 1e-6) { nx += dx / r; ny += dy / r; nz += dz / r; }
       in = true;
+      const double de = fmax(ds, 1e-6), d2 = de * de;  // collision_pipeline.rs:301-305
+      hd += fabs(kappa * barrier_hessian(d2, dh) * 4.0 * d2);
     }
   }
+  if (hess) hess[i] = hd;
   if (in) {
     const double l2 = nx * nx + ny * ny + nz * nz;
     if (l2 > 1e-12) {
@@ -727,11 +776,6 @@ __global__ void k_ipc_detect_begin(IpcDev* ip) {
 }
 
 // Zero fill (instead of memset, so the sequence stays capturable).
-__global__ void __launch_bounds__(256) k_zero(int count, int* __restrict__ p, const int* skip) {
-  if (skip && *skip) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) p[i] = 0;
-}
 __global__ void __launch_bounds__(256) k_zero_d(int count, double* __restrict__ p, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -791,24 +835,25 @@ __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pair
                                                    IpcDev* ip) {
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
-  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= np) return;
-  const int2 pr = pairs[k];
-  const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
-  const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
   double best = 2.0;
-  for (int side = 0; side < 2; ++side) {
-    const int* V = side == 0 ? A : B;
-    const int* T = side == 0 ? B : A;
-    for (int q = 0; q < 3; ++q) {
-      const int v = V[q];
-      if (v == T[0] || v == T[1] || v == T[2]) continue;
-      const D3 p0 = ld(x, v), a0 = ld(x, T[0]), b0 = ld(x, T[1]), c0 = ld(x, T[2]);
-      best = fmin(best, vt_ccd(p0, p0 + ld(dl, v), a0, a0 + ld(dl, T[0]), b0, b0 + ld(dl, T[1]), c0,
-                               c0 + ld(dl, T[2])));
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+    const int2 pr = pairs[k];
+    const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
+    const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    for (int side = 0; side < 2; ++side) {
+      const int* V = side == 0 ? A : B;
+      const int* T = side == 0 ? B : A;
+      for (int q = 0; q < 3; ++q) {
+        const int v = V[q];
+        if (v == T[0] || v == T[1] || v == T[2]) continue;
+        const D3 p0 = ld(x, v), a0 = ld(x, T[0]), b0 = ld(x, T[1]), c0 = ld(x, T[2]);
+        best = fmin(best, vt_ccd(p0, p0 + ld(dl, v), a0, a0 + ld(dl, T[0]), b0, b0 + ld(dl, T[1]), c0,
+                                 c0 + ld(dl, T[2])));
+      }
     }
   }
-  if (best <= 1.0) toi_min(ip, fmax(best, 1e-6));
+  for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o));
+  if ((threadIdx.x & 63) == 0 && best <= 1.0) toi_min(ip, fmax(best, 1e-6));
 }
 
 // Ground and sphere parts of compute_ccd_step (ground_plane.rs:112-134,
@@ -994,9 +1039,13 @@ __global__ void __launch_bounds__(256) k_ipc_finalize(int n, const double* __res
 
 // ---- intersection-freeness -------------------------------------------------
 
-// Sign of the volume of the tetrahedron (a, b, c, d): -1, 0 or 1.
+// Sign of the volume of the tetrahedron (a, b, c, d): -1, 0 or 1. A volume
+// below 1e-10 of |b-a| |c-a| |d-a| counts as 0: for (nearly) coplanar points
+// the computed volume is rounding noise with a random sign.
 __device__ __forceinline__ int orient(D3 a, D3 b, D3 c, D3 d) {
-  const double v = dot(cross(b - a, c - a), d - a);
+  const D3 u = b - a, w = c - a, z = d - a;
+  const double v = dot(cross(u, w), z);
+  if (v * v <= 1e-20 * (len2(u) * len2(w) * len2(z))) return 0;
   return (v > 0.0) - (v < 0.0);
 }
 // Segment pq properly crosses triangle abc: p and q strictly on opposite
@@ -1024,9 +1073,8 @@ __global__ void __launch_bounds__(256) k_tri_crossings(const int2* __restrict__ 
                                                        unsigned int cap, const int* __restrict__ tri,
                                                        const double* __restrict__ x, BvhDev* bd) {
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
-  const unsigned int k = blockIdx.x * blockDim.x + threadIdx.x;
-  bool hit = false;
-  if (k < np) {
+  unsigned int hits = 0;
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
     const int2 pr = pairs[k];
     const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
     const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
@@ -1036,11 +1084,11 @@ __global__ void __launch_bounds__(256) k_tri_crossings(const int2* __restrict__ 
     if (!shared) {
       const D3 P[3] = {ld(x, A[0]), ld(x, A[1]), ld(x, A[2])};
       const D3 Q[3] = {ld(x, B[0]), ld(x, B[1]), ld(x, B[2])};
-      hit = tris_cross(P, Q);
+      hits += tris_cross(P, Q) ? 1u : 0u;
     }
   }
-  const unsigned long long m = __ballot(hit);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&bd->crossings, (unsigned int)__popcll(m));
+  for (int o = 32; o > 0; o >>= 1) hits += __shfl_xor(hits, o);
+  if ((threadIdx.x & 63) == 0 && hits) atomicAdd(&bd->crossings, hits);
 }
 
 }  // namespace vhc

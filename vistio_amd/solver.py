@@ -404,12 +404,14 @@ class HipPdSolver:
         p = None if pos is None else np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(3, n))
         g, nr = np.empty((3, n)), np.empty((3, n))
         ic = np.empty(n, dtype=np.uint8)
+        hd = np.empty(n)
         st = nat.IpcStats()
         _check(self._L.vh_ipc_detect(self._ctx, None if p is None else nat.ptr(p, C.c_double), float(d_hat),
                                      float(kappa), nat.ptr(g, C.c_double), nat.ptr(nr, C.c_double),
-                                     nat.ptr(ic, C.c_uint8), C.byref(st)), self._ctx, "detect_contacts")
-        return dict(grad=g, normal=nr, in_contact=ic, max_violation=st.max_violation, active=st.active,
-                    self_contacts=st.self_contacts, pairs=st.pairs)
+                                     nat.ptr(ic, C.c_uint8), nat.ptr(hd, C.c_double), C.byref(st)),
+               self._ctx, "detect_contacts")
+        return dict(grad=g, normal=nr, in_contact=ic, hessian_diag=hd, max_violation=st.max_violation,
+                    active=st.active, self_contacts=st.self_contacts, pairs=st.pairs)
 
     def compute_ccd_step(self, p0, p1) -> float:
         """CollisionPipeline::compute_ccd_step from p0 to p1 ((3, n) each)."""
