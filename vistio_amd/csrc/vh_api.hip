@@ -19,6 +19,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "vh_kernels.hpp"
 #include "vh_contact.hpp"
@@ -34,13 +35,19 @@ struct IpcState {
   double *partials = nullptr, *scratch_a = nullptr, *scratch_b = nullptr;
   uint8_t *inc = nullptr, *ever = nullptr;
   int *xptr = nullptr, *xidx = nullptr;
-  vhc::Box *box = nullptr, *node = nullptr;
+  vhc::Box* box = nullptr;
+  vhc::Node* node = nullptr;
+  vhc::Leaf* leaf = nullptr;
   unsigned int *code = nullptr, *code_s = nullptr;
   int *id = nullptr, *id_s = nullptr;
-  int *left = nullptr, *right = nullptr, *range_lo = nullptr, *range_hi = nullptr;
+  int *left = nullptr, *range_lo = nullptr, *range_hi = nullptr;
   vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t
   int st_levels = 0;
   int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
+  unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
+  float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
+  void* scan_tmp = nullptr;
+  size_t scan_bytes = 0;
   int* orig = nullptr;  // caller's index of each device triangle
   int2* pairs = nullptr;
   unsigned int cap = 0;
@@ -371,22 +378,28 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   const int t = c->S.t;
   hipLaunchKernelGGL(vhc::k_bvh_reset, dim3(1), dim3(64), 0, c->stream, I.bd, skip);
   if (t == 0) return;
-  hipLaunchKernelGGL(vhc::k_boxes, dim3(nblk(t)), dim3(256), 0, c->stream, t, c->tri_idx, x, margin, I.box, I.bd, skip);
+  hipLaunchKernelGGL(vhc::k_boxes, dim3(nblk(t)), dim3(256), 0, c->stream, t, c->tri_idx, x, margin, I.box,
+                     I.bpart, skip);
+  hipLaunchKernelGGL(vhc::k_scene_bounds, dim3(1), dim3(256), 0, c->stream, nblk(t), I.bpart, I.bd, skip);
   hipLaunchKernelGGL(vhc::k_morton, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.bd, I.code, I.id, skip);
   size_t bytes = I.sort_bytes;
   (void)rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t, 0u, 30u,
                                   c->stream);
   hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.st, skip);
   if (t < 2) return;
-  hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.right,
-                     I.range_lo, I.range_hi, skip);
+  hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.range_lo,
+                     I.range_hi, skip);
   for (int k = 1; k < I.st_levels; ++k)
     hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
-  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.st, I.range_lo,
-                     I.range_hi, I.node, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left, I.right,
-                     I.range_hi, I.node, I.orig, I.pairs, I.cap, I.bd, skip);
+  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left,
+                     I.range_lo, I.range_hi, I.node, I.leaf, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
+  size_t sbytes = I.scan_bytes;
+  (void)rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t, rocprim::plus<unsigned int>(),
+                                c->stream);
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.orig, I.pcount,
+                     I.poff, I.pairs, I.cap, I.bd, skip);
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
@@ -1000,10 +1013,10 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.partials, 3 * (size_t)nblk(n)));
   UP(dalloc(ctx, &I.inc, (size_t)n)); UP(dalloc(ctx, &I.ever, (size_t)n));
   UP(upload(ctx, &I.xptr, xptr)); UP(upload(ctx, &I.xidx, xidx));
-  UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, (size_t)tt));
+  UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, (size_t)tt)); UP(dalloc(ctx, &I.leaf, (size_t)tt));
   UP(dalloc(ctx, &I.code, (size_t)tt)); UP(dalloc(ctx, &I.code_s, (size_t)tt));
   UP(dalloc(ctx, &I.id, (size_t)tt)); UP(dalloc(ctx, &I.id_s, (size_t)tt));
-  UP(dalloc(ctx, &I.left, (size_t)tt)); UP(dalloc(ctx, &I.right, (size_t)tt));
+  UP(dalloc(ctx, &I.left, (size_t)tt));
   UP(dalloc(ctx, &I.range_lo, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
   I.st_levels = 1;
   while ((1ll << I.st_levels) <= (long long)tt) ++I.st_levels;  // levels 0 .. floor(log2 t)
@@ -1013,15 +1026,19 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
   VH_HIP(hipHostMalloc((void**)&I.h_done, sizeof(int), hipHostMallocDefault));
   *I.h_done = 0;
-#undef UP
   size_t bytes = 0;
   VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
                                    ctx->stream));
   I.sort_bytes = std::max<size_t>(bytes, 16);
-  {
-    vh_status s_ = dalloc_bytes(ctx, &I.sort_tmp, I.sort_bytes);
-    if (s_ != VH_OK) return s_;
-  }
+  UP(dalloc(ctx, &I.pcount, (size_t)tt)); UP(dalloc(ctx, &I.poff, (size_t)tt));
+  UP(dalloc(ctx, &I.bpart, 6 * (size_t)nblk(tt)));
+  bytes = 0;
+  VH_HIP(rocprim::exclusive_scan(nullptr, bytes, I.pcount, I.poff, 0u, (size_t)tt, rocprim::plus<unsigned int>(),
+                                 ctx->stream));
+  I.scan_bytes = std::max<size_t>(bytes, 16);
+  UP(dalloc_bytes(ctx, &I.sort_tmp, I.sort_bytes));
+  UP(dalloc_bytes(ctx, &I.scan_tmp, I.scan_bytes));
+#undef UP
   VH_HIP(hipMemset(I.gself, 0, n3 * 8));
   VH_HIP(hipMemset(I.lambda, 0, n3 * 8));
   VH_HIP(hipMemset(I.fnrm, 0, n3 * 8));

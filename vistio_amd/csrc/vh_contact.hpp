@@ -48,6 +48,7 @@ __device__ __forceinline__ double len2(D3 a) { return dot(a, a); }
 __device__ __forceinline__ double clamp01(double v) { return fmin(fmax(v, 0.0), 1.0); }
 __device__ __forceinline__ D3 ld(const double* x, int i) { return d3(x[3 * i], x[3 * i + 1], x[3 * i + 2]); }
 
+
 // ---- barrier.rs ------------------------------------------------------------
 
 // barrier.rs:40-53: db/dd of b(d) = -(d - dh)^2 ln(d/dh), d clamped to
@@ -287,10 +288,11 @@ struct BvhDev {
   unsigned int crossings; // triangle pairs that cross (k_tri_crossings)
 };
 
-// bvh.rs:28-42: box of f32 positions, inflated by margin. Also folds the
-// centroid into the scene bounds.
+// bvh.rs:28-42: box of f32 positions, inflated by margin. Each block also
+// writes the bounds of its box centroids (min xyz, max xyz) to part.
 __global__ void __launch_bounds__(256) k_boxes(int t, const int* __restrict__ tri, const double* __restrict__ x,
-                                               float margin, Box* __restrict__ box, BvhDev* bd, const int* skip) {
+                                               float margin, Box* __restrict__ box, float* __restrict__ part,
+                                               const int* skip) {
   if (skip && *skip) return;
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   float c[3] = {0, 0, 0};
@@ -323,9 +325,33 @@ __global__ void __launch_bounds__(256) k_boxes(int t, const int* __restrict__ tr
   }
   if (threadIdx.x == 0)
     for (int a = 0; a < 3; ++a) {
-      atomicMin(&bd->scene[a], f2o(smin[a][0]));
-      atomicMax(&bd->scene[3 + a], f2o(smax[a][0]));
+      part[6 * blockIdx.x + a] = smin[a][0];
+      part[6 * blockIdx.x + 3 + a] = smax[a][0];
     }
+}
+// Scene bounds from the per-block partials of k_boxes (one block).
+__global__ void __launch_bounds__(256) k_scene_bounds(int nb, const float* __restrict__ part, BvhDev* bd,
+                                                      const int* skip) {
+  if (skip && *skip) return;
+  __shared__ float smin[3][256], smax[3][256];
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], part[6 * b + a]);
+      hi[a] = fmaxf(hi[a], part[6 * b + 3 + a]);
+    }
+  for (int a = 0; a < 3; ++a) { smin[a][threadIdx.x] = lo[a]; smax[a][threadIdx.x] = hi[a]; }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        smin[a][threadIdx.x] = fminf(smin[a][threadIdx.x], smin[a][threadIdx.x + s]);
+        smax[a][threadIdx.x] = fmaxf(smax[a][threadIdx.x], smax[a][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 3; ++a) { bd->scene[a] = f2o(smin[a][0]); bd->scene[3 + a] = f2o(smax[a][0]); }
 }
 
 // Scene bounds to empty, pair count to 0.
@@ -372,11 +398,11 @@ __device__ __forceinline__ int delta(const unsigned int* code, int t, int i, int
   return 32 + __clz((unsigned int)(i ^ j));
 }
 
-// Karras (2012) internal node i of t - 1: its key range [lo, hi] and split.
-// Children >= 0 are internal nodes, < 0 are leaves (~leaf).
+// Karras (2012) internal node i of t - 1: its key range [lo, hi] and its
+// left child (>= 0 internal node, < 0 leaf ~j).
 __global__ void __launch_bounds__(256) k_karras(int t, const unsigned int* __restrict__ code, int* __restrict__ left,
-                                                int* __restrict__ right, int* __restrict__ range_lo,
-                                                int* __restrict__ range_hi, const int* skip) {
+                                                int* __restrict__ range_lo, int* __restrict__ range_hi,
+                                                const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= t - 1) return;
@@ -396,8 +422,7 @@ __global__ void __launch_bounds__(256) k_karras(int t, const unsigned int* __res
   }
   const int g = i + sp * d + (d < 0 ? d : 0);
   const int lo = i < j ? i : j, hi = i < j ? j : i;
-  left[i] = lo == g ? ~g : g;
-  right[i] = hi == g + 1 ? ~(g + 1) : g + 1;
+  left[i] = lo == g ? ~g : g;  // the right child (g + 1) is reached through escapes
   range_lo[i] = lo;
   range_hi[i] = hi;
 }
@@ -428,75 +453,115 @@ __global__ void __launch_bounds__(256) k_st_level(int t, int half, const Box* __
   const int o = j + half < t ? j + half : j;
   cur[j] = box_union(prev[j], prev[o]);
 }
-__global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st,
-                                                     const int* __restrict__ range_lo,
-                                                     const int* __restrict__ range_hi, Box* __restrict__ node,
-                                                     const int* skip) {
+// Traversal records. A subtree is skipped by jumping to its escape (rope):
+// for a range ending at leaf hi that is the highest node starting at hi + 1,
+// the right child of the split between hi and hi + 1. In a Karras tree a
+// right child is internal node s exactly when its range starts at s, so the
+// escape is internal node hi + 1 if range_lo[hi + 1] == hi + 1, else leaf
+// hi + 1, and kEnd after the last leaf.
+constexpr int kEnd = 0x7fffffff;
+struct __align__(16) Node {
+  Box box;
+  int left;    // >= 0 internal node, < 0 leaf ~j
+  int escape;  // same encoding, or kEnd
+  int hi;      // last leaf of the range
+  int pad[3];
+};
+struct __align__(16) Leaf {
+  Box box;
+  int escape;
+  int id;      // triangle of sorted leaf j
+};
+__device__ __forceinline__ int escape_of(int hi, int t, const int* __restrict__ range_lo) {
+  if (hi >= t - 1) return kEnd;
+  return (hi + 1 <= t - 2 && range_lo[hi + 1] == hi + 1) ? hi + 1 : ~(hi + 1);
+}
+__global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st, const int* __restrict__ id,
+                                                     const int* __restrict__ left, const int* __restrict__ range_lo,
+                                                     const int* __restrict__ range_hi, Node* __restrict__ node,
+                                                     Leaf* __restrict__ leaf, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < t) {
+    Leaf L;
+    L.box = st[i];
+    L.escape = escape_of(i, t, range_lo);
+    L.id = id[i];
+    leaf[i] = L;
+  }
   if (i >= t - 1) return;
   const int lo = range_lo[i], hi = range_hi[i];
   const int k = 31 - __clz((unsigned int)(hi - lo + 1));
   const Box* lv = st + (size_t)k * (size_t)t;
-  node[i] = box_union(lv[lo], lv[hi - (1 << k) + 1]);
-}
-
-// One slot of a global counter per calling lane, one atomic per wave: the
-// lanes active here take consecutive slots from their leader's reservation.
-__device__ __forceinline__ unsigned int wave_reserve(unsigned int* counter) {
-  const unsigned long long m = __ballot(1);
-  const unsigned int lane = __lane_id();
-  const unsigned int leader = (unsigned int)__ffsll((long long)m) - 1u;
-  const unsigned int rank = (unsigned int)__popcll(m & ((1ull << lane) - 1ull));
-  unsigned int base = 0;
-  if (lane == leader) base = atomicAdd(counter, (unsigned int)__popcll(m));
-  base = __shfl(base, (int)leader);
-  return base + rank;
+  Node N;
+  N.box = box_union(lv[lo], lv[hi - (1 << k) + 1]);
+  N.left = left[i];
+  N.escape = escape_of(hi, t, range_lo);
+  N.hi = hi;
+  N.pad[0] = N.pad[1] = N.pad[2] = 0;
+  node[i] = N;
 }
 
 // Every leaf k collects the leaves j > k (Morton order) whose boxes overlap
-// its own: each unordered pair once. A pair is stored as (a, b) with the
-// caller's triangle index orig[a] < orig[b]: the reference tests vertices of
-// its first triangle against the second and edges of the first against the
-// second (self_collision.rs:199-276), and for parallel edges the closest
-// points, hence the gradient, depend on that order. The count may exceed cap.
-// Stack bound: delta() strictly grows from a node to its internal children and
-// lies in [2, 63] (30-bit keys, index ties add 32), so a root-to-leaf path
-// has at most 62 internal nodes and the stack never holds more than 63.
-__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Box* __restrict__ st0, const int* __restrict__ id,
-                                                   const int* __restrict__ left, const int* __restrict__ right,
-                                                   const int* __restrict__ range_hi, const Box* __restrict__ node,
-                                                   const int* __restrict__ orig, int2* __restrict__ pairs,
+// its own: each unordered pair once. Two passes over the same walk: the
+// first counts the pairs of each leaf, an exclusive scan turns the counts
+// into offsets, the second writes each leaf's pairs at its offset (no
+// shared counter). A pair is stored as (a, b) with the caller's triangle
+// index orig[a] < orig[b]: the reference tests vertices of its first
+// triangle against the second and edges of the first against the second
+// (self_collision.rs:199-276), and for parallel edges the closest points,
+// hence the gradient, depend on that order.
+// Stackless: pre-order walk from the root that enters a node when its box
+// overlaps and its range reaches past k, and otherwise takes its escape;
+// every step moves forward in pre-order, so the walk ends at kEnd.
+template <bool kEmit>
+__device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+                                                 const int* __restrict__ orig, int2* __restrict__ pairs,
+                                                 unsigned int base, unsigned int cap) {
+  const Leaf me = leaf[k];
+  const int ome = kEmit ? orig[me.id] : 0;
+  unsigned int cnt = 0;
+  int nd = 0;
+  while (nd != kEnd) {
+    if (nd >= 0) {
+      const Node N = node[nd];
+      nd = (N.hi > k && overlaps(me.box, N.box)) ? N.left : N.escape;
+    } else {
+      const int j = ~nd;
+      const Leaf L = leaf[j];
+      if (j > k && overlaps(me.box, L.box)) {
+        if (kEmit) {
+          const unsigned int slot = base + cnt;
+          const bool first = ome < orig[L.id];
+          if (slot < cap) pairs[slot] = make_int2(first ? me.id : L.id, first ? L.id : me.id);
+        }
+        ++cnt;
+      }
+      nd = L.escape;
+    }
+  }
+  return cnt;
+}
+__global__ void __launch_bounds__(256) k_bvh_count(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+                                                   unsigned int* __restrict__ count, const int* skip) {
+  if (skip && *skip) return;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= t) return;
+  count[k] = t < 2 ? 0u : bvh_walk<false>(k, node, leaf, nullptr, nullptr, 0, 0);
+}
+// offset: exclusive scan of count. The last leaf records the total.
+__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+                                                   const int* __restrict__ orig,
+                                                   const unsigned int* __restrict__ count,
+                                                   const unsigned int* __restrict__ offset, int2* __restrict__ pairs,
                                                    unsigned int cap, BvhDev* bd, const int* skip) {
   if (skip && *skip) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= t || t < 2) return;
-  const int me = id[k];
-  const int ome = orig[me];
-  const Box q = st0[k];
-  int stack[64];
-  int sp = 0;
-  stack[sp++] = 0;
-  while (sp > 0) {
-    const int nd = stack[--sp];
-    const int ch[2] = {left[nd], right[nd]};
-    for (int c = 0; c < 2; ++c) {
-      const int x = ch[c];
-      if (x < 0) {
-        const int j = ~x;
-        if (j <= k) continue;
-        if (overlaps(q, st0[j])) {
-          const int other = id[j];
-          const unsigned int slot = wave_reserve(&bd->npairs);
-          const bool first = ome < orig[other];
-          if (slot < cap) pairs[slot] = make_int2(first ? me : other, first ? other : me);
-        }
-      } else {
-        if (range_hi[x] <= k) continue;  // every leaf below precedes k
-        if (overlaps(q, node[x]) && sp < 64) stack[sp++] = x;
-      }
-    }
-  }
+  if (k >= t) return;
+  const unsigned int base = offset[k];
+  if (k == t - 1) bd->npairs = base + count[k];
+  if (t < 2 || count[k] == 0 || base >= cap) return;
+  bvh_walk<true>(k, node, leaf, orig, pairs, base, cap);
 }
 
 // ---- IPC step state --------------------------------------------------------
@@ -568,54 +633,61 @@ __global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ 
     // B, and the relation is symmetric (within `depth` rings): bit 3i + j
     // holds excluded(A[i], B[j]).
     unsigned int ex = 0;
+#pragma unroll
     for (int a = 0; a < 3; ++a)
+#pragma unroll
       for (int b = 0; b < 3; ++b) ex |= (excluded(xptr, xidx, A[a], B[b]) ? 1u : 0u) << (3 * a + b);
-    for (int side = 0; side < 2; ++side) {
-      const int* V = side == 0 ? A : B;
-      const int* T = side == 0 ? B : A;
-      for (int q = 0; q < 3; ++q) {
-        const int v = V[q];
-        if (v == T[0] || v == T[1] || v == T[2]) continue;
-        const unsigned int row = side == 0 ? (ex >> (3 * q)) & 7u
-                                           : ((ex >> q) & 1u) | ((ex >> (q + 2)) & 2u) | ((ex >> (q + 4)) & 4u);
-        if (row) continue;
-        const D3 p = ld(x, v), a = ld(x, T[0]), b = ld(x, T[1]), c = ld(x, T[2]);
-        const double d2 = pt_dist2(p, a, b, c);
-        if (d2 < dh && d2 > 0.0) {
-          D3 g[4];
-          pt_grad(p, a, b, c, g);
-          const double bg = bgrad(d2);
-          atomic_add3(gself, v, g[0], bg);
-          atomic_add3(gself, T[0], g[1], bg);
-          atomic_add3(gself, T[1], g[2], bg);
-          atomic_add3(gself, T[2], g[3], bg);
-          ++cnt;
-          if (d2 <= 1e-12) viol = true;
-        }
+    if (ex == 0x1ffu) continue;  // every test of the pair is excluded
+    // vertex-triangle tests, one at a time (A's vertices against B first)
+    auto pick = [](const int* P, int q) { return q == 0 ? P[0] : (q == 1 ? P[1] : P[2]); };
+#pragma unroll 1
+    for (int u = 0; u < 6; ++u) {
+      const bool fa = u < 3;
+      const int q = fa ? u : u - 3;
+      const int v = fa ? pick(A, q) : pick(B, q);
+      const int t0 = fa ? B[0] : A[0], t1 = fa ? B[1] : A[1], t2 = fa ? B[2] : A[2];
+      if (v == t0 || v == t1 || v == t2) continue;
+      // excluded(v, T[*]): row q of ex for a vertex of A, column q for one of B
+      const unsigned int row = fa ? (ex >> (3 * q)) & 7u : (ex >> q) & 0x49u;
+      if (row) continue;
+      const D3 p = ld(x, v), a = ld(x, t0), b = ld(x, t1), c = ld(x, t2);
+      const double d2 = pt_dist2(p, a, b, c);
+      if (d2 < dh && d2 > 0.0) {
+        D3 g[4];
+        pt_grad(p, a, b, c, g);
+        const double bg = bgrad(d2);
+        atomic_add3(gself, v, g[0], bg);
+        atomic_add3(gself, t0, g[1], bg);
+        atomic_add3(gself, t1, g[2], bg);
+        atomic_add3(gself, t2, g[3], bg);
+        ++cnt;
+        if (d2 <= 1e-12) viol = true;
       }
     }
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        const int i1 = i == 2 ? 0 : i + 1, j1 = j == 2 ? 0 : j + 1;
-        const int a0 = A[i], a1 = A[i1], b0 = B[j], b1 = B[j1];
-        if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-        const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
-                                  (1u << (3 * i1 + j1));
-        if ((ex & need) == need) continue;
-        const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
-        const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
-        if (d2 < dh && d2 > 0.0) {
-          D3 g[4];
-          ee_grad(pa0, pa1, pb0, pb1, g);
-          const double bg = bgrad(d2);
-          atomic_add3(gself, a0, g[0], bg);
-          atomic_add3(gself, a1, g[1], bg);
-          atomic_add3(gself, b0, g[2], bg);
-          atomic_add3(gself, b1, g[3], bg);
-          ++cnt;
-          if (d2 <= 1e-12) viol = true;
-        }
+    // edge-edge tests: edge i of A (A[i], A[i+1]) against edge j of B
+#pragma unroll 1
+    for (int e = 0; e < 9; ++e) {
+      const int i = e / 3, j = e - 3 * i;
+      const int i1 = i == 2 ? 0 : i + 1, j1 = j == 2 ? 0 : j + 1;
+      const int a0 = pick(A, i), a1 = pick(A, i1), b0 = pick(B, j), b1 = pick(B, j1);
+      if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+      const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
+                                (1u << (3 * i1 + j1));
+      if ((ex & need) == need) continue;
+      const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
+      const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
+      if (d2 < dh && d2 > 0.0) {
+        D3 g[4];
+        ee_grad(pa0, pa1, pb0, pb1, g);
+        const double bg = bgrad(d2);
+        atomic_add3(gself, a0, g[0], bg);
+        atomic_add3(gself, a1, g[1], bg);
+        atomic_add3(gself, b0, g[2], bg);
+        atomic_add3(gself, b1, g[3], bg);
+        ++cnt;
+        if (d2 <= 1e-12) viol = true;
       }
+    }
   }
   // one counter update per wave
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
@@ -838,17 +910,25 @@ __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pair
   double best = 2.0;
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
     const int2 pr = pairs[k];
-    const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
-    const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    // The six corners (A then B) with their start and end positions, loaded
+    // once per pair: measured faster than reloading them per test, despite
+    // the register cost of the unrolled tests.
+    const int V[6] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2],
+                      tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    D3 P0[6], P1[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      P0[u] = ld(x, V[u]);
+      P1[u] = P0[u] + ld(dl, V[u]);
+    }
+#pragma unroll
     for (int side = 0; side < 2; ++side) {
-      const int* V = side == 0 ? A : B;
-      const int* T = side == 0 ? B : A;
+      const int tb = side == 0 ? 3 : 0;  // the triangle's first corner in V
+#pragma unroll
       for (int q = 0; q < 3; ++q) {
-        const int v = V[q];
-        if (v == T[0] || v == T[1] || v == T[2]) continue;
-        const D3 p0 = ld(x, v), a0 = ld(x, T[0]), b0 = ld(x, T[1]), c0 = ld(x, T[2]);
-        best = fmin(best, vt_ccd(p0, p0 + ld(dl, v), a0, a0 + ld(dl, T[0]), b0, b0 + ld(dl, T[1]), c0,
-                                 c0 + ld(dl, T[2])));
+        const int vi = 3 * side + q;
+        if (V[vi] == V[tb] || V[vi] == V[tb + 1] || V[vi] == V[tb + 2]) continue;
+        best = fmin(best, vt_ccd(P0[vi], P1[vi], P0[tb], P1[tb], P0[tb + 1], P1[tb + 1], P0[tb + 2], P1[tb + 2]));
       }
     }
   }
