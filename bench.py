@@ -279,6 +279,82 @@ def run_contact(args):
     print(json.dumps(out))
 
 
+def run_batch(args):
+    """The batched-sheets workload: 64 pinned sheets (grids 32..256 cells a
+    side, per-sheet density and stiffness; vistio_amd.batch.cfg3_specs, seeds
+    SplitMix64(0x5EED ^ sheet id)) stepped as one block-diagonal system in one
+    graph-captured launch sequence. value = batch steps/s (every sheet
+    advances one step); the error is the max vertex distance between the
+    FP32-refined and the FP64 GPU batch after the timed steps from rest."""
+    import vistio_amd as va
+    from vistio_amd.batch import build_batch, cfg3_specs
+    dt = 1.0 / 60.0
+    specs = cfg3_specs(args.sheets)
+    b = build_batch(specs)
+    steps = args.steps
+    res = {}
+    for prec in ("f32_ir", "f64"):
+        cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                              rayleigh_mass_damping=0.01, precision=prec, ir_sweeps=args.ir_sweeps,
+                              ir_final_sweeps=args.ir_final_sweeps)
+        s = va.HipPdSolver(0)
+        t0 = time.perf_counter()
+        s.init_raw(b.mesh, b.topology, cfg, b.pinned, "co_rotational", b.k_membrane, b.k_bending, b.density)
+        setup = time.perf_counter() - t0
+        print(f"[batched_sheets {prec}] set-up {setup:.1f} s", file=sys.stderr, flush=True)
+        if args.warmup:
+            s.step_resident(dt, args.warmup)
+        reset(s, b.mesh)
+        s.step_resident(dt, 0)
+        t0 = time.perf_counter()
+        s.step_resident(dt, steps)
+        wall = time.perf_counter() - t0
+        res[prec] = dict(wall=wall, x=s.positions(), prof=s.profile(dt, 10), info=s.info(), setup=setup)
+        s.close()
+    f32, f64 = res["f32_ir"], res["f64"]
+    value = steps / f32["wall"]
+    err = float(np.sqrt(((f32["x"] - f64["x"]) ** 2).sum(0)).max())
+    cells = np.array([sp.cells for sp in specs])
+    out = {
+        "metric": "batch steps/s (batched sheets)",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * f32["wall"] / steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batched_sheets: {len(specs)} pinned sheets, grids {cells.min()}..{cells.max()} "
+                               f"cells a side (mean {cells.mean():.0f}), {b.mesh.vertex_count()} verts, "
+                               f"{b.mesh.triangle_count()} tris, per-sheet density 0.08-0.40 kg/m^2, membrane "
+                               "3e2-3e3 N/m, bending 1e-5-1e-3 N m (SplitMix64 seeds 0x5EED ^ id), co-rotational, "
+                               "Rayleigh 0.01, dt 1/60 s, 10 PD iterations; FP32 local step + FP32 solves, FP64 "
+                               "residual refinement; one block-diagonal system, one graph per step",
+                   "sheets": len(specs)},
+        "roofline": roofline_of(f32["prof"]),
+        "extra": {
+            "sheet_steps_per_s": round(value * len(specs), 1),
+            "vertex_steps_per_s": round(value * b.mesh.vertex_count(), 0),
+            "max_vertex_err_vs_fp64_gpu_m": err,
+            "fp64_gpu_steps_per_s": round(steps / f64["wall"], 3),
+            "fp32_ir_over_fp64_gpu": round(value / (steps / f64["wall"]), 3),
+            "iteration_us": {"fp32_ir": round(f32["prof"]["iteration_us"], 2),
+                             "fp64": round(f64["prof"]["iteration_us"], 2)},
+            "local_step": {
+                "fp32_us": round(f32["prof"]["local_us"], 3),
+                "fp32_gbs": round(f32["prof"]["local_bytes"] / (f32["prof"]["local_us"] * 1e-6) / 1e9, 1)},
+            "solve_bands": f32["info"]["levels"],
+            "factor_entries": f32["info"]["nnz_factor"],
+            "setup_s": {"fp32_ir": round(f32["setup"], 1), "fp64": round(f64["setup"], 1)},
+        },
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,11 +369,12 @@ def main():
                     help="first run a plain copy-like kernel (b = a + 1, float32) over this many MB, "
                          "to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE")
     ap.add_argument("--workload", default="hanging_sheet",
-                    choices=("hanging_sheet", "sphere_drape", "self_fold_ribbon"),
+                    choices=("hanging_sheet", "sphere_drape", "self_fold_ribbon", "batched_sheets"),
                     help="hanging_sheet is the headline metric; the contact workloads print their own line")
     ap.add_argument("--check-every", type=int, default=1,
                     help="contact workloads: count crossing triangle pairs every this many steps")
     ap.add_argument("--check-all", action="store_true", help="contact workloads: also check the FP64 run")
+    ap.add_argument("--sheets", type=int, default=64, help="batched_sheets: number of sheets")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -305,6 +382,9 @@ def main():
         print(f"bench.py measures one GPU: the PD step of one sheet runs on a single MI355X; "
               f"--gpus {args.gpus} (WORLD_SIZE {world}) is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched_sheets":
+        run_batch(args)
+        return
     if args.workload != "hanging_sheet":
         run_contact(args)
         return

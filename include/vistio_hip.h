@@ -132,6 +132,24 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
                      uint32_t t, const uint32_t* tri, uint32_t h, const uint32_t* hinge,
                      const uint8_t* pinned, const vh_pd_params* params, float* lumped_mass_out);
 
+/* Per-element materials, for several garments stepped as one system (their
+ * meshes concatenated; the system matrix is then block diagonal and one
+ * launch sequence steps them all). Each array may be NULL (the uniform value
+ * of vh_pd_params is used): k_membrane per triangle (t), k_bending per hinge
+ * (h), density per triangle (t, kg/m^2; a vertex gets a third of each
+ * incident triangle's area * density, pd_solver.rs:1219-1242). Every garment
+ * corresponds to one ProjectiveDynamicsSolver::init_with_material
+ * (pd_solver.rs:106-143) with its own FabricProperties. */
+typedef struct {
+  const double* k_membrane;
+  const double* k_bending;
+  const double* density;
+} vh_material_arrays;
+vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz,
+                        uint32_t t, const uint32_t* tri, uint32_t h, const uint32_t* hinge,
+                        const uint8_t* pinned, const vh_pd_params* params, const vh_material_arrays* mat,
+                        float* lumped_mass_out);
+
 /* SolverStrategy::step (pd_solver.rs:259-476) on host SoA state: uploads
  * pos/vel and inv_mass, runs one step on the device, downloads
  * pos/vel/prev/pred. */

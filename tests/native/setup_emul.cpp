@@ -46,9 +46,9 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       for (int j = 0; j < w; ++j)
         for (int k = 0; k < 3; ++k) front[3 * j + k] = r[3 * (size_t)(P.col0 + j) + k] - front[3 * j + k];
       if (T.kind == 2) {  // root: d = Sroot front, pushed to the descendants
-        if (P.r != 0 || S.Sroot.size() != (size_t)w * w) return -13;
+        if (P.r != 0 || P.s_off < 0 || S.Sroot.size() < (size_t)P.s_off + (size_t)w * w) return -13;
         for (int row = T.b; row < T.e; ++row) {
-          const double* Sr = &S.Sroot[(size_t)row * w];
+          const double* Sr = &S.Sroot[(size_t)P.s_off + (size_t)row * w];
           double s3[3] = {0, 0, 0};
           for (int kk = 0; kk < w; ++kk)
             for (int k = 0; k < 3; ++k) s3[k] += Sr[kk] * front[3 * kk + k];

@@ -28,7 +28,7 @@ EXPORTS = (
     "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
     "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
-    "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings",
+    "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings", "vh_pd_init_ex",
 )
 
 
@@ -43,6 +43,12 @@ class PdParams(C.Structure):
         ("max_band_levels", C.c_int32), ("band_launch_us", C.c_double), ("band_gbps", C.c_double),
         ("task_entries", C.c_int32),
     ]
+
+
+class MaterialArrays(C.Structure):
+    """vh_material_arrays: per-element k_membrane (t), k_bending (h), density (t); NULL = uniform."""
+    _fields_ = [("k_membrane", C.POINTER(C.c_double)), ("k_bending", C.POINTER(C.c_double)),
+                ("density", C.POINTER(C.c_double))]
 
 
 class StepResultC(C.Structure):
@@ -115,6 +121,8 @@ def load():
         "vh_last_error": ([vp], C.c_char_p),
         "vh_pd_init": ([vp, C.c_uint32, fp, fp, fp, C.c_uint32, up, C.c_uint32, up, C.POINTER(C.c_uint8),
                         C.POINTER(PdParams), fp], C.c_int),
+        "vh_pd_init_ex": ([vp, C.c_uint32, fp, fp, fp, C.c_uint32, up, C.c_uint32, up, C.POINTER(C.c_uint8),
+                           C.POINTER(PdParams), C.POINTER(MaterialArrays), fp], C.c_int),
         "vh_pd_step": ([vp, C.c_double, C.POINTER(StateF32), C.POINTER(StepResultC)], C.c_int),
         "vh_pd_step_resident": ([vp, C.c_double, C.c_uint32, C.POINTER(StepResultC)], C.c_int),
         "vh_state_set_f64": ([vp, C.c_int32, dp, dp, dp], C.c_int),

@@ -576,6 +576,12 @@ const char* vh_last_error(const vh_ctx* ctx) { return ctx ? ctx->err.c_str() : "
 vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz, uint32_t t,
                      const uint32_t* tri, uint32_t h, const uint32_t* hinge, const uint8_t* pinned,
                      const vh_pd_params* params, float* lumped_mass_out) {
+  return vh_pd_init_ex(ctx, n, px, py, pz, t, tri, h, hinge, pinned, params, nullptr, lumped_mass_out);
+}
+
+vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz, uint32_t t,
+                        const uint32_t* tri, uint32_t h, const uint32_t* hinge, const uint8_t* pinned,
+                        const vh_pd_params* params, const vh_material_arrays* mat, float* lumped_mass_out) {
   if (!ctx) return VH_E_INVALID_ARG;
   if (!params) return fail(ctx, VH_E_INVALID_ARG, "params is null");
   VH_HIP(hipSetDevice(ctx->device));
@@ -601,6 +607,11 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
   in.n = (int)n; in.t = (int)t; in.h = P.bending == VH_BENDING_NONE ? 0 : (int)h;
   in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge; in.pinned = pinned;
   in.k_membrane = P.k_membrane; in.k_bending = P.k_bending; in.density = P.density; in.dt_matrix = P.dt_matrix;
+  if (mat) {
+    in.k_membrane_tri = mat->k_membrane;
+    in.k_bending_hinge = mat->k_bending;
+    in.density_tri = mat->density;
+  }
   in.leaf_size = P.leaf_size > 0 ? P.leaf_size : 16;
   in.value_bytes = (int)ctx->tsz;
   if (P.max_band_levels > 0) in.max_band_levels = P.max_band_levels;
@@ -642,7 +653,7 @@ vh_status vh_pd_init(vh_ctx* ctx, uint32_t n, const float* px, const float* py, 
     const vh::Supernode& q = S.sn[T.sn];
     td[k].q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
     td[k].q1 = make_int4(T.b, T.e, T.g, q.u_off);
-    td[k].q2 = make_int4(q.fm_off, q.fslot_k, 0, 0);
+    td[k].q2 = make_int4(q.fm_off, q.fslot_k, q.parent < 0 ? (int)q.s_off : 0, 0);
   }
   UP(upload(ctx, &ctx->tasks, td));
   UP(upload(ctx, &ctx->uslot, S.uslot));

@@ -828,14 +828,16 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
   }
 }
 
-// Root task: the root supernode R has no off-diagonal rows, so its forward
-// and backward sweeps collapse into d_R = (L_RR L_RR^T)^{-1} t_R with the
-// dense symmetric inverse Sroot (row-major w x w); rows [b, e) of it. Same
-// front assembly as a forward task, same stores as a backward task.
+// Root task: a root supernode R has no off-diagonal rows, so its forward
+// and backward sweeps collapse into d_R = (L_RR L_RR^T)^{-1} t_R with its
+// dense symmetric inverse (row-major w x w at Sroot + td.q2.z, one per
+// dissection tree); rows [b, e) of it. Same front assembly as a forward
+// task, same stores as a backward task.
 template <int G, bool S, bool C, class T>
-__device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot,
+__device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot_all,
                           T* __restrict__ F, T* __restrict__ V, T* __restrict__ d, T* front, double& df, double& nm) {
   constexpr int kPrefRows = pref_rows<T>();
+  const T* __restrict__ Sroot = Sroot_all + (size_t)td.q2.z;
   const int col0 = td.q0.x, w = td.q0.y;
   const int b = td.q1.x, e = td.q1.y;
   const int K = td.q2.y;

@@ -183,12 +183,12 @@ bool build_setup(const SetupInput& in, Setup& S) {
     const D3 e1 = sub(P(i1), P(i0)), e2 = sub(P(i2), P(i0));
     area[e] = 0.5 * len(cross(e1, e2));
     dm_inverse(e1, e2, &dm[4 * (size_t)e]);
-    tw[e] = in.k_membrane * area[e];
+    tw[e] = (in.k_membrane_tri ? in.k_membrane_tri[e] : in.k_membrane) * area[e];
   }
   // Lumped masses, pd_solver.rs:1219-1242.
   std::vector<double> mass(n, 0.0);
   for (int e = 0; e < t; ++e) {
-    const double third = (area[e] * in.density) / 3.0;
+    const double third = (area[e] * (in.density_tri ? in.density_tri[e] : in.density)) / 3.0;
     for (int k = 0; k < 3; ++k) mass[in.tri[3 * e + k]] += third;
   }
   for (int i = 0; i < n; ++i) {
@@ -204,7 +204,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
     if (std::fabs(a - M_PI) < 1e-3) a = M_PI;
     else if (std::fabs(a) < 1e-3) a = 0.0;
     hrest[e] = a;
-    hw[e] = in.k_bending * len(sub(P(q[1]), P(q[0])));
+    hw[e] = (in.k_bending_hinge ? in.k_bending_hinge[e] : in.k_bending) * len(sub(P(q[1]), P(q[0])));
   }
 
   // System matrix A = M/h^2 + sum w g g^T + sum w_b s s^T, assembly.rs:44-134,
@@ -244,12 +244,31 @@ bool build_setup(const SetupInput& in, Setup& S) {
   for (int i = 0; i < n; ++i) Ap[i + 1] += Ap[i];
   trip.clear(); trip.shrink_to_fit();
 
-  // Nested dissection.
+  // Nested dissection, one tree per connected component (a forest for
+  // several garments; their blocks never couple, so each tree has its own
+  // root at depth 0 and the levels of all trees run in the same launches).
   NdBuilder nb{Ap, Ai, X.data(), Y.data(), Z.data(), std::max(1, in.leaf_size), std::vector<int>(n, 0)};
+  std::vector<int> roots;
   {
-    std::vector<int> all(n);
-    std::iota(all.begin(), all.end(), 0);
-    nb.build(all);
+    std::vector<int> comp(n, -1), queue;
+    std::vector<std::vector<int>> members;
+    for (int s0 = 0; s0 < n; ++s0) {
+      if (comp[s0] >= 0) continue;
+      const int id = (int)members.size();
+      members.emplace_back();
+      queue.assign(1, s0);
+      comp[s0] = id;
+      for (size_t q = 0; q < queue.size(); ++q) {
+        const int v = queue[q];
+        members[id].push_back(v);
+        for (int p = Ap[v]; p < Ap[v + 1]; ++p)
+          if (comp[Ai[p]] < 0) { comp[Ai[p]] = id; queue.push_back(Ai[p]); }
+      }
+    }
+    for (auto& m : members) {
+      std::sort(m.begin(), m.end());
+      roots.push_back(nb.build(m));
+    }
   }
   // Fine dissection tree: parents, depths, sizes, and each subtree's
   // boundary size r (vertices outside the subtree adjacent to it, all of
@@ -367,7 +386,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
       for (int c : ch) S.sn[c].parent = id;
       return id;
     };
-    visit(nf - 1);
+    for (int r : roots) visit(r);
     for (size_t s = 0; s < S.sn.size(); ++s) {
       S.sn[s].child_off = (int)S.children.size();
       S.sn[s].nchild = (int)kids[s].size();
@@ -485,7 +504,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
   std::vector<std::vector<double>> upd(ns);
   std::vector<double> F, Linv;
   for (int s = 0; s < ns; ++s) {
-    const Supernode& x = S.sn[s];
+    Supernode& x = S.sn[s];
     const int w = x.w, r = x.r, m = w + r;
     F.assign((size_t)m * m, 0.0);
     for (int q = 0; q < r; ++q) loc[S.rowidx[x.rows_off + q]] = w + q;
@@ -569,8 +588,9 @@ bool build_setup(const SetupInput& in, Setup& S) {
     }
     if (x.parent < 0) {  // root: full inverse of L_RR L_RR^T = Linv^T Linv
       if (r != 0) { S.status = VH_E_INTERNAL; S.err = "root supernode has off-diagonal rows"; return false; }
-      S.Sroot.assign((size_t)w * w, 0.0);
-      double* Sp = S.Sroot.data();
+      x.s_off = (int64_t)S.Sroot.size();
+      S.Sroot.resize(S.Sroot.size() + (size_t)w * w, 0.0);
+      double* Sp = S.Sroot.data() + x.s_off;
       for (int k = 0; k < w; ++k) {
         const double* Lk = &Linv[(size_t)k * w];
         for (int i = 0; i <= k; ++i) {
@@ -644,6 +664,9 @@ bool build_setup(const SetupInput& in, Setup& S) {
   }
   std::sort(S.upper_cols.begin(), S.upper_cols.end());
   if (S.m_total >= (int64_t)1 << 31) { S.status = VH_E_UNSUPPORTED; S.err = "solve blocks exceed 2^31 entries"; return false; }
+  if ((int64_t)S.Sroot.size() >= (int64_t)1 << 31) {
+    S.status = VH_E_UNSUPPORTED; S.err = "root inverses exceed 2^31 entries"; return false;
+  }
 
   // Task lists. A task is a run of rows of one stored form; g lanes share a
   // row (g ~ row length / 8, 4..64, so a row is usually 8 loads per lane)

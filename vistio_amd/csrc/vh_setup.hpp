@@ -27,6 +27,11 @@ struct SetupInput {
   double k_membrane = 1e3;          // weight = k * area
   double k_bending = 1e-4;          // weight = k * edge length
   double density = 0.15;            // kg/m^2
+  // Per-element materials (several garments in one system); when set they
+  // replace the uniform values above: t, h and t entries.
+  const double* k_membrane_tri = nullptr;
+  const double* k_bending_hinge = nullptr;
+  const double* density_tri = nullptr;
   double dt_matrix = 1.0 / 60.0;
   int leaf_size = 16;               // nested-dissection leaf size
   // Level banding of the dissection tree. Each band of consecutive levels
@@ -59,6 +64,7 @@ struct Supernode {
   int depth = 0;
   int fm_off = 0;      // first forward slot of this node's front (fslot_k slots per position)
   int fslot_k = 0;     // slots per front position: most child update rows feeding one position
+  int64_t s_off = -1;  // roots: offset of this root's dense inverse in Sroot
 };
 
 // A solve task: rows [b, e) of one of supernode sn's stored forms with g
@@ -108,8 +114,10 @@ struct Setup {
   std::vector<int32_t> band_front;   // max (w + r) per band
   std::vector<double> M;             // M_P row-major trapezoids (forward sweep)
   std::vector<double> Mt;            // M_P^T: column j of M_P stored contiguously (backward sweep)
-  // The root supernode (no off-diagonal rows) is solved in one pass:
-  // d_R = (L_RR L_RR^T)^{-1} t_R with this dense w_R x w_R row-major inverse.
+  // A root supernode (no off-diagonal rows) is solved in one pass:
+  // d_R = (L_RR L_RR^T)^{-1} t_R with its dense w_R x w_R row-major inverse,
+  // stored at Sroot[R.s_off]. A mesh of several connected components (several
+  // garments) has one dissection tree, one root, per component.
   std::vector<double> Sroot;
   int64_t m_total = 0;
   int u_total = 0;

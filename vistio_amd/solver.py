@@ -265,7 +265,9 @@ class HipPdSolver:
 
     # -- init -------------------------------------------------------------
     def _init(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: int,
-              k_membrane: float, k_bending: float, density_kgm2: float):
+              k_membrane, k_bending, density_kgm2):
+        """k_membrane, k_bending and density_kgm2 are each a scalar or a
+        per-element array (triangles, interior edges, triangles)."""
         n = mesh.vertex_count()
         pin = np.ascontiguousarray(np.asarray(pinned).astype(np.uint8))
         if pin.size != n:
@@ -275,7 +277,18 @@ class HipPdSolver:
         p = nat.PdParams()
         p.model = model
         p.bending = nat.BENDING_DIHEDRAL
-        p.k_membrane, p.k_bending, p.density = k_membrane, k_bending, density_kgm2
+        per = {}
+        for key, val, count in (("k_membrane", k_membrane, mesh.triangle_count()),
+                                ("k_bending", k_bending, topology.interior_edges.shape[0]),
+                                ("density", density_kgm2, mesh.triangle_count())):
+            if np.ndim(val) == 0:
+                setattr(p, key, float(val))
+            else:
+                a = np.ascontiguousarray(val, dtype=np.float64)
+                if a.size != count:
+                    raise InvalidMesh(f"{key} has {a.size} entries, expected {count}")
+                per[key] = a
+                setattr(p, key, float(a.mean()) if a.size else 0.0)
         p.dt_matrix = 1.0 / 60.0  # pd_solver.rs:130,180,244
         for k in range(3):
             p.gravity[k] = float(config.gravity[k])
@@ -292,9 +305,12 @@ class HipPdSolver:
         hg = np.ascontiguousarray(topology.interior_edges[:, :4], dtype=np.uint32)
         self._mass = np.empty(n, dtype=np.float32)
         self._params = p
-        _check(self._L.vh_pd_init(self._ctx, n, nat.ptr(px, C.c_float), nat.ptr(py, C.c_float), nat.ptr(pz, C.c_float),
-                                  tri.size // 3, nat.ptr(tri, C.c_uint32), hg.shape[0], nat.ptr(hg, C.c_uint32),
-                                  nat.ptr(pin, C.c_uint8), C.byref(p), nat.ptr(self._mass, C.c_float)),
+        mat = nat.MaterialArrays(*[nat.ptr(per[k], C.c_double) if k in per else None
+                                   for k in ("k_membrane", "k_bending", "density")])
+        _check(self._L.vh_pd_init_ex(self._ctx, n, nat.ptr(px, C.c_float), nat.ptr(py, C.c_float),
+                                     nat.ptr(pz, C.c_float), tri.size // 3, nat.ptr(tri, C.c_uint32), hg.shape[0],
+                                     nat.ptr(hg, C.c_uint32), nat.ptr(pin, C.c_uint8), C.byref(p),
+                                     C.byref(mat), nat.ptr(self._mass, C.c_float)),
                self._ctx, "init")
         self._n = n
         self.config = config
@@ -315,7 +331,9 @@ class HipPdSolver:
 
     def init_raw(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: str,
                  k_membrane: float, k_bending: float, density_kgm2: float):
-        """Init with explicit membrane/bending weights and areal density (kg/m^2)."""
+        """Init with explicit membrane/bending weights and areal density
+        (kg/m^2): scalars, or per-element arrays (per triangle, per interior
+        edge, per triangle) for several garments in one system."""
         if model not in _MODEL_BY_NAME:
             raise InvalidConfig(f"unsupported constitutive model {model!r}")
         self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], k_membrane, k_bending, density_kgm2)
