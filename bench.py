@@ -292,11 +292,23 @@ def run_batch(args):
     specs = cfg3_specs(args.sheets)
     b = build_batch(specs)
     steps = args.steps
-    res = {}
+    res, tuned = {}, {}
     for prec in ("f32_ir", "f64"):
         cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
                               rayleigh_mass_damping=0.01, precision=prec, ir_sweeps=args.ir_sweeps,
                               ir_final_sweeps=args.ir_final_sweeps)
+        if args.autotune_sheets > 0:
+            # schedule knobs tuned on a proxy: the first sheets of the same draw
+            from vistio_amd.autotune import autotune
+            pb = build_batch(specs[: args.autotune_sheets])
+            t0 = time.perf_counter()
+            cfg, table = autotune(pb.mesh, pb.topology, cfg, pb.pinned, "co_rotational", pb.k_membrane,
+                                  pb.k_bending, pb.density, dt, steps=10,
+                                  log=lambda row: print(f"[autotune {prec}] {row}", file=sys.stderr, flush=True))
+            tuned[prec] = dict(proxy_sheets=args.autotune_sheets, seconds=round(time.perf_counter() - t0, 1),
+                               chosen={k: getattr(cfg, k) for k in ("max_band_levels", "band_gbps",
+                                                                    "band_launch_us", "task_entries")},
+                               candidates=table)
         s = va.HipPdSolver(0)
         t0 = time.perf_counter()
         s.init_raw(b.mesh, b.topology, cfg, b.pinned, "co_rotational", b.k_membrane, b.k_bending, b.density)
@@ -350,6 +362,7 @@ def run_batch(args):
             "solve_bands": f32["info"]["levels"],
             "factor_entries": f32["info"]["nnz_factor"],
             "setup_s": {"fp32_ir": round(f32["setup"], 1), "fp64": round(f64["setup"], 1)},
+            "autotune": tuned or None,
         },
     }
     print(json.dumps(out))
@@ -375,6 +388,8 @@ def main():
                     help="contact workloads: count crossing triangle pairs every this many steps")
     ap.add_argument("--check-all", action="store_true", help="contact workloads: also check the FP64 run")
     ap.add_argument("--sheets", type=int, default=64, help="batched_sheets: number of sheets")
+    ap.add_argument("--autotune-sheets", type=int, default=8,
+                    help="batched_sheets: tune the solve schedule on this many sheets of the draw first (0: off)")
     args = ap.parse_args()
     args.steps_given = any(a == "--steps" or a.startswith("--steps=") for a in sys.argv[1:])
     world = int(os.environ.get("WORLD_SIZE", "1"))

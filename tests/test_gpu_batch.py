@@ -52,3 +52,27 @@ def test_batch_matches_per_sheet_oracles(precision, tol):
             print(f"  sheet {s} ({b.specs[s].cells}^2) after +{steps}: {err:.2e} m", flush=True)
             assert err <= tol
     g.close()
+
+
+def test_autotune_picks_a_candidate_that_keeps_parity():
+    import vistio_amd as va
+    from vistio_amd.autotune import DEFAULTS, KNOBS, autotune
+    from conftest import sheet
+    from oracle import ref
+    px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
+    m = va.TriangleMesh(px, py, pz, idx)
+    topo = va.Topology(np.concatenate([hg, np.zeros((hg.shape[0], 2), np.uint32)], 1))
+    cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                          rayleigh_mass_damping=0.01, precision="f64")
+    best, table = autotune(m, topo, cfg, pin, "co_rotational", 1e3, 1e-4, 0.15, DT, steps=3)
+    print(len(table), "candidates;", {k: getattr(best, k) for k in DEFAULTS}, flush=True)
+    assert len(table) >= 4 and all("steps_per_s" in r or "error" in r for r in table)
+    for name, values in KNOBS:
+        assert getattr(best, name) in values
+    g = va.HipPdSolver()
+    g.init_raw(m, topo, best, pin, "co_rotational", 1e3, 1e-4, 0.15)
+    o = ref.Oracle(ref.F64, px, py, pz, idx, hg, pin, ref.params())
+    g.step_resident(DT, 30)
+    o.step(DT, 30)
+    assert max_vertex_err(g.positions(), o.positions()) <= 1e-9
+    g.close()
