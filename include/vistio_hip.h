@@ -292,6 +292,24 @@ vh_status vh_mesh_quad_grid(uint32_t cols, uint32_t rows, float width, float hei
 vh_status vh_mesh_interior_edges(uint32_t n, uint32_t t, const uint32_t* indices, uint32_t* out,
                                  uint32_t cap, uint32_t* count);
 
+/* ---- GpuBackend (crates/vistio-gpu/src/backend.rs:19-46) --------------------
+ * The trait's primitives over the caller's f32 buffers (ComputeBuffer holds
+ * host data, buffers.rs:10-16): each call copies the operands to the device,
+ * runs one kernel and copies the result back (PCIe-bound; the solver itself
+ * keeps its state in HBM). Needs only vh_create. Length mismatches return
+ * VH_E_INVALID_ARG with the reference's messages ("AXPY buffer length
+ * mismatch", "Dot product buffer length mismatch"), which the binding maps
+ * to VistioError::Gpu as CpuFallback does (backend.rs:88-116).
+ *   axpy  out[i] = a[i] + scale * b[i], rounded as separate f32 multiply and add
+ *   dot   sum of a[i] * b[i] in f64 (exact products; the summation order is a
+ *         parallel tree, so the result can differ from the sequential sum by a
+ *         few f64 ulps of the partial sums)
+ *   fill  buf[i] = value */
+vh_status vh_backend_axpy(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb, float scale,
+                          float* out, uint64_t nout);
+vh_status vh_backend_dot(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb, double* result);
+vh_status vh_backend_fill(vh_ctx* ctx, float* buf, uint64_t n, float value);
+
 #ifdef __cplusplus
 }
 #endif

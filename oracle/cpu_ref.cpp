@@ -29,6 +29,9 @@
 // Build: see oracle/Makefile (g++ -O3 -ffp-contract=off, no FMA contraction,
 // matching Rust's default of never fusing a*b+c).
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -608,12 +611,16 @@ template <class R> struct Oracle : OracleBase {
     std::vector<V3<R>> tt(size_t(3) * t);
     std::vector<V3<R>> ht(size_t(4) * h);
     const int model = P.model;
+    // Parallel where it cannot change a result: per-element projections,
+    // then each RHS axis summed serially in the reference's order.
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < t; ++e) {
       const Tri& T = tris[e];
       V3<R> p[3];
       for (int k = 0; k < 3; ++k) p[k] = V3<R>(px[T.i[k]], py[T.i[k]], pz[T.i[k]]);
       membrane_project(p[0], p[1], p[2], T.dm_inv, model, &tt[size_t(3) * e]);
     }
+#pragma omp parallel for schedule(static)
     for (int e = 0; e < h; ++e) {
       const Hinge& H = hinges_[e];
       V3<R> q[4];
@@ -630,6 +637,7 @@ template <class R> struct Oracle : OracleBase {
       }
     const R inv_dt2 = R(1) / (dt * dt);
     const std::vector<R>* pred[3] = {&rx, &ry, &rz};
+#pragma omp parallel for schedule(static, 1)
     for (int ax = 0; ax < 3; ++ax) {
       std::vector<R>& b = rhs[ax];
       b.assign(n, R(0));
@@ -689,16 +697,21 @@ template <class R> struct Oracle : OracleBase {
     }
     px = rx; py = ry; pz = rz;
     std::vector<R> rhs[3];
-    std::vector<double> bd(n), xd(n), work(n);
+    std::vector<double> bd3[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)};
+    std::vector<double> xd3[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)};
+    std::vector<double> work3[3] = {std::vector<double>(n), std::vector<double>(n), std::vector<double>(n)};
     std::vector<R> sol[3] = {std::vector<R>(n), std::vector<R>(n), std::vector<R>(n)};
     int iterations = 0;
     double final_residual = std::numeric_limits<double>::max();
     for (int iter = 0; iter < P.max_iterations; ++iter) {
       local_and_rhs(dt, rhs, nullptr, nullptr);
       const auto s0 = std::chrono::steady_clock::now();
-      for (int ax = 0; ax < 3; ++ax) {  // FaerSolver::solve x3, faer_solver.rs:95-129
+#pragma omp parallel for schedule(static, 1)
+      for (int ax = 0; ax < 3; ++ax) {  // FaerSolver::solve x3, faer_solver.rs:95-129 (independent)
+        std::vector<double>& bd = bd3[ax];
+        std::vector<double>& xd = xd3[ax];
         for (int i = 0; i < n; ++i) bd[i] = double(rhs[ax][i]);
-        chol.solve(bd.data(), xd.data(), work);
+        chol.solve(bd.data(), xd.data(), work3[ax]);
         for (int i = 0; i < n; ++i) sol[ax][i] = R(xd[i]);
       }
       solve_time += std::chrono::duration<double>(std::chrono::steady_clock::now() - s0).count();
@@ -1105,6 +1118,21 @@ void or_membrane_project(int mode, int model, const double* p, const double* dm,
 
 
 // ---- IPC contact step and contact primitives --------------------------------
+// Threads of the oracle's parallel loops (1 = the reference's single thread).
+void or_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+int or_get_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
 int or_step_ipc(void* o, double dt, int nsteps, const or_ipc_params* ip, or_step_result* res, or_ipc_stats* st) {
   for (int k = 0; k < nsteps; ++k)
     if (static_cast<OracleBase*>(o)->step_ipc(dt, *ip, res, st)) return 1;

@@ -108,6 +108,8 @@ def lib():
                                 C.POINTER(C.c_uint8), C.POINTER(OrParams)]
         L.or_destroy.argtypes = [vp]
         L.or_step.argtypes = [vp, C.c_double, C.c_int, C.POINTER(OrStepResult)]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_threads.restype = C.c_int
         L.or_get.argtypes = [vp, C.c_int, dp]
         L.or_set.argtypes = [vp, C.c_int, dp]
         L.or_masses.argtypes = [vp, dp]
@@ -167,6 +169,16 @@ def lib():
         L.oc_ccd_step.argtypes = [C.c_int, dp, dp, C.c_int, up, ip]
         _lib = L
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (elements, the three RHS axes,
+    the three solves); results do not depend on it."""
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().or_get_threads())
 
 
 def _p(a, ct):

@@ -326,3 +326,23 @@ def test_f32_reference_mode_tracks_f64():
     a.step(DT, 20); b.step(DT, 20)
     err = np.sqrt(((a.positions() - b.positions()) ** 2).sum(0)).max()
     assert 0 < err < 1e-5
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's parallel loops (elements, RHS axes, solves) give the
+    single-thread result bit for bit, so the all-cores CPU baseline times the
+    same computation."""
+    from oracle import mesh, ref
+    px, py, pz, idx = mesh.quad_grid(16, 16, 1.0, 1.0)
+    pz = pz + np.float32(0.01) * np.sin(np.arange(px.size, dtype=np.float32))
+    hg = mesh.hinges4(idx, px.size)
+    pin = mesh.pinned_top_row(16, 16)
+    out = []
+    for th in (1, 4):
+        ref.set_threads(th)
+        for mode in (ref.F64, ref.F32_REF):
+            o = ref.Oracle(mode, px, py, pz, idx, hg, pin, ref.params())
+            o.step(1.0 / 60.0, 5)
+            out.append(o.positions())
+    ref.set_threads(1)
+    assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[3])

@@ -7,7 +7,8 @@ reference's solver interface.
 from .solver import (FabricProperties, GpuError, HipPdSolver, InvalidConfig, InvalidMesh, IpcConfig,
                      SimulationState, SolverConfig, StepResult, Topology, TriangleMesh, VistioError,
                      estimate_initial_kappa, quad_grid)
+from .backend import ComputeBuffer, HipBackend
 
-__all__ = ["FabricProperties", "GpuError", "HipPdSolver", "InvalidConfig", "InvalidMesh", "IpcConfig",
-           "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh", "VistioError",
-           "estimate_initial_kappa", "quad_grid"]
+__all__ = ["ComputeBuffer", "FabricProperties", "GpuError", "HipBackend", "HipPdSolver", "InvalidConfig",
+           "InvalidMesh", "IpcConfig", "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh",
+           "VistioError", "estimate_initial_kappa", "quad_grid"]

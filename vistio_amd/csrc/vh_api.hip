@@ -90,6 +90,11 @@ struct vh_ctx {
   IpcState ipc;
   std::vector<void*> allocs;
   std::vector<double> hbuf;  // host staging (3n)
+  // GpuBackend staging (vh_backend_*): device copies of the operands, kept
+  // across calls and grown on demand; independent of the solver's state.
+  float *bk_a = nullptr, *bk_b = nullptr, *bk_o = nullptr;
+  double* bk_part = nullptr;
+  size_t bk_cap = 0;
   hipGraphExec_t step_graph = nullptr;
   double graph_dt = 0;
   bool use_graph = true;
@@ -355,6 +360,64 @@ vh_status launch_step(vh_ctx* ctx, double dt) {
   return VH_OK;
 }
 
+// ---- GpuBackend primitives ----------------------------------------------
+
+constexpr int kBackendBlocks = 1024;
+
+// out = a + scale * b, rounded as the reference's separate multiply and add.
+__global__ void __launch_bounds__(256) k_backend_axpy(size_t n, const float* __restrict__ a,
+                                                      const float* __restrict__ b, float scale,
+                                                      float* __restrict__ out) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    out[i] = __fadd_rn(a[i], __fmul_rn(scale, b[i]));
+}
+__global__ void __launch_bounds__(256) k_backend_fill(size_t n, float* __restrict__ p, float value) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = value;
+}
+// Per-block sums of a[i] * b[i] in f64 (each product is exact in f64).
+__global__ void __launch_bounds__(256) k_backend_dot(size_t n, const float* __restrict__ a,
+                                                     const float* __restrict__ b, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    s += (double)a[i] * (double)b[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void __launch_bounds__(256) k_backend_dot_sum(int nb, double* __restrict__ part) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) s += part[k];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[kBackendBlocks] = sh[0];
+}
+
+vh_status backend_reserve(vh_ctx* ctx, size_t n) {
+  if (!ctx->bk_part) VH_HIP(hipMalloc((void**)&ctx->bk_part, (kBackendBlocks + 1) * sizeof(double)));
+  if (n <= ctx->bk_cap) return VH_OK;
+  for (float** p : {&ctx->bk_a, &ctx->bk_b, &ctx->bk_o})
+    if (*p) { (void)hipFree(*p); *p = nullptr; }
+  ctx->bk_cap = 0;
+  for (float** p : {&ctx->bk_a, &ctx->bk_b, &ctx->bk_o}) VH_HIP(hipMalloc((void**)p, n * sizeof(float)));
+  ctx->bk_cap = n;
+  return VH_OK;
+}
+
+unsigned int backend_blocks(size_t n) {
+  return (unsigned int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, kBackendBlocks));
+}
+
 // ---- contact step -------------------------------------------------------
 
 // Blocks of the grid-stride kernels over candidate pairs: enough to fill the
@@ -566,6 +629,8 @@ vh_status vh_destroy(vh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_all(ctx);
+  for (void* p : {(void*)ctx->bk_a, (void*)ctx->bk_b, (void*)ctx->bk_o, (void*)ctx->bk_part})
+    if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return VH_OK;
@@ -1264,6 +1329,61 @@ vh_status vh_state_lambda_set_f64(vh_ctx* ctx, const double* lambda) {
   const int n = ctx->S.n;
   to_dev_order(ctx->S, lambda, lambda + n, lambda + 2 * (size_t)n, ctx->hbuf);
   VH_HIP(hipMemcpyAsync(ctx->ipc.lambda, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_backend_axpy(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb, float scale,
+                          float* out, uint64_t nout) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (na != nb || na != nout) return fail(ctx, VH_E_INVALID_ARG, "AXPY buffer length mismatch");
+  if (na == 0) return VH_OK;
+  if (!a || !b || !out) return fail(ctx, VH_E_INVALID_ARG, "null buffer");
+  VH_HIP(hipSetDevice(ctx->device));
+  vh_status st = backend_reserve(ctx, na);
+  if (st != VH_OK) return st;
+  VH_HIP(hipMemcpyAsync(ctx->bk_a, a, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(ctx->bk_b, b, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_backend_axpy, dim3(backend_blocks(na)), dim3(256), 0, ctx->stream, (size_t)na, ctx->bk_a,
+                     ctx->bk_b, scale, ctx->bk_o);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(out, ctx->bk_o, na * 4, hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_backend_dot(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb, double* result) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!result) return fail(ctx, VH_E_INVALID_ARG, "null result");
+  if (na != nb) return fail(ctx, VH_E_INVALID_ARG, "Dot product buffer length mismatch");
+  *result = 0.0;
+  if (na == 0) return VH_OK;
+  if (!a || !b) return fail(ctx, VH_E_INVALID_ARG, "null buffer");
+  VH_HIP(hipSetDevice(ctx->device));
+  vh_status st = backend_reserve(ctx, na);
+  if (st != VH_OK) return st;
+  VH_HIP(hipMemcpyAsync(ctx->bk_a, a, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(ctx->bk_b, b, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned int nbk = backend_blocks(na);
+  hipLaunchKernelGGL(k_backend_dot, dim3(nbk), dim3(256), 0, ctx->stream, (size_t)na, ctx->bk_a, ctx->bk_b,
+                     ctx->bk_part);
+  hipLaunchKernelGGL(k_backend_dot_sum, dim3(1), dim3(256), 0, ctx->stream, (int)nbk, ctx->bk_part);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(result, ctx->bk_part + kBackendBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_backend_fill(vh_ctx* ctx, float* buf, uint64_t n, float value) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (n == 0) return VH_OK;
+  if (!buf) return fail(ctx, VH_E_INVALID_ARG, "null buffer");
+  VH_HIP(hipSetDevice(ctx->device));
+  vh_status st = backend_reserve(ctx, n);
+  if (st != VH_OK) return st;
+  hipLaunchKernelGGL(k_backend_fill, dim3(backend_blocks(n)), dim3(256), 0, ctx->stream, (size_t)n, ctx->bk_o, value);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(buf, ctx->bk_o, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
   return VH_OK;
 }

@@ -47,14 +47,16 @@ _STATUS_TO_ERROR = {1: InvalidConfig, 2: InvalidConfig, 3: InvalidConfig, 4: Inv
                     6: GpuError, 7: GpuError, 8: InvalidConfig, 9: InvalidConfig}
 
 
-def _check(status: int, ctx=None, what: str = ""):
+def _check(status: int, ctx=None, what: str = "", err_cls=None):
+    """Raises the VistioError variant of a vh_status (err_cls overrides it:
+    the GpuBackend calls report every failure as VistioError::Gpu)."""
     if status == nat.VH_OK:
         return
     msg = ""
     if ctx:
         raw = nat.load().vh_last_error(ctx)
         msg = raw.decode() if raw else ""
-    cls = _STATUS_TO_ERROR.get(status, VistioError)
+    cls = err_cls or _STATUS_TO_ERROR.get(status, VistioError)
     raise cls(f"{what}: {nat.STATUS_NAMES.get(status, status)}: {msg}".strip(": "))
 
 
