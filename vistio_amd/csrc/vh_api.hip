@@ -364,12 +364,16 @@ vh_status launch_step(vh_ctx* ctx, double dt) {
 
 constexpr int kBackendBlocks = 1024;
 
-// out = a + scale * b, rounded as the reference's separate multiply and add.
+// out = a + scale * b, rounded as the reference's separate multiply and add
+// (no contraction into an FMA).
 __global__ void __launch_bounds__(256) k_backend_axpy(size_t n, const float* __restrict__ a,
                                                       const float* __restrict__ b, float scale,
                                                       float* __restrict__ out) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = __fadd_rn(a[i], __fmul_rn(scale, b[i]));
+#pragma clang fp contract(off)
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const float p = scale * b[i];
+    out[i] = a[i] + p;
+  }
 }
 __global__ void __launch_bounds__(256) k_backend_fill(size_t n, float* __restrict__ p, float value) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
