@@ -42,14 +42,27 @@ typedef enum {
   VH_E_NONFINITE = 9
 } vh_status;
 
-/* Membrane projection, keyed by ConstitutiveModel::name():
- *   VH_MODEL_ARAP            ElementData::project (element.rs:127-144), used by init()
- *   VH_MODEL_COROTATIONAL    "co_rotational" (crates/vistio-material/src/corotational.rs:51-74)
- * Other models ("isotropic_linear", "orthotropic_linear",
- * "anisotropic_corotational") return VH_E_UNSUPPORTED. */
-enum { VH_MODEL_ARAP = 0, VH_MODEL_COROTATIONAL = 1 };
-/* Bending: dihedral springs (crates/vistio-solver/src/bending.rs). */
-enum { VH_BENDING_NONE = 0, VH_BENDING_DIHEDRAL = 1 };
+/* Membrane projection, keyed by ConstitutiveModel::name()
+ * (crates/vistio-material/src/):
+ *   VH_MODEL_ARAP                       ElementData::project (element.rs:127-144), used by init()
+ *   VH_MODEL_COROTATIONAL               "co_rotational" (corotational.rs:51-74)
+ *   VH_MODEL_ISOTROPIC_LINEAR           "isotropic_linear" (isotropic.rs:40-64): target F = I
+ *   VH_MODEL_ORTHOTROPIC_LINEAR         "orthotropic_linear" (orthotropic.rs:52-94),
+ *                                       warp_stiffness / weft_stiffness
+ *   VH_MODEL_ANISOTROPIC_COROTATIONAL   "anisotropic_corotational" (anisotropic.rs:78-143),
+ *                                       warp_stiffness / weft_stiffness
+ * Any other value returns VH_E_UNSUPPORTED. */
+enum {
+  VH_MODEL_ARAP = 0,
+  VH_MODEL_COROTATIONAL = 1,
+  VH_MODEL_ISOTROPIC_LINEAR = 2,
+  VH_MODEL_ORTHOTROPIC_LINEAR = 3,
+  VH_MODEL_ANISOTROPIC_COROTATIONAL = 4
+};
+/* Bending: dihedral springs (crates/vistio-solver/src/bending.rs) or Discrete
+ * Shells (discrete_shells.rs: normalised cotangent stencil per hinge, the
+ * same projection; init_with_material_tier3, pd_solver.rs:152-193). */
+enum { VH_BENDING_NONE = 0, VH_BENDING_DIHEDRAL = 1, VH_BENDING_DISCRETE_SHELLS = 2 };
 /* Precision of the local step and the triangular solves. State (positions,
  * velocities) is always held in FP64 on the device.
  *   VH_PREC_F64     FP64 local step, FP64 solve.
@@ -87,6 +100,15 @@ typedef struct {
   double band_launch_us;
   double band_gbps;
   int32_t task_entries;     /* block entries per solve task; 0 = default (1024) */
+  /* FabricProperties::stretch_stiffness_warp / _weft for the orthotropic and
+   * anisotropic membrane models (their from_properties); unused otherwise. */
+  double warp_stiffness, weft_stiffness;
+  /* FabricProperties::bending_stiffness_warp / _weft (raw values): with
+   * VH_BENDING_DISCRETE_SHELLS and a nonzero pair, each hinge weight is scaled
+   * by its edge's warp/weft alignment as
+   * DiscreteShellsBendingData::from_topology_with_anisotropic_material
+   * (discrete_shells.rs:206-268); k_bending is then the average times 100. */
+  double bend_warp, bend_weft;
 } vh_pd_params;
 
 /* StepResult, crates/vistio-solver/src/strategy.rs:15-24. */

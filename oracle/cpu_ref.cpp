@@ -224,23 +224,58 @@ template <class R> void bending_project(const V3<R>& p0, const V3<R>& p1, const 
   out[3] = rotate_around_axis(pb, mid, axis, -half);
 }
 
-enum Model { MODEL_ARAP = 0, MODEL_COROT = 1 };
+enum Model { MODEL_ARAP = 0, MODEL_COROT = 1, MODEL_ISO = 2, MODEL_ORTHO = 3, MODEL_ANISO = 4 };
 
-// ElementData::project / project_with_model + reconstruct_from_target_f —
-// element.rs:127-144, 153-174, 181-217; CoRotationalModel::project
-// crates/vistio-material/src/corotational.rs:51-72.
-template <class R> void membrane_project(const V3<R>& p0, const V3<R>& p1, const V3<R>& p2,
-                                         const R dm_inv[4], int model, V3<R> out[3]) {
-  M32<R> ds; ds.c0 = p1 - p0; ds.c1 = p2 - p0;
-  const M32<R> f = ds.mul2(dm_inv);  // deformation_gradient, decomposition.rs:147-155
+// Target deformation gradient of each membrane model (ConstitutiveModel::project):
+//   ARAP        R                                     element.rs:127-144
+//   COROT       R S(min(sigma, 1))                    corotational.rs:51-72
+//   ISO         I (Mat3x2::IDENTITY)                  isotropic.rs:46-58
+//   ORTHO       R V diag(s + (1 - s) k) V^T, k = (warp, weft) / max(warp, weft, 1e-6)
+//                                                     orthotropic.rs:58-88
+//   ANISO       R V diag(s > 1 ? 1 + (s - 1)(1 - k) : s) V^T with k0 = kw c^2 + kf s^2,
+//               k1 = kw s^2 + kf c^2 (c, s from v0; kw, kf clamped to [0, 1])
+//                                                     anisotropic.rs:78-143
+template <class R> M32<R> target_of(const M32<R>& f, int model, R warp, R weft) {
+  if (model == MODEL_ISO) {
+    M32<R> I; I.c0 = V3<R>(R(1), R(0), R(0)); I.c1 = V3<R>(R(0), R(1), R(0));
+    return I;
+  }
   const Polar<R> pd = polar_decomposition(f);
-  M32<R> target;
   if (model == MODEL_COROT) {
     R cs[4]; clamp_stretch(pd, cs);
-    target = pd.rot.mul2(cs);
-  } else {
-    target = pd.rot;
+    return pd.rot.mul2(cs);
   }
+  if (model == MODEL_ORTHO || model == MODEL_ANISO) {
+    R t0, t1;
+    if (model == MODEL_ORTHO) {
+      const R mk = std::max(std::max(warp, weft), R(1e-6));
+      const R kw = warp / mk, kf = weft / mk;
+      t0 = pd.s0 + (R(1) - pd.s0) * kw;
+      t1 = pd.s1 + (R(1) - pd.s1) * kf;
+    } else {
+      const R kw = std::min(std::max(warp, R(0)), R(1)), kf = std::min(std::max(weft, R(0)), R(1));
+      const R c2 = pd.v0.x * pd.v0.x, s2 = pd.v0.y * pd.v0.y;
+      const R k0 = kw * c2 + kf * s2, k1 = kw * s2 + kf * c2;
+      t0 = pd.s0 > R(1) ? R(1) + (pd.s0 - R(1)) * (R(1) - k0) : pd.s0;
+      t1 = pd.s1 > R(1) ? R(1) + (pd.s1 - R(1)) * (R(1) - k1) : pd.s1;
+    }
+    const R t00 = pd.v0.x * pd.v0.x * t0 + pd.v1.x * pd.v1.x * t1;
+    const R t01 = pd.v0.x * pd.v0.y * t0 + pd.v1.x * pd.v1.y * t1;
+    const R t11 = pd.v0.y * pd.v0.y * t0 + pd.v1.y * pd.v1.y * t1;
+    const R ts[4] = {t00, t01, t01, t11};
+    return pd.rot.mul2(ts);
+  }
+  return pd.rot;
+}
+
+// ElementData::project / project_with_model + reconstruct_from_target_f —
+// element.rs:127-144, 153-174, 181-217, with the model's target above.
+template <class R> void membrane_project(const V3<R>& p0, const V3<R>& p1, const V3<R>& p2,
+                                         const R dm_inv[4], int model, V3<R> out[3], R warp = R(0),
+                                         R weft = R(0)) {
+  M32<R> ds; ds.c0 = p1 - p0; ds.c1 = p2 - p0;
+  const M32<R> f = ds.mul2(dm_inv);  // deformation_gradient, decomposition.rs:147-155
+  const M32<R> target = target_of(f, model, warp, weft);
   const V3<R> centroid = (p0 + p1 + p2) / R(3);
   const R det = dm_inv[0] * dm_inv[3] - dm_inv[1] * dm_inv[2];
   if (!(std::fabs(det) > R(1e-12))) { out[0] = p0; out[1] = p1; out[2] = p2; return; }
@@ -436,6 +471,9 @@ struct or_params {
   double tolerance;
   int has_ground;
   double ground_height;
+  int bending_kind;       // 1 = dihedral (bending.rs), 2 = Discrete Shells (discrete_shells.rs)
+  double warp, weft;      // membrane model stiffness (orthotropic / anisotropic models)
+  double bend_warp, bend_weft;  // Discrete Shells warp/weft modulation (0, 0 = isotropic)
 };
 
 struct or_step_result { int iterations; double final_residual; int converged; double wall_time; };
@@ -494,6 +532,7 @@ struct OracleBase {
   virtual void local_rhs(double* rhs_out, double* tri_targets, double* hinge_targets) = 0;
   virtual void elements(double* dm, double* area, double* weight) const = 0;
   virtual void hinges(double* rest, double* weight) const = 0;
+  virtual void hinge_stencils(double* st) const = 0;
   virtual int step_ipc(double dt, const or_ipc_params& ip, or_step_result* res, or_ipc_stats* st) = 0;
   virtual void get_lambda(double* out) const = 0;
   virtual void set_lambda(const double* in) = 0;
@@ -501,7 +540,7 @@ struct OracleBase {
 
 template <class R> struct Oracle : OracleBase {
   struct Tri { int i[3]; R dm_inv[4]; R area; R weight; };
-  struct Hinge { int v[4]; R rest; R weight; };
+  struct Hinge { int v[4]; R rest; R weight; R st[4]; };
   std::vector<Tri> tris;
   std::vector<Hinge> hinges_;
   std::vector<R> mass;  // solver lumped masses (pinned -> 1e8)
@@ -556,6 +595,8 @@ template <class R> struct Oracle : OracleBase {
       else if (std::fabs(rest) < R(1e-3)) rest = R(0);
       H.rest = rest;
       H.weight = k_bend * (q[1] - q[0]).length();
+      H.st[0] = R(-1); H.st[1] = R(-1); H.st[2] = R(1); H.st[3] = R(1);  // assembly.rs:97-101
+      if (p->bending_kind == 2) discrete_shells(H, q, p);
     }
     // state — state.rs:85-140
     vx.assign(n, 0); vy.assign(n, 0); vz.assign(n, 0);
@@ -580,11 +621,10 @@ template <class R> struct Oracle : OracleBase {
           if (std::fabs(dot) > R(1e-12)) trip.push_back({T.i[ia], T.i[ib], double(T.weight * dot)});
         }
     }
-    const R st[4] = {-1, -1, 1, 1};
     for (const Hinge& H : hinges_)
       for (int ia = 0; ia < 4; ++ia)
         for (int ib = 0; ib < 4; ++ib) {
-          const R val = H.weight * st[ia] * st[ib];
+          const R val = H.weight * H.st[ia] * H.st[ib];  // assembly.rs:96-128
           if (std::fabs(val) > R(1e-12)) trip.push_back({H.v[ia], H.v[ib], double(val)});
         }
     std::sort(trip.begin(), trip.end(), [](const Trip& A, const Trip& B) {
@@ -607,6 +647,45 @@ template <class R> struct Oracle : OracleBase {
 
   // Local step (membrane + bending) and RHS assembly for one axis each,
   // exactly as pd_solver.rs:296-389 and assembly.rs:157-240.
+  // DiscreteShellsBendingData::from_topology (discrete_shells.rs:106-195) and
+  // the warp/weft modulation of from_topology_with_anisotropic_material
+  // (:206-268): normalised cotangent stencil; weight = k * edge length, times
+  // k_edge / k_avg for the edge's alignment. A zero-length edge has no element.
+  void discrete_shells(Hinge& H, const V3<R> q[4], const or_params* p) {
+    const V3<R> &p0 = q[0], &p1 = q[1], &pa = q[2], &pb = q[3];
+    const R area_a = R(0.5) * (p1 - p0).cross(pa - p0).length();
+    const R area_b = R(0.5) * (p1 - p0).cross(pb - p0).length();
+    const V3<R> edge = p1 - p0;
+    const R el = edge.length();
+    if (el < R(1e-10)) { H.weight = R(0); return; }
+    const R inv_e2 = R(1) / (el * el);
+    const R c_wa = el / (R(2) * std::max(area_a, R(1e-8)));
+    const R c_wb = el / (R(2) * std::max(area_b, R(1e-8)));
+    const R dot0_a = (p1 - pa).dot(edge) * inv_e2, dot0_b = (p1 - pb).dot(edge) * inv_e2;
+    const R c_0 = -dot0_a * c_wa - dot0_b * c_wb;
+    const R dot1_a = (pa - p0).dot(edge) * inv_e2, dot1_b = (pb - p0).dot(edge) * inv_e2;
+    const R c_1 = -dot1_a * c_wa - dot1_b * c_wb;
+    const R raw[4] = {c_0, c_1, c_wa, c_wb};
+    const R nrm = std::sqrt(raw[0] * raw[0] + raw[1] * raw[1] + raw[2] * raw[2] + raw[3] * raw[3]);
+    if (nrm > R(1e-10)) {
+      for (int k = 0; k < 4; ++k) H.st[k] = raw[k] / nrm;
+    } else {
+      H.st[0] = R(-0.5); H.st[1] = R(-0.5); H.st[2] = R(0.5); H.st[3] = R(0.5);
+    }
+    const R kw = R(p->bend_warp), kf = R(p->bend_weft);
+    const R avg_raw = (kw + kf) / R(2);
+    if (avg_raw > R(1e-8)) {
+      const V3<R> dir = edge / el;
+      const V3<R> hz(dir.x, R(0), dir.z);
+      const R hl = hz.length();
+      if (hl >= R(1e-10)) {
+        const R c = std::fabs(hz.x / hl);
+        const R c2 = c * c, s2 = R(1) - c2;
+        H.weight *= (kf * c2 + kw * s2) / avg_raw;
+      }
+    }
+  }
+
   void local_and_rhs(const R dt, std::vector<R> rhs[3], double* tri_targets, double* hinge_targets) {
     std::vector<V3<R>> tt(size_t(3) * t);
     std::vector<V3<R>> ht(size_t(4) * h);
@@ -618,7 +697,7 @@ template <class R> struct Oracle : OracleBase {
       const Tri& T = tris[e];
       V3<R> p[3];
       for (int k = 0; k < 3; ++k) p[k] = V3<R>(px[T.i[k]], py[T.i[k]], pz[T.i[k]]);
-      membrane_project(p[0], p[1], p[2], T.dm_inv, model, &tt[size_t(3) * e]);
+      membrane_project(p[0], p[1], p[2], T.dm_inv, model, &tt[size_t(3) * e], R(P.warp), R(P.weft));
     }
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < h; ++e) {
@@ -659,9 +738,9 @@ template <class R> struct Oracle : OracleBase {
           b[T.i[k]] += T.weight * contrib;
         }
       }
-      const R st[4] = {-1, -1, 1, 1};
       for (int e = 0; e < h; ++e) {
         const Hinge& H = hinges_[e];
+        const R* st = H.st;  // [-1, -1, 1, 1] (assembly.rs:210) or the Discrete Shells stencil (:222-236)
         R tg[4];
         for (int k = 0; k < 4; ++k) {
           const V3<R>& v = ht[size_t(4) * e + k];
@@ -989,6 +1068,10 @@ template <class R> struct Oracle : OracleBase {
       weight[e] = double(tris[e].weight);
     }
   }
+  void hinge_stencils(double* st) const override {
+    for (int e = 0; e < h; ++e)
+      for (int k = 0; k < 4; ++k) st[4 * e + k] = double(hinges_[e].st[k]);
+  }
   void hinges(double* rest, double* weight) const override {
     for (int e = 0; e < h; ++e) { rest[e] = double(hinges_[e].rest); weight[e] = double(hinges_[e].weight); }
   }
@@ -1060,13 +1143,13 @@ void or_polar(int mode, const double* F, double* out) {
   if (mode == 0) run(0.0); else run(0.0f);
 }
 // Co-rotational (model=1) or ARAP (model=0) target gradient R*S_clamped / R. out: 6 values.
-void or_target_f(int mode, int model, const double* F, double* out) {
+void or_target_f_ex(int mode, int model, const double* F, double warp, double weft, double* out);
+void or_target_f(int mode, int model, const double* F, double* out) { or_target_f_ex(mode, model, F, 0.0, 0.0, out); }
+void or_target_f_ex(int mode, int model, const double* F, double warp, double weft, double* out) {
   auto run = [&](auto zero) {
     using R = decltype(zero);
     M32<R> f; f.c0 = V3<R>(R(F[0]), R(F[1]), R(F[2])); f.c1 = V3<R>(R(F[3]), R(F[4]), R(F[5]));
-    Polar<R> p = polar_decomposition(f);
-    M32<R> tg = p.rot;
-    if (model == MODEL_COROT) { R cs[4]; clamp_stretch(p, cs); tg = p.rot.mul2(cs); }
+    const M32<R> tg = target_of(f, model, R(warp), R(weft));
     const R v[6] = {tg.c0.x, tg.c0.y, tg.c0.z, tg.c1.x, tg.c1.y, tg.c1.z};
     for (int k = 0; k < 6; ++k) out[k] = double(v[k]);
   };
@@ -1104,6 +1187,7 @@ void or_bend_project(int mode, const double* p, double rest, double* out) {
   if (mode == 0) run(0.0); else run(0.0f);
 }
 // p: 9 values (p0, p1, p2), dm_inv: 4. out: 9 values (targets).
+void or_hinge_stencils(void* o, double* st) { static_cast<OracleBase*>(o)->hinge_stencils(st); }
 void or_membrane_project(int mode, int model, const double* p, const double* dm, double* out) {
   auto run = [&](auto zero) {
     using R = decltype(zero);

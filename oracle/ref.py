@@ -34,7 +34,16 @@ class OrParams(C.Structure):
         ("tolerance", C.c_double),
         ("has_ground", C.c_int),
         ("ground_height", C.c_double),
+        ("bending_kind", C.c_int),
+        ("warp", C.c_double),
+        ("weft", C.c_double),
+        ("bend_warp", C.c_double),
+        ("bend_weft", C.c_double),
     ]
+
+
+MODEL_ARAP, MODEL_COROT, MODEL_ISO, MODEL_ORTHO, MODEL_ANISO = 0, 1, 2, 3, 4
+BENDING_DIHEDRAL, BENDING_DISCRETE_SHELLS = 1, 2
 
 
 class OrStepResult(C.Structure):
@@ -108,6 +117,8 @@ def lib():
                                 C.POINTER(C.c_uint8), C.POINTER(OrParams)]
         L.or_destroy.argtypes = [vp]
         L.or_step.argtypes = [vp, C.c_double, C.c_int, C.POINTER(OrStepResult)]
+        L.or_target_f_ex.argtypes = [C.c_int, C.c_int, dp, C.c_double, C.c_double, dp]
+        L.or_hinge_stencils.argtypes = [vp, dp]
         L.or_set_threads.argtypes = [C.c_int]
         L.or_get_threads.restype = C.c_int
         L.or_get.argtypes = [vp, C.c_int, dp]
@@ -191,8 +202,10 @@ def _d(a):
 
 def params(model=1, k_membrane=1e3, k_bending=1e-4, density=0.15, dt_matrix=1.0 / 60.0,
            gravity=(0.0, -9.81, 0.0), damping=0.0, rayleigh=0.01, max_iterations=10,
-           tolerance=0.0, ground_height=None) -> OrParams:
+           tolerance=0.0, ground_height=None, bending_kind=BENDING_DIHEDRAL, warp=0.0, weft=0.0,
+           bend_warp=0.0, bend_weft=0.0) -> OrParams:
     p = OrParams()
+    p.bending_kind, p.warp, p.weft, p.bend_warp, p.bend_weft = bending_kind, warp, weft, bend_warp, bend_weft
     p.model = model
     p.k_membrane, p.k_bending, p.density, p.dt_matrix = k_membrane, k_bending, density, dt_matrix
     for k in range(3):
@@ -275,6 +288,11 @@ class Oracle:
         lib().or_elements(self._h, _p(dm, C.c_double), _p(area, C.c_double), _p(w, C.c_double))
         return dm.reshape(self.t, 4), area, w
 
+    def hinge_stencils(self) -> np.ndarray:
+        st = np.empty(4 * self.h)
+        lib().or_hinge_stencils(self._h, _p(st, C.c_double))
+        return st.reshape(self.h, 4)
+
     def hinges(self):
         rest, w = np.empty(self.h), np.empty(self.h)
         lib().or_hinges(self._h, _p(rest, C.c_double), _p(w, C.c_double))
@@ -311,10 +329,10 @@ def polar(F, mode=F64):
     return dict(rot=out[0:6].reshape(2, 3), stretch=out[6:10], s=out[10:12], v0=out[12:14], v1=out[14:16])
 
 
-def target_f(F, model=1, mode=F64):
+def target_f(F, model=1, mode=F64, warp=0.0, weft=0.0):
     F = _d(F).reshape(6)
     out = np.empty(6)
-    lib().or_target_f(mode, model, _p(F, C.c_double), _p(out, C.c_double))
+    lib().or_target_f_ex(mode, model, _p(F, C.c_double), warp, weft, _p(out, C.c_double))
     return out.reshape(2, 3)
 
 

@@ -5,10 +5,10 @@ include/vistio_hip.h); this package is the host-side mirror of the
 reference's solver interface.
 """
 from .solver import (FabricProperties, GpuError, HipPdSolver, InvalidConfig, InvalidMesh, IpcConfig,
-                     SimulationState, SolverConfig, StepResult, Topology, TriangleMesh, VistioError,
-                     estimate_initial_kappa, quad_grid)
+                     MaterialDatabase, SimulationState, SolverConfig, StepResult, Topology, TriangleMesh,
+                     VistioError, estimate_initial_kappa, quad_grid)
 from .backend import ComputeBuffer, HipBackend
 
 __all__ = ["ComputeBuffer", "FabricProperties", "GpuError", "HipBackend", "HipPdSolver", "InvalidConfig",
-           "InvalidMesh", "IpcConfig", "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh",
+           "InvalidMesh", "IpcConfig", "MaterialDatabase", "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh",
            "VistioError", "estimate_initial_kappa", "quad_grid"]

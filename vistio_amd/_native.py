@@ -18,8 +18,9 @@ STATUS_NAMES = {
     4: "VH_E_UNSUPPORTED", 5: "VH_E_INVALID_MESH", 6: "VH_E_GPU", 7: "VH_E_INTERNAL",
     8: "VH_E_INVALID_ARG", 9: "VH_E_NONFINITE",
 }
-MODEL_ARAP, MODEL_COROTATIONAL = 0, 1
-BENDING_NONE, BENDING_DIHEDRAL = 0, 1
+MODEL_ARAP, MODEL_COROTATIONAL, MODEL_ISOTROPIC_LINEAR, MODEL_ORTHOTROPIC_LINEAR, MODEL_ANISOTROPIC_COROTATIONAL = \
+    0, 1, 2, 3, 4
+BENDING_NONE, BENDING_DIHEDRAL, BENDING_DISCRETE_SHELLS = 0, 1, 2
 PREC_F64, PREC_F32_IR = 0, 1
 
 # Every symbol include/vistio_hip.h declares.
@@ -43,6 +44,8 @@ class PdParams(C.Structure):
         ("ir_sweeps", C.c_int32), ("ir_final_sweeps", C.c_int32), ("leaf_size", C.c_int32),
         ("max_band_levels", C.c_int32), ("band_launch_us", C.c_double), ("band_gbps", C.c_double),
         ("task_entries", C.c_int32),
+        ("warp_stiffness", C.c_double), ("weft_stiffness", C.c_double),
+        ("bend_warp", C.c_double), ("bend_weft", C.c_double),
     ]
 
 

@@ -75,6 +75,8 @@ struct vh_ctx {
   int *tri_idx = nullptr, *tri_slot = nullptr, *hinge_idx = nullptr, *hinge_slot = nullptr;
   double* tri_dm = nullptr;
   void *tri_w = nullptr, *hinge_rest = nullptr, *hinge_w = nullptr;
+  void* hinge_st = nullptr;  // Discrete Shells stencils (T [h][4]); null for dihedral springs
+  vhk::MemModel mem{};       // membrane model and its normalised warp/weft factors
   void* cslot = nullptr;  // RHS contribution slots, T [3t + 4h][3]
   int* inc_ptr = nullptr;
   // solve
@@ -275,9 +277,9 @@ void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
   const int tb = nblk(S.t), hb = nblk(S.h);
   if (tb + hb > 0)
-    hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb,
-                       c->P.model == VH_MODEL_COROTATIONAL ? 1 : 0, c->x, c->tri_idx, c->tri_dm, (const T*)c->tri_w,
-                       c->tri_slot, c->hinge_idx, (const T*)c->hinge_rest, (const T*)c->hinge_w, c->hinge_slot,
+    hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb, c->mem, c->x,
+                       c->tri_idx, c->tri_dm, (const T*)c->tri_w, c->tri_slot, c->hinge_idx,
+                       (const T*)c->hinge_rest, (const T*)c->hinge_w, (const T*)c->hinge_st, c->hinge_slot,
                        (T*)c->cslot, st);
 }
 
@@ -657,10 +659,12 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   VH_HIP(hipStreamSynchronize(ctx->stream));
   free_all(ctx);
   const vh_pd_params& P = *params;
-  if (P.model != VH_MODEL_ARAP && P.model != VH_MODEL_COROTATIONAL)
+  if (P.model < VH_MODEL_ARAP || P.model > VH_MODEL_ANISOTROPIC_COROTATIONAL)
     return fail(ctx, VH_E_UNSUPPORTED, "unsupported constitutive model");
-  if (P.bending != VH_BENDING_NONE && P.bending != VH_BENDING_DIHEDRAL)
+  if (P.bending != VH_BENDING_NONE && P.bending != VH_BENDING_DIHEDRAL && P.bending != VH_BENDING_DISCRETE_SHELLS)
     return fail(ctx, VH_E_UNSUPPORTED, "unsupported bending model");
+  if (!(P.warp_stiffness >= 0) || !(P.weft_stiffness >= 0) || !(P.bend_warp >= 0) || !(P.bend_weft >= 0))
+    return fail(ctx, VH_E_INVALID_CONFIG, "warp/weft stiffness must be >= 0");
   if (P.precision != VH_PREC_F64 && P.precision != VH_PREC_F32_IR)
     return fail(ctx, VH_E_UNSUPPORTED, "unsupported precision mode");
   if (P.max_iterations < 0) return fail(ctx, VH_E_INVALID_CONFIG, "max_iterations must be >= 0");
@@ -669,11 +673,23 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
     return fail(ctx, VH_E_INVALID_CONFIG, "ir_sweeps and ir_final_sweeps must be in 1..3");
   if (n == 0) return fail(ctx, VH_E_INVALID_MESH, "empty mesh");
   ctx->P = P;
+  ctx->mem.kind = P.model;
+  ctx->mem.kw = ctx->mem.kf = 0.0;
+  if (P.model == VH_MODEL_ORTHOTROPIC_LINEAR) {  // orthotropic.rs:64-66
+    const double mk = std::max(std::max(P.warp_stiffness, P.weft_stiffness), 1e-6);
+    ctx->mem.kw = P.warp_stiffness / mk;
+    ctx->mem.kf = P.weft_stiffness / mk;
+  } else if (P.model == VH_MODEL_ANISOTROPIC_COROTATIONAL) {  // anisotropic.rs:88-89
+    ctx->mem.kw = std::min(std::max(P.warp_stiffness, 0.0), 1.0);
+    ctx->mem.kf = std::min(std::max(P.weft_stiffness, 0.0), 1.0);
+  }
   ctx->prec = P.precision;
   ctx->tsz = P.precision == VH_PREC_F64 ? 8 : 4;
   const auto t0 = std::chrono::steady_clock::now();
   vh::SetupInput in;
   in.n = (int)n; in.t = (int)t; in.h = P.bending == VH_BENDING_NONE ? 0 : (int)h;
+  in.bending_kind = P.bending == VH_BENDING_DISCRETE_SHELLS ? 2 : 1;
+  in.bend_warp = P.bend_warp; in.bend_weft = P.bend_weft;
   in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge; in.pinned = pinned;
   in.k_membrane = P.k_membrane; in.k_bending = P.k_bending; in.density = P.density; in.dt_matrix = P.dt_matrix;
   if (mat) {
@@ -714,6 +730,8 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->hinge_slot, S.hinge_slot));
   UP(upload_t(ctx, &ctx->hinge_rest, S.hinge_rest));
   UP(upload_t(ctx, &ctx->hinge_w, S.hinge_w));
+  ctx->hinge_st = nullptr;
+  if (!S.hinge_st.empty()) UP(upload_t(ctx, &ctx->hinge_st, S.hinge_st));
   UP(dalloc_bytes(ctx, &ctx->cslot, 3 * (size_t)std::max(1, S.inc_ptr[S.n]) * ctx->tsz));
   UP(upload(ctx, &ctx->inc_ptr, S.inc_ptr));
   std::vector<vhk::TaskDev> td(S.ptasks.size());

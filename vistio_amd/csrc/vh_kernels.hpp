@@ -61,15 +61,49 @@ __device__ __forceinline__ float3 fcross(float3 a, float3 b) {
 }
 __device__ __forceinline__ float flen(float3 a) { return sqrtf(fdot(a, a)); }
 
+// Membrane model (ConstitutiveModel, keyed by name) and its stiffness
+// factors, already normalised on the host: orthotropic_linear divides warp
+// and weft by their maximum (orthotropic.rs:64-66), anisotropic_corotational
+// clamps them to [0, 1] (anisotropic.rs:88-89).
+enum MemKind { kArap = 0, kCorot = 1, kIsoLinear = 2, kOrtho = 3, kAniso = 4 };
+struct MemModel {
+  int kind;
+  double kw, kf;
+};
+
+// Target singular values of the stretch-based models for sigma = (s0, s1)
+// with principal direction v0 = (v0x, v0y) of s0:
+//   co-rotational  min(s, 1)                                  corotational.rs:51-72
+//   orthotropic    s + (1 - s) (kw, kf)                       orthotropic.rs:68-70
+//   anisotropic    s > 1 ? 1 + (s - 1)(1 - k) : s, k0 = kw c^2 + kf s^2,
+//                  k1 = kw s^2 + kf c^2 with c^2 = v0x^2       anisotropic.rs:96-126
+__device__ __forceinline__ void stretch_targets(const MemModel& m, double s0, double s1, double v0x, double v0y,
+                                                double& t0, double& t1) {
+  if (m.kind == kOrtho) {
+    t0 = s0 + (1.0 - s0) * m.kw;
+    t1 = s1 + (1.0 - s1) * m.kf;
+  } else if (m.kind == kAniso) {
+    const double c2 = v0x * v0x, s2 = v0y * v0y;
+    const double k0 = m.kw * c2 + m.kf * s2, k1 = m.kw * s2 + m.kf * c2;
+    t0 = s0 > 1.0 ? 1.0 + (s0 - 1.0) * (1.0 - k0) : s0;
+    t1 = s1 > 1.0 ? 1.0 + (s1 - 1.0) * (1.0 - k1) : s1;
+  } else {
+    t0 = fmin(s0, 1.0);
+    t1 = fmin(s1, 1.0);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Membrane projection, FP64, following the reference formulas:
 // deformation_gradient (decomposition.rs:147-155), polar_decomposition_3x2
 // (decomposition.rs:76-138), clamp_stretch (decomposition.rs:50-66),
 // CoRotationalModel::project (corotational.rs:51-72), ARAP target = R
 // (element.rs:127-144), degenerate Dm^{-1} -> unchanged positions
-// (element.rs:192-202). Returns D = F_target - F.
+// (element.rs:192-202), and the other models' targets (isotropic_linear: I,
+// isotropic.rs:46-58; orthotropic / anisotropic: stretch_targets above).
+// Returns D = F_target - F.
 // ---------------------------------------------------------------------------
-__device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, const double* dm, int corot,
+__device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, const double* dm, const MemModel& mm,
                                           double3& D0, double3& D1) {
   const double3 s0v = p1 - p0, s1v = p2 - p0;
   const double3 F0 = s0v * dm[0] + s1v * dm[1];
@@ -78,6 +112,12 @@ __device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, co
   D1 = make_double3(0, 0, 0);
   const double det = dm[0] * dm[3] - dm[1] * dm[2];
   if (!(fabs(det) > 1e-12)) return;
+  if (mm.kind == kIsoLinear) {
+    D0 = make_double3(1.0 - F0.x, -F0.y, -F0.z);
+    D1 = make_double3(-F1.x, 1.0 - F1.y, -F1.z);
+    return;
+  }
+  const bool stretch_model = mm.kind != kArap;
   const double a = ddot(F0, F0), b = ddot(F0, F1), d = ddot(F1, F1);
   const double ht = 0.5 * (a + d), hd = 0.5 * (a - d);
   const double disc = sqrt(hd * hd + b * b);
@@ -86,12 +126,14 @@ __device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, co
   const double eps = 1e-10;
   double3 R0, R1;
   double v0x, v0y, v1x, v1y;
-  if (s0 < eps) {
+  if (s0 < eps) {  // polar_decomposition_3x2's degenerate result: R = I, sigma = 0, V = I
     R0 = make_double3(1, 0, 0); R1 = make_double3(0, 1, 0);
     v0x = 1; v0y = 0; v1x = 0; v1y = 1;
-    if (corot) {  // stretch 0, singular values 0 -> clamped target 0
-      D0 = make_double3(-F0.x, -F0.y, -F0.z);
-      D1 = make_double3(-F1.x, -F1.y, -F1.z);
+    if (stretch_model) {
+      double t0, t1;
+      stretch_targets(mm, 0.0, 0.0, 1.0, 0.0, t0, t1);
+      D0 = R0 * t0 - F0;
+      D1 = R1 * t1 - F1;
     } else {
       D0 = R0 - F0; D1 = R1 - F1;
     }
@@ -114,8 +156,9 @@ __device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, co
   R0 = F0 * si00 + F1 * si01;
   R1 = F0 * si01 + F1 * si11;
   double3 T0 = R0, T1 = R1;
-  if (corot) {
-    const double c0 = fmin(s0, 1.0), c1 = fmin(s1, 1.0);
+  if (stretch_model) {
+    double c0, c1;
+    stretch_targets(mm, s0, s1, v0x, v0y, c0, c1);
     const double t00 = v0x * v0x * c0 + v1x * v1x * c1;
     const double t01 = v0x * v0y * c0 + v1x * v1y * c1;
     const double t11 = v0y * v0y * c0 + v1y * v1y * c1;
@@ -135,7 +178,7 @@ __device__ inline void membrane_delta_f64(double3 p0, double3 p1, double3 p2, co
 // eigenvector uses whichever of (mu0 - e_d, b) and (b, mu0 - e_a) avoids
 // cancellation (both are the reference's (lambda0 - d, b) direction).
 // Degenerate singular values fall back to the FP64 path.
-__device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, const double* dm, int corot,
+__device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, const double* dm, const MemModel& mm,
                                           float3& D0, float3& D1) {
   const double3 s0v = p1 - p0, s1v = p2 - p0;
   const double3 F0 = s0v * dm[0] + s1v * dm[1];
@@ -144,6 +187,11 @@ __device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, co
   D1 = make_float3(0, 0, 0);
   const double det = dm[0] * dm[3] - dm[1] * dm[2];
   if (!(fabs(det) > 1e-12)) return;
+  if (mm.kind == kIsoLinear) {  // I - F, formed in FP64 and rounded once
+    D0 = make_float3((float)(1.0 - F0.x), (float)-F0.y, (float)-F0.z);
+    D1 = make_float3((float)-F1.x, (float)(1.0 - F1.y), (float)-F1.z);
+    return;
+  }
   const float ea = (float)(ddot(F0, F0) - 1.0);
   const float eb = (float)ddot(F0, F1);
   const float ed = (float)(ddot(F1, F1) - 1.0);
@@ -155,7 +203,7 @@ __device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, co
   // in FP32, so such elements take the FP64 path.
   if (s0 < 1e-10f || s1 < 0.25f) {
     double3 E0, E1;
-    membrane_delta_f64(p0, p1, p2, dm, corot, E0, E1);
+    membrane_delta_f64(p0, p1, p2, dm, mm, E0, E1);
     D0 = f3(E0); D1 = f3(E1);
     return;
   }
@@ -172,8 +220,20 @@ __device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, co
   }
   const float v1x = -v0y, v1y = v0x;  // sign of v1 is irrelevant in V diag V^T
   const float m0 = mu0 / (1.0f + s0), m1 = mu1 / (1.0f + s1);  // sigma - 1
-  const float d0 = corot ? -fmaxf(m0, 0.0f) : -m0;
-  const float d1 = corot ? -fmaxf(m1, 0.0f) : -m1;
+  // d_i = target_i - sigma_i in terms of m_i = sigma_i - 1 (stretch_targets)
+  float d0, d1;
+  if (mm.kind == kArap) {
+    d0 = -m0; d1 = -m1;
+  } else if (mm.kind == kOrtho) {
+    d0 = -m0 * (float)mm.kw; d1 = -m1 * (float)mm.kf;
+  } else if (mm.kind == kAniso) {
+    const float c2 = v0x * v0x, s2 = v0y * v0y;
+    const float kw = (float)mm.kw, kf = (float)mm.kf;
+    d0 = -fmaxf(m0, 0.0f) * (kw * c2 + kf * s2);
+    d1 = -fmaxf(m1, 0.0f) * (kw * s2 + kf * c2);
+  } else {
+    d0 = -fmaxf(m0, 0.0f); d1 = -fmaxf(m1, 0.0f);
+  }
   const float k0 = d0 / s0, k1 = d1 / s1;
   const float K00 = v0x * v0x * k0 + v1x * v1x * k1;
   const float K01 = v0x * v0y * k0 + v1x * v1y * k1;
@@ -185,9 +245,12 @@ __device__ inline void membrane_delta_f32(double3 p0, double3 p1, double3 p2, co
 
 // Dihedral bending, FP64, following compute_dihedral_angle
 // (bending.rs:56-91), BendingData::project (bending.rs:182-227) and
-// rotate_around_axis (bending.rs:231-239). Returns s^T(target - x) =
-// (t_a - x_a) + (t_b - x_b) (the edge vertices are not moved).
-__device__ inline double3 bending_delta_f64(double3 p0, double3 p1, double3 pa, double3 pb, double rest) {
+// rotate_around_axis (bending.rs:231-239); the Discrete Shells projection
+// is the same rotation (discrete_shells.rs:285-340). Returns s^T(target - x)
+// = s_a (t_a - x_a) + s_b (t_b - x_b) (the edge vertices are not moved);
+// the dihedral stencil has s_a = s_b = 1.
+__device__ inline double3 bending_delta_f64(double3 p0, double3 p1, double3 pa, double3 pb, double rest, double sa_,
+                                           double sb_) {
   const double3 zero = make_double3(0, 0, 0);
   const double3 edge = p1 - p0;
   const double el = dlen(edge);
@@ -219,14 +282,15 @@ __device__ inline double3 bending_delta_f64(double3 p0, double3 p1, double3 pa, 
   const double3 va = pa - mid, vb = pb - mid;
   const double3 ra = va * ca + dcross(axis, va) * sa + axis * ddot(axis, va) * (1.0 - ca);
   const double3 rb = vb * ca + dcross(axis, vb) * (-sa) + axis * ddot(axis, vb) * (1.0 - ca);
-  return ((mid + ra) - pa) + ((mid + rb) - pb);
+  return ((mid + ra) - pa) * sa_ + ((mid + rb) - pb) * sb_;
 }
 
 // Dihedral bending, FP32 residual form: vectors relative to v0 (formed in
 // FP64, rounded once), angle by atan2(|qa x qb|, qa . qb) (same value as the
 // reference's acos of the normalised dot, without its loss near pi), and the
 // displacement of each wing as sin(h) (axis x v) - 2 sin^2(h/2) v_perp.
-__device__ inline float3 bending_delta_f32(double3 p0, double3 p1, double3 pa, double3 pb, float rest) {
+__device__ inline float3 bending_delta_f32(double3 p0, double3 p1, double3 pa, double3 pb, float rest, float sa_,
+                                          float sb_) {
   const float3 zero = make_float3(0, 0, 0);
   const float3 e = f3(p1 - p0), a = f3(pa - p0), b = f3(pb - p0);
   const float el = flen(e);
@@ -248,7 +312,7 @@ __device__ inline float3 bending_delta_f32(double3 p0, double3 p1, double3 pa, d
   const float3 va = a - m, vb = b - m;
   const float3 da = fcross(axis, va) * sh - (va - axis * fdot(axis, va)) * omc;
   const float3 db = fcross(axis, vb) * (-sh) - (vb - axis * fdot(axis, vb)) * omc;
-  return da + db;
+  return da * sa_ + db * sb_;
 }
 
 template <class T> struct Prec;
@@ -259,15 +323,17 @@ template <> struct Prec<float> { static constexpr bool f32 = true; };
 // pd_solver.rs:300-367 (the loops) and the per-element RHS terms of
 // assembly.rs:176-235. Each element corner's contribution goes to its own
 // slot of the vertex-sorted slot array (a hinge's contribution enters with
-// the stencil sign s = (-1, -1, +1, +1)[k], assembly.rs:210), so the RHS
+// the stencil weight s[k]: (-1, -1, +1, +1) for dihedral springs,
+// assembly.rs:210, or the hinge's Discrete Shells stencil, :222-236), so the RHS
 // gather is a contiguous segmented sum with no atomics.
 template <class T>
-__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, const double* __restrict__ x,
+__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, MemModel mm, const double* __restrict__ x,
                                                const int* __restrict__ tri_idx, const double* __restrict__ tri_dm,
                                                const T* __restrict__ tri_w, const int* __restrict__ tri_slot,
                                                const int* __restrict__ hinge_idx, const T* __restrict__ hinge_rest,
-                                               const T* __restrict__ hinge_w, const int* __restrict__ hinge_slot,
-                                               T* __restrict__ cslot, const StepDev* st) {
+                                               const T* __restrict__ hinge_w, const T* __restrict__ hinge_st,
+                                               const int* __restrict__ hinge_slot, T* __restrict__ cslot,
+                                               const StepDev* st) {
   if (st && st->done) return;
   if ((int)blockIdx.x < tb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -282,11 +348,11 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, 
     T Dx0, Dy0, Dz0, Dx1, Dy1, Dz1;
     if (Prec<T>::f32) {
       float3 D0, D1;
-      membrane_delta_f32(p0, p1, p2, dm, corot, D0, D1);
+      membrane_delta_f32(p0, p1, p2, dm, mm, D0, D1);
       Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
     } else {
       double3 D0, D1;
-      membrane_delta_f64(p0, p1, p2, dm, corot, D0, D1);
+      membrane_delta_f64(p0, p1, p2, dm, mm, D0, D1);
       Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
     }
 #pragma unroll
@@ -302,20 +368,24 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, int corot, 
     const int* q = hinge_idx + 4 * (size_t)e;
     const double3 p0 = ld3(x, q[0]), p1 = ld3(x, q[1]), pa = ld3(x, q[2]), pb = ld3(x, q[3]);
     const T w = hinge_w[e];
+    // stencil s: (-1, -1, 1, 1) for dihedral springs, else the hinge's
+    // Discrete Shells stencil
+    T sg[4] = {T(-1), T(-1), T(1), T(1)};
+    if (hinge_st)
+      for (int k = 0; k < 4; ++k) sg[k] = hinge_st[4 * (size_t)e + k];
     T cx, cy, cz;
     if (Prec<T>::f32) {
-      const float3 c = bending_delta_f32(p0, p1, pa, pb, (float)hinge_rest[e]);
+      const float3 c = bending_delta_f32(p0, p1, pa, pb, (float)hinge_rest[e], (float)sg[2], (float)sg[3]);
       cx = c.x; cy = c.y; cz = c.z;
     } else {
-      const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)hinge_rest[e]);
+      const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)hinge_rest[e], (double)sg[2], (double)sg[3]);
       cx = c.x; cy = c.y; cz = c.z;
     }
     cx *= w; cy *= w; cz *= w;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       T* o = cslot + 3 * (size_t)hinge_slot[4 * e + k];
-      const T sg = k < 2 ? T(-1) : T(1);
-      o[0] = sg * cx; o[1] = sg * cy; o[2] = sg * cz;
+      o[0] = sg[k] * cx; o[1] = sg[k] * cy; o[2] = sg[k] * cz;
     }
   }
 }

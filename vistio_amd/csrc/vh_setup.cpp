@@ -197,7 +197,8 @@ bool build_setup(const SetupInput& in, Setup& S) {
   }
   S.mass_orig = mass;
   // Bending rest data, bending.rs:98-148 (snap to pi / 0 within 1e-3).
-  std::vector<double> hrest(h), hw(h);
+  const bool ds = in.bending_kind == 2;
+  std::vector<double> hrest(h), hw(h), hst(4 * (size_t)h);
   for (int e = 0; e < h; ++e) {
     const uint32_t* q = in.hinge + 4 * (size_t)e;
     double a = dihedral(P(q[0]), P(q[1]), P(q[2]), P(q[3]));
@@ -205,6 +206,39 @@ bool build_setup(const SetupInput& in, Setup& S) {
     else if (std::fabs(a) < 1e-3) a = 0.0;
     hrest[e] = a;
     hw[e] = (in.k_bending_hinge ? in.k_bending_hinge[e] : in.k_bending) * len(sub(P(q[1]), P(q[0])));
+    double* st = &hst[4 * (size_t)e];
+    st[0] = -1.0; st[1] = -1.0; st[2] = 1.0; st[3] = 1.0;  // assembly.rs:97-101
+    if (!ds) continue;
+    // DiscreteShellsBendingData::from_topology (discrete_shells.rs:106-195):
+    // cotangent stencil from the dihedral angle's gradient, normalised to
+    // unit length; a zero-length edge has no element.
+    const D3 p0 = P(q[0]), p1 = P(q[1]), pa = P(q[2]), pb = P(q[3]);
+    const D3 edge = sub(p1, p0);
+    const double el = len(edge);
+    if (el < 1e-10) { hw[e] = 0.0; continue; }
+    const double area_a = 0.5 * len(cross(edge, sub(pa, p0))), area_b = 0.5 * len(cross(edge, sub(pb, p0)));
+    const double inv_e2 = 1.0 / (el * el);
+    const double c_wa = el / (2.0 * std::max(area_a, 1e-8)), c_wb = el / (2.0 * std::max(area_b, 1e-8));
+    const double c_0 = -(dot(sub(p1, pa), edge) * inv_e2) * c_wa - (dot(sub(p1, pb), edge) * inv_e2) * c_wb;
+    const double c_1 = -(dot(sub(pa, p0), edge) * inv_e2) * c_wa - (dot(sub(pb, p0), edge) * inv_e2) * c_wb;
+    const double nrm = std::sqrt(c_0 * c_0 + c_1 * c_1 + c_wa * c_wa + c_wb * c_wb);
+    if (nrm > 1e-10) {
+      st[0] = c_0 / nrm; st[1] = c_1 / nrm; st[2] = c_wa / nrm; st[3] = c_wb / nrm;
+    } else {
+      st[0] = -0.5; st[1] = -0.5; st[2] = 0.5; st[3] = 0.5;
+    }
+    // from_topology_with_anisotropic_material (discrete_shells.rs:206-268):
+    // weight *= (k_weft cos^2 + k_warp sin^2) / k_avg, the angle taken between
+    // the edge's XZ projection and X (warp).
+    const double avg_raw = 0.5 * (in.bend_warp + in.bend_weft);
+    if (avg_raw > 1e-8) {
+      const D3 dir = D3{edge.x / el, edge.y / el, edge.z / el};
+      const double hl = std::sqrt(dir.x * dir.x + dir.z * dir.z);
+      if (hl >= 1e-10) {
+        const double c = std::fabs(dir.x / hl), c2 = c * c, s2 = 1.0 - c2;
+        hw[e] *= (in.bend_weft * c2 + in.bend_warp * s2) / avg_raw;
+      }
+    }
   }
 
   // System matrix A = M/h^2 + sum w g g^T + sum w_b s s^T, assembly.rs:44-134,
@@ -223,10 +257,10 @@ bool build_setup(const SetupInput& in, Setup& S) {
         if (std::fabs(v) > 1e-12) trip.push_back({(int32_t)in.tri[3 * e + a], (int32_t)in.tri[3 * e + b], tw[e] * v});
       }
   }
-  const double st[4] = {-1, -1, 1, 1};
   for (int e = 0; e < h; ++e)
     for (int a = 0; a < 4; ++a)
       for (int b = 0; b < 4; ++b) {
+        const double* st = &hst[4 * (size_t)e];
         const double v = hw[e] * st[a] * st[b];
         if (std::fabs(v) > 1e-12)
           trip.push_back({(int32_t)in.hinge[4 * e + a], (int32_t)in.hinge[4 * e + b], v});
@@ -630,11 +664,14 @@ bool build_setup(const SetupInput& in, Setup& S) {
     };
     std::stable_sort(hord.begin(), hord.end(), [&](int a, int b) { return hmin(a) < hmin(b); });
     S.hinge_idx.resize(4 * (size_t)h); S.hinge_rest.resize(h); S.hinge_w.resize(h);
+    S.hinge_st.assign(ds ? 4 * (size_t)h : 0, 0.0);
     for (int k = 0; k < h; ++k) {
       const int e = hord[k];
       for (int c = 0; c < 4; ++c) S.hinge_idx[4 * k + c] = S.iperm[in.hinge[4 * e + c]];
       S.hinge_rest[k] = hrest[e];
       S.hinge_w[k] = hw[e];
+      if (ds)
+        for (int c = 0; c < 4; ++c) S.hinge_st[4 * k + c] = hst[4 * (size_t)e + c];
     }
   }
   S.mass.resize(n); S.pinned.resize(n);

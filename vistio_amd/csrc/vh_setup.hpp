@@ -32,6 +32,12 @@ struct SetupInput {
   const double* k_membrane_tri = nullptr;
   const double* k_bending_hinge = nullptr;
   const double* density_tri = nullptr;
+  // Bending kind: 1 = dihedral springs (stencil -1, -1, 1, 1), 2 = Discrete
+  // Shells (per-hinge normalised cotangent stencil); for the latter a nonzero
+  // bend_warp / bend_weft pair modulates each hinge's weight by its edge's
+  // warp/weft alignment.
+  int bending_kind = 1;
+  double bend_warp = 0.0, bend_weft = 0.0;
   double dt_matrix = 1.0 / 60.0;
   int leaf_size = 16;               // nested-dissection leaf size
   // Level banding of the dissection tree. Each band of consecutive levels
@@ -87,6 +93,7 @@ struct Setup {
   std::vector<int32_t> hinge_idx;    // 4h
   std::vector<double> hinge_rest;    // h
   std::vector<double> hinge_w;       // h
+  std::vector<double> hinge_st;      // 4h stencils (Discrete Shells only; empty = -1, -1, 1, 1)
   std::vector<double> mass;          // n, permuted (pinned -> 1e8)
   std::vector<double> mass_orig;     // n, original vertex order (lumped_masses())
   std::vector<uint8_t> pinned;       // n, permuted

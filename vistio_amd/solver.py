@@ -84,6 +84,7 @@ class SolverConfig:
 @dataclass
 class FabricProperties:
     """The FabricProperties fields init_with_material reads (properties.rs:23-98)."""
+    name: str = "custom"
     density: float = 150.0                  # g/m^2
     stretch_stiffness_warp: float = 100.0
     stretch_stiffness_weft: float = 100.0
@@ -99,6 +100,35 @@ class FabricProperties:
     def is_anisotropic(self) -> bool:
         ratio = self.stretch_stiffness_warp / self.stretch_stiffness_weft if self.stretch_stiffness_weft > 1e-8 else 1.0
         return abs(ratio - 1.0) > 0.05
+
+
+class MaterialDatabase:
+    """MaterialDatabase::with_defaults (database.rs:20-33): the five fabric
+    presets (database.rs:80-160), the fields the solver reads."""
+
+    _PRESETS = (  # name, density g/m^2, stretch warp, weft, bending warp, weft
+        ("cotton_twill", 200.0, 0.90, 0.85, 0.50, 0.45),
+        ("silk_charmeuse", 80.0, 0.70, 0.65, 0.10, 0.08),
+        ("denim_14oz", 400.0, 0.98, 0.92, 0.85, 0.75),
+        ("jersey_knit", 180.0, 0.50, 0.60, 0.20, 0.25),
+        ("chiffon", 50.0, 0.60, 0.55, 0.05, 0.04),
+    )
+
+    def __init__(self):
+        self.materials = {}
+
+    @classmethod
+    def with_defaults(cls) -> "MaterialDatabase":
+        db = cls()
+        for name, dens, sw, sf, bw, bf in cls._PRESETS:
+            db.register(FabricProperties(name, dens, sw, sf, bw, bf))
+        return db
+
+    def register(self, props: "FabricProperties") -> None:
+        self.materials[props.name] = props
+
+    def get(self, name: str):
+        return self.materials.get(name)
 
 
 @dataclass
@@ -197,7 +227,9 @@ class StepResult:
     wall_time: float
 
 
-_MODEL_BY_NAME = {"arap": nat.MODEL_ARAP, "co_rotational": nat.MODEL_COROTATIONAL}
+_MODEL_BY_NAME = {"arap": nat.MODEL_ARAP, "co_rotational": nat.MODEL_COROTATIONAL,
+                  "isotropic_linear": nat.MODEL_ISOTROPIC_LINEAR, "orthotropic_linear": nat.MODEL_ORTHOTROPIC_LINEAR,
+                  "anisotropic_corotational": nat.MODEL_ANISOTROPIC_COROTATIONAL}
 
 
 @dataclass
@@ -267,7 +299,8 @@ class HipPdSolver:
 
     # -- init -------------------------------------------------------------
     def _init(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: int,
-              k_membrane, k_bending, density_kgm2):
+              k_membrane, k_bending, density_kgm2, bending: int = nat.BENDING_DIHEDRAL, warp: float = 0.0,
+              weft: float = 0.0, bend_warp: float = 0.0, bend_weft: float = 0.0):
         """k_membrane, k_bending and density_kgm2 are each a scalar or a
         per-element array (triangles, interior edges, triangles)."""
         n = mesh.vertex_count()
@@ -278,7 +311,9 @@ class HipPdSolver:
             raise InvalidConfig(f"unknown precision {config.precision!r}")
         p = nat.PdParams()
         p.model = model
-        p.bending = nat.BENDING_DIHEDRAL
+        p.bending = bending
+        p.warp_stiffness, p.weft_stiffness = float(warp), float(weft)
+        p.bend_warp, p.bend_weft = float(bend_warp), float(bend_weft)
         per = {}
         for key, val, count in (("k_membrane", k_membrane, mesh.triangle_count()),
                                 ("k_bending", k_bending, topology.interior_edges.shape[0]),
@@ -329,16 +364,44 @@ class HipPdSolver:
         if model not in _MODEL_BY_NAME:
             raise InvalidConfig(f"unsupported constitutive model {model!r}")
         self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], properties.avg_stretch_stiffness() * 10.0,
-                   properties.avg_bending_stiffness() * 100.0, properties.density / 1000.0)
+                   properties.avg_bending_stiffness() * 100.0, properties.density / 1000.0,
+                   warp=properties.stretch_stiffness_warp, weft=properties.stretch_stiffness_weft)
+
+    def init_with_material_tier3(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig,
+                                 properties: FabricProperties, model: str, pinned):
+        """init_with_material_tier3 (pd_solver.rs:152-193): Discrete Shells
+        bending, warp/weft-modulated for anisotropic fabrics
+        (discrete_shells.rs:206-268); the orthotropic and anisotropic models
+        take their warp/weft stiffness from the properties (from_properties)."""
+        if model not in _MODEL_BY_NAME:
+            raise InvalidConfig(f"unsupported constitutive model {model!r}")
+        aniso = properties.is_anisotropic()
+        self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], properties.avg_stretch_stiffness() * 10.0,
+                   properties.avg_bending_stiffness() * 100.0, properties.density / 1000.0,
+                   bending=nat.BENDING_DISCRETE_SHELLS, warp=properties.stretch_stiffness_warp,
+                   weft=properties.stretch_stiffness_weft,
+                   bend_warp=properties.bending_stiffness_warp if aniso else 0.0,
+                   bend_weft=properties.bending_stiffness_weft if aniso else 0.0)
+
+    def init_with_material_tier4(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig,
+                                 properties: FabricProperties, model: str, pinned):
+        """init_with_material_tier4 (pd_solver.rs:203-214) = tier 3; contact
+        is then enabled with init_ipc."""
+        self.init_with_material_tier3(mesh, topology, config, properties, model, pinned)
 
     def init_raw(self, mesh: TriangleMesh, topology: Topology, config: SolverConfig, pinned, model: str,
-                 k_membrane: float, k_bending: float, density_kgm2: float):
+                 k_membrane, k_bending, density_kgm2, bending: str = "dihedral", warp: float = 0.0,
+                 weft: float = 0.0, bend_warp: float = 0.0, bend_weft: float = 0.0):
         """Init with explicit membrane/bending weights and areal density
         (kg/m^2): scalars, or per-element arrays (per triangle, per interior
         edge, per triangle) for several garments in one system."""
         if model not in _MODEL_BY_NAME:
             raise InvalidConfig(f"unsupported constitutive model {model!r}")
-        self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], k_membrane, k_bending, density_kgm2)
+        kinds = {"dihedral": nat.BENDING_DIHEDRAL, "discrete_shells": nat.BENDING_DISCRETE_SHELLS}
+        if bending not in kinds:
+            raise InvalidConfig(f"unsupported bending model {bending!r}")
+        self._init(mesh, topology, config, pinned, _MODEL_BY_NAME[model], k_membrane, k_bending, density_kgm2,
+                   bending=kinds[bending], warp=warp, weft=weft, bend_warp=bend_warp, bend_weft=bend_weft)
 
     def lumped_masses(self) -> np.ndarray:
         if not self._initialized:
