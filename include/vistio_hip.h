@@ -249,7 +249,7 @@ typedef struct {
   double friction_coefficient, contact_damping;
   int32_t adaptive_contact_damping;
   double contact_damping_max, contact_velocity_threshold;
-  uint32_t max_pairs;             /* candidate-pair buffer; 0 = 32 per triangle */
+  uint32_t max_pairs;             /* initial candidate-pair buffer (grows on demand); 0 = 32 per triangle */
 } vh_ipc_params;
 
 /* Diagnostics of the last IPC step or detection. */

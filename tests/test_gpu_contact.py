@@ -305,3 +305,102 @@ def test_ribbon_drop_parity_and_intersection_free():
     assert most > 0 and x[1].min() > -1e-3
     assert ref.crossings(x, idx) == 0
     s.close()
+
+
+def test_pair_buffer_grows_on_demand():
+    """A candidate-pair buffer that starts far too small (4 pairs) grows
+    inside every entry point: pairs, detection, CCD, crossings and the IPC
+    step give the oracle's results, as the reference's growable pair list
+    does."""
+    from dataclasses import replace
+    from oracle import ref
+    pos, idx, hg = _folded_sheet(c=6, gap=0.015)
+    n = pos.shape[1]
+    pin = np.zeros(n, dtype=np.uint8)
+    pos64 = pos.astype(np.float64)
+    pos64[1] += 0.012
+    g, o = _cfgs(self_collision=True, ground=0.0, d_hat=4e-4, kappa=1e3)
+    s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, replace(g, max_pairs=4))
+    got = s.bvh_pairs(pos.astype(np.float64), 0.03)
+    want = ref.tri_pairs(pos[0], pos[1], pos[2], idx, 0.03)
+    print(f"pairs: {len(got)}", flush=True)
+    assert len(want) > 4 and np.array_equal(got.astype(np.int64), want.astype(np.int64))
+    s.close()
+    for what in ("detect", "ccd"):
+        s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, replace(g, max_pairs=4))
+        if what == "detect":
+            d = s.detect_contacts(pos64, 4e-4, 1e3)
+            w = ref.detect(pos64, idx, o, 4e-4, 1e3)
+            print(f"detect: pairs {d['pairs']} self {d['self_contacts']}", flush=True)
+            assert d["pairs"] > 4 and d["self_contacts"] == w["self_contacts"] > 0
+            assert np.abs(d["grad"] - w["grad"]).max() <= 1e-10 * max(1.0, np.abs(w["grad"]).max())
+        elif what == "ccd":
+            dl = np.zeros_like(pos64)
+            dl[1, pos64[1] > 0.02] = -0.03
+            a = s.compute_ccd_step(pos64, pos64 + dl)
+            print(f"ccd: alpha {a}", flush=True)
+            assert a == pytest.approx(ref.ccd_step(pos64, pos64 + dl, idx, o), rel=1e-12, abs=1e-15) and a < 1
+        s.close()
+    from oracle import mesh
+    px, py, pz, idx1 = mesh.quad_grid(8, 8, 1.0, 1.0)
+    rng = np.random.default_rng(0)
+    one = np.stack([px, py, pz]).astype(np.float64)
+    one[2] = rng.normal(size=px.size) * 0.02
+    two = one + np.array([[0.37 / 8], [0.23 / 8], [0.0]])
+    two[2] = rng.normal(size=px.size) * 0.02
+    both = np.concatenate([one, two], 1)
+    idx2 = np.concatenate([idx1, idx1 + px.size]).astype(np.uint32)
+    p32 = both.astype(np.float32)
+    s = _solver(p32[0], p32[1], p32[2], idx2, mesh.hinges4(idx2, both.shape[1]), np.zeros(both.shape[1], np.uint8),
+                replace(g, max_pairs=4))
+    c = s.count_crossings(both)
+    print(f"crossings: {c}", flush=True)
+    assert c == ref.crossings(both, idx2) > 10
+    s.close()
+    pos, idx, hg = _folded_sheet(c=6, gap=0.03)
+    y = pos[1] + np.float32(0.002)
+    g, o = _cfgs(self_collision=True, ground=0.0, d_hat=4e-4, kappa=1e2, thickness=0.01)
+    errs, v, lam, st = _run_pair(pos[0], y, pos[2], idx, hg, pin, replace(g, max_pairs=4), o, 12)
+    print("grown buffer self fold: max err per step", ["%.1e" % e for e in errs], "v", v, st, flush=True)
+    assert st["pairs"] > 4 and max(errs) <= 1e-8 and v <= 1e-6
+
+
+def test_pair_buffer_grows_at_an_al_update():
+    """Growth at the detection that closes an AL iteration: a solver whose
+    buffer holds exactly the step-start pairs of a step whose candidate set
+    then widens takes that step, and the next ones, as a solver with room to
+    spare does from the same state."""
+    from dataclasses import replace
+    pos, idx, hg = _folded_sheet(c=6, gap=0.03)
+    pin = np.zeros(pos.shape[1], dtype=np.uint8)
+    y = pos[1] + np.float32(0.002)
+    g, _ = _cfgs(self_collision=True, ground=0.0, d_hat=4e-4, kappa=1e2, thickness=0.01)
+    a = _solver(pos[0], y, pos[2], idx, hg, pin, g)
+    found = None
+    for k in range(30):
+        state = a.get_state(0), a.get_state(1), a.get_lambda()
+        r, st = a.step_ipc_resident(DT, 1)
+        start = len(a.bvh_pairs(a.get_state(3), 0.01))  # the step-start query's positions and margin
+        if st["pairs"] > start:
+            found = k, state, start, st["pairs"]
+            break
+    a.close()
+    assert found, "no step widened its candidate set"
+    k, (x0, v0, l0), start, end = found
+    print(f"step {k}: {start} pairs at the start, {end} at the last AL update", flush=True)
+    runs = []
+    for cap in (0, start):
+        s = _solver(pos[0], y, pos[2], idx, hg, pin, replace(g, max_pairs=cap))
+        s.set_state(0, x0)
+        s.set_state(1, v0)
+        s.set_lambda(l0)
+        r, st = s.step_ipc_resident(DT, 1)
+        first = (s.positions(), st["pairs"], r.iterations, st["al_iterations"])
+        s.step_ipc_resident(DT, 5)
+        runs.append((first, s.positions(), s.get_lambda()))
+        s.close()
+    (f0, x5a, la), (f1, x5b, lb) = runs
+    print("after the step:", max_vertex_err(f0[0], f1[0]), "after 5 more:", max_vertex_err(x5a, x5b), flush=True)
+    assert f1[1] == f0[1] == end and f1[2:] == f0[2:]
+    assert max_vertex_err(f0[0], f1[0]) <= 1e-10 and max_vertex_err(x5a, x5b) <= 1e-8
+    assert np.abs(la - lb).max() <= 1e-6 * max(1.0, np.abs(la).max())

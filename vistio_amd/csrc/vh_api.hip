@@ -44,6 +44,7 @@ struct IpcState {
   vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t
   int st_levels = 0;
   int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
+  unsigned int* h_np = nullptr;  // pinned host copy of BvhDev::npairs (same allocation)
   unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
   float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
   void* scan_tmp = nullptr;
@@ -441,6 +442,55 @@ __global__ void k_ipc_clear(vhc::IpcDev* ip) {
   ip->viol = 0; ip->active = 0; ip->self_contacts = 0;
 }
 
+__global__ void k_ipc_clear_overflow(vhc::IpcDev* ip) {
+  if (threadIdx.x == 0) ip->overflow = 0;
+}
+
+// Replaces the candidate-pair buffer by one with room for `need` pairs and
+// half again, and clears the overflow flag. The caller has synchronized the
+// stream (the old buffer is freed). The reference collects its pairs in a
+// growable Vec (bvh.rs:207-209), so no pair count is ever refused.
+hipError_t grow_pairs(vh_ctx* c, unsigned int need) {
+  IpcState& I = c->ipc;
+  const uint64_t cap = std::min<uint64_t>((uint64_t)need + need / 2 + 256, 1ull << 31);
+  if (need > cap) return hipErrorOutOfMemory;
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), static_cast<void*>(I.pairs));
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  hipError_t e = hipFree(I.pairs);
+  I.pairs = nullptr;
+  I.cap = 0;
+  if (e != hipSuccess) return e;
+  void* q = nullptr;
+  e = hipMalloc(&q, cap * sizeof(int2));
+  if (e != hipSuccess) return e;
+  c->allocs.push_back(q);
+  I.pairs = static_cast<int2*>(q);
+  I.cap = (unsigned int)cap;
+  hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
+  return hipGetLastError();
+}
+
+// After a candidate-pair query has been enqueued: reads the pair count (one
+// 4-byte read and a synchronization) and, when it exceeded the buffer, grows
+// the buffer and enqueues `redo`, which repeats the query and what consumed
+// it. The count is a function of the positions, so one regrowth suffices.
+// `flag`, when given, is read into h_done with the count.
+template <class F>
+hipError_t fit_pairs(vh_ctx* c, F&& redo, const int* flag = nullptr) {
+  IpcState& I = c->ipc;
+  for (int pass = 0; pass < 3; ++pass) {
+    hipError_t e = hipMemcpyAsync(I.h_np, &I.bd->npairs, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess && flag) e = hipMemcpyAsync(I.h_done, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) return e;
+    if (*I.h_np <= I.cap) return hipSuccess;
+    e = grow_pairs(c, *I.h_np);
+    if (e != hipSuccess) return e;
+    redo();
+  }
+  return hipErrorUnknown;
+}
+
 // Candidate triangle pairs of positions x (boxes inflated by margin).
 void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   IpcState& I = c->ipc;
@@ -504,6 +554,10 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
                      c->pinned, I.ever, dt, P.gravity[0], P.gravity[1], P.gravity[2], max_vel, Q.has_ground,
                      Q.ground_height, Q.al_mu_initial, I.ip, c->st);
   enqueue_detect(c, c->x, I.sc, true, skip);
+  if (I.sc.self_collision) {
+    hipError_t e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); });
+    if (e != hipSuccess) return e;
+  }
   const double* dl = f64 ? (const double*)c->d : c->acc;
   for (int al = 0; al < Q.al_max_iterations; ++al) {
     hipLaunchKernelGGL(vhc::k_ipc_ext, dim3(nblk(3 * n)), dim3(256), 0, c->stream, n, I.grad, I.lambda, I.ext, I.ip);
@@ -525,12 +579,22 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
       hipLaunchKernelGGL(vhc::k_ipc_pd_latch, dim3(1), dim3(64), 0, c->stream, I.ip, c->st);
     }
     c->cur_ext = nullptr;
-    enqueue_detect(c, c->x, I.sc, false, skip);
-    hipLaunchKernelGGL(vhc::k_ipc_al_end, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.grad, I.inc, I.lambda,
-                       Q.al_tolerance, al + 1 < Q.al_max_iterations ? 1 : 0, I.ip);
-    hipLaunchKernelGGL(vhc::k_ipc_al_commit, dim3(1), dim3(64), 0, c->stream, Q.al_tolerance, Q.al_mu_growth,
-                       al + 1 == Q.al_max_iterations ? 1 : 0, I.ip, c->st);
-    if (al + 1 < Q.al_max_iterations) {
+    // The AL update: detection at the PD result, multiplier update, decision.
+    // After a pair overflow the last two do nothing and all three are redone
+    // with a larger buffer; the pair count is read with the AL flag.
+    auto al_close = [&] {
+      enqueue_detect(c, c->x, I.sc, false, skip);
+      hipLaunchKernelGGL(vhc::k_ipc_al_end, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.grad, I.inc, I.lambda,
+                         Q.al_tolerance, al + 1 < Q.al_max_iterations ? 1 : 0, I.ip);
+      hipLaunchKernelGGL(vhc::k_ipc_al_commit, dim3(1), dim3(64), 0, c->stream, Q.al_tolerance, Q.al_mu_growth,
+                         al + 1 == Q.al_max_iterations ? 1 : 0, I.ip, c->st);
+    };
+    al_close();
+    if (I.sc.self_collision) {
+      hipError_t e = fit_pairs(c, al_close, skip);
+      if (e != hipSuccess) return e;
+      if (*I.h_done) break;
+    } else if (al + 1 < Q.al_max_iterations) {
       hipError_t e = hipMemcpyAsync(I.h_done, skip, sizeof(int), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) return e;
@@ -565,7 +629,7 @@ vh_status read_ipc_result(vh_ctx* ctx, vh_step_result* out, vh_ipc_stats* st) {
     st->active = (int64_t)h.active; st->self_contacts = (int64_t)h.self_contacts;
     st->pairs = ctx->ipc.sc.self_collision ? (int64_t)np : 0; st->last_alpha = h.last_alpha;
   }
-  if (h.overflow) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  if (h.overflow) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded the pair buffer");
   if (!(h.resid == h.resid)) return fail(ctx, VH_E_NONFINITE, "non-finite positions in the IPC step");
   return VH_OK;
 }
@@ -1122,8 +1186,10 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(upload(ctx, &I.orig, S.tri_orig));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
-  VH_HIP(hipHostMalloc((void**)&I.h_done, sizeof(int), hipHostMallocDefault));
+  VH_HIP(hipHostMalloc((void**)&I.h_done, 2 * sizeof(int), hipHostMallocDefault));
   *I.h_done = 0;
+  I.h_np = reinterpret_cast<unsigned int*>(I.h_done + 1);
+  *I.h_np = 0;
   size_t bytes = 0;
   VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
                                    ctx->stream));
@@ -1229,9 +1295,13 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   vhc::Scene sc = ctx->ipc.sc;
   sc.dh = d_hat;
   sc.kappa = kappa;
-  hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, ctx->ipc.ip);
-  enqueue_detect(ctx, x, sc, false, nullptr, ctx->ipc.scratch_b);
+  auto detect = [&] {
+    hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, ctx->ipc.ip);
+    enqueue_detect(ctx, x, sc, false, nullptr, ctx->ipc.scratch_b);
+  };
+  detect();
   VH_HIP(hipGetLastError());
+  if (sc.self_collision) VH_HIP(fit_pairs(ctx, detect));
   const size_t n3 = 3 * (size_t)n;
   std::vector<double> g(n3), nr(n3), hd(n);
   std::vector<uint8_t> ic(n);
@@ -1265,22 +1335,26 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   for (size_t k = 0; k < a.size(); ++k) ctx->hbuf[k] -= a[k];  // delta = p1 - p0
   VH_HIP(hipMemcpyAsync(I.scratch_a, a.data(), a.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(hipMemcpyAsync(I.scratch_b, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-  hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
-  if (I.sc.self_collision) {
-    enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
-    hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream, I.pairs,
-                       &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ip);
-  }
-  if (I.sc.has_ground || I.sc.has_sphere)
-    hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, I.scratch_a, I.scratch_b,
-                       I.sc, I.ip);
+  auto ccd = [&] {
+    hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
+    if (I.sc.self_collision) {
+      enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
+      hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream, I.pairs,
+                         &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ip);
+    }
+    if (I.sc.has_ground || I.sc.has_sphere)
+      hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, I.scratch_a,
+                         I.scratch_b, I.sc, I.ip);
+  };
+  ccd();
   VH_HIP(hipGetLastError());
+  if (I.sc.self_collision) VH_HIP(fit_pairs(ctx, ccd));
   vhc::IpcDev h{};
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&h, I.ip, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
-  if (I.sc.self_collision && np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  if (I.sc.self_collision && np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded the pair buffer");
   const double toi = std::min(1.0, __builtin_bit_cast(double, h.toi));
   *alpha = std::max(toi, 1e-6);
   return VH_OK;
@@ -1300,14 +1374,18 @@ vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
     VH_HIP(hipMemcpyAsync(I.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
     x = I.scratch_a;
   }
-  enqueue_pairs(ctx, x, 0.0f, nullptr);
-  hipLaunchKernelGGL(vhc::k_tri_crossings, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream,
-                     I.pairs, &I.bd->npairs, I.cap, ctx->tri_idx, x, I.bd);
+  auto cross = [&] {
+    enqueue_pairs(ctx, x, 0.0f, nullptr);
+    hipLaunchKernelGGL(vhc::k_tri_crossings, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream,
+                       I.pairs, &I.bd->npairs, I.cap, ctx->tri_idx, x, I.bd);
+  };
+  cross();
   VH_HIP(hipGetLastError());
+  VH_HIP(fit_pairs(ctx, cross));
   vhc::BvhDev h{};
   VH_HIP(hipMemcpyAsync(&h, I.bd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
-  if (h.npairs > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  if (h.npairs > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded the pair buffer");
   *count = h.crossings;
   return VH_OK;
 }
@@ -1326,10 +1404,11 @@ vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* 
   VH_HIP(hipMemcpyAsync(I.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr);
   VH_HIP(hipGetLastError());
+  VH_HIP(fit_pairs(ctx, [&] { enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr); }));
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
-  if (np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded vh_ipc_params.max_pairs");
+  if (np > I.cap) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded the pair buffer");
   std::vector<int2> hp(np);
   if (np) VH_HIP(hipMemcpy(hp.data(), I.pairs, (size_t)np * sizeof(int2), hipMemcpyDeviceToHost));
   std::vector<std::pair<uint32_t, uint32_t>> pr(np);

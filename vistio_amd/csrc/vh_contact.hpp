@@ -714,6 +714,9 @@ __global__ void __launch_bounds__(256) k_vertex_contacts(int n, const double* __
                                                          double* __restrict__ lambda, int lambda_reset, IpcDev* ip,
                                                          double* __restrict__ hess, const int* skip) {
   if (skip && *skip) return;
+  // A detection whose candidate pairs overflowed the buffer writes nothing:
+  // the host grows the buffer and runs the detection again.
+  if (ip->overflow) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double gx = 0, gy = 0, gz = 0, nx = 0, ny = 0, nz = 0, hd = 0;
@@ -1036,7 +1039,7 @@ __global__ void k_ipc_pd_latch(IpcDev* ip, vhk::StepDev* st) {
 __global__ void __launch_bounds__(256) k_ipc_al_end(int n, const double* __restrict__ grad,
                                                     const uint8_t* __restrict__ inc, double* __restrict__ lambda,
                                                     double al_tol, int reset, const IpcDev* ip) {
-  if (ip->al_done) return;
+  if (ip->al_done || ip->overflow) return;
   if (__longlong_as_double((long long)ip->viol) < al_tol) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1052,9 +1055,11 @@ __global__ void __launch_bounds__(256) k_ipc_al_end(int n, const double* __restr
 
 // The AL decision (one thread, after k_ipc_al_end): converged -> stop;
 // else mu *= growth, and the next AL iteration starts unless this was the
-// last one. The PD-loop flags are re-armed for that iteration.
+// last one. The PD-loop flags are re-armed for that iteration. Like
+// k_ipc_al_end it does nothing after a pair overflow: the host then grows
+// the buffer and runs the detection, k_ipc_al_end and this again.
 __global__ void k_ipc_al_commit(double al_tol, double growth, int last, IpcDev* ip, vhk::StepDev* st) {
-  if (threadIdx.x != 0 || ip->al_done) return;
+  if (threadIdx.x != 0 || ip->al_done || ip->overflow) return;
   if (__longlong_as_double((long long)ip->viol) < al_tol) {
     ip->al_done = 1; ip->converged = 1; st->done = 1;
     return;

@@ -255,7 +255,7 @@ class IpcConfig:
     adaptive_contact_damping: bool = True
     contact_damping_max: float = 0.5
     contact_velocity_threshold: float = 0.1
-    max_pairs: int = 0                        # 0: 32 candidate pairs per triangle
+    max_pairs: int = 0                        # initial pair buffer, grows on demand; 0: 32 per triangle
 
 
 def estimate_initial_kappa(configured: float, avg_mass: float, gravity_mag: float, d_hat: float) -> float:
