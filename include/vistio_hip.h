@@ -185,6 +185,13 @@ vh_status vh_state_set_f64(vh_ctx* ctx, int32_t which, const double* x, const do
 vh_status vh_state_get_f64(vh_ctx* ctx, int32_t which, double* x, double* y, double* z);
 /* Pinning flags from SimulationState::inv_mass (inv_mass[i] == 0 -> pinned). */
 vh_status vh_state_set_inv_mass(vh_ctx* ctx, const float* inv_mass);
+/* SimulationState::ground_height (state.rs:52-55) for the plain step: when
+ * set, predict clamps y to it (state.rs:160-165), every PD iteration's commit
+ * enforces it (enforce_ground, state.rs:180-193, pd_solver.rs:437-438) and
+ * the velocity update applies enforce_ground_velocities (state.rs:195-212,
+ * pd_solver.rs:450-451). has_ground = 0 turns it off. Initialised from
+ * vh_pd_params. */
+vh_status vh_pd_set_ground(vh_ctx* ctx, int32_t has_ground, double ground_height);
 
 /* Parity / inspection entry points. */
 /* r = b - A x at the current device state (pred as set): the local step and

@@ -130,3 +130,31 @@ def test_tolerance_early_exit_matches_oracle():
         assert rg.iterations == ro.iterations, (step, rg.iterations, ro.iterations)
         assert abs(rg.final_residual - ro.final_residual) <= 1e-6 * ro.final_residual + 1e-15
     assert max_vertex_err(g.positions(), o.positions()) < 1e-9
+
+
+def test_step_with_ground_matches_oracle():
+    """SimulationState.ground_height in the plain step (state.rs:160-212,
+    pd_solver.rs:437-451): the hanging sheet swings onto a ground at
+    y = 0.6; resident and host-state stepping both follow the oracle."""
+    import vistio_amd as va
+    px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
+    o = oracle_solver(px, py, pz, idx, hg, pin, ground_height=0.6)
+    g = gpu_solver(px, py, pz, idx, hg, pin)
+    g.set_ground_height(0.6)
+    o.step(DT, 60)
+    g.step_resident(DT, 60)
+    err = max_vertex_err(g.positions(), o.positions())
+    low = float(g.positions()[1].min())
+    print(f"ground 0.6, 60 steps: max vertex err {err:.3e} m, lowest y {low:.6f}", flush=True)
+    assert err < 1e-9 and 0.6 <= low < 0.61  # resting on the ground (without it the sheet swings to y < 0)
+    h = gpu_solver(px, py, pz, idx, hg, pin)
+    st = va.SimulationState.from_mesh_with_masses(va.TriangleMesh(px, py, pz, idx), h.lumped_masses(), pin)
+    st.ground_height = 0.6
+    for _ in range(60):
+        h.step(st, DT)
+    x = np.stack([st.pos_x, st.pos_y, st.pos_z]).astype(np.float64)
+    err = max_vertex_err(x, o.positions())
+    print(f"host-state stepping with ground: max vertex err {err:.3e} m (f32 state)", flush=True)
+    assert err < 1e-5 and float(st.pos_y.min()) >= np.float32(0.6)
+    g.close()
+    h.close()

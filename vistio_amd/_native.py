@@ -26,7 +26,8 @@ PREC_F64, PREC_F32_IR = 0, 1
 # Every symbol include/vistio_hip.h declares.
 EXPORTS = (
     "vh_create", "vh_destroy", "vh_last_error", "vh_pd_init", "vh_pd_step", "vh_pd_step_resident",
-    "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_residual", "vh_pd_solve",
+    "vh_state_set_f64", "vh_state_get_f64", "vh_state_set_inv_mass", "vh_pd_set_ground", "vh_pd_residual",
+    "vh_pd_solve",
     "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
     "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings", "vh_pd_init_ex",
@@ -135,6 +136,7 @@ def load():
         "vh_state_set_f64": ([vp, C.c_int32, dp, dp, dp], C.c_int),
         "vh_state_get_f64": ([vp, C.c_int32, dp, dp, dp], C.c_int),
         "vh_state_set_inv_mass": ([vp, fp], C.c_int),
+        "vh_pd_set_ground": ([vp, C.c_int32, C.c_double], C.c_int),
         "vh_pd_residual": ([vp, C.c_double, dp], C.c_int),
         "vh_pd_solve": ([vp, dp, dp, C.c_int32], C.c_int),
         "vh_pd_info_get": ([vp, C.POINTER(PdInfo)], C.c_int),

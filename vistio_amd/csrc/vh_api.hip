@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -898,6 +899,20 @@ vh_status vh_state_set_inv_mass(vh_ctx* ctx, const float* inv_mass) {
   for (int k = 0; k < ctx->S.n; ++k) pin[k] = inv_mass[ctx->S.perm[k]] == 0.0f ? 1 : 0;
   VH_HIP(hipMemcpyAsync(ctx->pinned, pin.data(), pin.size(), hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_pd_set_ground(vh_ctx* ctx, int32_t has_ground, double ground_height) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (has_ground && !std::isfinite(ground_height)) return fail(ctx, VH_E_INVALID_ARG, "ground height must be finite");
+  const int32_t on = has_ground ? 1 : 0;
+  const double h = on ? ground_height : 0.0;
+  if (on != ctx->P.has_ground || h != ctx->P.ground_height) {
+    ctx->P.has_ground = on;
+    ctx->P.ground_height = h;
+    drop_graph(ctx);  // the captured step holds the old value in its kernel arguments
+  }
   return VH_OK;
 }
 

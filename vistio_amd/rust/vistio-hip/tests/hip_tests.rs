@@ -1,0 +1,147 @@
+//! Tests of the binding against the reference's own CPU implementations, in
+//! the same workspace: the GpuBackend vectors of
+//! crates/vistio-gpu/tests/gpu_tests.rs, the solver against
+//! ProjectiveDynamicsSolver on the 20 x 20 hanging sheet, and the broad phase
+//! against BvhBroadPhase. They need an MI355X and libvistio_hip.so.
+
+use vistio_contact::broad::BroadPhase;
+use vistio_contact::BvhBroadPhase;
+use vistio_gpu::{ComputeBuffer, CpuFallback, GpuBackend};
+use vistio_hip::{HipBackend, HipBroadPhase, HipPdSolver, HipSettings, IpcScene};
+use vistio_material::{CoRotationalModel, FabricProperties};
+use vistio_mesh::{generators::quad_grid, topology::Topology};
+use vistio_solver::{
+    config::SolverConfig, pd_solver::EmptyIpcHandler, state::SimulationState, strategy::SolverStrategy,
+    ProjectiveDynamicsSolver,
+};
+
+#[test]
+fn hip_backend_matches_cpu_fallback() {
+    let mut gpu = HipBackend::new(0);
+    gpu.init().unwrap();
+    assert_eq!(gpu.name(), "hip_gfx950");
+    assert!(gpu.is_gpu());
+    let a = ComputeBuffer::from_data(vec![1.0, 2.0, 3.0]);
+    let b = ComputeBuffer::from_data(vec![4.0, 5.0, 6.0]);
+    let mut out = ComputeBuffer::zeros(3);
+    gpu.axpy(&a, &b, 2.0, &mut out).unwrap();
+    assert_eq!(out.as_slice(), &[9.0, 12.0, 15.0]);
+    assert!((gpu.dot(&a, &b).unwrap() - 32.0).abs() < 1e-10);
+    let short = ComputeBuffer::from_data(vec![1.0, 2.0]);
+    assert!(gpu.axpy(&short, &b, 1.0, &mut out).is_err());
+    assert!(gpu.dot(&short, &b).is_err());
+    // bit-exact against the CPU loop on larger vectors
+    let cpu = CpuFallback::new();
+    let n = 100_003;
+    let x = ComputeBuffer::from_data((0..n).map(|i| ((i * 7919) % 1000) as f32 * 1e-3 - 0.5).collect());
+    let y = ComputeBuffer::from_data((0..n).map(|i| ((i * 104_729) % 997) as f32 * 1e-2).collect());
+    let (mut og, mut oc) = (ComputeBuffer::zeros(n), ComputeBuffer::zeros(n));
+    gpu.axpy(&x, &y, 0.37, &mut og).unwrap();
+    cpu.axpy(&x, &y, 0.37, &mut oc).unwrap();
+    assert_eq!(og.as_slice(), oc.as_slice());
+    let (dg, dc) = (gpu.dot(&x, &y).unwrap(), cpu.dot(&x, &y).unwrap());
+    assert!((dg - dc).abs() <= 1e-9 * dc.abs().max(1.0));
+    let mut buf = ComputeBuffer::zeros(5);
+    gpu.fill(&mut buf, std::f32::consts::PI);
+    assert!(buf.as_slice().iter().all(|&v| v == std::f32::consts::PI));
+}
+
+fn sheet(c: usize) -> (vistio_mesh::TriangleMesh, Topology, Vec<bool>) {
+    let mesh = quad_grid(c, c, 1.0, 1.0);
+    let topo = Topology::build(&mesh);
+    let pinned: Vec<bool> = (0..mesh.vertex_count()).map(|i| i <= c).collect();
+    (mesh, topo, pinned)
+}
+
+fn config() -> SolverConfig {
+    SolverConfig { max_iterations: 10, tolerance: 0.0, damping: 0.0, rayleigh_mass_damping: 0.01, ..Default::default() }
+}
+
+/// The hanging sheet of the headline workload at 20 x 20, 120 steps, both
+/// solvers from init_with_material with the same fabric. The reference keeps
+/// f32 state and solves in f64 after an f32 round trip; the HIP step keeps
+/// FP64 state. Their trajectories agree to what that f32 rounding allows.
+#[test]
+fn hip_solver_follows_projective_dynamics_solver() {
+    let (mesh, topo, pinned) = sheet(20);
+    let cfg = config();
+    let props = FabricProperties {
+        density: 150.0,
+        stretch_stiffness_warp: 100.0,
+        stretch_stiffness_weft: 100.0,
+        bending_stiffness_warp: 1e-6,
+        bending_stiffness_weft: 1e-6,
+        ..FabricProperties::cotton_twill()
+    };
+    let mut cpu = ProjectiveDynamicsSolver::new();
+    cpu.init_with_material(&mesh, &topo, &cfg, &props, Box::new(CoRotationalModel::new()), &pinned).unwrap();
+    for prec in [vistio_hip::ffi::VH_PREC_F64, vistio_hip::ffi::VH_PREC_F32_IR] {
+        let mut gpu = HipPdSolver::with_settings(0, HipSettings { precision: prec, ..Default::default() }).unwrap();
+        gpu.init_with_material(&mesh, &topo, &cfg, &props, Box::new(CoRotationalModel::new()), &pinned).unwrap();
+        for (a, b) in gpu.lumped_masses().iter().zip(cpu.lumped_masses()) {
+            assert!((a - b).abs() <= 1e-6 * b.abs().max(1.0), "lumped mass {a} vs {b}");
+        }
+        let mut sc = SimulationState::from_mesh_with_masses(&mesh, cpu.lumped_masses(), &pinned).unwrap();
+        let mut sg = SimulationState::from_mesh_with_masses(&mesh, gpu.lumped_masses(), &pinned).unwrap();
+        let mut cpu_run = ProjectiveDynamicsSolver::new();
+        cpu_run.init_with_material(&mesh, &topo, &cfg, &props, Box::new(CoRotationalModel::new()), &pinned).unwrap();
+        for _ in 0..120 {
+            cpu_run.step(&mut sc, 1.0 / 60.0).unwrap();
+            let r = gpu.step(&mut sg, 1.0 / 60.0).unwrap();
+            assert_eq!(r.iterations, 10);
+        }
+        let mut worst = 0.0f32;
+        for i in 0..mesh.vertex_count() {
+            let d = ((sc.pos_x[i] - sg.pos_x[i]).powi(2) + (sc.pos_y[i] - sg.pos_y[i]).powi(2)
+                + (sc.pos_z[i] - sg.pos_z[i]).powi(2))
+            .sqrt();
+            worst = worst.max(d);
+        }
+        assert!(worst < 1e-4, "max vertex distance {worst} m after 120 steps");
+    }
+}
+
+#[test]
+fn hip_broad_phase_contains_bvh_pairs() {
+    let mut mesh = quad_grid(12, 12, 1.0, 1.0);
+    for (i, z) in mesh.pos_z.iter_mut().enumerate() {
+        *z = (((i * 2654435761usize) % 1000) as f32 / 1000.0 - 0.5) * 0.1;
+    }
+    let mut cpu = BvhBroadPhase::new(&mesh);
+    let mut gpu = HipBroadPhase::new(0, &mesh).unwrap();
+    cpu.update(&mesh.pos_x, &mesh.pos_y, &mesh.pos_z, 0.01).unwrap();
+    gpu.update(&mesh.pos_x, &mesh.pos_y, &mesh.pos_z, 0.01).unwrap();
+    let all: std::collections::HashSet<(u32, u32)> =
+        gpu.query_triangle_pairs().iter().map(|p| (p.ta.min(p.tb), p.ta.max(p.tb))).collect();
+    for p in cpu.query_triangle_pairs() {
+        assert!(all.contains(&(p.ta.min(p.tb), p.ta.max(p.tb))), "pair ({}, {}) missing", p.ta, p.tb);
+    }
+    assert!(all.len() >= cpu.query_triangle_pairs().len());
+}
+
+#[test]
+fn hip_step_with_ipc_rests_on_the_ground() {
+    let mut mesh = quad_grid(16, 16, 0.5, 0.5);
+    for i in 0..mesh.vertex_count() {
+        mesh.pos_z[i] = mesh.pos_y[i];
+        mesh.pos_y[i] = 0.03;
+    }
+    let topo = Topology::build(&mesh);
+    let pinned = vec![false; mesh.vertex_count()];
+    let mut cfg = config();
+    cfg.ipc_enabled = true;
+    cfg.barrier_d_hat = 1e-3;
+    cfg.compliant_contact = false;
+    let mut gpu = HipPdSolver::new(0).unwrap();
+    gpu.init_with_material_tier4(&mesh, &topo, &cfg, &FabricProperties::cotton_twill(),
+                                 Box::new(CoRotationalModel::new()), &pinned).unwrap();
+    let scene = IpcScene { ground: Some(0.0), thickness: 1.5e-3, exclusion_depth: 1, ..Default::default() };
+    gpu.init_ipc(&scene, &cfg, 100.0).unwrap();
+    let mut st = SimulationState::from_mesh_with_masses(&mesh, gpu.lumped_masses(), &pinned).unwrap();
+    st.ground_height = Some(0.0);
+    for _ in 0..60 {
+        gpu.step_with_ipc(&mut st, 1.0 / 60.0, &mut EmptyIpcHandler).unwrap();
+    }
+    assert!(st.pos_y.iter().all(|&y| y.is_finite() && y > -1e-3));
+    assert!(st.pos_y.iter().any(|&y| y < 0.01));
+}

@@ -193,6 +193,8 @@ class SimulationState:
     al_lambda_x: np.ndarray = None
     al_lambda_y: np.ndarray = None
     al_lambda_z: np.ndarray = None
+    # ground plane height (Y) enforced by the plain step (state.rs:52-55)
+    ground_height: Optional[float] = None
 
     def __post_init__(self):
         for nm in ("al_lambda_x", "al_lambda_y", "al_lambda_z"):
@@ -419,10 +421,21 @@ class HipPdSolver:
         for a in arrs:
             if a.dtype != np.float32 or a.size != n or not a.flags.c_contiguous:
                 raise InvalidMesh("state arrays must be contiguous float32 of the mesh's vertex count")
+        self.set_ground_height(state.ground_height)
         s = nat.StateF32(*[nat.ptr(a, C.c_float) for a in arrs])
         r = nat.StepResultC()
         _check(self._L.vh_pd_step(self._ctx, float(dt), C.byref(s), C.byref(r)), self._ctx, "step")
         return StepResult(r.iterations, r.final_residual, bool(r.converged), r.wall_time)
+
+    def set_ground_height(self, height: Optional[float]) -> None:
+        """The ground plane (Y = height, None = none) of the plain step: what
+        SimulationState.ground_height does for the reference's step (predict
+        clamp, enforce_ground, enforce_ground_velocities)."""
+        if not self._initialized:
+            raise InvalidConfig("Solver not initialized. Call init() first.")
+        on = height is not None
+        _check(self._L.vh_pd_set_ground(self._ctx, int(on), float(np.float32(height)) if on else 0.0), self._ctx,
+               "set_ground_height")
 
     def step_resident(self, dt: float, nsteps: int = 1) -> StepResult:
         if not self._initialized:
