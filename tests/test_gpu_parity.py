@@ -138,7 +138,8 @@ def test_step_with_ground_matches_oracle():
     y = 0.6; resident and host-state stepping both follow the oracle."""
     import vistio_amd as va
     px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
-    o = oracle_solver(px, py, pz, idx, hg, pin, ground_height=0.6)
+    gh = float(np.float32(0.6))  # SimulationState.ground_height is an f32
+    o = oracle_solver(px, py, pz, idx, hg, pin, ground_height=gh)
     g = gpu_solver(px, py, pz, idx, hg, pin)
     g.set_ground_height(0.6)
     o.step(DT, 60)
