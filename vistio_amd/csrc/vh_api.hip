@@ -53,6 +53,7 @@ struct IpcState {
   int* orig = nullptr;  // caller's index of each device triangle
   int2* pairs = nullptr;
   unsigned int cap = 0;
+  vhc::CcdTri* ccdtri = nullptr;  // per-triangle bounds of the CCD pair cull
   vhc::BvhDev* bd = nullptr;
   vhc::IpcDev* ip = nullptr;
   void* sort_tmp = nullptr;
@@ -570,9 +571,12 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
       enqueue_correction(c, dt, sweeps, true, false, false, c->st, none);
       hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
                          I.partials, I.ip);
-      if (I.sc.self_collision)
+      if (I.sc.self_collision) {
+        hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(S.t))), dim3(256), 0, c->stream, S.t, c->tri_idx, c->x,
+                           dl, I.ccdtri, I.ip);
         hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
-                           I.pairs, &I.bd->npairs, I.cap, c->tri_idx, c->x, dl, I.ip);
+                           I.pairs, &I.bd->npairs, I.cap, c->tri_idx, c->x, dl, I.ccdtri, I.ip);
+      }
       if (I.sc.has_ground || I.sc.has_sphere)
         hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
       hipLaunchKernelGGL(vhc::k_ipc_alpha, dim3(1), dim3(256), 0, c->stream, I.partials, nblk(n), P.tolerance, I.ip);
@@ -1200,6 +1204,7 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.st, (size_t)I.st_levels * (size_t)tt));
   UP(upload(ctx, &I.orig, S.tri_orig));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
+  UP(dalloc(ctx, &I.ccdtri, (size_t)tt));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
   VH_HIP(hipHostMalloc((void**)&I.h_done, 2 * sizeof(int), hipHostMallocDefault));
   *I.h_done = 0;
@@ -1354,8 +1359,10 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
     hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
     if (I.sc.self_collision) {
       enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
+      hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(S.t))), dim3(256), 0, ctx->stream, S.t, ctx->tri_idx,
+                         I.scratch_a, I.scratch_b, I.ccdtri, I.ip);
       hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream, I.pairs,
-                         &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ip);
+                         &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ccdtri, I.ip);
     }
     if (I.sc.has_ground || I.sc.has_sphere)
       hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, I.scratch_a,

@@ -902,17 +902,65 @@ __device__ __forceinline__ void toi_min(IpcDev* ip, double t) {
   atomicMin(&ip->toi, (unsigned long long)__double_as_longlong(t));
 }
 
+// Per-triangle bounds for the pair cull of k_ccd_pairs: a float centre c, the
+// largest distance r of a corner from c and the largest corner displacement
+// m, both rounded up. For a pair (A, B) every relative position in vt_ccd
+// (a corner of one triangle minus a corner of the other) is then at most
+// D = |c_A - c_B| + r_A + r_B long, and every relative motion at most
+// W = m_A + m_B.
+struct CcdTri {
+  float c[3], r, m, pad[3];
+};
+
+__device__ __forceinline__ float round_up(double v) { return (float)(v * (1.0 + 1e-6)) + 1e-30f; }
+
+__global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __restrict__ tri,
+                                                        const double* __restrict__ x, const double* __restrict__ dl,
+                                                        CcdTri* __restrict__ out, const IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= t) return;
+  const D3 a = ld(x, tri[3 * e]), b = ld(x, tri[3 * e + 1]), c = ld(x, tri[3 * e + 2]);
+  const float cx = (float)((a.x + b.x + c.x) / 3.0), cy = (float)((a.y + b.y + c.y) / 3.0),
+              cz = (float)((a.z + b.z + c.z) / 3.0);
+  const D3 cc = d3(cx, cy, cz);
+  const double r2 = fmax(len2(a - cc), fmax(len2(b - cc), len2(c - cc)));
+  const double m2 = fmax(len2(ld(dl, tri[3 * e])), fmax(len2(ld(dl, tri[3 * e + 1])), len2(ld(dl, tri[3 * e + 2]))));
+  CcdTri o;
+  o.c[0] = cx; o.c[1] = cy; o.c[2] = cz;
+  o.r = round_up(sqrt(r2));
+  o.m = round_up(sqrt(m2));
+  o.pad[0] = o.pad[1] = o.pad[2] = 0.0f;
+  out[e] = o;
+}
+
 // Self part of compute_ccd_step (ccd.rs:238-292) over the candidate pairs of
 // the last detection, from x to x + delta.
+//
+// Pair cull (exact): the cubic of a vertex-triangle test has coefficients
+// |ca| <= W^3, |cb| <= 3 D W^2, |cc| <= 3 D^2 W (D, W from CcdTri). When all
+// three are below half of the solver's 1e-8 threshold, cubic_roots falls
+// through quadratic_roots to the linear case and finds no root for any test
+// of the pair (the computed coefficients differ from the bounded ones by a
+// few ulps), so the pair cannot report a hit and is skipped.
 __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pairs, const unsigned int* npairs_p,
                                                    unsigned int cap, const int* __restrict__ tri,
                                                    const double* __restrict__ x, const double* __restrict__ dl,
-                                                   IpcDev* ip) {
+                                                   const CcdTri* __restrict__ tb_, IpcDev* ip) {
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
   double best = 2.0;
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
     const int2 pr = pairs[k];
+    {
+      const CcdTri A = tb_[pr.x], B = tb_[pr.y];
+      const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
+                   gz = (double)A.c[2] - (double)B.c[2];
+      const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
+      const double W = (double)A.m + (double)B.m;
+      constexpr double kHalfEps = 0.5e-8;
+      if (W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps) continue;
+    }
     // The six corners (A then B) with their start and end positions, loaded
     // once per pair: measured faster than reloading them per test, despite
     // the register cost of the unrolled tests.
