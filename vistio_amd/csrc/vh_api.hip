@@ -532,8 +532,11 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
   if (sc.self_collision) {
     enqueue_pairs(c, x, (float)sc.thickness, skip);
     hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
+    hipLaunchKernelGGL(vhc::k_tri_spheres, dim3(std::max(1, nblk(c->S.t))), dim3(256), 0, c->stream, c->S.t,
+                       c->tri_idx, x, I.ccdtri, skip);
     hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
-                       I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.gself, I.ip, skip);
+                       I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.ccdtri, I.gself, I.ip,
+                       skip);
   }
   hipLaunchKernelGGL(vhc::k_vertex_contacts, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, sc, I.gself, I.grad, I.nrm,
                      I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, hess, skip);

@@ -595,10 +595,35 @@ struct Scene {
   double dh, kappa;
 };
 
-__device__ __forceinline__ bool excluded(const int* __restrict__ xptr, const int* __restrict__ xidx, int i, int j) {
-  for (int q = xptr[i]; q < xptr[i + 1]; ++q)
-    if (xidx[q] == j) return true;
-  return false;
+// Per-triangle bounds for the pair cull of k_ccd_pairs: a float centre c, the
+// largest distance r of a corner from c and the largest corner displacement
+// m, both rounded up. For a pair (A, B) every relative position in vt_ccd
+// (a corner of one triangle minus a corner of the other) is then at most
+// D = |c_A - c_B| + r_A + r_B long, and every relative motion at most
+// W = m_A + m_B.
+struct CcdTri {
+  float c[3], r, m, pad[3];
+};
+
+__device__ __forceinline__ float round_up(double v) { return (float)(v * (1.0 + 1e-6)) + 1e-30f; }
+
+// Bounding sphere of each triangle at x (CcdTri with m = 0), for the pair
+// cull of k_self_contacts.
+__global__ void __launch_bounds__(256) k_tri_spheres(int t, const int* __restrict__ tri, const double* __restrict__ x,
+                                                     CcdTri* __restrict__ out, const int* skip) {
+  if (skip && *skip) return;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= t) return;
+  const D3 a = ld(x, tri[3 * e]), b = ld(x, tri[3 * e + 1]), c = ld(x, tri[3 * e + 2]);
+  CcdTri o;
+  o.c[0] = (float)((a.x + b.x + c.x) / 3.0);
+  o.c[1] = (float)((a.y + b.y + c.y) / 3.0);
+  o.c[2] = (float)((a.z + b.z + c.z) / 3.0);
+  const D3 cc = d3(o.c[0], o.c[1], o.c[2]);
+  o.r = round_up(sqrt(fmax(len2(a - cc), fmax(len2(b - cc), len2(c - cc)))));
+  o.m = 0.0f;
+  o.pad[0] = o.pad[1] = o.pad[2] = 0.0f;
+  out[e] = o;
 }
 
 __device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
@@ -610,11 +635,18 @@ __device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
 // self_collision.rs:199-276 with ipc_response.rs:58-219: the vertex-triangle
 // (6) and edge-edge (9) tests of one candidate pair, barrier gradients added
 // to gself (FP64 atomics).
+//
+// Pair cull (exact): no two points of A and B are closer than the gap of
+// their bounding spheres (CcdTri centre and radius at x, k_tri_spheres). When
+// that gap exceeds sqrt(dh) with a relative 1e-6 to spare, every distance the
+// tests compute is beyond the barrier's reach, so the pair adds nothing and
+// is skipped before its exclusion lookups.
 __global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ pairs, const unsigned int* npairs_p,
                                                        unsigned int cap, const int* __restrict__ tri,
                                                        const double* __restrict__ x, const int* __restrict__ xptr,
                                                        const int* __restrict__ xidx, Scene sc,
-                                                       double* __restrict__ gself, IpcDev* ip, const int* skip) {
+                                                       const CcdTri* __restrict__ sph, double* __restrict__ gself,
+                                                       IpcDev* ip, const int* skip) {
   if (skip && *skip) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
   if (blockIdx.x == 0 && threadIdx.x == 0 && *npairs_p > cap) ip->overflow = 1;
@@ -627,16 +659,30 @@ __global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ 
   bool viol = false;
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
     const int2 pr = pairs[k];
+    {
+      const CcdTri SA = sph[pr.x], SB = sph[pr.y];
+      const double gx = (double)SA.c[0] - (double)SB.c[0], gy = (double)SA.c[1] - (double)SB.c[1],
+                   gz = (double)SA.c[2] - (double)SB.c[2];
+      const double gap = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 - 1e-12) - (double)SA.r - (double)SB.r;
+      if (gap > 0.0 && gap * gap > dh * (1.0 + 1e-6)) continue;
+    }
     const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
     const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
     // Every exclusion test of the pair is between a vertex of A and one of
     // B, and the relation is symmetric (within `depth` rings): bit 3i + j
     // holds excluded(A[i], B[j]).
+    // One pass over each of A's three exclusion lists tests it against all
+    // of B's corners (the set of matches of nine separate searches).
     unsigned int ex = 0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a)
-#pragma unroll
-      for (int b = 0; b < 3; ++b) ex |= (excluded(xptr, xidx, A[a], B[b]) ? 1u : 0u) << (3 * a + b);
+    for (int a = 0; a < 3; ++a) {
+      const int q1 = xptr[A[a] + 1];
+      for (int q = xptr[A[a]]; q < q1; ++q) {
+        const int w = xidx[q];
+        ex |= ((w == B[0]) ? 1u : 0u) << (3 * a) | ((w == B[1]) ? 1u : 0u) << (3 * a + 1) |
+              ((w == B[2]) ? 1u : 0u) << (3 * a + 2);
+      }
+    }
     if (ex == 0x1ffu) continue;  // every test of the pair is excluded
     // vertex-triangle tests, one at a time (A's vertices against B first)
     auto pick = [](const int* P, int q) { return q == 0 ? P[0] : (q == 1 ? P[1] : P[2]); };
@@ -902,17 +948,6 @@ __device__ __forceinline__ void toi_min(IpcDev* ip, double t) {
   atomicMin(&ip->toi, (unsigned long long)__double_as_longlong(t));
 }
 
-// Per-triangle bounds for the pair cull of k_ccd_pairs: a float centre c, the
-// largest distance r of a corner from c and the largest corner displacement
-// m, both rounded up. For a pair (A, B) every relative position in vt_ccd
-// (a corner of one triangle minus a corner of the other) is then at most
-// D = |c_A - c_B| + r_A + r_B long, and every relative motion at most
-// W = m_A + m_B.
-struct CcdTri {
-  float c[3], r, m, pad[3];
-};
-
-__device__ __forceinline__ float round_up(double v) { return (float)(v * (1.0 + 1e-6)) + 1e-30f; }
 
 __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __restrict__ tri,
                                                         const double* __restrict__ x, const double* __restrict__ dl,
