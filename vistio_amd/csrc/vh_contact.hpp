@@ -673,14 +673,28 @@ __global__ void __launch_bounds__(256) k_self_contacts(const int2* __restrict__ 
     // holds excluded(A[i], B[j]).
     // One pass over each of A's three exclusion lists tests it against all
     // of B's corners (the set of matches of nine separate searches).
+    // The first kExPref entries of the three lists are loaded together; the
+    // rest of a longer list follows one entry at a time.
+    constexpr int kExPref = 8;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { lo[a] = xptr[A[a]]; hi[a] = xptr[A[a] + 1]; }
+    int w[3][kExPref];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int j = 0; j < kExPref; ++j) w[a][j] = lo[a] + j < hi[a] ? xidx[lo[a] + j] : -1;
     unsigned int ex = 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const int q1 = xptr[A[a] + 1];
-      for (int q = xptr[A[a]]; q < q1; ++q) {
-        const int w = xidx[q];
-        ex |= ((w == B[0]) ? 1u : 0u) << (3 * a) | ((w == B[1]) ? 1u : 0u) << (3 * a + 1) |
-              ((w == B[2]) ? 1u : 0u) << (3 * a + 2);
+#pragma unroll
+      for (int j = 0; j < kExPref; ++j)
+        ex |= ((w[a][j] == B[0]) ? 1u : 0u) << (3 * a) | ((w[a][j] == B[1]) ? 1u : 0u) << (3 * a + 1) |
+              ((w[a][j] == B[2]) ? 1u : 0u) << (3 * a + 2);
+      for (int q = lo[a] + kExPref; q < hi[a]; ++q) {
+        const int v = xidx[q];
+        ex |= ((v == B[0]) ? 1u : 0u) << (3 * a) | ((v == B[1]) ? 1u : 0u) << (3 * a + 1) |
+              ((v == B[2]) ? 1u : 0u) << (3 * a + 2);
       }
     }
     if (ex == 0x1ffu) continue;  // every test of the pair is excluded
