@@ -67,9 +67,8 @@ def test_rope_walk_equals_brute_force():
         blo, bhi = c - w, c + w
         nlo = [blo[rlo[i]:rhi[i] + 1].min() for i in range(t - 1)]
         nhi = [bhi[rlo[i]:rhi[i] + 1].max() for i in range(t - 1)]
-        got = set()
-        for k in range(t):
-            nd, steps = 0, 0
+        def walk(k, start):
+            out, nd, steps = [], start, 0
             while nd != END:
                 steps += 1
                 assert steps < 4 * t
@@ -79,7 +78,19 @@ def test_rope_walk_equals_brute_force():
                 else:
                     j = ~nd
                     if j > k and blo[k] <= bhi[j] and bhi[k] >= blo[j]:
-                        got.add((k, j))
+                        out.append((k, j))
                     nd = _escape(j, t, rlo)
+            return out, steps
+
+        got = set()
+        for k in range(t):
+            # The kernel starts at leaf k's escape: the nodes after leaf k in
+            # pre-order are the right subtrees of its ancestors, exactly the
+            # ranges past k, so the walk from the root finds the same pairs
+            # in the same order after descending to k.
+            from_root, s_root = walk(k, 0)
+            from_leaf, s_leaf = walk(k, _escape(k, t, rlo))
+            assert from_leaf == from_root and s_leaf <= s_root
+            got.update(from_leaf)
         want = {(a, b) for a in range(t) for b in range(a + 1, t) if blo[a] <= bhi[b] and bhi[a] >= blo[b]}
         assert got == want

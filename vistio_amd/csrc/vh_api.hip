@@ -51,6 +51,7 @@ struct IpcState {
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
   int* orig = nullptr;  // caller's index of each device triangle
+  int* sorig = nullptr;  // caller's index of each sorted leaf
   int2* pairs = nullptr;
   unsigned int cap = 0;
   vhc::CcdTri* ccdtri = nullptr;  // per-triangle bounds of the CCD pair cull
@@ -514,12 +515,12 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
     hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
   hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left,
-                     I.range_lo, I.range_hi, I.node, I.leaf, skip);
+                     I.range_lo, I.range_hi, I.node, I.leaf, I.orig, I.sorig, skip);
   hipLaunchKernelGGL(vhc::k_bvh_count, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
   size_t sbytes = I.scan_bytes;
   (void)rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t, rocprim::plus<unsigned int>(),
                                 c->stream);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.orig, I.pcount,
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
 }
 
@@ -1206,6 +1207,7 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   while ((1ll << I.st_levels) <= (long long)tt) ++I.st_levels;  // levels 0 .. floor(log2 t)
   UP(dalloc(ctx, &I.st, (size_t)I.st_levels * (size_t)tt));
   UP(upload(ctx, &I.orig, S.tri_orig));
+  UP(dalloc(ctx, &I.sorig, (size_t)tt));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
   UP(dalloc(ctx, &I.ccdtri, (size_t)tt));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));

@@ -479,7 +479,8 @@ __device__ __forceinline__ int escape_of(int hi, int t, const int* __restrict__ 
 __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st, const int* __restrict__ id,
                                                      const int* __restrict__ left, const int* __restrict__ range_lo,
                                                      const int* __restrict__ range_hi, Node* __restrict__ node,
-                                                     Leaf* __restrict__ leaf, const int* skip) {
+                                                     Leaf* __restrict__ leaf, const int* __restrict__ orig,
+                                                     int* __restrict__ sorig, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < t) {
@@ -488,6 +489,7 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
     L.escape = escape_of(i, t, range_lo);
     L.id = id[i];
     leaf[i] = L;
+    sorig[i] = orig[L.id];  // the caller's index of sorted leaf i
   }
   if (i >= t - 1) return;
   const int lo = range_lo[i], hi = range_hi[i];
@@ -511,17 +513,27 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
 // triangle against the second and edges of the first against the second
 // (self_collision.rs:199-276), and for parallel edges the closest points,
 // hence the gradient, depend on that order.
-// Stackless: pre-order walk from the root that enters a node when its box
-// overlaps and its range reaches past k, and otherwise takes its escape;
-// every step moves forward in pre-order, so the walk ends at kEnd.
+// Stackless: pre-order walk, from the node after leaf k, that enters a node
+// when its box overlaps and its range reaches past k, and otherwise takes
+// its escape; every step moves forward in pre-order, so the walk ends at
+// kEnd.
+// Pairs of one workgroup of k_bvh_pairs staged in LDS before a coalesced
+// copy: the group's leaves are consecutive, so their pairs form one
+// contiguous run of the output. Pairs past the staging room go straight out.
+constexpr int kPairStage = 2048;  // 16 KB: 2048 pairs, about 8 per leaf
+
 template <bool kEmit>
 __device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
-                                                 const int* __restrict__ orig, int2* __restrict__ pairs,
-                                                 unsigned int base, unsigned int cap) {
+                                                 const int* __restrict__ sorig, int2* __restrict__ pairs,
+                                                 unsigned int base, unsigned int cap, int2* stage,
+                                                 unsigned int stage_base) {
   const Leaf me = leaf[k];
-  const int ome = kEmit ? orig[me.id] : 0;
+  const int ome = kEmit ? sorig[k] : 0;
   unsigned int cnt = 0;
-  int nd = 0;
+  // Start after leaf k in pre-order: the nodes there are the right subtrees
+  // of k's ancestors, i.e. exactly the ranges past k, so the descent from the
+  // root to k (which only re-enters k's ancestors) is skipped.
+  int nd = me.escape;
   while (nd != kEnd) {
     if (nd >= 0) {
       const Node N = node[nd];
@@ -529,11 +541,15 @@ __device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__
     } else {
       const int j = ~nd;
       const Leaf L = leaf[j];
+      const int oj = kEmit ? sorig[j] : 0;  // loaded with the leaf, not after it
       if (j > k && overlaps(me.box, L.box)) {
         if (kEmit) {
           const unsigned int slot = base + cnt;
-          const bool first = ome < orig[L.id];
-          if (slot < cap) pairs[slot] = make_int2(first ? me.id : L.id, first ? L.id : me.id);
+          const bool first = ome < oj;
+          const int2 pr = make_int2(first ? me.id : L.id, first ? L.id : me.id);
+          const unsigned int ls = slot - stage_base;
+          if (ls < (unsigned int)kPairStage) stage[ls] = pr;
+          else if (slot < cap) pairs[slot] = pr;
         }
         ++cnt;
       }
@@ -547,21 +563,30 @@ __global__ void __launch_bounds__(256) k_bvh_count(int t, const Node* __restrict
   if (skip && *skip) return;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= t) return;
-  count[k] = t < 2 ? 0u : bvh_walk<false>(k, node, leaf, nullptr, nullptr, 0, 0);
+  count[k] = t < 2 ? 0u : bvh_walk<false>(k, node, leaf, nullptr, nullptr, 0, 0, nullptr, 0);
 }
 // offset: exclusive scan of count. The last leaf records the total.
 __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
-                                                   const int* __restrict__ orig,
+                                                   const int* __restrict__ sorig,
                                                    const unsigned int* __restrict__ count,
                                                    const unsigned int* __restrict__ offset, int2* __restrict__ pairs,
                                                    unsigned int cap, BvhDev* bd, const int* skip) {
-  if (skip && *skip) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= t) return;
-  const unsigned int base = offset[k];
-  if (k == t - 1) bd->npairs = base + count[k];
-  if (t < 2 || count[k] == 0 || base >= cap) return;
-  bvh_walk<true>(k, node, leaf, orig, pairs, base, cap);
+  if (skip && *skip) return;  // uniform over the grid
+  __shared__ int2 stage[kPairStage];
+  const int k0 = blockIdx.x * blockDim.x;
+  const int kl = min(k0 + (int)blockDim.x, t) - 1;  // the group's last leaf
+  const unsigned int gbase = offset[k0];
+  const unsigned int gend = offset[kl] + count[kl];
+  const int k = k0 + threadIdx.x;
+  if (k < t) {
+    const unsigned int base = offset[k];
+    if (k == t - 1) bd->npairs = base + count[k];
+    if (t >= 2 && count[k] != 0 && base < cap) bvh_walk<true>(k, node, leaf, sorig, pairs, base, cap, stage, gbase);
+  }
+  __syncthreads();
+  const unsigned int staged = min(gend - gbase, (unsigned int)kPairStage);
+  for (unsigned int q = threadIdx.x; q < staged; q += blockDim.x)
+    if (gbase + q < cap) pairs[gbase + q] = stage[q];
 }
 
 // ---- IPC step state --------------------------------------------------------
