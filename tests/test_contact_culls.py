@@ -1,0 +1,130 @@
+"""The pair culls of the GPU contact kernels are exact: a culled pair can
+never contribute. The bounds are restated here as the kernels compute them
+(vh_contact.hpp: k_ccd_tri_bounds + k_ccd_pairs, k_tri_spheres +
+k_self_contacts) and checked against the oracle's own vertex-triangle CCD and
+distance functions on random triangle pairs at many scales. A bound made too
+loose on purpose (here, on the CPU) is caught by the same checks."""
+import numpy as np
+
+from oracle import ref
+
+F32 = np.float32
+
+
+def _round_up(v):
+    return float(F32(v * (1.0 + 1e-6))) + float(F32(1e-30))
+
+
+def _tri_record(P, D=None):
+    """(centre as float32, radius rounded up, max displacement rounded up)."""
+    c = np.array([F32(v) for v in P.mean(0)], dtype=np.float64)
+    r = _round_up(np.sqrt(max(((p - c) ** 2).sum() for p in P)))
+    m = 0.0 if D is None else _round_up(np.sqrt(max((d ** 2).sum() for d in D)))
+    return c, r, m
+
+
+def _ccd_culled(A0, DA, B0, DB, half_eps=0.5e-8):
+    ca, ra, ma = _tri_record(A0, DA)
+    cb, rb, mb = _tri_record(B0, DB)
+    D = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 + 1e-12) + ra + rb
+    W = ma + mb
+    return W ** 3 < half_eps and 3 * D * W * W < half_eps and 3 * D * D * W < half_eps
+
+
+def _ccd_hits(A0, A1, B0, B1):
+    hits = []
+    for P0, P1, T0, T1 in ((A0, A1, B0, B1), (B0, B1, A0, A1)):
+        for q in range(3):
+            t = ref.vt_ccd(P0[q], P1[q], T0[0], T1[0], T0[1], T1[1], T0[2], T1[2])
+            if t is not None:
+                hits.append(t)
+    return hits
+
+
+def _random_pair(rng, scale, motion):
+    A0 = rng.normal(size=(3, 3)) * scale
+    B0 = A0.mean(0) + rng.normal(size=(3, 3)) * scale * rng.uniform(0.5, 3.0)
+    DA = rng.normal(size=(3, 3)) * motion
+    DB = rng.normal(size=(3, 3)) * motion
+    return A0, DA, B0, DB
+
+
+def _ccd_trials(half_eps):
+    rng = np.random.default_rng(11)
+    culled = culled_with_hit = 0
+    hits_seen = 0
+    for scale in (3e-4, 1e-3, 3e-3, 1e-2, 3e-2):
+        for motion in (1e-7, 1e-6, 1e-5, 1e-4, 1e-3):
+            for _ in range(160):
+                A0, DA, B0, DB = _random_pair(rng, scale, motion)
+                hits = _ccd_hits(A0, A0 + DA, B0, B0 + DB)
+                hits_seen += bool(hits)
+                if _ccd_culled(A0, DA, B0, DB, half_eps):
+                    culled += 1
+                    culled_with_hit += bool(hits)
+    return culled, culled_with_hit, hits_seen
+
+
+def test_ccd_pair_cull_never_drops_a_hit():
+    culled, bad, hits = _ccd_trials(0.5e-8)
+    print(f"ccd cull: {culled} of 4000 pairs culled, {hits} pairs with a hit", flush=True)
+    assert culled > 1000 and hits > 50
+    assert bad == 0
+
+
+def test_ccd_cull_check_catches_a_loose_bound():
+    """At 50x the kernel's threshold the bound admits pairs whose cubic has
+    roots the solver reports, and the check above flags them."""
+    culled, bad, _ = _ccd_trials(25e-8)
+    assert bad > 0, culled
+
+
+def _sphere_culled(A, B, dh, rel=1e-6):
+    ca, ra, _ = _tri_record(A)
+    cb, rb, _ = _tri_record(B)
+    gap = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 - 1e-12) - ra - rb
+    return gap > 0 and gap * gap > dh * (1.0 + rel)
+
+
+def _min_dist2(A, B):
+    d = [ref.pt_dist2(A[q], B[0], B[1], B[2])[0] for q in range(3)]
+    d += [ref.pt_dist2(B[q], A[0], A[1], A[2])[0] for q in range(3)]
+    d += [ref.ee_dist2(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3])[0] for i in range(3) for j in range(3)]
+    return min(d)
+
+
+def _sphere_trials(shrink):
+    rng = np.random.default_rng(12)
+    culled = bad = near = 0
+    for scale in (1e-3, 3e-3, 1e-2):
+        for _ in range(700):
+            A = rng.normal(size=(3, 3)) * scale
+            u = rng.normal(size=3)
+            B = A.mean(0) + u / np.linalg.norm(u) * scale * rng.uniform(0.5, 6.0) + rng.normal(size=(3, 3)) * scale
+            dh = (scale * rng.uniform(0.05, 3.0)) ** 2
+            d2 = _min_dist2(A, B)
+            near += d2 < dh
+            # shrink < 1 makes the spheres too small: a planted mistake
+            ca, ra, _ = _tri_record(A)
+            cb, rb, _ = _tri_record(B)
+            gap = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 - 1e-12) - shrink * (ra + rb)
+            if gap > 0 and gap * gap > dh * (1.0 + 1e-6):
+                culled += 1
+                bad += d2 < dh
+    return culled, bad, near
+
+
+def test_sphere_cull_never_drops_a_contact():
+    culled, bad, near = _sphere_trials(1.0)
+    print(f"sphere cull: {culled} of 2100 pairs culled, {near} pairs within reach", flush=True)
+    assert culled > 300 and near > 300
+    assert bad == 0
+    # and the helper agrees with the test's inline form
+    rng = np.random.default_rng(3)
+    A, B = rng.normal(size=(3, 3)), rng.normal(size=(3, 3)) + 5.0
+    assert _sphere_culled(A, B, 1e-2)
+
+
+def test_sphere_cull_check_catches_a_loose_bound():
+    culled, bad, _ = _sphere_trials(0.5)
+    assert bad > 0, culled
