@@ -96,6 +96,12 @@ struct vh_ctx {
   IpcState ipc;
   std::vector<void*> allocs;
   std::vector<double> hbuf;  // host staging (3n)
+  // Drop-in steps on host f32 state: pinned staging of the caller-order SoA
+  // arrays (16 n floats: in pos, vel, inv_mass, lambda; out pos, vel, prev,
+  // pred, lambda), its device copy and the permutation, made on first use.
+  float* h_io = nullptr;
+  float* d_io = nullptr;
+  int* d_perm = nullptr;
   // GpuBackend staging (vh_backend_*): device copies of the operands, kept
   // across calls and grown on demand; independent of the solver's state.
   float *bk_a = nullptr, *bk_b = nullptr, *bk_o = nullptr;
@@ -175,6 +181,10 @@ void free_all(vh_ctx* ctx) {
   ctx->initialized = false;
   if (ctx->ipc.h_done) (void)hipHostFree(ctx->ipc.h_done);
   ctx->ipc = IpcState{};
+  if (ctx->h_io) (void)hipHostFree(ctx->h_io);
+  ctx->h_io = nullptr;
+  ctx->d_io = nullptr;
+  ctx->d_perm = nullptr;
 }
 
 inline int nblk(int n) { return (n + 255) / 256; }
@@ -939,6 +949,97 @@ vh_status vh_pd_step_resident(vh_ctx* ctx, double dt, uint32_t nsteps, vh_step_r
   return st;
 }
 
+// ---- host f32 state in and out of the drop-in steps ------------------------
+// The caller's SoA arrays are packed into pinned memory and copied in one
+// transfer; a kernel permutes them into the device order and widens them to
+// FP64. On the way out a kernel narrows (the same round-to-nearest as a host
+// cast) and permutes into caller order, and one transfer brings them back.
+// Slots of h_io / d_io, n floats each: in 0-2 pos, 3-5 vel, 6 inv_mass,
+// 7-9 lambda; out 0-2 pos, 3-5 vel, 6-8 prev, 9-11 pred, 12-14 lambda.
+constexpr int kIoSlots = 16;
+
+__global__ void __launch_bounds__(256) k_io_in(int n, const int* __restrict__ perm, const float* __restrict__ in,
+                                               int has_inv, int has_lambda, double* __restrict__ x,
+                                               double* __restrict__ v, uint8_t* __restrict__ pinned,
+                                               double* __restrict__ lambda) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t o = (size_t)perm[k], nn = (size_t)n;
+  for (int c = 0; c < 3; ++c) {
+    x[3 * (size_t)k + c] = (double)in[c * nn + o];
+    v[3 * (size_t)k + c] = (double)in[(3 + c) * nn + o];
+    if (has_lambda) lambda[3 * (size_t)k + c] = (double)in[(7 + c) * nn + o];
+  }
+  if (has_inv) pinned[k] = in[6 * nn + o] == 0.0f ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) k_io_out(int n, const int* __restrict__ perm, const double* __restrict__ x,
+                                                const double* __restrict__ v, const double* __restrict__ prev,
+                                                const double* __restrict__ pred, const double* __restrict__ lambda,
+                                                float* __restrict__ out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const size_t o = (size_t)perm[k], nn = (size_t)n;
+  for (int c = 0; c < 3; ++c) {
+    out[c * nn + o] = (float)x[3 * (size_t)k + c];
+    out[(3 + c) * nn + o] = (float)v[3 * (size_t)k + c];
+    out[(6 + c) * nn + o] = (float)prev[3 * (size_t)k + c];
+    out[(9 + c) * nn + o] = (float)pred[3 * (size_t)k + c];
+    if (lambda) out[(12 + c) * nn + o] = (float)lambda[3 * (size_t)k + c];
+  }
+}
+
+vh_status io_reserve(vh_ctx* ctx) {
+  if (ctx->h_io) return VH_OK;
+  const size_t n = (size_t)ctx->S.n;
+  VH_HIP(hipHostMalloc((void**)&ctx->h_io, kIoSlots * std::max<size_t>(n, 1) * sizeof(float), hipHostMallocDefault));
+  vh_status st = dalloc(ctx, &ctx->d_io, kIoSlots * std::max<size_t>(n, 1));
+  if (st == VH_OK) st = upload(ctx, &ctx->d_perm, ctx->S.perm);
+  if (st != VH_OK) {
+    (void)hipHostFree(ctx->h_io);
+    ctx->h_io = nullptr;
+  }
+  return st;
+}
+
+// Packs and uploads pos, vel, inv_mass (optional) and lambda (optional), then
+// scatters them into the device state.
+vh_status io_upload(vh_ctx* ctx, const vh_state_f32* s, const float* lx, const float* ly, const float* lz) {
+  const int n = ctx->S.n;
+  const size_t nb = (size_t)n * sizeof(float);
+  float* h = ctx->h_io;
+  const float* src[10] = {s->pos_x, s->pos_y, s->pos_z, s->vel_x, s->vel_y, s->vel_z, s->inv_mass, lx, ly, lz};
+  int last = 0;
+  for (int q = 0; q < 10; ++q)
+    if (src[q]) { std::memcpy(h + (size_t)q * n, src[q], nb); last = q; }
+  VH_HIP(hipMemcpyAsync(ctx->d_io, h, (size_t)(last + 1) * nb, hipMemcpyHostToDevice, ctx->stream));
+  hipLaunchKernelGGL(k_io_in, dim3(std::max(1, nblk(n))), dim3(256), 0, ctx->stream, n, ctx->d_perm, ctx->d_io,
+                     s->inv_mass ? 1 : 0, lx ? 1 : 0, ctx->x, ctx->v, ctx->pinned, ctx->ipc.lambda);
+  VH_HIP(hipGetLastError());
+  return VH_OK;
+}
+
+// Gathers pos, vel, prev, pred (and lambda) into caller order, downloads them
+// and synchronizes; io_scatter then copies them into the caller's arrays.
+vh_status io_download(vh_ctx* ctx, bool with_lambda) {
+  const int n = ctx->S.n;
+  const size_t nb = (size_t)n * sizeof(float);
+  hipLaunchKernelGGL(k_io_out, dim3(std::max(1, nblk(n))), dim3(256), 0, ctx->stream, n, ctx->d_perm, ctx->x, ctx->v,
+                     ctx->prev, ctx->pred, with_lambda ? ctx->ipc.lambda : nullptr, ctx->d_io);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(ctx->h_io, ctx->d_io, (size_t)(with_lambda ? 15 : 12) * nb, hipMemcpyDeviceToHost,
+                        ctx->stream));
+  return VH_OK;
+}
+
+void io_scatter(vh_ctx* ctx, vh_state_f32* s, float* lx, float* ly, float* lz) {
+  const size_t n = (size_t)ctx->S.n, nb = n * sizeof(float);
+  float* dst[15] = {s->pos_x, s->pos_y, s->pos_z, s->vel_x, s->vel_y, s->vel_z, s->prev_x, s->prev_y, s->prev_z,
+                    s->pred_x, s->pred_y, s->pred_z, lx, ly, lz};
+  for (int q = 0; q < 15; ++q)
+    if (dst[q]) std::memcpy(dst[q], ctx->h_io + q * n, nb);
+}
+
 vh_status vh_pd_step(vh_ctx* ctx, double dt, vh_state_f32* s, vh_step_result* out) {
   if (!ctx) return VH_E_INVALID_ARG;
   if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
@@ -947,38 +1048,22 @@ vh_status vh_pd_step(vh_ctx* ctx, double dt, vh_state_f32* s, vh_step_result* ou
       !s->prev_y || !s->prev_z || !s->pred_x || !s->pred_y || !s->pred_z)
     return fail(ctx, VH_E_INVALID_ARG, "null state array");
   const auto t0 = std::chrono::steady_clock::now();
-  const vh::Setup& S = ctx->S;
-  const size_t n3 = 3 * (size_t)S.n;
-  std::vector<double> hp(n3), hv(n3);
-  for (int k = 0; k < S.n; ++k) {
-    const int o = S.perm[k];
-    hp[3 * k] = s->pos_x[o]; hp[3 * k + 1] = s->pos_y[o]; hp[3 * k + 2] = s->pos_z[o];
-    hv[3 * k] = s->vel_x[o]; hv[3 * k + 1] = s->vel_y[o]; hv[3 * k + 2] = s->vel_z[o];
-  }
   VH_HIP(hipSetDevice(ctx->device));
-  if (s->inv_mass) {
-    vh_status e = vh_state_set_inv_mass(ctx, s->inv_mass);
+  {
+    vh_status e = io_reserve(ctx);
+    if (e == VH_OK) e = io_upload(ctx, s, nullptr, nullptr, nullptr);
     if (e != VH_OK) return e;
   }
-  VH_HIP(hipMemcpyAsync(ctx->x, hp.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  VH_HIP(hipMemcpyAsync(ctx->v, hv.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
   {
     const vh_status ls = launch_step(ctx, dt);
     if (ls != VH_OK) return ls;
   }
-  std::vector<double> hq(n3), hr(n3);
-  VH_HIP(hipMemcpyAsync(hp.data(), ctx->x, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hv.data(), ctx->v, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hq.data(), ctx->prev, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hr.data(), ctx->pred, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  vh_status st = read_step_result(ctx, out);
-  for (int k = 0; k < S.n; ++k) {
-    const int o = S.perm[k];
-    s->pos_x[o] = (float)hp[3 * k]; s->pos_y[o] = (float)hp[3 * k + 1]; s->pos_z[o] = (float)hp[3 * k + 2];
-    s->vel_x[o] = (float)hv[3 * k]; s->vel_y[o] = (float)hv[3 * k + 1]; s->vel_z[o] = (float)hv[3 * k + 2];
-    s->prev_x[o] = (float)hq[3 * k]; s->prev_y[o] = (float)hq[3 * k + 1]; s->prev_z[o] = (float)hq[3 * k + 2];
-    s->pred_x[o] = (float)hr[3 * k]; s->pred_y[o] = (float)hr[3 * k + 1]; s->pred_z[o] = (float)hr[3 * k + 2];
+  {
+    const vh_status e = io_download(ctx, false);
+    if (e != VH_OK) return e;
   }
+  vh_status st = read_step_result(ctx, out);  // synchronizes
+  io_scatter(ctx, s, nullptr, nullptr, nullptr);
   if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
@@ -1264,39 +1349,19 @@ vh_status vh_pd_step_ipc(vh_ctx* ctx, double dt, vh_state_f32* s, float* lambda_
       !s->prev_y || !s->prev_z || !s->pred_x || !s->pred_y || !s->pred_z || !lambda_x || !lambda_y || !lambda_z)
     return fail(ctx, VH_E_INVALID_ARG, "null state array");
   const auto t0 = std::chrono::steady_clock::now();
-  const vh::Setup& S = ctx->S;
-  const size_t n3 = 3 * (size_t)S.n;
-  std::vector<double> hp(n3), hv(n3), hl(n3);
-  for (int k = 0; k < S.n; ++k) {
-    const int o = S.perm[k];
-    hp[3 * k] = s->pos_x[o]; hp[3 * k + 1] = s->pos_y[o]; hp[3 * k + 2] = s->pos_z[o];
-    hv[3 * k] = s->vel_x[o]; hv[3 * k + 1] = s->vel_y[o]; hv[3 * k + 2] = s->vel_z[o];
-    hl[3 * k] = lambda_x[o]; hl[3 * k + 1] = lambda_y[o]; hl[3 * k + 2] = lambda_z[o];
-  }
   VH_HIP(hipSetDevice(ctx->device));
-  if (s->inv_mass) {
-    vh_status e = vh_state_set_inv_mass(ctx, s->inv_mass);
+  {
+    vh_status e = io_reserve(ctx);
+    if (e == VH_OK) e = io_upload(ctx, s, lambda_x, lambda_y, lambda_z);
     if (e != VH_OK) return e;
   }
-  VH_HIP(hipMemcpyAsync(ctx->x, hp.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  VH_HIP(hipMemcpyAsync(ctx->v, hv.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
-  VH_HIP(hipMemcpyAsync(ctx->ipc.lambda, hl.data(), n3 * 8, hipMemcpyHostToDevice, ctx->stream));
   VH_HIP(enqueue_ipc_step(ctx, dt));
-  std::vector<double> hq(n3), hr(n3);
-  VH_HIP(hipMemcpyAsync(hp.data(), ctx->x, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hv.data(), ctx->v, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hq.data(), ctx->prev, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hr.data(), ctx->pred, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  VH_HIP(hipMemcpyAsync(hl.data(), ctx->ipc.lambda, n3 * 8, hipMemcpyDeviceToHost, ctx->stream));
-  vh_status st = read_ipc_result(ctx, out, nullptr);
-  for (int k = 0; k < S.n; ++k) {
-    const int o = S.perm[k];
-    s->pos_x[o] = (float)hp[3 * k]; s->pos_y[o] = (float)hp[3 * k + 1]; s->pos_z[o] = (float)hp[3 * k + 2];
-    s->vel_x[o] = (float)hv[3 * k]; s->vel_y[o] = (float)hv[3 * k + 1]; s->vel_z[o] = (float)hv[3 * k + 2];
-    s->prev_x[o] = (float)hq[3 * k]; s->prev_y[o] = (float)hq[3 * k + 1]; s->prev_z[o] = (float)hq[3 * k + 2];
-    s->pred_x[o] = (float)hr[3 * k]; s->pred_y[o] = (float)hr[3 * k + 1]; s->pred_z[o] = (float)hr[3 * k + 2];
-    lambda_x[o] = (float)hl[3 * k]; lambda_y[o] = (float)hl[3 * k + 1]; lambda_z[o] = (float)hl[3 * k + 2];
+  {
+    const vh_status e = io_download(ctx, true);
+    if (e != VH_OK) return e;
   }
+  vh_status st = read_ipc_result(ctx, out, nullptr);  // synchronizes
+  io_scatter(ctx, s, lambda_x, lambda_y, lambda_z);
   if (out) out->wall_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return st;
 }
