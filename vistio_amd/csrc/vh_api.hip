@@ -287,14 +287,25 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
 }
 
 template <class T>
+vhk::ElemArgs<T> elem_args(vh_ctx* c) {
+  return vhk::ElemArgs<T>{c->mem, c->x, c->tri_idx, c->tri_dm, (const T*)c->tri_w, c->hinge_idx,
+                          (const T*)c->hinge_rest, (const T*)c->hinge_w, (const T*)c->hinge_st};
+}
+
+template <class T>
 void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
   const int tb = nblk(S.t), hb = nblk(S.h);
   if (tb + hb > 0)
-    hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb, c->mem, c->x,
-                       c->tri_idx, c->tri_dm, (const T*)c->tri_w, c->tri_slot, c->hinge_idx,
-                       (const T*)c->hinge_rest, (const T*)c->hinge_w, (const T*)c->hinge_st, c->hinge_slot,
-                       (T*)c->cslot, st);
+    hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb, elem_args<T>(c),
+                       c->tri_slot, c->hinge_slot, (T*)c->cslot, st);
+}
+
+// The local step of one PD iteration (k_local into the slots). Returns true:
+// the solve then starts from the slots.
+bool enqueue_local_step(vh_ctx* c, const vhk::StepDev* st) {
+  if (c->prec == VH_PREC_F64) enqueue_local<double>(c, st); else enqueue_local<float>(c, st);
+  return true;
 }
 
 // One correction of the iterate: x += A^{-1} (b - A x). FP64: one solve whose
@@ -331,10 +342,9 @@ void enqueue_step(vh_ctx* c, double dt) {
   for (int it = 0; it < P.max_iterations; ++it) {
     const bool last = it + 1 == P.max_iterations;
     const bool want_resid = last || P.tolerance > 0;
-    if (c->prec == VH_PREC_F64) enqueue_local<double>(c, stk);
-    else enqueue_local<float>(c, stk);
+    const bool from_slots = enqueue_local_step(c, stk);
     Marks none;
-    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, true, true, want_resid, stk, none);
+    enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, from_slots, true, want_resid, stk, none);
     if (want_resid)
       hipLaunchKernelGGL(vhk::k_resid_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, c->resid_blocks,
                          P.tolerance, it, c->st);
@@ -579,10 +589,10 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
     hipLaunchKernelGGL(vhc::k_ipc_ext, dim3(nblk(3 * n)), dim3(256), 0, c->stream, n, I.grad, I.lambda, I.ext, I.ip);
     c->cur_ext = I.ext;
     for (int it = 0; it < P.max_iterations; ++it) {
-      if (f64) enqueue_local<double>(c, c->st); else enqueue_local<float>(c, c->st);
+      const bool from_slots = enqueue_local_step(c, c->st);
       Marks none;
       const int sweeps = it + 1 == P.max_iterations ? P.ir_final_sweeps : P.ir_sweeps;
-      enqueue_correction(c, dt, sweeps, true, false, false, c->st, none);
+      enqueue_correction(c, dt, sweeps, from_slots, false, false, c->st, none);
       hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
                          I.partials, I.ip);
       if (I.sc.self_collision) {
@@ -1075,14 +1085,13 @@ vh_status vh_pd_residual(vh_ctx* ctx, double dt, double* out) {
   if (!(dt > 0)) return fail(ctx, VH_E_INVALID_CONFIG, "dt must be > 0");
   VH_HIP(hipSetDevice(ctx->device));
   const int n = ctx->S.n;
-  if (ctx->prec == VH_PREC_F64) {
-    enqueue_local<double>(ctx, nullptr);
-    hipLaunchKernelGGL(vhk::k_gather<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, rhs_src<double>(ctx, dt),
-                       ctx->r);
-  } else {
-    enqueue_local<float>(ctx, nullptr);
-    hipLaunchKernelGGL(vhk::k_gather<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, rhs_src<float>(ctx, dt),
-                       ctx->r);
+  if (enqueue_local_step(ctx, nullptr)) {
+    if (ctx->prec == VH_PREC_F64)
+      hipLaunchKernelGGL(vhk::k_gather<double>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n,
+                         rhs_src<double>(ctx, dt), ctx->r);
+    else
+      hipLaunchKernelGGL(vhk::k_gather<float>, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, rhs_src<float>(ctx, dt),
+                         ctx->r);
   }
   VH_HIP(hipGetLastError());
   const size_t n3 = 3 * (size_t)n;
@@ -1177,9 +1186,9 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
   mk.ev = ev.data();
   mk.mark(ctx->stream);
   for (uint32_t it = 0; it < iterations; ++it) {
-    if (f64) enqueue_local<double>(ctx, nullptr); else enqueue_local<float>(ctx, nullptr);
+    const bool from_slots = enqueue_local_step(ctx, nullptr);
     mk.mark(ctx->stream);
-    enqueue_correction(ctx, dt, sweeps, true, true, false, nullptr, mk);
+    enqueue_correction(ctx, dt, sweeps, from_slots, true, false, nullptr, mk);
   }
   VH_HIP(hipGetLastError());
   VH_HIP(hipStreamSynchronize(ctx->stream));

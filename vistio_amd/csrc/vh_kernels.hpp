@@ -319,6 +319,75 @@ template <class T> struct Prec;
 template <> struct Prec<double> { static constexpr bool f32 = false; };
 template <> struct Prec<float> { static constexpr bool f32 = true; };
 
+// Element data of the local step (k_local: one thread per element, corners
+// written to their slots).
+template <class T>
+struct ElemArgs {
+  MemModel mm;
+  const double* x;
+  const int* tri_idx;
+  const double* tri_dm;
+  const T* tri_w;
+  const int* hinge_idx;
+  const T* hinge_rest;
+  const T* hinge_w;
+  const T* hinge_st;  // null: dihedral springs
+};
+
+// Triangle e: w and its two D columns (the membrane projection's F_target - F).
+template <class T>
+__device__ __forceinline__ void tri_delta(const ElemArgs<T>& ea, int e, T& w, T (&g)[3][2], T (&D)[2][3]) {
+  const int i0 = ea.tri_idx[3 * e], i1 = ea.tri_idx[3 * e + 1], i2 = ea.tri_idx[3 * e + 2];
+  const double3 p0 = ld3(ea.x, i0), p1 = ld3(ea.x, i1), p2 = ld3(ea.x, i2);
+  double dm[4];
+  for (int k = 0; k < 4; ++k) dm[k] = ea.tri_dm[4 * e + k];
+  w = ea.tri_w[e];
+  const T a = (T)dm[0], b = (T)dm[1], c = (T)dm[2], d = (T)dm[3];
+  g[0][0] = -(a + b); g[0][1] = -(c + d);
+  g[1][0] = a; g[1][1] = c;
+  g[2][0] = b; g[2][1] = d;
+  if (Prec<T>::f32) {
+    float3 D0, D1;
+    membrane_delta_f32(p0, p1, p2, dm, ea.mm, D0, D1);
+    D[0][0] = D0.x; D[0][1] = D0.y; D[0][2] = D0.z; D[1][0] = D1.x; D[1][1] = D1.y; D[1][2] = D1.z;
+  } else {
+    double3 D0, D1;
+    membrane_delta_f64(p0, p1, p2, dm, ea.mm, D0, D1);
+    D[0][0] = D0.x; D[0][1] = D0.y; D[0][2] = D0.z; D[1][0] = D1.x; D[1][1] = D1.y; D[1][2] = D1.z;
+  }
+}
+
+// Corner k of triangle e: w g_k^T D (assembly.rs:176-205 minus A x).
+template <class T>
+__device__ __forceinline__ void tri_corner(T w, const T (&g)[3][2], const T (&D)[2][3], int k, T& ox, T& oy, T& oz) {
+  const T g0 = k == 0 ? g[0][0] : (k == 1 ? g[1][0] : g[2][0]);
+  const T g1 = k == 0 ? g[0][1] : (k == 1 ? g[1][1] : g[2][1]);
+  ox = w * (g0 * D[0][0] + g1 * D[1][0]);
+  oy = w * (g0 * D[0][1] + g1 * D[1][1]);
+  oz = w * (g0 * D[0][2] + g1 * D[1][2]);
+}
+
+// Hinge e: its stencil and w_b (s^T target - s^T x).
+template <class T>
+__device__ __forceinline__ void hinge_delta(const ElemArgs<T>& ea, int e, T (&sg)[4], T& cx, T& cy, T& cz) {
+  const int* q = ea.hinge_idx + 4 * (size_t)e;
+  const double3 p0 = ld3(ea.x, q[0]), p1 = ld3(ea.x, q[1]), pa = ld3(ea.x, q[2]), pb = ld3(ea.x, q[3]);
+  const T w = ea.hinge_w[e];
+  // stencil s: (-1, -1, 1, 1) for dihedral springs, else the hinge's
+  // Discrete Shells stencil
+  sg[0] = T(-1); sg[1] = T(-1); sg[2] = T(1); sg[3] = T(1);
+  if (ea.hinge_st)
+    for (int k = 0; k < 4; ++k) sg[k] = ea.hinge_st[4 * (size_t)e + k];
+  if (Prec<T>::f32) {
+    const float3 c = bending_delta_f32(p0, p1, pa, pb, (float)ea.hinge_rest[e], (float)sg[2], (float)sg[3]);
+    cx = c.x; cy = c.y; cz = c.z;
+  } else {
+    const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)ea.hinge_rest[e], (double)sg[2], (double)sg[3]);
+    cx = c.x; cy = c.y; cz = c.z;
+  }
+  cx *= w; cy *= w; cz *= w;
+}
+
 // Local step: one thread per triangle (blocks [0, tb)) or hinge (the rest).
 // pd_solver.rs:300-367 (the loops) and the per-element RHS terms of
 // assembly.rs:176-235. Each element corner's contribution goes to its own
@@ -327,61 +396,25 @@ template <> struct Prec<float> { static constexpr bool f32 = true; };
 // assembly.rs:210, or the hinge's Discrete Shells stencil, :222-236), so the RHS
 // gather is a contiguous segmented sum with no atomics.
 template <class T>
-__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, MemModel mm, const double* __restrict__ x,
-                                               const int* __restrict__ tri_idx, const double* __restrict__ tri_dm,
-                                               const T* __restrict__ tri_w, const int* __restrict__ tri_slot,
-                                               const int* __restrict__ hinge_idx, const T* __restrict__ hinge_rest,
-                                               const T* __restrict__ hinge_w, const T* __restrict__ hinge_st,
+__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, ElemArgs<T> ea, const int* __restrict__ tri_slot,
                                                const int* __restrict__ hinge_slot, T* __restrict__ cslot,
                                                const StepDev* st) {
   if (st && st->done) return;
   if ((int)blockIdx.x < tb) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= t) return;
-    const int i0 = tri_idx[3 * e], i1 = tri_idx[3 * e + 1], i2 = tri_idx[3 * e + 2];
-    const double3 p0 = ld3(x, i0), p1 = ld3(x, i1), p2 = ld3(x, i2);
-    double dm[4];
-    for (int k = 0; k < 4; ++k) dm[k] = tri_dm[4 * e + k];
-    const T w = tri_w[e];
-    const T a = (T)dm[0], b = (T)dm[1], c = (T)dm[2], d = (T)dm[3];
-    const T g[3][2] = {{-(a + b), -(c + d)}, {a, c}, {b, d}};
-    T Dx0, Dy0, Dz0, Dx1, Dy1, Dz1;
-    if (Prec<T>::f32) {
-      float3 D0, D1;
-      membrane_delta_f32(p0, p1, p2, dm, mm, D0, D1);
-      Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
-    } else {
-      double3 D0, D1;
-      membrane_delta_f64(p0, p1, p2, dm, mm, D0, D1);
-      Dx0 = D0.x; Dy0 = D0.y; Dz0 = D0.z; Dx1 = D1.x; Dy1 = D1.y; Dz1 = D1.z;
-    }
+    T w, g[3][2], D[2][3];
+    tri_delta(ea, e, w, g, D);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       T* o = cslot + 3 * (size_t)tri_slot[3 * e + k];
-      o[0] = w * (g[k][0] * Dx0 + g[k][1] * Dx1);
-      o[1] = w * (g[k][0] * Dy0 + g[k][1] * Dy1);
-      o[2] = w * (g[k][0] * Dz0 + g[k][1] * Dz1);
+      tri_corner(w, g, D, k, o[0], o[1], o[2]);
     }
   } else {
     const int e = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
     if (e >= h) return;
-    const int* q = hinge_idx + 4 * (size_t)e;
-    const double3 p0 = ld3(x, q[0]), p1 = ld3(x, q[1]), pa = ld3(x, q[2]), pb = ld3(x, q[3]);
-    const T w = hinge_w[e];
-    // stencil s: (-1, -1, 1, 1) for dihedral springs, else the hinge's
-    // Discrete Shells stencil
-    T sg[4] = {T(-1), T(-1), T(1), T(1)};
-    if (hinge_st)
-      for (int k = 0; k < 4; ++k) sg[k] = hinge_st[4 * (size_t)e + k];
-    T cx, cy, cz;
-    if (Prec<T>::f32) {
-      const float3 c = bending_delta_f32(p0, p1, pa, pb, (float)hinge_rest[e], (float)sg[2], (float)sg[3]);
-      cx = c.x; cy = c.y; cz = c.z;
-    } else {
-      const double3 c = bending_delta_f64(p0, p1, pa, pb, (double)hinge_rest[e], (double)sg[2], (double)sg[3]);
-      cx = c.x; cy = c.y; cz = c.z;
-    }
-    cx *= w; cy *= w; cz *= w;
+    T sg[4], cx, cy, cz;
+    hinge_delta(ea, e, sg, cx, cy, cz);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
       T* o = cslot + 3 * (size_t)hinge_slot[4 * e + k];
