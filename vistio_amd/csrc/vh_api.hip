@@ -77,6 +77,8 @@ struct vh_ctx {
   uint8_t* pinned = nullptr;
   // elements
   int *tri_idx = nullptr, *tri_slot = nullptr, *hinge_idx = nullptr, *hinge_slot = nullptr;
+  int* chunk_of = nullptr;  // k_local's block -> element chunk map
+  int local_blocks = 0;
   double* tri_dm = nullptr;
   void *tri_w = nullptr, *hinge_rest = nullptr, *hinge_w = nullptr;
   void* hinge_st = nullptr;  // Discrete Shells stencils (T [h][4]); null for dihedral springs
@@ -202,6 +204,23 @@ vhk::CommitArgs commit_args(vh_ctx* c, bool on, bool want_resid, bool add_acc) {
                          want_resid ? 1 : 0, c->acc, add_acc ? 1 : 0};
 }
 
+// Task blocks of a band launch: one contiguous run of tasks per XCD
+// (BandArgs::xper; see task_of) for bands of 64 to 4096 tasks, and for the
+// first launch of a sweep (it also gathers the RHS slots of its columns)
+// whatever its size; otherwise one block per task, dealt to the XCDs in turn.
+// Measured: the runs take a single 256x256 sheet from 949 to 974 steps/s;
+// on 64 sheets in one system the runs slowed the larger bands by 4-7 %
+// (tasks of unequal cost gather on one XCD) but sped up the first launch.
+constexpr int kXcdMinTasks = 64, kXcdMaxTasks = 4096;
+int band_xper(int ntasks, bool first) {
+  const bool runs = ntasks >= kXcdMinTasks && (first || ntasks <= kXcdMaxTasks);
+  return runs ? (ntasks + vhk::kXcds - 1) / vhk::kXcds : 0;
+}
+int band_task_blocks(int ntasks, bool first) {
+  const int x = band_xper(ntasks, first);
+  return x ? vhk::kXcds * x : ntasks;
+}
+
 // Records an event after each launch when profiling (ev == nullptr: off).
 struct Marks {
   hipEvent_t* ev = nullptr;
@@ -229,8 +248,8 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
   for (int l = 0; l + 1 < nb; ++l) {
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
-    a.tasks = c->tasks + b; a.ntasks = e - b;
-    const int grid = (e - b) + (first ? nblk(4 * nupper) : 0);
+    a.tasks = c->tasks + b; a.ntasks = e - b; a.xper = band_xper(e - b, l == 0);
+    const int grid = band_task_blocks(e - b, l == 0) + (first ? nblk(4 * nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * c->tsz;
     if (grid > 0) {
       if (first)
@@ -247,22 +266,23 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
     const int b = S.fwd_level_off[nb - 1], e = S.fwd_level_off[nb];
     const bool only = nb == 1;
     const bool slots = only && from_slots, cm = only && commit;
-    a.tasks = c->tasks + b; a.ntasks = e - b;
+    a.tasks = c->tasks + b; a.ntasks = e - b; a.xper = band_xper(e - b, false);
     const vhk::CommitArgs cc = commit_args(c, cm, want_resid, add_acc);
     const size_t lds = 3 * (size_t)S.band_front[0] * c->tsz;
     const T* Sr = (const T*)c->Sroot;
+    const int rg = band_task_blocks(e - b, false);
     if (e > b) {
       if (slots && cm)
-        hipLaunchKernelGGL((vhk::k_root_band<T, true, true>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+        hipLaunchKernelGGL((vhk::k_root_band<T, true, true>), dim3(rg), dim3(256), lds, c->stream, a, cc, Sr,
                            (T*)c->F, (T*)c->V, (T*)c->d, st);
       else if (slots)
-        hipLaunchKernelGGL((vhk::k_root_band<T, true, false>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+        hipLaunchKernelGGL((vhk::k_root_band<T, true, false>), dim3(rg), dim3(256), lds, c->stream, a, cc, Sr,
                            (T*)c->F, (T*)c->V, (T*)c->d, st);
       else if (cm)
-        hipLaunchKernelGGL((vhk::k_root_band<T, false, true>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+        hipLaunchKernelGGL((vhk::k_root_band<T, false, true>), dim3(rg), dim3(256), lds, c->stream, a, cc, Sr,
                            (T*)c->F, (T*)c->V, (T*)c->d, st);
       else
-        hipLaunchKernelGGL((vhk::k_root_band<T, false, false>), dim3(e - b), dim3(256), lds, c->stream, a, cc, Sr,
+        hipLaunchKernelGGL((vhk::k_root_band<T, false, false>), dim3(rg), dim3(256), lds, c->stream, a, cc, Sr,
                            (T*)c->F, (T*)c->V, (T*)c->d, st);
     }
     mk.mark(c->stream);
@@ -270,9 +290,9 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
   for (int l = 1; l < nb; ++l) {
     const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
     const bool cm = l == nb - 1 && commit;
-    a.tasks = c->tasks + nfwd + b; a.ntasks = e - b;
+    a.tasks = c->tasks + nfwd + b; a.ntasks = e - b; a.xper = band_xper(e - b, false);
     const vhk::CommitArgs cc = commit_args(c, cm, want_resid, add_acc);
-    const int grid = (e - b) + (cm ? nblk(nupper) : 0);
+    const int grid = band_task_blocks(e - b, false) + (cm ? nblk(nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[l] * c->tsz;
     if (grid > 0) {
       if (cm)
@@ -295,10 +315,9 @@ vhk::ElemArgs<T> elem_args(vh_ctx* c) {
 template <class T>
 void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
-  const int tb = nblk(S.t), hb = nblk(S.h);
-  if (tb + hb > 0)
-    hipLaunchKernelGGL(vhk::k_local<T>, dim3(tb + hb), dim3(256), 0, c->stream, S.t, S.h, tb, elem_args<T>(c),
-                       c->tri_slot, c->hinge_slot, (T*)c->cslot, st);
+  if (c->local_blocks > 0)
+    hipLaunchKernelGGL(vhk::k_local<T>, dim3(c->local_blocks), dim3(256), 0, c->stream, S.t, S.h, c->chunk_of,
+                       elem_args<T>(c), c->tri_slot, c->hinge_slot, (T*)c->cslot, st);
 }
 
 // The local step of one PD iteration (k_local into the slots). Returns true:
@@ -817,6 +836,36 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->pinned, S.pinned));
   UP(upload(ctx, &ctx->tri_idx, S.tri_idx));
   UP(upload(ctx, &ctx->tri_slot, S.tri_slot));
+  {
+    // k_local's block order: triangle and hinge chunks merged by the smallest
+    // vertex label of their first element (both lists are sorted by it), the
+    // merged list cut into one contiguous run per XCD.
+    constexpr int kXcds = 8;
+    const int tb = nblk(S.t), hb = nblk(S.h);
+    std::vector<int> order;
+    order.reserve((size_t)tb + hb);
+    auto tkey = [&](int j) {
+      const int* q = &S.tri_idx[3 * (size_t)j * 256];
+      return std::min(q[0], std::min(q[1], q[2]));
+    };
+    auto hkey = [&](int j) {
+      const int* q = &S.hinge_idx[4 * (size_t)j * 256];
+      return std::min(std::min(q[0], q[1]), std::min(q[2], q[3]));
+    };
+    for (int a = 0, b = 0; a < tb || b < hb;) {
+      if (b >= hb || (a < tb && tkey(a) <= hkey(b))) order.push_back(a++);
+      else order.push_back(~(b++));
+    }
+    const int nch = (int)order.size();
+    const int per = (nch + kXcds - 1) / kXcds;
+    std::vector<int> map((size_t)per * kXcds, vhk::kIdleChunk);
+    for (int blk = 0; blk < per * kXcds; ++blk) {
+      const int i = (blk % kXcds) * per + blk / kXcds;
+      if (i < nch) map[blk] = order[i];
+    }
+    ctx->local_blocks = nch > 0 ? per * kXcds : 0;
+    UP(upload(ctx, &ctx->chunk_of, map));
+  }
   UP(upload(ctx, &ctx->tri_dm, S.tri_dm));
   UP(upload_t(ctx, &ctx->tri_w, S.tri_w));
   UP(upload(ctx, &ctx->hinge_idx, S.hinge_idx));
@@ -855,8 +904,9 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   // Residual partials come from the committing launch: the last backward
   // band, or the root band when it is the only band.
   const int nb = S.max_depth + 1;
-  ctx->resid_blocks = nb == 1 ? S.fwd_level_off[1] - S.fwd_level_off[0]
-                              : S.bwd_level_off[nb] - S.bwd_level_off[nb - 1] + nblk((int)S.upper_cols.size());
+  ctx->resid_blocks = nb == 1 ? band_task_blocks(S.fwd_level_off[1] - S.fwd_level_off[0], false)
+                              : band_task_blocks(S.bwd_level_off[nb] - S.bwd_level_off[nb - 1], false) +
+                                    nblk((int)S.upper_cols.size());
   UP(dalloc(ctx, &ctx->partials, 2 * (size_t)std::max(1, ctx->resid_blocks)));
   UP(dalloc(ctx, &ctx->st, 1));
 #undef UP

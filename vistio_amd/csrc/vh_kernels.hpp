@@ -395,13 +395,24 @@ __device__ __forceinline__ void hinge_delta(const ElemArgs<T>& ea, int e, T (&sg
 // the stencil weight s[k]: (-1, -1, +1, +1) for dihedral springs,
 // assembly.rs:210, or the hinge's Discrete Shells stencil, :222-236), so the RHS
 // gather is a contiguous segmented sum with no atomics.
+//
+// Block order (chunk_of): the triangle and hinge chunks of 256 elements are
+// merged by position in the vertex order, and each XCD (workgroups are
+// handed to the eight XCDs in turn) takes one contiguous run of that list.
+// The slots of a vertex, written by its triangles and its hinges, are then
+// written by one XCD at about the same time and complete in its L2.
+// chunk_of[b] >= 0: triangle chunk, < 0: hinge chunk ~c, kIdleChunk: none.
+constexpr int kIdleChunk = -0x7fffffff - 1;
+
 template <class T>
-__global__ void __launch_bounds__(256) k_local(int t, int h, int tb, ElemArgs<T> ea, const int* __restrict__ tri_slot,
-                                               const int* __restrict__ hinge_slot, T* __restrict__ cslot,
-                                               const StepDev* st) {
+__global__ void __launch_bounds__(256) k_local(int t, int h, const int* __restrict__ chunk_of, ElemArgs<T> ea,
+                                               const int* __restrict__ tri_slot, const int* __restrict__ hinge_slot,
+                                               T* __restrict__ cslot, const StepDev* st) {
   if (st && st->done) return;
-  if ((int)blockIdx.x < tb) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = chunk_of[blockIdx.x];
+  if (ch == kIdleChunk) return;
+  if (ch >= 0) {
+    const int e = ch * blockDim.x + threadIdx.x;
     if (e >= t) return;
     T w, g[3][2], D[2][3];
     tri_delta(ea, e, w, g, D);
@@ -411,7 +422,7 @@ __global__ void __launch_bounds__(256) k_local(int t, int h, int tb, ElemArgs<T>
       tri_corner(w, g, D, k, o[0], o[1], o[2]);
     }
   } else {
-    const int e = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
+    const int e = (~ch) * blockDim.x + threadIdx.x;
     if (e >= h) return;
     T sg[4], cx, cy, cz;
     hinge_delta(ea, e, sg, cx, cy, cz);
@@ -628,7 +639,26 @@ struct BandArgs {
   // columns outside the deepest band
   const int* upper_cols;
   int nupper;
+  // Task blocks: 0 = block b runs task b; else the tasks are cut into one
+  // contiguous run of xper tasks per XCD (workgroups go to the eight XCDs
+  // in turn), so the tasks of one supernode, which gather the same front,
+  // share an L2; the grid then has 8 xper task blocks, the last few idle.
+  int xper;
 };
+
+constexpr int kXcds = 8;
+
+template <class T>
+__device__ __forceinline__ int task_blocks(const BandArgs<T>& a) {
+  return a.xper ? kXcds * a.xper : a.ntasks;
+}
+// The task of task block b, or -1 for an idle block of the padding.
+template <class T>
+__device__ __forceinline__ int task_of(const BandArgs<T>& a, int b) {
+  if (!a.xper) return b;
+  const int i = (b % kXcds) * a.xper + b / kXcds;
+  return i < a.ntasks ? i : -1;
+}
 
 // Forward task (L y = r): rows [b, e) of supernode P's block
 //   M_P = [L_PP^{-1}; L_IP L_PP^{-1}] (row-major, row i < w has i+1 entries).
@@ -880,8 +910,11 @@ template <class T, bool S>
 __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
                                                   T* __restrict__ F, const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  if ((int)blockIdx.x < a.ntasks) {
-    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
+  const int ntb = task_blocks(a);
+  if ((int)blockIdx.x < ntb) {
+    const int ti = task_of(a, (int)blockIdx.x);
+    if (ti < 0) return;
+    const TaskDev td = a.tasks[ti];  // issued together with the flag load
     if (st && st->done) return;
     with_lanes<4>(td.q1.z, [&](auto GC) {
       fwd_task<decltype(GC)::value, S, T>(td, a, M, y, F, reinterpret_cast<T*>(smem_raw));
@@ -890,7 +923,7 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __rest
   }
   if (!S || (st && st->done)) return;
   // 4 lanes per column: blockDim/4 columns per extra block
-  const int k = ((int)blockIdx.x - a.ntasks) * (blockDim.x / 4) + (int)threadIdx.x / 4;
+  const int k = ((int)blockIdx.x - ntb) * (blockDim.x / 4) + (int)threadIdx.x / 4;
   const int lane = threadIdx.x & 3;
   if (k >= a.nupper) return;  // uniform within each 4-lane group
   const int col = a.upper_cols[k];
@@ -910,15 +943,21 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
                                                   const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double df = 0, nm = 0;
-  if ((int)blockIdx.x < a.ntasks) {
-    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
-    if (st && st->done) return;
-    with_lanes<4>(td.q1.z, [&](auto GC) {
-      bwd_task<decltype(GC)::value, C, T>(td, a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
-    });
+  const int ntb = task_blocks(a);
+  if ((int)blockIdx.x < ntb) {
+    const int ti = task_of(a, (int)blockIdx.x);
+    if (ti >= 0) {  // an idle padding block still writes its (zero) partials
+      const TaskDev td = a.tasks[ti];  // issued together with the flag load
+      if (st && st->done) return;
+      with_lanes<4>(td.q1.z, [&](auto GC) {
+        bwd_task<decltype(GC)::value, C, T>(td, a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+      });
+    } else if (st && st->done) {
+      return;
+    }
   } else {
     if (st && st->done) return;
-    const int k = ((int)blockIdx.x - a.ntasks) * blockDim.x + threadIdx.x;
+    const int k = ((int)blockIdx.x - ntb) * blockDim.x + threadIdx.x;
     if (C && k < a.nupper) {
       const int col = a.upper_cols[k];
       const T* s = d + 3 * (size_t)col;
@@ -984,12 +1023,15 @@ __global__ void __launch_bounds__(256) k_root_band(BandArgs<T> a, CommitArgs c, 
                                                    const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double df = 0, nm = 0;
-  if ((int)blockIdx.x < a.ntasks) {
-    const TaskDev td = a.tasks[blockIdx.x];  // issued together with the flag load
+  const int ti = (int)blockIdx.x < task_blocks(a) ? task_of(a, (int)blockIdx.x) : -1;
+  if (ti >= 0) {  // an idle padding block still writes its (zero) partials
+    const TaskDev td = a.tasks[ti];  // issued together with the flag load
     if (st && st->done) return;
     with_lanes<4>(td.q1.z, [&](auto GC) {
       root_task<decltype(GC)::value, S, C, T>(td, a, c, Sroot, F, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
     });
+  } else if (st && st->done) {
+    return;
   }
   if (C && c.want_resid) {
     __syncthreads();
