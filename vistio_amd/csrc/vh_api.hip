@@ -472,7 +472,10 @@ unsigned int backend_blocks(size_t n) {
 // Blocks of the grid-stride kernels over candidate pairs: enough to fill the
 // chip, independent of the pair capacity (a launch whose work is skipped
 // stays cheap).
-unsigned int pair_blocks(unsigned int cap) { return std::max(1u, std::min((cap + 255) / 256, 4096u)); }
+unsigned int pair_blocks(unsigned int cap) {
+  const unsigned int b = std::max(1u, std::min((cap + 255) / 256, 4096u));
+  return (b + 7) / 8 * 8;  // a multiple of 8: pair_range gives each XCD an equal share of blocks
+}
 
 const int* ipc_done_flag(vh_ctx* c) {
   return reinterpret_cast<const int*>(reinterpret_cast<const char*>(c->ipc.ip) + offsetof(vhc::IpcDev, al_done));

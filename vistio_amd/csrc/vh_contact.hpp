@@ -589,6 +589,21 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict
     if (gbase + q < cap) pairs[gbase + q] = stage[q];
 }
 
+// Grid-stride loop over candidate pairs [0, np), one contiguous share per
+// XCD: workgroups go to the eight XCDs in turn (the grid is a multiple of 8
+// blocks), and the pairs are in Morton order of their first triangle, so each
+// XCD works on one region of the mesh and its L2 keeps that region's corners.
+struct PairRange {
+  unsigned int k, end, stride;
+};
+__device__ __forceinline__ PairRange pair_range(unsigned int np) {
+  constexpr unsigned int kX = 8;
+  const unsigned int x = blockIdx.x % kX, q = blockIdx.x / kX, nbx = gridDim.x / kX;
+  const unsigned int per = (np + kX - 1) / kX;
+  const unsigned int b = min(np, x * per), e = min(np, b + per);
+  return PairRange{b + q * blockDim.x + threadIdx.x, e, nbx * blockDim.x};
+}
+
 // ---- IPC step state --------------------------------------------------------
 
 // "No time of impact": 2.0, above every real step fraction; the minimum is
@@ -684,7 +699,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
   };
   unsigned long long cnt = 0;
   bool viol = false;
-  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+  for (PairRange pr_ = pair_range(np); pr_.k < pr_.end; pr_.k += pr_.stride) {
+    const unsigned int k = pr_.k;
     const int2 pr = pairs[k];
     {
       const CcdTri SA = sph[pr.x], SB = sph[pr.y];
@@ -1026,7 +1042,8 @@ __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pair
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
   double best = 2.0;
-  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+  for (PairRange pr_ = pair_range(np); pr_.k < pr_.end; pr_.k += pr_.stride) {
+    const unsigned int k = pr_.k;
     const int2 pr = pairs[k];
     {
       const CcdTri A = tb_[pr.x], B = tb_[pr.y];
@@ -1283,7 +1300,8 @@ __global__ void __launch_bounds__(256) k_tri_crossings(const int2* __restrict__ 
                                                        const double* __restrict__ x, BvhDev* bd) {
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
   unsigned int hits = 0;
-  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < np; k += gridDim.x * blockDim.x) {
+  for (PairRange pr_ = pair_range(np); pr_.k < pr_.end; pr_.k += pr_.stride) {
+    const unsigned int k = pr_.k;
     const int2 pr = pairs[k];
     const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
     const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
