@@ -558,11 +558,12 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
   hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left,
                      I.range_lo, I.range_hi, I.node, I.leaf, I.orig, I.sorig, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
+  const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
+  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
   size_t sbytes = I.scan_bytes;
   (void)rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t, rocprim::plus<unsigned int>(),
                                 c->stream);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
 }
 

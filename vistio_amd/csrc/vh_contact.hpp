@@ -558,10 +558,24 @@ __device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__
   }
   return cnt;
 }
+// Leaf chunk of block b for the walks: the chunks of 256 Morton-ordered
+// leaves are cut into one contiguous run per XCD (workgroups go to the eight
+// XCDs in turn; the grid is 8 * ceil(chunks / 8) blocks), so neighbouring
+// walks, which visit the same nodes, share an L2. -1: an idle block.
+__device__ __forceinline__ int leaf_chunk(int t) {
+  constexpr int kX = 8;
+  const int nch = (t + (int)blockDim.x - 1) / (int)blockDim.x;
+  const int per = (nch + kX - 1) / kX;
+  const int c = ((int)blockIdx.x % kX) * per + (int)blockIdx.x / kX;
+  return c < nch ? c : -1;
+}
+
 __global__ void __launch_bounds__(256) k_bvh_count(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
                                                    unsigned int* __restrict__ count, const int* skip) {
   if (skip && *skip) return;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ch = leaf_chunk(t);
+  if (ch < 0) return;
+  const int k = ch * blockDim.x + threadIdx.x;
   if (k >= t) return;
   count[k] = t < 2 ? 0u : bvh_walk<false>(k, node, leaf, nullptr, nullptr, 0, 0, nullptr, 0);
 }
@@ -573,7 +587,9 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict
                                                    unsigned int cap, BvhDev* bd, const int* skip) {
   if (skip && *skip) return;  // uniform over the grid
   __shared__ int2 stage[kPairStage];
-  const int k0 = blockIdx.x * blockDim.x;
+  const int ch = leaf_chunk(t);
+  if (ch < 0) return;  // uniform over the block
+  const int k0 = ch * blockDim.x;
   const int kl = min(k0 + (int)blockDim.x, t) - 1;  // the group's last leaf
   const unsigned int gbase = offset[k0];
   const unsigned int gend = offset[kl] + count[kl];
