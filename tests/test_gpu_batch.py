@@ -54,6 +54,33 @@ def test_batch_matches_per_sheet_oracles(precision, tol):
     g.close()
 
 
+@pytest.mark.parametrize("precision,tol", [("f64", 1e-9), ("f32_ir", 1e-6)])
+def test_batch_with_bands_over_4096_tasks(precision, tol):
+    """Eight 128 x 128 sheets: the setup cuts forward band 1 into 5304 tasks
+    and backward bands 2 and 3 into 5080 and 4304, above the 4096 beyond
+    which a band launch gives up the one contiguous run of tasks per XCD
+    and runs task b on block b. Every sheet still follows its own oracle."""
+    import vistio_amd as va
+    from vistio_amd.batch import SheetSpec, build_batch
+    b = build_batch([SheetSpec(128, 0.15, 1e3, 1e-4) if s % 2 == 0 else SheetSpec(128, 0.3, 4e2, 8e-4)
+                     for s in range(8)])
+    cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                          rayleigh_mass_damping=0.01, precision=precision, ir_sweeps=1, ir_final_sweeps=1)
+    g = va.HipPdSolver()
+    g.init_raw(b.mesh, b.topology, cfg, b.pinned, "co_rotational", b.k_membrane, b.k_bending, b.density)
+    info = g.info()
+    print(f"{precision}: {info['n']} verts, {info['supernodes']} supernodes, {info['levels']} bands", flush=True)
+    g.step_resident(DT, 8)
+    x = g.positions()
+    for s in range(len(b.specs)):
+        o = _oracle_sheet(b, s)
+        o.step(DT, 8)
+        err = max_vertex_err(x[:, b.sheet_slice(s)], o.positions())
+        print(f"  sheet {s}: {err:.2e} m", flush=True)
+        assert err <= tol
+    g.close()
+
+
 def test_autotune_picks_a_candidate_that_keeps_parity():
     import vistio_amd as va
     from vistio_amd.autotune import DEFAULTS, KNOBS, autotune
