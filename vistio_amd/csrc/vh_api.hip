@@ -578,7 +578,7 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
     hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
     hipLaunchKernelGGL(vhc::k_tri_spheres, dim3(std::max(1, nblk(c->S.t))), dim3(256), 0, c->stream, c->S.t,
                        c->tri_idx, x, I.ccdtri, skip);
-    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
+    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(vhc::kPairQueue), 0, c->stream,
                        I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.ccdtri, I.gself, I.ip,
                        skip);
   }
