@@ -55,6 +55,9 @@ struct IpcState {
   int2* pairs = nullptr;
   unsigned int cap = 0;
   vhc::CcdTri* ccdtri = nullptr;  // per-triangle bounds of the CCD pair cull
+  int2* ccd_surv = nullptr;        // pairs that survive the CCD cull, one region of ccd_reg per block
+  unsigned int ccd_reg = 0;
+  unsigned int* ccd_cnt = nullptr;  // survivors per block
   vhc::BvhDev* bd = nullptr;
   vhc::IpcDev* ip = nullptr;
   void* sort_tmp = nullptr;
@@ -472,9 +475,17 @@ unsigned int backend_blocks(size_t n) {
 // Blocks of the grid-stride kernels over candidate pairs: enough to fill the
 // chip, independent of the pair capacity (a launch whose work is skipped
 // stays cheap).
+constexpr unsigned int kMaxPairBlocks = 4096;
 unsigned int pair_blocks(unsigned int cap) {
-  const unsigned int b = std::max(1u, std::min((cap + 255) / 256, 4096u));
+  const unsigned int b = std::max(1u, std::min((cap + 255) / 256, kMaxPairBlocks));
   return (b + 7) / 8 * 8;  // a multiple of 8: pair_range gives each XCD an equal share of blocks
+}
+
+// The most pairs one block of a pair_blocks(cap) grid visits (pair_range
+// over at most cap pairs): the survivor region of a k_ccd_cull block.
+unsigned int ccd_region(unsigned int cap) {
+  const unsigned int span = pair_blocks(cap) / 8 * 256, per = (cap + 7) / 8;
+  return std::max(1u, (per + span - 1) / span) * 256;
 }
 
 const int* ipc_done_flag(vh_ctx* c) {
@@ -511,8 +522,33 @@ hipError_t grow_pairs(vh_ctx* c, unsigned int need) {
   c->allocs.push_back(q);
   I.pairs = static_cast<int2*>(q);
   I.cap = (unsigned int)cap;
+  it = std::find(c->allocs.begin(), c->allocs.end(), static_cast<void*>(I.ccd_surv));
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  e = hipFree(I.ccd_surv);
+  I.ccd_surv = nullptr;
+  if (e != hipSuccess) return e;
+  const unsigned int reg = ccd_region(I.cap);
+  q = nullptr;
+  e = hipMalloc(&q, (size_t)pair_blocks(I.cap) * reg * sizeof(int2));
+  if (e != hipSuccess) return e;
+  c->allocs.push_back(q);
+  I.ccd_surv = static_cast<int2*>(q);
+  I.ccd_reg = reg;
   hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
   return hipGetLastError();
+}
+
+// The self part of the CCD over the candidate pairs of the last detection,
+// from x to x + dl: the cull, then the tests of its survivors.
+void enqueue_ccd_pairs(vh_ctx* c, const double* x, const double* dl) {
+  IpcState& I = c->ipc;
+  const int t = (int)c->S.t;
+  hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(t))), dim3(256), 0, c->stream, t, c->tri_idx, x, dl,
+                     I.ccdtri, I.ip);
+  hipLaunchKernelGGL(vhc::k_ccd_cull, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.pairs, &I.bd->npairs,
+                     I.cap, I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, I.ip);
+  hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.ccd_surv, I.ccd_reg,
+                     I.ccd_cnt, c->tri_idx, x, dl, I.ip);
 }
 
 // After a candidate-pair query has been enqueued: reads the pair count (one
@@ -619,10 +655,7 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
       hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
                          I.partials, I.ip);
       if (I.sc.self_collision) {
-        hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(S.t))), dim3(256), 0, c->stream, S.t, c->tri_idx, c->x,
-                           dl, I.ccdtri, I.ip);
-        hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream,
-                           I.pairs, &I.bd->npairs, I.cap, c->tri_idx, c->x, dl, I.ccdtri, I.ip);
+        enqueue_ccd_pairs(c, c->x, dl);
       }
       if (I.sc.has_ground || I.sc.has_sphere)
         hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
@@ -1358,6 +1391,9 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.sorig, (size_t)tt));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));
   UP(dalloc(ctx, &I.ccdtri, (size_t)tt));
+  I.ccd_reg = ccd_region(I.cap);
+  UP(dalloc(ctx, &I.ccd_surv, (size_t)pair_blocks(I.cap) * I.ccd_reg));
+  UP(dalloc(ctx, &I.ccd_cnt, (size_t)kMaxPairBlocks));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
   VH_HIP(hipHostMalloc((void**)&I.h_done, 2 * sizeof(int), hipHostMallocDefault));
   *I.h_done = 0;
@@ -1492,10 +1528,7 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
     hipLaunchKernelGGL(k_ipc_clear, dim3(1), dim3(64), 0, ctx->stream, I.ip);
     if (I.sc.self_collision) {
       enqueue_pairs(ctx, I.scratch_a, (float)I.sc.thickness, nullptr);
-      hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(S.t))), dim3(256), 0, ctx->stream, S.t, ctx->tri_idx,
-                         I.scratch_a, I.scratch_b, I.ccdtri, I.ip);
-      hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, ctx->stream, I.pairs,
-                         &I.bd->npairs, I.cap, ctx->tri_idx, I.scratch_a, I.scratch_b, I.ccdtri, I.ip);
+      enqueue_ccd_pairs(ctx, I.scratch_a, I.scratch_b);
     }
     if (I.sc.has_ground || I.sc.has_sphere)
       hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, ctx->stream, n, I.scratch_a,

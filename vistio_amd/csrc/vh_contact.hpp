@@ -1106,28 +1106,67 @@ __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __rest
 // through quadratic_roots to the linear case and finds no root for any test
 // of the pair (the computed coefficients differ from the bounded ones by a
 // few ulps), so the pair cannot report a hit and is skipped.
-// The survivors are tested where they lie, not queued (culled_pairs): at
-// the two waves per SIMD that the tests' registers allow, the queue's
-// barriers cost more than the idle lanes (106.6 -> 137.4 us at 4.4 M pairs).
-__global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ pairs, const unsigned int* npairs_p,
-                                                   unsigned int cap, const int* __restrict__ tri,
-                                                   const double* __restrict__ x, const double* __restrict__ dl,
-                                                   const CcdTri* __restrict__ tb_, IpcDev* ip) {
+//
+// k_ccd_cull applies the cull; block b writes the survivors of its share of
+// the pairs (pair_range) to its own region surv[b reg, (b + 1) reg) and their
+// count to scnt[b]. reg bounds the pairs one block visits (ccd_region on the
+// host, from the same grid and capacity). k_ccd_pairs then runs the tests on
+// the survivors only, block b on region b. The cull runs at full occupancy;
+// the 205-VGPR tests run at two waves per SIMD with every lane on a pair
+// that needs them. (Testing the survivors where they lie left most lanes of
+// a wave idle; queueing them in LDS inside the test kernel cost more in
+// barriers at two waves per SIMD: 106.6 -> 137.4 us at 4.4 M pairs.)
+__global__ void __launch_bounds__(256) k_ccd_cull(const int2* __restrict__ pairs, const unsigned int* npairs_p,
+                                                  unsigned int cap, const CcdTri* __restrict__ tb_,
+                                                  int2* __restrict__ surv, unsigned int reg,
+                                                  unsigned int* __restrict__ scnt, const IpcDev* ip) {
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
-  double best = 2.0;
-  for (PairRange pr_ = pair_range(np); pr_.k < pr_.end; pr_.k += pr_.stride) {
-    const unsigned int k = pr_.k;
-    const int2 pr = pairs[k];
-    {
+  __shared__ unsigned int wave_kept[4];
+  const PairRange rg = pair_range(np);
+  const unsigned int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int2* out = surv + (size_t)blockIdx.x * reg;
+  unsigned int n = 0;  // survivors so far; the same in every thread
+  for (unsigned int base = rg.k - threadIdx.x; base < rg.end; base += rg.stride) {
+    const unsigned int k = base + threadIdx.x;
+    int2 pr = make_int2(0, 0);
+    bool keep = false;
+    if (k < rg.end) {
+      pr = pairs[k];
       const CcdTri A = tb_[pr.x], B = tb_[pr.y];
       const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
                    gz = (double)A.c[2] - (double)B.c[2];
       const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
       const double W = (double)A.m + (double)B.m;
       constexpr double kHalfEps = 0.5e-8;
-      if (W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps) continue;
+      keep = !(W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps);
     }
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wave_kept[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    unsigned int at = n, total = 0;
+#pragma unroll
+    for (unsigned int w = 0; w < 4; ++w) {
+      at += w < wave ? wave_kept[w] : 0u;
+      total += wave_kept[w];
+    }
+    if (keep) out[at + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = pr;
+    n += total;
+    __syncthreads();  // wave_kept is rewritten by the next round
+  }
+  if (threadIdx.x == 0) scnt[blockIdx.x] = n;
+}
+
+__global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ surv, unsigned int reg,
+                                                   const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
+                                                   const double* __restrict__ x, const double* __restrict__ dl,
+                                                   IpcDev* ip) {
+  if (ip->al_done || ip->pd_done) return;
+  const unsigned int n = scnt[blockIdx.x];
+  const int2* in = surv + (size_t)blockIdx.x * reg;
+  double best = 2.0;
+  for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) {
+    const int2 pr = in[k];
     // The six corners (A then B) with their start and end positions, loaded
     // once per pair: measured faster than reloading them per test, despite
     // the register cost of the unrolled tests.
