@@ -17,6 +17,7 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -224,6 +225,12 @@ int band_task_blocks(int ntasks, bool first) {
   return x ? vhk::kXcds * x : ntasks;
 }
 
+// FP32 forward bands of more tasks than this run the eight-wave kernel
+// (k_fwd_band W8). On 64 sheets in one system it took forward bands 1-5
+// from 377 to 333 us; on one 256x256 sheet (bands of at most 2745 tasks)
+// the same kernel everywhere cost 3.6 % of the step.
+constexpr int kWideBandTasks = 4096;
+
 // Records an event after each launch when profiling (ev == nullptr: off).
 struct Marks {
   hipEvent_t* ev = nullptr;
@@ -258,9 +265,18 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
       if (first)
         hipLaunchKernelGGL((vhk::k_fwd_band<T, true>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
                            (T*)c->y, (T*)c->F, st);
-      else
-        hipLaunchKernelGGL((vhk::k_fwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
-                           (T*)c->y, (T*)c->F, st);
+      else {
+        bool wide = false;
+        if constexpr (std::is_same<T, float>::value) {
+          wide = e - b > kWideBandTasks;
+          if (wide)
+            hipLaunchKernelGGL(vhk::k_fwd_band_w8, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
+                               (T*)c->y, (T*)c->F, st);
+        }
+        if (!wide)
+          hipLaunchKernelGGL((vhk::k_fwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
+                             (T*)c->y, (T*)c->F, st);
+      }
     }
     mk.mark(c->stream);
   }

@@ -907,8 +907,8 @@ __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArg
 // in the first band of a sweep that starts from the slots, the remaining
 // blocks compute (and store) r for the columns outside the deepest band.
 template <class T, bool S>
-__global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
-                                                  T* __restrict__ F, const StepDev* st) {
+__device__ __forceinline__ void fwd_band(const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
+                                         T* __restrict__ F, const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int ntb = task_blocks(a);
   if ((int)blockIdx.x < ntb) {
@@ -932,6 +932,22 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __rest
     double* o = a.r + 3 * (size_t)col;
     o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
   }
+}
+
+template <class T, bool S>
+__global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
+                                                  T* __restrict__ F, const StepDev* st) {
+  fwd_band<T, S>(a, M, y, F, st);
+}
+
+// The FP32 forward band compiled for eight waves per SIMD (64 VGPRs, a few
+// bytes of spill) instead of the compiler's own six. The host picks it for
+// bands of many tasks, where occupancy pays, and not for small bands, where
+// the spills' latency costs more.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+k_fwd_band_w8(BandArgs<float> a, const float* __restrict__ M, float* __restrict__ y, float* __restrict__ F,
+              const StepDev* st) {
+  fwd_band<float, false>(a, M, y, F, st);
 }
 
 // One band of the backward sweep per launch. In the last band of an FP64
