@@ -56,7 +56,7 @@ struct IpcState {
   int2* pairs = nullptr;
   unsigned int cap = 0;
   vhc::CcdTri* ccdtri = nullptr;  // per-triangle bounds of the CCD pair cull
-  int2* ccd_surv = nullptr;        // pairs that survive the CCD cull, one region of ccd_reg per block
+  int2* ccd_surv = nullptr;        // pairs that survive the self-contact or CCD cull, one region of ccd_reg per block
   unsigned int ccd_reg = 0;
   unsigned int* ccd_cnt = nullptr;  // survivors per block
   vhc::BvhDev* bd = nullptr;
@@ -630,9 +630,11 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
     hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
     hipLaunchKernelGGL(vhc::k_tri_spheres, dim3(std::max(1, nblk(c->S.t))), dim3(256), 0, c->stream, c->S.t,
                        c->tri_idx, x, I.ccdtri, skip);
-    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(vhc::kPairQueue), 0, c->stream,
-                       I.pairs, &I.bd->npairs, I.cap, c->tri_idx, x, I.xptr, I.xidx, sc, I.ccdtri, I.gself, I.ip,
-                       skip);
+    // the survivor regions are shared with the CCD, which runs after the detection
+    hipLaunchKernelGGL(vhc::k_self_cull, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.pairs, &I.bd->npairs,
+                       I.cap, sc, I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, skip);
+    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, &I.bd->npairs, I.cap,
+                       I.ccd_surv, I.ccd_reg, I.ccd_cnt, c->tri_idx, x, I.xptr, I.xidx, sc, I.gself, I.ip, skip);
   }
   hipLaunchKernelGGL(vhc::k_vertex_contacts, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, sc, I.gself, I.grad, I.nrm,
                      I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, hess, skip);

@@ -621,57 +621,39 @@ __device__ __forceinline__ PairRange pair_range(unsigned int np) {
 }
 
 // The candidate pairs of this block's share (pair_range) that pass `keep`,
-// each handed to `tests` by one thread. Each round culls the block's next
-// kPairQueue pairs and appends the survivors to an LDS queue; once the queue
-// holds kPairQueue pairs, every thread tests one of them, and the rest are
-// tested when the share is done. A wave then runs the costly tests with all
-// its lanes busy instead of with the few whose pair survived the cull.
-// Launch with blockDim.x == kPairQueue.
-constexpr unsigned int kPairQueue = 256;
-template <class Keep, class Tests>
-__device__ __forceinline__ void culled_pairs(const int2* __restrict__ pairs, unsigned int np, Keep keep_fn,
-                                             Tests tests) {
-  __shared__ int2 queue[2 * kPairQueue];
-  __shared__ unsigned int wave_kept[kPairQueue / 64];
+// written in order to out[0, n) (compacted with a wave ballot; no global
+// counter); n, the same in every thread, is returned. out must hold the
+// most pairs one block visits (ccd_region on the host). Launch with 256
+// threads per block.
+template <class Keep>
+__device__ __forceinline__ unsigned int cull_to_region(const int2* __restrict__ pairs, unsigned int np, Keep keep_fn,
+                                                       int2* __restrict__ out) {
+  __shared__ unsigned int wave_kept[4];
   const PairRange rg = pair_range(np);
   const unsigned int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  unsigned int qn = 0;                      // queued pairs, < kPairQueue between rounds; the same in every thread
-  unsigned int base = rg.k - threadIdx.x;  // the block's next round; the same in every thread
-  for (;;) {
-    const bool more = base < rg.end;
-    if (more) {
-      const unsigned int k = base + threadIdx.x;
-      base += rg.stride;
-      int2 pr = make_int2(0, 0);
-      bool keep = false;
-      if (k < rg.end) {
-        pr = pairs[k];
-        keep = keep_fn(pr);
-      }
-      const unsigned long long m = __ballot(keep);
-      if (lane == 0) wave_kept[wave] = (unsigned int)__popcll(m);
-      __syncthreads();
-      unsigned int at = qn, total = 0;
+  unsigned int n = 0;
+  for (unsigned int base = rg.k - threadIdx.x; base < rg.end; base += rg.stride) {
+    const unsigned int k = base + threadIdx.x;
+    int2 pr = make_int2(0, 0);
+    bool keep = false;
+    if (k < rg.end) {
+      pr = pairs[k];
+      keep = keep_fn(pr);
+    }
+    const unsigned long long m = __ballot(keep);
+    if (lane == 0) wave_kept[wave] = (unsigned int)__popcll(m);
+    __syncthreads();
+    unsigned int at = n, total = 0;
 #pragma unroll
-      for (unsigned int w = 0; w < kPairQueue / 64; ++w) {
-        at += w < wave ? wave_kept[w] : 0u;
-        total += wave_kept[w];
-      }
-      if (keep) queue[at + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = pr;
-      qn += total;  // <= 2 kPairQueue - 1
-      __syncthreads();
+    for (unsigned int w = 0; w < 4; ++w) {
+      at += w < wave ? wave_kept[w] : 0u;
+      total += wave_kept[w];
     }
-    // a full queue, or the rest once the share is culled
-    if (qn >= kPairQueue || (!more && qn > 0)) {
-      const unsigned int take = qn < kPairQueue ? qn : kPairQueue;
-      if (threadIdx.x < take) tests(queue[threadIdx.x]);
-      __syncthreads();  // every thread has read its entry before the move
-      if (threadIdx.x + take < qn) queue[threadIdx.x] = queue[take + threadIdx.x];
-      qn -= take;
-      __syncthreads();
-    }
-    if (!more && qn == 0) break;
+    if (keep) out[at + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = pr;
+    n += total;
+    __syncthreads();  // wave_kept is rewritten by the next round
   }
+  return n;
 }
 
 // ---- IPC step state --------------------------------------------------------
@@ -750,19 +732,39 @@ __device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
 // their bounding spheres (CcdTri centre and radius at x, k_tri_spheres). When
 // that gap exceeds sqrt(dh) with a relative 1e-6 to spare, every distance the
 // tests compute is beyond the barrier's reach, so the pair adds nothing and
-// is skipped before its exclusion lookups. The survivors are queued in LDS
-// (culled_pairs) so that the tests run with full waves: 1002 -> 512 us at
-// 4.4 M candidate pairs.
-// Four waves per SIMD (128 VGPRs, 8 bytes of spill) measured 14 % faster than
-// the compiler's own 130 VGPRs at three; five or six spill 100+ bytes a lane.
-__global__ void __launch_bounds__(kPairQueue) __attribute__((amdgpu_waves_per_eu(4))) k_self_contacts(const int2* __restrict__ pairs, const unsigned int* npairs_p,
-                                                       unsigned int cap, const int* __restrict__ tri,
-                                                       const double* __restrict__ x, const int* __restrict__ xptr,
-                                                       const int* __restrict__ xidx, Scene sc,
-                                                       const CcdTri* __restrict__ sph, double* __restrict__ gself,
-                                                       IpcDev* ip, const int* skip) {
+// is skipped before its exclusion lookups.
+//
+// k_self_cull applies the cull at full occupancy and writes each block's
+// survivors to its region of surv (as k_ccd_cull); k_self_contacts runs the
+// tests on block b's survivors, every lane on a pair that needs them. Tested
+// where they lay, most lanes of a wave sat idle (1002 us at 4.4 M candidate
+// pairs); queued in LDS inside the test kernel, 512 us.
+__global__ void __launch_bounds__(256) k_self_cull(const int2* __restrict__ pairs, const unsigned int* npairs_p,
+                                                   unsigned int cap, Scene sc, const CcdTri* __restrict__ sph,
+                                                   int2* __restrict__ surv, unsigned int reg,
+                                                   unsigned int* __restrict__ scnt, const int* skip) {
   if (skip && *skip) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
+  const double dh = sc.dh;
+  const unsigned int n = cull_to_region(pairs, np, [&](const int2 pr) {
+    const CcdTri SA = sph[pr.x], SB = sph[pr.y];
+    const double gx = (double)SA.c[0] - (double)SB.c[0], gy = (double)SA.c[1] - (double)SB.c[1],
+                 gz = (double)SA.c[2] - (double)SB.c[2];
+    const double gap = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 - 1e-12) - (double)SA.r - (double)SB.r;
+    return !(gap > 0.0 && gap * gap > dh * (1.0 + 1e-6));
+  }, surv + (size_t)blockIdx.x * reg);
+  if (threadIdx.x == 0) scnt[blockIdx.x] = n;
+}
+
+// Four waves per SIMD (128 VGPRs, 8 bytes of spill) measured 14 % faster than
+// the compiler's own 130 VGPRs at three; five or six spill 100+ bytes a lane.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_self_contacts(const unsigned int* npairs_p,
+                                                       unsigned int cap, const int2* __restrict__ surv, unsigned int reg,
+                                                       const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
+                                                       const double* __restrict__ x, const int* __restrict__ xptr,
+                                                       const int* __restrict__ xidx, Scene sc,
+                                                       double* __restrict__ gself, IpcDev* ip, const int* skip) {
+  if (skip && *skip) return;
   if (blockIdx.x == 0 && threadIdx.x == 0 && *npairs_p > cap) ip->overflow = 1;
   const double dh = sc.dh;
   auto bgrad = [&](double d2) {
@@ -856,13 +858,9 @@ __global__ void __launch_bounds__(kPairQueue) __attribute__((amdgpu_waves_per_eu
       }
     }
   };
-  culled_pairs(pairs, np, [&](const int2 pr) {
-    const CcdTri SA = sph[pr.x], SB = sph[pr.y];
-    const double gx = (double)SA.c[0] - (double)SB.c[0], gy = (double)SA.c[1] - (double)SB.c[1],
-                 gz = (double)SA.c[2] - (double)SB.c[2];
-    const double gap = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 - 1e-12) - (double)SA.r - (double)SB.r;
-    return !(gap > 0.0 && gap * gap > dh * (1.0 + 1e-6));
-  }, pair_tests);
+  const unsigned int n = scnt[blockIdx.x];
+  const int2* in = surv + (size_t)blockIdx.x * reg;
+  for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) pair_tests(in[k]);
   // one counter update per wave
   for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
   const bool any_viol = __ballot(viol) != 0;
@@ -1122,38 +1120,15 @@ __global__ void __launch_bounds__(256) k_ccd_cull(const int2* __restrict__ pairs
                                                   unsigned int* __restrict__ scnt, const IpcDev* ip) {
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
-  __shared__ unsigned int wave_kept[4];
-  const PairRange rg = pair_range(np);
-  const unsigned int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int2* out = surv + (size_t)blockIdx.x * reg;
-  unsigned int n = 0;  // survivors so far; the same in every thread
-  for (unsigned int base = rg.k - threadIdx.x; base < rg.end; base += rg.stride) {
-    const unsigned int k = base + threadIdx.x;
-    int2 pr = make_int2(0, 0);
-    bool keep = false;
-    if (k < rg.end) {
-      pr = pairs[k];
-      const CcdTri A = tb_[pr.x], B = tb_[pr.y];
-      const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
-                   gz = (double)A.c[2] - (double)B.c[2];
-      const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
-      const double W = (double)A.m + (double)B.m;
-      constexpr double kHalfEps = 0.5e-8;
-      keep = !(W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps);
-    }
-    const unsigned long long m = __ballot(keep);
-    if (lane == 0) wave_kept[wave] = (unsigned int)__popcll(m);
-    __syncthreads();
-    unsigned int at = n, total = 0;
-#pragma unroll
-    for (unsigned int w = 0; w < 4; ++w) {
-      at += w < wave ? wave_kept[w] : 0u;
-      total += wave_kept[w];
-    }
-    if (keep) out[at + (unsigned int)__popcll(m & ((1ull << lane) - 1ull))] = pr;
-    n += total;
-    __syncthreads();  // wave_kept is rewritten by the next round
-  }
+  const unsigned int n = cull_to_region(pairs, np, [&](const int2 pr) {
+    const CcdTri A = tb_[pr.x], B = tb_[pr.y];
+    const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
+                 gz = (double)A.c[2] - (double)B.c[2];
+    const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
+    const double W = (double)A.m + (double)B.m;
+    constexpr double kHalfEps = 0.5e-8;
+    return !(W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps);
+  }, surv + (size_t)blockIdx.x * reg);
   if (threadIdx.x == 0) scnt[blockIdx.x] = n;
 }
 
