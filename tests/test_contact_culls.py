@@ -1,7 +1,7 @@
 """The pair culls of the GPU contact kernels are exact: a culled pair can
 never contribute. The bounds are restated here as the kernels compute them
-(vh_contact.hpp: k_ccd_tri_bounds + k_ccd_pairs, k_tri_spheres +
-k_self_contacts) and checked against the oracle's own vertex-triangle CCD and
+(vh_contact.hpp: k_ccd_tri_bounds + k_ccd_cull, k_tri_spheres +
+k_self_cull) and checked against the oracle's own vertex-triangle CCD and
 distance functions on random triangle pairs at many scales. A bound made too
 loose on purpose (here, on the CPU) is caught by the same checks."""
 import numpy as np
