@@ -542,11 +542,13 @@ hipError_t grow_pairs(vh_ctx* c, unsigned int need) {
   if (it != c->allocs.end()) c->allocs.erase(it);
   e = hipFree(I.ccd_surv);
   I.ccd_surv = nullptr;
-  if (e != hipSuccess) return e;
   const unsigned int reg = ccd_region(I.cap);
   q = nullptr;
-  e = hipMalloc(&q, (size_t)pair_blocks(I.cap) * reg * sizeof(int2));
-  if (e != hipSuccess) return e;
+  if (e == hipSuccess) e = hipMalloc(&q, (size_t)pair_blocks(I.cap) * reg * sizeof(int2));
+  if (e != hipSuccess) {
+    I.cap = 0;  // with no survivor regions, no pair kernel may visit a pair
+    return e;
+  }
   c->allocs.push_back(q);
   I.ccd_surv = static_cast<int2*>(q);
   I.ccd_reg = reg;
