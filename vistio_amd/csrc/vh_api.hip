@@ -225,10 +225,11 @@ int band_task_blocks(int ntasks, bool first) {
   return x ? vhk::kXcds * x : ntasks;
 }
 
-// FP32 forward bands of more tasks than this run the eight-wave kernel
-// (k_fwd_band W8). On 64 sheets in one system it took forward bands 1-5
-// from 377 to 333 us; on one 256x256 sheet (bands of at most 2745 tasks)
-// the same kernel everywhere cost 3.6 % of the step.
+// Forward bands of more tasks than this run the kernels compiled for more
+// waves per SIMD (k_fwd_band_w8 for FP32, k_fwd_band_d5 for FP64). On 64
+// sheets in one system the FP32 one took forward bands 1-5 from 377 to
+// 333 us; on one 256x256 sheet (bands of at most 2745 tasks) the same
+// kernel everywhere cost 3.6 % of the step.
 constexpr int kWideBandTasks = 4096;
 
 // Records an event after each launch when profiling (ev == nullptr: off).
@@ -267,10 +268,14 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
                            (T*)c->y, (T*)c->F, st);
       else {
         bool wide = false;
+        wide = e - b > kWideBandTasks;
         if constexpr (std::is_same<T, float>::value) {
-          wide = e - b > kWideBandTasks;
           if (wide)
             hipLaunchKernelGGL(vhk::k_fwd_band_w8, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
+                               (T*)c->y, (T*)c->F, st);
+        } else {
+          if (wide)
+            hipLaunchKernelGGL(vhk::k_fwd_band_d5, dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M,
                                (T*)c->y, (T*)c->F, st);
         }
         if (!wide)

@@ -950,6 +950,14 @@ k_fwd_band_w8(BandArgs<float> a, const float* __restrict__ M, float* __restrict_
   fwd_band<float, false>(a, M, y, F, st);
 }
 
+// The FP64 forward band for the same wide bands, at five waves per SIMD
+// instead of four.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+k_fwd_band_d5(BandArgs<double> a, const double* __restrict__ M, double* __restrict__ y, double* __restrict__ F,
+              const StepDev* st) {
+  fwd_band<double, false>(a, M, y, F, st);
+}
+
 // One band of the backward sweep per launch. In the last band of an FP64
 // solve the tasks also commit their columns and the extra blocks commit the
 // columns outside the deepest band (solved by earlier launches).
