@@ -225,8 +225,9 @@ int band_task_blocks(int ntasks, bool first) {
   return x ? vhk::kXcds * x : ntasks;
 }
 
-// Forward bands of more tasks than this run the kernels compiled for more
-// waves per SIMD (k_fwd_band_w8 for FP32, k_fwd_band_d5 for FP64). On 64
+// Bands of more tasks than this run the kernels compiled for more waves
+// per SIMD (k_fwd_band_w8 for FP32; k_fwd_band_d5, k_bwd_band_d5 for FP64,
+// not the committing backward band). On 64
 // sheets in one system the FP32 one took forward bands 1-5 from 377 to
 // 333 us; on one 256x256 sheet (bands of at most 2745 tasks) the same
 // kernel everywhere cost 3.6 % of the step.
@@ -322,9 +323,18 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
       if (cm)
         hipLaunchKernelGGL((vhk::k_bwd_band<T, true>), dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
                            (const T*)c->y, (T*)c->V, (T*)c->d, st);
-      else
-        hipLaunchKernelGGL((vhk::k_bwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
-                           (const T*)c->y, (T*)c->V, (T*)c->d, st);
+      else {
+        bool wide = false;
+        if constexpr (std::is_same<T, double>::value) {
+          wide = e - b > kWideBandTasks;
+          if (wide)
+            hipLaunchKernelGGL(vhk::k_bwd_band_d5, dim3(grid), dim3(256), lds, c->stream, a, cc, (const T*)c->Mt,
+                               (const T*)c->y, (T*)c->V, (T*)c->d, st);
+        }
+        if (!wide)
+          hipLaunchKernelGGL((vhk::k_bwd_band<T, false>), dim3(grid), dim3(256), lds, c->stream, a, cc,
+                             (const T*)c->Mt, (const T*)c->y, (T*)c->V, (T*)c->d, st);
+      }
     }
     mk.mark(c->stream);
   }

@@ -962,9 +962,9 @@ k_fwd_band_d5(BandArgs<double> a, const double* __restrict__ M, double* __restri
 // solve the tasks also commit their columns and the extra blocks commit the
 // columns outside the deepest band (solved by earlier launches).
 template <class T, bool C>
-__global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Mt,
-                                                  const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
-                                                  const StepDev* st) {
+__device__ __forceinline__ void bwd_band(const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
+                                         const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
+                                         const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double df = 0, nm = 0;
   const int ntb = task_blocks(a);
@@ -992,6 +992,21 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
     __syncthreads();
     block_partials(df, nm, c.partials);
   }
+}
+
+template <class T, bool C>
+__global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, const T* __restrict__ Mt,
+                                                  const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
+                                                  const StepDev* st) {
+  bwd_band<T, C>(a, c, Mt, y, V, d, st);
+}
+
+// The FP64 backward band (not the committing one) at five waves per SIMD,
+// for bands of many tasks, as k_fwd_band_d5.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
+k_bwd_band_d5(BandArgs<double> a, CommitArgs c, const double* __restrict__ Mt, const double* __restrict__ y,
+              double* __restrict__ V, double* __restrict__ d, const StepDev* st) {
+  bwd_band<double, false>(a, c, Mt, y, V, d, st);
 }
 
 // Root task: a root supernode R has no off-diagonal rows, so its forward
