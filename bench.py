@@ -44,12 +44,8 @@ def cpu_model() -> str:
 
 
 def make_sheet(cells):
-    import vistio_amd as va
-    mesh = va.quad_grid(cells, cells, 1.0, 1.0)
-    topo = va.Topology.build(mesh)
-    pinned = np.zeros(mesh.vertex_count(), dtype=np.uint8)
-    pinned[: cells + 1] = 1
-    return mesh, topo, pinned
+    from vistio_amd.scenes import hanging_sheet
+    return hanging_sheet(cells)
 
 
 def make_solver(mesh, topo, pinned, precision, ir_sweeps, ir_final_sweeps):
@@ -169,61 +165,13 @@ def cpu_baseline(cells, sample_steps, dt, threads=1):
                        f"{wall:.1f} s; CPU: {cpu_model()} ({os.cpu_count()} logical CPUs on the host)")
 
 
-def contact_scene(name):
-    """The two contact workloads: the sphere drape (quad_grid(512, 512, 1.5 m,
-    1.5 m) laid flat at y = 0.8 m over an analytic sphere of radius 0.5 m at
-    the origin, ground at y = -0.3 m, d_hat 1e-3, kappa 1e4, AL <= 5, 180
-    steps) and the self-fold ribbon (quad_grid(64, 640, 0.2 m, 2.0 m) turned
-    45 degrees about its short axis, lowest point 0.5 m above the ground at
-    y = 0, self-contact with 1-ring exclusion, d_hat 5e-4, AL <= 8, 300
-    steps). Both have self-collision on and compliant contact off.
-
-    d_hat is a distance here: the barrier acts on squared distances, so it
-    gets barrier_d_hat = d_hat^2, and the broad-phase margin / self-contact
-    thickness is 1.5 d_hat (the reference runner's thickness = 1.5 *
-    barrier_d_hat, read in metres). Read as m^2 instead, 1e-3 and 5e-4 would
-    switch the self-contact barrier on between vertices two rings apart on
-    the flat sheet (cells of 2.9 and 3.1 mm vs 3.2 and 2.2 cm activation
-    radii)."""
-    import vistio_amd as va
-    if name == "sphere_drape":
-        cells, w = 512, 1.5
-        m = va.quad_grid(cells, cells, w, w)
-        px = m.pos_x.astype(np.float32)
-        pz = m.pos_y.astype(np.float32)
-        py = np.full_like(px, 0.8)
-        dh, al, steps = 1e-3, 5, 180
-        ipc = va.IpcConfig(ground_height=-0.3, sphere=((0.0, 0.0, 0.0), 0.5), self_collision=True,
-                           barrier_d_hat=dh * dh, barrier_kappa=1e4, thickness=1.5 * dh, al_max_iterations=al,
-                           compliant_contact=False)
-        desc = (f"sphere_drape {cells}x{cells} cells, 1.5 m sheet flat at y 0.8 m over a sphere r 0.5 m, "
-                f"ground -0.3 m, self-collision on (1-ring exclusion), d_hat 1e-3 m (barrier_d_hat 1e-6 m^2, "
-                f"thickness 1.5e-3 m), kappa 1e4, AL <= 5")
-    elif name == "self_fold_ribbon":
-        m = va.quad_grid(64, 640, 0.2, 2.0)
-        c = np.float32(np.sqrt(0.5))
-        px = m.pos_x.astype(np.float32)
-        y = m.pos_y.astype(np.float64) * c
-        pz = (m.pos_y.astype(np.float64) * c).astype(np.float32)
-        py = (y + 0.5 - y.min()).astype(np.float32)
-        dh, al, steps = 5e-4, 8, 300
-        ipc = va.IpcConfig(ground_height=0.0, self_collision=True, exclusion_depth=1, barrier_d_hat=dh * dh,
-                           barrier_kappa=1e4, thickness=1.5 * dh, al_max_iterations=al, compliant_contact=False)
-        desc = ("self_fold_ribbon 64x640 cells, 0.2 m x 2.0 m at 45 degrees, lowest point 0.5 m above the "
-                "ground y 0, self-contact (1-ring exclusion), d_hat 5e-4 m (barrier_d_hat 2.5e-7 m^2, "
-                "thickness 7.5e-4 m), kappa 1e4, AL <= 8")
-    else:
-        raise SystemExit(f"unknown workload {name}")
-    mesh = va.TriangleMesh(px, py, pz, m.indices)
-    return mesh, ipc, steps, desc
-
-
 def run_contact(args):
     """Steps/s of the IPC step (state resident in HBM). Every step is one
     call that ends with a device synchronisation; the intersection check
     between steps is outside the timed intervals."""
     import vistio_amd as va
     dt = 1.0 / 60.0
+    from vistio_amd.scenes import contact_scene
     mesh, ipc, steps, desc = contact_scene(args.workload)
     steps = args.steps if args.steps_given else steps
     topo = va.Topology.build(mesh)

@@ -257,6 +257,15 @@ typedef struct {
   int32_t adaptive_contact_damping;
   double contact_damping_max, contact_velocity_threshold;
   uint32_t max_pairs;             /* initial candidate-pair buffer (grows on demand); 0 = 32 per triangle */
+  /* 1: the caller's handler ignores set_d_hat (IpcCollisionHandler's default
+   * no-op, pd_solver.rs:488, as the handler of bench_tests.rs:181-199):
+   * detection keeps d_hat while the CFL clamp uses the compliant-scaled one.
+   * 0: detection follows set_d_hat, as the runner's handler (runner.rs:34-36). */
+  int32_t detect_keeps_d_hat;
+  /* 1: the CCD step also runs edge_edge_ccd (crates/vistio-contact/src/ccd.rs:138-229)
+   * on the nine edge pairs (no shared vertex) of every candidate triangle
+   * pair; 0: vertex-triangle only, as compute_max_step_size (ccd.rs:238-292). */
+  int32_t ccd_edge_edge;
 } vh_ipc_params;
 
 /* Diagnostics of the last IPC step or detection. */

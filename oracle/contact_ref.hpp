@@ -476,6 +476,7 @@ struct IpcScene {
   double radius = 0;
   bool self_collision = false;
   double thickness = 0;  // broad-phase margin and self-contact thickness
+  bool ccd_edge_edge = false;  // the CCD step also runs edge_edge_ccd on each candidate pair
 };
 
 template <class R> struct Forces {
@@ -646,6 +647,14 @@ R ccd_step(int n, const R* x0, const R* y0, const R* z0, const R* x1, const R* y
           if (vt_ccd(P0(v), P1(v), P0(T[0]), P1(T[0]), P0(T[1]), P1(T[1]), P0(T[2]), P1(T[2]), &tt)) self = std::min(self, tt);
         }
       }
+      if (sc.ccd_edge_edge)  // edges (0,1), (1,2), (2,0) of A against those of B, no shared vertex
+        for (int ea = 0; ea < 3; ++ea)
+          for (int eb = 0; eb < 3; ++eb) {
+            const int a0 = A[ea], a1 = A[(ea + 1) % 3], b0 = B[eb], b1 = B[(eb + 1) % 3];
+            if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
+            R tt;
+            if (ee_ccd(P0(a0), P1(a0), P0(a1), P1(a1), P0(b0), P1(b0), P0(b1), P1(b1), &tt)) self = std::min(self, tt);
+          }
     }
     toi = std::min(toi, std::max(self, R(kCcdMinToi)));
   }

@@ -499,6 +499,15 @@ struct or_ipc_params {
   double friction_coefficient, contact_damping;
   int adaptive_contact_damping;
   double contact_damping_max, contact_velocity_threshold;
+  // 1: the handler's set_d_hat is IpcCollisionHandler's default no-op
+  // (pd_solver.rs:488), as in the reference's own drape test
+  // (crates/vistio-bench/tests/bench_tests.rs:181-199): detection keeps
+  // d_hat while the CFL clamp uses the compliant-scaled one.
+  int keep_d_hat;
+  // 1: the CCD step also runs edge_edge_ccd (ccd.rs:138-229) on the 9 edge
+  // pairs of every candidate triangle pair; 0: vertex-triangle only, as
+  // compute_max_step_size (ccd.rs:238-292).
+  int ccd_edge_edge;
 };
 
 // Diagnostics of one IPC step (no counterpart in the reference's StepResult).
@@ -516,6 +525,105 @@ struct or_ipc_stats {
 }  // extern "C"
 
 namespace {
+
+// ---- pieces of step_with_ipc, each exported for closed-form tests ----------
+// Test hook: a nonzero value plants one known mistake in the pieces below, so
+// that a test can show its closed-form cases reject that mistake. The oracle
+// sets it only through or_set_mutation; 0 (the default) is the reference.
+//   1 friction acts on the normal component and the tangential one is dropped
+//   2 lambda -= mu grad          3 the normal component is kept
+//   4 backtracking halves when the barrier pushes along the step
+//   5 global contact damping multiplies by 0.05 instead of 0.95
+//   6 mu grows before the multiplier update
+//   7 adaptive damping blends the wrong way (fast vertices damped most)
+//   8 the 0.8 CCD safety factor also scales a full step
+int g_mutation = 0;
+
+// Step fraction of one PD iteration from the CCD bound (pd_solver.rs:895-906)
+// and the backtracking on the frozen barrier gradient (pd_solver.rs:909-932):
+// grad_dot_d = sum over free vertices of grad . (sol - pos), diff_sq = sum of
+// |sol - pos|^2.
+template <class R> R ipc_step_fraction(R ccd_alpha, double grad_dot_d, R mu, double diff_sq) {
+  R max_alpha = ccd_alpha;
+  if (max_alpha < R(1) || g_mutation == 8) max_alpha *= R(0.8);
+  max_alpha = std::min(max_alpha, R(1));
+  R alpha = max_alpha;
+  for (int bt = 0; bt < 4; ++bt) {
+    if (alpha < R(0.1)) break;
+    const double g = grad_dot_d * double(mu);
+    const bool against = g_mutation == 4 ? -g > 1e-4 * diff_sq * 100.0 : g > 1e-4 * diff_sq * 100.0;
+    if (against) alpha *= R(0.5);
+    else break;
+  }
+  return alpha;
+}
+
+// Augmented-Lagrangian multiplier update and penalty growth,
+// pd_solver.rs:1006-1021: lambda += mu grad where in contact, then mu *= growth.
+template <class R>
+void ipc_al_update(int n, const uint8_t* in_contact, const R* gx, const R* gy, const R* gz, R* lx, R* ly, R* lz,
+                   R& mu, R growth) {
+  const R m = g_mutation == 6 ? mu * growth : mu;
+  for (int i = 0; i < n; ++i)
+    if (in_contact[i]) {
+      if (g_mutation == 2) { lx[i] -= m * gx[i]; ly[i] -= m * gy[i]; lz[i] -= m * gz[i]; }
+      else { lx[i] += m * gx[i]; ly[i] += m * gy[i]; lz[i] += m * gz[i]; }
+    }
+  mu *= growth;
+}
+
+// The inelastic velocity filter, pd_solver.rs:1036-1111: for free vertices
+// that were in contact during the step and have a unit contact normal,
+// remove the normal velocity, apply Coulomb friction with |v.n| as the
+// normal-impulse proxy (stick when mu_f |v_n| >= |v_t|, else scale v_t by
+// 1 - mu_f |v_n| / |v_t|), then damp by the contact damping (adaptive: blended
+// towards contact_damping_max as the speed falls below the threshold).
+template <class R> struct VelFilter {
+  R mu_f, cd, cd_max, v_thresh;
+  bool adaptive;
+};
+template <class R>
+void ipc_velocity_filter(int n, R* vx, R* vy, R* vz, const R* inv_mass, const uint8_t* ever, const R* fnx,
+                         const R* fny, const R* fnz, const VelFilter<R>& f) {
+  for (int i = 0; i < n; ++i) {
+    if (inv_mass[i] == R(0) || !ever[i]) continue;
+    const R nx = fnx[i], ny = fny[i], nz = fnz[i];
+    if (nx * nx + ny * ny + nz * nz < R(0.5)) continue;
+    const R vn = vx[i] * nx + vy[i] * ny + vz[i] * nz;
+    if (g_mutation == 1) {  // the components swapped: friction on the normal part
+      const R tx = vx[i] - vn * nx, ty = vy[i] - vn * ny, tz = vz[i] - vn * nz;
+      const R vt = std::sqrt(tx * tx + ty * ty + tz * tz), vnm = std::fabs(vn);
+      const R s2 = vnm > R(1e-8) && f.mu_f * vt < vnm ? R(1) - f.mu_f * vt / vnm : R(0);
+      vx[i] = vn * nx * s2; vy[i] = vn * ny * s2; vz[i] = vn * nz * s2;
+    } else {
+      if (g_mutation != 3) { vx[i] -= vn * nx; vy[i] -= vn * ny; vz[i] -= vn * nz; }
+      const R vnm = std::fabs(vn);
+      const R tx = vx[i], ty = vy[i], tz = vz[i];
+      const R vt = std::sqrt(tx * tx + ty * ty + tz * tz);
+      if (vt > R(1e-8)) {
+        const R fi = f.mu_f * vnm;
+        if (fi >= vt) { vx[i] = 0; vy[i] = 0; vz[i] = 0; }
+        else { const R s2 = R(1) - fi / vt; vx[i] = tx * s2; vy[i] = ty * s2; vz[i] = tz * s2; }
+      }
+    }
+    R damp_i = f.cd;
+    if (f.adaptive) {
+      const R v = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
+      const R q = v / std::max(f.v_thresh, R(1e-6));
+      const R tt = std::min(std::max(g_mutation == 7 ? q : R(1) - q, R(0)), R(1));
+      damp_i = f.cd + tt * (f.cd_max - f.cd);
+    }
+    vx[i] *= R(1) - damp_i; vy[i] *= R(1) - damp_i; vz[i] *= R(1) - damp_i;
+  }
+}
+
+// Global contact damping, pd_solver.rs:1115-1117: state.damp_velocities(0.05)
+// (state.rs:233-240, every vertex) when the last detection had any contact.
+template <class R> void ipc_contact_damping(int n, long long active, R* vx, R* vy, R* vz) {
+  if (active <= 0) return;
+  const R f = g_mutation == 5 ? R(0.05) : R(1) - R(0.05);
+  for (int i = 0; i < n; ++i) { vx[i] *= f; vy[i] *= f; vz[i] *= f; }
+}
 
 struct OracleBase {
   virtual ~OracleBase() {}
@@ -877,10 +985,14 @@ template <class R> struct Oracle : OracleBase {
     const bool gnd = sc.has_ground;  // runner.rs:118-122 sets state.ground_height
     const R gh = R(ip.ground_height);
     qx = px; qy = py; qz = pz;  // save_previous
-    const R dh = R(ip.compliant_contact ? ip.d_hat * ip.compliant_d_hat_scale : ip.d_hat);
+    // d_hat_effective (pd_solver.rs:612-618) drives the CFL clamp; the
+    // handler detects with it unless its set_d_hat is the no-op default.
+    const R dh_eff = R(ip.compliant_contact ? ip.d_hat * ip.compliant_d_hat_scale : ip.d_hat);
+    const R dh = ip.keep_d_hat ? R(ip.d_hat) : dh_eff;
+    sc.ccd_edge_edge = ip.ccd_edge_edge != 0;
     const R kappa = R(ip.kappa);
     {  // CFL-like velocity clamp, pd_solver.rs:572-595
-      const R max_vel = R(0.5) * std::sqrt(dh) / dt;
+      const R max_vel = R(0.5) * std::sqrt(dh_eff) / dt;
       for (int i = 0; i < n; ++i) {
         if (inv_mass[i] == R(0)) continue;
         const R v2 = vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i];
@@ -953,20 +1065,13 @@ template <class R> struct Oracle : OracleBase {
         }
         final_residual = norm > 1e-12 ? std::sqrt(diff / norm) : std::sqrt(diff);
         // CCD-limited step with the pairs of the last detection, pd_solver.rs:897-906
-        R max_alpha = oc::ccd_step<R>(n, px.data(), py.data(), pz.data(), sol[0].data(), sol[1].data(), sol[2].data(),
-                                      tri_flat.data(), pairs, sc);
-        if (max_alpha < R(1)) max_alpha *= R(0.8);
-        max_alpha = std::min(max_alpha, R(1));
-        alpha = max_alpha;  // backtracking, pd_solver.rs:909-930
+        const R ccd_alpha = oc::ccd_step<R>(n, px.data(), py.data(), pz.data(), sol[0].data(), sol[1].data(),
+                                            sol[2].data(), tri_flat.data(), pairs, sc);
         double gdd = 0;
         for (int i = 0; i < n; ++i)
           if (inv_mass[i] > R(0))
             gdd += double(bf.gx[i] * (sol[0][i] - px[i]) + bf.gy[i] * (sol[1][i] - py[i]) + bf.gz[i] * (sol[2][i] - pz[i]));
-        for (int bt = 0; bt < 4; ++bt) {
-          if (alpha < R(0.1)) break;
-          if (gdd * double(mu) > 1e-4 * diff * 100.0) alpha *= R(0.5);
-          else break;
-        }
+        alpha = ipc_step_fraction<R>(ccd_alpha, gdd, mu, diff);
         for (int i = 0; i < n; ++i)  // pd_solver.rs:934-950
           if (inv_mass[i] > R(0)) {
             px[i] += alpha * (sol[0][i] - px[i]);
@@ -978,41 +1083,18 @@ template <class R> struct Oracle : OracleBase {
       }
       detect_at(uf);  // pd_solver.rs:966-978
       if (uf.max_violation < R(ip.al_tolerance)) { al_converged = true; break; }
-      for (int i = 0; i < n; ++i)  // pd_solver.rs:1006-1015
-        if (uf.in_contact[i]) { lx[i] += mu * uf.gx[i]; ly[i] += mu * uf.gy[i]; lz[i] += mu * uf.gz[i]; }
-      mu *= R(ip.al_mu_growth);
+      ipc_al_update<R>(n, uf.in_contact.data(), uf.gx.data(), uf.gy.data(), uf.gz.data(), lx.data(), ly.data(),
+                       lz.data(), mu, R(ip.al_mu_growth));
     }
     const R inv_dt = R(1) / dt;  // update_velocities
     for (int i = 0; i < n; ++i) {
       vx[i] = (px[i] - qx[i]) * inv_dt; vy[i] = (py[i] - qy[i]) * inv_dt; vz[i] = (pz[i] - qz[i]) * inv_dt;
     }
-    const R muf = R(ip.friction_coefficient), cd = R(ip.contact_damping);
-    for (int i = 0; i < n; ++i) {  // velocity filter, pd_solver.rs:1036-1111
-      if (inv_mass[i] == R(0) || !ever[i]) continue;
-      const R nx = fnx[i], ny = fny[i], nz = fnz[i];
-      if (nx * nx + ny * ny + nz * nz < R(0.5)) continue;
-      const R vn = vx[i] * nx + vy[i] * ny + vz[i] * nz;
-      vx[i] -= vn * nx; vy[i] -= vn * ny; vz[i] -= vn * nz;
-      const R vnm = std::fabs(vn);
-      const R tx = vx[i], ty = vy[i], tz = vz[i];
-      const R vt = std::sqrt(tx * tx + ty * ty + tz * tz);
-      if (vt > R(1e-8)) {
-        const R fi = muf * vnm;
-        if (fi >= vt) { vx[i] = 0; vy[i] = 0; vz[i] = 0; }
-        else { const R s2 = R(1) - fi / vt; vx[i] = tx * s2; vy[i] = ty * s2; vz[i] = tz * s2; }
-      }
-      R damp_i = cd;
-      if (ip.adaptive_contact_damping) {
-        const R v = std::sqrt(vx[i] * vx[i] + vy[i] * vy[i] + vz[i] * vz[i]);
-        const R tt = std::min(std::max(R(1) - v / std::max(R(ip.contact_velocity_threshold), R(1e-6)), R(0)), R(1));
-        damp_i = cd + tt * (R(ip.contact_damping_max) - cd);
-      }
-      vx[i] *= R(1) - damp_i; vy[i] *= R(1) - damp_i; vz[i] *= R(1) - damp_i;
-    }
-    if (uf.active > 0) {  // pd_solver.rs:1115-1117
-      const R f = R(1) - R(0.05);
-      for (int i = 0; i < n; ++i) { vx[i] *= f; vy[i] *= f; vz[i] *= f; }
-    }
+    const VelFilter<R> vf{R(ip.friction_coefficient), R(ip.contact_damping), R(ip.contact_damping_max),
+                          R(ip.contact_velocity_threshold), ip.adaptive_contact_damping != 0};
+    ipc_velocity_filter<R>(n, vx.data(), vy.data(), vz.data(), inv_mass.data(), ever.data(), fnx.data(), fny.data(),
+                           fnz.data(), vf);
+    ipc_contact_damping<R>(n, uf.active, vx.data(), vy.data(), vz.data());
     if (gnd)  // enforce_ground_velocities, state.rs:195-213
       for (int i = 0; i < n; ++i)
         if (inv_mass[i] > R(0) && py[i] <= gh) {
@@ -1225,6 +1307,28 @@ int or_step_ipc(void* o, double dt, int nsteps, const or_ipc_params* ip, or_step
 void or_get_lambda(void* o, double* out) { static_cast<OracleBase*>(o)->get_lambda(out); }
 void or_set_lambda(void* o, const double* in) { static_cast<OracleBase*>(o)->set_lambda(in); }
 
+// The pieces of step_with_ipc that step_ipc calls, in f64, for closed-form
+// tests. or_set_mutation plants one of the known mistakes listed at
+// g_mutation (0 restores the reference).
+void or_set_mutation(int m) { g_mutation = m; }
+double or_ipc_step_fraction(double ccd_alpha, double grad_dot_d, double mu, double diff_sq) {
+  return ipc_step_fraction<double>(ccd_alpha, grad_dot_d, mu, diff_sq);
+}
+// g, lambda: 3n SoA; mu in/out.
+void or_ipc_al_update(int n, const uint8_t* in_contact, const double* g, double* lambda, double* mu, double growth) {
+  ipc_al_update<double>(n, in_contact, g, g + n, g + 2 * size_t(n), lambda, lambda + n, lambda + 2 * size_t(n), *mu,
+                        growth);
+}
+// v, nrm: 3n SoA; v in/out.
+void or_ipc_velocity_filter(int n, double* v, const double* inv_mass, const uint8_t* ever, const double* nrm,
+                            double mu_f, double cd, int adaptive, double cd_max, double v_thresh) {
+  const VelFilter<double> f{mu_f, cd, cd_max, v_thresh, adaptive != 0};
+  ipc_velocity_filter<double>(n, v, v + n, v + 2 * size_t(n), inv_mass, ever, nrm, nrm + n, nrm + 2 * size_t(n), f);
+}
+void or_ipc_contact_damping(int n, long long active, double* v) {
+  ipc_contact_damping<double>(n, active, v, v + n, v + 2 * size_t(n));
+}
+
 double oc_barrier_energy(double d, double dh) { return oc::barrier_energy(d, dh); }
 double oc_barrier_gradient(double d, double dh) { return oc::barrier_gradient(d, dh); }
 double oc_barrier_hessian(double d, double dh) { return oc::barrier_hessian(d, dh); }
@@ -1315,6 +1419,7 @@ static oc::IpcScene scene_of(const or_ipc_params* ip) {
   sc.radius = ip->sphere_radius;
   sc.self_collision = ip->self_collision != 0;
   sc.thickness = ip->thickness;
+  sc.ccd_edge_edge = ip->ccd_edge_edge != 0;
   return sc;
 }
 // CollisionPipeline::detect_ipc_contacts at positions (SoA, 3n doubles),

@@ -63,6 +63,7 @@ class OrIpcParams(C.Structure):
         ("friction_coefficient", C.c_double), ("contact_damping", C.c_double),
         ("adaptive_contact_damping", C.c_int), ("contact_damping_max", C.c_double),
         ("contact_velocity_threshold", C.c_double),
+        ("keep_d_hat", C.c_int), ("ccd_edge_edge", C.c_int),
     ]
 
 
@@ -76,9 +77,11 @@ def ipc_params(ground=None, sphere=None, self_collision=False, exclusion_depth=1
                thickness=None, al_max_iterations=5, al_mu_initial=1.0, al_mu_growth=10.0, al_tolerance=1e-4,
                compliant_contact=False, compliant_d_hat_scale=2.0, friction_coefficient=0.4, contact_damping=0.3,
                adaptive_contact_damping=True, contact_damping_max=0.5,
-               contact_velocity_threshold=0.1) -> OrIpcParams:
+               contact_velocity_threshold=0.1, keep_d_hat=False, ccd_edge_edge=False) -> OrIpcParams:
     """Contact settings; defaults follow SolverConfig::default (config.rs:113-133)
-    except compliant_contact (off: literal d_hat) and kappa (given)."""
+    except compliant_contact (off: literal d_hat) and kappa (given). keep_d_hat:
+    the handler ignores set_d_hat (detection keeps d_hat, the CFL clamp uses
+    the compliant one); ccd_edge_edge: the CCD step adds edge-edge tests."""
     q = OrIpcParams()
     q.has_ground = 0 if ground is None else 1
     q.ground_height = 0.0 if ground is None else float(ground)
@@ -97,6 +100,7 @@ def ipc_params(ground=None, sphere=None, self_collision=False, exclusion_depth=1
     q.friction_coefficient, q.contact_damping = friction_coefficient, contact_damping
     q.adaptive_contact_damping = int(adaptive_contact_damping)
     q.contact_damping_max, q.contact_velocity_threshold = contact_damping_max, contact_velocity_threshold
+    q.keep_d_hat, q.ccd_edge_edge = int(keep_d_hat), int(ccd_edge_edge)
     return q
 
 
@@ -144,6 +148,13 @@ def lib():
         L.or_membrane_project.argtypes = [C.c_int, C.c_int, dp, dp, dp]
         ip = C.POINTER(OrIpcParams)
         L.or_step_ipc.argtypes = [vp, C.c_double, C.c_int, ip, C.POINTER(OrStepResult), C.POINTER(OrIpcStats)]
+        L.or_set_mutation.argtypes = [C.c_int]
+        L.or_ipc_step_fraction.restype = C.c_double
+        L.or_ipc_step_fraction.argtypes = [C.c_double] * 4
+        L.or_ipc_al_update.argtypes = [C.c_int, C.POINTER(C.c_uint8), dp, dp, dp, C.c_double]
+        L.or_ipc_velocity_filter.argtypes = [C.c_int, dp, dp, C.POINTER(C.c_uint8), dp, C.c_double, C.c_double,
+                                             C.c_int, C.c_double, C.c_double]
+        L.or_ipc_contact_damping.argtypes = [C.c_int, C.c_longlong, dp]
         L.or_get_lambda.argtypes = [vp, dp]
         L.or_set_lambda.argtypes = [vp, dp]
         for nm in ("oc_barrier_energy", "oc_barrier_gradient", "oc_barrier_hessian", "oc_barrier_energy_thick",
@@ -488,3 +499,45 @@ def ccd_step(p0, p1, indices, ipc: OrIpcParams):
     tri = np.ascontiguousarray(indices, dtype=np.uint32)
     return lib().oc_ccd_step(p0.shape[1], _p(p0, C.c_double), _p(p1, C.c_double), tri.size // 3,
                              _p(tri, C.c_uint32), C.byref(ipc))
+
+
+# ---- pieces of step_with_ipc (closed-form tests) ---------------------------
+
+def set_mutation(m: int) -> None:
+    """Plants known mistake m in the oracle's contact-step pieces (0 = none);
+    test use only, see g_mutation in cpu_ref.cpp."""
+    lib().or_set_mutation(int(m))
+
+
+def ipc_step_fraction(ccd_alpha, grad_dot_d, mu, diff_sq) -> float:
+    """CCD safety factor and backtracking, pd_solver.rs:895-932."""
+    return float(lib().or_ipc_step_fraction(ccd_alpha, grad_dot_d, mu, diff_sq))
+
+
+def ipc_al_update(in_contact, grad, lam, mu, growth):
+    """lambda += mu grad where in contact, then mu *= growth (pd_solver.rs:1006-1021).
+    grad, lam: (3, n). Returns (lambda, mu)."""
+    inc = np.ascontiguousarray(in_contact, dtype=np.uint8)
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    lo = np.ascontiguousarray(lam, dtype=np.float64).copy()
+    m = C.c_double(mu)
+    lib().or_ipc_al_update(inc.size, _p(inc, C.c_uint8), _p(g, C.c_double), _p(lo, C.c_double), C.byref(m), growth)
+    return lo, m.value
+
+
+def ipc_velocity_filter(v, inv_mass, ever, normal, mu_f, cd, adaptive=False, cd_max=0.5, v_thresh=0.1):
+    """The velocity filter of pd_solver.rs:1036-1111 on (3, n) velocities."""
+    vo = np.ascontiguousarray(v, dtype=np.float64).copy()
+    im = np.ascontiguousarray(inv_mass, dtype=np.float64)
+    ev = np.ascontiguousarray(ever, dtype=np.uint8)
+    nr = np.ascontiguousarray(normal, dtype=np.float64)
+    lib().or_ipc_velocity_filter(im.size, _p(vo, C.c_double), _p(im, C.c_double), _p(ev, C.c_uint8), _p(nr, C.c_double), mu_f, cd, int(adaptive),
+                                 cd_max, v_thresh)
+    return vo
+
+
+def ipc_contact_damping(v, active):
+    """state.damp_velocities(0.05) when active > 0 (pd_solver.rs:1115-1117)."""
+    vo = np.ascontiguousarray(v, dtype=np.float64).copy()
+    lib().or_ipc_contact_damping(vo.shape[1], int(active), _p(vo, C.c_double))
+    return vo

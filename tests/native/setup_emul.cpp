@@ -106,3 +106,9 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
   return 0;
 }
 
+
+// 1 when the device's 32-bit offsets hold these totals (the set-up's own check).
+extern "C" int emul_index_fits(long long m_total, long long sroot_entries, long long front_slots,
+                               long long rhs_values) {
+  return vh::device_index_overflow(m_total, sroot_entries, front_slots, rhs_values).empty() ? 1 : 0;
+}

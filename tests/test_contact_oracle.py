@@ -423,3 +423,23 @@ def test_step_ipc_sheet_on_ground_stays_above():
     assert min(miny) > 0.0
     assert abs(o.get(1)).max() < 0.05
     assert np.all(np.isfinite(o.positions()))
+
+
+def test_product_estimate_initial_kappa():
+    """vistio_amd.estimate_initial_kappa (the product's barrier.rs:97-115):
+    the reference's KATs (barrier_tests.rs:73-87), the closed form at the
+    midpoint of the barrier zone and both clamps. With d = d_hat / 2 the
+    barrier gradient is |b'(d)| = (d_hat / 2)(1 + 2 ln 2), so
+    kappa = sqrt(2) m g / (d_hat^1.5 (1 + 2 ln 2))."""
+    import vistio_amd as va
+    k1 = va.estimate_initial_kappa(0.0, 0.001, 9.81, 1e-3)
+    k2 = va.estimate_initial_kappa(0.0, 0.01, 9.81, 1e-3)
+    assert 0 < k1 < k2 < 1e8
+    assert va.estimate_initial_kappa(5000.0, 0.001, 9.81, 1e-3) == 5000.0
+    for m, g, d in ((0.002, 9.81, 1e-3), (0.01, 9.81, 1e-3), (0.002, 9.81, 4e-4), (0.05, 3.0, 2e-3)):
+        want = np.sqrt(2.0) * m * g / (d ** 1.5 * (1.0 + 2.0 * np.log(2.0)))
+        assert 1.0 < want < 1e6
+        assert va.estimate_initial_kappa(0.0, m, g, d) == pytest.approx(want, rel=1e-13)
+        assert va.estimate_initial_kappa(0.0, m, g, d) == pytest.approx(ref.estimate_kappa(0.0, m, g, d), rel=1e-13)
+    assert va.estimate_initial_kappa(0.0, 1e-9, 9.81, 1e-3) == 1.0     # clamped below
+    assert va.estimate_initial_kappa(0.0, 1e3, 9.81, 1e-3) == 1e6      # clamped above

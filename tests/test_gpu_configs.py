@@ -1,0 +1,221 @@
+"""The benchmark workloads at their full sizes on the GPU, through the C ABI.
+
+* hanging sheet 256 x 256, 120 steps (the headline): FP64 within 1e-9 m and
+  FP32-IR at the benchmark's settings (one refinement sweep per PD
+  iteration, one on the last) within 1e-6 m of the FP64 oracle
+  (pd_solver.rs:259-476), which runs on all host threads (its results do not
+  depend on the thread count).
+* batched sheets: the 64 seeded sheets as one system; the smallest, the
+  median and the largest follow their own oracle runs.
+* sphere drape 512 x 512, 180 steps, and self-fold ribbon 64 x 640, 300 steps:
+  no crossing triangle pair at any step, no vertex inside the sphere or below
+  the ground, FP32-IR next to FP64.
+* the reference's own verify_sphere_drape_no_penetration
+  (crates/vistio-bench/tests/bench_tests.rs:138-224) through the HIP step,
+  with the FP64 step also held to the oracle.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import DT, max_vertex_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def all_threads():
+    from oracle import ref
+    ref.set_threads(min(16, os.cpu_count() or 1))
+    yield
+    ref.set_threads(1)
+
+
+def _solver(mesh, topo, pinned, precision, ir_sweeps=1, ir_final_sweeps=1, **cfg_over):
+    import vistio_amd as va
+    kw = dict(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0, rayleigh_mass_damping=0.01,
+              precision=precision, ir_sweeps=ir_sweeps, ir_final_sweeps=ir_final_sweeps)
+    kw.update(cfg_over)
+    s = va.HipPdSolver()
+    s.init_raw(mesh, topo, va.SolverConfig(**kw), pinned, "co_rotational", 1e3, 1e-4, 0.15)
+    return s
+
+
+def test_headline_256_matches_fp64_oracle(all_threads):
+    """BASELINE configs[1] exactly as bench.py runs it, 120 steps from rest."""
+    from oracle import ref
+    from vistio_amd.scenes import hanging_sheet
+    mesh, topo, pinned = hanging_sheet(256)
+    print("256x256: oracle (FP64) 120 steps ...", flush=True)
+    o = ref.Oracle(ref.F64, mesh.pos_x, mesh.pos_y, mesh.pos_z, mesh.indices, topo.interior_edges[:, :4], pinned,
+                   ref.params())
+    o.step(DT, 120)
+    xo = o.positions()
+    errs = {}
+    for prec, tol in (("f64", 1e-9), ("f32_ir", 1e-6)):
+        g = _solver(mesh, topo, pinned, prec)
+        r = g.step_resident(DT, 120)
+        errs[prec] = max_vertex_err(g.positions(), xo)
+        print(f"256x256 {prec} 120 steps: max vertex error vs FP64 oracle {errs[prec]:.3e} m "
+              f"(iterations {r.iterations}, final residual {r.final_residual:.3e})", flush=True)
+        g.close()
+        assert r.iterations == 10
+        assert errs[prec] <= tol, (prec, errs[prec])
+    # the sheet has actually moved: the lowest point sags well below rest
+    assert float(xo[1].min()) < -0.6
+
+
+def _batch_sheet_oracle(b, s, mode):
+    from oracle import ref
+    sl = b.sheet_slice(s)
+    v0 = b.vert_off[s]
+    t0, t1 = b.tri_off[s], b.tri_off[s + 1]
+    h0, h1 = b.hinge_off[s], b.hinge_off[s + 1]
+    idx = (b.mesh.indices.reshape(-1, 3)[t0:t1] - v0).astype(np.uint32).ravel()
+    hg = (b.topology.interior_edges[h0:h1, :4].astype(np.int64) - v0).astype(np.uint32)
+    sp = b.specs[s]
+    return ref.Oracle(mode, b.mesh.pos_x[sl], b.mesh.pos_y[sl], b.mesh.pos_z[sl], idx, hg, b.pinned[sl],
+                      ref.params(k_membrane=sp.k_membrane, k_bending=sp.k_bending, density=sp.density))
+
+
+@pytest.mark.parametrize("precision,steps,tol", [("f32_ir", 120, 1e-6), ("f64", 10, 1e-9)])
+def test_batched_64_sheets_follow_their_oracles(precision, steps, tol, all_threads):
+    """64 sheets (SplitMix64 seeds 0x5EED ^ id, grids
+    32..256, per-sheet density and stiffness) in one system and one graph;
+    the smallest, median and largest sheet each follow their own oracle."""
+    import vistio_amd as va
+    from oracle import ref
+    from vistio_amd.batch import build_batch, cfg3_specs
+    specs = cfg3_specs(64)
+    b = build_batch(specs)
+    cells = np.array([sp.cells for sp in specs])
+    order = np.argsort(cells, kind="stable")
+    picks = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    cfg = va.SolverConfig(max_iterations=10, tolerance=0.0, gravity=(0.0, -9.81, 0.0), damping=0.0,
+                          rayleigh_mass_damping=0.01, precision=precision, ir_sweeps=1, ir_final_sweeps=1)
+    g = va.HipPdSolver()
+    g.init_raw(b.mesh, b.topology, cfg, b.pinned, "co_rotational", b.k_membrane, b.k_bending, b.density)
+    info = g.info()
+    print(f"batch {precision}: {info['n']} verts, {len(specs)} sheets, {info['levels']} bands, "
+          f"set-up {info['setup_seconds']:.1f} s; checked sheets {picks} ({cells[picks].tolist()} cells)", flush=True)
+    assert info["n"] == b.mesh.vertex_count() and len(specs) == 64
+    r = g.step_resident(DT, steps)
+    assert r.iterations == 10
+    x = g.positions()
+    assert np.isfinite(x).all()
+    for s in picks:
+        o = _batch_sheet_oracle(b, s, ref.F64)
+        assert np.allclose(g.lumped_masses()[b.sheet_slice(s)], o.masses(), rtol=1e-6)
+        o.step(DT, steps)
+        err = max_vertex_err(x[:, b.sheet_slice(s)], o.positions())
+        print(f"  sheet {s} ({specs[s].cells}^2) after {steps} steps: {err:.3e} m", flush=True)
+        assert err <= tol, (s, err)
+    g.close()
+
+
+def _run_contact(name, precision, check_every=1):
+    """Steps a contact workload and checks every `check_every` steps: no
+    crossing triangle pair, no vertex below the ground, none inside the
+    sphere. Returns the final positions and the worst values seen."""
+    import vistio_amd as va
+    from vistio_amd.scenes import contact_scene
+    mesh, ipc, steps, desc = contact_scene(name)
+    topo = va.Topology.build(mesh)
+    pinned = np.zeros(mesh.vertex_count(), np.uint8)
+    g = _solver(mesh, topo, pinned, precision)
+    g.init_ipc(ipc)
+    worst = dict(crossings=0, below_ground=0.0, inside_sphere=0.0, al=0, contacts=0)
+    for k in range(steps):
+        r, st = g.step_ipc_resident(DT, 1)
+        worst["al"] = max(worst["al"], st["al_iterations"])
+        worst["contacts"] = max(worst["contacts"], st["active"])
+        if k % check_every == 0 or k == steps - 1:
+            x = g.positions()
+            assert np.isfinite(x).all(), k
+            worst["crossings"] = max(worst["crossings"], g.count_crossings())
+            worst["below_ground"] = max(worst["below_ground"], float(ipc.ground_height - x[1].min()))
+            if ipc.sphere is not None:
+                (c, rad) = ipc.sphere
+                d = np.sqrt(((x - np.array(c)[:, None]) ** 2).sum(0))
+                worst["inside_sphere"] = max(worst["inside_sphere"], float(rad - d.min()))
+        if k % 60 == 59:
+            print(f"  [{name} {precision}] step {k + 1}/{steps}: worst so far {worst}", flush=True)
+    x = g.positions()
+    g.close()
+    return x, worst, steps
+
+
+@pytest.mark.parametrize("name,check_every", [("sphere_drape", 1), ("self_fold_ribbon", 1)])
+def test_contact_workload_full_size(name, check_every):
+    """BASELINE configs[2] (512 x 512 sphere drape, 180 steps) and configs[4]
+    (64 x 640 self-fold ribbon, 300 steps) in FP32-IR, checked at every
+    step; then the FP64 run, and the gap between the two."""
+    x32, w32, steps = _run_contact(name, "f32_ir", check_every)
+    print(f"{name} f32_ir {steps} steps: {w32}", flush=True)
+    assert w32["crossings"] == 0
+    assert w32["below_ground"] <= 0.0
+    assert w32["inside_sphere"] <= 0.0
+    assert w32["contacts"] > 0
+    x64, w64, _ = _run_contact(name, "f64", 10)
+    gap = max_vertex_err(x32, x64)
+    print(f"{name} f64: {w64}; FP32-IR vs FP64 after {steps} steps: {gap:.3e} m", flush=True)
+    assert w64["crossings"] == 0 and w64["below_ground"] <= 0.0 and w64["inside_sphere"] <= 0.0
+    assert gap < 0.05
+
+
+def test_reference_sphere_drape_no_penetration():
+    """verify_sphere_drape_no_penetration (bench_tests.rs:138-224) on the HIP
+    step: Scenario::sphere_drape, the solver's init(), a handler that keeps
+    d_hat, 40 steps, and no vertex closer than 0.3 - 0.02 m to the sphere's
+    centre after any step. The FP64 step also follows the oracle's."""
+    import vistio_amd as va
+    from oracle import ref
+    from vistio_amd.scenes import reference_sphere_drape
+    sc = reference_sphere_drape(40)
+    m = sc.mesh
+    o = ref.Oracle(ref.F64, m.pos_x, m.pos_y, m.pos_z, m.indices, sc.topology.interior_edges[:, :4], sc.pinned,
+                   ref.params(model=0, k_membrane=1000.0, k_bending=50.0, density=0.2, damping=0.01, rayleigh=2.0,
+                              max_iterations=10, tolerance=1e-6))
+    q = sc.ipc
+    oip = ref.ipc_params(ground=q.ground_height, sphere=q.sphere, self_collision=False, d_hat=q.barrier_d_hat,
+                         kappa=q.barrier_kappa, thickness=q.thickness, al_max_iterations=q.al_max_iterations,
+                         al_mu_initial=q.al_mu_initial, al_mu_growth=q.al_mu_growth, al_tolerance=q.al_tolerance,
+                         compliant_contact=True, compliant_d_hat_scale=2.0, friction_coefficient=0.4,
+                         contact_damping=0.3, adaptive_contact_damping=True, contact_damping_max=0.5,
+                         contact_velocity_threshold=0.1, keep_d_hat=True)
+    # FP64 with the state resident on the device, step by step against the oracle
+    g = va.HipPdSolver()
+    g.init(m, sc.topology, va.SolverConfig(precision="f64"), sc.pinned)
+    g.init_ipc(q)
+    closest, err = 1e9, 0.0
+    for k in range(sc.timesteps):
+        g.step_ipc_resident(DT, 1)
+        o.step_ipc(oip)
+        x = g.positions()
+        err = max(err, max_vertex_err(x, o.positions()))
+        d = float(np.sqrt((x ** 2).sum(0)).min())
+        closest = min(closest, d)
+        assert d >= 0.3 - 0.02, ("f64", k, d)
+    print(f"reference drape f64 (resident): closest approach to the sphere centre {closest:.5f} m, "
+          f"max vertex error vs the FP64 oracle {err:.3e} m", flush=True)
+    assert err < 1e-7
+    assert closest < 0.3 + 0.04  # the sheet reached the sphere's barrier zone
+    g.close()
+    # the reference's own flow: f32 SimulationState handed over every step (FP32-IR)
+    g = va.HipPdSolver()
+    g.init(m, sc.topology, va.SolverConfig(precision="f32_ir", ir_sweeps=1, ir_final_sweeps=1), sc.pinned)
+    g.init_ipc(q)
+    state = va.SimulationState.from_mesh_with_masses(m, np.full(m.vertex_count(), sc.vertex_mass, np.float32),
+                                                     sc.pinned)
+    state.ground_height = q.ground_height
+    closest = 1e9
+    for k in range(sc.timesteps):
+        g.step_with_ipc(state, DT)
+        d = np.sqrt(state.pos_x.astype(np.float64) ** 2 + state.pos_y.astype(np.float64) ** 2
+                    + state.pos_z.astype(np.float64) ** 2)
+        closest = min(closest, float(d.min()))
+        assert d.min() >= np.float32(0.3) - np.float32(0.02), ("f32_ir", k, float(d.min()))
+    print(f"reference drape f32_ir (host f32 state): closest approach {closest:.5f} m", flush=True)
+    assert closest < 0.3 + 0.04
+    g.close()

@@ -404,3 +404,73 @@ def test_pair_buffer_grows_at_an_al_update():
     assert f1[1] == f0[1] == end and f1[2:] == f0[2:]
     assert max_vertex_err(f0[0], f1[0]) <= 1e-10 and max_vertex_err(x5a, x5b) <= 1e-8
     assert np.abs(la - lb).max() <= 1e-6 * max(1.0, np.abs(la).max())
+
+
+def test_edge_edge_ccd_matches_oracle():
+    """IpcConfig.ccd_edge_edge adds edge_edge_ccd (ccd.rs:138-229) on the nine
+    edge pairs of every candidate triangle pair to the CCD step. Two
+    triangles whose edges cross like ccd_tests.rs:54-66 (edge (a, b) sweeps
+    down through edge (c, d) at t = 0.5), while no vertex ever crosses the
+    other triangle's plane: vertex-triangle CCD allows the full step, the
+    edge-edge tests stop it at 0.9 t = 0.45. On the folded sheet's moves
+    the step fraction equals the oracle's too (same FP64 formula), to 1e-12."""
+    from oracle import ref
+    P = np.array([[-1.0, 0.0, 0.05], [1.0, 0.0, 0.05], [0.0, 0.0, 1.0],
+                  [0.0, -1.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, -1.0]]).T.astype(np.float32)
+    idx = np.arange(6, dtype=np.uint32)
+    hg = np.zeros((0, 4), np.uint32)
+    d = np.zeros((3, 6))
+    d[2, :3] = -0.1
+    pos, fidx, fhg = _folded_sheet(gap=0.02)
+    fp0 = pos.astype(np.float64)
+    fp0[1] += 0.03
+    rng = np.random.default_rng(9)
+    fmoves = [rng.normal(size=fp0.shape) * 0.01, np.where(fp0[1:2] > 0.04, [[0.0], [-0.03], [0.0]], 0.0)]
+    got = {}
+    for ee in (False, True):
+        g, o = _cfgs(self_collision=True, thickness=0.1)
+        g.ccd_edge_edge = ee
+        o.ccd_edge_edge = int(ee)
+        s = _solver(P[0], P[1], P[2], idx, hg, np.zeros(6, np.uint8), g)
+        p0 = P.astype(np.float64)
+        got[ee] = s.compute_ccd_step(p0, p0 + d)
+        want = ref.ccd_step(p0, p0 + d, idx, o)
+        print(f"crossing edges, edge_edge={ee}: alpha gpu {got[ee]} oracle {want}", flush=True)
+        assert got[ee] == pytest.approx(want, rel=1e-12, abs=1e-15)
+        s.close()
+        g, o = _cfgs(self_collision=True)
+        g.ccd_edge_edge = ee
+        o.ccd_edge_edge = int(ee)
+        s = _solver(pos[0], pos[1], pos[2], fidx, fhg, np.zeros(pos.shape[1], np.uint8), g)
+        for mv in fmoves:
+            a = s.compute_ccd_step(fp0, fp0 + mv)
+            b = ref.ccd_step(fp0, fp0 + mv, fidx, o)
+            print(f"folded sheet, edge_edge={ee}: alpha gpu {a} oracle {b}", flush=True)
+            assert a == pytest.approx(b, rel=1e-12, abs=1e-15)
+        s.close()
+    assert got[False] == 1.0
+    assert got[True] == pytest.approx(0.45, rel=1e-9)
+
+
+def test_edge_edge_ccd_step_matches_oracle():
+    """A contact step with edge-edge CCD on: the folded sheet falls onto
+    itself and the ground for 6 steps; positions follow the FP64 oracle with
+    the same option, and the AL and PD iteration counts agree."""
+    pos, idx, hg = _folded_sheet(gap=0.02)
+    n = pos.shape[1]
+    pin = np.zeros(n, np.uint8)
+    pos = pos.copy()
+    pos[1] += np.float32(0.01)
+    g, o = _cfgs(ground=0.0, self_collision=True, kappa=1e2, d_hat=1e-5)
+    g.ccd_edge_edge = True
+    o.ccd_edge_edge = 1
+    s = _solver(pos[0], pos[1], pos[2], idx, hg, pin, g)
+    orc = _oracle(pos[0], pos[1], pos[2], idx, hg, pin)
+    for k in range(6):
+        r, st = s.step_ipc_resident(DT, 1)
+        ro, so = orc.step_ipc(o)
+        assert (st["al_iterations"], r.iterations) == (so.al_iterations, ro.iterations), k
+    err = max_vertex_err(s.positions(), orc.positions())
+    print(f"edge-edge CCD step: 6 steps, max vertex err {err:.3e} m, last alpha {st['last_alpha']}", flush=True)
+    assert err < 1e-8
+    s.close()

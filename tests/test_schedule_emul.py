@@ -79,3 +79,18 @@ def test_single_band_root_only(emul):
 def test_leaf_sizes(emul, leaf):
     x, xo, _ = replay(emul, 24, leaf=leaf)
     assert np.abs(x - xo).max() <= 1e-11 * np.abs(xo).max()
+
+
+def test_device_index_limits(emul):
+    """The set-up refuses a system whose block, root-inverse, front-slot or
+    RHS-slot totals would overflow the kernels' 32-bit offsets, instead of
+    letting the device index out of bounds."""
+    f = emul.emul_index_fits
+    f.argtypes = [C.c_longlong] * 4
+    lim = 2 ** 31 - 1
+    assert f(lim, lim, lim, lim) == 1
+    assert f(0, 0, 0, 0) == 1
+    for k in range(4):
+        args = [1000] * 4
+        args[k] = lim + 1
+        assert f(*args) == 0, k

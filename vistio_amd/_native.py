@@ -85,6 +85,7 @@ class IpcParams(C.Structure):
         ("friction_coefficient", C.c_double), ("contact_damping", C.c_double),
         ("adaptive_contact_damping", C.c_int32), ("contact_damping_max", C.c_double),
         ("contact_velocity_threshold", C.c_double), ("max_pairs", C.c_uint32),
+        ("detect_keeps_d_hat", C.c_int32), ("ccd_edge_edge", C.c_int32),
     ]
 
 

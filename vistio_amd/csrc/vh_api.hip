@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -63,6 +64,11 @@ struct IpcState {
   vhc::IpcDev* ip = nullptr;
   void* sort_tmp = nullptr;
   size_t sort_bytes = 0;
+  double cfl_dh = 0;  // d_hat of the CFL velocity clamp (compliant-scaled when compliant_contact)
+  // First error a library routine (the rocPRIM sort and scan of the broad
+  // phase) returned while a launch sequence was enqueued; take_lib_err reads
+  // and clears it.
+  hipError_t lib_err = hipSuccess;
 };
 
 struct vh_ctx {
@@ -578,10 +584,15 @@ void enqueue_ccd_pairs(vh_ctx* c, const double* x, const double* dl) {
   const int t = (int)c->S.t;
   hipLaunchKernelGGL(vhc::k_ccd_tri_bounds, dim3(std::max(1, nblk(t))), dim3(256), 0, c->stream, t, c->tri_idx, x, dl,
                      I.ccdtri, I.ip);
+  const int ee = I.P.ccd_edge_edge ? 1 : 0;
   hipLaunchKernelGGL(vhc::k_ccd_cull, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.pairs, &I.bd->npairs,
-                     I.cap, I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, I.ip);
-  hipLaunchKernelGGL(vhc::k_ccd_pairs, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.ccd_surv, I.ccd_reg,
-                     I.ccd_cnt, c->tri_idx, x, dl, I.ip);
+                     I.cap, I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, I.ip, ee);
+  if (ee)
+    hipLaunchKernelGGL(vhc::k_ccd_pairs<true>, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.ccd_surv,
+                       I.ccd_reg, I.ccd_cnt, c->tri_idx, x, dl, I.ip);
+  else
+    hipLaunchKernelGGL(vhc::k_ccd_pairs<false>, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.ccd_surv,
+                       I.ccd_reg, I.ccd_cnt, c->tri_idx, x, dl, I.ip);
 }
 
 // After a candidate-pair query has been enqueued: reads the pair count (one
@@ -616,8 +627,9 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   hipLaunchKernelGGL(vhc::k_scene_bounds, dim3(1), dim3(256), 0, c->stream, nblk(t), I.bpart, I.bd, skip);
   hipLaunchKernelGGL(vhc::k_morton, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.bd, I.code, I.id, skip);
   size_t bytes = I.sort_bytes;
-  (void)rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t, 0u, 30u,
-                                  c->stream);
+  const hipError_t re = rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t,
+                                                  0u, 30u, c->stream);
+  if (re != hipSuccess && I.lib_err == hipSuccess) I.lib_err = re;
   hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.st, skip);
   if (t < 2) return;
   hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.range_lo,
@@ -630,10 +642,19 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
   hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
   size_t sbytes = I.scan_bytes;
-  (void)rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t, rocprim::plus<unsigned int>(),
-                                c->stream);
+  const hipError_t se = rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t,
+                                                rocprim::plus<unsigned int>(), c->stream);
+  if (se != hipSuccess && I.lib_err == hipSuccess) I.lib_err = se;
   hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
+}
+
+// The first library error of the launches enqueued since the last call
+// (hipSuccess if none), then clears it.
+hipError_t take_lib_err(vh_ctx* c) {
+  const hipError_t e = c->ipc.lib_err;
+  c->ipc.lib_err = hipSuccess;
+  return e;
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
@@ -669,7 +690,7 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   const bool f64 = c->prec == VH_PREC_F64;
   const vh_ipc_params& Q = I.P;
   const int* skip = ipc_done_flag(c);
-  const double max_vel = 0.5 * std::sqrt(I.sc.dh) / dt;
+  const double max_vel = 0.5 * std::sqrt(I.cfl_dh) / dt;
   hipLaunchKernelGGL(vhc::k_ipc_begin, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->v, c->prev, c->pred,
                      c->pinned, I.ever, dt, P.gravity[0], P.gravity[1], P.gravity[2], max_vel, Q.has_ground,
                      Q.ground_height, Q.al_mu_initial, I.ip, c->st);
@@ -728,6 +749,8 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   f.ground = Q.ground_height; f.adaptive = Q.adaptive_contact_damping; f.has_ground = Q.has_ground;
   hipLaunchKernelGGL(vhc::k_ipc_finalize, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->prev, c->v, c->pinned,
                      I.ever, I.fnrm, f, I.ip);
+  const hipError_t le = take_lib_err(c);
+  if (le != hipSuccess) return le;
   return hipGetLastError();
 }
 
@@ -948,6 +971,9 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   if (!S.hinge_st.empty()) UP(upload_t(ctx, &ctx->hinge_st, S.hinge_st));
   UP(dalloc_bytes(ctx, &ctx->cslot, 3 * (size_t)std::max(1, S.inc_ptr[S.n]) * ctx->tsz));
   UP(upload(ctx, &ctx->inc_ptr, S.inc_ptr));
+  // Task records hold block and root-inverse offsets as 32-bit integers.
+  if (S.m_total > INT32_MAX || (int64_t)S.Sroot.size() > INT32_MAX || S.fslot_total > INT32_MAX)
+    return fail(ctx, VH_E_UNSUPPORTED, "solve blocks exceed 2^31 entries (split the batch)");
   std::vector<vhk::TaskDev> td(S.ptasks.size());
   for (size_t k = 0; k < td.size(); ++k) {
     const vh::Task& T = S.ptasks[k];
@@ -998,7 +1024,9 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
         f64 ? (const void*)vhk::k_root_band<double, false, false> : (const void*)vhk::k_root_band<float, false, false>,
         f64 ? (const void*)vhk::k_root_band<double, false, true> : (const void*)vhk::k_root_band<float, false, true>,
         f64 ? (const void*)vhk::k_root_band<double, true, false> : (const void*)vhk::k_root_band<float, true, false>,
-        f64 ? (const void*)vhk::k_root_band<double, true, true> : (const void*)vhk::k_root_band<float, true, true>};
+        f64 ? (const void*)vhk::k_root_band<double, true, true> : (const void*)vhk::k_root_band<float, true, true>,
+        f64 ? (const void*)vhk::k_fwd_band_d5 : (const void*)vhk::k_fwd_band_w8,
+        f64 ? (const void*)vhk::k_bwd_band_d5 : (const void*)vhk::k_bwd_band<float, false>};
     for (const void* k : ks) VH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
   // Initial state (state.rs:119-139): pos = prev = mesh positions, vel = 0.
@@ -1375,7 +1403,10 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   I.sc.radius = Q.sphere_radius;
   I.sc.self_collision = Q.self_collision;
   I.sc.thickness = Q.thickness;
-  I.sc.dh = Q.compliant_contact ? Q.d_hat * Q.compliant_d_hat_scale : Q.d_hat;
+  // d_hat_effective (pd_solver.rs:612-618) sets the CFL clamp; detection
+  // uses it too unless the handler ignores set_d_hat (detect_keeps_d_hat).
+  I.cfl_dh = Q.compliant_contact ? Q.d_hat * Q.compliant_d_hat_scale : Q.d_hat;
+  I.sc.dh = Q.detect_keeps_d_hat ? Q.d_hat : I.cfl_dh;
   I.sc.kappa = Q.kappa;
   // Exclusion sets (exclusion.rs:22-41): v plus `depth` one-ring expansions,
   // in the device's vertex labels.
@@ -1526,6 +1557,7 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   detect();
   VH_HIP(hipGetLastError());
   if (sc.self_collision) VH_HIP(fit_pairs(ctx, detect));
+  VH_HIP(take_lib_err(ctx));
   const size_t n3 = 3 * (size_t)n;
   std::vector<double> g(n3), nr(n3), hd(n);
   std::vector<uint8_t> ic(n);
@@ -1572,6 +1604,7 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   ccd();
   VH_HIP(hipGetLastError());
   if (I.sc.self_collision) VH_HIP(fit_pairs(ctx, ccd));
+  VH_HIP(take_lib_err(ctx));
   vhc::IpcDev h{};
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&h, I.ip, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -1605,6 +1638,7 @@ vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
   cross();
   VH_HIP(hipGetLastError());
   VH_HIP(fit_pairs(ctx, cross));
+  VH_HIP(take_lib_err(ctx));
   vhc::BvhDev h{};
   VH_HIP(hipMemcpyAsync(&h, I.bd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
@@ -1628,6 +1662,7 @@ vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* 
   enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr);
   VH_HIP(hipGetLastError());
   VH_HIP(fit_pairs(ctx, [&] { enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr); }));
+  VH_HIP(take_lib_err(ctx));
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));

@@ -262,6 +262,41 @@ __device__ double vt_ccd(D3 p0, D3 p1, D3 a0, D3 a1, D3 b0, D3 b1, D3 c0, D3 c1)
   return best;
 }
 
+// ccd.rs:138-229: smallest safe TOI (0.9 t, at least 1e-6) of edge (a, b)
+// against edge (c, d), or 2 (no hit). The coplanarity cubic of
+// (c - a) . ((b - a) x (d - c)); at each root in [-1e-6, 1 + 1e-6] the closest
+// points' parameters are clamped to [0, 1] before the 0.05 tolerance test, as
+// the reference does, so every root with non-degenerate, non-parallel edges
+// is a hit.
+__device__ double ee_ccd(D3 a0, D3 a1, D3 b0, D3 b1, D3 c0, D3 c1, D3 d0, D3 d1) {
+  const D3 da = a1 - a0, db = b1 - b0, dc = c1 - c0, dd = d1 - d0;
+  const D3 r0 = c0 - a0, rv = dc - da, e10 = b0 - a0, e1v = db - da, e20 = d0 - c0, e2v = dd - dc;
+  const D3 vv = cross(e1v, e2v), rvx = cross(e10, e2v), vrx = cross(e1v, e20), rr = cross(e10, e20);
+  const double ca = dot(rv, vv);
+  const double cb = dot(r0, vv) + dot(rv, rvx + vrx);
+  const double cc = dot(r0, rvx + vrx) + dot(rv, rr);
+  const double cd = dot(r0, rr);
+  double roots[3];
+  const int nr = cubic_roots(ca, cb, cc, cd, roots);
+  double best = 2.0;
+  for (int k = 0; k < nr; ++k) {
+    double t = roots[k];
+    if (t < -1e-6 || t > 1.0 + 1e-6) continue;
+    t = clamp01(t);
+    const D3 at = a0 + da * t, bt = b0 + db * t, ct = c0 + dc * t, dtt = d0 + dd * t;
+    const D3 e1 = bt - at, e2 = dtt - ct, r = ct - at;
+    const double l1 = dot(e1, e1), l2 = dot(e2, e2);
+    if (l1 < 1e-20 || l2 < 1e-20) continue;
+    const double bb = dot(e1, e2), ff = dot(e2, r), c2 = dot(e1, r);
+    const double den = l1 * l2 - bb * bb;
+    if (fabs(den) < 1e-10) continue;
+    const double sp = clamp01((bb * ff - c2 * l2) / den);
+    const double up = clamp01((bb * sp + ff) / l2);
+    if (sp >= -0.05 && sp <= 1.05 && up >= -0.05 && up <= 1.05) best = fmin(best, fmax(t * 0.9, 1e-6));
+  }
+  return best;
+}
+
 // ---- LBVH (bvh.rs semantics: f32 boxes, closed-interval overlap) ----------
 
 struct Box {
@@ -1096,10 +1131,14 @@ __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __rest
 }
 
 // Self part of compute_ccd_step (ccd.rs:238-292) over the candidate pairs of
-// the last detection, from x to x + delta.
+// the last detection, from x to x + delta; with EE, also edge_edge_ccd
+// (ccd.rs:138-229) on the nine edge pairs of each triangle pair.
 //
 // Pair cull (exact): the cubic of a vertex-triangle test has coefficients
-// |ca| <= W^3, |cb| <= 3 D W^2, |cc| <= 3 D^2 W (D, W from CcdTri). When all
+// |ca| <= W^3, |cb| <= 3 D W^2, |cc| <= 3 D^2 W (D, W from CcdTri). An
+// edge-edge cubic has the same bounds with W = 2 max(m_A, m_B), since both
+// ends of one edge move (|db - da| <= 2 m_A); the cull then uses that W,
+// which also bounds the vertex-triangle tests (m_A + m_B <= it). When all
 // three are below half of the solver's 1e-8 threshold, cubic_roots falls
 // through quadratic_roots to the linear case and finds no root for any test
 // of the pair (the computed coefficients differ from the bounded ones by a
@@ -1117,7 +1156,7 @@ __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __rest
 __global__ void __launch_bounds__(256) k_ccd_cull(const int2* __restrict__ pairs, const unsigned int* npairs_p,
                                                   unsigned int cap, const CcdTri* __restrict__ tb_,
                                                   int2* __restrict__ surv, unsigned int reg,
-                                                  unsigned int* __restrict__ scnt, const IpcDev* ip) {
+                                                  unsigned int* __restrict__ scnt, const IpcDev* ip, int ee) {
   if (ip->al_done || ip->pd_done) return;
   const unsigned int np = *npairs_p < cap ? *npairs_p : cap;
   const unsigned int n = cull_to_region(pairs, np, [&](const int2 pr) {
@@ -1125,13 +1164,14 @@ __global__ void __launch_bounds__(256) k_ccd_cull(const int2* __restrict__ pairs
     const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
                  gz = (double)A.c[2] - (double)B.c[2];
     const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
-    const double W = (double)A.m + (double)B.m;
+    const double W = ee ? 2.0 * fmax((double)A.m, (double)B.m) : (double)A.m + (double)B.m;
     constexpr double kHalfEps = 0.5e-8;
     return !(W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps);
   }, surv + (size_t)blockIdx.x * reg);
   if (threadIdx.x == 0) scnt[blockIdx.x] = n;
 }
 
+template <bool EE>
 __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ surv, unsigned int reg,
                                                    const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
                                                    const double* __restrict__ x, const double* __restrict__ dl,
@@ -1162,6 +1202,16 @@ __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ surv
         if (V[vi] == V[tb] || V[vi] == V[tb + 1] || V[vi] == V[tb + 2]) continue;
         best = fmin(best, vt_ccd(P0[vi], P1[vi], P0[tb], P1[tb], P0[tb + 1], P1[tb + 1], P0[tb + 2], P1[tb + 2]));
       }
+    }
+    if constexpr (EE) {
+      // edges (0,1), (1,2), (2,0) of A against those of B, skipping pairs
+      // that share a vertex
+      for (int ea = 0; ea < 3; ++ea)
+        for (int eb = 0; eb < 3; ++eb) {
+          const int a0 = ea, a1 = ea == 2 ? 0 : ea + 1, b0 = 3 + eb, b1 = eb == 2 ? 3 : 4 + eb;
+          if (V[a0] == V[b0] || V[a0] == V[b1] || V[a1] == V[b0] || V[a1] == V[b1]) continue;
+          best = fmin(best, ee_ccd(P0[a0], P1[a0], P0[a1], P1[a1], P0[b0], P1[b0], P0[b1], P1[b1]));
+        }
     }
   }
   for (int o = 32; o > 0; o >>= 1) best = fmin(best, __shfl_xor(best, o));

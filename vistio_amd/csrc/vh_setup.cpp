@@ -515,12 +515,13 @@ bool build_setup(const SetupInput& in, Setup& S) {
       }
       int k = 0;
       for (int j = 0; j < m; ++j) k = std::max(k, (int)lists[j].size());
+      // fm_off and the slot numbers below are 32-bit on the device.
+      if (slot + (int64_t)m * k > INT32_MAX) { S.status = VH_E_UNSUPPORTED; S.err = "front slots exceed 2^31"; return false; }
       x.fm_off = (int)slot;
       x.fslot_k = k;
       for (int j = 0; j < m; ++j)
         for (size_t q = 0; q < lists[j].size(); ++q) S.uslot[lists[j][q]] = (int32_t)(slot + (int64_t)j * k + q);
       slot += (int64_t)m * k;
-      if (slot >= ((int64_t)1 << 31)) { S.status = VH_E_UNSUPPORTED; S.err = "front slots exceed 2^31"; return false; }
     }
     S.fslot_total = slot;
     std::vector<int32_t> cnt(n + 1, 0);
@@ -767,8 +768,19 @@ bool build_setup(const SetupInput& in, Setup& S) {
     for (int c = 0; c < S.sn[s].nchild; ++c) S.need_f[s] += nfc[S.children[S.sn[s].child_off + c]];
     S.need_b[s] = S.sn[s].parent >= 0 ? nbc[S.sn[s].parent] : nfc[s];
   }
+  const std::string why = device_index_overflow(S.m_total, (int64_t)S.Sroot.size(), S.fslot_total,
+                                                3 * ((int64_t)3 * t + (int64_t)4 * h));
+  if (!why.empty()) { S.status = VH_E_UNSUPPORTED; S.err = why; return false; }
   S.status = VH_OK;
   return true;
+}
+
+std::string device_index_overflow(int64_t m_total, int64_t sroot_entries, int64_t front_slots, int64_t rhs_values) {
+  if (m_total > INT32_MAX) return "solve blocks exceed 2^31 entries (split the batch)";
+  if (sroot_entries > INT32_MAX) return "root inverses exceed 2^31 entries (split the batch)";
+  if (front_slots > INT32_MAX) return "front slots exceed 2^31";
+  if (rhs_values > INT32_MAX) return "RHS slot values exceed 2^31 (split the batch)";
+  return std::string();
 }
 
 }  // namespace vh

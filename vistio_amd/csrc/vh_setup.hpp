@@ -149,4 +149,11 @@ struct Setup {
 // Builds everything. Returns false and sets err/status on failure.
 bool build_setup(const SetupInput& in, Setup& out);
 
+// The device kernels hold block offsets (task records), root-inverse
+// offsets, front-slot numbers and RHS-slot value indices (3 per element
+// corner) as 32-bit integers. Returns the reason when one of these totals
+// does not fit, else an empty string; build_setup refuses such a system
+// with VH_E_UNSUPPORTED.
+std::string device_index_overflow(int64_t m_total, int64_t sroot_entries, int64_t front_slots, int64_t rhs_values);
+
 }  // namespace vh

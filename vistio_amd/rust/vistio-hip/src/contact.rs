@@ -31,7 +31,24 @@ pub struct IpcScene {
     pub exclusion_depth: i32,
     /// Broad-phase margin and self-contact thickness (runner: 1.5 * barrier_d_hat).
     pub thickness: f32,
+    /// The handler's `set_d_hat` is the trait's default no-op (pd_solver.rs:488),
+    /// as in bench_tests.rs:181-199: detection keeps `barrier_d_hat` while the
+    /// CFL clamp uses the compliant-scaled one. The runner's handler follows
+    /// set_d_hat (runner.rs:34-36): false.
+    pub handler_ignores_set_d_hat: bool,
+    /// The CCD step also runs `edge_edge_ccd` (ccd.rs:138-229) on the nine
+    /// edge pairs of every candidate triangle pair. false: vertex-triangle
+    /// only, as `compute_max_step_size` (ccd.rs:238-292).
+    pub ccd_edge_edge: bool,
 }
+
+/// The handler argument of [`HipPdSolver::step_with_ipc`]: the scene that
+/// `init_ipc` registered on the device. Detection and CCD run inside the
+/// device step, which cannot call back into a host `IpcCollisionHandler`;
+/// taking this marker instead of any handler makes a caller's own handler a
+/// compile error rather than an argument that would be silently ignored.
+#[derive(Debug, Clone, Copy, Default)]
+pub struct DeviceScene;
 
 impl IpcScene {
     /// Ground, sphere, self-collision and thickness of a pipeline. Cylinder
@@ -49,6 +66,8 @@ impl IpcScene {
             self_collision: p.self_collision.is_some(),
             exclusion_depth,
             thickness: p.thickness,
+            handler_ignores_set_d_hat: false,
+            ccd_edge_edge: false,
         })
     }
 }
@@ -90,10 +109,13 @@ impl HipPdSolver {
             contact_damping_max: config.contact_damping_max as f64,
             contact_velocity_threshold: config.contact_velocity_threshold as f64,
             max_pairs: 0,
+            detect_keeps_d_hat: scene.handler_ignores_set_d_hat as i32,
+            ccd_edge_edge: scene.ccd_edge_edge as i32,
         };
         self.ipc_ready = false;
         self.ctx.check(unsafe { vh_ipc_init(self.ctx.0, &q) })?;
         self.ipc_ready = true;
+        self.ipc_ground = scene.ground;
         Ok(())
     }
 
@@ -107,17 +129,26 @@ impl HipPdSolver {
     }
 
     /// step_with_ipc (pd_solver.rs:549-1157). Detection, CCD and the AL loop
-    /// all run on the device with the scene given to `init_ipc`, so no
-    /// callback crosses the FFI and `handler` is not called: it stays in the
-    /// signature so that call sites read as they do for the CPU solver.
-    /// state.al_lambda_{x,y,z} carry the multipliers in and out.
-    pub fn step_with_ipc<H: IpcCollisionHandler>(
+    /// all run on the device with the scene given to `init_ipc`, so the
+    /// handler is the [`DeviceScene`] marker (a caller's own
+    /// IpcCollisionHandler does not type-check). The predict clamp and the
+    /// ground-velocity fix use the scene's ground, so a `state.ground_height`
+    /// that differs from it is refused (the reference reads the state's,
+    /// state.rs:160-212). state.al_lambda_{x,y,z} carry the multipliers in
+    /// and out.
+    pub fn step_with_ipc(
         &mut self,
         state: &mut SimulationState,
         dt: f32,
-        _handler: &mut H,
+        _scene: &mut DeviceScene,
     ) -> VistioResult<StepResult> {
         self.require_ipc()?;
+        if state.ground_height != self.ipc_ground {
+            return Err(VistioError::InvalidConfig(format!(
+                "state.ground_height {:?} differs from the contact scene's ground {:?} given to init_ipc",
+                state.ground_height, self.ipc_ground
+            )));
+        }
         let n = self.n;
         if state.al_lambda_x.len() != n || state.al_lambda_y.len() != n || state.al_lambda_z.len() != n {
             return Err(VistioError::InvalidMesh(format!("al_lambda arrays must have {n} entries")));

@@ -168,6 +168,8 @@ pub struct VhIpcParams {
     pub contact_damping_max: f64,
     pub contact_velocity_threshold: f64,
     pub max_pairs: u32,
+    pub detect_keeps_d_hat: i32,
+    pub ccd_edge_edge: i32,
 }
 
 /// `vh_ipc_stats`.

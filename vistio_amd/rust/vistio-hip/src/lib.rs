@@ -20,5 +20,5 @@ pub mod ffi;
 pub mod solver;
 
 pub use backend::HipBackend;
-pub use contact::{HipBroadPhase, HipIpcHandler, IpcScene};
+pub use contact::{DeviceScene, HipBroadPhase, HipIpcHandler, IpcScene};
 pub use solver::{HipPdSolver, HipSettings};

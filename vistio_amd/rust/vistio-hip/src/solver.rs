@@ -86,6 +86,8 @@ pub struct HipPdSolver {
     pub(crate) initialized: bool,
     ground: Option<f32>,
     pub(crate) ipc_ready: bool,
+    /// Ground of the contact scene given to init_ipc.
+    pub(crate) ipc_ground: Option<f32>,
 }
 
 impl HipPdSolver {
@@ -103,6 +105,7 @@ impl HipPdSolver {
             initialized: false,
             ground: None,
             ipc_ready: false,
+            ipc_ground: None,
         })
     }
 

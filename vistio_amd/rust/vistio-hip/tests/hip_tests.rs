@@ -7,11 +7,11 @@
 use vistio_contact::broad::BroadPhase;
 use vistio_contact::BvhBroadPhase;
 use vistio_gpu::{ComputeBuffer, CpuFallback, GpuBackend};
-use vistio_hip::{HipBackend, HipBroadPhase, HipPdSolver, HipSettings, IpcScene};
+use vistio_hip::{DeviceScene, HipBackend, HipBroadPhase, HipPdSolver, HipSettings, IpcScene};
 use vistio_material::{CoRotationalModel, FabricProperties};
 use vistio_mesh::{generators::quad_grid, topology::Topology};
 use vistio_solver::{
-    config::SolverConfig, pd_solver::EmptyIpcHandler, state::SimulationState, strategy::SolverStrategy,
+    config::SolverConfig, state::SimulationState, strategy::SolverStrategy,
     ProjectiveDynamicsSolver,
 };
 
@@ -140,8 +140,11 @@ fn hip_step_with_ipc_rests_on_the_ground() {
     let mut st = SimulationState::from_mesh_with_masses(&mesh, gpu.lumped_masses(), &pinned).unwrap();
     st.ground_height = Some(0.0);
     for _ in 0..60 {
-        gpu.step_with_ipc(&mut st, 1.0 / 60.0, &mut EmptyIpcHandler).unwrap();
+        gpu.step_with_ipc(&mut st, 1.0 / 60.0, &mut DeviceScene).unwrap();
     }
     assert!(st.pos_y.iter().all(|&y| y.is_finite() && y > -1e-3));
     assert!(st.pos_y.iter().any(|&y| y < 0.01));
+    // the device step clamps to the scene's ground, so another state ground is refused
+    st.ground_height = Some(0.5);
+    assert!(gpu.step_with_ipc(&mut st, 1.0 / 60.0, &mut DeviceScene).is_err());
 }

@@ -258,6 +258,11 @@ class IpcConfig:
     contact_damping_max: float = 0.5
     contact_velocity_threshold: float = 0.1
     max_pairs: int = 0                        # initial pair buffer, grows on demand; 0: 32 per triangle
+    # The handler ignores set_d_hat (the trait's no-op, pd_solver.rs:488):
+    # detection keeps barrier_d_hat, the CFL clamp uses the compliant one.
+    detect_keeps_d_hat: bool = False
+    # The CCD step adds edge_edge_ccd (ccd.rs:138-229) on each candidate pair.
+    ccd_edge_edge: bool = False
 
 
 def estimate_initial_kappa(configured: float, avg_mass: float, gravity_mag: float, d_hat: float) -> float:
@@ -468,6 +473,8 @@ class HipPdSolver:
         q.adaptive_contact_damping = int(cfg.adaptive_contact_damping)
         q.contact_damping_max, q.contact_velocity_threshold = cfg.contact_damping_max, cfg.contact_velocity_threshold
         q.max_pairs = int(cfg.max_pairs)
+        q.detect_keeps_d_hat = int(bool(cfg.detect_keeps_d_hat))
+        q.ccd_edge_edge = int(bool(cfg.ccd_edge_edge))
         _check(self._L.vh_ipc_init(self._ctx, C.byref(q)), self._ctx, "init_ipc")
 
     def step_with_ipc(self, state: SimulationState, dt: float) -> StepResult:
