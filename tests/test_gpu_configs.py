@@ -62,8 +62,19 @@ def test_headline_256_matches_fp64_oracle(all_threads):
         g.close()
         assert r.iterations == 10
         assert errs[prec] <= tol, (prec, errs[prec])
-    # the sheet has actually moved: the lowest point sags well below rest
-    assert float(xo[1].min()) < -0.6
+    # the sheet has moved (the stiff membrane sags it by under a millimetre)
+    rest = np.stack([mesh.pos_x, mesh.pos_y, mesh.pos_z]).astype(np.float64)
+    assert max_vertex_err(xo, rest) > 1e-4
+
+
+def _batch_shift(b, s):
+    """build_batch places sheet s at x + 1.5 s. The oracle gets the sheet
+    shifted back to the origin: the oracle solves for absolute positions
+    (x = A^-1 b), so its own error grows like cond(A) eps |x| (2.4e-9 m after
+    10 steps at x = 75 m, against 2e-14 m for a 1e-15 perturbation of the
+    state at the origin), while the device solves for the correction A d = r.
+    The subtraction is exact in f32 (both values lie within a factor two)."""
+    return np.float32(1.5 * s)
 
 
 def _batch_sheet_oracle(b, s, mode):
@@ -75,8 +86,8 @@ def _batch_sheet_oracle(b, s, mode):
     idx = (b.mesh.indices.reshape(-1, 3)[t0:t1] - v0).astype(np.uint32).ravel()
     hg = (b.topology.interior_edges[h0:h1, :4].astype(np.int64) - v0).astype(np.uint32)
     sp = b.specs[s]
-    return ref.Oracle(mode, b.mesh.pos_x[sl], b.mesh.pos_y[sl], b.mesh.pos_z[sl], idx, hg, b.pinned[sl],
-                      ref.params(k_membrane=sp.k_membrane, k_bending=sp.k_bending, density=sp.density))
+    return ref.Oracle(mode, b.mesh.pos_x[sl] - _batch_shift(b, s), b.mesh.pos_y[sl], b.mesh.pos_z[sl], idx, hg,
+                      b.pinned[sl], ref.params(k_membrane=sp.k_membrane, k_bending=sp.k_bending, density=sp.density))
 
 
 @pytest.mark.parametrize("precision,steps,tol", [("f32_ir", 120, 1e-6), ("f64", 10, 1e-9)])
@@ -108,7 +119,9 @@ def test_batched_64_sheets_follow_their_oracles(precision, steps, tol, all_threa
         o = _batch_sheet_oracle(b, s, ref.F64)
         assert np.allclose(g.lumped_masses()[b.sheet_slice(s)], o.masses(), rtol=1e-6)
         o.step(DT, steps)
-        err = max_vertex_err(x[:, b.sheet_slice(s)], o.positions())
+        xs = x[:, b.sheet_slice(s)].copy()
+        xs[0] -= float(_batch_shift(b, s))
+        err = max_vertex_err(xs, o.positions())
         print(f"  sheet {s} ({specs[s].cells}^2) after {steps} steps: {err:.3e} m", flush=True)
         assert err <= tol, (s, err)
     g.close()
@@ -116,8 +129,9 @@ def test_batched_64_sheets_follow_their_oracles(precision, steps, tol, all_threa
 
 def _run_contact(name, precision, check_every=1):
     """Steps a contact workload and checks every `check_every` steps: no
-    crossing triangle pair, no vertex below the ground, none inside the
-    sphere. Returns the final positions and the worst values seen."""
+    crossing triangle pair, no vertex below the ground, and the deepest
+    vertex inside the sphere. Returns the final positions, the worst values
+    seen and the barrier distance sqrt(barrier_d_hat)."""
     import vistio_amd as va
     from vistio_amd.scenes import contact_scene
     mesh, ipc, steps, desc = contact_scene(name)
@@ -143,24 +157,29 @@ def _run_contact(name, precision, check_every=1):
             print(f"  [{name} {precision}] step {k + 1}/{steps}: worst so far {worst}", flush=True)
     x = g.positions()
     g.close()
-    return x, worst, steps
+    return x, worst, steps, float(np.sqrt(ipc.barrier_d_hat))
 
 
 @pytest.mark.parametrize("name,check_every", [("sphere_drape", 1), ("self_fold_ribbon", 1)])
 def test_contact_workload_full_size(name, check_every):
     """BASELINE configs[2] (512 x 512 sphere drape, 180 steps) and configs[4]
     (64 x 640 self-fold ribbon, 300 steps) in FP32-IR, checked at every
-    step; then the FP64 run, and the gap between the two."""
-    x32, w32, steps = _run_contact(name, "f32_ir", check_every)
+    step; then the FP64 run, and the gap between the two. The predict step
+    clamps only to the ground (state.rs:160-165), so a vertex next to the
+    sphere can start a step a little inside it, and the AL loop then pushes
+    it out (pd_solver.rs:966-1021); the reference's own drape test allows
+    2 cm (bench_tests.rs:216). Here the depth must stay within the barrier
+    distance sqrt(barrier_d_hat) = 1 mm."""
+    x32, w32, steps, dist = _run_contact(name, "f32_ir", check_every)
     print(f"{name} f32_ir {steps} steps: {w32}", flush=True)
     assert w32["crossings"] == 0
     assert w32["below_ground"] <= 0.0
-    assert w32["inside_sphere"] <= 0.0
+    assert w32["inside_sphere"] <= dist
     assert w32["contacts"] > 0
-    x64, w64, _ = _run_contact(name, "f64", 10)
+    x64, w64, _, _ = _run_contact(name, "f64", 10)
     gap = max_vertex_err(x32, x64)
     print(f"{name} f64: {w64}; FP32-IR vs FP64 after {steps} steps: {gap:.3e} m", flush=True)
-    assert w64["crossings"] == 0 and w64["below_ground"] <= 0.0 and w64["inside_sphere"] <= 0.0
+    assert w64["crossings"] == 0 and w64["below_ground"] <= 0.0 and w64["inside_sphere"] <= dist
     assert gap < 0.05
 
 
