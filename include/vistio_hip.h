@@ -348,6 +348,18 @@ vh_status vh_backend_axpy(vh_ctx* ctx, const float* a, uint64_t na, const float*
 vh_status vh_backend_dot(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb, double* result);
 vh_status vh_backend_fill(vh_ctx* ctx, float* buf, uint64_t n, float value);
 
+/* vh_backend_dot without any device allocation. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc). Same arguments, result and
+ * errors as vh_backend_dot, and the same kernels, so the same value bit for
+ * bit. Its partial sums and operand copies live in one fixed device array
+ * per GPU (2^21 floats per operand plus 8.2 KB, about 16 MB, held until the
+ * process ends); more than 2^21 elements returns VH_E_INVALID_ARG. Every call
+ * on one GPU uses that GPU's array, whichever context makes it: calls that
+ * overlap on one GPU (other threads, other contexts) race. */
+vh_status vh_backend_dot_noalloc(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb,
+                                 double* result);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,7 @@ EXPORTS = (
     "vh_pd_info_get", "vh_pd_lumped_masses", "vh_mesh_quad_grid", "vh_mesh_interior_edges", "vh_pd_profile",
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
     "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings", "vh_pd_init_ex",
-    "vh_backend_axpy", "vh_backend_dot", "vh_backend_fill",
+    "vh_backend_axpy", "vh_backend_dot", "vh_backend_fill", "vh_backend_dot_noalloc",
 )
 
 
@@ -131,6 +131,7 @@ def load():
                            C.POINTER(PdParams), C.POINTER(MaterialArrays), fp], C.c_int),
         "vh_backend_axpy": ([vp, fp, C.c_uint64, fp, C.c_uint64, C.c_float, fp, C.c_uint64], C.c_int),
         "vh_backend_dot": ([vp, fp, C.c_uint64, fp, C.c_uint64, dp], C.c_int),
+        "vh_backend_dot_noalloc": ([vp, fp, C.c_uint64, fp, C.c_uint64, dp], C.c_int),
         "vh_backend_fill": ([vp, fp, C.c_uint64, C.c_float], C.c_int),
         "vh_pd_step": ([vp, C.c_double, C.POINTER(StateF32), C.POINTER(StepResultC)], C.c_int),
         "vh_pd_step_resident": ([vp, C.c_double, C.c_uint32, C.POINTER(StepResultC)], C.c_int),

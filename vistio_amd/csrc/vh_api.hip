@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <type_traits>
@@ -507,6 +508,36 @@ unsigned int backend_blocks(size_t n) {
   return (unsigned int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, kBackendBlocks));
 }
 
+// ---- allocation-free dot product ---------------------------------------------
+// vh_backend_dot_noalloc keeps everything its kernels touch beyond the
+// caller's inputs and result in one file-scope device array per GPU: the
+// partial sums (kBackendBlocks + 1 doubles, padded to 256 bytes), then room
+// for the two operands (kNoAllocMaxElems floats each). Host code never
+// passes the array's name to a kernel: vh_create looks up its address on the
+// device (hipGetSymbolAddress) once per device and keeps it in
+// g_noalloc_scratch; every use goes through that address.
+constexpr uint64_t kNoAllocMaxElems = 1ull << 21;  // 2 Mi floats per operand
+constexpr size_t kNoAllocPartBytes = ((kBackendBlocks + 1) * sizeof(double) + 255) / 256 * 256;
+constexpr size_t kNoAllocBytes = kNoAllocPartBytes + 2 * kNoAllocMaxElems * sizeof(float);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+constexpr int kMaxDevices = 64;
+void* g_noalloc_scratch[kMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` (the device is current). Called
+// by vh_create; later calls for the same device return at once.
+hipError_t noalloc_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lock(mu);
+  if (g_noalloc_scratch[device]) return hipSuccess;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (!p) return hipErrorInvalidSymbol;
+  g_noalloc_scratch[device] = p;
+  return hipSuccess;
+}
+
 // ---- contact step -------------------------------------------------------
 
 // Blocks of the grid-stride kernels over candidate pairs: enough to fill the
@@ -830,6 +861,11 @@ vh_status vh_create(int32_t device, vh_ctx** out) {
   ctx->device = device;
   if (const char* g = getenv("VISTIO_HIP_GRAPH")) ctx->use_graph = atoi(g) != 0;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete ctx;
+    return VH_E_GPU;
+  }
+  if (noalloc_setup(device) != hipSuccess) {
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VH_E_GPU;
   }
@@ -1729,6 +1765,36 @@ vh_status vh_backend_dot(vh_ctx* ctx, const float* a, uint64_t na, const float* 
   hipLaunchKernelGGL(k_backend_dot_sum, dim3(1), dim3(256), 0, ctx->stream, (int)nbk, ctx->bk_part);
   VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(result, ctx->bk_part + kBackendBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_backend_dot_noalloc(vh_ctx* ctx, const float* a, uint64_t na, const float* b, uint64_t nb,
+                                 double* result) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!result) return fail(ctx, VH_E_INVALID_ARG, "null result");
+  if (na != nb) return fail(ctx, VH_E_INVALID_ARG, "Dot product buffer length mismatch");
+  if (na > kNoAllocMaxElems)
+    return fail(ctx, VH_E_INVALID_ARG,
+                "vh_backend_dot_noalloc: " + std::to_string(na) + " elements exceed its maximum of " +
+                    std::to_string(kNoAllocMaxElems) + " (use vh_backend_dot)");
+  *result = 0.0;
+  if (na == 0) return VH_OK;
+  if (!a || !b) return fail(ctx, VH_E_INVALID_ARG, "null buffer");
+  VH_HIP(hipSetDevice(ctx->device));
+  unsigned char* base = static_cast<unsigned char*>(g_noalloc_scratch[ctx->device]);
+  if (!base) return fail(ctx, VH_E_INTERNAL, "scratch address not set up for this device");
+  double* part = reinterpret_cast<double*>(base);
+  float* da = reinterpret_cast<float*>(base + kNoAllocPartBytes);
+  float* db = da + kNoAllocMaxElems;
+  VH_HIP(hipMemcpyAsync(da, a, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  VH_HIP(hipMemcpyAsync(db, b, na * 4, hipMemcpyHostToDevice, ctx->stream));
+  const unsigned int nbk = backend_blocks(na);
+  // every partial the sum reads is written by this call's first kernel
+  hipLaunchKernelGGL(k_backend_dot, dim3(nbk), dim3(256), 0, ctx->stream, (size_t)na, da, db, part);
+  hipLaunchKernelGGL(k_backend_dot_sum, dim3(1), dim3(256), 0, ctx->stream, (int)nbk, part);
+  VH_HIP(hipGetLastError());
+  VH_HIP(hipMemcpyAsync(result, part + kBackendBlocks, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
   return VH_OK;
 }

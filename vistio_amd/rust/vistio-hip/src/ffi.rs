@@ -303,4 +303,6 @@ extern "C" {
     pub fn vh_backend_dot(ctx: *mut VhCtx, a: *const f32, na: u64, b: *const f32, nb: u64, result: *mut f64)
         -> VhStatus;
     pub fn vh_backend_fill(ctx: *mut VhCtx, buf: *mut f32, n: u64, value: f32) -> VhStatus;
+    pub fn vh_backend_dot_noalloc(ctx: *mut VhCtx, a: *const f32, na: u64, b: *const f32, nb: u64, result: *mut f64)
+        -> VhStatus;
 }
