@@ -134,7 +134,8 @@ typedef struct {
   int64_t nnz_a;      /* stored entries of A (full symmetric) */
   int64_t nnz_factor; /* entries of all solve blocks M_P (= supernodal L) */
   int32_t max_front;  /* max (w + r) over supernodes */
-  double setup_seconds;
+  double setup_seconds;   /* host set-up: rest data, matrix, ordering, symbolic structure */
+  double factor_seconds;  /* device factorization into the solve blocks */
 } vh_pd_info;
 
 typedef struct vh_ctx vh_ctx;
@@ -203,6 +204,13 @@ vh_status vh_pd_residual(vh_ctx* ctx, double dt, double* out);
  * In VH_PREC_F32_IR mode runs `sweeps` refinement sweeps (>= 1). */
 vh_status vh_pd_solve(vh_ctx* ctx, const double* b, double* x, int32_t sweeps);
 vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out);
+/* The solve blocks the device factorization built (FaerSolver::factorize,
+ * crates/vistio-math/src/faer_solver.rs:67-93, as partitioned-inverse
+ * blocks), widened to double: which = 0 M (each supernode's
+ * [L_PP^{-1}; L_IP L_PP^{-1}] row-major, lower rows packed), 1 M^T, 2 the
+ * roots' dense inverses (L L^T)^{-1}. Writes up to cap values and the total
+ * count (cap = 0 queries the count). */
+vh_status vh_pd_blocks_get(vh_ctx* ctx, int32_t which, double* out, uint64_t cap, uint64_t* count);
 
 /* Per-kernel timing of PD iterations, measured with HIP events on the
  * context's stream around every launch (no graph). An iteration is the

@@ -1,4 +1,6 @@
+import ctypes
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -60,3 +62,17 @@ def oracle_solver(px, py, pz, idx, hg, pin, mode=0, **over):
 
 def max_vertex_err(a, b):
     return float(np.sqrt(((np.asarray(a) - np.asarray(b)) ** 2).sum(0)).max())
+
+
+@pytest.fixture(scope="session")
+def emul(tmp_path_factory):
+    """tests/native: the product's host set-up (vistio_amd/csrc/vh_setup.cpp)
+    with a CPU replay of the solve schedule and a host copy of the
+    factorization, built with g++ for this test session."""
+    out = tmp_path_factory.mktemp("emul") / "libemul.so"
+    nat = os.path.join(ROOT, "tests", "native")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+                    "-I", os.path.join(ROOT, "vistio_amd", "csrc"), os.path.join(nat, "setup_emul.cpp"),
+                    os.path.join(nat, "host_factor.cpp"), os.path.join(ROOT, "vistio_amd", "csrc", "vh_setup.cpp"),
+                    "-o", str(out)], check=True)
+    return ctypes.CDLL(str(out))

@@ -11,6 +11,8 @@
 
 #include "vh_setup.hpp"
 
+bool host_factor(const vh::Setup& S, std::vector<double>& M, std::vector<double>& Mt, std::vector<double>& Sroot);
+
 extern "C" int emul_solve(int n, const float* px, const float* py, const float* pz, int t, const uint32_t* tri,
                           int h, const uint32_t* hinge, const uint8_t* pinned, double k_mem, double k_bend,
                           double density, double dt_matrix, int leaf, int max_band_levels, const double* b,
@@ -22,6 +24,8 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
   if (max_band_levels > 0) in.max_band_levels = max_band_levels;
   vh::Setup S;
   if (!vh::build_setup(in, S)) return S.status ? S.status : -1;
+  std::vector<double> Mv, Mtv, Srv;
+  if (!host_factor(S, Mv, Mtv, Srv)) return -2;
   std::vector<double> r(3 * (size_t)n), y(3 * (size_t)n), d(3 * (size_t)n), U(3 * (size_t)S.u_total + 3);
   for (int k = 0; k < n; ++k)
     for (int c = 0; c < 3; ++c) r[3 * k + c] = b[(size_t)c * n + S.perm[k]];
@@ -46,9 +50,9 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       for (int j = 0; j < w; ++j)
         for (int k = 0; k < 3; ++k) front[3 * j + k] = r[3 * (size_t)(P.col0 + j) + k] - front[3 * j + k];
       if (T.kind == 2) {  // root: d = Sroot front, pushed to the descendants
-        if (P.r != 0 || P.s_off < 0 || S.Sroot.size() < (size_t)P.s_off + (size_t)w * w) return -13;
+        if (P.r != 0 || P.s_off < 0 || Srv.size() < (size_t)P.s_off + (size_t)w * w) return -13;
         for (int row = T.b; row < T.e; ++row) {
-          const double* Sr = &S.Sroot[(size_t)P.s_off + (size_t)row * w];
+          const double* Sr = &Srv[(size_t)P.s_off + (size_t)row * w];
           double s3[3] = {0, 0, 0};
           for (int kk = 0; kk < w; ++kk)
             for (int k = 0; k < 3; ++k) s3[k] += Sr[kk] * front[3 * kk + k];
@@ -59,7 +63,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
         }
         continue;
       }
-      const double* Mp = &S.M[P.m_off];
+      const double* Mp = &Mv[P.m_off];
       const long long tri_n = (long long)w * (w + 1) / 2;
       for (int row = T.b; row < T.e; ++row) {
         const int len = row < w ? row + 1 : w;
@@ -83,7 +87,7 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
       for (int kk = T.b; kk < m; ++kk)
         for (int k = 0; k < 3; ++k)
           v[3 * kk + k] = kk < w ? y[3 * (size_t)(P.col0 + kk) + k] : V[3 * (size_t)(P.u_off + kk - w) + k];
-      const double* Tp = &S.Mt[P.m_off];
+      const double* Tp = &Mtv[P.m_off];
       for (int j = T.b; j < T.e; ++j) {
         const double* row = Tp + (long long)j * m - (long long)j * (j - 1) / 2;
         double s3[3] = {0, 0, 0};
@@ -111,4 +115,31 @@ extern "C" int emul_solve(int n, const float* px, const float* py, const float* 
 extern "C" int emul_index_fits(long long m_total, long long sroot_entries, long long front_slots,
                                long long rhs_values) {
   return vh::device_index_overflow(m_total, sroot_entries, front_slots, rhs_values).empty() ? 1 : 0;
+}
+
+// The set-up of vh_pd_init_ex's defaults (value_bytes 8 or 4, the band cost
+// model's defaults, per-element materials optional) and the host factor of
+// it: sizes[0] = m_total, sizes[1] = sroot_total; M / Mt / Sr are written
+// when not null (m_total, m_total, sroot_total doubles). Returns 0, or the
+// set-up's status, or -2 when the factor fails.
+extern "C" int emul_factor(int n, const float* px, const float* py, const float* pz, int t, const uint32_t* tri,
+                           int h, const uint32_t* hinge, const uint8_t* pinned, double k_mem, double k_bend,
+                           double density, const double* km, const double* kb, const double* dens, int value_bytes,
+                           double* M, double* Mt, double* Sr, long long* sizes) {
+  vh::SetupInput in;
+  in.n = n; in.t = t; in.h = h; in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge;
+  in.pinned = pinned; in.k_membrane = k_mem; in.k_bending = k_bend; in.density = density;
+  in.k_membrane_tri = km; in.k_bending_hinge = kb; in.density_tri = dens;
+  in.value_bytes = value_bytes;
+  vh::Setup S;
+  if (!vh::build_setup(in, S)) return S.status ? S.status : -1;
+  sizes[0] = S.m_total;
+  sizes[1] = S.sroot_total;
+  if (!M) return 0;
+  std::vector<double> Mv, Mtv, Srv;
+  if (!host_factor(S, Mv, Mtv, Srv)) return -2;
+  std::copy(Mv.begin(), Mv.end(), M);
+  std::copy(Mtv.begin(), Mtv.end(), Mt);
+  std::copy(Srv.begin(), Srv.end(), Sr);
+  return 0;
 }

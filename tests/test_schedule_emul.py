@@ -19,15 +19,6 @@ from oracle import mesh, ref
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-@pytest.fixture(scope="module")
-def emul(tmp_path_factory):
-    out = tmp_path_factory.mktemp("emul") / "libemul.so"
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
-                    "-I", os.path.join(ROOT, "vistio_amd", "csrc"), os.path.join(ROOT, "tests", "native", "setup_emul.cpp"),
-                    os.path.join(ROOT, "vistio_amd", "csrc", "vh_setup.cpp"), "-o", str(out)], check=True)
-    return C.CDLL(str(out))
-
-
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 

@@ -27,6 +27,7 @@
 
 #include "vh_kernels.hpp"
 #include "vh_contact.hpp"
+#include "vh_factor.hpp"
 #include "vh_setup.hpp"
 #include "vistio_hip.h"
 
@@ -79,7 +80,8 @@ struct vh_ctx {
   bool initialized = false;
   vh_pd_params P{};
   vh::Setup S;
-  double setup_seconds = 0;
+  double setup_seconds = 0;   // host set-up (symbolic)
+  double factor_seconds = 0;  // device factorization
   int prec = VH_PREC_F64;
   size_t tsz = 8;  // bytes of the compute type
   // state, FP64 [n][3]
@@ -846,6 +848,186 @@ vh_status read_step_result(vh_ctx* ctx, vh_step_result* out) {
   return VH_OK;
 }
 
+// FaerSolver::factorize (faer_solver.rs:67-93) on the device: the level
+// loop of vh_factor.hpp, deepest level first, writing M, Mt and Sroot in the
+// compute type. A level's fronts live until its parent level has assembled
+// them. Synchronizes; VH_E_NOT_SPD when a pivot is not positive.
+struct FactorTemps {
+  std::vector<void*> p;
+  ~FactorTemps() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  void* take(void* q) {
+    p.push_back(q);
+    return q;
+  }
+  void drop(void* q) {
+    auto it = std::find(p.begin(), p.end(), q);
+    if (it != p.end()) p.erase(it);
+    (void)hipFree(q);
+  }
+};
+
+vh_status device_factor(vh_ctx* ctx) {
+  const vh::Setup& S = ctx->S;
+  const int ns = (int)S.sn.size();
+  const int nb = S.max_depth + 1;
+  constexpr int FT = vhf::kFT;
+  auto up32 = [](int v) { return (v + FT - 1) / FT * FT; };
+  std::vector<vhf::FrontDev> fr(ns);
+  std::vector<std::vector<int>> lev(nb);
+  std::vector<long long> fsz(nb, 0), esz(nb, 0);
+  for (int s = 0; s < ns; ++s) {
+    const vh::Supernode& x = S.sn[s];
+    vhf::FrontDev& f = fr[s];
+    f.w = x.w; f.r = x.r; f.wp = up32(x.w); f.mp = f.wp + up32(x.r);
+    f.f = fsz[x.depth]; fsz[x.depth] += (long long)f.mp * f.mp;
+    f.e = esz[x.depth]; esz[x.depth] += (long long)f.mp * f.wp;
+    f.m_off = x.m_off; f.s_off = x.parent < 0 ? x.s_off : 0;
+    f.col0 = x.col0; f.rows_off = x.rows_off; f.child_off = x.child_off; f.nchild = x.nchild; f.u_off = x.u_off;
+    f.lpos = (int)lev[x.depth].size();
+    lev[x.depth].push_back(s);
+  }
+  // Launch lists: per level, per panel k, the supernodes with a panel k and
+  // the tile tasks of the two tile launches; the roots' inverse tasks last.
+  std::vector<int> ints;  // level lists and diag lists
+  std::vector<int4> tasks;
+  struct L3 { size_t diag_off, trsm_off, upd_off; int ndiag, ntrsm, nupd; };
+  std::vector<std::vector<L3>> plan(nb);
+  std::vector<size_t> lev_off(nb);
+  size_t root_off = 0, root_tasks_off = 0;
+  int nroot_tasks = 0;
+  for (int d = 0; d < nb; ++d) {
+    lev_off[d] = ints.size();
+    ints.insert(ints.end(), lev[d].begin(), lev[d].end());
+    int kmax = 0;
+    for (int s : lev[d]) kmax = std::max(kmax, fr[s].wp / FT);
+    for (int k = 0; k < kmax; ++k) {
+      L3 L{};
+      L.diag_off = ints.size();
+      for (int s : lev[d])
+        if (fr[s].wp / FT > k) ints.push_back(s);
+      L.ndiag = (int)(ints.size() - L.diag_off);
+      L.trsm_off = tasks.size();
+      for (int s : lev[d]) {
+        const int T = fr[s].mp / FT;
+        if (fr[s].wp / FT <= k) continue;
+        for (int i = k + 1; i < T; ++i) tasks.push_back(make_int4(s, i, k, vhf::kFTrsm));
+        for (int c = 0; c <= k; ++c) tasks.push_back(make_int4(s, k, c, vhf::kETrsm));
+      }
+      L.ntrsm = (int)(tasks.size() - L.trsm_off);
+      L.upd_off = tasks.size();
+      for (int s : lev[d]) {
+        const int T = fr[s].mp / FT;
+        if (fr[s].wp / FT <= k) continue;
+        for (int i = k + 1; i < T; ++i) {
+          for (int j = k + 1; j <= i; ++j) tasks.push_back(make_int4(s, i, j, vhf::kFUpd));
+          for (int c = 0; c <= k; ++c) tasks.push_back(make_int4(s, i, c, vhf::kEUpd));
+        }
+      }
+      L.nupd = (int)(tasks.size() - L.upd_off);
+      plan[d].push_back(L);
+    }
+  }
+  root_off = ints.size();
+  for (int s : lev[0])
+    if (S.sn[s].parent < 0) ints.push_back(s);
+  const int nroots = (int)(ints.size() - root_off);
+  root_tasks_off = tasks.size();
+  for (int s : lev[0]) {
+    if (S.sn[s].parent >= 0) continue;
+    const int T = fr[s].wp / FT;
+    for (int i = 0; i < T; ++i)
+      for (int j = 0; j < T; ++j) tasks.push_back(make_int4(s, i, j, vhf::kSRoot));
+  }
+  nroot_tasks = (int)(tasks.size() - root_tasks_off);
+
+  FactorTemps tmp;
+  auto dmalloc = [&](size_t bytes, void** q) -> hipError_t {
+    *q = nullptr;
+    const hipError_t e = hipMalloc(q, std::max<size_t>(bytes, 8));
+    if (e == hipSuccess) tmp.take(*q);
+    return e;
+  };
+  void *d_fr, *d_ints, *d_tasks, *d_rowidx, *d_children, *d_emap, *d_fail;
+  VH_HIP(dmalloc(fr.size() * sizeof(vhf::FrontDev), &d_fr));
+  VH_HIP(dmalloc(ints.size() * sizeof(int), &d_ints));
+  VH_HIP(dmalloc(tasks.size() * sizeof(int4), &d_tasks));
+  VH_HIP(dmalloc(S.rowidx.size() * sizeof(int), &d_rowidx));
+  VH_HIP(dmalloc(S.children.size() * sizeof(int), &d_children));
+  VH_HIP(dmalloc(S.emap.size() * sizeof(int), &d_emap));
+  VH_HIP(dmalloc(sizeof(int), &d_fail));
+  VH_HIP(hipMemcpy(d_fr, fr.data(), fr.size() * sizeof(vhf::FrontDev), hipMemcpyHostToDevice));
+  if (!ints.empty()) VH_HIP(hipMemcpy(d_ints, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!tasks.empty()) VH_HIP(hipMemcpy(d_tasks, tasks.data(), tasks.size() * sizeof(int4), hipMemcpyHostToDevice));
+  if (!S.rowidx.empty())
+    VH_HIP(hipMemcpy(d_rowidx, S.rowidx.data(), S.rowidx.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!S.children.empty())
+    VH_HIP(hipMemcpy(d_children, S.children.data(), S.children.size() * sizeof(int), hipMemcpyHostToDevice));
+  if (!S.emap.empty()) VH_HIP(hipMemcpy(d_emap, S.emap.data(), S.emap.size() * sizeof(int), hipMemcpyHostToDevice));
+  VH_HIP(hipMemset(d_fail, 0, sizeof(int)));
+  const vhf::FrontDev* frp = static_cast<const vhf::FrontDev*>(d_fr);
+  const int* ip = static_cast<const int*>(d_ints);
+  const int4* tp = static_cast<const int4*>(d_tasks);
+  hipStream_t st = ctx->stream;
+  void *Fc = nullptr, *Ec = nullptr, *Dc = nullptr;  // the child level's buffers
+  for (int d = nb - 1; d >= 0; --d) {
+    const int nl = (int)lev[d].size();
+    void *Fd, *Ed, *Dd;
+    VH_HIP(dmalloc(fsz[d] * sizeof(double), &Fd));
+    VH_HIP(dmalloc(esz[d] * sizeof(double), &Ed));
+    VH_HIP(dmalloc((size_t)std::max(nl, 1) * FT * FT * sizeof(double), &Dd));
+    VH_HIP(hipMemsetAsync(Fd, 0, fsz[d] * sizeof(double), st));
+    VH_HIP(hipMemsetAsync(Ed, 0, esz[d] * sizeof(double), st));
+    if (nl > 0)
+      hipLaunchKernelGGL(vhf::k_fac_assemble, dim3(nl), dim3(256), 0, st, frp, ip + lev_off[d], ctx->Ap, ctx->Ai,
+                         ctx->Ax, (const int*)d_rowidx, (const int*)d_children, (const int*)d_emap, (double*)Fd,
+                         (double*)Ed, (const double*)Fc);
+    VH_HIP(hipGetLastError());
+    if (Fc) {  // the children are assembled into this level: their fronts can go
+      VH_HIP(hipStreamSynchronize(st));
+      tmp.drop(Fc); tmp.drop(Ec); tmp.drop(Dc);
+    }
+    for (size_t k = 0; k < plan[d].size(); ++k) {
+      const L3& L = plan[d][k];
+      hipLaunchKernelGGL(vhf::k_fac_diag, dim3(L.ndiag), dim3(256), 0, st, frp, ip + L.diag_off, (int)k, (double*)Fd,
+                         (double*)Dd, (int*)d_fail);
+      if (L.ntrsm > 0)
+        hipLaunchKernelGGL(vhf::k_fac_tiles, dim3(L.ntrsm), dim3(64), 0, st, frp, tp + L.trsm_off, (int)k,
+                           (double*)Fd, (double*)Ed, (const double*)Dd);
+      if (L.nupd > 0)
+        hipLaunchKernelGGL(vhf::k_fac_tiles, dim3(L.nupd), dim3(64), 0, st, frp, tp + L.upd_off, (int)k,
+                           (double*)Fd, (double*)Ed, (const double*)Dd);
+    }
+    if (nl > 0) {
+      if (ctx->prec == VH_PREC_F64)
+        hipLaunchKernelGGL(vhf::k_fac_blocks<double>, dim3(nl), dim3(256), 0, st, frp, ip + lev_off[d],
+                           (const double*)Ed, (double*)ctx->M, (double*)ctx->Mt);
+      else
+        hipLaunchKernelGGL(vhf::k_fac_blocks<float>, dim3(nl), dim3(256), 0, st, frp, ip + lev_off[d],
+                           (const double*)Ed, (float*)ctx->M, (float*)ctx->Mt);
+    }
+    if (d == 0 && nroots > 0) {
+      if (nroot_tasks > 0)
+        hipLaunchKernelGGL(vhf::k_fac_tiles, dim3(nroot_tasks), dim3(64), 0, st, frp, tp + root_tasks_off, 0,
+                           (double*)Fd, (double*)Ed, (const double*)Dd);
+      if (ctx->prec == VH_PREC_F64)
+        hipLaunchKernelGGL(vhf::k_fac_root<double>, dim3(nroots), dim3(256), 0, st, frp, ip + root_off,
+                           (const double*)Fd, (double*)ctx->Sroot);
+      else
+        hipLaunchKernelGGL(vhf::k_fac_root<float>, dim3(nroots), dim3(256), 0, st, frp, ip + root_off,
+                           (const double*)Fd, (float*)ctx->Sroot);
+    }
+    VH_HIP(hipGetLastError());
+    Fc = Fd; Ec = Ed; Dc = Dd;
+  }
+  int h_fail = 0;
+  VH_HIP(hipMemcpyAsync(&h_fail, d_fail, sizeof(int), hipMemcpyDeviceToHost, st));
+  VH_HIP(hipStreamSynchronize(st));
+  if (h_fail) return fail(ctx, VH_E_NOT_SPD, "Cholesky factorization failed: matrix is not positive definite");
+  return VH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1008,7 +1190,7 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(dalloc_bytes(ctx, &ctx->cslot, 3 * (size_t)std::max(1, S.inc_ptr[S.n]) * ctx->tsz));
   UP(upload(ctx, &ctx->inc_ptr, S.inc_ptr));
   // Task records hold block and root-inverse offsets as 32-bit integers.
-  if (S.m_total > INT32_MAX || (int64_t)S.Sroot.size() > INT32_MAX || S.fslot_total > INT32_MAX)
+  if (S.m_total > INT32_MAX || S.sroot_total > INT32_MAX || S.fslot_total > INT32_MAX)
     return fail(ctx, VH_E_UNSUPPORTED, "solve blocks exceed 2^31 entries (split the batch)");
   std::vector<vhk::TaskDev> td(S.ptasks.size());
   for (size_t k = 0; k < td.size(); ++k) {
@@ -1023,17 +1205,22 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->dpush_ptr, S.dpush_ptr));
   UP(upload(ctx, &ctx->dpush_idx, S.dpush_idx));
   UP(upload(ctx, &ctx->upper_cols, S.upper_cols));
-  UP(upload_t(ctx, &ctx->M, S.M));
-  UP(upload_t(ctx, &ctx->Mt, S.Mt));
-  UP(upload_t(ctx, &ctx->Sroot, S.Sroot));
+  UP(dalloc_bytes(ctx, &ctx->M, (size_t)S.m_total * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->Mt, (size_t)S.m_total * ctx->tsz));
+  UP(dalloc_bytes(ctx, &ctx->Sroot, (size_t)S.sroot_total * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->y, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->d, n3 * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->F, 3 * (size_t)std::max<int64_t>(1, S.fslot_total) * ctx->tsz));
   UP(dalloc_bytes(ctx, &ctx->V, 3 * (size_t)std::max(1, S.u_total) * ctx->tsz));
-  if (ctx->prec == VH_PREC_F32_IR) {
-    UP(upload(ctx, &ctx->Ap, S.Ap));
-    UP(upload(ctx, &ctx->Ai, S.Ai));
-    UP(upload(ctx, &ctx->Ax, S.Ax));
+  // A in FP64: the device factorization assembles its fronts from it, and
+  // the FP32-IR step forms its residual with it.
+  UP(upload(ctx, &ctx->Ap, S.Ap));
+  UP(upload(ctx, &ctx->Ai, S.Ai));
+  UP(upload(ctx, &ctx->Ax, S.Ax));
+  {
+    const auto tf = std::chrono::steady_clock::now();
+    UP(device_factor(ctx));
+    ctx->factor_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
   }
   // Residual partials come from the committing launch: the last backward
   // band, or the root band when it is the only band.
@@ -1322,6 +1509,30 @@ vh_status vh_pd_info_get(vh_ctx* ctx, vh_pd_info* out) {
   out->nnz_factor = S.nnz_factor;
   out->max_front = S.max_front;
   out->setup_seconds = ctx->setup_seconds;
+  out->factor_seconds = ctx->factor_seconds;
+  return VH_OK;
+}
+
+vh_status vh_pd_blocks_get(vh_ctx* ctx, int32_t which, double* out, uint64_t cap, uint64_t* count) {
+  if (!ctx || !count) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (which < 0 || which > 2) return fail(ctx, VH_E_INVALID_ARG, "which must be 0 (M), 1 (M^T) or 2 (root inverses)");
+  const vh::Setup& S = ctx->S;
+  const uint64_t total = (uint64_t)(which == 2 ? S.sroot_total : S.m_total);
+  *count = total;
+  const uint64_t k = std::min(cap, total);
+  if (k == 0) return VH_OK;
+  if (!out) return fail(ctx, VH_E_INVALID_ARG, "null output");
+  VH_HIP(hipSetDevice(ctx->device));
+  const void* src = which == 0 ? ctx->M : which == 1 ? ctx->Mt : ctx->Sroot;
+  VH_HIP(hipStreamSynchronize(ctx->stream));
+  if (ctx->tsz == 8) {
+    VH_HIP(hipMemcpy(out, src, k * 8, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> f(k);
+    VH_HIP(hipMemcpy(f.data(), src, k * 4, hipMemcpyDeviceToHost));
+    for (uint64_t q = 0; q < k; ++q) out[q] = (double)f[q];
+  }
   return VH_OK;
 }
 

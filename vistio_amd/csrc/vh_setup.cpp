@@ -137,26 +137,6 @@ struct NdBuilder {
   }
 };
 
-// Dense lower Cholesky in place on the leading w x w block of F (ld = m).
-bool dense_chol(double* F, int m, int w) {
-  for (int j = 0; j < w; ++j) {
-    double* Fj = F + (size_t)j * m;
-    double d = Fj[j];
-    for (int k = 0; k < j; ++k) d -= Fj[k] * Fj[k];
-    if (!(d > 0.0)) return false;
-    const double ljj = std::sqrt(d);
-    Fj[j] = ljj;
-    const double inv = 1.0 / ljj;
-    for (int i = j + 1; i < w; ++i) {
-      double* Fi = F + (size_t)i * m;
-      double s = Fi[j];
-      for (int k = 0; k < j; ++k) s -= Fi[k] * Fj[k];
-      Fi[j] = s * inv;
-    }
-  }
-  return true;
-}
-
 }  // namespace
 
 bool build_setup(const SetupInput& in, Setup& S) {
@@ -478,8 +458,6 @@ bool build_setup(const SetupInput& in, Setup& S) {
   S.m_total = moff;
   S.nnz_factor = moff;
   S.u_total = uoff;
-  S.M.assign((size_t)moff, 0.0);
-  S.Mt.assign((size_t)moff, 0.0);
   S.emap.assign((size_t)uoff, 0);
   std::vector<int> loc(n, -1);
   for (int s = 0; s < ns; ++s) {
@@ -535,108 +513,16 @@ bool build_setup(const SetupInput& in, Setup& S) {
       for (int q = 0; q < S.sn[s].r; ++q) S.dpush_idx[cur[S.rowidx[S.sn[s].rows_off + q]]++] = S.sn[s].u_off + q;
   }
 
-  // Multifrontal numeric LL^T (f64) and the partitioned-inverse blocks.
-  std::vector<std::vector<double>> upd(ns);
-  std::vector<double> F, Linv;
+  // Root inverses: every root (no off-diagonal rows) gets its dense w x w
+  // inverse at Sroot offset s_off. The numeric factorization itself runs on
+  // the device (vh_factor.hpp) from this symbolic structure.
+  S.sroot_total = 0;
   for (int s = 0; s < ns; ++s) {
     Supernode& x = S.sn[s];
-    const int w = x.w, r = x.r, m = w + r;
-    F.assign((size_t)m * m, 0.0);
-    for (int q = 0; q < r; ++q) loc[S.rowidx[x.rows_off + q]] = w + q;
-    for (int jj = 0; jj < w; ++jj) {
-      const int J = x.col0 + jj;
-      for (int p = S.Ap[J]; p < S.Ap[J + 1]; ++p) {
-        const int i = S.Ai[p];
-        if (i < J) continue;
-        const int pos = i < x.col0 + w ? i - x.col0 : loc[i];
-        if (pos < 0) { S.status = VH_E_INTERNAL; S.err = "matrix entry outside front"; return false; }
-        F[(size_t)pos * m + jj] += S.Ax[p];
-      }
-    }
-    for (int q = 0; q < r; ++q) loc[S.rowidx[x.rows_off + q]] = -1;
-    for (int c = 0; c < x.nchild; ++c) {
-      const int ch = S.children[x.child_off + c];
-      const Supernode& y = S.sn[ch];
-      const std::vector<double>& U = upd[ch];
-      const int* mp = &S.emap[y.u_off];
-      for (int a = 0; a < y.r; ++a)
-        for (int b = 0; b <= a; ++b) F[(size_t)mp[a] * m + mp[b]] += U[(size_t)a * y.r + b];
-      upd[ch].clear(); upd[ch].shrink_to_fit();
-    }
-    if (!dense_chol(F.data(), m, w)) {
-      S.status = VH_E_NOT_SPD; S.err = "Cholesky factorization failed: matrix is not positive definite"; return false;
-    }
-    // L_IP = F_IP L_PP^{-T}
-    for (int q = 0; q < r; ++q) {
-      double* Fq = &F[(size_t)(w + q) * m];
-      for (int j = 0; j < w; ++j) {
-        const double* Lj = &F[(size_t)j * m];
-        double v = Fq[j];
-        for (int k = 0; k < j; ++k) v -= Fq[k] * Lj[k];
-        Fq[j] = v / Lj[j];
-      }
-    }
-    // Update matrix U = F_II - L_IP L_IP^T (lower)
-    if (x.parent >= 0 && r > 0) {
-      std::vector<double>& U = upd[s];
-      U.assign((size_t)r * r, 0.0);
-      for (int a = 0; a < r; ++a) {
-        const double* La = &F[(size_t)(w + a) * m];
-        for (int b = 0; b <= a; ++b) {
-          const double* Lb = &F[(size_t)(w + b) * m];
-          double v = F[(size_t)(w + a) * m + w + b];
-          for (int k = 0; k < w; ++k) v -= La[k] * Lb[k];
-          U[(size_t)a * r + b] = v;
-        }
-      }
-    }
-    // Linv = L_PP^{-1} (lower), column by column.
-    Linv.assign((size_t)w * w, 0.0);
-    for (int j = 0; j < w; ++j) {
-      Linv[(size_t)j * w + j] = 1.0 / F[(size_t)j * m + j];
-      for (int i = j + 1; i < w; ++i) {
-        const double* Li = &F[(size_t)i * m];
-        double v = 0;
-        for (int k = j; k < i; ++k) v -= Li[k] * Linv[(size_t)k * w + j];
-        Linv[(size_t)i * w + j] = v / Li[i];
-      }
-    }
-    double* Mp = &S.M[(size_t)x.m_off];
-    for (int i = 0; i < w; ++i)
-      for (int j = 0; j <= i; ++j) Mp[(size_t)i * (i + 1) / 2 + j] = Linv[(size_t)i * w + j];
-    double* Wp = Mp + (size_t)w * (w + 1) / 2;
-    for (int q = 0; q < r; ++q) {
-      const double* Lq = &F[(size_t)(w + q) * m];
-      for (int j = 0; j < w; ++j) {
-        double v = 0;
-        for (int k = j; k < w; ++k) v += Lq[k] * Linv[(size_t)k * w + j];
-        Wp[(size_t)q * w + j] = v;
-      }
-    }
-    // Transposed copy: row j holds column j of M_P, i.e. Linv[j..w-1][j]
-    // followed by W[0..r-1][j]; entry i multiplies front position j + i.
-    double* Tp = &S.Mt[(size_t)x.m_off];
-    for (int j = 0; j < w; ++j) {
-      double* Tj = Tp + (size_t)j * m - (size_t)j * (j - 1) / 2;
-      for (int k = j; k < w; ++k) Tj[k - j] = Mp[(size_t)k * (k + 1) / 2 + j];
-      for (int q = 0; q < r; ++q) Tj[w - j + q] = Wp[(size_t)q * w + j];
-    }
-    if (x.parent < 0) {  // root: full inverse of L_RR L_RR^T = Linv^T Linv
-      if (r != 0) { S.status = VH_E_INTERNAL; S.err = "root supernode has off-diagonal rows"; return false; }
-      x.s_off = (int64_t)S.Sroot.size();
-      S.Sroot.resize(S.Sroot.size() + (size_t)w * w, 0.0);
-      double* Sp = S.Sroot.data() + x.s_off;
-      for (int k = 0; k < w; ++k) {
-        const double* Lk = &Linv[(size_t)k * w];
-        for (int i = 0; i <= k; ++i) {
-          const double a = Lk[i];
-          double* Si = Sp + (size_t)i * w;
-          for (int j = i; j <= k; ++j) Si[j] += a * Lk[j];
-        }
-      }
-      for (int i = 0; i < w; ++i)
-        for (int j = 0; j < i; ++j) Sp[(size_t)i * w + j] = Sp[(size_t)j * w + i];
-    }
+    if (x.parent >= 0) continue;
+    if (x.r != 0) { S.status = VH_E_INTERNAL; S.err = "root supernode has off-diagonal rows"; return false; }
+    x.s_off = S.sroot_total;
+    S.sroot_total += (int64_t)x.w * x.w;
   }
 
   // Elements with permuted vertex labels, sorted by their smallest label for
@@ -702,7 +588,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
   }
   std::sort(S.upper_cols.begin(), S.upper_cols.end());
   if (S.m_total >= (int64_t)1 << 31) { S.status = VH_E_UNSUPPORTED; S.err = "solve blocks exceed 2^31 entries"; return false; }
-  if ((int64_t)S.Sroot.size() >= (int64_t)1 << 31) {
+  if (S.sroot_total >= (int64_t)1 << 31) {
     S.status = VH_E_UNSUPPORTED; S.err = "root inverses exceed 2^31 entries"; return false;
   }
 
@@ -768,7 +654,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
     for (int c = 0; c < S.sn[s].nchild; ++c) S.need_f[s] += nfc[S.children[S.sn[s].child_off + c]];
     S.need_b[s] = S.sn[s].parent >= 0 ? nbc[S.sn[s].parent] : nfc[s];
   }
-  const std::string why = device_index_overflow(S.m_total, (int64_t)S.Sroot.size(), S.fslot_total,
+  const std::string why = device_index_overflow(S.m_total, S.sroot_total, S.fslot_total,
                                                 3 * ((int64_t)3 * t + (int64_t)4 * h));
   if (!why.empty()) { S.status = VH_E_UNSUPPORTED; S.err = why; return false; }
   S.status = VH_OK;

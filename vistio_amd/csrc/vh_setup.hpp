@@ -119,13 +119,14 @@ struct Setup {
   std::vector<int32_t> tri_slot, hinge_slot;
   std::vector<int32_t> upper_cols;   // columns outside the deepest band
   std::vector<int32_t> band_front;   // max (w + r) per band
-  std::vector<double> M;             // M_P row-major trapezoids (forward sweep)
-  std::vector<double> Mt;            // M_P^T: column j of M_P stored contiguously (backward sweep)
-  // A root supernode (no off-diagonal rows) is solved in one pass:
-  // d_R = (L_RR L_RR^T)^{-1} t_R with its dense w_R x w_R row-major inverse,
-  // stored at Sroot[R.s_off]. A mesh of several connected components (several
-  // garments) has one dissection tree, one root, per component.
-  std::vector<double> Sroot;
+  // The solve blocks (built on the device, vh_factor.hpp): M holds M_P as
+  // row-major trapezoids at m_off (forward sweep), Mt holds M_P^T, column j
+  // of M_P stored contiguously (backward sweep). A root supernode (no
+  // off-diagonal rows) is solved in one pass: d_R = (L_RR L_RR^T)^{-1} t_R
+  // with its dense w_R x w_R row-major inverse at Sroot[R.s_off]; a mesh of
+  // several connected components (several garments) has one dissection tree,
+  // one root, per component. m_total and sroot_total are the array sizes.
+  int64_t sroot_total = 0;
   int64_t m_total = 0;
   int u_total = 0;
   int max_front = 0;                 // max (w + r)

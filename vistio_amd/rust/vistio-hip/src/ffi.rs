@@ -111,6 +111,7 @@ pub struct VhPdInfo {
     pub nnz_factor: i64,
     pub max_front: i32,
     pub setup_seconds: f64,
+    pub factor_seconds: f64,
 }
 
 /// `vh_material_arrays`: per-element materials; a null pointer selects the
@@ -303,6 +304,7 @@ extern "C" {
     pub fn vh_backend_dot(ctx: *mut VhCtx, a: *const f32, na: u64, b: *const f32, nb: u64, result: *mut f64)
         -> VhStatus;
     pub fn vh_backend_fill(ctx: *mut VhCtx, buf: *mut f32, n: u64, value: f32) -> VhStatus;
+    pub fn vh_pd_blocks_get(ctx: *mut VhCtx, which: i32, out: *mut f64, cap: u64, count: *mut u64) -> VhStatus;
     pub fn vh_backend_dot_noalloc(ctx: *mut VhCtx, a: *const f32, na: u64, b: *const f32, nb: u64, result: *mut f64)
         -> VhStatus;
 }

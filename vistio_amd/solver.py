@@ -598,6 +598,16 @@ class HipPdSolver:
                     bwd_bytes=list(p.bwd_bytes[:nb]), ir_us=p.ir_us, ir_bytes=p.ir_bytes,
                     iteration_us=p.iteration_us)
 
+    def blocks(self, which: int) -> np.ndarray:
+        """The device-built solve blocks as float64: 0 = M, 1 = M^T, 2 = the
+        roots' dense inverses (vh_pd_blocks_get)."""
+        cnt = C.c_uint64(0)
+        _check(self._L.vh_pd_blocks_get(self._ctx, int(which), None, 0, C.byref(cnt)), self._ctx, "blocks")
+        out = np.empty(cnt.value, dtype=np.float64)
+        _check(self._L.vh_pd_blocks_get(self._ctx, int(which), nat.ptr(out, C.c_double), out.size, C.byref(cnt)),
+               self._ctx, "blocks")
+        return out
+
     def info(self) -> dict:
         i = nat.PdInfo()
         _check(self._L.vh_pd_info_get(self._ctx, C.byref(i)), self._ctx, "info")
