@@ -11,9 +11,14 @@ gravity -9.81 m/s^2 along y, 0.15 kg/m^2, co-rotational membrane weight
 
 Timing: W warm-up steps, the state is reset to rest, then exactly K steps
 are timed from a device synchronisation to the next (state resident in
-HBM). The error is the max per-vertex distance between the FP32-refined and
-the FP64 GPU positions after those K steps from rest. The FP64 GPU path is
-itself held to <= 1e-9 m against the FP64 CPU oracle by the GPU tests.
+HBM), as replays of the step's HIP graph. The roofline figures time the
+same kernels the same way: one solve sweep and the local step, each
+captured into a graph and replayed (vh_pd_profile_graph).
+
+Accuracy: the max per-vertex distance of the FP32-refined and of the FP64
+GPU positions from the FP64 oracle after the same K steps from rest (the
+oracle runs after the timed region, on all host threads; its results do not
+depend on the thread count), and between the two GPU paths.
 """
 import argparse
 import json
@@ -76,10 +81,11 @@ def run_path(mesh, topo, pinned, precision, steps, warmup, dt, ir_sweeps=1, ir_f
     r = s.step_resident(dt, steps)  # ends with a device synchronisation
     wall = time.perf_counter() - t0
     x = s.positions()
+    graph = s.profile_graph(dt, 100)
     prof = s.profile(dt, 10)
     info = s.info()
     s.close()
-    return dict(wall=wall, x=x, prof=prof, info=info, setup=setup, iterations=r.iterations,
+    return dict(wall=wall, x=x, prof=prof, graph=graph, info=info, setup=setup, iterations=r.iterations,
                 final_residual=r.final_residual)
 
 
@@ -100,18 +106,68 @@ def run_drop_in(mesh, topo, pinned, steps, dt, ir_sweeps, ir_final_sweeps):
     return steps / wall
 
 
-def roofline_of(prof, traffic=None):
+def roofline_of(graph, ms_per_step, sweeps_per_step, traffic=None):
     """Dominant kernels: the band launches of the supernodal solve (forward
-    bands, the root band, backward bands; 2 * bands - 1 launches a sweep)."""
-    us = sum(prof["fwd_us"]) + sum(prof["bwd_us"])
-    by = sum(prof["fwd_bytes"]) + sum(prof["bwd_bytes"])
-    launches = prof["sweeps"] * (2 * prof["bands"] - 1)
+    bands, the root band, backward bands; 2 * bands - 1 launches a sweep),
+    timed as the steps run them: one sweep captured into a HIP graph and
+    replayed (vh_pd_profile_graph), HIP events on the solver's stream around
+    the replays. achieved = algorithmic bytes per launch / average launch
+    time. The band launches of one step cannot take longer than the step:
+    launch_avg_us x launches_per_step <= ms_per_step is checked and reported."""
+    launches = graph["solve_launches"]
+    us = graph["solve_us"] / launches
+    by = graph["solve_bytes"] / launches
     achieved = by / (us * 1e-6) / 1e9
+    per_step = launches * sweeps_per_step
+    band_ms = us * per_step / 1000.0
+    if band_ms > ms_per_step:
+        print(f"roofline check failed: {per_step} band launches x {us:.3f} us = {band_ms:.4f} ms > "
+              f"{ms_per_step:.4f} ms per step", file=sys.stderr, flush=True)
     return dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_fwd_band/k_root_band/k_bwd_band (supernodal sweeps)",
-                launch_avg_us=round(us / launches, 3), bytes_per_launch=round(by / launches),
-                launches_per_iteration=prof["launches"])
+                launch_avg_us=round(us, 3), bytes_per_launch=round(by), launches_per_step=per_step,
+                band_ms_per_step=round(band_ms, 4), fits_in_step=bool(band_ms <= ms_per_step),
+                timing="graph replay of one solve sweep, HIP events")
+
+
+def sweeps_per_step(args, precision):
+    """Solve sweeps in one step: one per PD iteration in FP64; in FP32-IR
+    ir_sweeps per iteration and ir_final_sweeps on the last."""
+    if precision == "f64":
+        return 10
+    return 9 * args.ir_sweeps + args.ir_final_sweeps
+
+
+def local_step_of(graph):
+    """The local step (k_local) timed in graph replay: algorithmic bytes per
+    launch over the average launch time."""
+    return dict(us=round(graph["local_us"], 3), bytes=round(graph["local_bytes"]),
+                gbs=round(graph["local_bytes"] / (graph["local_us"] * 1e-6) / 1e9, 1),
+                frac=round(graph["local_bytes"] / (graph["local_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
+
+
+def oracle_errors(cells, steps, dt, positions):
+    """Max per-vertex distance of each GPU result from the FP64 oracle after
+    the same number of steps from rest (the checker, run after the timed
+    region on all host threads)."""
+    from oracle import mesh as omesh
+    from oracle import ref
+    ref.lib()
+    threads = min(16, int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1))
+    ref.set_threads(threads)
+    px, py, pz, idx = omesh.quad_grid(cells, cells, 1.0, 1.0)
+    o = ref.Oracle(ref.F64, px, py, pz, idx, omesh.hinges4(idx, px.size), omesh.pinned_top_row(cells, cells),
+                   ref.params())
+    t0 = time.perf_counter()
+    o.step(dt, steps)
+    wall = time.perf_counter() - t0
+    ref.set_threads(1)
+    xo = o.positions()
+    out = {k: float(np.sqrt(((x - xo) ** 2).sum(0)).max()) for k, x in positions.items()}
+    out["oracle_seconds"] = round(wall, 1)
+    out["oracle_threads"] = threads
+    return out
 
 
 def copy_probe(mb):
@@ -291,7 +347,8 @@ def run_batch(args):
         t0 = time.perf_counter()
         s.step_resident(dt, steps)
         wall = time.perf_counter() - t0
-        res[prec] = dict(wall=wall, x=s.positions(), prof=s.profile(dt, 10), info=s.info(), setup=setup)
+        res[prec] = dict(wall=wall, x=s.positions(), graph=s.profile_graph(dt, 20), prof=s.profile(dt, 10),
+                         info=s.info(), setup=setup)
         s.close()
     f32, f64 = res["f32_ir"], res["f64"]
     value = steps / f32["wall"]
@@ -317,7 +374,7 @@ def run_batch(args):
                                "Rayleigh 0.01, dt 1/60 s, 10 PD iterations; FP32 local step + FP32 solves, FP64 "
                                "residual refinement; one block-diagonal system, one graph per step",
                    "sheets": len(specs)},
-        "roofline": roofline_of(f32["prof"]),
+        "roofline": roofline_of(f32["graph"], 1000.0 * f32["wall"] / steps, sweeps_per_step(args, "f32_ir")),
         "extra": {
             "sheet_steps_per_s": round(value * len(specs), 1),
             "vertex_steps_per_s": round(value * b.mesh.vertex_count(), 0),
@@ -326,12 +383,12 @@ def run_batch(args):
             "fp32_ir_over_fp64_gpu": round(value / (steps / f64["wall"]), 3),
             "iteration_us": {"fp32_ir": round(f32["prof"]["iteration_us"], 2),
                              "fp64": round(f64["prof"]["iteration_us"], 2)},
-            "local_step": {
-                "fp32_us": round(f32["prof"]["local_us"], 3),
-                "fp32_gbs": round(f32["prof"]["local_bytes"] / (f32["prof"]["local_us"] * 1e-6) / 1e9, 1)},
+            "local_step": local_step_of(f32["graph"]),
             "solve_bands": f32["info"]["levels"],
             "factor_entries": f32["info"]["nnz_factor"],
-            "setup_s": {"fp32_ir": round(f32["setup"], 1), "fp64": round(f64["setup"], 1)},
+            "setup_s": {"fp32_ir": round(f32["setup"], 1), "fp64": round(f64["setup"], 1),
+                        "host_symbolic_fp32_ir": round(f32["info"]["setup_seconds"], 2),
+                        "device_factor_fp32_ir": round(f32["info"]["factor_seconds"], 3)},
             "autotune": tuned or None,
         },
     }
@@ -367,6 +424,8 @@ def main():
         print(f"bench.py measures one GPU: the PD step of one sheet runs on a single MI355X; "
               f"--gpus {args.gpus} (WORLD_SIZE {world}) is not supported", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mb > 0:
+        copy_probe(args.copy_probe_mb)
     if args.workload == "batched_sheets":
         run_batch(args)
         return
@@ -374,8 +433,6 @@ def main():
         run_contact(args)
         return
     dt = 1.0 / 60.0
-    if args.copy_probe_mb > 0:
-        copy_probe(args.copy_probe_mb)
     mesh, topo, pinned = make_sheet(args.cells)
     f64 = run_path(mesh, topo, pinned, "f64", args.steps, args.warmup, dt)
     f32 = run_path(mesh, topo, pinned, "f32_ir", args.steps, args.warmup, dt, args.ir_sweeps, args.ir_final_sweeps)
@@ -383,6 +440,8 @@ def main():
     drop_in = run_drop_in(mesh, topo, pinned, min(args.steps, 60), dt, args.ir_sweeps, args.ir_final_sweeps)
     value = args.steps / f32["wall"]
     v64 = args.steps / f64["wall"]
+    ms32 = 1000.0 * f32["wall"] / args.steps
+    ms64 = 1000.0 * f64["wall"] / args.steps
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -405,29 +464,33 @@ def main():
             "cells": args.cells,
             "pd_iterations_per_step": 10,
         },
-        "roofline": roofline_of(f32["prof"], traffic_from_profiles("fp32") if args.cells == 256 else None),
+        "roofline": roofline_of(f32["graph"], ms32, sweeps_per_step(args, "f32_ir"),
+                                traffic_from_profiles("fp32") if args.cells == 256 else None),
         "extra": {
             "pd_iterations_per_s": round(10 * value, 1),
             "max_vertex_err_vs_fp64_gpu_m": err,
             "fp64_gpu_steps_per_s": round(v64, 3),
             "fp32_ir_over_fp64_gpu": round(value / v64, 3),
-            "fp64_roofline": roofline_of(f64["prof"], traffic_from_profiles("fp64") if args.cells == 256 else None),
-            "local_step": {
-                "fp32_us": round(f32["prof"]["local_us"], 3),
-                "fp32_gbs": round(f32["prof"]["local_bytes"] / (f32["prof"]["local_us"] * 1e-6) / 1e9, 1),
-                "fp64_us": round(f64["prof"]["local_us"], 3),
-                "fp64_gbs": round(f64["prof"]["local_bytes"] / (f64["prof"]["local_us"] * 1e-6) / 1e9, 1),
-            },
-            "iteration_us": {"fp32_ir": round(f32["prof"]["iteration_us"], 2),
-                             "fp64": round(f64["prof"]["iteration_us"], 2)},
+            "fp64_roofline": roofline_of(f64["graph"], ms64, sweeps_per_step(args, "f64"),
+                                         traffic_from_profiles("fp64") if args.cells == 256 else None),
+            "local_step": {"fp32": local_step_of(f32["graph"]), "fp64": local_step_of(f64["graph"])},
+            "event_profile_us": {"fp32_ir_iteration": round(f32["prof"]["iteration_us"], 2),
+                                 "fp64_iteration": round(f64["prof"]["iteration_us"], 2),
+                                 "note": "one event after every launch, no graph (vh_pd_profile)"},
             "solve_bands": f32["info"]["levels"],
             "factor_entries": f32["info"]["nnz_factor"],
-            "setup_s": {"fp32_ir": round(f32["setup"], 2), "fp64": round(f64["setup"], 2)},
+            "setup_s": {"fp32_ir": round(f32["setup"], 2), "fp64": round(f64["setup"], 2),
+                        "host_symbolic": round(f32["info"]["setup_seconds"], 3),
+                        "device_factor_fp32_ir": round(f32["info"]["factor_seconds"], 4),
+                        "device_factor_fp64": round(f64["info"]["factor_seconds"], 4)},
             "final_residual": f32["final_residual"],
             "drop_in_host_state_steps_per_s": round(drop_in, 3),
         },
     }
     if not args.no_cpu:
+        # the checker: the FP64 oracle over the same steps from rest
+        oe = oracle_errors(args.cells, args.steps, dt, {"fp32_ir": f32["x"], "fp64": f64["x"]})
+        out["extra"]["max_vertex_err_vs_fp64_oracle_m"] = oe
         cb = cpu_baseline(args.cells, args.cpu_sample_steps, dt)
         out["cpu_baseline"] = cb
         out["extra"]["fp32_ir_over_cpu"] = round(value / cb["value"], 1)

@@ -237,6 +237,20 @@ typedef struct {
 /* Times `iterations` PD iterations from the current device state (the state
  * advances like the loop of one step; call after a step). */
 vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile* out);
+/* The same launches timed the way steps run: one solve of a PD iteration
+ * (one sweep from the RHS slots, committing; 2 * bands - 1 launches) and
+ * ten local steps are each captured into a HIP graph and replayed `reps`
+ * times between two HIP events on the context's stream; the averages per
+ * solve and per local step, and their algorithmic bytes (as vh_profile
+ * counts them). The state advances with every replay: call after the steps you
+ * measure. */
+typedef struct {
+  uint32_t reps;
+  uint32_t solve_launches;
+  double solve_us, solve_bytes;
+  double local_us, local_bytes;
+} vh_graph_profile;
+vh_status vh_pd_profile_graph(vh_ctx* ctx, double dt, uint32_t reps, vh_graph_profile* out);
 vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out);
 
 /* ---- IPC contact (augmented-Lagrangian step) --------------------------------

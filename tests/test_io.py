@@ -103,3 +103,47 @@ def test_snapshot_replay_of_resident_state():
     print("replay vs continued run:", err, flush=True)
     assert err < 1e-5
     a.close(); b.close()
+
+
+@pytest.mark.gpu
+def test_snapshot_of_gpu_state_matches_oracle_snapshot():
+    """StateSnapshot (snapshot.rs:14-71) of the GPU state after 30 steps and
+    of the FP64 oracle's state after the same steps: the bincode bytes decode
+    to the same f32 positions and velocities, to one f32 rounding (the two
+    FP64 states are within 1e-9 m). A snapshot written from the oracle and
+    restored into the GPU solver then continues like the oracle restored
+    from the same snapshot."""
+    from conftest import DT, gpu_solver, max_vertex_err, oracle_solver, sheet
+    from vistio_amd.io import StateSnapshot
+    px, py, pz, idx, hg, pin = sheet(20, rotate_xz=True)
+    g = gpu_solver(px, py, pz, idx, hg, pin)
+    o = oracle_solver(px, py, pz, idx, hg, pin)
+    g.step_resident(DT, 30)
+    o.step(DT, 30)
+    sg = StateSnapshot.from_bytes(StateSnapshot.from_solver(g, 30, 30 * DT).to_bytes())
+    xo, vo = o.get(0), o.get(1)
+    so = StateSnapshot.from_bytes(StateSnapshot.from_soa(30, 30 * DT, *xo, *vo).to_bytes())
+    assert (sg.timestep, sg.vertex_count) == (so.timestep, so.vertex_count) == (30, px.size)
+    # the bytes hold exactly the f32 rounding of the GPU's own FP64 state
+    xg, vg = g.get_state(0), g.get_state(1)
+    assert np.array_equal(sg.positions, xg.T.reshape(-1).astype(np.float32))
+    assert np.array_equal(sg.velocities, vg.T.reshape(-1).astype(np.float32))
+    # and agree with the oracle's state within the FP64 parity bound (1e-9 m,
+    # so 1e-9 / dt = 6e-8 m/s for velocities) plus one f32 rounding
+    ulp_p = np.spacing(np.abs(so.positions).astype(np.float32)).astype(np.float64)
+    ulp_v = np.spacing(np.abs(so.velocities).astype(np.float32)).astype(np.float64)
+    dp = np.abs(sg.positions.astype(np.float64) - xo.T.reshape(-1))
+    dv = np.abs(sg.velocities.astype(np.float64) - vo.T.reshape(-1))
+    print(f"snapshot vs oracle state: positions max {dp.max():.2e} m, velocities max {dv.max():.2e} m/s", flush=True)
+    assert np.all(dp <= 1e-9 + ulp_p) and np.all(dv <= 1e-9 / DT + ulp_v)
+    # replay: the oracle's snapshot restored into both
+    so.apply_to_solver(g)
+    x = so.positions.reshape(-1, 3).T.astype(np.float64)
+    v = so.velocities.reshape(-1, 3).T.astype(np.float64)
+    o.set(0, x); o.set(2, x); o.set(1, v)
+    g.step_resident(DT, 20)
+    o.step(DT, 20)
+    err = max_vertex_err(g.positions(), o.positions())
+    print(f"replayed from the oracle's snapshot: max vertex err after 20 steps {err:.3e} m", flush=True)
+    assert err < 1e-9
+    g.close()

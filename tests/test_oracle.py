@@ -7,6 +7,8 @@ Each group names what pins it:
     polar, matrix-exponential rotations, numpy dense solve) on tiny meshes;
   * scipy's sparse LU for the oracle's own Cholesky.
 """
+import os
+
 import numpy as np
 import pytest
 import scipy.linalg as sla
@@ -346,3 +348,27 @@ def test_oracle_threads_do_not_change_results():
             out.append(o.positions())
     ref.set_threads(1)
     assert np.array_equal(out[0], out[2]) and np.array_equal(out[1], out[3])
+
+
+@pytest.mark.parametrize("c,floor", [(64, 5e-6), (256, 1e-4)])
+def test_reference_precision_split_exceeds_the_error_budget(c, floor):
+    """The reference's own precision split (f32 state and elements, f64 solve:
+    the oracle's F32_REF mode) leaves the FP64 algorithm by more than the
+    1e-6 m budget within two steps (1.2e-5 m at 64 x 64, 1.9e-4 m at
+    256 x 256): the reason the device keeps positions and velocities in FP64
+    in both of its precision modes."""
+    from oracle import mesh
+    ref.set_threads(min(8, os.cpu_count() or 1))
+    try:
+        px, py, pz, idx = mesh.quad_grid(c, c, 1.0, 1.0)
+        hg = mesh.hinges4(idx, px.size)
+        pin = mesh.pinned_top_row(c, c)
+        a = ref.Oracle(ref.F64, px, py, pz, idx, hg, pin, ref.params())
+        b = ref.Oracle(ref.F32_REF, px, py, pz, idx, hg, pin, ref.params())
+        a.step(1.0 / 60.0, 2)
+        b.step(1.0 / 60.0, 2)
+    finally:
+        ref.set_threads(1)
+    err = float(np.sqrt(((a.positions() - b.positions()) ** 2).sum(0)).max())
+    print(f"{c}x{c}: F32_REF vs F64 after 2 steps {err:.3e} m", flush=True)
+    assert err > floor

@@ -32,6 +32,7 @@ EXPORTS = (
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
     "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings", "vh_pd_init_ex",
     "vh_backend_axpy", "vh_backend_dot", "vh_backend_fill", "vh_backend_dot_noalloc", "vh_pd_blocks_get",
+    "vh_pd_profile_graph",
 )
 
 
@@ -71,6 +72,12 @@ class PdInfo(C.Structure):
     _fields_ = [("n", C.c_uint32), ("t", C.c_uint32), ("h", C.c_uint32), ("supernodes", C.c_uint32),
                 ("levels", C.c_uint32), ("nnz_a", C.c_int64), ("nnz_factor", C.c_int64),
                 ("max_front", C.c_int32), ("setup_seconds", C.c_double), ("factor_seconds", C.c_double)]
+
+
+class GraphProfile(C.Structure):
+    """vh_graph_profile: one solve sweep and the local step, timed in graph replay."""
+    _fields_ = [("reps", C.c_uint32), ("solve_launches", C.c_uint32), ("solve_us", C.c_double),
+                ("solve_bytes", C.c_double), ("local_us", C.c_double), ("local_bytes", C.c_double)]
 
 
 class IpcParams(C.Structure):
@@ -143,6 +150,7 @@ def load():
         "vh_pd_solve": ([vp, dp, dp, C.c_int32], C.c_int),
         "vh_pd_info_get": ([vp, C.POINTER(PdInfo)], C.c_int),
         "vh_pd_blocks_get": ([vp, C.c_int32, dp, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
+        "vh_pd_profile_graph": ([vp, C.c_double, C.c_uint32, C.POINTER(GraphProfile)], C.c_int),
         "vh_pd_lumped_masses": ([vp, fp], C.c_int),
         "vh_pd_profile": ([vp, C.c_double, C.c_uint32, C.POINTER(Profile)], C.c_int),
         "vh_mesh_quad_grid": ([C.c_uint32, C.c_uint32, C.c_float, C.c_float, fp, fp, fp, up], C.c_int),

@@ -13,6 +13,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstddef>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -49,7 +50,7 @@ struct IpcState {
   vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t
   int st_levels = 0;
   int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
-  unsigned int* h_np = nullptr;  // pinned host copy of BvhDev::npairs (same allocation)
+  unsigned int* h_np = nullptr;  // pinned host copies of BvhDev::npairs and ::ncont (same allocation)
   unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
   float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
   void* scan_tmp = nullptr;
@@ -67,6 +68,19 @@ struct IpcState {
   void* sort_tmp = nullptr;
   size_t sort_bytes = 0;
   double cfl_dh = 0;  // d_hat of the CFL velocity clamp (compliant-scaled when compliant_contact)
+  // Self-contact gradients summed in a fixed order (vh_contact.hpp,
+  // k_self_count .. k_vgather): per-survivor offsets (one region of ccd_reg
+  // per block, as ccd_surv), per-block totals and bases, the contact entries
+  // (4 per contact: vertex, gradient) for up to ecap contacts, and their
+  // grouping by vertex (counts, fill, segment starts over n + 1).
+  unsigned int *cof = nullptr, *btot = nullptr, *sbase = nullptr;
+  int* ev = nullptr;
+  double* eg = nullptr;
+  unsigned int* vidx = nullptr;
+  unsigned int *vcnt = nullptr, *vfill = nullptr, *vptr = nullptr;
+  unsigned int ecap = 0;
+  void* vscan_tmp = nullptr;
+  size_t vscan_bytes = 0;
   // First error a library routine (the rocPRIM sort and scan of the broad
   // phase) returned while a launch sequence was enqueued; take_lib_err reads
   // and clears it.
@@ -572,6 +586,39 @@ __global__ void k_ipc_clear_overflow(vhc::IpcDev* ip) {
   if (threadIdx.x == 0) ip->overflow = 0;
 }
 
+// Replaces a device buffer tracked in ctx->allocs by one of `bytes` bytes
+// (contents not kept). The caller has synchronized the stream.
+hipError_t realloc_tracked(vh_ctx* c, void** p, size_t bytes) {
+  auto it = std::find(c->allocs.begin(), c->allocs.end(), *p);
+  if (it != c->allocs.end()) c->allocs.erase(it);
+  hipError_t e = *p ? hipFree(*p) : hipSuccess;
+  *p = nullptr;
+  if (e != hipSuccess) return e;
+  e = hipMalloc(p, std::max<size_t>(bytes, 8));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return e;
+  }
+  c->allocs.push_back(*p);
+  return hipSuccess;
+}
+
+// Replaces the self-contact entry buffers by ones with room for `need`
+// contacts and half again, and clears the overflow flag (as grow_pairs).
+hipError_t grow_contacts(vh_ctx* c, unsigned int need) {
+  IpcState& I = c->ipc;
+  const uint64_t cap = std::min<uint64_t>((uint64_t)need + need / 2 + 256, 1ull << 29);
+  if (need > cap) return hipErrorOutOfMemory;
+  I.ecap = 0;  // with no room, k_self_emit writes nothing
+  hipError_t e = realloc_tracked(c, (void**)&I.ev, 4 * cap * sizeof(int));
+  if (e == hipSuccess) e = realloc_tracked(c, (void**)&I.eg, 12 * cap * sizeof(double));
+  if (e == hipSuccess) e = realloc_tracked(c, (void**)&I.vidx, 4 * cap * sizeof(unsigned int));
+  if (e != hipSuccess) return e;
+  I.ecap = (unsigned int)cap;
+  hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
+  return hipGetLastError();
+}
+
 // Replaces the candidate-pair buffer by one with room for `need` pairs and
 // half again, and clears the overflow flag. The caller has synchronized the
 // stream (the old buffer is freed). The reference collects its pairs in a
@@ -606,6 +653,12 @@ hipError_t grow_pairs(vh_ctx* c, unsigned int need) {
   c->allocs.push_back(q);
   I.ccd_surv = static_cast<int2*>(q);
   I.ccd_reg = reg;
+  // the self-contact offsets share the survivor regions' layout
+  e = realloc_tracked(c, (void**)&I.cof, (size_t)pair_blocks(I.cap) * reg * sizeof(unsigned int));
+  if (e != hipSuccess) {
+    I.cap = 0;
+    return e;
+  }
   hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
   return hipGetLastError();
 }
@@ -628,21 +681,28 @@ void enqueue_ccd_pairs(vh_ctx* c, const double* x, const double* dl) {
                        I.ccd_reg, I.ccd_cnt, c->tri_idx, x, dl, I.ip);
 }
 
-// After a candidate-pair query has been enqueued: reads the pair count (one
-// 4-byte read and a synchronization) and, when it exceeded the buffer, grows
-// the buffer and enqueues `redo`, which repeats the query and what consumed
-// it. The count is a function of the positions, so one regrowth suffices.
-// `flag`, when given, is read into h_done with the count.
+// After a candidate-pair query has been enqueued: reads the pair count and
+// the self-contact count (one 8-byte read and a synchronization) and, when
+// one exceeded its buffer, grows it and enqueues `redo`, which repeats the
+// query and what consumed it. The counts are functions of the positions, so
+// a regrowth per buffer suffices. contacts: the enqueued work includes a
+// self-contact detection. `flag`, when given, is read into h_done with the
+// counts.
 template <class F>
-hipError_t fit_pairs(vh_ctx* c, F&& redo, const int* flag = nullptr) {
+hipError_t fit_pairs(vh_ctx* c, F&& redo, bool contacts, const int* flag = nullptr) {
   IpcState& I = c->ipc;
-  for (int pass = 0; pass < 3; ++pass) {
-    hipError_t e = hipMemcpyAsync(I.h_np, &I.bd->npairs, sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream);
+  static_assert(offsetof(vhc::BvhDev, ncont) == offsetof(vhc::BvhDev, npairs) + sizeof(unsigned int),
+                "npairs and ncont are read together");
+  for (int pass = 0; pass < 4; ++pass) {
+    hipError_t e =
+        hipMemcpyAsync(I.h_np, &I.bd->npairs, 2 * sizeof(unsigned int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess && flag) e = hipMemcpyAsync(I.h_done, flag, sizeof(int), hipMemcpyDeviceToHost, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return e;
-    if (*I.h_np <= I.cap) return hipSuccess;
-    e = grow_pairs(c, *I.h_np);
+    const bool pairs_over = I.h_np[0] > I.cap;
+    const bool cont_over = contacts && !pairs_over && I.h_np[1] > I.ecap;
+    if (!pairs_over && !cont_over) return hipSuccess;
+    e = pairs_over ? grow_pairs(c, I.h_np[0]) : grow_contacts(c, I.h_np[1]);
     if (e != hipSuccess) return e;
     redo();
   }
@@ -698,14 +758,30 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
   hipLaunchKernelGGL(vhc::k_ipc_detect_begin, dim3(1), dim3(64), 0, c->stream, I.ip);
   if (sc.self_collision) {
     enqueue_pairs(c, x, (float)sc.thickness, skip);
-    hipLaunchKernelGGL(vhc::k_zero_d, dim3(nblk(3 * n)), dim3(256), 0, c->stream, 3 * n, I.gself, skip);
     hipLaunchKernelGGL(vhc::k_tri_spheres, dim3(std::max(1, nblk(c->S.t))), dim3(256), 0, c->stream, c->S.t,
                        c->tri_idx, x, I.ccdtri, skip);
     // the survivor regions are shared with the CCD, which runs after the detection
-    hipLaunchKernelGGL(vhc::k_self_cull, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, I.pairs, &I.bd->npairs,
-                       I.cap, sc, I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, skip);
-    hipLaunchKernelGGL(vhc::k_self_contacts, dim3(pair_blocks(I.cap)), dim3(256), 0, c->stream, &I.bd->npairs, I.cap,
-                       I.ccd_surv, I.ccd_reg, I.ccd_cnt, c->tri_idx, x, I.xptr, I.xidx, sc, I.gself, I.ip, skip);
+    const unsigned int pb = pair_blocks(I.cap);
+    hipLaunchKernelGGL(vhc::k_self_cull, dim3(pb), dim3(256), 0, c->stream, I.pairs, &I.bd->npairs, I.cap, sc,
+                       I.ccdtri, I.ccd_surv, I.ccd_reg, I.ccd_cnt, skip);
+    // the gradients in a fixed order: count, offsets, entries, by vertex
+    hipLaunchKernelGGL(vhc::k_self_count, dim3(pb), dim3(256), 0, c->stream, &I.bd->npairs, I.cap, I.ccd_surv,
+                       I.ccd_reg, I.ccd_cnt, c->tri_idx, x, I.xptr, I.xidx, sc, I.cof, I.btot, I.ip, skip);
+    hipLaunchKernelGGL(vhc::k_self_base, dim3(1), dim3(256), 0, c->stream, (int)pb, I.btot, I.sbase, I.ecap, I.bd,
+                       I.ip, skip);
+    hipLaunchKernelGGL(vhc::k_self_emit, dim3(pb), dim3(256), 0, c->stream, I.ccd_surv, I.ccd_reg, I.ccd_cnt, I.cof,
+                       I.sbase, c->tri_idx, x, I.xptr, I.xidx, sc, I.ecap, I.ev, I.eg, skip);
+    hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vcnt, skip);
+    hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vfill, skip);
+    hipLaunchKernelGGL(vhc::k_vcount, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vcnt, skip);
+    size_t vb = I.vscan_bytes;
+    const hipError_t se = rocprim::exclusive_scan(I.vscan_tmp, vb, I.vcnt, I.vptr, 0u, (size_t)n + 1,
+                                                  rocprim::plus<unsigned int>(), c->stream);
+    if (se != hipSuccess && I.lib_err == hipSuccess) I.lib_err = se;
+    hipLaunchKernelGGL(vhc::k_vscatter, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vptr, I.vfill,
+                       I.vidx, skip);
+    hipLaunchKernelGGL(vhc::k_vgather, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.vptr, I.vidx, I.eg, I.gself,
+                       skip);
   }
   hipLaunchKernelGGL(vhc::k_vertex_contacts, dim3(nblk(n)), dim3(256), 0, c->stream, n, x, sc, I.gself, I.grad, I.nrm,
                      I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, hess, skip);
@@ -729,7 +805,7 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
                      Q.ground_height, Q.al_mu_initial, I.ip, c->st);
   enqueue_detect(c, c->x, I.sc, true, skip);
   if (I.sc.self_collision) {
-    hipError_t e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); });
+    hipError_t e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); }, true);
     if (e != hipSuccess) return e;
   }
   const double* dl = f64 ? (const double*)c->d : c->acc;
@@ -765,7 +841,7 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
     };
     al_close();
     if (I.sc.self_collision) {
-      hipError_t e = fit_pairs(c, al_close, skip);
+      hipError_t e = fit_pairs(c, al_close, true, skip);
       if (e != hipSuccess) return e;
       if (*I.h_done) break;
     } else if (al + 1 < Q.al_max_iterations) {
@@ -805,7 +881,7 @@ vh_status read_ipc_result(vh_ctx* ctx, vh_step_result* out, vh_ipc_stats* st) {
     st->active = (int64_t)h.active; st->self_contacts = (int64_t)h.self_contacts;
     st->pairs = ctx->ipc.sc.self_collision ? (int64_t)np : 0; st->last_alpha = h.last_alpha;
   }
-  if (h.overflow) return fail(ctx, VH_E_INTERNAL, "candidate pairs exceeded the pair buffer");
+  if (h.overflow) return fail(ctx, VH_E_INTERNAL, "candidate pairs or self contacts exceeded their buffers");
   if (!(h.resid == h.resid)) return fail(ctx, VH_E_NONFINITE, "non-finite positions in the IPC step");
   return VH_OK;
 }
@@ -1620,6 +1696,79 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
   return VH_OK;
 }
 
+vh_status vh_pd_profile_graph(vh_ctx* ctx, double dt, uint32_t reps, vh_graph_profile* out) {
+  constexpr int kLocalPerGraph = 10;
+  if (!ctx || !out) return VH_E_INVALID_ARG;
+  if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
+  if (!(dt > 0) || reps == 0) return fail(ctx, VH_E_INVALID_ARG, "dt > 0 and reps > 0 required");
+  VH_HIP(hipSetDevice(ctx->device));
+  // algorithmic bytes of one sweep and of the local step, as vh_pd_profile counts them
+  vh_profile p{};
+  {
+    const vh::Setup& S = ctx->S;
+    if (S.max_depth + 1 > VH_MAX_BANDS) return fail(ctx, VH_E_UNSUPPORTED, "more bands than vh_profile holds");
+  }
+  const int saved = ctx->P.ir_sweeps;
+  ctx->P.ir_sweeps = 1;
+  const vh_status ps = vh_pd_profile(ctx, dt, 1, &p);
+  ctx->P.ir_sweeps = saved;
+  if (ps != VH_OK) return ps;
+  std::memset(out, 0, sizeof(*out));
+  out->reps = reps;
+  out->solve_launches = 2 * p.bands - 1;
+  for (uint32_t l = 0; l < p.bands; ++l) out->solve_bytes += p.fwd_bytes[l] + p.bwd_bytes[l];
+  out->local_bytes = p.local_bytes;
+  hipGraphExec_t ex[2] = {nullptr, nullptr};
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto cleanup = [&] {
+    for (auto& x : ex)
+      if (x) (void)hipGraphExecDestroy(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  for (int which = 0; which < 2; ++which) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      if (which == 0) {
+        // several local steps in one graph: gaps between launches as in a step
+        for (int k = 0; k < kLocalPerGraph; ++k) (void)enqueue_local_step(ctx, nullptr);
+      } else {
+        Marks none;
+        if (ctx->prec == VH_PREC_F64) enqueue_solve<double>(ctx, dt, true, true, false, false, nullptr, none);
+        else enqueue_solve<float>(ctx, dt, true, true, false, false, nullptr, none);
+      }
+      const hipError_t le = hipGetLastError();
+      e = hipStreamEndCapture(ctx->stream, &g);
+      if (e == hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&ex[which], g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      cleanup();
+      return fail(ctx, VH_E_GPU, std::string("profile graph: ") + hipGetErrorString(e));
+    }
+  }
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  double us[2] = {0, 0};
+  for (int which = 0; which < 2 && e == hipSuccess; ++which) {
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = hipGraphLaunch(ex[which], ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, ctx->stream);
+    for (uint32_t k = 0; k < reps && e == hipSuccess; ++k) e = hipGraphLaunch(ex[which], ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    us[which] = 1000.0 * ms / reps;
+  }
+  cleanup();
+  if (e != hipSuccess) return fail(ctx, VH_E_GPU, std::string("profile graph: ") + hipGetErrorString(e));
+  out->local_us = us[0] / kLocalPerGraph;
+  out->solve_us = us[1];
+  return VH_OK;
+}
+
 vh_status vh_pd_lumped_masses(vh_ctx* ctx, float* out) {
   if (!ctx || !out) return VH_E_INVALID_ARG;
   if (!ctx->initialized) return fail(ctx, VH_E_NOT_INITIALIZED, "Solver not initialized. Call init() first.");
@@ -1707,11 +1856,19 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   I.ccd_reg = ccd_region(I.cap);
   UP(dalloc(ctx, &I.ccd_surv, (size_t)pair_blocks(I.cap) * I.ccd_reg));
   UP(dalloc(ctx, &I.ccd_cnt, (size_t)kMaxPairBlocks));
+  UP(dalloc(ctx, &I.cof, (size_t)pair_blocks(I.cap) * I.ccd_reg));
+  UP(dalloc(ctx, &I.btot, (size_t)kMaxPairBlocks)); UP(dalloc(ctx, &I.sbase, (size_t)kMaxPairBlocks));
+  I.ecap = (unsigned int)std::max(8192, t / 4);
+  UP(dalloc(ctx, &I.ev, 4 * (size_t)I.ecap)); UP(dalloc(ctx, &I.eg, 12 * (size_t)I.ecap));
+  UP(dalloc(ctx, &I.vidx, 4 * (size_t)I.ecap));
+  UP(dalloc(ctx, &I.vcnt, (size_t)n + 1)); UP(dalloc(ctx, &I.vfill, (size_t)n + 1));
+  UP(dalloc(ctx, &I.vptr, (size_t)n + 1));
   UP(dalloc(ctx, &I.bd, 1)); UP(dalloc(ctx, &I.ip, 1));
-  VH_HIP(hipHostMalloc((void**)&I.h_done, 2 * sizeof(int), hipHostMallocDefault));
+  // pinned: [0] the AL flag, [1] the pair count, [2] the self-contact count
+  VH_HIP(hipHostMalloc((void**)&I.h_done, 4 * sizeof(int), hipHostMallocDefault));
   *I.h_done = 0;
   I.h_np = reinterpret_cast<unsigned int*>(I.h_done + 1);
-  *I.h_np = 0;
+  I.h_np[0] = I.h_np[1] = 0;
   size_t bytes = 0;
   VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
                                    ctx->stream));
@@ -1722,8 +1879,13 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   VH_HIP(rocprim::exclusive_scan(nullptr, bytes, I.pcount, I.poff, 0u, (size_t)tt, rocprim::plus<unsigned int>(),
                                  ctx->stream));
   I.scan_bytes = std::max<size_t>(bytes, 16);
+  bytes = 0;
+  VH_HIP(rocprim::exclusive_scan(nullptr, bytes, I.vcnt, I.vptr, 0u, (size_t)n + 1, rocprim::plus<unsigned int>(),
+                                 ctx->stream));
+  I.vscan_bytes = std::max<size_t>(bytes, 16);
   UP(dalloc_bytes(ctx, &I.sort_tmp, I.sort_bytes));
   UP(dalloc_bytes(ctx, &I.scan_tmp, I.scan_bytes));
+  UP(dalloc_bytes(ctx, &I.vscan_tmp, I.vscan_bytes));
 #undef UP
   VH_HIP(hipMemset(I.gself, 0, n3 * 8));
   VH_HIP(hipMemset(I.lambda, 0, n3 * 8));
@@ -1803,7 +1965,7 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   };
   detect();
   VH_HIP(hipGetLastError());
-  if (sc.self_collision) VH_HIP(fit_pairs(ctx, detect));
+  if (sc.self_collision) VH_HIP(fit_pairs(ctx, detect, true));
   VH_HIP(take_lib_err(ctx));
   const size_t n3 = 3 * (size_t)n;
   std::vector<double> g(n3), nr(n3), hd(n);
@@ -1850,7 +2012,7 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   };
   ccd();
   VH_HIP(hipGetLastError());
-  if (I.sc.self_collision) VH_HIP(fit_pairs(ctx, ccd));
+  if (I.sc.self_collision) VH_HIP(fit_pairs(ctx, ccd, false));
   VH_HIP(take_lib_err(ctx));
   vhc::IpcDev h{};
   unsigned int np = 0;
@@ -1884,7 +2046,7 @@ vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
   };
   cross();
   VH_HIP(hipGetLastError());
-  VH_HIP(fit_pairs(ctx, cross));
+  VH_HIP(fit_pairs(ctx, cross, false));
   VH_HIP(take_lib_err(ctx));
   vhc::BvhDev h{};
   VH_HIP(hipMemcpyAsync(&h, I.bd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -1908,7 +2070,7 @@ vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* 
   VH_HIP(hipMemcpyAsync(I.scratch_a, ctx->hbuf.data(), ctx->hbuf.size() * 8, hipMemcpyHostToDevice, ctx->stream));
   enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr);
   VH_HIP(hipGetLastError());
-  VH_HIP(fit_pairs(ctx, [&] { enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr); }));
+  VH_HIP(fit_pairs(ctx, [&] { enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr); }, false));
   VH_HIP(take_lib_err(ctx));
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));

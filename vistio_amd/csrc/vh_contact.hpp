@@ -320,6 +320,7 @@ __device__ __forceinline__ float o2f(unsigned int u) {
 struct BvhDev {
   unsigned int scene[6];  // order-mapped lo xyz, hi xyz of the box centroids
   unsigned int npairs;    // pairs found (may exceed the capacity)
+  unsigned int ncont;     // self contacts of the last detection (may exceed the entry buffer)
   unsigned int crossings; // triangle pairs that cross (k_tri_crossings)
 };
 
@@ -735,7 +736,7 @@ struct CcdTri {
 __device__ __forceinline__ float round_up(double v) { return (float)(v * (1.0 + 1e-6)) + 1e-30f; }
 
 // Bounding sphere of each triangle at x (CcdTri with m = 0), for the pair
-// cull of k_self_contacts.
+// cull of the self-contact tests.
 __global__ void __launch_bounds__(256) k_tri_spheres(int t, const int* __restrict__ tri, const double* __restrict__ x,
                                                      CcdTri* __restrict__ out, const int* skip) {
   if (skip && *skip) return;
@@ -753,15 +754,10 @@ __global__ void __launch_bounds__(256) k_tri_spheres(int t, const int* __restric
   out[e] = o;
 }
 
-__device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
-  atomicAdd(g + 3 * i + 0, s * v.x);
-  atomicAdd(g + 3 * i + 1, s * v.y);
-  atomicAdd(g + 3 * i + 2, s * v.z);
-}
-
 // self_collision.rs:199-276 with ipc_response.rs:58-219: the vertex-triangle
-// (6) and edge-edge (9) tests of one candidate pair, barrier gradients added
-// to gself (FP64 atomics).
+// (6) and edge-edge (9) tests of each candidate pair (self_pair_contacts),
+// their barrier gradients summed per vertex in a fixed order (k_self_count
+// .. k_vgather below).
 //
 // Pair cull (exact): no two points of A and B are closer than the gap of
 // their bounding spheres (CcdTri centre and radius at x, k_tri_spheres). When
@@ -770,10 +766,10 @@ __device__ __forceinline__ void atomic_add3(double* g, int i, D3 v, double s) {
 // is skipped before its exclusion lookups.
 //
 // k_self_cull applies the cull at full occupancy and writes each block's
-// survivors to its region of surv (as k_ccd_cull); k_self_contacts runs the
-// tests on block b's survivors, every lane on a pair that needs them. Tested
-// where they lay, most lanes of a wave sat idle (1002 us at 4.4 M candidate
-// pairs); queued in LDS inside the test kernel, 512 us.
+// survivors to its region of surv (as k_ccd_cull); the test kernels run on
+// block b's survivors, every lane on a pair that needs them. Tested where
+// they lay, most lanes of a wave sat idle (1002 us at 4.4 M candidate pairs);
+// queued in LDS inside the test kernel, 512 us.
 __global__ void __launch_bounds__(256) k_self_cull(const int2* __restrict__ pairs, const unsigned int* npairs_p,
                                                    unsigned int cap, Scene sc, const CcdTri* __restrict__ sph,
                                                    int2* __restrict__ surv, unsigned int reg,
@@ -791,25 +787,20 @@ __global__ void __launch_bounds__(256) k_self_cull(const int2* __restrict__ pair
   if (threadIdx.x == 0) scnt[blockIdx.x] = n;
 }
 
-// Four waves per SIMD (128 VGPRs, 8 bytes of spill) measured 14 % faster than
-// the compiler's own 130 VGPRs at three; five or six spill 100+ bytes a lane.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_self_contacts(const unsigned int* npairs_p,
-                                                       unsigned int cap, const int2* __restrict__ surv, unsigned int reg,
-                                                       const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
-                                                       const double* __restrict__ x, const int* __restrict__ xptr,
-                                                       const int* __restrict__ xidx, Scene sc,
-                                                       double* __restrict__ gself, IpcDev* ip, const int* skip) {
-  if (skip && *skip) return;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && *npairs_p > cap) ip->overflow = 1;
+// The tests of one candidate pair that survived the cull, in the reference's
+// order: on_contact(d2) for each active contact; with kWant, also
+// emit(vertices[4], gradients[4], scaled barrier gradient), the
+// contributions ipc_response.rs:170-219 adds to the four vertices.
+template <bool kWant, class OnContact, class Emit>
+__device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __restrict__ tri,
+                                                   const double* __restrict__ x, const int* __restrict__ xptr,
+                                                   const int* __restrict__ xidx, const Scene& sc, OnContact&& on_contact,
+                                                   Emit&& emit) {
   const double dh = sc.dh;
   auto bgrad = [&](double d2) {
     return sc.thickness > 0.0 ? scaled_barrier_gradient_thick(d2, dh, sc.kappa, sc.thickness)
                               : scaled_barrier_gradient(d2, dh, sc.kappa);
   };
-  unsigned long long cnt = 0;
-  bool viol = false;
-  // The tests of one pair that survived the cull.
-  auto pair_tests = [&](const int2 pr) {
     const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
     const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
     // Every exclusion test of the pair is between a vertex of A and one of
@@ -857,15 +848,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       const D3 p = ld(x, v), a = ld(x, t0), b = ld(x, t1), c = ld(x, t2);
       const double d2 = pt_dist2(p, a, b, c);
       if (d2 < dh && d2 > 0.0) {
-        D3 g[4];
-        pt_grad(p, a, b, c, g);
-        const double bg = bgrad(d2);
-        atomic_add3(gself, v, g[0], bg);
-        atomic_add3(gself, t0, g[1], bg);
-        atomic_add3(gself, t1, g[2], bg);
-        atomic_add3(gself, t2, g[3], bg);
-        ++cnt;
-        if (d2 <= 1e-12) viol = true;
+        if constexpr (kWant) {
+          D3 g[4];
+          pt_grad(p, a, b, c, g);
+          const int vs[4] = {v, t0, t1, t2};
+          emit(vs, g, bgrad(d2));
+        }
+        on_contact(d2);
       }
     }
     // edge-edge tests: edge i of A (A[i], A[i+1]) against edge j of B
@@ -881,31 +870,207 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
       const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
       const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
       if (d2 < dh && d2 > 0.0) {
-        D3 g[4];
-        ee_grad(pa0, pa1, pb0, pb1, g);
-        const double bg = bgrad(d2);
-        atomic_add3(gself, a0, g[0], bg);
-        atomic_add3(gself, a1, g[1], bg);
-        atomic_add3(gself, b0, g[2], bg);
-        atomic_add3(gself, b1, g[3], bg);
-        ++cnt;
-        if (d2 <= 1e-12) viol = true;
+        if constexpr (kWant) {
+          D3 g[4];
+          ee_grad(pa0, pa1, pb0, pb1, g);
+          const int vs[4] = {a0, a1, b0, b1};
+          emit(vs, g, bgrad(d2));
+        }
+        on_contact(d2);
       }
     }
-  };
+}
+
+// The self-contact gradients, summed in a fixed order (no floating-point
+// atomics, so a detection gives the same bits every time):
+//   k_self_count  the contacts of each surviving pair, their offsets within
+//                 block b's survivors (cof) and the block's total (btot);
+//                 the contact counters and the violation flag
+//   k_self_base   the blocks' offsets and the total (BvhDev::ncont)
+//   k_self_emit   each contact's four (vertex, gradient) entries at its
+//                 offset, in survivor and test order
+//   k_vcount / k_vscatter  the entries grouped by vertex (integer counts)
+//   k_vgather     each vertex's sum over its entries in ascending entry order
+// Entries past the buffer (ecap contacts) are dropped and flag an overflow,
+// as a pair overflow does; the host grows the buffer and repeats.
+//
+// Four waves per SIMD (128 VGPRs) measured 14 % faster than three for the
+// tests with gradients (k_self_emit).
+__global__ void __launch_bounds__(256) k_self_count(const unsigned int* npairs_p, unsigned int cap,
+                                                    const int2* __restrict__ surv, unsigned int reg,
+                                                    const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
+                                                    const double* __restrict__ x, const int* __restrict__ xptr,
+                                                    const int* __restrict__ xidx, Scene sc,
+                                                    unsigned int* __restrict__ cof, unsigned int* __restrict__ btot,
+                                                    IpcDev* ip, const int* skip) {
+  if (skip && *skip) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && *npairs_p > cap) ip->overflow = 1;
+  __shared__ unsigned int wsum[4];
   const unsigned int n = scnt[blockIdx.x];
   const int2* in = surv + (size_t)blockIdx.x * reg;
-  for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) pair_tests(in[k]);
-  // one counter update per wave
-  for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+  unsigned int* out = cof + (size_t)blockIdx.x * reg;
+  const unsigned int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned int carry = 0;
+  unsigned long long total = 0;
+  bool viol = false;
+  for (unsigned int base = 0; base < n; base += blockDim.x) {
+    const unsigned int k = base + threadIdx.x;
+    unsigned int c = 0;
+    if (k < n)
+      self_pair_contacts<false>(in[k], tri, x, xptr, xidx, sc, [&](double d2) {
+        ++c;
+        if (d2 <= 1e-12) viol = true;
+      }, [](const int*, const D3*, double) {});
+    // exclusive scan of c over the block, in survivor order
+    unsigned int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned int v = __shfl_up(incl, o);
+      if ((int)lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    unsigned int before = 0, all = 0;
+#pragma unroll
+    for (unsigned int w = 0; w < 4; ++w) {
+      before += w < wave ? wsum[w] : 0u;
+      all += wsum[w];
+    }
+    if (k < n) out[k] = carry + before + incl - c;
+    carry += all;
+    total += c;
+    __syncthreads();  // wsum is rewritten by the next round
+  }
+  if (threadIdx.x == 0) btot[blockIdx.x] = carry;
+  for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
   const bool any_viol = __ballot(viol) != 0;
-  if ((threadIdx.x & 63) == 0) {
-    if (cnt) {
-      atomicAdd(&ip->self_contacts, cnt);
-      atomicAdd(&ip->active, cnt);
+  if (lane == 0) {
+    if (total) {
+      atomicAdd(&ip->self_contacts, total);
+      atomicAdd(&ip->active, total);
     }
     if (any_viol) atomicMax(&ip->viol, (unsigned long long)__double_as_longlong(1e-3));
   }
+}
+
+// One block: exclusive scan of the nb block totals into base, the total into
+// bd->ncont; a total past ecap contacts flags an overflow.
+__global__ void __launch_bounds__(256) k_self_base(int nb, const unsigned int* __restrict__ btot,
+                                                   unsigned int* __restrict__ base, unsigned int ecap, BvhDev* bd,
+                                                   IpcDev* ip, const int* skip) {
+  if (skip && *skip) return;
+  __shared__ unsigned int wsum[4];
+  const unsigned int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned int carry = 0;
+  for (int b0 = 0; b0 < nb; b0 += blockDim.x) {
+    const int b = b0 + threadIdx.x;
+    const unsigned int c = b < nb ? btot[b] : 0u;
+    unsigned int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned int v = __shfl_up(incl, o);
+      if ((int)lane >= o) incl += v;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    unsigned int before = 0, all = 0;
+#pragma unroll
+    for (unsigned int w = 0; w < 4; ++w) {
+      before += w < wave ? wsum[w] : 0u;
+      all += wsum[w];
+    }
+    if (b < nb) base[b] = carry + before + incl - c;
+    carry += all;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bd->ncont = carry;
+    if (carry > ecap) ip->overflow = 1;
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+k_self_emit(const int2* __restrict__ surv, unsigned int reg, const unsigned int* __restrict__ scnt,
+            const unsigned int* __restrict__ cof, const unsigned int* __restrict__ base, const int* __restrict__ tri,
+            const double* __restrict__ x, const int* __restrict__ xptr, const int* __restrict__ xidx, Scene sc,
+            unsigned int ecap, int* __restrict__ ev, double* __restrict__ eg, const int* skip) {
+  if (skip && *skip) return;
+  const unsigned int n = scnt[blockIdx.x];
+  const int2* in = surv + (size_t)blockIdx.x * reg;
+  const unsigned int* off = cof + (size_t)blockIdx.x * reg;
+  const unsigned int b0 = base[blockIdx.x];
+  const unsigned int bend = blockIdx.x + 1 < gridDim.x ? base[blockIdx.x + 1] : 0xffffffffu;
+  for (unsigned int k = threadIdx.x; k < n; k += blockDim.x) {
+    // the pair's contact count is the gap to the next offset: most survivors
+    // have none, and their lanes skip the tests
+    const unsigned int next = k + 1 < n ? b0 + off[k + 1] : bend;
+    unsigned int c = b0 + off[k];
+    if (next == c) continue;
+    self_pair_contacts<true>(in[k], tri, x, xptr, xidx, sc, [](double) {}, [&](const int* vs, const D3* g, double bg) {
+      if (c < ecap) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const size_t e = 4 * (size_t)c + q;
+          ev[e] = vs[q];
+          eg[3 * e + 0] = bg * g[q].x;
+          eg[3 * e + 1] = bg * g[q].y;
+          eg[3 * e + 2] = bg * g[q].z;
+        }
+      }
+      ++c;
+    });
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill_u(int count, unsigned int* __restrict__ p, const int* skip) {
+  if (skip && *skip) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = 0u;
+}
+
+// The entries of each vertex: counts (vcnt[n] stays 0, so the exclusive scan
+// of n + 1 counts ends with the number of entries).
+__global__ void __launch_bounds__(256) k_vcount(const BvhDev* bd, unsigned int ecap, const int* __restrict__ ev,
+                                                unsigned int* __restrict__ vcnt, const int* skip) {
+  if (skip && *skip) return;
+  const unsigned int ne = 4u * (bd->ncont < ecap ? bd->ncont : ecap);
+  for (unsigned int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x)
+    atomicAdd(&vcnt[ev[e]], 1u);
+}
+
+// Entry e into its vertex's segment [vptr[v], vptr[v + 1]), in any order.
+__global__ void __launch_bounds__(256) k_vscatter(const BvhDev* bd, unsigned int ecap, const int* __restrict__ ev,
+                                                  const unsigned int* __restrict__ vptr, unsigned int* __restrict__ vfill,
+                                                  unsigned int* __restrict__ vidx, const int* skip) {
+  if (skip && *skip) return;
+  const unsigned int ne = 4u * (bd->ncont < ecap ? bd->ncont : ecap);
+  for (unsigned int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+    const int v = ev[e];
+    vidx[vptr[v] + atomicAdd(&vfill[v], 1u)] = e;
+  }
+}
+
+// g[v] = the sum of vertex v's entries in ascending entry order (the
+// segment's entries are picked smallest first), 0 without entries.
+__global__ void __launch_bounds__(256) k_vgather(int n, const unsigned int* __restrict__ vptr,
+                                                 const unsigned int* __restrict__ vidx, const double* __restrict__ eg,
+                                                 double* __restrict__ g, const int* skip) {
+  if (skip && *skip) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const unsigned int b = vptr[v], e = vptr[v + 1];
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  unsigned int last = 0;
+  for (unsigned int done = 0; done < e - b; ++done) {
+    unsigned int pick = 0xffffffffu;
+    for (unsigned int q = b; q < e; ++q) {
+      const unsigned int id = vidx[q];
+      if ((done == 0 || id > last) && id < pick) pick = id;
+    }
+    sx += eg[3 * (size_t)pick + 0];
+    sy += eg[3 * (size_t)pick + 1];
+    sz += eg[3 * (size_t)pick + 2];
+    last = pick;
+  }
+  g[3 * v + 0] = sx; g[3 * v + 1] = sy; g[3 * v + 2] = sz;
 }
 
 // Per-vertex half of collision_pipeline.rs:207-351: self-contact normal from
@@ -1205,8 +1370,10 @@ __global__ void __launch_bounds__(256) k_ccd_pairs(const int2* __restrict__ surv
     }
     if constexpr (EE) {
       // edges (0,1), (1,2), (2,0) of A against those of B, skipping pairs
-      // that share a vertex
+      // that share a vertex (unrolled: the corner arrays stay in registers)
+#pragma unroll
       for (int ea = 0; ea < 3; ++ea)
+#pragma unroll
         for (int eb = 0; eb < 3; ++eb) {
           const int a0 = ea, a1 = ea == 2 ? 0 : ea + 1, b0 = 3 + eb, b1 = eb == 2 ? 3 : 4 + eb;
           if (V[a0] == V[b0] || V[a0] == V[b1] || V[a1] == V[b0] || V[a1] == V[b1]) continue;

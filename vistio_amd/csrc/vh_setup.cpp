@@ -223,40 +223,68 @@ bool build_setup(const SetupInput& in, Setup& S) {
 
   // System matrix A = M/h^2 + sum w g g^T + sum w_b s s^T, assembly.rs:44-134,
   // with the same |value| > 1e-12 filter on each triplet.
-  struct Trip { int32_t r, c; double v; };
-  std::vector<Trip> trip;
-  trip.reserve((size_t)n + 9 * (size_t)t + 16 * (size_t)h);
+  // The triplets are bucketed by row with a counting sort (stable: each
+  // row's triplets stay in the order they are generated), then each row's
+  // few entries are stably sorted by column and equal (row, column) runs are
+  // summed in that order. Two passes over the elements: count, then fill.
   const double inv_dt2 = 1.0 / (in.dt_matrix * in.dt_matrix);
-  for (int i = 0; i < n; ++i) trip.push_back({i, i, mass[i] * inv_dt2});
-  for (int e = 0; e < t; ++e) {
-    const double* d = &dm[4 * (size_t)e];
-    const double g[3][2] = {{-(d[0] + d[1]), -(d[2] + d[3])}, {d[0], d[2]}, {d[1], d[3]}};
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) {
-        const double v = g[a][0] * g[b][0] + g[a][1] * g[b][1];
-        if (std::fabs(v) > 1e-12) trip.push_back({(int32_t)in.tri[3 * e + a], (int32_t)in.tri[3 * e + b], tw[e] * v});
-      }
+  std::vector<int64_t> rs((size_t)n + 1, 0);
+  std::vector<int32_t> tc;
+  std::vector<double> tv;
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int64_t> at;
+    if (pass == 1) {
+      for (int i = 0; i < n; ++i) rs[i + 1] += rs[i];
+      tc.resize((size_t)rs[n]);
+      tv.resize((size_t)rs[n]);
+      at.assign(rs.begin(), rs.end() - 1);
+    }
+    auto put = [&](int32_t r, int32_t c, double v) {
+      if (pass == 0) { ++rs[(size_t)r + 1]; return; }
+      const int64_t k = at[r]++;
+      tc[(size_t)k] = c;
+      tv[(size_t)k] = v;
+    };
+    for (int i = 0; i < n; ++i) put(i, i, mass[i] * inv_dt2);
+    for (int e = 0; e < t; ++e) {
+      const double* d = &dm[4 * (size_t)e];
+      const double g[3][2] = {{-(d[0] + d[1]), -(d[2] + d[3])}, {d[0], d[2]}, {d[1], d[3]}};
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+          const double v = g[a][0] * g[b][0] + g[a][1] * g[b][1];
+          if (std::fabs(v) > 1e-12) put((int32_t)in.tri[3 * e + a], (int32_t)in.tri[3 * e + b], tw[e] * v);
+        }
+    }
+    for (int e = 0; e < h; ++e)
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b) {
+          const double* st = &hst[4 * (size_t)e];
+          const double v = hw[e] * st[a] * st[b];
+          if (std::fabs(v) > 1e-12) put((int32_t)in.hinge[4 * e + a], (int32_t)in.hinge[4 * e + b], v);
+        }
   }
-  for (int e = 0; e < h; ++e)
-    for (int a = 0; a < 4; ++a)
-      for (int b = 0; b < 4; ++b) {
-        const double* st = &hst[4 * (size_t)e];
-        const double v = hw[e] * st[a] * st[b];
-        if (std::fabs(v) > 1e-12)
-          trip.push_back({(int32_t)in.hinge[4 * e + a], (int32_t)in.hinge[4 * e + b], v});
-      }
-  std::sort(trip.begin(), trip.end(), [](const Trip& a, const Trip& b) { return a.r < b.r || (a.r == b.r && a.c < b.c); });
   std::vector<int32_t> Ap(n + 1, 0), Ai;
   std::vector<double> Ax;
-  for (size_t k = 0; k < trip.size();) {
-    size_t j = k;
-    double s = 0;
-    while (j < trip.size() && trip[j].r == trip[k].r && trip[j].c == trip[k].c) s += trip[j++].v;
-    Ai.push_back(trip[k].c); Ax.push_back(s); Ap[trip[k].r + 1]++;
-    k = j;
+  {
+    std::vector<int32_t> ord;
+    for (int r = 0; r < n; ++r) {
+      const int64_t b = rs[r], e = rs[r + 1];
+      ord.resize((size_t)(e - b));
+      std::iota(ord.begin(), ord.end(), 0);
+      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return tc[(size_t)(b + x)] < tc[(size_t)(b + y)]; });
+      for (size_t k = 0; k < ord.size();) {
+        const int32_t c = tc[(size_t)(b + ord[k])];
+        double sum = 0;
+        size_t j = k;
+        while (j < ord.size() && tc[(size_t)(b + ord[j])] == c) sum += tv[(size_t)(b + ord[j++])];
+        Ai.push_back(c); Ax.push_back(sum); Ap[r + 1]++;
+        k = j;
+      }
+    }
   }
   for (int i = 0; i < n; ++i) Ap[i + 1] += Ap[i];
-  trip.clear(); trip.shrink_to_fit();
+  tc.clear(); tc.shrink_to_fit();
+  tv.clear(); tv.shrink_to_fit();
 
   // Nested dissection, one tree per connected component (a forest for
   // several garments; their blocks never couple, so each tree has its own

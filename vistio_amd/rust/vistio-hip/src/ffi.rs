@@ -124,6 +124,18 @@ pub struct VhMaterialArrays {
     pub density: *const f64,
 }
 
+/// `vh_graph_profile`.
+#[repr(C)]
+#[derive(Debug, Clone, Copy, Default)]
+pub struct VhGraphProfile {
+    pub reps: u32,
+    pub solve_launches: u32,
+    pub solve_us: f64,
+    pub solve_bytes: f64,
+    pub local_us: f64,
+    pub local_bytes: f64,
+}
+
 /// `vh_profile`.
 #[repr(C)]
 #[derive(Debug, Clone, Copy, Default)]
@@ -231,6 +243,7 @@ extern "C" {
     pub fn vh_pd_solve(ctx: *mut VhCtx, b: *const f64, x: *mut f64, sweeps: i32) -> VhStatus;
     pub fn vh_pd_info_get(ctx: *mut VhCtx, out: *mut VhPdInfo) -> VhStatus;
     pub fn vh_pd_profile(ctx: *mut VhCtx, dt: f64, iterations: u32, out: *mut VhProfile) -> VhStatus;
+    pub fn vh_pd_profile_graph(ctx: *mut VhCtx, dt: f64, reps: u32, out: *mut VhGraphProfile) -> VhStatus;
     pub fn vh_pd_lumped_masses(ctx: *mut VhCtx, out: *mut f32) -> VhStatus;
     pub fn vh_ipc_init(ctx: *mut VhCtx, params: *const VhIpcParams) -> VhStatus;
     pub fn vh_pd_step_ipc(

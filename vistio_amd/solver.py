@@ -598,6 +598,13 @@ class HipPdSolver:
                     bwd_bytes=list(p.bwd_bytes[:nb]), ir_us=p.ir_us, ir_bytes=p.ir_bytes,
                     iteration_us=p.iteration_us)
 
+    def profile_graph(self, dt: float, reps: int = 50) -> dict:
+        """vh_pd_profile_graph: one solve sweep and the local step, each
+        captured into a graph and replayed `reps` times between HIP events."""
+        g = nat.GraphProfile()
+        _check(self._L.vh_pd_profile_graph(self._ctx, float(dt), int(reps), C.byref(g)), self._ctx, "profile_graph")
+        return {k: getattr(g, k) for k, _ in nat.GraphProfile._fields_}
+
     def blocks(self, which: int) -> np.ndarray:
         """The device-built solve blocks as float64: 0 = M, 1 = M^T, 2 = the
         roots' dense inverses (vh_pd_blocks_get)."""
