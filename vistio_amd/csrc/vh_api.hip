@@ -104,7 +104,7 @@ struct vh_ctx {
   uint8_t* pinned = nullptr;
   // elements
   int *tri_idx = nullptr, *tri_slot = nullptr, *hinge_idx = nullptr, *hinge_slot = nullptr;
-  int* chunk_of = nullptr;  // k_local's block -> element chunk map
+  vhk::LocalMap local_map{};  // k_local's block -> element chunk map
   int local_blocks = 0;
   double* tri_dm = nullptr;
   void *tri_w = nullptr, *hinge_rest = nullptr, *hinge_w = nullptr;
@@ -373,7 +373,7 @@ template <class T>
 void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
   if (c->local_blocks > 0)
-    hipLaunchKernelGGL(vhk::k_local<T>, dim3(c->local_blocks), dim3(256), 0, c->stream, S.t, S.h, c->chunk_of,
+    hipLaunchKernelGGL(vhk::k_local<T>, dim3(c->local_blocks), dim3(256), 0, c->stream, S.t, S.h, c->local_map,
                        elem_args<T>(c), c->tri_slot, c->hinge_slot, (T*)c->cslot, st);
 }
 
@@ -1226,34 +1226,12 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->tri_idx, S.tri_idx));
   UP(upload(ctx, &ctx->tri_slot, S.tri_slot));
   {
-    // k_local's block order: triangle and hinge chunks merged by the smallest
-    // vertex label of their first element (both lists are sorted by it), the
-    // merged list cut into one contiguous run per XCD.
-    constexpr int kXcds = 8;
+    // k_local's block order (vhk::local_chunk): per XCD, one eighth of the
+    // triangle chunks and one eighth of the hinge chunks.
     const int tb = nblk(S.t), hb = nblk(S.h);
-    std::vector<int> order;
-    order.reserve((size_t)tb + hb);
-    auto tkey = [&](int j) {
-      const int* q = &S.tri_idx[3 * (size_t)j * 256];
-      return std::min(q[0], std::min(q[1], q[2]));
-    };
-    auto hkey = [&](int j) {
-      const int* q = &S.hinge_idx[4 * (size_t)j * 256];
-      return std::min(std::min(q[0], q[1]), std::min(q[2], q[3]));
-    };
-    for (int a = 0, b = 0; a < tb || b < hb;) {
-      if (b >= hb || (a < tb && tkey(a) <= hkey(b))) order.push_back(a++);
-      else order.push_back(~(b++));
-    }
-    const int nch = (int)order.size();
-    const int per = (nch + kXcds - 1) / kXcds;
-    std::vector<int> map((size_t)per * kXcds, vhk::kIdleChunk);
-    for (int blk = 0; blk < per * kXcds; ++blk) {
-      const int i = (blk % kXcds) * per + blk / kXcds;
-      if (i < nch) map[blk] = order[i];
-    }
-    ctx->local_blocks = nch > 0 ? per * kXcds : 0;
-    UP(upload(ctx, &ctx->chunk_of, map));
+    ctx->local_map.tper = (tb + vhk::kLocalXcds - 1) / vhk::kLocalXcds;
+    ctx->local_map.hper = (hb + vhk::kLocalXcds - 1) / vhk::kLocalXcds;
+    ctx->local_blocks = tb + hb > 0 ? vhk::kLocalXcds * (ctx->local_map.tper + ctx->local_map.hper) : 0;
   }
   UP(upload(ctx, &ctx->tri_dm, S.tri_dm));
   UP(upload_t(ctx, &ctx->tri_w, S.tri_w));

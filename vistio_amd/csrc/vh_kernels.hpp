@@ -396,39 +396,67 @@ __device__ __forceinline__ void hinge_delta(const ElemArgs<T>& ea, int e, T (&sg
 // assembly.rs:210, or the hinge's Discrete Shells stencil, :222-236), so the RHS
 // gather is a contiguous segmented sum with no atomics.
 //
-// Block order (chunk_of): the triangle and hinge chunks of 256 elements are
-// merged by position in the vertex order, and each XCD (workgroups are
-// handed to the eight XCDs in turn) takes one contiguous run of that list.
-// The slots of a vertex, written by its triangles and its hinges, are then
-// written by one XCD at about the same time and complete in its L2.
-// chunk_of[b] >= 0: triangle chunk, < 0: hinge chunk ~c, kIdleChunk: none.
+// Block order: workgroups go to the eight XCDs in turn, and XCD x takes the
+// x-th eighth of the triangle chunks (256 elements each, sorted by their
+// smallest vertex label) and the x-th eighth of the hinge chunks,
+// interleaved in proportion (Bresenham: chunk j of the XCD's run is a
+// triangle chunk when floor((j+1) tper / (tper+hper)) steps). Both eighths
+// cover about the same vertices, so the slots of a vertex, written by its
+// triangles and its hinges, are written by one XCD at about the same time
+// and complete in its L2. The map is arithmetic: no load before the
+// element indices.
+struct LocalMap {
+  int tper, hper;  // triangle / hinge chunks per XCD (the grid is 8 (tper + hper) blocks)
+};
+constexpr int kLocalXcds = 8;
+
+// Element chunk of block b: >= 0 triangle chunk, < 0 hinge chunk ~c,
+// kIdleChunk for none.
 constexpr int kIdleChunk = -0x7fffffff - 1;
+__device__ __forceinline__ int local_chunk(const LocalMap& m, int b, int tb, int hb) {
+  const int x = b % kLocalXcds, j = b / kLocalXcds, run = m.tper + m.hper;
+  if (j >= run) return kIdleChunk;
+  const int tj = (int)((long long)j * m.tper / run), tn = (int)((long long)(j + 1) * m.tper / run);
+  if (tn > tj) {
+    const int c = x * m.tper + tj;
+    return c < tb ? c : kIdleChunk;
+  }
+  const int c = x * m.hper + (j - tj);
+  return c < hb ? ~c : kIdleChunk;
+}
 
 template <class T>
-__global__ void __launch_bounds__(256) k_local(int t, int h, const int* __restrict__ chunk_of, ElemArgs<T> ea,
+__global__ void __launch_bounds__(256) k_local(int t, int h, LocalMap lm, ElemArgs<T> ea,
                                                const int* __restrict__ tri_slot, const int* __restrict__ hinge_slot,
                                                T* __restrict__ cslot, const StepDev* st) {
   if (st && st->done) return;
-  const int ch = chunk_of[blockIdx.x];
+  const int ch = local_chunk(lm, (int)blockIdx.x, (t + 255) / 256, (h + 255) / 256);
   if (ch == kIdleChunk) return;
   if (ch >= 0) {
     const int e = ch * blockDim.x + threadIdx.x;
     if (e >= t) return;
+    // the output slots are loaded with the element's indices, not after it
+    int sl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) sl[k] = tri_slot[3 * e + k];
     T w, g[3][2], D[2][3];
     tri_delta(ea, e, w, g, D);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-      T* o = cslot + 3 * (size_t)tri_slot[3 * e + k];
+      T* o = cslot + 3 * (size_t)sl[k];
       tri_corner(w, g, D, k, o[0], o[1], o[2]);
     }
   } else {
     const int e = (~ch) * blockDim.x + threadIdx.x;
     if (e >= h) return;
+    int sl[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sl[k] = hinge_slot[4 * e + k];
     T sg[4], cx, cy, cz;
     hinge_delta(ea, e, sg, cx, cy, cz);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      T* o = cslot + 3 * (size_t)hinge_slot[4 * e + k];
+      T* o = cslot + 3 * (size_t)sl[k];
       o[0] = sg[k] * cx; o[1] = sg[k] * cy; o[2] = sg[k] * cz;
     }
   }
@@ -669,12 +697,16 @@ __device__ __forceinline__ int task_of(const BandArgs<T>& a, int b) {
 // so each update row lands in its slot of the parent's front. Only the front
 // entries the task's rows read are assembled.
 // Each lane group of G lanes takes rows b + grp, b + grp + ngrp, ...; the
-// first pref_rows<T>() of them (kPrefLoads entries per lane each) and
-// their output slots are loaded before the front is assembled. Two rows:
-// four cost more in occupancy than they save in front re-reads.
-template <class T>
-constexpr int pref_rows() { return 2; }
-constexpr int kPrefLoads = 8;
+// first pref_rows<G>() of them (pref_loads<G>() entries per lane each) and
+// their output slots are loaded before the front is assembled. Two rows of
+// 8 loads: four rows cost more in occupancy than they save in front
+// re-reads. Groups of 64 lanes (rows over 256 entries, one row per group
+// in a task) take one row of 16 loads instead: the same registers, and a
+// row of up to 1024 entries is in flight before the barrier.
+template <int G>
+constexpr int pref_rows() { return G == 64 ? 1 : 2; }
+template <int G>
+constexpr int pref_loads() { return G == 64 ? 16 : 8; }
 
 __device__ __forceinline__ long long fwd_row_off(int row, int w, long long tri) {
   return row < w ? (long long)row * (row + 1) / 2 : tri + (long long)(row - w) * w;
@@ -743,7 +775,8 @@ __device__ __forceinline__ void assemble_front(const BandArgs<T>& a, int col0, i
 template <int G, bool S, class T>
 __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
                          T* __restrict__ F, T* front) {
-  constexpr int kPrefRows = pref_rows<T>();
+  constexpr int kPrefRows = pref_rows<G>();
+  constexpr int kPrefLoads = pref_loads<G>();
   const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int K = td.q2.y;
@@ -860,7 +893,8 @@ __device__ __forceinline__ void col_store(const BandArgs<T>& a, const CommitArgs
 template <int G, bool C, class T>
 __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
                          const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, double& df, double& nm) {
-  constexpr int kPrefRows = pref_rows<T>();
+  constexpr int kPrefRows = pref_rows<G>();
+  constexpr int kPrefLoads = pref_loads<G>();
   const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int m = w + r;
@@ -1017,7 +1051,8 @@ k_bwd_band_d5(BandArgs<double> a, CommitArgs c, const double* __restrict__ Mt, c
 template <int G, bool S, bool C, class T>
 __device__ void root_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Sroot_all,
                           T* __restrict__ F, T* __restrict__ V, T* __restrict__ d, T* front, double& df, double& nm) {
-  constexpr int kPrefRows = pref_rows<T>();
+  constexpr int kPrefRows = pref_rows<G>();
+  constexpr int kPrefLoads = pref_loads<G>();
   const T* __restrict__ Sroot = Sroot_all + (size_t)td.q2.z;
   const int col0 = td.q0.x, w = td.q0.y;
   const int b = td.q1.x, e = td.q1.y;

@@ -624,8 +624,8 @@ bool build_setup(const SetupInput& in, Setup& S) {
   // row (g ~ row length / 8, 4..64, so a row is usually 8 loads per lane)
   // and each lane group takes up to pr = 2 rows, whose loads are issued
   // together before the task's front is assembled (the kernels'
-  // pref_rows). A task holds about task_entries block entries: at least one
-  // row per group, at most pr.
+  // pref_rows; groups of 64 lanes prefetch one longer row). A task holds
+  // about task_entries block entries: at least one row per group, at most pr.
   const int kChunk = std::max(256, in.task_entries);
   const int pr = 2;
   auto pow2ceil = [](int v) { int p = 1; while (p < v) p <<= 1; return p; };
