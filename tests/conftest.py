@@ -71,7 +71,7 @@ def emul(tmp_path_factory):
     factorization, built with g++ for this test session."""
     out = tmp_path_factory.mktemp("emul") / "libemul.so"
     nat = os.path.join(ROOT, "tests", "native")
-    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-pthread", "-I", os.path.join(ROOT, "include"),
                     "-I", os.path.join(ROOT, "vistio_amd", "csrc"), os.path.join(nat, "setup_emul.cpp"),
                     os.path.join(nat, "host_factor.cpp"), os.path.join(ROOT, "vistio_amd", "csrc", "vh_setup.cpp"),
                     "-o", str(out)], check=True)

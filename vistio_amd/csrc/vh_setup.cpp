@@ -9,10 +9,12 @@
 #include "vh_setup.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
 #include <numeric>
+#include <thread>
 
 #include "vistio_hip.h"
 
@@ -28,6 +30,35 @@ inline D3 divs(D3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
 inline double dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline D3 cross(D3 a, D3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline double len(D3 a) { return std::sqrt(dot(a, a)); }
+
+// ord (the identity on entry) reordered stably by key(e) in [0, nkeys): a
+// counting sort, the order std::stable_sort gives.
+template <class K>
+void stable_order_by_key(std::vector<int>& ord, int nkeys, K key) {
+  std::vector<int> at((size_t)nkeys + 1, 0);
+  for (int e : ord) at[(size_t)key(e) + 1]++;
+  for (int k = 0; k < nkeys; ++k) at[k + 1] += at[k];
+  std::vector<int> out(ord.size());
+  for (int e : ord) out[(size_t)at[key(e)]++] = e;
+  ord.swap(out);
+}
+
+// Host threads for `count` independent items, at least min_per items each,
+// at most 16.
+int host_threads(int64_t count, int64_t min_per) {
+  const int64_t hw = std::max(1u, std::thread::hardware_concurrency());
+  return (int)std::max<int64_t>(1, std::min<int64_t>(std::min<int64_t>(hw, 16), count / std::max<int64_t>(1, min_per)));
+}
+// Runs f(begin, end, k) over [0, count) cut into nt contiguous ranges, range
+// k on thread k (in the calling thread when nt == 1).
+template <class F>
+void parallel_ranges(int64_t count, int nt, F&& f) {
+  if (nt <= 1) { f((int64_t)0, count, 0); return; }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int k = 0; k < nt; ++k) th.emplace_back([&f, count, nt, k] { f(count * k / nt, count * (k + 1) / nt, k); });
+  for (auto& x : th) x.join();
+}
 
 // compute_dm_inv — crates/vistio-solver/src/element.rs:228-263.
 void dm_inverse(D3 e1, D3 e2, double out[4]) {
@@ -82,7 +113,7 @@ struct NdBuilder {
   const double* Y;
   const double* Z;
   int leaf;
-  std::vector<int> mark;
+  int* mark;  // per vertex; builders of disjoint components may share it
   int stamp = 0;
   std::vector<NdNode> nodes;
 
@@ -222,74 +253,111 @@ bool build_setup(const SetupInput& in, Setup& S) {
   }
 
   // System matrix A = M/h^2 + sum w g g^T + sum w_b s s^T, assembly.rs:44-134,
-  // with the same |value| > 1e-12 filter on each triplet.
-  // The triplets are bucketed by row with a counting sort (stable: each
-  // row's triplets stay in the order they are generated), then each row's
-  // few entries are stably sorted by column and equal (row, column) runs are
-  // summed in that order. Two passes over the elements: count, then fill.
+  // with the same |value| > 1e-12 filter on each triplet. Row by row, rows
+  // on several threads: a row's candidate columns are the vertices of its
+  // elements, and its triplets are added in the order they are generated
+  // (the mass term, then the triangles, then the hinges, each in element
+  // order), so every entry is the same sum in the same order as one pass
+  // over all triplets; an entry no triplet passed the filter for is left out.
   const double inv_dt2 = 1.0 / (in.dt_matrix * in.dt_matrix);
-  std::vector<int64_t> rs((size_t)n + 1, 0);
-  std::vector<int32_t> tc;
-  std::vector<double> tv;
-  for (int pass = 0; pass < 2; ++pass) {
-    std::vector<int64_t> at;
-    if (pass == 1) {
-      for (int i = 0; i < n; ++i) rs[i + 1] += rs[i];
-      tc.resize((size_t)rs[n]);
-      tv.resize((size_t)rs[n]);
-      at.assign(rs.begin(), rs.end() - 1);
-    }
-    auto put = [&](int32_t r, int32_t c, double v) {
-      if (pass == 0) { ++rs[(size_t)r + 1]; return; }
-      const int64_t k = at[r]++;
-      tc[(size_t)k] = c;
-      tv[(size_t)k] = v;
+  // vertex -> incident triangles, then hinges, ascending, each element once
+  std::vector<int32_t> itp((size_t)n + 1, 0), ihp((size_t)n + 1, 0), itl, ihl;
+  {
+    auto first = [](const uint32_t* q, int a) {
+      for (int b = 0; b < a; ++b)
+        if (q[b] == q[a]) return false;
+      return true;
     };
-    for (int i = 0; i < n; ++i) put(i, i, mass[i] * inv_dt2);
-    for (int e = 0; e < t; ++e) {
-      const double* d = &dm[4 * (size_t)e];
-      const double g[3][2] = {{-(d[0] + d[1]), -(d[2] + d[3])}, {d[0], d[2]}, {d[1], d[3]}};
+    for (int e = 0; e < t; ++e)
       for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) {
-          const double v = g[a][0] * g[b][0] + g[a][1] * g[b][1];
-          if (std::fabs(v) > 1e-12) put((int32_t)in.tri[3 * e + a], (int32_t)in.tri[3 * e + b], tw[e] * v);
-        }
-    }
+        if (first(in.tri + 3 * (size_t)e, a)) itp[in.tri[3 * (size_t)e + a] + 1]++;
     for (int e = 0; e < h; ++e)
       for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b) {
-          const double* st = &hst[4 * (size_t)e];
-          const double v = hw[e] * st[a] * st[b];
-          if (std::fabs(v) > 1e-12) put((int32_t)in.hinge[4 * e + a], (int32_t)in.hinge[4 * e + b], v);
-        }
+        if (first(in.hinge + 4 * (size_t)e, a)) ihp[in.hinge[4 * (size_t)e + a] + 1]++;
+    for (int i = 0; i < n; ++i) { itp[i + 1] += itp[i]; ihp[i + 1] += ihp[i]; }
+    itl.resize((size_t)itp[n]); ihl.resize((size_t)ihp[n]);
+    std::vector<int32_t> at(itp.begin(), itp.end() - 1), ah(ihp.begin(), ihp.end() - 1);
+    for (int e = 0; e < t; ++e)
+      for (int a = 0; a < 3; ++a)
+        if (first(in.tri + 3 * (size_t)e, a)) itl[(size_t)at[in.tri[3 * (size_t)e + a]]++] = e;
+    for (int e = 0; e < h; ++e)
+      for (int a = 0; a < 4; ++a)
+        if (first(in.hinge + 4 * (size_t)e, a)) ihl[(size_t)ah[in.hinge[4 * (size_t)e + a]]++] = e;
   }
   std::vector<int32_t> Ap(n + 1, 0), Ai;
   std::vector<double> Ax;
   {
-    std::vector<int32_t> ord;
-    for (int r = 0; r < n; ++r) {
-      const int64_t b = rs[r], e = rs[r + 1];
-      ord.resize((size_t)(e - b));
-      std::iota(ord.begin(), ord.end(), 0);
-      std::stable_sort(ord.begin(), ord.end(), [&](int32_t x, int32_t y) { return tc[(size_t)(b + x)] < tc[(size_t)(b + y)]; });
-      for (size_t k = 0; k < ord.size();) {
-        const int32_t c = tc[(size_t)(b + ord[k])];
-        double sum = 0;
-        size_t j = k;
-        while (j < ord.size() && tc[(size_t)(b + ord[j])] == c) sum += tv[(size_t)(b + ord[j++])];
-        Ai.push_back(c); Ax.push_back(sum); Ap[r + 1]++;
-        k = j;
+    const int nth = host_threads(n, 8192);
+    std::vector<std::vector<int32_t>> tAi(nth);
+    std::vector<std::vector<double>> tAx(nth);
+    parallel_ranges(n, nth, [&](int64_t rb, int64_t re, int k) {
+      std::vector<int32_t> cols;
+      std::vector<double> val;
+      std::vector<char> kept;
+      std::vector<int32_t>& oi = tAi[k];
+      std::vector<double>& ox = tAx[k];
+      for (int r = (int)rb; r < (int)re; ++r) {
+        cols.assign(1, r);
+        for (int p = itp[r]; p < itp[r + 1]; ++p)
+          for (int a = 0; a < 3; ++a) cols.push_back((int32_t)in.tri[3 * (size_t)itl[p] + a]);
+        for (int p = ihp[r]; p < ihp[r + 1]; ++p)
+          for (int a = 0; a < 4; ++a) cols.push_back((int32_t)in.hinge[4 * (size_t)ihl[p] + a]);
+        std::sort(cols.begin(), cols.end());
+        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+        val.assign(cols.size(), 0.0);
+        kept.assign(cols.size(), 0);
+        auto add = [&](int32_t c, double v) {
+          const size_t q = (size_t)(std::lower_bound(cols.begin(), cols.end(), c) - cols.begin());
+          val[q] += v;
+          kept[q] = 1;
+        };
+        add(r, mass[r] * inv_dt2);
+        for (int p = itp[r]; p < itp[r + 1]; ++p) {
+          const int e = itl[p];
+          const double* d = &dm[4 * (size_t)e];
+          const double g[3][2] = {{-(d[0] + d[1]), -(d[2] + d[3])}, {d[0], d[2]}, {d[1], d[3]}};
+          for (int a = 0; a < 3; ++a) {
+            if ((int)in.tri[3 * (size_t)e + a] != r) continue;
+            for (int b = 0; b < 3; ++b) {
+              const double v = g[a][0] * g[b][0] + g[a][1] * g[b][1];
+              if (std::fabs(v) > 1e-12) add((int32_t)in.tri[3 * (size_t)e + b], tw[e] * v);
+            }
+          }
+        }
+        for (int p = ihp[r]; p < ihp[r + 1]; ++p) {
+          const int e = ihl[p];
+          const double* st = &hst[4 * (size_t)e];
+          for (int a = 0; a < 4; ++a) {
+            if ((int)in.hinge[4 * (size_t)e + a] != r) continue;
+            for (int b = 0; b < 4; ++b) {
+              const double v = hw[e] * st[a] * st[b];
+              if (std::fabs(v) > 1e-12) add((int32_t)in.hinge[4 * (size_t)e + b], v);
+            }
+          }
+        }
+        int cnt = 0;
+        for (size_t q = 0; q < cols.size(); ++q)
+          if (kept[q]) { oi.push_back(cols[q]); ox.push_back(val[q]); ++cnt; }
+        Ap[r + 1] = cnt;
       }
+    });
+    for (int i = 0; i < n; ++i) Ap[i + 1] += Ap[i];
+    Ai.reserve((size_t)Ap[n]);
+    Ax.reserve((size_t)Ap[n]);
+    for (int k = 0; k < nth; ++k) {  // the ranges are contiguous and in row order
+      Ai.insert(Ai.end(), tAi[k].begin(), tAi[k].end());
+      Ax.insert(Ax.end(), tAx[k].begin(), tAx[k].end());
     }
   }
-  for (int i = 0; i < n; ++i) Ap[i + 1] += Ap[i];
-  tc.clear(); tc.shrink_to_fit();
-  tv.clear(); tv.shrink_to_fit();
 
   // Nested dissection, one tree per connected component (a forest for
   // several garments; their blocks never couple, so each tree has its own
   // root at depth 0 and the levels of all trees run in the same launches).
-  NdBuilder nb{Ap, Ai, X.data(), Y.data(), Z.data(), std::max(1, in.leaf_size), std::vector<int>(n, 0)};
+  // The components' trees are built on several threads (largest first, each
+  // thread taking the next), then concatenated in component order, as one
+  // builder would have appended them.
+  std::vector<int> nd_mark(n, 0);
+  NdBuilder nb{Ap, Ai, X.data(), Y.data(), Z.data(), std::max(1, in.leaf_size), nd_mark.data()};
   std::vector<int> roots;
   {
     std::vector<int> comp(n, -1), queue;
@@ -307,9 +375,29 @@ bool build_setup(const SetupInput& in, Setup& S) {
           if (comp[Ai[p]] < 0) { comp[Ai[p]] = id; queue.push_back(Ai[p]); }
       }
     }
-    for (auto& m : members) {
-      std::sort(m.begin(), m.end());
-      roots.push_back(nb.build(m));
+    const int nc = (int)members.size();
+    std::vector<int> by_size(nc);
+    std::iota(by_size.begin(), by_size.end(), 0);
+    std::stable_sort(by_size.begin(), by_size.end(),
+                     [&](int x, int y) { return members[x].size() > members[y].size(); });
+    std::vector<NdBuilder> sub(nc, NdBuilder{Ap, Ai, X.data(), Y.data(), Z.data(), nb.leaf, nd_mark.data()});
+    std::vector<int> sub_root(nc, -1);
+    std::atomic<int> next{0};
+    const int nth = host_threads(nc, 1);
+    parallel_ranges(nth, nth, [&](int64_t, int64_t, int) {
+      for (int q = next++; q < nc; q = next++) {
+        const int c = by_size[q];
+        std::sort(members[c].begin(), members[c].end());
+        sub_root[c] = sub[c].build(members[c]);
+      }
+    });
+    for (int c = 0; c < nc; ++c) {
+      const int off = (int)nb.nodes.size();
+      for (NdNode& nd : sub[c].nodes) {
+        if (nd.left >= 0) { nd.left += off; nd.right += off; }
+        nb.nodes.push_back(std::move(nd));
+      }
+      roots.push_back(off + sub_root[c]);
     }
   }
   // Fine dissection tree: parents, depths, sizes, and each subtree's
@@ -438,38 +526,47 @@ bool build_setup(const SetupInput& in, Setup& S) {
   const int ns = (int)S.sn.size();
   S.max_depth = (int)bands.size() - 1;
 
-  // Permuted matrix.
+  // Permuted matrix (rows on several threads).
   S.Ap.assign(n + 1, 0);
   S.Ai.resize(Ai.size()); S.Ax.resize(Ax.size());
-  {
+  for (int i = 0; i < n; ++i) S.Ap[i + 1] = S.Ap[i] + (Ap[S.perm[i] + 1] - Ap[S.perm[i]]);
+  parallel_ranges(n, host_threads(n, 8192), [&](int64_t rb, int64_t re, int) {
     std::vector<std::pair<int32_t, double>> row;
-    int64_t w = 0;
-    for (int i = 0; i < n; ++i) {
+    for (int i = (int)rb; i < (int)re; ++i) {
       const int o = S.perm[i];
       row.clear();
       for (int p = Ap[o]; p < Ap[o + 1]; ++p) row.push_back({S.iperm[Ai[p]], Ax[p]});
       std::sort(row.begin(), row.end());
+      int64_t w = S.Ap[i];
       for (auto& e : row) { S.Ai[w] = e.first; S.Ax[w] = e.second; ++w; }
-      S.Ap[i + 1] = (int32_t)w;
     }
-  }
+  });
 
-  // Row structure I_P = vertices outside P's subtree adjacent to it.
-  std::vector<int> mark(n, -1);
+  // Row structure I_P = vertices outside P's subtree adjacent to it
+  // (supernodes on several threads, each with its own marks).
   S.rowidx.clear();
-  for (int s = 0; s < ns; ++s) {
-    Supernode& x = S.sn[s];
-    const int lo = sub_lo[s], hi = x.col0 + x.w;
-    std::vector<int> rows;
-    for (int i = lo; i < hi; ++i)
-      for (int p = S.Ap[i]; p < S.Ap[i + 1]; ++p) {
-        const int j = S.Ai[p];
-        if (j >= hi && mark[j] != s) { mark[j] = s; rows.push_back(j); }
+  {
+    std::vector<std::vector<int>> rows_of(ns);
+    const int nth = host_threads(ns, 64);
+    parallel_ranges(ns, nth, [&](int64_t sb, int64_t se, int) {
+      std::vector<int> mark(n, -1);
+      for (int s = (int)sb; s < (int)se; ++s) {
+        const int lo = sub_lo[s], hi = S.sn[s].col0 + S.sn[s].w;
+        std::vector<int>& rows = rows_of[s];
+        for (int i = lo; i < hi; ++i)
+          for (int p = S.Ap[i]; p < S.Ap[i + 1]; ++p) {
+            const int j = S.Ai[p];
+            if (j >= hi && mark[j] != s) { mark[j] = s; rows.push_back(j); }
+          }
+        std::sort(rows.begin(), rows.end());
       }
-    std::sort(rows.begin(), rows.end());
-    x.r = (int)rows.size();
-    x.rows_off = (int)S.rowidx.size();
-    S.rowidx.insert(S.rowidx.end(), rows.begin(), rows.end());
+    });
+    for (int s = 0; s < ns; ++s) {
+      Supernode& x = S.sn[s];
+      x.r = (int)rows_of[s].size();
+      x.rows_off = (int)S.rowidx.size();
+      S.rowidx.insert(S.rowidx.end(), rows_of[s].begin(), rows_of[s].end());
+    }
   }
   // Offsets for M, update vectors and the extend-add maps.
   int64_t moff = 0;
@@ -561,7 +658,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
     auto tmin = [&](int e) {
       return std::min(S.iperm[in.tri[3 * e]], std::min(S.iperm[in.tri[3 * e + 1]], S.iperm[in.tri[3 * e + 2]]));
     };
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return tmin(a) < tmin(b); });
+    stable_order_by_key(ord, n, tmin);
     S.tri_idx.resize(3 * (size_t)t); S.tri_dm.resize(4 * (size_t)t); S.tri_w.resize(t);
     S.tri_orig = ord;
     for (int k = 0; k < t; ++k) {
@@ -577,7 +674,7 @@ bool build_setup(const SetupInput& in, Setup& S) {
       for (int c = 1; c < 4; ++c) m = std::min(m, S.iperm[in.hinge[4 * e + c]]);
       return m;
     };
-    std::stable_sort(hord.begin(), hord.end(), [&](int a, int b) { return hmin(a) < hmin(b); });
+    stable_order_by_key(hord, n, hmin);
     S.hinge_idx.resize(4 * (size_t)h); S.hinge_rest.resize(h); S.hinge_w.resize(h);
     S.hinge_st.assign(ds ? 4 * (size_t)h : 0, 0.0);
     for (int k = 0; k < h; ++k) {
