@@ -128,3 +128,47 @@ def test_sphere_cull_never_drops_a_contact():
 def test_sphere_cull_check_catches_a_loose_bound():
     culled, bad, _ = _sphere_trials(0.5)
     assert bad > 0, culled
+
+
+def _box_gap2(P, Q):
+    """vh_contact.hpp box_gap2: the squared gap between the axis-aligned boxes
+    of two point sets, 0 where they overlap."""
+    g = np.maximum(0.0, np.maximum(P.min(0) - Q.max(0), Q.min(0) - P.max(0)))
+    return float((g * g).sum())
+
+
+def _box_trials(shrink):
+    """Per-test pre-test of self_pair_contacts: for random triangle pairs at
+    several scales, every vertex-triangle and edge-edge test the box gap
+    skips (gap^2 > dh (1 + 1e-6)) must have d2 >= dh by the oracle's own
+    distance functions. shrink < 1 scales the gap up: a planted mistake."""
+    rng = np.random.default_rng(13)
+    skipped = bad = near = 0
+    for scale in (1e-3, 3e-3, 1e-2):
+        for _ in range(400):
+            A = rng.normal(size=(3, 3)) * scale
+            u = rng.normal(size=3)
+            B = A.mean(0) + u / np.linalg.norm(u) * scale * rng.uniform(0.2, 4.0) + rng.normal(size=(3, 3)) * scale
+            dh = (scale * rng.uniform(0.05, 2.0)) ** 2
+            tests = [(A[q:q + 1], B, ref.pt_dist2(A[q], B[0], B[1], B[2])[0]) for q in range(3)]
+            tests += [(B[q:q + 1], A, ref.pt_dist2(B[q], A[0], A[1], A[2])[0]) for q in range(3)]
+            tests += [(A[[i, (i + 1) % 3]], B[[j, (j + 1) % 3]],
+                       ref.ee_dist2(A[i], A[(i + 1) % 3], B[j], B[(j + 1) % 3])[0]) for i in range(3) for j in range(3)]
+            for P, Q, d2 in tests:
+                near += d2 < dh
+                if _box_gap2(P, Q) / shrink ** 2 > dh * (1.0 + 1e-6):
+                    skipped += 1
+                    bad += d2 < dh
+    return skipped, bad, near
+
+
+def test_box_pretest_never_skips_a_contact():
+    skipped, bad, near = _box_trials(1.0)
+    print(f"box pre-test: {skipped} of 18000 tests skipped, {near} tests within reach", flush=True)
+    assert skipped > 2000 and near > 1000
+    assert bad == 0
+
+
+def test_box_pretest_check_catches_a_loose_bound():
+    skipped, bad, _ = _box_trials(0.5)
+    assert bad > 0, skipped

@@ -556,7 +556,12 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
 // Pairs of one workgroup of k_bvh_pairs staged in LDS before a coalesced
 // copy: the group's leaves are consecutive, so their pairs form one
 // contiguous run of the output. Pairs past the staging room go straight out.
-constexpr int kPairStage = 2048;  // 16 KB: 2048 pairs, about 8 per leaf
+// 32 KB: 4096 pairs, 16 per leaf. On the flat 512x512 drape (about 25 pairs
+// per leaf) 16 KB left two thirds of the pairs to scattered 8-byte stores,
+// and the walk's node misses evicted their lines half written (265 MB written
+// for 103 MB of pairs); 32 KB (five workgroups per CU instead of eight) ran
+// the drape 2.4 % faster.
+constexpr int kPairStage = 4096;
 
 template <bool kEmit>
 __device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
@@ -787,6 +792,21 @@ __global__ void __launch_bounds__(256) k_self_cull(const int2* __restrict__ pair
   if (threadIdx.x == 0) scnt[blockIdx.x] = n;
 }
 
+// Squared gap between the axis-aligned boxes of two segments or of a point
+// and a triangle (0 where they overlap): a lower bound of the squared
+// distance between any two of their points.
+__device__ __forceinline__ double box_gap2(D3 p0, D3 p1, D3 p2, D3 q0, D3 q1, D3 q2, D3 q3) {
+  const double lx = fmin(fmin(p0.x, p1.x), p2.x), hx = fmax(fmax(p0.x, p1.x), p2.x);
+  const double ly = fmin(fmin(p0.y, p1.y), p2.y), hy = fmax(fmax(p0.y, p1.y), p2.y);
+  const double lz = fmin(fmin(p0.z, p1.z), p2.z), hz = fmax(fmax(p0.z, p1.z), p2.z);
+  const double mx = fmin(fmin(q0.x, q1.x), fmin(q2.x, q3.x)), Mx = fmax(fmax(q0.x, q1.x), fmax(q2.x, q3.x));
+  const double my = fmin(fmin(q0.y, q1.y), fmin(q2.y, q3.y)), My = fmax(fmax(q0.y, q1.y), fmax(q2.y, q3.y));
+  const double mz = fmin(fmin(q0.z, q1.z), fmin(q2.z, q3.z)), Mz = fmax(fmax(q0.z, q1.z), fmax(q2.z, q3.z));
+  const double gx = fmax(0.0, fmax(lx - Mx, mx - hx)), gy = fmax(0.0, fmax(ly - My, my - hy)),
+               gz = fmax(0.0, fmax(lz - Mz, mz - hz));
+  return gx * gx + gy * gy + gz * gz;
+}
+
 // The tests of one candidate pair that survived the cull, in the reference's
 // order: on_contact(d2) for each active contact; with kWant, also
 // emit(vertices[4], gradients[4], scaled barrier gradient), the
@@ -803,6 +823,10 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
   };
     const int A[3] = {tri[3 * pr.x], tri[3 * pr.x + 1], tri[3 * pr.x + 2]};
     const int B[3] = {tri[3 * pr.y], tri[3 * pr.y + 1], tri[3 * pr.y + 2]};
+    // The six corners, loaded once (with the exclusion lists below) for all
+    // fifteen tests.
+    const D3 pA0 = ld(x, A[0]), pA1 = ld(x, A[1]), pA2 = ld(x, A[2]);
+    const D3 pB0 = ld(x, B[0]), pB1 = ld(x, B[1]), pB2 = ld(x, B[2]);
     // Every exclusion test of the pair is between a vertex of A and one of
     // B, and the relation is symmetric (within `depth` rings): bit 3i + j
     // holds excluded(A[i], B[j]).
@@ -835,6 +859,11 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
     if (ex == 0x1ffu) return;  // every test of the pair is excluded
     // vertex-triangle tests, one at a time (A's vertices against B first)
     auto pick = [](const int* P, int q) { return q == 0 ? P[0] : (q == 1 ? P[1] : P[2]); };
+    // corner q of a triangle, selected component by component (no indexed array)
+    auto sel3 = [](int q, D3 u0, D3 u1, D3 u2) {
+      return d3(q == 0 ? u0.x : (q == 1 ? u1.x : u2.x), q == 0 ? u0.y : (q == 1 ? u1.y : u2.y),
+                q == 0 ? u0.z : (q == 1 ? u1.z : u2.z));
+    };
 #pragma unroll 1
     for (int u = 0; u < 6; ++u) {
       const bool fa = u < 3;
@@ -845,7 +874,11 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
       // excluded(v, T[*]): row q of ex for a vertex of A, column q for one of B
       const unsigned int row = fa ? (ex >> (3 * q)) & 7u : (ex >> q) & 0x49u;
       if (row) continue;
-      const D3 p = ld(x, v), a = ld(x, t0), b = ld(x, t1), c = ld(x, t2);
+      const D3 p = fa ? sel3(q, pA0, pA1, pA2) : sel3(q, pB0, pB1, pB2);
+      const D3 a = fa ? pB0 : pA0, b = fa ? pB1 : pA1, c = fa ? pB2 : pA2;
+      // Exact pre-test: boxes further apart than sqrt(dh) (1e-6 relative to
+      // spare) mean d2 >= dh, no contact; the distance is not evaluated.
+      if (box_gap2(p, p, p, a, b, c, c) > dh * (1.0 + 1e-6)) continue;
       const double d2 = pt_dist2(p, a, b, c);
       if (d2 < dh && d2 > 0.0) {
         if constexpr (kWant) {
@@ -867,7 +900,9 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
       const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
                                 (1u << (3 * i1 + j1));
       if ((ex & need) == need) continue;
-      const D3 pa0 = ld(x, a0), pa1 = ld(x, a1), pb0 = ld(x, b0), pb1 = ld(x, b1);
+      const D3 pa0 = sel3(i, pA0, pA1, pA2), pa1 = sel3(i1, pA0, pA1, pA2), pb0 = sel3(j, pB0, pB1, pB2),
+               pb1 = sel3(j1, pB0, pB1, pB2);
+      if (box_gap2(pa0, pa1, pa1, pb0, pb1, pb1, pb1) > dh * (1.0 + 1e-6)) continue;
       const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
       if (d2 < dh && d2 > 0.0) {
         if constexpr (kWant) {
@@ -894,8 +929,8 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
 // Entries past the buffer (ecap contacts) are dropped and flag an overflow,
 // as a pair overflow does; the host grows the buffer and repeats.
 //
-// Four waves per SIMD (128 VGPRs) measured 14 % faster than three for the
-// tests with gradients (k_self_emit).
+// The tests load the pair's six corners once (k_self_count: 126 VGPRs, four
+// waves per SIMD); k_self_emit runs them only for survivors with contacts.
 __global__ void __launch_bounds__(256) k_self_count(const unsigned int* npairs_p, unsigned int cap,
                                                     const int2* __restrict__ surv, unsigned int reg,
                                                     const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
@@ -987,8 +1022,7 @@ __global__ void __launch_bounds__(256) k_self_base(int nb, const unsigned int* _
   }
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
-k_self_emit(const int2* __restrict__ surv, unsigned int reg, const unsigned int* __restrict__ scnt,
+__global__ void __launch_bounds__(256) k_self_emit(const int2* __restrict__ surv, unsigned int reg, const unsigned int* __restrict__ scnt,
             const unsigned int* __restrict__ cof, const unsigned int* __restrict__ base, const int* __restrict__ tri,
             const double* __restrict__ x, const int* __restrict__ xptr, const int* __restrict__ xidx, Scene sc,
             unsigned int ecap, int* __restrict__ ev, double* __restrict__ eg, const int* skip) {
