@@ -857,28 +857,57 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
       }
     }
     if (ex == 0x1ffu) return;  // every test of the pair is excluded
-    // vertex-triangle tests, one at a time (A's vertices against B first)
     auto pick = [](const int* P, int q) { return q == 0 ? P[0] : (q == 1 ? P[1] : P[2]); };
     // corner q of a triangle, selected component by component (no indexed array)
     auto sel3 = [](int q, D3 u0, D3 u1, D3 u2) {
       return d3(q == 0 ? u0.x : (q == 1 ? u1.x : u2.x), q == 0 ? u0.y : (q == 1 ? u1.y : u2.y),
                 q == 0 ? u0.z : (q == 1 ? u1.z : u2.z));
     };
-#pragma unroll 1
+    const D3 PA[3] = {pA0, pA1, pA2}, PB[3] = {pB0, pB1, pB2};  // indexed with constants only
+    // Which of the fifteen tests run: bit u < 6 for vertex-triangle test u (A's
+    // vertices against B, then B's against A), bit 6 + e for edge-edge test e.
+    // A test is dropped when it shares a vertex, when the exclusion rule
+    // removes it, or by the exact pre-test: boxes further apart than sqrt(dh)
+    // (1e-6 relative to spare) mean d2 >= dh, no contact. The distances then
+    // run only for the tests left, each lane over its own, so a wave pays for
+    // its busiest lane rather than for all fifteen.
+    const double reach = dh * (1.0 + 1e-6);
+    unsigned int todo = 0;
+#pragma unroll
     for (int u = 0; u < 6; ++u) {
+      const bool fa = u < 3;
+      const int q = fa ? u : u - 3;
+      const int v = fa ? A[q] : B[q];
+      const int* T = fa ? B : A;
+      if (v == T[0] || v == T[1] || v == T[2]) continue;
+      // excluded(v, T[*]): row q of ex for a vertex of A, column q for one of B
+      const unsigned int row = fa ? (ex >> (3 * q)) & 7u : (ex >> q) & 0x49u;
+      if (row) continue;
+      const D3 p = fa ? PA[q] : PB[q];
+      const D3* TP = fa ? PB : PA;
+      if (box_gap2(p, p, p, TP[0], TP[1], TP[2], TP[2]) > reach) continue;
+      todo |= 1u << u;
+    }
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      const int i = e / 3, j = e - 3 * i;
+      const int i1 = i == 2 ? 0 : i + 1, j1 = j == 2 ? 0 : j + 1;
+      if (A[i] == B[j] || A[i] == B[j1] || A[i1] == B[j] || A[i1] == B[j1]) continue;
+      const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
+                                (1u << (3 * i1 + j1));
+      if ((ex & need) == need) continue;
+      if (box_gap2(PA[i], PA[i1], PA[i1], PB[j], PB[j1], PB[j1], PB[j1]) > reach) continue;
+      todo |= 1u << (6 + e);
+    }
+    // vertex-triangle tests in order (A's vertices against B first)
+    for (unsigned int vt = todo & 0x3fu; vt; vt &= vt - 1) {
+      const int u = __builtin_ctz(vt);
       const bool fa = u < 3;
       const int q = fa ? u : u - 3;
       const int v = fa ? pick(A, q) : pick(B, q);
       const int t0 = fa ? B[0] : A[0], t1 = fa ? B[1] : A[1], t2 = fa ? B[2] : A[2];
-      if (v == t0 || v == t1 || v == t2) continue;
-      // excluded(v, T[*]): row q of ex for a vertex of A, column q for one of B
-      const unsigned int row = fa ? (ex >> (3 * q)) & 7u : (ex >> q) & 0x49u;
-      if (row) continue;
       const D3 p = fa ? sel3(q, pA0, pA1, pA2) : sel3(q, pB0, pB1, pB2);
       const D3 a = fa ? pB0 : pA0, b = fa ? pB1 : pA1, c = fa ? pB2 : pA2;
-      // Exact pre-test: boxes further apart than sqrt(dh) (1e-6 relative to
-      // spare) mean d2 >= dh, no contact; the distance is not evaluated.
-      if (box_gap2(p, p, p, a, b, c, c) > dh * (1.0 + 1e-6)) continue;
       const double d2 = pt_dist2(p, a, b, c);
       if (d2 < dh && d2 > 0.0) {
         if constexpr (kWant) {
@@ -890,19 +919,14 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
         on_contact(d2);
       }
     }
-    // edge-edge tests: edge i of A (A[i], A[i+1]) against edge j of B
-#pragma unroll 1
-    for (int e = 0; e < 9; ++e) {
+    // edge-edge tests in order: edge i of A (A[i], A[i+1]) against edge j of B
+    for (unsigned int ee = todo >> 6; ee; ee &= ee - 1) {
+      const int e = __builtin_ctz(ee);
       const int i = e / 3, j = e - 3 * i;
       const int i1 = i == 2 ? 0 : i + 1, j1 = j == 2 ? 0 : j + 1;
       const int a0 = pick(A, i), a1 = pick(A, i1), b0 = pick(B, j), b1 = pick(B, j1);
-      if (a0 == b0 || a0 == b1 || a1 == b0 || a1 == b1) continue;
-      const unsigned int need = (1u << (3 * i + j)) | (1u << (3 * i + j1)) | (1u << (3 * i1 + j)) |
-                                (1u << (3 * i1 + j1));
-      if ((ex & need) == need) continue;
       const D3 pa0 = sel3(i, pA0, pA1, pA2), pa1 = sel3(i1, pA0, pA1, pA2), pb0 = sel3(j, pB0, pB1, pB2),
                pb1 = sel3(j1, pB0, pB1, pB2);
-      if (box_gap2(pa0, pa1, pa1, pb0, pb1, pb1, pb1) > dh * (1.0 + 1e-6)) continue;
       const double d2 = ee_dist2(pa0, pa1, pb0, pb1);
       if (d2 < dh && d2 > 0.0) {
         if constexpr (kWant) {
@@ -929,9 +953,12 @@ __device__ __forceinline__ void self_pair_contacts(const int2 pr, const int* __r
 // Entries past the buffer (ecap contacts) are dropped and flag an overflow,
 // as a pair overflow does; the host grows the buffer and repeats.
 //
-// The tests load the pair's six corners once (k_self_count: 126 VGPRs, four
-// waves per SIMD); k_self_emit runs them only for survivors with contacts.
-__global__ void __launch_bounds__(256) k_self_count(const unsigned int* npairs_p, unsigned int cap,
+// The tests load the pair's six corners once and evaluate only the
+// distances their exact pre-tests leave (self_pair_contacts); at the
+// 512 x 512 drape that took k_self_count from 426 to 297 us, and four waves
+// per SIMD (128 VGPRs, 12 bytes of spill) instead of the compiler's three to
+// 262 us. k_self_emit runs the tests only for survivors with contacts.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_self_count(const unsigned int* npairs_p, unsigned int cap,
                                                     const int2* __restrict__ surv, unsigned int reg,
                                                     const unsigned int* __restrict__ scnt, const int* __restrict__ tri,
                                                     const double* __restrict__ x, const int* __restrict__ xptr,
