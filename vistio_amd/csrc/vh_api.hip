@@ -1076,22 +1076,26 @@ vh_status device_factor(vh_ctx* ctx) {
                            (double*)Fd, (double*)Ed, (const double*)Dd);
     }
     if (nl > 0) {
+      // at least ~2048 workgroups per level: a level of few large
+      // supernodes splits each one's rows over several
+      const dim3 grid(nl, std::max(1, std::min(256, 2048 / nl)));
       if (ctx->prec == VH_PREC_F64)
-        hipLaunchKernelGGL(vhf::k_fac_blocks<double>, dim3(nl), dim3(256), 0, st, frp, ip + lev_off[d],
+        hipLaunchKernelGGL(vhf::k_fac_blocks<double>, grid, dim3(256), 0, st, frp, ip + lev_off[d],
                            (const double*)Ed, (double*)ctx->M, (double*)ctx->Mt);
       else
-        hipLaunchKernelGGL(vhf::k_fac_blocks<float>, dim3(nl), dim3(256), 0, st, frp, ip + lev_off[d],
+        hipLaunchKernelGGL(vhf::k_fac_blocks<float>, grid, dim3(256), 0, st, frp, ip + lev_off[d],
                            (const double*)Ed, (float*)ctx->M, (float*)ctx->Mt);
     }
     if (d == 0 && nroots > 0) {
       if (nroot_tasks > 0)
         hipLaunchKernelGGL(vhf::k_fac_tiles, dim3(nroot_tasks), dim3(64), 0, st, frp, tp + root_tasks_off, 0,
                            (double*)Fd, (double*)Ed, (const double*)Dd);
+      const dim3 rgrid(nroots, std::max(1, std::min(256, 2048 / nroots)));
       if (ctx->prec == VH_PREC_F64)
-        hipLaunchKernelGGL(vhf::k_fac_root<double>, dim3(nroots), dim3(256), 0, st, frp, ip + root_off,
+        hipLaunchKernelGGL(vhf::k_fac_root<double>, rgrid, dim3(256), 0, st, frp, ip + root_off,
                            (const double*)Fd, (double*)ctx->Sroot);
       else
-        hipLaunchKernelGGL(vhf::k_fac_root<float>, dim3(nroots), dim3(256), 0, st, frp, ip + root_off,
+        hipLaunchKernelGGL(vhf::k_fac_root<float>, rgrid, dim3(256), 0, st, frp, ip + root_off,
                            (const double*)Fd, (float*)ctx->Sroot);
     }
     VH_HIP(hipGetLastError());

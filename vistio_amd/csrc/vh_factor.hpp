@@ -262,7 +262,9 @@ __global__ void __launch_bounds__(64) k_fac_tiles(const FrontDev* __restrict__ f
 
 // The solve blocks of each supernode of the level, in the compute type T:
 // M_P row-major (rows 0 .. w-1 of L_PP^{-1}, lower; then the r rows of
-// L_IP L_PP^{-1} = -E_I) and M_P^T (column j of M_P contiguous).
+// L_IP L_PP^{-1} = -E_I) and M_P^T (column j of M_P contiguous). Block
+// (x, y) writes supernode lev[x]'s rows and columns y, y + gridDim.y, ...,
+// so a level of few large supernodes still spreads over the chip.
 template <class T>
 __global__ void __launch_bounds__(256) k_fac_blocks(const FrontDev* __restrict__ fr, const int* __restrict__ lev,
                                                     const double* __restrict__ Ed, T* __restrict__ M,
@@ -271,7 +273,7 @@ __global__ void __launch_bounds__(256) k_fac_blocks(const FrontDev* __restrict__
   const double* E = Ed + P.e;
   const int w = P.w, r = P.r, m = w + r;
   T* Mp = M + P.m_off;
-  for (int row = 0; row < m; ++row) {
+  for (int row = blockIdx.y; row < m; row += gridDim.y) {
     if (row < w) {
       T* dst = Mp + (size_t)row * (row + 1) / 2;
       for (int j = threadIdx.x; j <= row; j += blockDim.x) dst[j] = (T)E[(size_t)row * P.wp + j];
@@ -281,7 +283,7 @@ __global__ void __launch_bounds__(256) k_fac_blocks(const FrontDev* __restrict__
       for (int j = threadIdx.x; j < w; j += blockDim.x) dst[j] = (T)(-E[(size_t)(P.wp + q) * P.wp + j]);
     }
   }
-  for (int j = 0; j < w; ++j) {
+  for (int j = blockIdx.y; j < w; j += gridDim.y) {
     T* dst = Mt + P.m_off + (size_t)j * m - (size_t)j * (j - 1) / 2;
     for (int kk = j + threadIdx.x; kk < m; kk += blockDim.x) {
       const double v = kk < w ? E[(size_t)kk * P.wp + j] : -E[(size_t)(P.wp + kk - w) * P.wp + j];
@@ -290,7 +292,8 @@ __global__ void __launch_bounds__(256) k_fac_blocks(const FrontDev* __restrict__
   }
 }
 
-// A root's dense inverse S (left in its F by the kSRoot tasks) into Sroot.
+// A root's dense inverse S (left in its F by the kSRoot tasks) into Sroot;
+// block (x, y) copies root roots[x]'s rows y, y + gridDim.y, ...
 template <class T>
 __global__ void __launch_bounds__(256) k_fac_root(const FrontDev* __restrict__ fr, const int* __restrict__ roots,
                                                   const double* __restrict__ Fd, T* __restrict__ Sroot) {
@@ -298,10 +301,8 @@ __global__ void __launch_bounds__(256) k_fac_root(const FrontDev* __restrict__ f
   const double* S = Fd + P.f;
   const int w = P.w;
   T* dst = Sroot + P.s_off;
-  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) {
-    const int i = idx / w, j = idx - i * w;
-    dst[idx] = (T)S[(size_t)i * P.mp + j];
-  }
+  for (int i = blockIdx.y; i < w; i += gridDim.y)
+    for (int j = threadIdx.x; j < w; j += blockDim.x) dst[(size_t)i * w + j] = (T)S[(size_t)i * P.mp + j];
 }
 
 }  // namespace vhf
