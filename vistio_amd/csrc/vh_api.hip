@@ -114,6 +114,11 @@ struct vh_ctx {
   int* inc_ptr = nullptr;
   // solve
   vhk::TaskDev* tasks = nullptr;  // forward tasks then backward tasks
+  // The root band below other bands (k_root_dense): root_rows rows of one
+  // root per task, root_nl entries per lane loaded in the first round.
+  vhk::TaskDev* root_tasks = nullptr;
+  int nroot_tasks = 0, root_rows = 0, root_nl = 0;
+  size_t root_lds = 0;
   int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
   void *M = nullptr, *Mt = nullptr, *Sroot = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
@@ -256,6 +261,49 @@ int band_task_blocks(int ntasks, bool first) {
 // kernel everywhere cost 3.6 % of the step.
 constexpr int kWideBandTasks = 4096;
 
+// The dense root kernel's shape for a widest root of w columns: rows per
+// workgroup (one per wave; each workgroup gathers the root's front once)
+// and entries per lane loaded in the first round (a multiple of 8; longer
+// rows continue 8 per lane at a time). Measured at 256x256 (FP32): a
+// 1535-wide root in 512-thread workgroups 13.2 -> 11.7 us per launch (the
+// 256-thread band kernel before it), a 2295-wide one in 1024-thread
+// workgroups 18.3 -> 16.8 us. FP64 keeps k_root_band: its 24 entries per
+// lane spill in 1024-thread workgroups (15.0 -> 19.0 us at 1535 wide).
+int root_threads(int w, bool f64) {
+  if (f64) return 0;
+  return w > 2048 ? 1024 : 512;
+}
+int root_loads(int w, bool f64) {
+  const int cap = f64 ? 24 : 40;
+  return std::min(cap, std::max(8, ((w + 63) / 64 + 7) / 8 * 8));
+}
+
+#define VH_ROOT_NL(X) X(8) X(16) X(24) X(32) X(40)
+
+const void* root_dense_kernel(int nl) {
+  switch (nl) {
+#define VH_CASE(N) \
+  case N: return (const void*)vhk::k_root_dense<N, float>;
+    VH_ROOT_NL(VH_CASE)
+#undef VH_CASE
+    default: return nullptr;
+  }
+}
+
+void launch_root_dense(vh_ctx* c, const vhk::BandArgs<float>& a, const vhk::StepDev* st) {
+  const dim3 grid(c->nroot_tasks), block(64 * c->root_rows);
+  switch (c->root_nl) {
+#define VH_CASE(N)                                                                                         \
+  case N:                                                                                                  \
+    hipLaunchKernelGGL((vhk::k_root_dense<N, float>), grid, block, c->root_lds, c->stream, c->root_tasks, a, \
+                       (const float*)c->Sroot, (const float*)c->F, (float*)c->V, (float*)c->d, st);        \
+    break;
+    VH_ROOT_NL(VH_CASE)
+#undef VH_CASE
+    default: break;
+  }
+}
+
 // Records an event after each launch when profiling (ev == nullptr: off).
 struct Marks {
   hipEvent_t* ev = nullptr;
@@ -319,7 +367,12 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
     const size_t lds = 3 * (size_t)S.band_front[0] * c->tsz;
     const T* Sr = (const T*)c->Sroot;
     const int rg = band_task_blocks(e - b, false);
-    if (e > b) {
+    bool dense = false;
+    if constexpr (std::is_same<T, float>::value) {
+      dense = !only && c->nroot_tasks > 0;
+      if (dense) launch_root_dense(c, a, st);
+    }
+    if (!dense && e > b) {
       if (slots && cm)
         hipLaunchKernelGGL((vhk::k_root_band<T, true, true>), dim3(rg), dim3(256), lds, c->stream, a, cc, Sr,
                            (T*)c->F, (T*)c->V, (T*)c->d, st);
@@ -1259,6 +1312,31 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
     td[k].q2 = make_int4(q.fm_off, q.fslot_k, q.parent < 0 ? (int)q.s_off : 0, 0);
   }
   UP(upload(ctx, &ctx->tasks, td));
+  // Dense root tasks (k_root_dense) when the root band is not the only one.
+  ctx->root_tasks = nullptr;
+  ctx->nroot_tasks = 0; ctx->root_rows = 0; ctx->root_nl = 0; ctx->root_lds = 0;
+  if (S.max_depth > 0) {
+    int wmax = 0;
+    for (const vh::Supernode& q : S.sn)
+      if (q.parent < 0) wmax = std::max(wmax, q.w);
+    ctx->root_rows = root_threads(wmax, ctx->prec == VH_PREC_F64) / 64;
+    if (ctx->root_rows == 0) wmax = 0;
+    ctx->root_nl = root_loads(wmax, ctx->prec == VH_PREC_F64);
+    ctx->root_lds = 3 * (size_t)wmax * ctx->tsz;
+    std::vector<vhk::TaskDev> rt;
+    for (const vh::Supernode& q : S.sn) {
+      if (q.parent >= 0 || ctx->root_rows == 0) continue;
+      for (int b = 0; b < q.w; b += ctx->root_rows) {
+        vhk::TaskDev x{};
+        x.q0 = make_int4(q.col0, q.w, 0, 0);
+        x.q1 = make_int4(b, std::min(q.w, b + ctx->root_rows), 64, 0);
+        x.q2 = make_int4(q.fm_off, q.fslot_k, (int)q.s_off, 0);
+        rt.push_back(x);
+      }
+    }
+    ctx->nroot_tasks = (int)rt.size();
+    if (!rt.empty()) UP(upload(ctx, &ctx->root_tasks, rt));
+  }
   UP(upload(ctx, &ctx->uslot, S.uslot));
   UP(upload(ctx, &ctx->dpush_ptr, S.dpush_ptr));
   UP(upload(ctx, &ctx->dpush_idx, S.dpush_idx));
@@ -1309,6 +1387,10 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
         f64 ? (const void*)vhk::k_fwd_band_d5 : (const void*)vhk::k_fwd_band_w8,
         f64 ? (const void*)vhk::k_bwd_band_d5 : (const void*)vhk::k_bwd_band<float, false>};
     for (const void* k : ks) VH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  if (ctx->root_lds > 64 * 1024) {
+    VH_HIP(hipFuncSetAttribute(root_dense_kernel(ctx->root_nl), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)ctx->root_lds));
   }
   // Initial state (state.rs:119-139): pos = prev = mesh positions, vel = 0.
   std::vector<double> h3(n3);

@@ -733,6 +733,73 @@ __device__ __forceinline__ void front_sum(const T* __restrict__ f, int K, T& ax,
   for (int k = 3; k < K; ++k) { ax += f[3 * k]; ay += f[3 * k + 1]; az += f[3 * k + 2]; }
 }
 
+// Front assembly with every load issued before any is waited on. Loads in
+// flight complete in issue order as far as the wait counter goes, so a
+// front gathered after the block rows would wait for the rows at its first
+// sum, and a slot-by-slot sum would wait once per slot. Here each thread
+// loads, for its positions j = j0 + tid + p blockDim.x (p < P), up to four
+// child slots (clamped addresses, masked sums) and, with R, r; the caller
+// then issues its row loads, and front_finish sums in the order of
+// front_sum: r - ((0 + s0) + s1 + ...), or the slot sum alone without R.
+template <int P, class T>
+struct FrontLoads {
+  T f[P][4][3];
+  double rb[P][3];
+};
+
+template <int P, bool R, class T>
+__device__ __forceinline__ void front_issue(FrontLoads<P, T>& L, const double* __restrict__ r, int col0, int j0,
+                                            int n, const T* __restrict__ Fp, int K) {
+  if (n <= j0) return;  // no positions (uniform): nothing to load
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int j = j0 + (int)threadIdx.x + p * (int)blockDim.x;
+    const int jj = j < n ? j : j0;
+    if (K > 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const T* sl = Fp + 3 * ((size_t)jj * K + (q < K ? q : 0));
+        L.f[p][q][0] = sl[0]; L.f[p][q][1] = sl[1]; L.f[p][q][2] = sl[2];
+      }
+    }
+    if (R) {
+      const double* rb = r + 3 * (size_t)(col0 + jj);
+      L.rb[p][0] = rb[0]; L.rb[p][1] = rb[1]; L.rb[p][2] = rb[2];
+    }
+  }
+}
+
+template <int P, bool R, class T>
+__device__ __forceinline__ void front_finish(const FrontLoads<P, T>& L, int j0, int n, const T* __restrict__ Fp,
+                                             int K, T* front) {
+#pragma unroll
+  for (int p = 0; p < P; ++p) {
+    const int j = j0 + (int)threadIdx.x + p * (int)blockDim.x;
+    if (j < n) {
+      T ax = 0, ay = 0, az = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (q < K) { ax += L.f[p][q][0]; ay += L.f[p][q][1]; az += L.f[p][q][2]; }
+      for (int q = 4; q < K; ++q) {
+        const T* sl = Fp + 3 * ((size_t)j * K + q);
+        ax += sl[0]; ay += sl[1]; az += sl[2];
+      }
+      if (R) {
+        front[3 * j + 0] = (T)L.rb[p][0] - ax;
+        front[3 * j + 1] = (T)L.rb[p][1] - ay;
+        front[3 * j + 2] = (T)L.rb[p][2] - az;
+      } else {
+        front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
+      }
+    }
+  }
+}
+
+// Positions gathered up front per thread: two in FP32; one in FP64, whose
+// loads take twice the registers.
+template <class T>
+constexpr int front_pos() { return sizeof(T) == 4 ? 2 : 1; }
+
 // Front entries [0, n) of a task: r (or, in the first launch of a sweep
 // that starts from the RHS slots, the slot sums, also stored to r when
 // store_r) minus the K child slots of each position.
@@ -786,8 +853,15 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
   const int lane = tid & (G - 1), grp = tid / G, ngrp = blockDim.x / G;
   const T* Mp = M + moff;
   const long long tri = (long long)w * (w + 1) / 2;
-  // Block rows and output slots first: their addresses depend only on the
-  // task, so these loads are in flight while the front is assembled.
+  // The front's loads first (front_issue), then the block rows and output
+  // slots: their addresses depend only on the task, so all of them are in
+  // flight together, and the front's sums wait only for the front.
+  constexpr int kFP = front_pos<T>();
+  const int pt0 = b > w ? b : w;  // pass-through positions [pt0, e)
+  FrontLoads<kFP, T> fl;
+  FrontLoads<1, T> pl;
+  if constexpr (!S) front_issue<kFP, true>(fl, a.r, col0, 0, tlen, Fp, K);
+  front_issue<1, false>(pl, a.r, col0, pt0, e, Fp, K);
   T mv[kPrefRows][kPrefLoads];
   int us[kPrefRows];
 #pragma unroll
@@ -797,9 +871,21 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
     load_row<G, kPrefLoads>(mv[p], Mp + fwd_row_off(row < e ? row : b, w, tri), len, lane);
     us[p] = row < e && row >= w ? a.uslot[uoff + row - w] : 0;
   }
-  assemble_front<S>(a, col0, tlen, Fp, K, e >= w, front);
-#pragma unroll 4
-  for (int j = (b > w ? b : w) + tid; j < e; j += blockDim.x) {
+  if constexpr (S) {
+    assemble_front<S>(a, col0, tlen, Fp, K, e >= w, front);
+  } else {
+    front_finish<kFP, true>(fl, 0, tlen, Fp, K, front);
+    for (int j = kFP * (int)blockDim.x + tid; j < tlen; j += blockDim.x) {
+      T ax, ay, az;
+      front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+      const double* rb = a.r + 3 * (size_t)(col0 + j);
+      front[3 * j + 0] = (T)rb[0] - ax;
+      front[3 * j + 1] = (T)rb[1] - ay;
+      front[3 * j + 2] = (T)rb[2] - az;
+    }
+  }
+  front_finish<1, false>(pl, pt0, e, Fp, K, front);
+  for (int j = pt0 + (int)blockDim.x + tid; j < e; j += blockDim.x) {
     T ax, ay, az;
     front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
     front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
@@ -1111,6 +1197,71 @@ __global__ void __launch_bounds__(256) k_root_band(BandArgs<T> a, CommitArgs c, 
     __syncthreads();
     block_partials(df, nm, c.partials);
   }
+}
+
+// The root band below at least one other band (no RHS slots, no commit),
+// laid out for bandwidth: one row of the dense inverse per wave, its first
+// 64 NL entries loaded in one round before the front is assembled, and
+// blockDim / 64 rows per workgroup (the host gives each task at most that
+// many rows), so each workgroup gathers the front once for that many rows.
+// Task b covers rows [q1.x, q1.y) of the root in its record (q0.x col0,
+// q0.y w, q2.x first front slot, q2.y slots per position, q2.z offset of
+// the inverse in Sroot). The front and the products are taken eight
+// entries at a time (scheduling barriers between the groups), which keeps
+// the LDS reads of a group, not of the whole row, in registers.
+template <int NL, class T>
+__global__ void __launch_bounds__(1024) k_root_dense(const TaskDev* __restrict__ tasks, BandArgs<T> a,
+                                                     const T* __restrict__ Sroot_all, const T* __restrict__ F,
+                                                     T* __restrict__ V, T* __restrict__ d, const StepDev* st) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* front = reinterpret_cast<T*>(smem_raw);
+  const TaskDev td = tasks[blockIdx.x];
+  if (st && st->done) return;
+  const int col0 = td.q0.x, w = td.q0.y, b = td.q1.x, e = td.q1.y, K = td.q2.y;
+  const T* __restrict__ S = Sroot_all + (size_t)td.q2.z;
+  const T* __restrict__ Fp = F + 3 * (size_t)td.q2.x;
+  const int tid = threadIdx.x, lane = tid & 63, row = b + (tid >> 6);
+  const CommitArgs nc{};
+  const bool live = row < e;
+  const T* __restrict__ Srow = S + (size_t)(live ? row : b) * w;
+  // front[j] = r[col0 + j] - (the K child slots of position j): its loads
+  // first, then the row's (see front_issue)
+  FrontLoads<2, T> fl;
+  front_issue<2, true>(fl, a.r, col0, 0, w, Fp, K);
+  T mv[NL];
+  load_row<64, NL>(mv, Srow, live ? w : 0, lane);
+  ColPre cp;
+  col_pre<false>(a, nc, col0 + (live ? row : b), lane, cp);
+  front_finish<2, true>(fl, 0, w, Fp, K, front);
+  for (int j = 2 * (int)blockDim.x + tid; j < w; j += blockDim.x) {
+    T ax, ay, az;
+    front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+    const double* rb = a.r + 3 * (size_t)(col0 + j);
+    front[3 * j + 0] = (T)rb[0] - ax;
+    front[3 * j + 1] = (T)rb[1] - ay;
+    front[3 * j + 2] = (T)rb[2] - az;
+  }
+  __syncthreads();
+  if (!live) return;
+  T ax = 0, ay = 0, az = 0;
+#pragma unroll
+  for (int c = 0; c < NL; c += 8) {
+#pragma unroll
+    for (int i = c; i < c + 8; ++i) {
+      const int k = lane + i * 64;
+      const T* v = front + 3 * (k < w ? k : w - 1);
+      ax += mv[i] * v[0]; ay += mv[i] * v[1]; az += mv[i] * v[2];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  for (int base = NL * 64; base < w; base += 8 * 64) {
+    T m2[8];
+    load_row<64, 8>(m2, Srow + base, w - base, lane);
+    fma_row<64, 8>(m2, front + 3 * base, w - base, lane, ax, ay, az);
+  }
+  sum3<64>(ax, ay, az);
+  double df = 0, nm = 0;
+  col_store<false>(a, nc, col0 + row, ax, ay, az, cp, lane, 64, V, d, df, nm);
 }
 
 // Iterative refinement between FP32 sweeps (FP64 residual): acc += d and,
