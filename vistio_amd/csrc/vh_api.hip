@@ -119,6 +119,13 @@ struct vh_ctx {
   vhk::TaskDev* root_tasks = nullptr;
   int nroot_tasks = 0, root_rows = 0, root_nl = 0;
   size_t root_lds = 0;
+  // Forward bands below the first and above the root with their block rows
+  // staged in LDS (k_fwd_band_st): tasks of up to 256 st_E entries, those
+  // of forward level l at [st_off[l], st_off[l + 1]) (empty: that band runs
+  // k_fwd_band).
+  vhk::TaskDev* st_tasks = nullptr;
+  std::vector<int> st_off;
+  int st_E = 0;
   int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
   void *M = nullptr, *Mt = nullptr, *Sroot = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
@@ -280,6 +287,48 @@ int root_loads(int w, bool f64) {
 
 #define VH_ROOT_NL(X) X(8) X(16) X(24) X(32) X(40)
 
+// Staged forward bands (k_fwd_band_st): 8 entries per thread (tasks of up
+// to 2048 entries), for bands whose supernodes are at most 128 columns
+// wide. Measured at 256x256 (5 bands): the 87-wide band 16.9 -> 12.8 us
+// (FP32) and 21.8 -> 15.9 us (FP64), the 95-wide one unchanged; the
+// 475-wide band slower staged (12.7 -> 17.6 us, the rows' LDS dot products
+// run in sequence), 16 or 32 entries per thread slower everywhere (fewer
+// workgroups). Staged backward bands were measured too and gained nothing.
+constexpr int kStagedLoads = 8;
+constexpr int kStagedMaxW = 128;
+
+#define VH_ST_E(X) X(8)
+
+template <class T>
+const void* fwd_st_kernel(int E) {
+  switch (E) {
+#define VH_CASE(N) \
+  case N: return (const void*)vhk::k_fwd_band_st<N, T>;
+    VH_ST_E(VH_CASE)
+#undef VH_CASE
+    default: return nullptr;
+  }
+}
+
+template <class T>
+void launch_fwd_st(vh_ctx* c, const vhk::BandArgs<T>& a, int grid, size_t lds, int fmax, const vhk::StepDev* st) {
+  switch (c->st_E) {
+#define VH_CASE(N)                                                                                             \
+  case N:                                                                                                      \
+    hipLaunchKernelGGL((vhk::k_fwd_band_st<N, T>), dim3(grid), dim3(256), lds, c->stream, a, (const T*)c->M, \
+                       (T*)c->y, (T*)c->F, fmax, st);                                                          \
+    break;
+    VH_ST_E(VH_CASE)
+#undef VH_CASE
+    default: break;
+  }
+}
+
+size_t staged_lds(const vh_ctx* c, int fmax) {
+  return (3 * (size_t)fmax + 256 * (size_t)c->st_E) * c->tsz + 512 * sizeof(int);
+}
+
+
 const void* root_dense_kernel(int nl) {
   switch (nl) {
 #define VH_CASE(N) \
@@ -329,6 +378,14 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
   a.uslot = c->uslot; a.dpush_ptr = c->dpush_ptr; a.dpush_idx = c->dpush_idx;
   a.src = rhs_src<T>(c, dt); a.r = c->r; a.upper_cols = c->upper_cols; a.nupper = nupper;
   for (int l = 0; l + 1 < nb; ++l) {
+    if (l > 0 && c->st_E > 0 && c->st_off[l + 1] > c->st_off[l]) {
+      const int b = c->st_off[l], e = c->st_off[l + 1];
+      a.tasks = c->st_tasks + b; a.ntasks = e - b; a.xper = band_xper(e - b, false);
+      const int fmax = S.band_front[S.max_depth - l];
+      launch_fwd_st<T>(c, a, band_task_blocks(e - b, false), staged_lds(c, fmax), fmax, st);
+      mk.mark(c->stream);
+      continue;
+    }
     const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
     a.tasks = c->tasks + b; a.ntasks = e - b; a.xper = band_xper(e - b, l == 0);
@@ -1336,6 +1393,57 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
     }
     ctx->nroot_tasks = (int)rt.size();
     if (!rt.empty()) UP(upload(ctx, &ctx->root_tasks, rt));
+  }
+  // Staged forward tasks: each supernode's rows in runs of at most 256 rows
+  // and 256 st_E entries; a band with a row longer than that keeps
+  // k_fwd_band.
+  ctx->st_tasks = nullptr;
+  ctx->st_E = kStagedLoads;
+  ctx->st_off.assign(S.max_depth + 2, 0);
+  if (ctx->st_E > 0 && S.max_depth > 1) {
+    const int cap = 256 * ctx->st_E;
+    std::vector<vhk::TaskDev> sv;
+    for (int l = 0; l <= S.max_depth; ++l) {
+      ctx->st_off[l] = (int)sv.size();
+      const int depth = S.max_depth - l;
+      if (l == 0 || depth == 0) continue;
+      const size_t lvl0 = sv.size();
+      bool fits = true;
+      const int wlim = std::min(cap, kStagedMaxW);
+      for (const vh::Supernode& q : S.sn) {
+        if (q.depth != depth) continue;
+        if (q.w > wlim) { fits = false; break; }
+        const int m = q.w + q.r;
+        int g = 4;
+        while (g < 64 && g * 8 < q.w) g *= 2;
+        for (int b = 0; b < m;) {
+          int e = b;
+          int64_t ent = 0;
+          while (e < m && e - b < 256) {
+            const int len = e < q.w ? e + 1 : q.w;
+            if (e > b && ent + len > cap) break;
+            ent += len;
+            ++e;
+          }
+          vhk::TaskDev x{};
+          x.q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
+          x.q1 = make_int4(b, e, g, q.u_off);
+          x.q2 = make_int4(q.fm_off, q.fslot_k, 0, 0);
+          sv.push_back(x);
+          b = e;
+        }
+      }
+      if (!fits) sv.resize(lvl0);
+    }
+    ctx->st_off[S.max_depth + 1] = (int)sv.size();
+    if (!sv.empty()) UP(upload(ctx, &ctx->st_tasks, sv));
+    size_t lds = 0;
+    for (int f : S.band_front) lds = std::max(lds, staged_lds(ctx, f));
+    if (lds > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "staged solve task exceeds the 160 KB LDS");
+    if (lds > 64 * 1024) {
+      const void* k = ctx->prec == VH_PREC_F64 ? fwd_st_kernel<double>(ctx->st_E) : fwd_st_kernel<float>(ctx->st_E);
+      if (k) VH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
   }
   UP(upload(ctx, &ctx->uslot, S.uslot));
   UP(upload(ctx, &ctx->dpush_ptr, S.dpush_ptr));

@@ -1078,6 +1078,102 @@ k_fwd_band_d5(BandArgs<double> a, const double* __restrict__ M, double* __restri
   fwd_band<double, false>(a, M, y, F, st);
 }
 
+// Forward band with each task's block rows staged in LDS (k_fwd_band_st).
+// A task's entries are one contiguous run of M_P (rows [b, e) of the
+// row-major trapezoid): every thread loads E of them, flat and coalesced,
+// in the same round as the front (front_issue), writes them to LDS, and
+// after the barrier the lane groups take the rows' dot products from LDS
+// alone, however long and however many the rows. A task holds at most
+// 256 E entries and 256 rows (the host cuts them so); lane l of a group
+// sums entries l, l + G, ... and the group reduces as in fwd_task, so the
+// sums are the same.
+template <int E, class T>
+__device__ void fwd_task_st(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
+                            T* __restrict__ F, T* front, T* mat, int* usl) {
+  const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
+  const int K = td.q2.y;
+  const T* Fp = F + 3 * (size_t)td.q2.x;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tlen = e <= w ? e : w;
+  const long long tri = (long long)w * (w + 1) / 2;
+  const long long o0 = fwd_row_off(b, w, tri);
+  const int ne = (int)(fwd_row_off(e, w, tri) - o0);
+  constexpr int kFP = front_pos<T>();
+  const int pt0 = b > w ? b : w;  // pass-through positions [pt0, e)
+  FrontLoads<kFP, T> fl;
+  FrontLoads<1, T> pl;
+  front_issue<kFP, true>(fl, a.r, col0, 0, tlen, Fp, K);
+  front_issue<1, false>(pl, a.r, col0, pt0, e, Fp, K);
+  T mv[E];
+  const T* __restrict__ src = M + moff + o0;
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      mv[i] = __builtin_nontemporal_load(src + (k < ne ? k : 0));
+    }
+  }
+  const int rt = b + tid;
+  const int u = rt < e && rt >= w ? a.uslot[uoff + rt - w] : 0;
+  front_finish<kFP, true>(fl, 0, tlen, Fp, K, front);
+  for (int j = kFP * nt + tid; j < tlen; j += nt) {
+    T ax, ay, az;
+    front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+    const double* rb = a.r + 3 * (size_t)(col0 + j);
+    front[3 * j + 0] = (T)rb[0] - ax;
+    front[3 * j + 1] = (T)rb[1] - ay;
+    front[3 * j + 2] = (T)rb[2] - az;
+  }
+  front_finish<1, false>(pl, pt0, e, Fp, K, front);
+  for (int j = pt0 + nt + tid; j < e; j += nt) {
+    T ax, ay, az;
+    front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
+    front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
+  }
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      if (k < ne) mat[k] = mv[i];
+    }
+  }
+  if (rt < e) usl[tid] = u;
+  __syncthreads();
+  with_lanes<4>(td.q1.z, [&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    const int lane = tid & (G - 1), ngrp = nt / G;
+    for (int row = b + tid / G; row < e; row += ngrp) {  // uniform within the group
+      const int len = row < w ? row + 1 : w;
+      const T* __restrict__ mr = mat + (fwd_row_off(row, w, tri) - o0);
+      T ax = 0, ay = 0, az = 0;
+#pragma unroll 4
+      for (int k = lane; k < len; k += G) {
+        const T m = mr[k];
+        ax += m * front[3 * k]; ay += m * front[3 * k + 1]; az += m * front[3 * k + 2];
+      }
+      sum3<G>(ax, ay, az);
+      fwd_store(row, w, col0, usl[row - b], ax, ay, az, front, y, F, lane);
+    }
+  });
+}
+
+// Dynamic LDS of k_fwd_band_st: the front (3 fmax values), then the staged
+// entries (256 E), then the task's output slot numbers (256 ints).
+template <int E, class T>
+__global__ void __launch_bounds__(256) k_fwd_band_st(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
+                                                     T* __restrict__ F, int fmax, const StepDev* st) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ti = task_of(a, (int)blockIdx.x);
+  if (ti < 0) return;
+  const TaskDev td = a.tasks[ti];  // issued together with the flag load
+  if (st && st->done) return;
+  T* front = reinterpret_cast<T*>(smem_raw);
+  T* mat = front + 3 * (size_t)fmax;
+  int* usl = reinterpret_cast<int*>(mat + 256 * E);
+  fwd_task_st<E, T>(td, a, M, y, F, front, mat, usl);
+}
+
 // One band of the backward sweep per launch. In the last band of an FP64
 // solve the tasks also commit their columns and the extra blocks commit the
 // columns outside the deepest band (solved by earlier launches).
