@@ -95,7 +95,8 @@ typedef struct {
   /* Solve schedule: consecutive dissection levels are merged into bands, one
    * dense block per band node, chosen to minimise
    * 2*(band_launch_us + block bytes / band_gbps) per band. 0 = defaults
-   * (3 levels max, 6 us, 4000 GB/s); max_band_levels = 1 disables merging. */
+   * (3 levels max in FP64, 4 in FP32-IR, 6 us, 4000 GB/s); max_band_levels
+   * = 1 disables merging. */
   int32_t max_band_levels;
   double band_launch_us;
   double band_gbps;

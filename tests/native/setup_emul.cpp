@@ -125,12 +125,13 @@ extern "C" int emul_index_fits(long long m_total, long long sroot_entries, long 
 extern "C" int emul_factor(int n, const float* px, const float* py, const float* pz, int t, const uint32_t* tri,
                            int h, const uint32_t* hinge, const uint8_t* pinned, double k_mem, double k_bend,
                            double density, const double* km, const double* kb, const double* dens, int value_bytes,
-                           double* M, double* Mt, double* Sr, long long* sizes) {
+                           int max_band_levels, double* M, double* Mt, double* Sr, long long* sizes) {
   vh::SetupInput in;
   in.n = n; in.t = t; in.h = h; in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge;
   in.pinned = pinned; in.k_membrane = k_mem; in.k_bending = k_bend; in.density = density;
   in.k_membrane_tri = km; in.k_bending_hinge = kb; in.density_tri = dens;
   in.value_bytes = value_bytes;
+  if (max_band_levels > 0) in.max_band_levels = max_band_levels;
   vh::Setup S;
   if (!vh::build_setup(in, S)) return S.status ? S.status : -1;
   sizes[0] = S.m_total;

@@ -21,7 +21,7 @@ def _p(a, t):
 
 
 def host_blocks(emul, px, py, pz, idx, hg, pin, value_bytes=8, k_mem=1e3, k_bend=1e-4, density=0.15, km=None,
-                kb=None, dens=None):
+                kb=None, dens=None, max_band_levels=0):
     f = emul.emul_factor
     sizes = (C.c_longlong * 2)()
     args = [px.size, _p(px.astype(np.float32), C.c_float), _p(py.astype(np.float32), C.c_float),
@@ -30,7 +30,7 @@ def host_blocks(emul, px, py, pz, idx, hg, pin, value_bytes=8, k_mem=1e3, k_bend
             C.c_double(k_mem), C.c_double(k_bend), C.c_double(density),
             _p(None if km is None else np.asarray(km, np.float64), C.c_double),
             _p(None if kb is None else np.asarray(kb, np.float64), C.c_double),
-            _p(None if dens is None else np.asarray(dens, np.float64), C.c_double), value_bytes]
+            _p(None if dens is None else np.asarray(dens, np.float64), C.c_double), value_bytes, max_band_levels]
     assert f(*args, None, None, None, sizes) == 0
     M, Mt, Sr = np.empty(sizes[0]), np.empty(sizes[0]), np.empty(sizes[1])
     assert f(*args, _p(M, C.c_double), _p(Mt, C.c_double), _p(Sr, C.c_double), sizes) == 0
@@ -56,7 +56,10 @@ def _compare(g, ref_blocks, precision):
                                          (256, "f32_ir")])
 def test_device_factor_matches_host(emul, c, precision):
     px, py, pz, idx, hg, pin = sheet(c)
-    ref_blocks = host_blocks(emul, px, py, pz, idx, hg, pin, value_bytes=8 if precision == "f64" else 4)
+    # the device's default band depth: 3 levels in FP64, 4 in FP32-IR
+    mbl = 3 if precision == "f64" else 4
+    ref_blocks = host_blocks(emul, px, py, pz, idx, hg, pin, value_bytes=8 if precision == "f64" else 4,
+                             max_band_levels=mbl)
     g = gpu_solver(px, py, pz, idx, hg, pin, precision=precision)
     info = g.info()
     errs = _compare(g, ref_blocks, precision)

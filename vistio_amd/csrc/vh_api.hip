@@ -126,6 +126,11 @@ struct vh_ctx {
   vhk::TaskDev* st_tasks = nullptr;
   std::vector<int> st_off;
   int st_E = 0;
+  // The first forward launch from the RHS slots, staged (k_fwd_band_st1):
+  // nst1 tasks with their slot ranges, st1_NS slot values per thread (0:
+  // k_fwd_band<T, true>).
+  vhk::TaskDev* st1_tasks = nullptr;
+  int nst1 = 0, st1_NS = 0;
   int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
   void *M = nullptr, *Mt = nullptr, *Sroot = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
@@ -324,6 +329,38 @@ void launch_fwd_st(vh_ctx* c, const vhk::BandArgs<T>& a, int grid, size_t lds, i
   }
 }
 
+size_t staged1_lds(const vh_ctx* c, int fmax) {
+  return 3 * 256 * sizeof(double) + (3 * (size_t)fmax + 256 * (size_t)c->st_E + 256 * (size_t)c->st1_NS) * c->tsz +
+         512 * sizeof(int);
+}
+
+#define VH_ST1_NS(X) X(8) X(16)
+
+template <class T>
+const void* fwd_st1_kernel(int ns) {
+  switch (ns) {
+#define VH_CASE(N) \
+  case N: return (const void*)vhk::k_fwd_band_st1<kStagedLoads, N, T>;
+    VH_ST1_NS(VH_CASE)
+#undef VH_CASE
+    default: return nullptr;
+  }
+}
+
+template <class T>
+void launch_fwd_st1(vh_ctx* c, const vhk::BandArgs<T>& a, int grid, size_t lds, int fmax, const vhk::StepDev* st) {
+  switch (c->st1_NS) {
+#define VH_CASE(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL((vhk::k_fwd_band_st1<kStagedLoads, N, T>), dim3(grid), dim3(256), lds, c->stream, a, \
+                       (const T*)c->M, (T*)c->y, (T*)c->F, fmax, st);                                         \
+    break;
+    VH_ST1_NS(VH_CASE)
+#undef VH_CASE
+    default: break;
+  }
+}
+
 size_t staged_lds(const vh_ctx* c, int fmax) {
   return (3 * (size_t)fmax + 256 * (size_t)c->st_E) * c->tsz + 512 * sizeof(int);
 }
@@ -386,8 +423,15 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
       mk.mark(c->stream);
       continue;
     }
-    const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     const bool first = l == 0 && from_slots;
+    if (first && c->nst1 > 0) {
+      a.tasks = c->st1_tasks; a.ntasks = c->nst1; a.xper = band_xper(c->nst1, true);
+      const int fmax = S.band_front[S.max_depth];
+      launch_fwd_st1<T>(c, a, band_task_blocks(c->nst1, true) + nblk(4 * nupper), staged1_lds(c, fmax), fmax, st);
+      mk.mark(c->stream);
+      continue;
+    }
+    const int b = S.fwd_level_off[l], e = S.fwd_level_off[l + 1];
     a.tasks = c->tasks + b; a.ntasks = e - b; a.xper = band_xper(e - b, l == 0);
     const int grid = band_task_blocks(e - b, l == 0) + (first ? nblk(4 * nupper) : 0);
     const size_t lds = 3 * (size_t)S.band_front[S.max_depth - l] * c->tsz;
@@ -1314,7 +1358,11 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   }
   in.leaf_size = P.leaf_size > 0 ? P.leaf_size : 16;
   in.value_bytes = (int)ctx->tsz;
-  if (P.max_band_levels > 0) in.max_band_levels = P.max_band_levels;
+  // Default band depth: 4 levels in FP32-IR, whose staged narrow bands
+  // make deeper merging pay (256x256: 5 -> 4 bands, 1070 -> 1075 steps/s;
+  // 128x128: 4 -> 3 bands, 2411 -> 2626), 3 in FP64 (4 levels: 727 -> 655
+  // at 256x256).
+  in.max_band_levels = P.max_band_levels > 0 ? P.max_band_levels : (ctx->prec == VH_PREC_F64 ? 3 : 4);
   if (P.band_launch_us > 0) in.band_launch_us = P.band_launch_us;
   if (P.band_gbps > 0) in.band_gbps = P.band_gbps;
   if (P.task_entries > 0) in.task_entries = P.task_entries;
@@ -1437,6 +1485,57 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
     }
     ctx->st_off[S.max_depth + 1] = (int)sv.size();
     if (!sv.empty()) UP(upload(ctx, &ctx->st_tasks, sv));
+  }
+  // Staged first forward launch: the deepest band's supernodes (no
+  // children) cut into tasks of at most 255 columns, 256 rows, 256 st_E
+  // entries and 4096 RHS slot values; the slot run of the task's columns
+  // goes into the record.
+  ctx->st1_tasks = nullptr;
+  ctx->nst1 = 0; ctx->st1_NS = 0;
+  if (ctx->st_E > 0 && S.max_depth > 0) {
+    const int cap = 256 * ctx->st_E, scap = 256 * 16;
+    std::vector<vhk::TaskDev> fv;
+    bool fits = true;
+    int maxsv = 0;
+    for (const vh::Supernode& q : S.sn) {
+      if (q.depth != S.max_depth || !fits) continue;
+      if (q.fslot_k != 0 || q.w > std::min(cap, 255) ||
+          3 * (int64_t)(S.inc_ptr[q.col0 + q.w] - S.inc_ptr[q.col0]) > scap) { fits = false; break; }
+      const int m = q.w + q.r;
+      int g = 4;
+      while (g < 64 && g * 8 < q.w) g *= 2;
+      for (int b = 0; b < m;) {
+        int e = b;
+        int64_t ent = 0;
+        while (e < m && e - b < 256) {
+          const int len = e < q.w ? e + 1 : q.w;
+          if (e > b && ent + len > cap) break;
+          ent += len;
+          ++e;
+        }
+        const int tl = std::min(e, q.w);
+        vhk::TaskDev x{};
+        x.q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
+        x.q1 = make_int4(b, e, g, q.u_off);
+        x.q2 = make_int4(q.fm_off, 0, S.inc_ptr[q.col0], S.inc_ptr[q.col0 + tl]);
+        maxsv = std::max(maxsv, 3 * (S.inc_ptr[q.col0 + tl] - S.inc_ptr[q.col0]));
+        fv.push_back(x);
+        b = e;
+      }
+    }
+    if (fits && !fv.empty()) {
+      ctx->st1_NS = maxsv <= 256 * 8 ? 8 : 16;
+      ctx->nst1 = (int)fv.size();
+      UP(upload(ctx, &ctx->st1_tasks, fv));
+      const size_t l1 = staged1_lds(ctx, S.band_front[S.max_depth]);
+      if (l1 > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "staged solve task exceeds the 160 KB LDS");
+      if (l1 > 64 * 1024) {
+        const void* k = ctx->prec == VH_PREC_F64 ? fwd_st1_kernel<double>(ctx->st1_NS) : fwd_st1_kernel<float>(ctx->st1_NS);
+        if (k) VH_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l1));
+      }
+    }
+  }
+  {
     size_t lds = 0;
     for (int f : S.band_front) lds = std::max(lds, staged_lds(ctx, f));
     if (lds > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "staged solve task exceeds the 160 KB LDS");

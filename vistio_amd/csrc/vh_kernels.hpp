@@ -839,7 +839,7 @@ __device__ __forceinline__ void assemble_front(const BandArgs<T>& a, int col0, i
   }
 }
 
-template <int G, bool S, class T>
+template <int G, bool S, bool FB, class T>
 __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
                          T* __restrict__ F, T* front) {
   constexpr int kPrefRows = pref_rows<G>();
@@ -856,12 +856,14 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
   // The front's loads first (front_issue), then the block rows and output
   // slots: their addresses depend only on the task, so all of them are in
   // flight together, and the front's sums wait only for the front.
+  // FB = false (the kernels compiled for more waves per SIMD): the front is
+  // gathered after the rows, in fewer registers.
   constexpr int kFP = front_pos<T>();
   const int pt0 = b > w ? b : w;  // pass-through positions [pt0, e)
   FrontLoads<kFP, T> fl;
   FrontLoads<1, T> pl;
-  if constexpr (!S) front_issue<kFP, true>(fl, a.r, col0, 0, tlen, Fp, K);
-  front_issue<1, false>(pl, a.r, col0, pt0, e, Fp, K);
+  if constexpr (!S && FB) front_issue<kFP, true>(fl, a.r, col0, 0, tlen, Fp, K);
+  if constexpr (FB) front_issue<1, false>(pl, a.r, col0, pt0, e, Fp, K);
   T mv[kPrefRows][kPrefLoads];
   int us[kPrefRows];
 #pragma unroll
@@ -871,7 +873,7 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
     load_row<G, kPrefLoads>(mv[p], Mp + fwd_row_off(row < e ? row : b, w, tri), len, lane);
     us[p] = row < e && row >= w ? a.uslot[uoff + row - w] : 0;
   }
-  if constexpr (S) {
+  if constexpr (S || !FB) {
     assemble_front<S>(a, col0, tlen, Fp, K, e >= w, front);
   } else {
     front_finish<kFP, true>(fl, 0, tlen, Fp, K, front);
@@ -884,8 +886,8 @@ __device__ void fwd_task(const TaskDev td, const BandArgs<T>& a, const T* __rest
       front[3 * j + 2] = (T)rb[2] - az;
     }
   }
-  front_finish<1, false>(pl, pt0, e, Fp, K, front);
-  for (int j = pt0 + (int)blockDim.x + tid; j < e; j += blockDim.x) {
+  if constexpr (FB) front_finish<1, false>(pl, pt0, e, Fp, K, front);
+  for (int j = pt0 + (FB ? (int)blockDim.x : 0) + tid; j < e; j += blockDim.x) {
     T ax, ay, az;
     front_sum(Fp + 3 * (size_t)j * K, K, ax, ay, az);
     front[3 * j + 0] = ax; front[3 * j + 1] = ay; front[3 * j + 2] = az;
@@ -1026,7 +1028,7 @@ __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArg
 // One band of the forward sweep per launch. Blocks [0, ntasks) run tasks;
 // in the first band of a sweep that starts from the slots, the remaining
 // blocks compute (and store) r for the columns outside the deepest band.
-template <class T, bool S>
+template <class T, bool S, bool FB = true>
 __device__ __forceinline__ void fwd_band(const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
                                          T* __restrict__ F, const StepDev* st) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -1037,7 +1039,7 @@ __device__ __forceinline__ void fwd_band(const BandArgs<T>& a, const T* __restri
     const TaskDev td = a.tasks[ti];  // issued together with the flag load
     if (st && st->done) return;
     with_lanes<4>(td.q1.z, [&](auto GC) {
-      fwd_task<decltype(GC)::value, S, T>(td, a, M, y, F, reinterpret_cast<T*>(smem_raw));
+      fwd_task<decltype(GC)::value, S, FB, T>(td, a, M, y, F, reinterpret_cast<T*>(smem_raw));
     });
     return;
   }
@@ -1067,7 +1069,7 @@ __global__ void __launch_bounds__(256) k_fwd_band(BandArgs<T> a, const T* __rest
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
 k_fwd_band_w8(BandArgs<float> a, const float* __restrict__ M, float* __restrict__ y, float* __restrict__ F,
               const StepDev* st) {
-  fwd_band<float, false>(a, M, y, F, st);
+  fwd_band<float, false, false>(a, M, y, F, st);
 }
 
 // The FP64 forward band for the same wide bands, at five waves per SIMD
@@ -1075,7 +1077,7 @@ k_fwd_band_w8(BandArgs<float> a, const float* __restrict__ M, float* __restrict_
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_fwd_band_d5(BandArgs<double> a, const double* __restrict__ M, double* __restrict__ y, double* __restrict__ F,
               const StepDev* st) {
-  fwd_band<double, false>(a, M, y, F, st);
+  fwd_band<double, false, false>(a, M, y, F, st);
 }
 
 // Forward band with each task's block rows staged in LDS (k_fwd_band_st).
@@ -1156,6 +1158,146 @@ __device__ void fwd_task_st(const TaskDev td, const BandArgs<T>& a, const T* __r
       fwd_store(row, w, col0, usl[row - b], ax, ay, az, front, y, F, lane);
     }
   });
+}
+
+// The first forward launch of a sweep that starts from the RHS slots, staged
+// (k_fwd_band_st1). Its supernodes are the deepest, with no children (K =
+// 0), and their columns' RHS slots are one contiguous run, [q2.z, q2.w) in
+// the task record: every thread loads NS slot values of that run, the
+// column starts inc_ptr[col0 .. col0 + tlen] and its column's inertia
+// inputs, in the same round as E block entries (as fwd_task_st). After a
+// barrier, L lanes per column sum the slots from LDS in the order of
+// rhs_group (lane l: slots l, l + L, ... of the column, then the group
+// reduction), add the inertia term, store r when the task covers rows past
+// w, and form the front; after a second barrier the rows' dot products run
+// from LDS. Tasks hold at most 255 columns, 256 rows, 256 E entries and
+// 256 NS slot values (the host cuts them so).
+template <int E, int NS, class T>
+__device__ void fwd_task_st1(const TaskDev td, const BandArgs<T>& a, const T* __restrict__ M, T* __restrict__ y,
+                             T* __restrict__ F, T* front, T* mat, int* usl, T* sl, int* ip, double* ine) {
+  const int col0 = td.q0.x, w = td.q0.y, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
+  const int sb = td.q2.z, nsv = 3 * (td.q2.w - td.q2.z);
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int tlen = e <= w ? e : w;
+  const long long tri = (long long)w * (w + 1) / 2;
+  const long long o0 = fwd_row_off(b, w, tri);
+  const int ne = (int)(fwd_row_off(e, w, tri) - o0);
+  const RhsSrc<T>& rs = a.src;
+  T sv[NS];
+  const T* __restrict__ ssrc = rs.cslot + 3 * (size_t)sb;
+  if (nsv > 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const int k = tid + i * nt;
+      sv[i] = ssrc[k < nsv ? k : 0];
+    }
+  }
+  const int ipv = tid <= tlen ? rs.inc_ptr[col0 + tid] : 0;
+  double3 in = make_double3(0, 0, 0);
+  if (tid < tlen) in = inertia(rs, col0 + tid);
+  T mv[E];
+  const T* __restrict__ src = M + moff + o0;
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      mv[i] = __builtin_nontemporal_load(src + (k < ne ? k : 0));
+    }
+  }
+  const int rt = b + tid;
+  const int u = rt < e && rt >= w ? a.uslot[uoff + rt - w] : 0;
+  if (nsv > 0) {
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+      const int k = tid + i * nt;
+      if (k < nsv) sl[k] = sv[i];
+    }
+  }
+  if (tid <= tlen) ip[tid] = ipv;
+  if (tid < tlen) { ine[3 * tid] = in.x; ine[3 * tid + 1] = in.y; ine[3 * tid + 2] = in.z; }
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      if (k < ne) mat[k] = mv[i];
+    }
+  }
+  if (rt < e) usl[tid] = u;
+  // pass-through positions (rows past w): no children, so zero
+  for (int j = (b > w ? b : w) + tid; j < e; j += nt) { front[3 * j] = 0; front[3 * j + 1] = 0; front[3 * j + 2] = 0; }
+  __syncthreads();
+  const bool store_r = e >= w;
+  int L = 1;
+  while (L < 8 && nt / (2 * L) >= tlen) L *= 2;
+  with_lanes<1>(L, [&](auto LC) {
+    constexpr int Lc = decltype(LC)::value;
+    const int ln = tid & (Lc - 1), gp = tid / Lc, ng = nt / Lc;
+    for (int j = gp; j < tlen; j += ng) {
+      T ax = 0, ay = 0, az = 0;
+      const int p0 = ip[j] - sb, pe = ip[j + 1] - sb;
+      for (int p = p0 + ln; p < pe; p += Lc) { ax += sl[3 * p]; ay += sl[3 * p + 1]; az += sl[3 * p + 2]; }
+      sum3<Lc>(ax, ay, az);
+      if (ln == 0) {
+        const double rx = ine[3 * j] + (double)ax, ry = ine[3 * j + 1] + (double)ay, rz = ine[3 * j + 2] + (double)az;
+        if (store_r) { double* o = a.r + 3 * (size_t)(col0 + j); o[0] = rx; o[1] = ry; o[2] = rz; }
+        front[3 * j + 0] = (T)rx; front[3 * j + 1] = (T)ry; front[3 * j + 2] = (T)rz;
+      }
+    }
+  });
+  __syncthreads();
+  with_lanes<4>(td.q1.z, [&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    const int lane = tid & (G - 1), ngrp = nt / G;
+    for (int row = b + tid / G; row < e; row += ngrp) {  // uniform within the group
+      const int len = row < w ? row + 1 : w;
+      const T* __restrict__ mr = mat + (fwd_row_off(row, w, tri) - o0);
+      T ax = 0, ay = 0, az = 0;
+#pragma unroll 4
+      for (int k = lane; k < len; k += G) {
+        const T m = mr[k];
+        ax += m * front[3 * k]; ay += m * front[3 * k + 1]; az += m * front[3 * k + 2];
+      }
+      sum3<G>(ax, ay, az);
+      fwd_store(row, w, col0, usl[row - b], ax, ay, az, front, y, F, lane);
+    }
+  });
+}
+
+// Dynamic LDS of k_fwd_band_st1: the front (3 fmax values), the staged
+// entries (256 E), the slot values (256 NS), the output slot numbers (256
+// ints), the column starts (256 ints), the inertia terms (3 x 256 doubles).
+// Blocks past the task blocks form r for the columns outside the deepest
+// band, as k_fwd_band<T, true>.
+template <int E, int NS, class T>
+__global__ void __launch_bounds__(256) k_fwd_band_st1(BandArgs<T> a, const T* __restrict__ M, T* __restrict__ y,
+                                                      T* __restrict__ F, int fmax, const StepDev* st) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int ntb = task_blocks(a);
+  if ((int)blockIdx.x < ntb) {
+    const int ti = task_of(a, (int)blockIdx.x);
+    if (ti < 0) return;
+    const TaskDev td = a.tasks[ti];  // issued together with the flag load
+    if (st && st->done) return;
+    double* ine = reinterpret_cast<double*>(smem_raw);
+    T* front = reinterpret_cast<T*>(ine + 3 * 256);
+    T* mat = front + 3 * (size_t)fmax;
+    T* sl = mat + 256 * E;
+    int* usl = reinterpret_cast<int*>(sl + 256 * NS);
+    int* ip = usl + 256;
+    fwd_task_st1<E, NS, T>(td, a, M, y, F, front, mat, usl, sl, ip, ine);
+    return;
+  }
+  if (st && st->done) return;
+  const int k = ((int)blockIdx.x - ntb) * (blockDim.x / 4) + (int)threadIdx.x / 4;
+  const int lane = threadIdx.x & 3;
+  if (k >= a.nupper) return;  // uniform within each 4-lane group
+  const int col = a.upper_cols[k];
+  const double3 rb = rhs_group<4>(a.src, col, lane);
+  if (lane == 0) {
+    double* o = a.r + 3 * (size_t)col;
+    o[0] = rb.x; o[1] = rb.y; o[2] = rb.z;
+  }
 }
 
 // Dynamic LDS of k_fwd_band_st: the front (3 fmax values), then the staged
