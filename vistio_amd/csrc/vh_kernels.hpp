@@ -927,8 +927,10 @@ __device__ __forceinline__ void commit_col(const CommitArgs& c, int col, double 
 // so that the stores after the dot product wait on no further load: the
 // range of its pushes and, when committing (C), component `lane` (< 3) of
 // x, prev and acc.
+// pt: the lane's first push target dpush_idx[q0 + lane] when push_pre
+// loaded it, else -1.
 struct ColPre {
-  int q0, q1, pin;
+  int q0, q1, pin, pt;
   double xo, po, ao;
 };
 
@@ -936,6 +938,7 @@ template <bool C, class T>
 __device__ __forceinline__ void col_pre(const BandArgs<T>& a, const CommitArgs& c, int col, int lane, ColPre& p) {
   p.q0 = a.dpush_ptr[col];
   p.q1 = a.dpush_ptr[col + 1];
+  p.pt = -1;
   p.pin = 0; p.xo = 0; p.po = 0; p.ao = 0;
   if (C && lane < 3) {
     p.xo = c.x[3 * (size_t)col + lane];
@@ -956,7 +959,7 @@ __device__ __forceinline__ void col_store(const BandArgs<T>& a, const CommitArgs
   const T s = lane == 0 ? sx : (lane == 1 ? sy : sz);
   if (lane < 3) d[3 * (size_t)col + lane] = s;
   for (int q = p.q0 + lane; q < p.q1; q += g) {
-    T* o = V + 3 * (size_t)a.dpush_idx[q];
+    T* o = V + 3 * (size_t)(q == p.q0 + lane && p.pt >= 0 ? p.pt : a.dpush_idx[q]);
     o[0] = -sx; o[1] = -sy; o[2] = -sz;
   }
   if (C && lane < 3) {
@@ -978,7 +981,7 @@ __device__ __forceinline__ void col_store(const BandArgs<T>& a, const CommitArgs
 //          = sum_i Mt[j][i] v[j + i].
 // Each d_P[j] is stored, pushed (negated) to the v slots of the descendants
 // that read it, and, when committing, added to x.
-template <int G, bool C, class T>
+template <int G, bool C, bool FB, class T>
 __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
                          const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, double& df, double& nm) {
   constexpr int kPrefRows = pref_rows<G>();
@@ -986,23 +989,62 @@ __device__ void bwd_task(const TaskDev td, const BandArgs<T>& a, const CommitArg
   const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
   const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
   const int m = w + r;
-  const int tid = threadIdx.x;
-  const int lane = tid & (G - 1), grp = tid / G, ngrp = blockDim.x / G;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  const int lane = tid & (G - 1), grp = tid / G, ngrp = nt / G;
   const T* Tp = Mt + moff;
   auto row_ptr = [&](int j) { return Tp + (long long)j * m - (long long)j * (j - 1) / 2; };
   T mv[kPrefRows][kPrefLoads];
   ColPre cp[kPrefRows];
+  if constexpr (FB) {
+    // As in fwd_task: the small L2 loads (push ranges, commit values, v)
+    // are issued before the block rows, so their uses wait only for them;
+    // each lane's first push target is loaded before the barrier too.
+    constexpr int PV = front_pos<T>();
 #pragma unroll
-  for (int p = 0; p < kPrefRows; ++p) {
-    const int j = b + grp + p * ngrp;
-    const int jj = j < e ? j : b;
-    load_row<G, kPrefLoads>(mv[p], row_ptr(jj), j < e ? m - j : 0, lane);
-    col_pre<C>(a, c, col0 + jj, lane, cp[p]);
-  }
+    for (int p = 0; p < kPrefRows; ++p) {
+      const int j = b + grp + p * ngrp;
+      col_pre<C>(a, c, col0 + (j < e ? j : b), lane, cp[p]);
+    }
+    T vv[PV][3];
+#pragma unroll
+    for (int p = 0; p < PV; ++p) {
+      const int k = b + tid + p * nt;
+      const int kk = k < m ? k : b;
+      const T* sv = kk < w ? y + 3 * (size_t)(col0 + kk) : V + 3 * (size_t)(uoff + kk - w);
+      vv[p][0] = sv[0]; vv[p][1] = sv[1]; vv[p][2] = sv[2];
+    }
+#pragma unroll
+    for (int p = 0; p < kPrefRows; ++p) {
+      const int j = b + grp + p * ngrp;
+      load_row<G, kPrefLoads>(mv[p], row_ptr(j < e ? j : b), j < e ? m - j : 0, lane);
+    }
+#pragma unroll
+    for (int p = 0; p < kPrefRows; ++p) {
+      const int q = cp[p].q0 + lane;
+      if (q < cp[p].q1) cp[p].pt = a.dpush_idx[q];
+    }
+#pragma unroll
+    for (int p = 0; p < PV; ++p) {
+      const int k = b + tid + p * nt;
+      if (k < m) { v[3 * k + 0] = vv[p][0]; v[3 * k + 1] = vv[p][1]; v[3 * k + 2] = vv[p][2]; }
+    }
+    for (int k = b + PV * nt + tid; k < m; k += nt) {
+      const T* sv = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
+      v[3 * k + 0] = sv[0]; v[3 * k + 1] = sv[1]; v[3 * k + 2] = sv[2];
+    }
+  } else {
+#pragma unroll
+    for (int p = 0; p < kPrefRows; ++p) {
+      const int j = b + grp + p * ngrp;
+      const int jj = j < e ? j : b;
+      load_row<G, kPrefLoads>(mv[p], row_ptr(jj), j < e ? m - j : 0, lane);
+      col_pre<C>(a, c, col0 + jj, lane, cp[p]);
+    }
 #pragma unroll 4
-  for (int k = b + tid; k < m; k += blockDim.x) {
-    const T* s = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
-    v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
+    for (int k = b + tid; k < m; k += nt) {
+      const T* s = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
+      v[3 * k + 0] = s[0]; v[3 * k + 1] = s[1]; v[3 * k + 2] = s[2];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -1319,7 +1361,7 @@ __global__ void __launch_bounds__(256) k_fwd_band_st(BandArgs<T> a, const T* __r
 // One band of the backward sweep per launch. In the last band of an FP64
 // solve the tasks also commit their columns and the extra blocks commit the
 // columns outside the deepest band (solved by earlier launches).
-template <class T, bool C>
+template <class T, bool C, bool FB = true>
 __device__ __forceinline__ void bwd_band(const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
                                          const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
                                          const StepDev* st) {
@@ -1332,7 +1374,7 @@ __device__ __forceinline__ void bwd_band(const BandArgs<T>& a, const CommitArgs&
       const TaskDev td = a.tasks[ti];  // issued together with the flag load
       if (st && st->done) return;
       with_lanes<4>(td.q1.z, [&](auto GC) {
-        bwd_task<decltype(GC)::value, C, T>(td, a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
+        bwd_task<decltype(GC)::value, C, FB, T>(td, a, c, Mt, y, V, d, reinterpret_cast<T*>(smem_raw), df, nm);
       });
     } else if (st && st->done) {
       return;
@@ -1364,7 +1406,7 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
 k_bwd_band_d5(BandArgs<double> a, CommitArgs c, const double* __restrict__ Mt, const double* __restrict__ y,
               double* __restrict__ V, double* __restrict__ d, const StepDev* st) {
-  bwd_band<double, false>(a, c, Mt, y, V, d, st);
+  bwd_band<double, false, false>(a, c, Mt, y, V, d, st);
 }
 
 // Root task: a root supernode R has no off-diagonal rows, so its forward
