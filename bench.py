@@ -1,7 +1,10 @@
 #!/usr/bin/env python3
-"""PD steps/s of the 256x256 hanging sheet on one MI355X: FP32 local step +
-FP32 supernodal solves with FP64-residual refinement, next to the FP64 GPU
-path and the CPU restatement of the reference step.
+"""PD steps/s of the 256x256 hanging sheet on one MI355X: FP64 state, an FP32
+local step in residual form and FP32 supernodal solves of the correction
+(each PD iteration re-forms r = b - A x from the FP64 state, so the PD loop
+itself refines; extra sweeps per iteration refine with the FP64 CSR
+residual), next to the FP64 GPU path and the CPU restatement of the
+reference step.
 
 Workload: quad_grid(256, 256, 1 m, 1 m) in the z = 0
 plane (66,049 vertices, 131,072 triangles, 196,096 hinges), row 0 pinned,
@@ -371,8 +374,9 @@ def run_batch(args):
                                f"cells a side (mean {cells.mean():.0f}), {b.mesh.vertex_count()} verts, "
                                f"{b.mesh.triangle_count()} tris, per-sheet density 0.08-0.40 kg/m^2, membrane "
                                "3e2-3e3 N/m, bending 1e-5-1e-3 N m (SplitMix64 seeds 0x5EED ^ id), co-rotational, "
-                               "Rayleigh 0.01, dt 1/60 s, 10 PD iterations; FP32 local step + FP32 solves, FP64 "
-                               "residual refinement; one block-diagonal system, one graph per step",
+                               "Rayleigh 0.01, dt 1/60 s, 10 PD iterations; FP64 state, FP32 residual-form local "
+                               "step + one FP32 solve per PD iteration; one block-diagonal system, one graph per "
+                               "step",
                    "sheets": len(specs)},
         "roofline": roofline_of(f32["graph"], 1000.0 * f32["wall"] / steps, sweeps_per_step(args, "f32_ir")),
         "extra": {
@@ -458,9 +462,11 @@ def main():
         "config": {
             "workload": f"hanging_sheet {args.cells}x{args.cells} cells ({mesh.vertex_count()} verts, "
                         f"{mesh.triangle_count()} tris), co-rotational membrane 1e3 N/m + dihedral bending "
-                        f"1e-4 N m, 0.15 kg/m^2, Rayleigh 0.01, dt 1/60 s, 10 PD iterations/step; FP32 local "
-                        f"step + FP32 supernodal solves, FP64 residual refinement "
-                        f"({args.ir_sweeps} sweep/iteration, {args.ir_final_sweeps} on the last), FP64 state",
+                        f"1e-4 N m, 0.15 kg/m^2, Rayleigh 0.01, dt 1/60 s, 10 PD iterations/step; FP64 state; "
+                        f"each PD iteration forms r = b - A x from the FP64 state (element terms in FP32 "
+                        f"residual form after FP64 F, inertia in FP64) and solves A d = r with FP32 supernodal "
+                        f"sweeps ({args.ir_sweeps}/iteration, {args.ir_final_sweeps} on the last; further "
+                        f"sweeps refine with the FP64 CSR residual)",
             "cells": args.cells,
             "pd_iterations_per_step": 10,
         },
