@@ -131,6 +131,10 @@ struct vh_ctx {
   // k_fwd_band<T, true>).
   vhk::TaskDev* st1_tasks = nullptr;
   int nst1 = 0, st1_NS = 0;
+  // The committing last backward band, staged (k_bwd_band_stc): nstc tasks
+  // (0: k_bwd_band<T, true>).
+  vhk::TaskDev* stc_tasks = nullptr;
+  int nstc = 0;
   int *uslot = nullptr, *dpush_ptr = nullptr, *dpush_idx = nullptr, *upper_cols = nullptr;
   void *M = nullptr, *Mt = nullptr, *Sroot = nullptr, *y = nullptr, *d = nullptr, *F = nullptr, *V = nullptr;
   int *Ap = nullptr, *Ai = nullptr;
@@ -361,6 +365,10 @@ void launch_fwd_st1(vh_ctx* c, const vhk::BandArgs<T>& a, int grid, size_t lds, 
   }
 }
 
+size_t stagedc_lds(const vh_ctx* c, int fmax) {
+  return 9 * 256 * sizeof(double) + (3 * (size_t)fmax + 256 * (size_t)c->st_E) * c->tsz + 3 * 256 * sizeof(int);
+}
+
 size_t staged_lds(const vh_ctx* c, int fmax) {
   return (3 * (size_t)fmax + 256 * (size_t)c->st_E) * c->tsz + 512 * sizeof(int);
 }
@@ -490,8 +498,21 @@ void enqueue_solve(vh_ctx* c, double dt, bool from_slots, bool commit, bool add_
     mk.mark(c->stream);
   }
   for (int l = 1; l < nb; ++l) {
-    const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
     const bool cm = l == nb - 1 && commit;
+    if constexpr (std::is_same<T, float>::value) {
+      if (cm && c->nstc > 0) {
+        a.tasks = c->stc_tasks; a.ntasks = c->nstc; a.xper = band_xper(c->nstc, false);
+        const vhk::CommitArgs cc = commit_args(c, true, want_resid, add_acc);
+        const int fmax = S.band_front[l];
+        hipLaunchKernelGGL((vhk::k_bwd_band_stc<kStagedLoads, float>),
+                           dim3(band_task_blocks(c->nstc, false) + nblk(nupper)), dim3(256), stagedc_lds(c, fmax),
+                           c->stream, a, cc, (const float*)c->Mt, (const float*)c->y, (float*)c->V, (float*)c->d,
+                           fmax, st);
+        mk.mark(c->stream);
+        continue;
+      }
+    }
+    const int b = S.bwd_level_off[l], e = S.bwd_level_off[l + 1];
     a.tasks = c->tasks + nfwd + b; a.ntasks = e - b; a.xper = band_xper(e - b, false);
     const vhk::CommitArgs cc = commit_args(c, cm, want_resid, add_acc);
     const int grid = band_task_blocks(e - b, false) + (cm ? nblk(nupper) : 0);
@@ -1535,6 +1556,49 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
       }
     }
   }
+  // Staged committing band (FP32): the deepest band's columns (rows of Mt)
+  // in tasks of at most 256 rows and 256 st_E entries. Measured at 256x256:
+  // FP32-IR 1108 -> 1124 steps/s (4 bands); FP64 753 -> 748, so FP64 keeps
+  // k_bwd_band<double, true>.
+  ctx->stc_tasks = nullptr;
+  ctx->nstc = 0;
+  if (ctx->st_E > 0 && S.max_depth > 0 && ctx->prec != VH_PREC_F64) {
+    const int cap = 256 * ctx->st_E;
+    std::vector<vhk::TaskDev> cv;
+    bool fits = true;
+    for (const vh::Supernode& q : S.sn) {
+      if (q.depth != S.max_depth || q.parent < 0) continue;
+      const int m = q.w + q.r;
+      if (m > cap) { fits = false; break; }
+      int g = 4;
+      while (g < 64 && g * 8 < m) g *= 2;
+      for (int b = 0; b < q.w;) {
+        int e = b;
+        int64_t ent = 0;
+        while (e < q.w && e - b < 256) {
+          const int len = m - e;
+          if (e > b && ent + len > cap) break;
+          ent += len;
+          ++e;
+        }
+        vhk::TaskDev x{};
+        x.q0 = make_int4(q.col0, q.w, q.r, (int)q.m_off);
+        x.q1 = make_int4(b, e, g, q.u_off);
+        x.q2 = make_int4(q.fm_off, q.fslot_k, 0, 0);
+        cv.push_back(x);
+        b = e;
+      }
+    }
+    if (fits && !cv.empty()) {
+      ctx->nstc = (int)cv.size();
+      UP(upload(ctx, &ctx->stc_tasks, cv));
+      const size_t lc = stagedc_lds(ctx, S.band_front[S.max_depth]);
+      if (lc > 160 * 1024) return fail(ctx, VH_E_UNSUPPORTED, "staged solve task exceeds the 160 KB LDS");
+      if (lc > 64 * 1024)
+        VH_HIP(hipFuncSetAttribute((const void*)vhk::k_bwd_band_stc<kStagedLoads, float>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lc));
+    }
+  }
   {
     size_t lds = 0;
     for (int f : S.band_front) lds = std::max(lds, staged_lds(ctx, f));
@@ -1569,7 +1633,8 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   // band, or the root band when it is the only band.
   const int nb = S.max_depth + 1;
   ctx->resid_blocks = nb == 1 ? band_task_blocks(S.fwd_level_off[1] - S.fwd_level_off[0], false)
-                              : band_task_blocks(S.bwd_level_off[nb] - S.bwd_level_off[nb - 1], false) +
+                              : band_task_blocks(ctx->nstc > 0 ? ctx->nstc : S.bwd_level_off[nb] - S.bwd_level_off[nb - 1],
+                                                 false) +
                                     nblk((int)S.upper_cols.size());
   UP(dalloc(ctx, &ctx->partials, 2 * (size_t)std::max(1, ctx->resid_blocks)));
   UP(dalloc(ctx, &ctx->st, 1));

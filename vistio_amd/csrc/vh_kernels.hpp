@@ -1401,6 +1401,143 @@ __global__ void __launch_bounds__(256) k_bwd_band(BandArgs<T> a, CommitArgs c, c
   bwd_band<T, C>(a, c, Mt, y, V, d, st);
 }
 
+// The last backward band, committing, with each task's rows of Mt staged in
+// LDS (k_bwd_band_stc), as k_fwd_band_st: the task's rows [b, e) of Mt
+// are one contiguous run, loaded E per thread after v = [y_P ; V_P]
+// (positions b .. m) and each row's push range and commit values (one row
+// per thread, at most 256 rows); the dot products, pushes and commits then
+// take their operands from LDS. Lane l of a group sums entries l, l + G,
+// ... of its row as bwd_task does, so d and x are the same; only the
+// residual partials are summed over another thread order.
+template <int E, class T>
+__device__ void bwd_task_stc(const TaskDev td, const BandArgs<T>& a, const CommitArgs& c, const T* __restrict__ Mt,
+                             const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d, T* v, T* mat, int* meta,
+                             double* cmeta, double& df, double& nm) {
+  const int col0 = td.q0.x, w = td.q0.y, r = td.q0.z, moff = td.q0.w;
+  const int b = td.q1.x, e = td.q1.y, uoff = td.q1.w;
+  const int m = w + r;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  auto roff = [&](int j) { return (long long)j * m - (long long)j * (j - 1) / 2; };
+  const long long o0 = roff(b);
+  const int ne = (int)(roff(e) - o0);
+  // row metadata of row b + tid: push range, pinned flag, x, prev, acc
+  const int jr = b + tid;
+  const bool hr = jr < e;
+  const int cj = col0 + (hr ? jr : b);
+  const int q0 = a.dpush_ptr[cj], q1 = a.dpush_ptr[cj + 1];
+  const int pin = c.pinned[cj];
+  double xo[3], po[3], ao[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    xo[k] = c.x[3 * (size_t)cj + k];
+    po[k] = c.prev[3 * (size_t)cj + k];
+    ao[k] = c.add_acc ? c.acc[3 * (size_t)cj + k] : 0.0;
+  }
+  T vv[2][3];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int k = b + tid + p * nt;
+    const int kk = k < m ? k : b;
+    const T* sv = kk < w ? y + 3 * (size_t)(col0 + kk) : V + 3 * (size_t)(uoff + kk - w);
+    vv[p][0] = sv[0]; vv[p][1] = sv[1]; vv[p][2] = sv[2];
+  }
+  T mv[E];
+  const T* __restrict__ src = Mt + moff + o0;
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      mv[i] = __builtin_nontemporal_load(src + (k < ne ? k : 0));
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+    const int k = b + tid + p * nt;
+    if (k < m) { v[3 * k + 0] = vv[p][0]; v[3 * k + 1] = vv[p][1]; v[3 * k + 2] = vv[p][2]; }
+  }
+  for (int k = b + 2 * nt + tid; k < m; k += nt) {
+    const T* sv = k < w ? y + 3 * (size_t)(col0 + k) : V + 3 * (size_t)(uoff + k - w);
+    v[3 * k + 0] = sv[0]; v[3 * k + 1] = sv[1]; v[3 * k + 2] = sv[2];
+  }
+  if (ne > 0) {
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+      const int k = tid + i * nt;
+      if (k < ne) mat[k] = mv[i];
+    }
+  }
+  if (hr) {
+    meta[3 * tid] = q0; meta[3 * tid + 1] = q1; meta[3 * tid + 2] = pin;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      cmeta[9 * tid + k] = xo[k]; cmeta[9 * tid + 3 + k] = po[k]; cmeta[9 * tid + 6 + k] = ao[k];
+    }
+  }
+  __syncthreads();
+  with_lanes<4>(td.q1.z, [&](auto GC) {
+    constexpr int G = decltype(GC)::value;
+    const int lane = tid & (G - 1), ngrp = nt / G;
+    for (int j = b + tid / G; j < e; j += ngrp) {  // uniform within the group
+      const int len = m - j;
+      const T* __restrict__ mr = mat + (roff(j) - o0);
+      const T* __restrict__ vj = v + 3 * j;
+      T ax = 0, ay = 0, az = 0;
+#pragma unroll 4
+      for (int k = lane; k < len; k += G) {
+        const T mk = mr[k];
+        ax += mk * vj[3 * k]; ay += mk * vj[3 * k + 1]; az += mk * vj[3 * k + 2];
+      }
+      sum3<G>(ax, ay, az);
+      const int t = j - b;
+      ColPre cp;
+      cp.q0 = meta[3 * t]; cp.q1 = meta[3 * t + 1]; cp.pin = meta[3 * t + 2]; cp.pt = -1;
+      const int cl = lane < 3 ? lane : 0;
+      cp.xo = cmeta[9 * t + cl]; cp.po = cmeta[9 * t + 3 + cl]; cp.ao = cmeta[9 * t + 6 + cl];
+      col_store<true>(a, c, col0 + j, ax, ay, az, cp, lane, G, V, d, df, nm);
+    }
+  });
+}
+
+// Dynamic LDS of k_bwd_band_stc: the 256 rows' commit values (9 doubles
+// each), v (3 fmax values), the staged entries (256 E), the rows' push
+// ranges and pinned flags (3 x 256 ints). Blocks past the task blocks
+// commit the columns outside the deepest band, and every block writes its
+// residual partials when asked, as k_bwd_band<T, true>.
+template <int E, class T>
+__global__ void __launch_bounds__(256) k_bwd_band_stc(BandArgs<T> a, CommitArgs c, const T* __restrict__ Mt,
+                                                      const T* __restrict__ y, T* __restrict__ V, T* __restrict__ d,
+                                                      int fmax, const StepDev* st) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  double df = 0, nm = 0;
+  const int ntb = task_blocks(a);
+  if ((int)blockIdx.x < ntb) {
+    const int ti = task_of(a, (int)blockIdx.x);
+    if (ti >= 0) {  // an idle padding block still writes its (zero) partials
+      const TaskDev td = a.tasks[ti];  // issued together with the flag load
+      if (st && st->done) return;
+      double* cmeta = reinterpret_cast<double*>(smem_raw);
+      T* v = reinterpret_cast<T*>(cmeta + 9 * 256);
+      T* mat = v + 3 * (size_t)fmax;
+      int* meta = reinterpret_cast<int*>(mat + 256 * E);
+      bwd_task_stc<E, T>(td, a, c, Mt, y, V, d, v, mat, meta, cmeta, df, nm);
+    } else if (st && st->done) {
+      return;
+    }
+  } else {
+    if (st && st->done) return;
+    const int k = ((int)blockIdx.x - ntb) * blockDim.x + threadIdx.x;
+    if (k < a.nupper) {
+      const int col = a.upper_cols[k];
+      const T* s = d + 3 * (size_t)col;
+      commit_col(c, col, (double)s[0], (double)s[1], (double)s[2], df, nm);
+    }
+  }
+  if (c.want_resid) {
+    __syncthreads();
+    block_partials(df, nm, c.partials);
+  }
+}
+
 // The FP64 backward band (not the committing one) at five waves per SIMD,
 // for bands of many tasks, as k_fwd_band_d5.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5)))
