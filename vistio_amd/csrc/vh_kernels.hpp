@@ -727,10 +727,18 @@ __device__ __forceinline__ void fwd_store(int row, int w, int col0, int us, T sx
 template <class T>
 __device__ __forceinline__ void front_sum(const T* __restrict__ f, int K, T& ax, T& ay, T& az) {
   ax = 0; ay = 0; az = 0;
-  if (K > 0) { ax += f[0]; ay += f[1]; az += f[2]; }
-  if (K > 1) { ax += f[3]; ay += f[4]; az += f[5]; }
-  if (K > 2) { ax += f[6]; ay += f[7]; az += f[8]; }
-  for (int k = 3; k < K; ++k) { ax += f[3 * k]; ay += f[3 * k + 1]; az += f[3 * k + 2]; }
+  if (K <= 0) return;
+  // the first four slots loaded together (clamped), then summed in order
+  T v[4][3];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const T* s = f + 3 * (q < K ? q : 0);
+    v[q][0] = s[0]; v[q][1] = s[1]; v[q][2] = s[2];
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (q < K) { ax += v[q][0]; ay += v[q][1]; az += v[q][2]; }
+  for (int k = 4; k < K; ++k) { ax += f[3 * k]; ay += f[3 * k + 1]; az += f[3 * k + 2]; }
 }
 
 // Front assembly with every load issued before any is waited on. Loads in
