@@ -533,22 +533,26 @@ __device__ __forceinline__ void with_lanes(int g, F&& f) {
 // lane are issued together (clamped addresses, masked sums).
 template <int L, class T>
 __device__ __forceinline__ double3 rhs_group(const RhsSrc<T>& s, int i, int lane) {
+  // the inertia inputs are loaded with the slot range, then U slots per
+  // lane at a time (8 in FP32, 4 in FP64), each lane summing its slots in
+  // order
+  constexpr int U = sizeof(T) == 4 ? 8 : 4;
   T ax = 0, ay = 0, az = 0;
   const int p0 = s.inc_ptr[i], pe = s.inc_ptr[i + 1];
-  for (int p = p0 + lane; p < pe; p += 4 * L) {
-    T v[4][3];
+  const double3 in = inertia(s, i);
+  for (int p = p0 + lane; p < pe; p += U * L) {
+    T v[U][3];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < U; ++u) {
       const int q = p + u * L;
       const T* c = s.cslot + 3 * (size_t)(q < pe ? q : p);
       v[u][0] = c[0]; v[u][1] = c[1]; v[u][2] = c[2];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
       if (p + u * L < pe) { ax += v[u][0]; ay += v[u][1]; az += v[u][2]; }
   }
   sum3<L>(ax, ay, az);
-  const double3 in = inertia(s, i);
   return make_double3(in.x + (double)ax, in.y + (double)ay, in.z + (double)az);
 }
 
