@@ -338,7 +338,7 @@ size_t staged1_lds(const vh_ctx* c, int fmax) {
          512 * sizeof(int);
 }
 
-#define VH_ST1_NS(X) X(8) X(16)
+#define VH_ST1_NS(X) X(8) X(16) X(32)
 
 template <class T>
 const void* fwd_st1_kernel(int ns) {
@@ -1509,12 +1509,12 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   }
   // Staged first forward launch: the deepest band's supernodes (no
   // children) cut into tasks of at most 255 columns, 256 rows, 256 st_E
-  // entries and 4096 RHS slot values; the slot run of the task's columns
+  // entries and 8192 RHS slot values; the slot run of the task's columns
   // goes into the record.
   ctx->st1_tasks = nullptr;
   ctx->nst1 = 0; ctx->st1_NS = 0;
   if (ctx->st_E > 0 && S.max_depth > 0) {
-    const int cap = 256 * ctx->st_E, scap = 256 * 16;
+    const int cap = 256 * ctx->st_E, scap = 256 * 32;
     std::vector<vhk::TaskDev> fv;
     bool fits = true;
     int maxsv = 0;
@@ -1545,7 +1545,7 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
       }
     }
     if (fits && !fv.empty()) {
-      ctx->st1_NS = maxsv <= 256 * 8 ? 8 : 16;
+      ctx->st1_NS = maxsv <= 256 * 8 ? 8 : (maxsv <= 256 * 16 ? 16 : 32);
       ctx->nst1 = (int)fv.size();
       UP(upload(ctx, &ctx->st1_tasks, fv));
       const size_t l1 = staged1_lds(ctx, S.band_front[S.max_depth]);
