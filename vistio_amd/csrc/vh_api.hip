@@ -85,6 +85,13 @@ struct IpcState {
   // phase) returned while a launch sequence was enqueued; take_lib_err reads
   // and clears it.
   hipError_t lib_err = hipSuccess;
+  // The PD loop of one AL iteration (k_ipc_ext, then max_iterations x local
+  // step, solve, step fraction and update), captured once into a graph and
+  // replayed per AL iteration; recaptured when dt or a buffer the launches
+  // name changes (gen counts the regrowths).
+  hipGraphExec_t al_graph = nullptr;
+  double al_graph_dt = 0;
+  unsigned int gen = 0, al_graph_gen = 0;
 };
 
 struct vh_ctx {
@@ -224,8 +231,14 @@ void drop_graph(vh_ctx* ctx) {
   ctx->graph_dt = 0;
 }
 
+void drop_al_graph(vh_ctx* ctx) {
+  if (ctx->ipc.al_graph) (void)hipGraphExecDestroy(ctx->ipc.al_graph);
+  ctx->ipc.al_graph = nullptr;
+}
+
 void free_all(vh_ctx* ctx) {
   drop_graph(ctx);
+  drop_al_graph(ctx);
   for (void* p : ctx->allocs) (void)hipFree(p);
   ctx->allocs.clear();
   ctx->initialized = false;
@@ -790,6 +803,7 @@ hipError_t grow_contacts(vh_ctx* c, unsigned int need) {
   if (e == hipSuccess) e = realloc_tracked(c, (void**)&I.vidx, 4 * cap * sizeof(unsigned int));
   if (e != hipSuccess) return e;
   I.ecap = (unsigned int)cap;
+  ++I.gen;
   hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
   return hipGetLastError();
 }
@@ -834,6 +848,7 @@ hipError_t grow_pairs(vh_ctx* c, unsigned int need) {
     I.cap = 0;
     return e;
   }
+  ++I.gen;
   hipLaunchKernelGGL(k_ipc_clear_overflow, dim3(1), dim3(64), 0, c->stream, I.ip);
   return hipGetLastError();
 }
@@ -962,6 +977,68 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
                      I.inc, I.ever, I.fnrm, I.lambda, reset_lambda ? 1 : 0, I.ip, hess, skip);
 }
 
+// The PD loop of one AL iteration (pd_solver.rs:660-1000): the frozen
+// contact term, then max_iterations x (local step, solve, step fraction
+// from the CCD, update); every launch skips once the device flags say the
+// loop has converged, so the sequence is fixed and is replayed as a graph.
+void enqueue_al_pd(vh_ctx* c, double dt) {
+  IpcState& I = c->ipc;
+  const vh_pd_params& P = c->P;
+  const int n = c->S.n;
+  const double* dl = c->prec == VH_PREC_F64 ? (const double*)c->d : c->acc;
+  hipLaunchKernelGGL(vhc::k_ipc_ext, dim3(nblk(3 * n)), dim3(256), 0, c->stream, n, I.grad, I.lambda, I.ext, I.ip);
+  c->cur_ext = I.ext;
+  for (int it = 0; it < P.max_iterations; ++it) {
+    const bool from_slots = enqueue_local_step(c, c->st);
+    Marks none;
+    const int sweeps = it + 1 == P.max_iterations ? P.ir_final_sweeps : P.ir_sweeps;
+    enqueue_correction(c, dt, sweeps, from_slots, false, false, c->st, none);
+    hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
+                       I.partials, I.ip);
+    if (I.sc.self_collision) enqueue_ccd_pairs(c, c->x, dl);
+    if (I.sc.has_ground || I.sc.has_sphere)
+      hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
+    hipLaunchKernelGGL(vhc::k_ipc_alpha, dim3(1), dim3(256), 0, c->stream, I.partials, nblk(n), P.tolerance, I.ip);
+    hipLaunchKernelGGL(vhc::k_ipc_update, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, c->pinned, I.ip);
+    hipLaunchKernelGGL(vhc::k_ipc_pd_latch, dim3(1), dim3(64), 0, c->stream, I.ip, c->st);
+  }
+  c->cur_ext = nullptr;
+}
+
+// Enqueues enqueue_al_pd: replays its graph (captured on first use, and
+// again after dt or a pair buffer changed), or launches the kernels
+// directly when graphs are disabled. The sequence holds only kernels (no
+// memset or library call).
+hipError_t launch_al_pd(vh_ctx* c, double dt) {
+  IpcState& I = c->ipc;
+  if (!c->use_graph) {
+    enqueue_al_pd(c, dt);
+    return hipGetLastError();
+  }
+  if (!I.al_graph || I.al_graph_dt != dt || I.al_graph_gen != I.gen) {
+    drop_al_graph(c);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    enqueue_al_pd(c, dt);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+      if (g) (void)hipGraphDestroy(g);
+      return le != hipSuccess ? le : ce;
+    }
+    e = hipGraphInstantiate(&I.al_graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+      I.al_graph = nullptr;
+      return e;
+    }
+    I.al_graph_dt = dt;
+    I.al_graph_gen = I.gen;
+  }
+  return hipGraphLaunch(I.al_graph, c->stream);
+}
+
 // step_with_ipc (pd_solver.rs:549-1157) as one launch sequence: the PD loop
 // has a fixed trip count and stops through device flags; after each AL
 // iteration the host reads the AL flag and launches no further AL
@@ -971,7 +1048,6 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   const vh::Setup& S = c->S;
   const vh_pd_params& P = c->P;
   const int n = S.n;
-  const bool f64 = c->prec == VH_PREC_F64;
   const vh_ipc_params& Q = I.P;
   const int* skip = ipc_done_flag(c);
   const double max_vel = 0.5 * std::sqrt(I.cfl_dh) / dt;
@@ -983,27 +1059,11 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
     hipError_t e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); }, true);
     if (e != hipSuccess) return e;
   }
-  const double* dl = f64 ? (const double*)c->d : c->acc;
   for (int al = 0; al < Q.al_max_iterations; ++al) {
-    hipLaunchKernelGGL(vhc::k_ipc_ext, dim3(nblk(3 * n)), dim3(256), 0, c->stream, n, I.grad, I.lambda, I.ext, I.ip);
-    c->cur_ext = I.ext;
-    for (int it = 0; it < P.max_iterations; ++it) {
-      const bool from_slots = enqueue_local_step(c, c->st);
-      Marks none;
-      const int sweeps = it + 1 == P.max_iterations ? P.ir_final_sweeps : P.ir_sweeps;
-      enqueue_correction(c, dt, sweeps, from_slots, false, false, c->st, none);
-      hipLaunchKernelGGL(vhc::k_ipc_sums, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.grad, c->pinned,
-                         I.partials, I.ip);
-      if (I.sc.self_collision) {
-        enqueue_ccd_pairs(c, c->x, dl);
-      }
-      if (I.sc.has_ground || I.sc.has_sphere)
-        hipLaunchKernelGGL(vhc::k_ccd_vertices, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, I.sc, I.ip);
-      hipLaunchKernelGGL(vhc::k_ipc_alpha, dim3(1), dim3(256), 0, c->stream, I.partials, nblk(n), P.tolerance, I.ip);
-      hipLaunchKernelGGL(vhc::k_ipc_update, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, dl, c->pinned, I.ip);
-      hipLaunchKernelGGL(vhc::k_ipc_pd_latch, dim3(1), dim3(64), 0, c->stream, I.ip, c->st);
+    {
+      const hipError_t e = launch_al_pd(c, dt);
+      if (e != hipSuccess) return e;
     }
-    c->cur_ext = nullptr;
     // The AL update: detection at the PD result, multiplier update, decision.
     // After a pair overflow the last two do nothing and all three are redone
     // with a larger buffer; the pair count is read with the AL flag.
