@@ -186,6 +186,38 @@ def copy_probe(mb):
     del a, b
 
 
+def launch_floor_probe(nodes=200, reps=20):
+    """The launch floor of a graph replay: microseconds per node of a captured
+    HIP graph whose nodes are dependent one-element kernels (an in-place add
+    each), replayed reps times between HIP events. Every band launch of the
+    solve pays at least this. The tensor's final value is checked against the
+    count of adds, so a replay that skipped or repeated nodes would show."""
+    import torch
+    x = torch.zeros(1, dtype=torch.float64, device="cuda")
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        for _ in range(3):
+            x.add_(1.0)
+    torch.cuda.current_stream().wait_stream(side)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(nodes):
+            x.add_(1.0)
+    g.replay()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        g.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1000.0 / (reps * nodes)
+    # three eager adds, then nodes per replay (the capture itself runs nothing)
+    ok = float(x.item()) == float(3 + nodes * (reps + 1))
+    return dict(us_per_node=round(us, 3), nodes=nodes, replays=reps, value_check=ok)
+
+
 def traffic_from_profiles(prec):
     """HBM bytes per band launch measured with rocprofv3 FETCH_SIZE and
     WRITE_SIZE (profiles/band_kernels_256_traffic.md), or None."""
@@ -430,6 +462,8 @@ def main():
         sys.exit(2)
     if args.copy_probe_mb > 0:
         copy_probe(args.copy_probe_mb)
+    # torch's HIP runtime is initialised before the library's, as in copy_probe
+    floor = launch_floor_probe() if args.workload == "hanging_sheet" else None
     if args.workload == "batched_sheets":
         run_batch(args)
         return
@@ -491,6 +525,7 @@ def main():
                         "device_factor_fp64": round(f64["info"]["factor_seconds"], 4)},
             "final_residual": f32["final_residual"],
             "drop_in_host_state_steps_per_s": round(drop_in, 3),
+            "launch_floor": floor,
         },
     }
     if not args.no_cpu:
