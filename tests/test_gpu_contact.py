@@ -75,7 +75,9 @@ def _folded_sheet(c=8, gap=0.02, seed=0):
 
 def test_bvh_pairs_match_all_pairs():
     from oracle import ref
-    for c, margin, seed in ((12, 0.01, 0), (16, 0.05, 1), (10, 0.0, 2)):
+    # 64 and 100 cells: 8192 and 20000 triangles, several tiles of the
+    # Morton sort (2048 keys per tile) and of the pair-offset scan
+    for c, margin, seed in ((12, 0.01, 0), (16, 0.05, 1), (10, 0.0, 2), (64, 0.004, 3), (100, 0.002, 4)):
         px, py, pz, idx, hg = _mesh(c, c, 1.0, 1.0)
         rng = np.random.default_rng(seed)
         pz = (rng.normal(size=px.size) * 0.05).astype(np.float32)
@@ -502,8 +504,10 @@ def test_contact_steps_are_deterministic():
 
 @pytest.mark.parametrize("precision", ["f32_ir", "f64"])
 def test_contact_al_graph_replays_match_eager_launches(precision, monkeypatch):
-    """The PD loop of each AL iteration is captured once into a HIP graph and
-    replayed, with the detection's eager launches and host reads between the
+    """The head of a contact step (begin, first detection: boxes, Morton
+    sort, LBVH, pair walks and scans, self contacts) and each AL iteration
+    (PD loop, detection, multiplier update) are captured once into HIP graphs
+    and replayed, with host reads of the pair counts and AL flag between the
     replays. The same folded-sheet steps with graphs disabled
     (VISTIO_HIP_GRAPH=0: every kernel launched directly) give bitwise the
     same positions, multipliers and diagnostics at every step."""

@@ -23,13 +23,11 @@
 #include <type_traits>
 #include <vector>
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
-
 #include "vh_kernels.hpp"
 #include "vh_contact.hpp"
 #include "vh_factor.hpp"
 #include "vh_setup.hpp"
+#include "vh_sort.hpp"
 #include "vistio_hip.h"
 
 // Device buffers and settings of the contact step (vh_ipc_init).
@@ -53,8 +51,8 @@ struct IpcState {
   unsigned int* h_np = nullptr;  // pinned host copies of BvhDev::npairs and ::ncont (same allocation)
   unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
   float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
-  void* scan_tmp = nullptr;
-  size_t scan_bytes = 0;
+  unsigned int* sort_hist = nullptr;  // per-tile digit histograms of the Morton sort (vh_sort.hpp)
+  unsigned int* sbuf = nullptr;       // tile sums of every scan (the scans run one after another)
   int* orig = nullptr;  // caller's index of each device triangle
   int* sorig = nullptr;  // caller's index of each sorted leaf
   int2* pairs = nullptr;
@@ -65,8 +63,6 @@ struct IpcState {
   unsigned int* ccd_cnt = nullptr;  // survivors per block
   vhc::BvhDev* bd = nullptr;
   vhc::IpcDev* ip = nullptr;
-  void* sort_tmp = nullptr;
-  size_t sort_bytes = 0;
   double cfl_dh = 0;  // d_hat of the CFL velocity clamp (compliant-scaled when compliant_contact)
   // Self-contact gradients summed in a fixed order (vh_contact.hpp,
   // k_self_count .. k_vgather): per-survivor offsets (one region of ccd_reg
@@ -79,19 +75,19 @@ struct IpcState {
   unsigned int* vidx = nullptr;
   unsigned int *vcnt = nullptr, *vfill = nullptr, *vptr = nullptr;
   unsigned int ecap = 0;
-  void* vscan_tmp = nullptr;
-  size_t vscan_bytes = 0;
-  // First error a library routine (the rocPRIM sort and scan of the broad
-  // phase) returned while a launch sequence was enqueued; take_lib_err reads
-  // and clears it.
-  hipError_t lib_err = hipSuccess;
-  // The PD loop of one AL iteration (k_ipc_ext, then max_iterations x local
-  // step, solve, step fraction and update), captured once into a graph and
-  // replayed per AL iteration; recaptured when dt or a buffer the launches
+  // The launch sequences of the contact step, each captured once into a
+  // graph and replayed: [0] the head of a step (begin, first detection),
+  // [1] an AL iteration other than the last, [2] the last one (an AL
+  // iteration: the PD loop, then the detection at its result, the multiplier
+  // update and the decision). Recaptured when dt or a buffer the launches
   // name changes (gen counts the regrowths).
-  hipGraphExec_t al_graph = nullptr;
-  double al_graph_dt = 0;
-  unsigned int gen = 0, al_graph_gen = 0;
+  struct Seq {
+    hipGraphExec_t exec = nullptr;
+    double dt = 0;
+    unsigned int gen = 0;
+  };
+  Seq seq[3];
+  unsigned int gen = 0;
 };
 
 struct vh_ctx {
@@ -232,8 +228,10 @@ void drop_graph(vh_ctx* ctx) {
 }
 
 void drop_al_graph(vh_ctx* ctx) {
-  if (ctx->ipc.al_graph) (void)hipGraphExecDestroy(ctx->ipc.al_graph);
-  ctx->ipc.al_graph = nullptr;
+  for (auto& q : ctx->ipc.seq) {
+    if (q.exec) (void)hipGraphExecDestroy(q.exec);
+    q.exec = nullptr;
+  }
 }
 
 void free_all(vh_ctx* ctx) {
@@ -909,35 +907,24 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
                      I.bpart, skip);
   hipLaunchKernelGGL(vhc::k_scene_bounds, dim3(1), dim3(256), 0, c->stream, nblk(t), I.bpart, I.bd, skip);
   hipLaunchKernelGGL(vhc::k_morton, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.bd, I.code, I.id, skip);
-  size_t bytes = I.sort_bytes;
-  const hipError_t re = rocprim::radix_sort_pairs(I.sort_tmp, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)t,
-                                                  0u, 30u, c->stream);
-  if (re != hipSuccess && I.lib_err == hipSuccess) I.lib_err = re;
-  hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.id_s, I.st, skip);
+  // stable, so equal codes keep the triangle order
+  const int in_s = vhs::sort_pairs(c->stream, I.code, I.id, I.code_s, I.id_s, t, 30, I.sort_hist, I.sbuf, skip);
+  const unsigned int* code_s = in_s ? I.code_s : I.code;
+  const int* id_s = in_s ? I.id_s : I.id;
+  hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, id_s, I.st, skip);
   if (t < 2) return;
-  hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, I.code_s, I.left, I.range_lo,
+  hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, code_s, I.left, I.range_lo,
                      I.range_hi, skip);
   for (int k = 1; k < I.st_levels; ++k)
     hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
-  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.id_s, I.left,
+  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, id_s, I.left,
                      I.range_lo, I.range_hi, I.node, I.leaf, I.orig, I.sorig, skip);
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
   hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
-  size_t sbytes = I.scan_bytes;
-  const hipError_t se = rocprim::exclusive_scan(I.scan_tmp, sbytes, I.pcount, I.poff, 0u, (size_t)t,
-                                                rocprim::plus<unsigned int>(), c->stream);
-  if (se != hipSuccess && I.lib_err == hipSuccess) I.lib_err = se;
+  vhs::scan_exclusive(c->stream, I.pcount, I.poff, t, I.sbuf, skip);
   hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
-}
-
-// The first library error of the launches enqueued since the last call
-// (hipSuccess if none), then clears it.
-hipError_t take_lib_err(vh_ctx* c) {
-  const hipError_t e = c->ipc.lib_err;
-  c->ipc.lib_err = hipSuccess;
-  return e;
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
@@ -964,10 +951,7 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
     hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vcnt, skip);
     hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vfill, skip);
     hipLaunchKernelGGL(vhc::k_vcount, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vcnt, skip);
-    size_t vb = I.vscan_bytes;
-    const hipError_t se = rocprim::exclusive_scan(I.vscan_tmp, vb, I.vcnt, I.vptr, 0u, (size_t)n + 1,
-                                                  rocprim::plus<unsigned int>(), c->stream);
-    if (se != hipSuccess && I.lib_err == hipSuccess) I.lib_err = se;
+    vhs::scan_exclusive(c->stream, I.vcnt, I.vptr, (long long)n + 1, I.sbuf, skip);
     hipLaunchKernelGGL(vhc::k_vscatter, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vptr, I.vfill,
                        I.vidx, skip);
     hipLaunchKernelGGL(vhc::k_vgather, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.vptr, I.vidx, I.eg, I.gself,
@@ -1005,44 +989,47 @@ void enqueue_al_pd(vh_ctx* c, double dt) {
   c->cur_ext = nullptr;
 }
 
-// Enqueues enqueue_al_pd: replays its graph (captured on first use, and
-// again after dt or a pair buffer changed), or launches the kernels
-// directly when graphs are disabled. The sequence holds only kernels (no
-// memset or library call).
-hipError_t launch_al_pd(vh_ctx* c, double dt) {
+// Enqueues the launches of `enqueue` (kernels only: no memset or library
+// call) by replaying q's graph, captured on first use and again after dt or
+// a pair buffer changed; or launches them directly when graphs are disabled.
+template <class F>
+hipError_t launch_seq(vh_ctx* c, IpcState::Seq& q, double dt, F&& enqueue) {
   IpcState& I = c->ipc;
   if (!c->use_graph) {
-    enqueue_al_pd(c, dt);
+    enqueue();
     return hipGetLastError();
   }
-  if (!I.al_graph || I.al_graph_dt != dt || I.al_graph_gen != I.gen) {
-    drop_al_graph(c);
+  if (!q.exec || q.dt != dt || q.gen != I.gen) {
+    if (q.exec) (void)hipGraphExecDestroy(q.exec);
+    q.exec = nullptr;
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
-    enqueue_al_pd(c, dt);
+    enqueue();
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(c->stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
       if (g) (void)hipGraphDestroy(g);
       return le != hipSuccess ? le : ce;
     }
-    e = hipGraphInstantiate(&I.al_graph, g, nullptr, nullptr, 0);
+    e = hipGraphInstantiate(&q.exec, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) {
-      I.al_graph = nullptr;
+      q.exec = nullptr;
       return e;
     }
-    I.al_graph_dt = dt;
-    I.al_graph_gen = I.gen;
+    q.dt = dt;
+    q.gen = I.gen;
   }
-  return hipGraphLaunch(I.al_graph, c->stream);
+  return hipGraphLaunch(q.exec, c->stream);
 }
 
-// step_with_ipc (pd_solver.rs:549-1157) as one launch sequence: the PD loop
-// has a fixed trip count and stops through device flags; after each AL
-// iteration the host reads the AL flag and launches no further AL
-// iteration once it is set (the kernels of one would all be skipped).
+// step_with_ipc (pd_solver.rs:549-1157) as three replayed graphs: the head
+// of the step, then one per AL iteration. The PD loop has a fixed trip count
+// and stops through device flags; after each AL iteration the host reads the
+// AL flag (and, with self-collision, the pair and contact counts) and
+// launches no further AL iteration once it is set (the kernels of one would
+// all be skipped).
 hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   IpcState& I = c->ipc;
   const vh::Setup& S = c->S;
@@ -1051,36 +1038,40 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   const vh_ipc_params& Q = I.P;
   const int* skip = ipc_done_flag(c);
   const double max_vel = 0.5 * std::sqrt(I.cfl_dh) / dt;
-  hipLaunchKernelGGL(vhc::k_ipc_begin, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->v, c->prev, c->pred,
-                     c->pinned, I.ever, dt, P.gravity[0], P.gravity[1], P.gravity[2], max_vel, Q.has_ground,
-                     Q.ground_height, Q.al_mu_initial, I.ip, c->st);
-  enqueue_detect(c, c->x, I.sc, true, skip);
+  hipError_t e = launch_seq(c, I.seq[0], dt, [&] {
+    hipLaunchKernelGGL(vhc::k_ipc_begin, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->v, c->prev, c->pred,
+                       c->pinned, I.ever, dt, P.gravity[0], P.gravity[1], P.gravity[2], max_vel, Q.has_ground,
+                       Q.ground_height, Q.al_mu_initial, I.ip, c->st);
+    enqueue_detect(c, c->x, I.sc, true, skip);
+  });
+  if (e != hipSuccess) return e;
   if (I.sc.self_collision) {
-    hipError_t e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); }, true);
+    e = fit_pairs(c, [&] { enqueue_detect(c, c->x, I.sc, true, skip); }, true);
     if (e != hipSuccess) return e;
   }
   for (int al = 0; al < Q.al_max_iterations; ++al) {
-    {
-      const hipError_t e = launch_al_pd(c, dt);
-      if (e != hipSuccess) return e;
-    }
+    const bool last = al + 1 == Q.al_max_iterations;
     // The AL update: detection at the PD result, multiplier update, decision.
     // After a pair overflow the last two do nothing and all three are redone
     // with a larger buffer; the pair count is read with the AL flag.
     auto al_close = [&] {
       enqueue_detect(c, c->x, I.sc, false, skip);
       hipLaunchKernelGGL(vhc::k_ipc_al_end, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.grad, I.inc, I.lambda,
-                         Q.al_tolerance, al + 1 < Q.al_max_iterations ? 1 : 0, I.ip);
+                         Q.al_tolerance, last ? 0 : 1, I.ip);
       hipLaunchKernelGGL(vhc::k_ipc_al_commit, dim3(1), dim3(64), 0, c->stream, Q.al_tolerance, Q.al_mu_growth,
-                         al + 1 == Q.al_max_iterations ? 1 : 0, I.ip, c->st);
+                         last ? 1 : 0, I.ip, c->st);
     };
-    al_close();
+    e = launch_seq(c, I.seq[last ? 2 : 1], dt, [&] {
+      enqueue_al_pd(c, dt);
+      al_close();
+    });
+    if (e != hipSuccess) return e;
     if (I.sc.self_collision) {
-      hipError_t e = fit_pairs(c, al_close, true, skip);
+      e = fit_pairs(c, al_close, true, skip);
       if (e != hipSuccess) return e;
       if (*I.h_done) break;
-    } else if (al + 1 < Q.al_max_iterations) {
-      hipError_t e = hipMemcpyAsync(I.h_done, skip, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    } else if (!last) {
+      e = hipMemcpyAsync(I.h_done, skip, sizeof(int), hipMemcpyDeviceToHost, c->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
       if (e != hipSuccess) return e;
       if (*I.h_done) break;
@@ -1093,8 +1084,6 @@ hipError_t enqueue_ipc_step(vh_ctx* c, double dt) {
   f.ground = Q.ground_height; f.adaptive = Q.adaptive_contact_damping; f.has_ground = Q.has_ground;
   hipLaunchKernelGGL(vhc::k_ipc_finalize, dim3(nblk(n)), dim3(256), 0, c->stream, n, c->x, c->prev, c->v, c->pinned,
                      I.ever, I.fnrm, f, I.ip);
-  const hipError_t le = take_lib_err(c);
-  if (le != hipSuccess) return le;
   return hipGetLastError();
 }
 
@@ -1784,7 +1773,8 @@ vh_status vh_pd_set_ground(vh_ctx* ctx, int32_t has_ground, double ground_height
   if (on != ctx->P.has_ground || h != ctx->P.ground_height) {
     ctx->P.has_ground = on;
     ctx->P.ground_height = h;
-    drop_graph(ctx);  // the captured step holds the old value in its kernel arguments
+    drop_graph(ctx);  // the captured sequences hold the old value in their kernel arguments
+    drop_al_graph(ctx);
   }
   return VH_OK;
 }
@@ -2265,23 +2255,11 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   *I.h_done = 0;
   I.h_np = reinterpret_cast<unsigned int*>(I.h_done + 1);
   I.h_np[0] = I.h_np[1] = 0;
-  size_t bytes = 0;
-  VH_HIP(rocprim::radix_sort_pairs(nullptr, bytes, I.code, I.code_s, I.id, I.id_s, (unsigned int)tt, 0u, 30u,
-                                   ctx->stream));
-  I.sort_bytes = std::max<size_t>(bytes, 16);
   UP(dalloc(ctx, &I.pcount, (size_t)tt)); UP(dalloc(ctx, &I.poff, (size_t)tt));
   UP(dalloc(ctx, &I.bpart, 6 * (size_t)nblk(tt)));
-  bytes = 0;
-  VH_HIP(rocprim::exclusive_scan(nullptr, bytes, I.pcount, I.poff, 0u, (size_t)tt, rocprim::plus<unsigned int>(),
-                                 ctx->stream));
-  I.scan_bytes = std::max<size_t>(bytes, 16);
-  bytes = 0;
-  VH_HIP(rocprim::exclusive_scan(nullptr, bytes, I.vcnt, I.vptr, 0u, (size_t)n + 1, rocprim::plus<unsigned int>(),
-                                 ctx->stream));
-  I.vscan_bytes = std::max<size_t>(bytes, 16);
-  UP(dalloc_bytes(ctx, &I.sort_tmp, I.sort_bytes));
-  UP(dalloc_bytes(ctx, &I.scan_tmp, I.scan_bytes));
-  UP(dalloc_bytes(ctx, &I.vscan_tmp, I.vscan_bytes));
+  UP(dalloc(ctx, &I.sort_hist, (size_t)vhs::sort_hist_words(tt)));
+  UP(dalloc(ctx, &I.sbuf, (size_t)std::max({vhs::sort_scan_words(tt), vhs::scan_scratch_words(tt),
+                                            vhs::scan_scratch_words((long long)n + 1)})));
 #undef UP
   VH_HIP(hipMemset(I.gself, 0, n3 * 8));
   VH_HIP(hipMemset(I.lambda, 0, n3 * 8));
@@ -2362,7 +2340,6 @@ vh_status vh_ipc_detect(vh_ctx* ctx, const double* pos, double d_hat, double kap
   detect();
   VH_HIP(hipGetLastError());
   if (sc.self_collision) VH_HIP(fit_pairs(ctx, detect, true));
-  VH_HIP(take_lib_err(ctx));
   const size_t n3 = 3 * (size_t)n;
   std::vector<double> g(n3), nr(n3), hd(n);
   std::vector<uint8_t> ic(n);
@@ -2409,7 +2386,6 @@ vh_status vh_ipc_ccd_step(vh_ctx* ctx, const double* p0, const double* p1, doubl
   ccd();
   VH_HIP(hipGetLastError());
   if (I.sc.self_collision) VH_HIP(fit_pairs(ctx, ccd, false));
-  VH_HIP(take_lib_err(ctx));
   vhc::IpcDev h{};
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&h, I.ip, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
@@ -2443,7 +2419,6 @@ vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count) {
   cross();
   VH_HIP(hipGetLastError());
   VH_HIP(fit_pairs(ctx, cross, false));
-  VH_HIP(take_lib_err(ctx));
   vhc::BvhDev h{};
   VH_HIP(hipMemcpyAsync(&h, I.bd, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
@@ -2467,7 +2442,6 @@ vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* 
   enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr);
   VH_HIP(hipGetLastError());
   VH_HIP(fit_pairs(ctx, [&] { enqueue_pairs(ctx, I.scratch_a, (float)margin, nullptr); }, false));
-  VH_HIP(take_lib_err(ctx));
   unsigned int np = 0;
   VH_HIP(hipMemcpyAsync(&np, &I.bd->npairs, sizeof(np), hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
