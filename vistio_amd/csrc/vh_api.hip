@@ -53,6 +53,7 @@ struct IpcState {
   float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
   unsigned int* sort_hist = nullptr;  // per-tile digit histograms of the Morton sort (vh_sort.hpp)
   unsigned int* sbuf = nullptr;       // tile sums of every scan (the scans run one after another)
+  unsigned int* ticket = nullptr;     // the scans' finishing-block counter (0 between launches)
   int* orig = nullptr;  // caller's index of each device triangle
   int* sorig = nullptr;  // caller's index of each sorted leaf
   int2* pairs = nullptr;
@@ -908,7 +909,7 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   hipLaunchKernelGGL(vhc::k_scene_bounds, dim3(1), dim3(256), 0, c->stream, nblk(t), I.bpart, I.bd, skip);
   hipLaunchKernelGGL(vhc::k_morton, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, I.bd, I.code, I.id, skip);
   // stable, so equal codes keep the triangle order
-  const int in_s = vhs::sort_pairs(c->stream, I.code, I.id, I.code_s, I.id_s, t, 30, I.sort_hist, I.sbuf, skip);
+  const int in_s = vhs::sort_pairs(c->stream, I.code, I.id, I.code_s, I.id_s, t, 30, I.sort_hist, skip);
   const unsigned int* code_s = in_s ? I.code_s : I.code;
   const int* id_s = in_s ? I.id_s : I.id;
   hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, id_s, I.st, skip);
@@ -922,7 +923,7 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
                      I.range_lo, I.range_hi, I.node, I.leaf, I.orig, I.sorig, skip);
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
   hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
-  vhs::scan_exclusive(c->stream, I.pcount, I.poff, t, I.sbuf, skip);
+  vhs::scan_exclusive(c->stream, I.pcount, I.poff, t, I.sbuf, I.ticket, skip);
   hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
 }
@@ -951,7 +952,7 @@ void enqueue_detect(vh_ctx* c, const double* x, const vhc::Scene& sc, bool reset
     hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vcnt, skip);
     hipLaunchKernelGGL(vhc::k_fill_u, dim3(nblk(n + 1)), dim3(256), 0, c->stream, n + 1, I.vfill, skip);
     hipLaunchKernelGGL(vhc::k_vcount, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vcnt, skip);
-    vhs::scan_exclusive(c->stream, I.vcnt, I.vptr, (long long)n + 1, I.sbuf, skip);
+    vhs::scan_exclusive(c->stream, I.vcnt, I.vptr, (long long)n + 1, I.sbuf, I.ticket, skip);
     hipLaunchKernelGGL(vhc::k_vscatter, dim3(1024), dim3(256), 0, c->stream, I.bd, I.ecap, I.ev, I.vptr, I.vfill,
                        I.vidx, skip);
     hipLaunchKernelGGL(vhc::k_vgather, dim3(nblk(n)), dim3(256), 0, c->stream, n, I.vptr, I.vidx, I.eg, I.gself,
@@ -2258,13 +2259,14 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.pcount, (size_t)tt)); UP(dalloc(ctx, &I.poff, (size_t)tt));
   UP(dalloc(ctx, &I.bpart, 6 * (size_t)nblk(tt)));
   UP(dalloc(ctx, &I.sort_hist, (size_t)vhs::sort_hist_words(tt)));
-  UP(dalloc(ctx, &I.sbuf, (size_t)std::max({vhs::sort_scan_words(tt), vhs::scan_scratch_words(tt),
-                                            vhs::scan_scratch_words((long long)n + 1)})));
+  UP(dalloc(ctx, &I.sbuf, (size_t)std::max(vhs::scan_scratch_words(tt), vhs::scan_scratch_words((long long)n + 1))));
+  UP(dalloc(ctx, &I.ticket, 1));
 #undef UP
   VH_HIP(hipMemset(I.gself, 0, n3 * 8));
   VH_HIP(hipMemset(I.lambda, 0, n3 * 8));
   VH_HIP(hipMemset(I.fnrm, 0, n3 * 8));
   VH_HIP(hipMemset(I.ever, 0, (size_t)n));
+  VH_HIP(hipMemset(I.ticket, 0, sizeof(unsigned int)));
   VH_HIP(hipMemset(I.bd, 0, sizeof(vhc::BvhDev)));
   VH_HIP(hipMemset(I.ip, 0, sizeof(vhc::IpcDev)));
   I.on = true;

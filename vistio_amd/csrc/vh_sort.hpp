@@ -10,12 +10,12 @@
 // graph. All arithmetic is on integers: the results do not depend on the
 // order in which blocks run.
 //
-// Scan: reduce (one sum per tile of 2048), spine (one block scans the tile
-// sums with a running carry), down-sweep (each tile scans itself from its
-// base). Sort: per pass of 8 bits, a digit histogram per tile, the scan of
-// the digit-major histogram (tile b's first slot for digit d), and a scatter
-// in which each item's rank among the equal digits of its tile comes from
-// wave ballots, in item order, so the sort is stable.
+// Scan: reduce (one sum per tile of 2048; the last block to finish, which it
+// learns from an atomic ticket, scans the tile sums), then a down-sweep (each
+// tile scans itself from its base). Sort: per pass of 8 bits, a digit
+// histogram per tile, one block that turns them into each tile's first slot
+// per digit, and a scatter in which each item's rank among the equal digits
+// of its tile comes from wave ballots, in item order, so the sort is stable.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -54,11 +54,16 @@ __device__ __forceinline__ unsigned int block_exclusive(unsigned int v, unsigned
   return base + inc - v;
 }
 
-// bsum[b] = sum of in[b * kTile .. (b + 1) * kTile).
+// bsum[b] = sum of in[b * kTile .. (b + 1) * kTile). The last block to
+// finish then replaces bsum[0..nb) by its exclusive scan and re-arms the
+// ticket (a device word that is 0 between launches). No block waits for
+// another: each writes its sum, fences, and takes a ticket.
 __global__ void __launch_bounds__(kThreads) k_scan_reduce(const unsigned int* __restrict__ in, long long len,
-                                                          unsigned int* __restrict__ bsum, const int* skip) {
+                                                          unsigned int* bsum, long long nb, unsigned int* ticket,
+                                                          const int* skip) {
   if (skip && *skip) return;
   __shared__ unsigned int wsum[4];
+  __shared__ int last;
   const long long b0 = (long long)blockIdx.x * kTile;
   unsigned int s = 0;
 #pragma unroll
@@ -70,26 +75,25 @@ __global__ void __launch_bounds__(kThreads) k_scan_reduce(const unsigned int* __
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
   if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-}
-
-// In place: bsum[0..nb) -> its exclusive scan (one block, chunks of kTile
-// with a running carry).
-__global__ void __launch_bounds__(kThreads) k_scan_spine(unsigned int* __restrict__ bsum, long long nb,
-                                                         const int* skip) {
-  if (skip && *skip) return;
-  __shared__ unsigned int wsum[4];
+  if (threadIdx.x == 0) {
+    bsum[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __threadfence();  // the sum is visible on every XCD before the ticket is taken
+    last = atomicAdd(ticket, 1u) == (unsigned int)(nb - 1);
+  }
+  __syncthreads();
+  if (!last) return;
+  __threadfence();  // every block's sum was fenced before its ticket
   unsigned int carry = 0;
   for (long long c0 = 0; c0 < nb; c0 += kTile) {
     const long long i0 = c0 + (long long)threadIdx.x * kPer;
-    unsigned int v[kPer], s = 0;
+    unsigned int v[kPer], t = 0;
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       v[k] = i0 + k < nb ? bsum[i0 + k] : 0u;
-      s += v[k];
+      t += v[k];
     }
     unsigned int tot;
-    unsigned int p = carry + block_exclusive(s, wsum, &tot);
+    unsigned int p = carry + block_exclusive(t, wsum, &tot);
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       if (i0 + k < nb) bsum[i0 + k] = p;
@@ -97,6 +101,7 @@ __global__ void __launch_bounds__(kThreads) k_scan_spine(unsigned int* __restric
     }
     carry += tot;
   }
+  if (threadIdx.x == 0) atomicExch(ticket, 0u);
 }
 
 // out[i] = base of i's tile (bsum[tile], or 0 without bsum) + the sum of the
@@ -123,71 +128,138 @@ __global__ void __launch_bounds__(kThreads) k_scan_down(const unsigned int* in, 
   }
 }
 
-// Words of scan_scratch that scan_exclusive needs for len items.
+// Words of scan scratch (bsum) that scan_exclusive needs for len items.
 inline long long scan_scratch_words(long long len) { return tiles(len) > 1 ? tiles(len) : 1; }
 
 // out[0..len) = exclusive scan of in[0..len) (out may alias in). bsum:
-// scan_scratch_words(len) words of device memory.
+// scan_scratch_words(len) words of device memory; ticket: a device word that
+// holds 0 (the reduce leaves it 0).
 inline void scan_exclusive(hipStream_t s, const unsigned int* in, unsigned int* out, long long len,
-                           unsigned int* bsum, const int* skip) {
+                           unsigned int* bsum, unsigned int* ticket, const int* skip) {
   if (len <= 0) return;
   const long long nb = tiles(len);
   if (nb == 1) {
     hipLaunchKernelGGL(k_scan_down, dim3(1), dim3(kThreads), 0, s, in, len, (const unsigned int*)nullptr, out, skip);
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned int)nb), dim3(kThreads), 0, s, in, len, bsum, skip);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(kThreads), 0, s, bsum, nb, skip);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned int)nb), dim3(kThreads), 0, s, in, len, bsum, nb, ticket, skip);
   hipLaunchKernelGGL(k_scan_down, dim3((unsigned int)nb), dim3(kThreads), 0, s, in, len, bsum, out, skip);
 }
 
-// hist[d * nb + b] = items of tile b whose digit (key >> shift) & 255 is d.
+// hist[b * 256 + d] = items of tile b whose digit (key >> shift) & 255 is d
+// (tile-major, so that this write and every later access are coalesced).
 __global__ void __launch_bounds__(kThreads) k_rs_hist(const unsigned int* __restrict__ key, long long n, int shift,
-                                                      unsigned int* __restrict__ hist, long long nb,
-                                                      const int* skip) {
+                                                      unsigned int* __restrict__ hist, const int* skip) {
   if (skip && *skip) return;
   __shared__ unsigned int cnt[kBins];
   cnt[threadIdx.x] = 0u;
-  __syncthreads();
   const long long b0 = (long long)blockIdx.x * kTile;
+  unsigned int k8[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const long long i = b0 + k * kThreads + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(key[i] >> shift) & (kBins - 1)], 1u);
+    k8[k] = i < n ? (key[i] >> shift) & (kBins - 1) : kBins;
   }
   __syncthreads();
-  hist[(long long)threadIdx.x * nb + blockIdx.x] = cnt[threadIdx.x];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k)
+    if (k8[k] < kBins) atomicAdd(&cnt[k8[k]], 1u);
+  __syncthreads();
+  hist[(long long)blockIdx.x * kBins + threadIdx.x] = cnt[threadIdx.x];
 }
 
-// One stable pass: item i of tile b with digit d goes to off[d * nb + b] +
-// (items of tile b before i with digit d). The tile is taken in 8 rounds of
-// 256 consecutive items; within a round, a wave finds the lanes that share
-// its digit with 8 ballots, and the waves' per-digit counts order the waves.
+constexpr int kSpineThreads = 1024;  // 4 threads per digit
+constexpr int kSpineBatch = 32;      // tiles a thread loads at once
+
+// In place: hist[b * 256 + d] -> the first output slot of tile b's items of
+// digit d (all items of smaller digits, then those of digit d in tiles before
+// b). One block; thread (d, q) takes the q-th quarter of the tiles for digit
+// d, so each wave's loads cover 64 consecutive words.
+__global__ void __launch_bounds__(kSpineThreads) k_rs_spine(unsigned int* __restrict__ hist, long long nb,
+                                                            const int* skip) {
+  if (skip && *skip) return;
+  __shared__ unsigned int part[4][kBins];
+  __shared__ unsigned int wsum[4];
+  const int d = threadIdx.x & (kBins - 1), q = threadIdx.x >> 8, lane = threadIdx.x & 63;
+  const long long per = (nb + 3) / 4, lo = q * per, hi = lo + per < nb ? lo + per : nb;
+  unsigned int s = 0;
+  for (long long b0 = lo; b0 < hi; b0 += kSpineBatch) {
+    unsigned int v[kSpineBatch];
+#pragma unroll
+    for (int i = 0; i < kSpineBatch; ++i) v[i] = b0 + i < hi ? hist[(b0 + i) * kBins + d] : 0u;
+#pragma unroll
+    for (int i = 0; i < kSpineBatch; ++i) s += v[i];
+  }
+  part[q][d] = s;
+  __syncthreads();
+  unsigned int pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const unsigned int u = part[k][d];
+    if (k < q) pre += u;
+    tot += u;
+  }
+  // exclusive scan of the digit totals: every wave scans its 64 digits (the
+  // four quarters get the same values); the first quarter publishes the sums
+  unsigned int inc = tot;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned int u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (q == 0 && lane == 63) wsum[d >> 6] = inc;
+  __syncthreads();
+  unsigned int run = inc - tot + pre;
+  for (int k = 0; k < (d >> 6); ++k) run += wsum[k];
+  for (long long b0 = lo; b0 < hi; b0 += kSpineBatch) {
+    unsigned int v[kSpineBatch];
+#pragma unroll
+    for (int i = 0; i < kSpineBatch; ++i) v[i] = b0 + i < hi ? hist[(b0 + i) * kBins + d] : 0u;
+#pragma unroll
+    for (int i = 0; i < kSpineBatch; ++i) {
+      if (b0 + i < hi) hist[(b0 + i) * kBins + d] = run;
+      run += v[i];
+    }
+  }
+}
+
+// One stable pass: item i of tile b with digit d goes to off[b * 256 + d] +
+// (items of tile b before i with digit d). The tile's 8 rounds of 256
+// consecutive items are loaded first; within a round, a wave finds the lanes
+// that share its digit with 8 ballots, and the waves' per-digit counts order
+// the waves. The counts alternate between two buffers, so a round needs two
+// barriers: one before the counts are read, one before they are folded into
+// the digit bases and cleared.
 __global__ void __launch_bounds__(kThreads) k_rs_scatter(const unsigned int* __restrict__ kin,
                                                          const int* __restrict__ vin, unsigned int* __restrict__ kout,
                                                          int* __restrict__ vout, long long n, int shift,
-                                                         const unsigned int* __restrict__ off, long long nb,
-                                                         const int* skip) {
+                                                         const unsigned int* __restrict__ off, const int* skip) {
   if (skip && *skip) return;
-  __shared__ unsigned int base[kBins];     // next output slot of each digit from this tile
-  __shared__ unsigned int wcnt[4][kBins];  // items of each digit per wave in the current round
+  __shared__ unsigned int base[kBins];        // next output slot of each digit from this tile
+  __shared__ unsigned int wcnt[2][4][kBins];  // items of each digit per wave, by round parity
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  base[tid] = off[(long long)tid * nb + blockIdx.x];
-  const unsigned long long lower = (1ull << lane) - 1ull;
   const long long b0 = (long long)blockIdx.x * kTile;
-  for (int r = 0; r < kPer; ++r) {
+  unsigned int key[kPer];
+  int val[kPer];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) wcnt[k][tid] = 0u;
-    __syncthreads();
+  for (int r = 0; r < kPer; ++r) {
     const long long i = b0 + r * kThreads + tid;
-    const bool valid = i < n;
-    unsigned int key = 0u;
-    int val = 0;
-    if (valid) {
-      key = kin[i];
-      val = vin[i];
-    }
-    const unsigned int d = (key >> shift) & (kBins - 1);
+    key[r] = i < n ? kin[i] : 0u;
+    val[r] = i < n ? vin[i] : 0;
+  }
+  base[tid] = off[(long long)blockIdx.x * kBins + tid];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    wcnt[0][k][tid] = 0u;
+    wcnt[1][k][tid] = 0u;
+  }
+  const unsigned long long lower = (1ull << lane) - 1ull;
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kPer; ++r) {
+    unsigned int (*wc)[kBins] = wcnt[r & 1];
+    const bool valid = b0 + r * kThreads + tid < n;
+    const unsigned int d = (key[r] >> shift) & (kBins - 1);
     unsigned long long peers = __ballot(valid);
 #pragma unroll
     for (int bit = 0; bit < 8; ++bit) {
@@ -196,29 +268,32 @@ __global__ void __launch_bounds__(kThreads) k_rs_scatter(const unsigned int* __r
       peers &= on ? m : ~m;
     }
     const unsigned int rank = (unsigned int)__popcll(peers & lower);
-    if (valid && rank == 0u) wcnt[w][d] = (unsigned int)__popcll(peers);
+    if (valid && rank == 0u) wc[w][d] = (unsigned int)__popcll(peers);
     __syncthreads();
     if (valid) {
       unsigned int pos = base[d] + rank;
-      for (int k = 0; k < w; ++k) pos += wcnt[k][d];
-      kout[pos] = key;
-      vout[pos] = val;
+      for (int k = 0; k < w; ++k) pos += wc[k][d];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
     }
     __syncthreads();
-    base[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    // only thread tid touches column tid here; the other buffer is in use
+    // by the next round, whose writes follow the next barrier
+    base[tid] += wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wc[k][tid] = 0u;
   }
 }
 
-// Words of hist/scan scratch that sort_pairs needs for n items.
+// Words of the per-tile histograms that sort_pairs needs for n items.
 inline long long sort_hist_words(long long n) { return (long long)kBins * (tiles(n) > 0 ? tiles(n) : 1); }
-inline long long sort_scan_words(long long n) { return scan_scratch_words(sort_hist_words(n)); }
 
 // Stable sort of (key, val) by the low `bits` bits of key, in passes of 8
 // bits. The pairs start in (k0, v0); with an even pass count (bits 25..32)
 // they end there, with an odd one in (k1, v1): the return value says which
-// (0 or 1). hist: sort_hist_words(n) words; bsum: sort_scan_words(n) words.
+// (0 or 1). hist: sort_hist_words(n) words.
 inline int sort_pairs(hipStream_t s, unsigned int* k0, int* v0, unsigned int* k1, int* v1, long long n, int bits,
-                      unsigned int* hist, unsigned int* bsum, const int* skip) {
+                      unsigned int* hist, const int* skip) {
   if (n <= 1) return 0;
   const long long nb = tiles(n);
   int cur = 0;
@@ -227,10 +302,10 @@ inline int sort_pairs(hipStream_t s, unsigned int* k0, int* v0, unsigned int* k1
     int* vi = cur ? v1 : v0;
     unsigned int* ko = cur ? k0 : k1;
     int* vo = cur ? v0 : v1;
-    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned int)nb), dim3(kThreads), 0, s, ki, n, shift, hist, nb, skip);
-    scan_exclusive(s, hist, hist, (long long)kBins * nb, bsum, skip);
+    hipLaunchKernelGGL(k_rs_hist, dim3((unsigned int)nb), dim3(kThreads), 0, s, ki, n, shift, hist, skip);
+    hipLaunchKernelGGL(k_rs_spine, dim3(1), dim3(kSpineThreads), 0, s, hist, nb, skip);
     hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned int)nb), dim3(kThreads), 0, s, ki, vi, ko, vo, n, shift,
-                       (const unsigned int*)hist, nb, skip);
+                       (const unsigned int*)hist, skip);
     cur ^= 1;
   }
   return cur;
