@@ -144,3 +144,80 @@ extern "C" int emul_factor(int n, const float* px, const float* py, const float*
   std::copy(Srv.begin(), Srv.end(), Sr);
   return 0;
 }
+
+// The local step's RHS slot plan (vh::Setup::lv_ptr .. lc_pos), replayed as
+// k_local uses it: every corner of every chunk is summed exactly once, into
+// a slot of its own vertex; every slot is written by exactly one (chunk,
+// vertex); a vertex's slots follow chunk order; a local vertex's corners are
+// in ascending place in the block. Corner (element e, k) carries the value
+// 1 + (index of the corner in 3t + 4h) * 0.5 (exact in double), so the plan's
+// per-vertex sums equal the direct per-corner sums exactly. stats[0] = slots,
+// stats[1] = corners, stats[2] = most local vertices of one chunk. Returns 0,
+// or a negative code naming the first broken property.
+extern "C" int emul_local_plan(int n, const float* px, const float* py, const float* pz, int t, const uint32_t* tri,
+                               int h, const uint32_t* hinge, const uint8_t* pinned, int leaf, long long* stats) {
+  vh::SetupInput in;
+  in.n = n; in.t = t; in.h = h; in.px = px; in.py = py; in.pz = pz; in.tri = tri; in.hinge = hinge;
+  in.pinned = pinned; in.k_membrane = 1e3; in.k_bending = 1e-4; in.density = 0.15;
+  in.dt_matrix = 1.0 / 60.0; in.leaf_size = leaf;
+  vh::Setup S;
+  if (!vh::build_setup(in, S)) return S.status ? S.status : -1;
+  const int tb = (t + 255) / 256, hb = (h + 255) / 256, nch = tb + hb;
+  if ((int)S.lv_ptr.size() != nch + 1 || S.lv_ptr[0] != 0) return -2;
+  const int64_t nslot = S.inc_ptr[n];
+  if ((int64_t)S.lv_slot.size() != S.lv_ptr[nch] || nslot != S.lv_ptr[nch]) return -3;
+  std::vector<int> owner(nslot, -1);  // vertex of each slot (from inc_ptr)
+  for (int v = 0; v < n; ++v)
+    for (int s = S.inc_ptr[v]; s < S.inc_ptr[v + 1]; ++s) owner[s] = v;
+  std::vector<int> slot_chunk(nslot, -1);
+  std::vector<double> got(nslot, 0.0), want_v(n, 0.0), got_v(n, 0.0);
+  long long most = 0;
+  for (int c = 0; c < nch; ++c) {
+    const bool tr = c < tb;
+    const int K = tr ? 3 : 4;
+    const int64_t e0 = (int64_t)(tr ? c : c - tb) * 256;
+    const int64_t ne = std::min<int64_t>(256, (tr ? t : h) - e0);
+    const int32_t* idx = tr ? S.tri_idx.data() : S.hinge_idx.data();
+    const size_t cb = tr ? 3 * (size_t)e0 : 3 * (size_t)t + 4 * (size_t)e0;
+    std::vector<int> seen((size_t)ne * K, 0);
+    int prev_end = 0;
+    most = std::max<long long>(most, S.lv_ptr[c + 1] - S.lv_ptr[c]);
+    for (int j = S.lv_ptr[c]; j < S.lv_ptr[c + 1]; ++j) {
+      const int a = prev_end, b = S.lv_end[j];
+      if (b <= a) return -4;
+      prev_end = b;
+      const int s = S.lv_slot[j];
+      if (s < 0 || s >= nslot || slot_chunk[s] != -1) return -5;
+      slot_chunk[s] = c;
+      const int v = owner[s];
+      int last = -1;
+      for (int q = a; q < b; ++q) {
+        const int p = S.lc_pos[cb + q];
+        if (p <= last || p >= ne * K) return -6;
+        last = p;
+        seen[p]++;
+        const size_t corner = cb + (size_t)p;  // == e0 * K + p
+        if (idx[(size_t)e0 * K + p] != v) return -7;
+        got[s] += 1.0 + 0.5 * (double)corner;
+      }
+    }
+    if (prev_end != ne * K) return -8;
+    for (int x : seen)
+      if (x != 1) return -9;
+  }
+  for (int v = 0; v < n; ++v) {
+    for (int s = S.inc_ptr[v]; s < S.inc_ptr[v + 1]; ++s) {
+      if (slot_chunk[s] < 0) return -10;
+      if (s > S.inc_ptr[v] && slot_chunk[s] <= slot_chunk[s - 1]) return -11;
+      got_v[v] += got[s];
+    }
+  }
+  for (int64_t k = 0; k < 3 * (int64_t)t; ++k) want_v[S.tri_idx[k]] += 1.0 + 0.5 * (double)k;
+  for (int64_t k = 0; k < 4 * (int64_t)h; ++k) want_v[S.hinge_idx[k]] += 1.0 + 0.5 * (double)(3 * (int64_t)t + k);
+  for (int v = 0; v < n; ++v)
+    if (got_v[v] != want_v[v]) return -12;
+  stats[0] = nslot;
+  stats[1] = 3 * (long long)t + 4 * (long long)h;
+  stats[2] = most;
+  return 0;
+}

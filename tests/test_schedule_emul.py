@@ -85,3 +85,26 @@ def test_device_index_limits(emul):
         args = [1000] * 4
         args[k] = lim + 1
         assert f(*args) == 0, k
+
+
+@pytest.mark.parametrize("c,leaf", [(1, 16), (7, 16), (20, 16), (64, 16), (64, 64), (130, 16)])
+def test_local_slot_plan(emul, c, leaf):
+    """The local step sums each block's corner contributions per vertex and
+    writes one RHS slot per (block, vertex): replayed on the CPU, the plan
+    sums every corner once into a slot of its own vertex, writes every slot
+    once, and gives each vertex the same total as the per-corner sums. The
+    sizes cover a chunk that is a single element, partial last chunks
+    (20x20: 800 triangles, 1160 hinges) and many chunks."""
+    px, py, pz, idx = mesh.quad_grid(c, c, 1.0, 1.0)
+    hg = np.ascontiguousarray(mesh.hinges4(idx, px.size)).ravel()
+    pin = mesh.pinned_top_row(c, c)
+    st = np.zeros(3, dtype=np.int64)
+    f = emul.emul_local_plan
+    rc = f(px.size, _p(px, C.c_float), _p(py, C.c_float), _p(pz, C.c_float), idx.size // 3, _p(idx, C.c_uint32),
+           hg.size // 4, _p(hg, C.c_uint32), _p(pin, C.c_uint8), leaf, _p(st, C.c_longlong))
+    assert rc == 0, f"slot plan check failed with code {rc}"
+    slots, corners, most = st.tolist()
+    print(f"{c}x{c}: {slots} slots for {corners} corners, at most {most} vertices per block", flush=True)
+    assert slots <= corners
+    if c >= 64:
+        assert slots * 3 < corners  # per-block sums pay off on a sheet of several blocks

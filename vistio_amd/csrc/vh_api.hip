@@ -107,7 +107,9 @@ struct vh_ctx {
   double* mass = nullptr;
   uint8_t* pinned = nullptr;
   // elements
-  int *tri_idx = nullptr, *tri_slot = nullptr, *hinge_idx = nullptr, *hinge_slot = nullptr;
+  int *tri_idx = nullptr, *hinge_idx = nullptr;
+  int *lv_ptr = nullptr, *lv_slot = nullptr;              // the local step's RHS slot plan (vh::Setup)
+  unsigned short *lv_end = nullptr, *lc_pos = nullptr;
   vhk::LocalMap local_map{};  // k_local's block -> element chunk map
   int local_blocks = 0;
   double* tri_dm = nullptr;
@@ -561,7 +563,8 @@ void enqueue_local(vh_ctx* c, const vhk::StepDev* st) {
   const vh::Setup& S = c->S;
   if (c->local_blocks > 0)
     hipLaunchKernelGGL(vhk::k_local<T>, dim3(c->local_blocks), dim3(256), 0, c->stream, S.t, S.h, c->local_map,
-                       elem_args<T>(c), c->tri_slot, c->hinge_slot, (T*)c->cslot, st);
+                       elem_args<T>(c), vhk::LocalPlan{c->lv_ptr, c->lv_slot, c->lv_end, c->lc_pos}, (T*)c->cslot,
+                       st);
 }
 
 // The local step of one PD iteration (k_local into the slots). Returns true:
@@ -1457,7 +1460,8 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->mass, S.mass));
   UP(upload(ctx, &ctx->pinned, S.pinned));
   UP(upload(ctx, &ctx->tri_idx, S.tri_idx));
-  UP(upload(ctx, &ctx->tri_slot, S.tri_slot));
+  UP(upload(ctx, &ctx->lv_ptr, S.lv_ptr)); UP(upload(ctx, &ctx->lv_slot, S.lv_slot));
+  UP(upload(ctx, &ctx->lv_end, S.lv_end)); UP(upload(ctx, &ctx->lc_pos, S.lc_pos));
   {
     // k_local's block order (vhk::local_chunk): per XCD, one eighth of the
     // triangle chunks and one eighth of the hinge chunks.
@@ -1469,7 +1473,6 @@ vh_status vh_pd_init_ex(vh_ctx* ctx, uint32_t n, const float* px, const float* p
   UP(upload(ctx, &ctx->tri_dm, S.tri_dm));
   UP(upload_t(ctx, &ctx->tri_w, S.tri_w));
   UP(upload(ctx, &ctx->hinge_idx, S.hinge_idx));
-  UP(upload(ctx, &ctx->hinge_slot, S.hinge_slot));
   UP(upload_t(ctx, &ctx->hinge_rest, S.hinge_rest));
   UP(upload_t(ctx, &ctx->hinge_w, S.hinge_w));
   ctx->hinge_st = nullptr;
@@ -2019,7 +2022,10 @@ vh_status vh_pd_profile(vh_ctx* ctx, double dt, uint32_t iterations, vh_profile*
   const double ts = (double)ctx->tsz;
   // Algorithmic bytes per launch.
   const double n = S.n, t = S.t, h = S.h;
-  out->local_bytes = n * 24 + t * (12 + 32 + ts + 12 + 9 * ts) + h * (16 + 2 * ts + 16 + 12 * ts);
+  // k_local: positions once, element data, the slot plan (6 bytes per slot
+  // and 2 per corner), one slot written per (block, vertex)
+  const double lslots = (double)S.inc_ptr[S.n];
+  out->local_bytes = n * 24 + t * (12 + 32 + ts + 6) + h * (16 + 2 * ts + 8) + lslots * (6 + 3 * ts);
   std::vector<double> band_entries(nb, 0), band_rows(nb, 0), band_cols(nb, 0);
   for (const vh::Supernode& q : S.sn) {
     band_entries[q.depth] += q.parent < 0 ? (double)q.w * q.w : (double)q.w * (q.w + 1) / 2 + (double)q.r * q.w;

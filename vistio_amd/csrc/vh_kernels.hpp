@@ -425,40 +425,96 @@ __device__ __forceinline__ int local_chunk(const LocalMap& m, int b, int tb, int
   return c < hb ? ~c : kIdleChunk;
 }
 
+// The RHS slots of each local-step block (see vh::Setup::lv_ptr): chunk c's
+// local vertices [lv_ptr[c], lv_ptr[c + 1]), local vertex j's slot lv_slot[j]
+// and the end lv_end[j] of its corners in lc_pos (from the chunk's first
+// corner), each corner given by its place in the block (element * K + k).
+struct LocalPlan {
+  const int* lv_ptr;
+  const int* lv_slot;
+  const unsigned short* lv_end;
+  const unsigned short* lc_pos;
+};
+
+// One block per chunk of 256 triangles or hinges: each thread computes its
+// element's corner contributions into LDS; then each local vertex's corners
+// are summed in a fixed order (ascending place in the block) and written to
+// its slot, one slot per (block, vertex) instead of one per corner. The
+// plan entries of a thread's first local vertex are loaded with the
+// element's indices.
 template <class T>
-__global__ void __launch_bounds__(256) k_local(int t, int h, LocalMap lm, ElemArgs<T> ea,
-                                               const int* __restrict__ tri_slot, const int* __restrict__ hinge_slot,
+__global__ void __launch_bounds__(256) k_local(int t, int h, LocalMap lm, ElemArgs<T> ea, LocalPlan lp,
                                                T* __restrict__ cslot, const StepDev* st) {
   if (st && st->done) return;
-  const int ch = local_chunk(lm, (int)blockIdx.x, (t + 255) / 256, (h + 255) / 256);
-  if (ch == kIdleChunk) return;
-  if (ch >= 0) {
-    const int e = ch * blockDim.x + threadIdx.x;
-    if (e >= t) return;
-    // the output slots are loaded with the element's indices, not after it
-    int sl[3];
+  const int tb = (t + 255) / 256;
+  const int ch = local_chunk(lm, (int)blockIdx.x, tb, (h + 255) / 256);
+  if (ch == kIdleChunk) return;  // uniform over the block
+  __shared__ T cv[256 * 4 * 3];
+  const bool is_tri = ch >= 0;
+  const int c = is_tri ? ch : tb + ~ch;
+  const int lv0 = lp.lv_ptr[c], lv1 = lp.lv_ptr[c + 1];
+  const size_t cbase = is_tri ? (size_t)3 * 256 * ch : (size_t)3 * t + (size_t)4 * 256 * (~ch);
+  const unsigned short* pos = lp.lc_pos + cbase;
+  constexpr int kPre = 8;
+  const int j0 = lv0 + (int)threadIdx.x;
+  int sl0 = 0, a0 = 0, b0 = 0;
+  if (j0 < lv1) {
+    sl0 = lp.lv_slot[j0];
+    a0 = j0 > lv0 ? lp.lv_end[j0 - 1] : 0;
+    b0 = lp.lv_end[j0];
+  }
+  int pre[kPre];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) sl[k] = tri_slot[3 * e + k];
-    T w, g[3][2], D[2][3];
-    tri_delta(ea, e, w, g, D);
+  for (int u = 0; u < kPre; ++u) pre[u] = a0 + u < b0 ? pos[a0 + u] : 0;
+  if (is_tri) {
+    const int e = ch * 256 + (int)threadIdx.x;
+    if (e < t) {
+      T w, g[3][2], D[2][3];
+      tri_delta(ea, e, w, g, D);
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      T* o = cslot + 3 * (size_t)sl[k];
-      tri_corner(w, g, D, k, o[0], o[1], o[2]);
+      for (int k = 0; k < 3; ++k) {
+        T* o = cv + 3 * (3 * threadIdx.x + k);
+        tri_corner(w, g, D, k, o[0], o[1], o[2]);
+      }
     }
   } else {
-    const int e = (~ch) * blockDim.x + threadIdx.x;
-    if (e >= h) return;
-    int sl[4];
+    const int e = (~ch) * 256 + (int)threadIdx.x;
+    if (e < h) {
+      T sg[4], cx, cy, cz;
+      hinge_delta(ea, e, sg, cx, cy, cz);
 #pragma unroll
-    for (int k = 0; k < 4; ++k) sl[k] = hinge_slot[4 * e + k];
-    T sg[4], cx, cy, cz;
-    hinge_delta(ea, e, sg, cx, cy, cz);
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      T* o = cslot + 3 * (size_t)sl[k];
-      o[0] = sg[k] * cx; o[1] = sg[k] * cy; o[2] = sg[k] * cz;
+      for (int k = 0; k < 4; ++k) {
+        T* o = cv + 3 * (4 * threadIdx.x + k);
+        o[0] = sg[k] * cx; o[1] = sg[k] * cy; o[2] = sg[k] * cz;
+      }
     }
+  }
+  __syncthreads();
+  if (j0 < lv1) {
+    T ax = 0, ay = 0, az = 0;
+#pragma unroll
+    for (int u = 0; u < kPre; ++u)
+      if (a0 + u < b0) {
+        const T* v = cv + 3 * pre[u];
+        ax += v[0]; ay += v[1]; az += v[2];
+      }
+    for (int q = a0 + kPre; q < b0; ++q) {
+      const T* v = cv + 3 * pos[q];
+      ax += v[0]; ay += v[1]; az += v[2];
+    }
+    T* o = cslot + 3 * (size_t)sl0;
+    o[0] = ax; o[1] = ay; o[2] = az;
+  }
+  // chunks that touch more than 256 vertices
+  for (int j = j0 + 256; j < lv1; j += 256) {
+    const int a = lp.lv_end[j - 1], b = lp.lv_end[j];
+    T ax = 0, ay = 0, az = 0;
+    for (int q = a; q < b; ++q) {
+      const T* v = cv + 3 * pos[q];
+      ax += v[0]; ay += v[1]; az += v[2];
+    }
+    T* o = cslot + 3 * (size_t)lp.lv_slot[j];
+    o[0] = ax; o[1] = ay; o[2] = az;
   }
 }
 

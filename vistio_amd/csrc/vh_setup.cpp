@@ -688,20 +688,58 @@ bool build_setup(const SetupInput& in, Setup& S) {
   }
   S.mass.resize(n); S.pinned.resize(n);
   for (int i = 0; i < n; ++i) { S.mass[i] = mass[S.perm[i]]; S.pinned[i] = in.pinned[S.perm[i]] ? 1 : 0; }
-  // Vertex -> slot incidence: triangle slot 3e+k as e>=0 code 3e+k, hinge
-  // slot (e, k) as -(4e+k)-1. Triangle slots first, each group ascending.
+  // RHS slots, one per (local-step chunk, vertex it touches): per chunk, its
+  // corners sorted by (vertex, place in the block) (chunks on several
+  // threads); then each vertex's slots numbered in chunk order.
   {
-    std::vector<int32_t> cnt(n + 1, 0);
-    for (int k = 0; k < 3 * t; ++k) cnt[S.tri_idx[k] + 1]++;
-    for (int k = 0; k < 4 * h; ++k) cnt[S.hinge_idx[k] + 1]++;
+    constexpr int kC = 256;
+    const int tb = (t + kC - 1) / kC, hb = (h + kC - 1) / kC, nch = tb + hb;
+    S.lc_pos.assign(3 * (size_t)t + 4 * (size_t)h, 0);
+    std::vector<std::vector<int32_t>> lvv(nch);
+    std::vector<std::vector<uint16_t>> lve(nch);
+    parallel_ranges(nch, host_threads(nch, 64), [&](int64_t cb, int64_t ce, int) {
+      std::vector<int64_t> key;
+      for (int64_t c = cb; c < ce; ++c) {
+        const bool tri = c < tb;
+        const int K = tri ? 3 : 4;
+        const int64_t e0 = (tri ? c : c - tb) * kC;
+        const int64_t ne = std::min<int64_t>(kC, (tri ? t : h) - e0);
+        const int32_t* idx = tri ? S.tri_idx.data() : S.hinge_idx.data();
+        key.clear();
+        for (int64_t le = 0; le < ne; ++le)
+          for (int k = 0; k < K; ++k) key.push_back(((int64_t)idx[(size_t)(e0 + le) * K + k] << 10) | (le * K + k));
+        std::sort(key.begin(), key.end());
+        const size_t base = tri ? 3 * (size_t)e0 : 3 * (size_t)t + 4 * (size_t)e0;
+        std::vector<int32_t>& V = lvv[c];
+        std::vector<uint16_t>& E = lve[c];
+        for (size_t q = 0; q < key.size(); ++q) {
+          const int32_t v = (int32_t)(key[q] >> 10);
+          if (V.empty() || V.back() != v) {
+            V.push_back(v);
+            E.push_back(0);
+          }
+          S.lc_pos[base + q] = (uint16_t)(key[q] & 1023);
+          E.back() = (uint16_t)(q + 1);
+        }
+      }
+    });
+    S.lv_ptr.assign((size_t)nch + 1, 0);
+    for (int c = 0; c < nch; ++c) S.lv_ptr[c + 1] = S.lv_ptr[c] + (int32_t)lvv[c].size();
+    std::vector<int32_t> cnt((size_t)n + 1, 0);
+    for (int c = 0; c < nch; ++c)
+      for (int32_t v : lvv[c]) cnt[(size_t)v + 1]++;
     for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
     S.inc_ptr = cnt;
-    S.inc.assign(cnt[n], 0);
     std::vector<int32_t> cur(cnt.begin(), cnt.end() - 1);
-    S.tri_slot.assign(3 * (size_t)t, 0);
-    S.hinge_slot.assign(4 * (size_t)h, 0);
-    for (int k = 0; k < 3 * t; ++k) { S.tri_slot[k] = cur[S.tri_idx[k]]; S.inc[cur[S.tri_idx[k]]++] = k; }
-    for (int k = 0; k < 4 * h; ++k) { S.hinge_slot[k] = cur[S.hinge_idx[k]]; S.inc[cur[S.hinge_idx[k]]++] = -k - 1; }
+    S.lv_slot.resize((size_t)S.lv_ptr[nch]);
+    S.lv_end.resize((size_t)S.lv_ptr[nch]);
+    for (int c = 0; c < nch; ++c) {
+      const size_t b = (size_t)S.lv_ptr[c];
+      for (size_t j = 0; j < lvv[c].size(); ++j) {
+        S.lv_slot[b + j] = cur[lvv[c][j]]++;
+        S.lv_end[b + j] = lve[c][j];
+      }
+    }
   }
   S.upper_cols.clear();
   S.band_front.assign(S.max_depth + 1, 0);

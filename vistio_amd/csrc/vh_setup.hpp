@@ -97,7 +97,7 @@ struct Setup {
   std::vector<double> mass;          // n, permuted (pinned -> 1e8)
   std::vector<double> mass_orig;     // n, original vertex order (lumped_masses())
   std::vector<uint8_t> pinned;       // n, permuted
-  std::vector<int32_t> inc_ptr, inc; // vertex -> element slot incidence (see kernels)
+  std::vector<int32_t> inc_ptr;      // vertex i sums RHS slots [inc_ptr[i], inc_ptr[i + 1])
   // system matrix in the permuted order, full symmetric CSR
   std::vector<int32_t> Ap, Ai;
   std::vector<double> Ax;
@@ -114,9 +114,19 @@ struct Setup {
   // from slot c.u_off + q; column k's value is pushed to
   // dpush_idx[dpush_ptr[k] .. dpush_ptr[k + 1]).
   std::vector<int32_t> dpush_ptr, dpush_idx;
-  // RHS slots: element corner (e, k) adds into slot tri_slot[3e + k]
-  // (hinges: hinge_slot[4e + k]); vertex i sums slots [inc_ptr[i], inc_ptr[i+1]).
-  std::vector<int32_t> tri_slot, hinge_slot;
+  // RHS slots. The local step runs one block per chunk of 256 triangles or
+  // 256 hinges (chunk c: the triangle chunks, then the hinge chunks); a block
+  // sums its corners' contributions per vertex in LDS, in a fixed order, and
+  // writes one slot per vertex it touches. Chunk c's local vertices are
+  // [lv_ptr[c], lv_ptr[c + 1]); local vertex j writes slot lv_slot[j] with
+  // the sum of the chunk's corners lc_pos[cb + lv_end[j - 1] .. cb + lv_end[j])
+  // (lv_end counts from the chunk's first local vertex, which starts at 0;
+  // cb = 3 * 256 * c for triangle chunk c, 3t + 4 * 256 * c' for hinge chunk
+  // c'; lc_pos holds a corner's place in its block, element * K + k with
+  // K = 3 for triangles and 4 for hinges). Each vertex's slots are ordered by
+  // chunk.
+  std::vector<int32_t> lv_ptr, lv_slot;
+  std::vector<uint16_t> lv_end, lc_pos;
   std::vector<int32_t> upper_cols;   // columns outside the deepest band
   std::vector<int32_t> band_front;   // max (w + r) per band
   // The solve blocks (built on the device, vh_factor.hpp): M holds M_P as
