@@ -297,8 +297,11 @@ constexpr int kWideBandTasks = 4096;
 // rows continue 8 per lane at a time). Measured at 256x256 (FP32): a
 // 1535-wide root in 512-thread workgroups 13.2 -> 11.7 us per launch (the
 // 256-thread band kernel before it), a 2295-wide one in 1024-thread
-// workgroups 18.3 -> 16.8 us. FP64 keeps k_root_band: its 24 entries per
-// lane spill in 1024-thread workgroups (15.0 -> 19.0 us at 1535 wide).
+// workgroups 18.3 -> 16.8 us; 512-thread workgroups for the 2295-wide root
+// (288 workgroups, all CUs busy, the front gathered twice as often) measured
+// slower than 1024 (1150 -> 1118 steps/s). FP64 keeps k_root_band: its 24
+// entries per lane spill in 1024-thread workgroups (15.0 -> 19.0 us at 1535
+// wide).
 int root_threads(int w, bool f64) {
   if (f64) return 0;
   return w > 2048 ? 1024 : 512;
