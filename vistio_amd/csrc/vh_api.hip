@@ -40,8 +40,7 @@ struct IpcState {
   uint8_t *inc = nullptr, *ever = nullptr;
   int *xptr = nullptr, *xidx = nullptr;
   vhc::Box* box = nullptr;
-  vhc::Node* node = nullptr;
-  vhc::Leaf* leaf = nullptr;
+  vhc::Rec* node = nullptr;  // t - 1 internal nodes, then the t leaves (leaf_of)
   unsigned int *code = nullptr, *code_s = nullptr;
   int *id = nullptr, *id_s = nullptr;
   int *left = nullptr, *range_lo = nullptr, *range_hi = nullptr;
@@ -904,6 +903,9 @@ hipError_t fit_pairs(vh_ctx* c, F&& redo, bool contacts, const int* flag = nullp
   return hipErrorUnknown;
 }
 
+// The leaf records, after the t - 1 internal nodes in the same allocation.
+vhc::Rec* leaf_of(vh_ctx* c) { return c->ipc.node + (c->S.t > 0 ? c->S.t - 1 : 0); }
+
 // Candidate triangle pairs of positions x (boxes inflated by margin).
 void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   IpcState& I = c->ipc;
@@ -926,11 +928,12 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
     hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
   hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, id_s, I.left,
-                     I.range_lo, I.range_hi, I.node, I.leaf, I.orig, I.sorig, skip);
+                     I.range_lo, I.range_hi, I.node, leaf_of(c), I.orig, I.sorig, skip);
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
-  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.pcount, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.pcount,
+                     skip);
   vhs::scan_exclusive(c->stream, I.pcount, I.poff, t, I.sbuf, I.ticket, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, I.leaf, I.sorig, I.pcount,
+  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.sorig, I.pcount,
                      I.poff, I.pairs, I.cap, I.bd, skip);
 }
 
@@ -2237,7 +2240,7 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.partials, 3 * (size_t)nblk(n)));
   UP(dalloc(ctx, &I.inc, (size_t)n)); UP(dalloc(ctx, &I.ever, (size_t)n));
   UP(upload(ctx, &I.xptr, xptr)); UP(upload(ctx, &I.xidx, xidx));
-  UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, (size_t)tt)); UP(dalloc(ctx, &I.leaf, (size_t)tt));
+  UP(dalloc(ctx, &I.box, (size_t)tt)); UP(dalloc(ctx, &I.node, 2 * (size_t)tt));
   UP(dalloc(ctx, &I.code, (size_t)tt)); UP(dalloc(ctx, &I.code_s, (size_t)tt));
   UP(dalloc(ctx, &I.id, (size_t)tt)); UP(dalloc(ctx, &I.id_s, (size_t)tt));
   UP(dalloc(ctx, &I.left, (size_t)tt));

@@ -495,18 +495,14 @@ __global__ void __launch_bounds__(256) k_st_level(int t, int half, const Box* __
 // right child is internal node s exactly when its range starts at s, so the
 // escape is internal node hi + 1 if range_lo[hi + 1] == hi + 1, else leaf
 // hi + 1, and kEnd after the last leaf.
+// Internal nodes and leaves share one 32-byte record, so a walk step is one
+// load whichever it visits. A node's range ends at leaf hi with its escape
+// encoding hi + 1 (kEnd: hi = t - 1), so hi needs no field of its own.
 constexpr int kEnd = 0x7fffffff;
-struct __align__(16) Node {
+struct __align__(16) Rec {
   Box box;
-  int left;    // >= 0 internal node, < 0 leaf ~j
-  int escape;  // same encoding, or kEnd
-  int hi;      // last leaf of the range
-  int pad[3];
-};
-struct __align__(16) Leaf {
-  Box box;
-  int escape;
-  int id;      // triangle of sorted leaf j
+  int a;       // node: left child (>= 0 internal node, < 0 leaf ~j); leaf: its triangle
+  int escape;  // same encoding as left, or kEnd
 };
 __device__ __forceinline__ int escape_of(int hi, int t, const int* __restrict__ range_lo) {
   if (hi >= t - 1) return kEnd;
@@ -514,29 +510,27 @@ __device__ __forceinline__ int escape_of(int hi, int t, const int* __restrict__ 
 }
 __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st, const int* __restrict__ id,
                                                      const int* __restrict__ left, const int* __restrict__ range_lo,
-                                                     const int* __restrict__ range_hi, Node* __restrict__ node,
-                                                     Leaf* __restrict__ leaf, const int* __restrict__ orig,
+                                                     const int* __restrict__ range_hi, Rec* __restrict__ node,
+                                                     Rec* __restrict__ leaf, const int* __restrict__ orig,
                                                      int* __restrict__ sorig, const int* skip) {
   if (skip && *skip) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < t) {
-    Leaf L;
+    Rec L;
     L.box = st[i];
     L.escape = escape_of(i, t, range_lo);
-    L.id = id[i];
+    L.a = id[i];
     leaf[i] = L;
-    sorig[i] = orig[L.id];  // the caller's index of sorted leaf i
+    sorig[i] = orig[L.a];  // the caller's index of sorted leaf i
   }
   if (i >= t - 1) return;
   const int lo = range_lo[i], hi = range_hi[i];
   const int k = 31 - __clz((unsigned int)(hi - lo + 1));
   const Box* lv = st + (size_t)k * (size_t)t;
-  Node N;
+  Rec N;
   N.box = box_union(lv[lo], lv[hi - (1 << k) + 1]);
-  N.left = left[i];
+  N.a = left[i];
   N.escape = escape_of(hi, t, range_lo);
-  N.hi = hi;
-  N.pad[0] = N.pad[1] = N.pad[2] = 0;
   node[i] = N;
 }
 
@@ -564,11 +558,11 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
 constexpr int kPairStage = 4096;
 
 template <bool kEmit>
-__device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+__device__ __forceinline__ unsigned int bvh_walk(int k, int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
                                                  const int* __restrict__ sorig, int2* __restrict__ pairs,
                                                  unsigned int base, unsigned int cap, int2* stage,
                                                  unsigned int stage_base) {
-  const Leaf me = leaf[k];
+  const Rec me = leaf[k];
   const int ome = kEmit ? sorig[k] : 0;
   unsigned int cnt = 0;
   // Start after leaf k in pre-order: the nodes there are the right subtrees
@@ -576,25 +570,31 @@ __device__ __forceinline__ unsigned int bvh_walk(int k, const Node* __restrict__
   // root to k (which only re-enters k's ancestors) is skipped.
   int nd = me.escape;
   while (nd != kEnd) {
-    if (nd >= 0) {
-      const Node N = node[nd];
-      nd = (N.hi > k && overlaps(me.box, N.box)) ? N.left : N.escape;
-    } else {
-      const int j = ~nd;
-      const Leaf L = leaf[j];
-      const int oj = kEmit ? sorig[j] : 0;  // loaded with the leaf, not after it
-      if (j > k && overlaps(me.box, L.box)) {
+    const bool is_leaf = nd < 0;
+    const int j = ~nd;
+    const Rec* rp = is_leaf ? leaf + j : node + nd;  // one load for either kind
+    const Rec R = *rp;
+    const int oj = kEmit ? sorig[is_leaf ? j : k] : 0;  // loaded with the record, not after it
+    const bool ov = overlaps(me.box, R.box);
+    if (is_leaf) {
+      if (j > k && ov) {
         if (kEmit) {
           const unsigned int slot = base + cnt;
           const bool first = ome < oj;
-          const int2 pr = make_int2(first ? me.id : L.id, first ? L.id : me.id);
+          const int2 pr = make_int2(first ? me.a : R.a, first ? R.a : me.a);
           const unsigned int ls = slot - stage_base;
           if (ls < (unsigned int)kPairStage) stage[ls] = pr;
           else if (slot < cap) pairs[slot] = pr;
         }
         ++cnt;
       }
-      nd = L.escape;
+      nd = R.escape;
+    } else {
+      // enter the node when its box overlaps and its range (ending at hi,
+      // hi + 1 encoded in the escape) reaches past k
+      const int e = R.escape;
+      const int h1 = e == kEnd ? t : (e >= 0 ? e : ~e);
+      nd = (h1 > k + 1 && ov) ? R.a : e;
     }
   }
   return cnt;
@@ -611,17 +611,17 @@ __device__ __forceinline__ int leaf_chunk(int t) {
   return c < nch ? c : -1;
 }
 
-__global__ void __launch_bounds__(256) k_bvh_count(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+__global__ void __launch_bounds__(256) k_bvh_count(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
                                                    unsigned int* __restrict__ count, const int* skip) {
   if (skip && *skip) return;
   const int ch = leaf_chunk(t);
   if (ch < 0) return;
   const int k = ch * blockDim.x + threadIdx.x;
   if (k >= t) return;
-  count[k] = t < 2 ? 0u : bvh_walk<false>(k, node, leaf, nullptr, nullptr, 0, 0, nullptr, 0);
+  count[k] = t < 2 ? 0u : bvh_walk<false>(k, t, node, leaf, nullptr, nullptr, 0, 0, nullptr, 0);
 }
 // offset: exclusive scan of count. The last leaf records the total.
-__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict__ node, const Leaf* __restrict__ leaf,
+__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
                                                    const int* __restrict__ sorig,
                                                    const unsigned int* __restrict__ count,
                                                    const unsigned int* __restrict__ offset, int2* __restrict__ pairs,
@@ -638,7 +638,7 @@ __global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Node* __restrict
   if (k < t) {
     const unsigned int base = offset[k];
     if (k == t - 1) bd->npairs = base + count[k];
-    if (t >= 2 && count[k] != 0 && base < cap) bvh_walk<true>(k, node, leaf, sorig, pairs, base, cap, stage, gbase);
+    if (t >= 2 && count[k] != 0 && base < cap) bvh_walk<true>(k, t, node, leaf, sorig, pairs, base, cap, stage, gbase);
   }
   __syncthreads();
   const unsigned int staged = min(gend - gbase, (unsigned int)kPairStage);
