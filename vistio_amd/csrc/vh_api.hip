@@ -920,11 +920,13 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   const int in_s = vhs::sort_pairs(c->stream, I.code, I.id, I.code_s, I.id_s, t, 30, I.sort_hist, skip);
   const unsigned int* code_s = in_s ? I.code_s : I.code;
   const int* id_s = in_s ? I.id_s : I.id;
-  hipLaunchKernelGGL(vhc::k_st_leaves, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.box, id_s, I.st, skip);
+  // sparse-table levels 0 .. low - 1 in one launch, the rest one level each
+  const int low = t < 2 ? 1 : std::min(I.st_levels, vhc::kStLowLevels + 1);
+  hipLaunchKernelGGL(vhc::k_st_low, dim3(nblk(t)), dim3(256), 0, c->stream, t, low, I.box, id_s, I.st, skip);
   if (t < 2) return;
   hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, code_s, I.left, I.range_lo,
                      I.range_hi, skip);
-  for (int k = 1; k < I.st_levels; ++k)
+  for (int k = low; k < I.st_levels; ++k)
     hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
                        I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
   hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, id_s, I.left,

@@ -472,12 +472,41 @@ __device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
   for (int q = 0; q < 3; ++q) { m.lo[q] = fminf(a.lo[q], b.lo[q]); m.hi[q] = fmaxf(a.hi[q], b.hi[q]); }
   return m;
 }
-// Level 0: the leaf boxes in Morton order.
-__global__ void __launch_bounds__(256) k_st_leaves(int t, const Box* __restrict__ box, const int* __restrict__ id,
-                                                   Box* __restrict__ st0, const int* skip) {
+// Levels 0 .. levels - 1 (at most kStLowLevels + 1) in one launch: block b
+// gathers the leaf boxes of [256 b, 256 b + 512) into LDS and forms each
+// level of its 256 entries from the previous one there (level k at j needs
+// level k - 1 at j and j + 2^(k-1), so a 256-entry halo covers 8 levels).
+// Level 0 is the leaf boxes in Morton order; the unions are those of
+// k_st_level.
+constexpr int kStLowLevels = 8;
+__global__ void __launch_bounds__(256) k_st_low(int t, int levels, const Box* __restrict__ box,
+                                                const int* __restrict__ id, Box* __restrict__ st, const int* skip) {
   if (skip && *skip) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < t) st0[j] = box[id[j]];
+  __shared__ Box w[512];
+  const int base = blockIdx.x * 256, tid = threadIdx.x;
+  for (int i = tid; i < 512; i += 256) {
+    const int j = base + i;
+    if (j < t) w[i] = box[id[j]];
+  }
+  if (base + tid < t) st[base + tid] = w[tid];
+  __syncthreads();
+  for (int k = 1; k < levels; ++k) {
+    const int half = 1 << (k - 1), valid = 512 - ((1 << k) - 1);  // entries of level k this block can form
+    Box nv[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = tid + 256 * u, j = base + i;
+      if (i < valid && j < t) nv[u] = j + half < t ? box_union(w[i], w[i + half]) : w[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = tid + 256 * u, j = base + i;
+      if (i < valid && j < t) w[i] = nv[u];
+    }
+    __syncthreads();
+    if (base + tid < t) st[(size_t)k * t + base + tid] = w[tid];
+  }
 }
 // Level k from level k - 1 (half = 2^(k-1)); windows reaching past t are
 // never queried.
