@@ -44,8 +44,9 @@ struct IpcState {
   unsigned int *code = nullptr, *code_s = nullptr;
   int *id = nullptr, *id_s = nullptr;
   int *left = nullptr, *range_lo = nullptr, *range_hi = nullptr;
-  vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t
+  vhc::Box* st = nullptr;  // sparse table of leaf-box unions, st_levels x t (levels 0..8)
   int st_levels = 0;
+  vhc::Box* coarse = nullptr;  // sparse table over whole 256-leaf cells (k_st_coarse)
   int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
   unsigned int* h_np = nullptr;  // pinned host copies of BvhDev::npairs and ::ncont (same allocation)
   unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
@@ -903,6 +904,13 @@ hipError_t fit_pairs(vh_ctx* c, F&& redo, bool contacts, const int* flag = nullp
   return hipErrorUnknown;
 }
 
+// Levels of the sparse table over nc whole cells: 0 .. floor(log2 nc).
+int coarse_levels(int nc) {
+  int m = 0;
+  while (nc > 0 && (1 << (m + 1)) <= nc) ++m;
+  return nc > 0 ? m + 1 : 0;
+}
+
 // The leaf records, after the t - 1 internal nodes in the same allocation.
 vhc::Rec* leaf_of(vh_ctx* c) { return c->ipc.node + (c->S.t > 0 ? c->S.t - 1 : 0); }
 
@@ -920,16 +928,17 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   const int in_s = vhs::sort_pairs(c->stream, I.code, I.id, I.code_s, I.id_s, t, 30, I.sort_hist, skip);
   const unsigned int* code_s = in_s ? I.code_s : I.code;
   const int* id_s = in_s ? I.id_s : I.id;
-  // sparse-table levels 0 .. low - 1 in one launch, the rest one level each
+  // sparse-table levels 0 .. low - 1 in one launch
   const int low = t < 2 ? 1 : std::min(I.st_levels, vhc::kStLowLevels + 1);
   hipLaunchKernelGGL(vhc::k_st_low, dim3(nblk(t)), dim3(256), 0, c->stream, t, low, I.box, id_s, I.st, skip);
   if (t < 2) return;
   hipLaunchKernelGGL(vhc::k_karras, dim3(nblk(t - 1)), dim3(256), 0, c->stream, t, code_s, I.left, I.range_lo,
                      I.range_hi, skip);
-  for (int k = low; k < I.st_levels; ++k)
-    hipLaunchKernelGGL(vhc::k_st_level, dim3(nblk(t)), dim3(256), 0, c->stream, t, 1 << (k - 1),
-                       I.st + (size_t)(k - 1) * t, I.st + (size_t)k * t, skip);
-  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, id_s, I.left,
+  const int nc = t >> 8;  // whole 256-leaf cells; ranges of 512 leaves or more use them
+  if (nc >= 2)
+    hipLaunchKernelGGL(vhc::k_st_coarse, dim3(1), dim3(vhc::kCoarseThreads), 0, c->stream, t, nc, coarse_levels(nc),
+                       I.st + (size_t)vhc::kStLowLevels * t, I.coarse, skip);
+  hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.coarse, nc, id_s, I.left,
                      I.range_lo, I.range_hi, I.node, leaf_of(c), I.orig, I.sorig, skip);
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
   hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.pcount,
@@ -2248,8 +2257,9 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   UP(dalloc(ctx, &I.left, (size_t)tt));
   UP(dalloc(ctx, &I.range_lo, (size_t)tt)); UP(dalloc(ctx, &I.range_hi, (size_t)tt));
   I.st_levels = 1;
-  while ((1ll << I.st_levels) <= (long long)tt) ++I.st_levels;  // levels 0 .. floor(log2 t)
+  while ((1ll << I.st_levels) <= (long long)tt && I.st_levels <= vhc::kStLowLevels) ++I.st_levels;  // levels 0 .. 8
   UP(dalloc(ctx, &I.st, (size_t)I.st_levels * (size_t)tt));
+  UP(dalloc(ctx, &I.coarse, (size_t)std::max(1, coarse_levels(tt >> 8)) * (size_t)std::max(1, tt >> 8)));
   UP(upload(ctx, &I.orig, S.tri_orig));
   UP(dalloc(ctx, &I.sorig, (size_t)tt));
   UP(dalloc(ctx, &I.pairs, (size_t)I.cap));

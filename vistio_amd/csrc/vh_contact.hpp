@@ -476,8 +476,8 @@ __device__ __forceinline__ Box box_union(const Box& a, const Box& b) {
 // gathers the leaf boxes of [256 b, 256 b + 512) into LDS and forms each
 // level of its 256 entries from the previous one there (level k at j needs
 // level k - 1 at j and j + 2^(k-1), so a 256-entry halo covers 8 levels).
-// Level 0 is the leaf boxes in Morton order; the unions are those of
-// k_st_level.
+// Level 0 is the leaf boxes in Morton order; level k at j is the union of
+// leaves [j, j + 2^k), clipped at t.
 constexpr int kStLowLevels = 8;
 __global__ void __launch_bounds__(256) k_st_low(int t, int levels, const Box* __restrict__ box,
                                                 const int* __restrict__ id, Box* __restrict__ st, const int* skip) {
@@ -508,15 +508,54 @@ __global__ void __launch_bounds__(256) k_st_low(int t, int levels, const Box* __
     if (base + tid < t) st[(size_t)k * t + base + tid] = w[tid];
   }
 }
-// Level k from level k - 1 (half = 2^(k-1)); windows reaching past t are
-// never queried.
-__global__ void __launch_bounds__(256) k_st_level(int t, int half, const Box* __restrict__ prev,
-                                                  Box* __restrict__ cur, const int* skip) {
+// Wider ranges: cell c is leaves [256 c, 256 c + 256) (its box is level 8
+// at 256 c), for the nc = t / 256 whole cells; coarse level m at c is the
+// union of cells [c, c + 2^m) (clipped at nc), stored at coarse[m nc + c].
+// One block forms all levels (each from the previous, a barrier apart), in
+// LDS when the cells fit.
+constexpr int kCoarseThreads = 1024;
+constexpr int kCoarseLds = 2048;  // cells held in LDS (48 KB); more go through global memory
+__global__ void __launch_bounds__(kCoarseThreads) k_st_coarse(int t, int nc, int levels, const Box* __restrict__ st8,
+                                                             Box* __restrict__ coarse, const int* skip) {
   if (skip && *skip) return;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= t) return;
-  const int o = j + half < t ? j + half : j;
-  cur[j] = box_union(prev[j], prev[o]);
+  if (nc <= kCoarseLds) {
+    __shared__ Box w[kCoarseLds];
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      const Box b = st8[(size_t)256 * c];
+      w[c] = b;
+      coarse[c] = b;
+    }
+    __syncthreads();
+    for (int m = 1; m < levels; ++m) {
+      const int half = 1 << (m - 1);
+      Box nv[kCoarseLds / kCoarseThreads];
+#pragma unroll
+      for (int u = 0; u < kCoarseLds / kCoarseThreads; ++u) {
+        const int c = threadIdx.x + u * kCoarseThreads;
+        if (c < nc) nv[u] = c + half < nc ? box_union(w[c], w[c + half]) : w[c];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int u = 0; u < kCoarseLds / kCoarseThreads; ++u) {
+        const int c = threadIdx.x + u * kCoarseThreads;
+        if (c < nc) {
+          w[c] = nv[u];
+          coarse[(size_t)m * nc + c] = nv[u];
+        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
+  for (int c = threadIdx.x; c < nc; c += blockDim.x) coarse[c] = st8[(size_t)256 * c];
+  __syncthreads();
+  for (int m = 1; m < levels; ++m) {
+    const int half = 1 << (m - 1);
+    const Box* prev = coarse + (size_t)(m - 1) * nc;
+    Box* cur = coarse + (size_t)m * nc;
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) cur[c] = c + half < nc ? box_union(prev[c], prev[c + half]) : prev[c];
+    __syncthreads();
+  }
 }
 // Traversal records. A subtree is skipped by jumping to its escape (rope):
 // for a range ending at leaf hi that is the highest node starting at hi + 1,
@@ -537,7 +576,8 @@ __device__ __forceinline__ int escape_of(int hi, int t, const int* __restrict__ 
   if (hi >= t - 1) return kEnd;
   return (hi + 1 <= t - 2 && range_lo[hi + 1] == hi + 1) ? hi + 1 : ~(hi + 1);
 }
-__global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st, const int* __restrict__ id,
+__global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restrict__ st, const Box* __restrict__ coarse,
+                                                     int nc, const int* __restrict__ id,
                                                      const int* __restrict__ left, const int* __restrict__ range_lo,
                                                      const int* __restrict__ range_hi, Rec* __restrict__ node,
                                                      Rec* __restrict__ leaf, const int* __restrict__ orig,
@@ -555,9 +595,20 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
   if (i >= t - 1) return;
   const int lo = range_lo[i], hi = range_hi[i];
   const int k = 31 - __clz((unsigned int)(hi - lo + 1));
-  const Box* lv = st + (size_t)k * (size_t)t;
   Rec N;
-  N.box = box_union(lv[lo], lv[hi - (1 << k) + 1]);
+  if (k <= kStLowLevels) {
+    // two windows of 2^k leaves that cover [lo, hi]
+    const Box* lv = st + (size_t)k * (size_t)t;
+    N.box = box_union(lv[lo], lv[hi - (1 << k) + 1]);
+  } else {
+    // at least 512 leaves: the 256-leaf windows at both ends, and the whole
+    // cells [a, b) between them from the coarse table (two windows again)
+    const Box* l8 = st + (size_t)kStLowLevels * (size_t)t;
+    const int a = (lo + 255) >> 8, b = (hi + 1) >> 8;
+    const int m = 31 - __clz((unsigned int)(b - a));
+    const Box* cm = coarse + (size_t)m * nc;
+    N.box = box_union(box_union(l8[lo], l8[hi - 255]), box_union(cm[a], cm[b - (1 << m)]));
+  }
   N.a = left[i];
   N.escape = escape_of(hi, t, range_lo);
   node[i] = N;
