@@ -339,6 +339,39 @@ vh_status vh_bvh_pairs(vh_ctx* ctx, const double* pos, double margin, uint32_t* 
  * reference has no such function; it checks what its contact scenes
  * promise (no interpenetration) on the GPU's own state. */
 vh_status vh_ipc_crossings(vh_ctx* ctx, const double* pos, uint64_t* count);
+/* ---- NarrowPhase (crates/vistio-contact/src/narrow.rs:21-35) -----------------
+ * The pre-IPC narrow phase of the reference's collision pipeline
+ * (CollisionPipeline::step with VertexTriangleTest, used by
+ * BenchmarkRunner::run_with_collision, runner.rs:66). */
+
+/* ContactPair (crates/vistio-contact/src/contact.rs:22-44) as
+ * VertexTriangleTest fills it: indices = [vertex, tri_v0, tri_v1, tri_v2],
+ * distance = -|plane distance|, normal from the triangle toward the vertex,
+ * barycentric (u, v, w) of the vertex's projection; contact_type 0
+ * (ContactType::VertexTriangle) and is_self 1 for every such contact. */
+typedef struct {
+  uint32_t indices[4];
+  float distance;
+  float normal[3];
+  float barycentric[3];
+  int32_t contact_type;
+  int32_t is_self;
+} vh_contact_pair;
+
+/* VertexTriangleTest::detect (vertex_triangle.rs:25-81, with
+ * point_triangle_proximity :88-152), in f32 as the reference: for each
+ * candidate (cand_a = vertex vi, cand_b = vertex vj), in order, every
+ * triangle of tri (t triangles, 3t indices, caller order), ascending, that
+ * contains vj and not vi, whose plane lies within thickness of vi and whose
+ * projection of vi passes the barycentric test (each coordinate >= -0.01).
+ * px/py/pz hold n f32 positions (SimulationState's SoA). Writes up to cap
+ * contacts and the total count. An index past n returns VH_E_INVALID_ARG
+ * (candidates) or VH_E_INVALID_MESH (triangles). Needs only vh_create. */
+vh_status vh_narrow_vertex_triangle(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz,
+                                    uint32_t t, const uint32_t* tri, uint64_t ncand, const uint32_t* cand_a,
+                                    const uint32_t* cand_b, float thickness, vh_contact_pair* out, uint64_t cap,
+                                    uint64_t* count);
+
 /* Resident multipliers (3n SoA, caller order). */
 vh_status vh_state_lambda_set_f64(vh_ctx* ctx, const double* lambda);
 vh_status vh_state_lambda_get_f64(vh_ctx* ctx, double* lambda);

@@ -13,6 +13,7 @@
 //                          (ta < tb), found here by sort-and-sweep on min_x
 //   self_collision.rs, ipc_response.rs, collision_pipeline.rs
 //                          narrow phase and barrier-force assembly
+//   vertex_triangle.rs    the pre-IPC narrow phase (VertexTriangleTest), in f32
 // Each function cites the reference lines it follows.
 #pragma once
 
@@ -690,6 +691,81 @@ R ccd_step(int n, const R* x0, const R* y0, const R* z0, const R* x1, const R* y
     toi = std::min(toi, std::max(s, R(1e-6)));
   }
   return std::max(toi, R(1e-6));
+}
+
+// ---- vertex_triangle.rs: the pre-IPC narrow phase ---------------------------
+
+// One contact of VertexTriangleTest (contact.rs:22-44 ContactPair; the type is
+// always VertexTriangle and is_self always true there).
+struct VtContact {
+  uint32_t idx[4];  // vertex, triangle's three vertices
+  float distance;   // -|plane distance|
+  float normal[3];  // from the triangle to the vertex
+  float bary[3];    // (u, v, w)
+};
+
+// point_triangle_proximity, vertex_triangle.rs:88-152, in f32 with the
+// reference's operation order (glam Vec3: component-wise ops, cross as
+// y*z' - y'*z, dot left to right, length = sqrt(dot), division by the
+// scalar per component); the oracle is compiled without FMA contraction.
+inline bool vt_proximity(const float p[3], const float a[3], const float b[3], const float c[3], float thickness,
+                         VtContact& o) {
+  const float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const float ap[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+  float nrm[3] = {ab[1] * ac[2] - ac[1] * ab[2], ab[2] * ac[0] - ac[2] * ab[0], ab[0] * ac[1] - ac[0] * ab[1]};
+  const float area2 = std::sqrt(nrm[0] * nrm[0] + nrm[1] * nrm[1] + nrm[2] * nrm[2]);
+  if (area2 < 1e-10f) return false;  // :108-111 degenerate triangle
+  nrm[0] = nrm[0] / area2; nrm[1] = nrm[1] / area2; nrm[2] = nrm[2] / area2;
+  const float sd = ap[0] * nrm[0] + ap[1] * nrm[1] + ap[2] * nrm[2];  // :114 signed plane distance
+  const float ad = std::fabs(sd);
+  if (ad > thickness) return false;  // :117-119
+  const float proj[3] = {p[0] - nrm[0] * sd, p[1] - nrm[1] * sd, p[2] - nrm[2] * sd};
+  const float app[3] = {proj[0] - a[0], proj[1] - a[1], proj[2] - a[2]};
+  const float d00 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+  const float d01 = ab[0] * ac[0] + ab[1] * ac[1] + ab[2] * ac[2];
+  const float d11 = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+  const float d20 = app[0] * ab[0] + app[1] * ab[1] + app[2] * ab[2];
+  const float d21 = app[0] * ac[0] + app[1] * ac[1] + app[2] * ac[2];
+  const float den = d00 * d11 - d01 * d01;
+  if (std::fabs(den) < 1e-10f) return false;  // :133-136
+  const float inv = 1.0f / den;
+  const float v = (d11 * d20 - d01 * d21) * inv;
+  const float w = (d00 * d21 - d01 * d20) * inv;
+  const float u = 1.0f - v - w;
+  const float tol = -0.01f;
+  if (u < tol || v < tol || w < tol) return false;  // :142-146
+  const float sgn = sd >= 0.0f ? 1.0f : -1.0f;          // :148-149 normal toward the vertex
+  o.distance = -ad;
+  o.normal[0] = sgn > 0 ? nrm[0] : -nrm[0];
+  o.normal[1] = sgn > 0 ? nrm[1] : -nrm[1];
+  o.normal[2] = sgn > 0 ? nrm[2] : -nrm[2];
+  o.bary[0] = u; o.bary[1] = v; o.bary[2] = w;
+  return true;
+}
+
+// VertexTriangleTest::detect, vertex_triangle.rs:25-81: for each candidate
+// (a = vertex vi, b = vertex vj) in order, every triangle t, ascending, that
+// contains vj and not vi.
+inline std::vector<VtContact> vt_detect(const float* x, const float* y, const float* z, int t, const uint32_t* tri,
+                                        long long nc, const uint32_t* ca, const uint32_t* cb, float thickness) {
+  std::vector<VtContact> out;
+  for (long long k = 0; k < nc; ++k) {
+    const uint32_t vi = ca[k], vj = cb[k];
+    const float p[3] = {x[vi], y[vi], z[vi]};
+    for (int e = 0; e < t; ++e) {
+      const uint32_t A = tri[3 * e], B = tri[3 * e + 1], Cc = tri[3 * e + 2];
+      if (vi == A || vi == B || vi == Cc) continue;
+      if (vj != A && vj != B && vj != Cc) continue;
+      const float pa[3] = {x[A], y[A], z[A]}, pb[3] = {x[B], y[B], z[B]}, pc[3] = {x[Cc], y[Cc], z[Cc]};
+      VtContact o;
+      if (vt_proximity(p, pa, pb, pc, thickness, o)) {
+        o.idx[0] = vi; o.idx[1] = A; o.idx[2] = B; o.idx[3] = Cc;
+        out.push_back(o);
+      }
+    }
+  }
+  return out;
 }
 
 }  // namespace oc

@@ -1468,4 +1468,22 @@ double oc_ccd_step(int n, const double* p0, const double* p1, int t, const uint3
   return oc::ccd_step<double>(n, p0, p0 + n, p0 + 2 * size_t(n), p1, p1 + n, p1 + 2 * size_t(n), tri, pairs, sc);
 }
 
+// VertexTriangleTest::detect (vertex_triangle.rs:25-81). out: up to cap
+// contacts of 14 words each (4 uint32 indices as their bits, distance,
+// normal[3], barycentric[3], then 3 zero words); returns the total count.
+long long oc_narrow_vt(int n, const float* x, const float* y, const float* z, int t, const uint32_t* tri,
+                       long long nc, const uint32_t* ca, const uint32_t* cb, float thickness, float* out,
+                       long long cap) {
+  (void)n;
+  const std::vector<oc::VtContact> c = oc::vt_detect(x, y, z, t, tri, nc, ca, cb, thickness);
+  for (long long k = 0; k < (long long)c.size() && k < cap; ++k) {
+    float* o = out + 14 * k;
+    std::memcpy(o, c[k].idx, 16);
+    o[4] = c[k].distance;
+    for (int q = 0; q < 3; ++q) { o[5 + q] = c[k].normal[q]; o[8 + q] = c[k].bary[q]; }
+    o[11] = o[12] = o[13] = 0.0f;
+  }
+  return (long long)c.size();
+}
+
 }  // extern "C"

@@ -183,6 +183,9 @@ def lib():
         L.oc_tri_pairs.restype = C.c_longlong
         L.oc_tri_pairs.argtypes = [C.c_int, fp, fp, fp, C.c_int, up, C.c_float, C.c_int, C.POINTER(C.c_int),
                                    C.c_longlong]
+        L.oc_narrow_vt.restype = C.c_longlong
+        L.oc_narrow_vt.argtypes = [C.c_int, fp, fp, fp, C.c_int, up, C.c_longlong, up, up, C.c_float, fp,
+                                   C.c_longlong]
         L.oc_crossings.restype = C.c_longlong
         L.oc_crossings.argtypes = [C.c_int, dp, C.c_int, up, C.c_int]
         L.oc_detect.argtypes = [C.c_int, dp, C.c_int, up, ip, C.c_double, C.c_double, dp, dp,
@@ -466,6 +469,26 @@ def tri_pairs(px, py, pz, indices, margin, brute=False):
     out = np.empty((max(cnt, 1), 2), dtype=np.int32)
     L.oc_tri_pairs(x.size, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), t, _p(tri, C.c_uint32),
                    margin, int(brute), out.ctypes.data_as(C.POINTER(C.c_int)), cnt)
+    return out[:cnt]
+
+
+VT_CONTACT = np.dtype([("indices", np.uint32, 4), ("distance", np.float32), ("normal", np.float32, 3),
+                       ("barycentric", np.float32, 3), ("pad", np.uint32, 3)])
+
+
+def narrow_vt(px, py, pz, indices, cand, thickness):
+    """VertexTriangleTest::detect (vertex_triangle.rs:25-81) in f32: a
+    VT_CONTACT record per contact, in the reference's order."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float32) for a in (px, py, pz))
+    tri = np.ascontiguousarray(indices, dtype=np.uint32)
+    c = np.ascontiguousarray(np.asarray(cand, dtype=np.uint32).reshape(-1, 2))
+    ca, cb = np.ascontiguousarray(c[:, 0]), np.ascontiguousarray(c[:, 1])
+    L = lib()
+    args = (x.size, _p(x, C.c_float), _p(y, C.c_float), _p(z, C.c_float), tri.size // 3, _p(tri, C.c_uint32),
+            ca.size, _p(ca, C.c_uint32), _p(cb, C.c_uint32), float(thickness))
+    cnt = L.oc_narrow_vt(*args, None, 0)
+    out = np.zeros(max(cnt, 1), dtype=VT_CONTACT)
+    L.oc_narrow_vt(*args, out.ctypes.data_as(C.POINTER(C.c_float)), cnt)
     return out[:cnt]
 
 

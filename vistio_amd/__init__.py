@@ -8,7 +8,8 @@ from .solver import (FabricProperties, GpuError, HipPdSolver, InvalidConfig, Inv
                      MaterialDatabase, SimulationState, SolverConfig, StepResult, Topology, TriangleMesh,
                      VistioError, estimate_initial_kappa, quad_grid)
 from .backend import ComputeBuffer, HipBackend
+from .narrow import CandidatePair, HipVertexTriangleTest
 
-__all__ = ["ComputeBuffer", "FabricProperties", "GpuError", "HipBackend", "HipPdSolver", "InvalidConfig",
+__all__ = ["CandidatePair", "ComputeBuffer", "FabricProperties", "GpuError", "HipBackend", "HipPdSolver", "HipVertexTriangleTest", "InvalidConfig",
            "InvalidMesh", "IpcConfig", "MaterialDatabase", "SimulationState", "SolverConfig", "StepResult", "Topology", "TriangleMesh",
            "VistioError", "estimate_initial_kappa", "quad_grid"]

@@ -32,7 +32,7 @@ EXPORTS = (
     "vh_ipc_init", "vh_pd_step_ipc", "vh_pd_step_ipc_resident", "vh_ipc_detect", "vh_ipc_ccd_step", "vh_bvh_pairs",
     "vh_state_lambda_set_f64", "vh_state_lambda_get_f64", "vh_ipc_crossings", "vh_pd_init_ex",
     "vh_backend_axpy", "vh_backend_dot", "vh_backend_fill", "vh_backend_dot_noalloc", "vh_pd_blocks_get",
-    "vh_pd_profile_graph",
+    "vh_pd_profile_graph", "vh_narrow_vertex_triangle",
 )
 
 
@@ -49,6 +49,12 @@ class PdParams(C.Structure):
         ("warp_stiffness", C.c_double), ("weft_stiffness", C.c_double),
         ("bend_warp", C.c_double), ("bend_weft", C.c_double),
     ]
+
+
+class ContactPairC(C.Structure):
+    """vh_contact_pair (ContactPair, contact.rs:22-44, as VertexTriangleTest fills it)."""
+    _fields_ = [("indices", C.c_uint32 * 4), ("distance", C.c_float), ("normal", C.c_float * 3),
+                ("barycentric", C.c_float * 3), ("contact_type", C.c_int32), ("is_self", C.c_int32)]
 
 
 class MaterialArrays(C.Structure):
@@ -164,6 +170,8 @@ def load():
         "vh_ipc_ccd_step": ([vp, dp, dp, dp], C.c_int),
         "vh_bvh_pairs": ([vp, dp, C.c_double, up, up, C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
         "vh_ipc_crossings": ([vp, dp, C.POINTER(C.c_uint64)], C.c_int),
+        "vh_narrow_vertex_triangle": ([vp, C.c_uint32, fp, fp, fp, C.c_uint32, up, C.c_uint64, up, up, C.c_float,
+                                       C.POINTER(ContactPairC), C.c_uint64, C.POINTER(C.c_uint64)], C.c_int),
         "vh_state_lambda_set_f64": ([vp, dp], C.c_int),
         "vh_state_lambda_get_f64": ([vp, dp], C.c_int),
     }

@@ -28,6 +28,7 @@
 #include "vh_factor.hpp"
 #include "vh_setup.hpp"
 #include "vh_sort.hpp"
+#include "vh_narrow.hpp"
 #include "vistio_hip.h"
 
 // Device buffers and settings of the contact step (vh_ipc_init).
@@ -161,6 +162,10 @@ struct vh_ctx {
   // GpuBackend staging (vh_backend_*): device copies of the operands, kept
   // across calls and grown on demand; independent of the solver's state.
   float *bk_a = nullptr, *bk_b = nullptr, *bk_o = nullptr;
+  // vh_narrow_vertex_triangle's device buffers, one growable allocation; its
+  // first word is the scan's ticket (0 between launches)
+  void* nb_arena = nullptr;
+  size_t nb_bytes = 0;
   double* bk_part = nullptr;
   size_t bk_cap = 0;
   hipGraphExec_t step_graph = nullptr;
@@ -1385,7 +1390,7 @@ vh_status vh_destroy(vh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   free_all(ctx);
-  for (void* p : {(void*)ctx->bk_a, (void*)ctx->bk_b, (void*)ctx->bk_o, (void*)ctx->bk_part})
+  for (void* p : {(void*)ctx->bk_a, (void*)ctx->bk_b, (void*)ctx->bk_o, (void*)ctx->bk_part, ctx->nb_arena})
     if (p) (void)hipFree(p);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -2579,6 +2584,98 @@ vh_status vh_backend_fill(vh_ctx* ctx, float* buf, uint64_t n, float value) {
   VH_HIP(hipGetLastError());
   VH_HIP(hipMemcpyAsync(buf, ctx->bk_o, n * 4, hipMemcpyDeviceToHost, ctx->stream));
   VH_HIP(hipStreamSynchronize(ctx->stream));
+  return VH_OK;
+}
+
+vh_status vh_narrow_vertex_triangle(vh_ctx* ctx, uint32_t n, const float* px, const float* py, const float* pz,
+                                    uint32_t t, const uint32_t* tri, uint64_t ncand, const uint32_t* cand_a,
+                                    const uint32_t* cand_b, float thickness, vh_contact_pair* out, uint64_t cap,
+                                    uint64_t* count) {
+  if (!ctx) return VH_E_INVALID_ARG;
+  if (!count) return fail(ctx, VH_E_INVALID_ARG, "count is null");
+  *count = 0;
+  if (ncand && (!cand_a || !cand_b)) return fail(ctx, VH_E_INVALID_ARG, "candidates are null");
+  if (n && (!px || !py || !pz)) return fail(ctx, VH_E_INVALID_ARG, "positions are null");
+  if (t && !tri) return fail(ctx, VH_E_INVALID_ARG, "triangles are null");
+  if (cap && !out) return fail(ctx, VH_E_INVALID_ARG, "out is null");
+  if (ncand > (1ull << 31)) return fail(ctx, VH_E_UNSUPPORTED, "more than 2^31 candidates");
+  for (uint64_t k = 0; k < ncand; ++k)
+    if (cand_a[k] >= n || cand_b[k] >= n) return fail(ctx, VH_E_INVALID_ARG, "candidate vertex index out of range");
+  for (uint64_t i = 0; i < 3 * (uint64_t)t; ++i)
+    if (tri[i] >= n) return fail(ctx, VH_E_INVALID_MESH, "triangle index out of range");
+  if (ncand == 0 || t == 0) return VH_OK;
+  VH_HIP(hipSetDevice(ctx->device));
+  // vertex -> triangles, ascending, a triangle once per vertex it contains
+  std::vector<int> vptr((size_t)n + 1, 0), vtri;
+  auto each_vertex = [&](uint32_t e, auto&& f) {
+    const uint32_t A = tri[3 * (size_t)e], B = tri[3 * (size_t)e + 1], C = tri[3 * (size_t)e + 2];
+    f(A);
+    if (B != A) f(B);
+    if (C != A && C != B) f(C);
+  };
+  for (uint32_t e = 0; e < t; ++e) each_vertex(e, [&](uint32_t v) { ++vptr[(size_t)v + 1]; });
+  for (uint32_t i = 0; i < n; ++i) vptr[(size_t)i + 1] += vptr[i];
+  vtri.resize((size_t)vptr[n]);
+  std::vector<int> cur(vptr.begin(), vptr.end() - 1);
+  for (uint32_t e = 0; e < t; ++e) each_vertex(e, [&](uint32_t v) { vtri[(size_t)cur[v]++] = (int)e; });
+  uint64_t bound = 0;  // the most contacts the candidates can give
+  for (uint64_t k = 0; k < ncand; ++k) bound += (uint64_t)(vptr[(size_t)cand_b[k] + 1] - vptr[cand_b[k]]);
+  if (bound > 0xffffffffull) return fail(ctx, VH_E_UNSUPPORTED, "candidates could give more than 2^32 contacts");
+  const uint64_t ocap = std::min<uint64_t>(cap, bound);
+  // one allocation: ticket | positions | tri | vptr | vtri | ca | cb | cnt | off | tile sums | contacts
+  size_t at = 256;
+  auto carve = [&](size_t bytes) {
+    const size_t o = at;
+    at += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t o_pos = carve(3 * (size_t)n * 4), o_tri = carve(3 * (size_t)t * 4), o_vptr = carve(((size_t)n + 1) * 4),
+               o_vtri = carve(vtri.size() * 4 + 4), o_ca = carve(ncand * 4), o_cb = carve(ncand * 4),
+               o_cnt = carve(ncand * 4), o_off = carve(ncand * 4),
+               o_bsum = carve((size_t)vhs::scan_scratch_words((long long)ncand) * 4),
+               o_out = carve(std::max<uint64_t>(ocap, 1) * sizeof(vh_contact_pair));
+  if (at > ctx->nb_bytes) {
+    if (ctx->nb_arena) VH_HIP(hipFree(ctx->nb_arena));
+    ctx->nb_arena = nullptr;
+    ctx->nb_bytes = 0;
+    VH_HIP(hipMalloc(&ctx->nb_arena, at));
+    ctx->nb_bytes = at;
+    VH_HIP(hipMemsetAsync(ctx->nb_arena, 0, 256, ctx->stream));
+  }
+  char* base = static_cast<char*>(ctx->nb_arena);
+  float* dpos = reinterpret_cast<float*>(base + o_pos);
+  auto* dcnt = reinterpret_cast<unsigned int*>(base + o_cnt);
+  auto* doff = reinterpret_cast<unsigned int*>(base + o_off);
+  auto* dout = reinterpret_cast<vh_contact_pair*>(base + o_out);
+  hipStream_t s = ctx->stream;
+  VH_HIP(hipMemcpyAsync(dpos, px, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dpos + n, py, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(dpos + 2 * (size_t)n, pz, (size_t)n * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(base + o_tri, tri, 3 * (size_t)t * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(base + o_vptr, vptr.data(), vptr.size() * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(base + o_vtri, vtri.data(), vtri.size() * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(base + o_ca, cand_a, ncand * 4, hipMemcpyHostToDevice, s));
+  VH_HIP(hipMemcpyAsync(base + o_cb, cand_b, ncand * 4, hipMemcpyHostToDevice, s));
+  vhn::NarrowArgs a{dpos, dpos + n, dpos + 2 * (size_t)n, reinterpret_cast<const unsigned int*>(base + o_tri),
+                    reinterpret_cast<const int*>(base + o_vptr), reinterpret_cast<const int*>(base + o_vtri),
+                    reinterpret_cast<const unsigned int*>(base + o_ca), reinterpret_cast<const unsigned int*>(base + o_cb),
+                    (long long)ncand, thickness};
+  const dim3 grid((unsigned int)((ncand + 255) / 256));
+  hipLaunchKernelGGL(vhn::k_narrow_vt<false>, grid, dim3(256), 0, s, a, dcnt, (const unsigned int*)nullptr,
+                     (vh_contact_pair*)nullptr, 0ull);
+  vhs::scan_exclusive(s, dcnt, doff, (long long)ncand, reinterpret_cast<unsigned int*>(base + o_bsum),
+                      reinterpret_cast<unsigned int*>(base), nullptr);
+  hipLaunchKernelGGL(vhn::k_narrow_vt<true>, grid, dim3(256), 0, s, a, (unsigned int*)nullptr,
+                     (const unsigned int*)doff, dout, (unsigned long long)ocap);
+  VH_HIP(hipGetLastError());
+  unsigned int last[2] = {0, 0};
+  VH_HIP(hipMemcpyAsync(&last[0], doff + (ncand - 1), 4, hipMemcpyDeviceToHost, s));
+  VH_HIP(hipMemcpyAsync(&last[1], dcnt + (ncand - 1), 4, hipMemcpyDeviceToHost, s));
+  VH_HIP(hipStreamSynchronize(s));
+  const uint64_t total = (uint64_t)last[0] + last[1];
+  const uint64_t nout = std::min<uint64_t>(total, ocap);
+  if (nout) VH_HIP(hipMemcpy(out, dout, nout * sizeof(vh_contact_pair), hipMemcpyDeviceToHost));
+  *count = total;
   return VH_OK;
 }
 

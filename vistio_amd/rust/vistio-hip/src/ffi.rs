@@ -199,6 +199,18 @@ pub struct VhIpcStats {
     pub last_alpha: f64,
 }
 
+/// `vh_contact_pair` (ContactPair as VertexTriangleTest fills it).
+#[repr(C)]
+#[derive(Debug, Clone, Copy, Default)]
+pub struct VhContactPair {
+    pub indices: [u32; 4],
+    pub distance: f32,
+    pub normal: [f32; 3],
+    pub barycentric: [f32; 3],
+    pub contact_type: i32,
+    pub is_self: i32,
+}
+
 #[link(name = "vistio_hip")]
 extern "C" {
     pub fn vh_create(device: i32, out: *mut *mut VhCtx) -> VhStatus;
@@ -284,6 +296,22 @@ extern "C" {
         count: *mut u64,
     ) -> VhStatus;
     pub fn vh_ipc_crossings(ctx: *mut VhCtx, pos: *const f64, count: *mut u64) -> VhStatus;
+    pub fn vh_narrow_vertex_triangle(
+        ctx: *mut VhCtx,
+        n: u32,
+        px: *const f32,
+        py: *const f32,
+        pz: *const f32,
+        t: u32,
+        tri: *const u32,
+        ncand: u64,
+        cand_a: *const u32,
+        cand_b: *const u32,
+        thickness: f32,
+        out: *mut VhContactPair,
+        cap: u64,
+        count: *mut u64,
+    ) -> VhStatus;
     pub fn vh_state_lambda_set_f64(ctx: *mut VhCtx, lambda: *const f64) -> VhStatus;
     pub fn vh_state_lambda_get_f64(ctx: *mut VhCtx, lambda: *mut f64) -> VhStatus;
     pub fn vh_mesh_quad_grid(

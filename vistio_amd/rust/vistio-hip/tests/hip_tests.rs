@@ -7,7 +7,8 @@
 use vistio_contact::broad::BroadPhase;
 use vistio_contact::BvhBroadPhase;
 use vistio_gpu::{ComputeBuffer, CpuFallback, GpuBackend};
-use vistio_hip::{DeviceScene, HipBackend, HipBroadPhase, HipPdSolver, HipSettings, IpcScene};
+use vistio_contact::{broad::CandidatePair, narrow::NarrowPhase, VertexTriangleTest};
+use vistio_hip::{DeviceScene, HipBackend, HipBroadPhase, HipNarrowPhase, HipPdSolver, HipSettings, IpcScene};
 use vistio_material::{CoRotationalModel, FabricProperties};
 use vistio_mesh::{generators::quad_grid, topology::Topology};
 use vistio_solver::{
@@ -117,6 +118,45 @@ fn hip_broad_phase_contains_bvh_pairs() {
         assert!(all.contains(&(p.ta.min(p.tb), p.ta.max(p.tb))), "pair ({}, {}) missing", p.ta, p.tb);
     }
     assert!(all.len() >= cpu.query_triangle_pairs().len());
+}
+
+#[test]
+fn hip_narrow_phase_matches_vertex_triangle_test() {
+    // a sheet folded back over itself 6 mm above, every vertex pair closer than 5 cm
+    let mut mesh = quad_grid(16, 16, 1.0, 1.0);
+    for i in 0..mesh.vertex_count() {
+        mesh.pos_z[i] = (((i * 2654435761usize) % 1000) as f32 / 1000.0 - 0.5) * 0.008;
+        if mesh.pos_x[i] > 0.0 {
+            mesh.pos_x[i] = -mesh.pos_x[i];
+            mesh.pos_z[i] += 0.006;
+        }
+    }
+    let n = mesh.vertex_count();
+    let pinned = vec![false; n];
+    let state = SimulationState::from_mesh(&mesh, 0.15, &pinned).unwrap();
+    let mut cand = Vec::new();
+    for a in 0..n {
+        for b in 0..n {
+            let d = (mesh.pos_x[a] - mesh.pos_x[b]).powi(2) + (mesh.pos_y[a] - mesh.pos_y[b]).powi(2)
+                + (mesh.pos_z[a] - mesh.pos_z[b]).powi(2);
+            if a != b && d < 0.05 * 0.05 {
+                cand.push(CandidatePair { a: a as u32, b: b as u32, is_self: true });
+            }
+        }
+    }
+    let cpu = VertexTriangleTest.detect(&cand, &state, &mesh, 0.01).unwrap();
+    let hip = HipNarrowPhase::new(0).unwrap();
+    let gpu = hip.detect(&cand, &state, &mesh, 0.01).unwrap();
+    assert_eq!(hip.name(), "vertex_triangle_test");
+    assert!(cpu.len() > 50);
+    assert_eq!(gpu.len(), cpu.len());
+    for (g, c) in gpu.iter().zip(&cpu) {
+        assert_eq!(g.indices, c.indices);
+        assert_eq!(g.distance.to_bits(), c.distance.to_bits());
+        assert_eq!(g.normal.map(f32::to_bits), c.normal.map(f32::to_bits));
+        assert_eq!(g.barycentric.map(f32::to_bits), c.barycentric.map(f32::to_bits));
+        assert_eq!(g.is_self, c.is_self);
+    }
 }
 
 #[test]
