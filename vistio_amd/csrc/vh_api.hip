@@ -619,13 +619,16 @@ void enqueue_step(vh_ctx* c, double dt) {
     const bool from_slots = enqueue_local_step(c, stk);
     Marks none;
     enqueue_correction(c, dt, last ? P.ir_final_sweeps : P.ir_sweeps, from_slots, true, want_resid, stk, none);
-    if (want_resid)
+    if (want_resid && !last)
       hipLaunchKernelGGL(vhk::k_resid_reduce, dim3(1), dim3(256), 0, c->stream, c->partials, c->resid_blocks,
                          P.tolerance, it, c->st);
   }
+  // the last iteration's residual is formed by one extra block of k_finalize
   const double frayl = P.rayleigh > 0 ? 1.0 / (1.0 + P.rayleigh * dt) : 0.0;
-  hipLaunchKernelGGL(vhk::k_finalize, dim3(nblk(S.n)), dim3(256), 0, c->stream, S.n, c->x, c->prev, c->v, c->pinned,
-                     1.0 / dt, P.has_ground, P.ground_height, 1.0 - P.damping, frayl);
+  const vhk::ResidArgs rs{c->partials, c->resid_blocks, P.tolerance, P.max_iterations - 1,
+                          P.max_iterations > 0 ? c->st : nullptr};
+  hipLaunchKernelGGL(vhk::k_finalize, dim3(nblk(S.n) + (rs.st ? 1 : 0)), dim3(256), 0, c->stream, S.n, c->x, c->prev,
+                     c->v, c->pinned, 1.0 / dt, P.has_ground, P.ground_height, 1.0 - P.damping, frayl, rs);
 }
 
 // Enqueues one step: replays the captured graph (captured on first use for

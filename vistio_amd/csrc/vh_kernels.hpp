@@ -1777,9 +1777,11 @@ __global__ void __launch_bounds__(256) k_ir_update(int n, const int* __restrict_
 
 // Final residual of a PD iteration (pd_solver.rs:402-421, 440-444): sums the
 // per-block (|delta|^2, |x|^2) partials in a fixed order.
-__global__ void __launch_bounds__(256) k_resid_reduce(const double* __restrict__ partials, int count, double tol,
-                                                      int iter, StepDev* st) {
-  if (st->done) return;
+// One block of 256 threads: the sums in a fixed order, the residual, and
+// the early-exit flag.
+__device__ __forceinline__ void resid_reduce_block(const double* __restrict__ partials, int count, double tol,
+                                                   int iter, StepDev* st) {
+  if (st->done) return;  // uniform over the block
   __shared__ double sd[256], sn[256];
   double a = 0, b = 0;
   for (int k = threadIdx.x; k < count; k += blockDim.x) { a += partials[2 * k]; b += partials[2 * k + 1]; }
@@ -1795,6 +1797,10 @@ __global__ void __launch_bounds__(256) k_resid_reduce(const double* __restrict__
     st->iterations = iter + 1;
     if (res < tol) st->done = 1;
   }
+}
+__global__ void __launch_bounds__(256) k_resid_reduce(const double* __restrict__ partials, int count, double tol,
+                                                      int iter, StepDev* st) {
+  resid_reduce_block(partials, count, tol, iter, st);
 }
 
 // Prediction, state.rs:145-169 and pd_solver.rs:274-283: prev = x;
@@ -1819,10 +1825,24 @@ __global__ void __launch_bounds__(256) k_predict(int n, double* __restrict__ x, 
 }
 
 // Finalize, pd_solver.rs:447-466 and state.rs:195-240.
+// With rs.st set, one block more than the vertices need forms the last
+// iteration's residual (resid_reduce_block), so the step needs no separate
+// launch for it.
+struct ResidArgs {
+  const double* partials;
+  int count;
+  double tol;
+  int iter;
+  StepDev* st;
+};
 __global__ void __launch_bounds__(256) k_finalize(int n, const double* __restrict__ x, double* __restrict__ prev,
                                                   double* __restrict__ v, const uint8_t* __restrict__ pinned,
                                                   double inv_dt, int has_ground, double ground, double fdamp,
-                                                  double frayl) {
+                                                  double frayl, ResidArgs rs) {
+  if (rs.st && (int)blockIdx.x == (int)gridDim.x - 1) {
+    resid_reduce_block(rs.partials, rs.count, rs.tol, rs.iter, rs.st);
+    return;
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double vel[3];
