@@ -2,8 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY. This file is the oracle: it is compiled into
 // oracle/_build/libvh_oracle.so and is loaded by tests/, by the smoke check
-// and by the CPU-baseline leg of bench.py, always as the checker or the
-// CPU baseline, never as the product path. The product (vistio_amd/) never
+// and by the CPU-baseline leg of bench.py, always as the reference the GPU
+// results are compared with, or as the CPU baseline, never as the product path. The product (vistio_amd/) never
 // links, loads or calls it.
 //
 // It follows the reference algorithm line by line (citations are

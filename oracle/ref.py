@@ -1,7 +1,8 @@
 """ctypes front end of the CPU oracle (oracle/cpu_ref.cpp).
 
 TEST INFRASTRUCTURE ONLY. Imported by tests/, ``__graft_entry__.smoke()`` and
-the ``cpu_baseline`` leg of bench.py, as the checker or the CPU baseline.
+the ``cpu_baseline`` leg of bench.py, as the reference the GPU results are
+compared with, or as the CPU baseline.
 The product package (vistio_amd) never imports this module.
 """
 from __future__ import annotations

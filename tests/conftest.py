@@ -19,7 +19,7 @@ DT = 1.0 / 60.0
 
 
 def sheet(c, rotate_xz=False):
-    """quad_grid(c, c, 1, 1) with row 0 pinned (BASELINE configs 0/1); rotate_xz
+    """quad_grid(c, c, 1, 1) with row 0 pinned (the hanging sheets); rotate_xz
     gives the reference's Scenario::hanging_sheet pose (scenarios.rs:91-96)."""
     from oracle import mesh
     px, py, pz, idx = mesh.quad_grid(c, c, 1.0, 1.0)

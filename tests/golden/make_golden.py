@@ -1,15 +1,15 @@
 #!/usr/bin/env python3
 """Writes the golden fixtures of this directory from the oracle (oracle/).
 
-Fixtures (all BASELINE config 0 scale, small, numpy .npz):
+Fixtures (all at the scale of the 20 x 20 sheet, small, numpy .npz):
 
-  cfg0_steps.npz     the 20 x 20 hanging sheet (z = 0 curtain) and the
+  sheet20_steps.npz  the 20 x 20 hanging sheet (z = 0 curtain) and the
                      reference's rotated pose (XZ plane at y = 1): inputs
                      (positions, triangles, hinges, pins) and the positions
                      after steps 1, 10 and 120 (rotated: 1, 10, 60) in FP64
                      and in the reference's precision split (f32 state, f64
                      solve).
-  cfg0_local.npz     the local step at one perturbed state of the rotated
+  sheet20_local.npz  the local step at one perturbed state of the rotated
                      sheet (seed 7): per-triangle and per-hinge targets, the
                      RHS and the residual rhs - A x.
   folded_pairs.npz   the candidate triangle pairs of a folded 8 x 8 patch
@@ -67,7 +67,7 @@ def steps_fixture():
                 o.step(DT, m - done)
                 done = m
                 out[f"{name}_{tag}_x{m}"] = o.positions()
-    np.savez_compressed(os.path.join(HERE, "cfg0_steps.npz"), **out)
+    np.savez_compressed(os.path.join(HERE, "sheet20_steps.npz"), **out)
 
 
 def perturbed_state(px, py, pz, seed, amp=2e-3):
@@ -88,7 +88,7 @@ def local_fixture():
     rhs, tt, ht = o.local_rhs()
     A = o.matrix()
     resid = np.stack([rhs[k] - A @ x[k] for k in range(3)])
-    np.savez_compressed(os.path.join(HERE, "cfg0_local.npz"), x=x, pred=pred, tri_targets=tt,
+    np.savez_compressed(os.path.join(HERE, "sheet20_local.npz"), x=x, pred=pred, tri_targets=tt,
                         hinge_targets=ht, rhs=rhs, residual=resid)
 
 

@@ -25,7 +25,7 @@ def _case(d, name):
 
 def test_oracle_reproduces_golden_steps():
     from oracle import ref
-    d = _load("cfg0_steps.npz")
+    d = _load("sheet20_steps.npz")
     for name in ("curtain", "rotated"):
         px, py, pz, idx, hg, pin = _case(d, name)
         for tag, mode in (("f64", ref.F64), ("f32ref", ref.F32_REF)):
@@ -38,7 +38,7 @@ def test_oracle_reproduces_golden_steps():
 
 
 def test_golden_steps_invariants():
-    d = _load("cfg0_steps.npz")
+    d = _load("sheet20_steps.npz")
     for name in ("curtain", "rotated"):
         px, py, pz, idx, hg, pin = _case(d, name)
         rest = np.stack([px, py, pz]).astype(np.float64)
@@ -56,8 +56,8 @@ def test_golden_steps_invariants():
 
 def test_oracle_reproduces_golden_local_step():
     from oracle import ref
-    d = _load("cfg0_local.npz")
-    s = _load("cfg0_steps.npz")
+    d = _load("sheet20_local.npz")
+    s = _load("sheet20_steps.npz")
     px, py, pz, idx, hg, pin = _case(s, "rotated")
     o = ref.Oracle(ref.F64, px, py, pz, idx, hg, pin, ref.params())
     o.set(0, d["x"])
@@ -102,7 +102,7 @@ def _gpu(px, py, pz, idx, hg, pin, precision):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-9), ("f32_ir", 1e-6)])
 def test_gpu_matches_golden_steps(precision, tol):
-    d = _load("cfg0_steps.npz")
+    d = _load("sheet20_steps.npz")
     for name in ("curtain", "rotated"):
         px, py, pz, idx, hg, pin = _case(d, name)
         g = _gpu(px, py, pz, idx, hg, pin, precision)
@@ -119,8 +119,8 @@ def test_gpu_matches_golden_steps(precision, tol):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,tol", [("f64", 1e-11), ("f32_ir", 2e-6)])
 def test_gpu_matches_golden_residual(precision, tol):
-    d = _load("cfg0_local.npz")
-    s = _load("cfg0_steps.npz")
+    d = _load("sheet20_local.npz")
+    s = _load("sheet20_steps.npz")
     px, py, pz, idx, hg, pin = _case(s, "rotated")
     g = _gpu(px, py, pz, idx, hg, pin, precision)
     g.set_state(0, d["x"])

@@ -43,7 +43,7 @@ def _solver(mesh, topo, pinned, precision, ir_sweeps=1, ir_final_sweeps=1, **cfg
 
 
 def test_headline_256_matches_fp64_oracle(all_threads):
-    """BASELINE configs[1] exactly as bench.py runs it, 120 steps from rest."""
+    """The 256 x 256 hanging sheet exactly as bench.py runs it, 120 steps from rest."""
     from oracle import ref
     from vistio_amd.scenes import hanging_sheet
     mesh, topo, pinned = hanging_sheet(256)
@@ -92,13 +92,13 @@ def _batch_sheet_oracle(b, s, mode):
 
 @pytest.mark.parametrize("precision,steps,tol", [("f32_ir", 120, 1e-6), ("f64", 10, 1e-9)])
 def test_batched_64_sheets_follow_their_oracles(precision, steps, tol, all_threads):
-    """64 sheets (SplitMix64 seeds 0x5EED ^ id, grids
+    """The batched-sheets workload: 64 sheets (SplitMix64 seeds 0x5EED ^ id, grids
     32..256, per-sheet density and stiffness) in one system and one graph;
     the smallest, median and largest sheet each follow their own oracle."""
     import vistio_amd as va
     from oracle import ref
-    from vistio_amd.batch import build_batch, cfg3_specs
-    specs = cfg3_specs(64)
+    from vistio_amd.batch import build_batch, batched_sheet_specs
+    specs = batched_sheet_specs(64)
     b = build_batch(specs)
     cells = np.array([sp.cells for sp in specs])
     order = np.argsort(cells, kind="stable")
@@ -162,8 +162,8 @@ def _run_contact(name, precision, check_every=1):
 
 @pytest.mark.parametrize("name,check_every", [("sphere_drape", 1), ("self_fold_ribbon", 1)])
 def test_contact_workload_full_size(name, check_every):
-    """BASELINE configs[2] (512 x 512 sphere drape, 180 steps) and configs[4]
-    (64 x 640 self-fold ribbon, 300 steps) in FP32-IR, checked at every
+    """The 512 x 512 sphere drape (180 steps) and the 64 x 640 self-fold
+    ribbon (300 steps) in FP32-IR, checked at every
     step; then the FP64 run, and the gap between the two. The predict step
     clamps only to the ground (state.rs:160-165), so a vertex next to the
     sphere can start a step a little inside it, and the AL loop then pushes

@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/cpu_ref.cpp) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/cpu_ref.cpp) before the GPU results are compared with it.
 
 Each group names what pins it:
   * the reference's own known-answer tests (file:line in the Vistio tree);
@@ -178,7 +178,7 @@ def test_matrix_invariants(c):
     # membrane gradients and the bending stencil sum to zero: A 1 = M / h^2 1
     assert np.allclose(A @ one, o.masses() / DT**2, rtol=1e-9, atol=1e-9)
     if c == 20:
-        assert A.nnz == 4517  # nnz(A) for configs[0], matches the reference's triplet pattern
+        assert A.nnz == 4517  # nnz(A) of the 20 x 20 sheet, matches the reference's triplet pattern
 
 
 @pytest.mark.parametrize("c", [4, 20, 64])
@@ -321,7 +321,7 @@ def test_predict_closed_form():  # solver_tests.rs:39-63 (pred = pos + dt^2 g at
 
 
 def test_f32_reference_mode_tracks_f64():
-    """The reference's own precision split stays near the f64 re-evaluation on configs[0]."""
+    """The reference's own precision split stays near the f64 re-evaluation on the 20 x 20 sheet."""
     px, py, pz, idx, hg, pin = sheet(20)
     a = oracle_solver(px, py, pz, idx, hg, pin, mode=F64)
     b = oracle_solver(px, py, pz, idx, hg, pin, mode=F32)

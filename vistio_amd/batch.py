@@ -9,7 +9,7 @@ per-element arrays of vh_pd_init_ex; each is what one
 ProjectiveDynamicsSolver::init_with_material (pd_solver.rs:106-143) would
 simulate on its own.
 
-`cfg3_specs` draws the batched-sheets workload: 64 pinned sheets with grids
+`batched_sheet_specs` draws the batched-sheets workload: 64 pinned sheets with grids
 uniform in 32..256 cells a side, densities uniform in [0.08, 0.40] kg/m^2,
 membrane weights log-uniform in [3e2, 3e3] N/m and bending weights
 log-uniform in [1e-5, 1e-3] N m, from SplitMix64 streams seeded
@@ -47,7 +47,7 @@ class SheetSpec:
     k_bending: float    # N m (hinge weight = k * edge length)
 
 
-def cfg3_specs(count: int = 64, seed: int = 0x5EED, min_cells: int = 32, max_cells: int = 256) -> List[SheetSpec]:
+def batched_sheet_specs(count: int = 64, seed: int = 0x5EED, min_cells: int = 32, max_cells: int = 256) -> List[SheetSpec]:
     specs = []
     for sid in range(count):
         st = (seed ^ sid) & _MASK
