@@ -7,9 +7,10 @@
  * returns a vh_status and vh_last_error() gives the message.
  *
  * Reference interfaces each entry point replaces (file:line in the Vistio
- * tree; the Rust-side binding is shown in INTEGRATION.md):
+ * tree; the Rust-side binding is the crate vistio_amd/rust/vistio-hip):
  *   SolverStrategy            crates/vistio-solver/src/strategy.rs:42-69
  *   ProjectiveDynamicsSolver  crates/vistio-solver/src/pd_solver.rs:53-476
+ *   BroadPhase, NarrowPhase   crates/vistio-contact/src/broad.rs:39-56, narrow.rs:21-35
  *   GpuBackend                crates/vistio-gpu/src/backend.rs:19-46
  *   VistioError               crates/vistio-types/src/error.rs:9-44
  */
