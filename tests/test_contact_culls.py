@@ -16,52 +16,69 @@ def _round_up(v):
 
 
 def _tri_record(P, D=None):
-    """(centre as float32, radius rounded up, max displacement rounded up)."""
+    """(centre as float32, radius rounded up, mean displacement as float32,
+    largest distance of a corner's displacement from it rounded up)."""
     c = np.array([F32(v) for v in P.mean(0)], dtype=np.float64)
     r = _round_up(np.sqrt(max(((p - c) ** 2).sum() for p in P)))
-    m = 0.0 if D is None else _round_up(np.sqrt(max((d ** 2).sum() for d in D)))
-    return c, r, m
+    if D is None:
+        return c, r, np.zeros(3), 0.0
+    g = np.array([F32(v) for v in D.mean(0)], dtype=np.float64)
+    s = _round_up(np.sqrt(max(((d - g) ** 2).sum() for d in D)))
+    return c, r, g, s
 
 
-def _ccd_culled(A0, DA, B0, DB, half_eps=0.5e-8):
-    ca, ra, ma = _tri_record(A0, DA)
-    cb, rb, mb = _tri_record(B0, DB)
+def _ccd_culled(A0, DA, B0, DB, half_eps=0.5e-8, ee=False):
+    ca, ra, ga, sa = _tri_record(A0, DA)
+    cb, rb, gb, sb = _tri_record(B0, DB)
     D = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 + 1e-12) + ra + rb
-    W = ma + mb
+    W = np.sqrt(((ga - gb) ** 2).sum()) * (1.0 + 1e-12) + sa + sb
+    if ee:
+        W = max(W, 2.0 * max(sa, sb))
     return W ** 3 < half_eps and 3 * D * W * W < half_eps and 3 * D * D * W < half_eps
 
 
-def _ccd_hits(A0, A1, B0, B1):
+def _ccd_hits(A0, A1, B0, B1, ee=False):
     hits = []
     for P0, P1, T0, T1 in ((A0, A1, B0, B1), (B0, B1, A0, A1)):
         for q in range(3):
             t = ref.vt_ccd(P0[q], P1[q], T0[0], T1[0], T0[1], T1[1], T0[2], T1[2])
             if t is not None:
                 hits.append(t)
+    if ee:
+        for i in range(3):
+            for j in range(3):
+                i1, j1 = (i + 1) % 3, (j + 1) % 3
+                t = ref.ee_ccd(A0[i], A1[i], A0[i1], A1[i1], B0[j], B1[j], B0[j1], B1[j1])
+                if t is not None:
+                    hits.append(t)
     return hits
 
 
-def _random_pair(rng, scale, motion):
+def _random_pair(rng, scale, motion, common):
+    """Two triangles near each other; each corner moves by a shared
+    translation of size common plus its own motion of size motion."""
     A0 = rng.normal(size=(3, 3)) * scale
     B0 = A0.mean(0) + rng.normal(size=(3, 3)) * scale * rng.uniform(0.5, 3.0)
-    DA = rng.normal(size=(3, 3)) * motion
-    DB = rng.normal(size=(3, 3)) * motion
+    T = rng.normal(size=3) * common
+    DA = T + rng.normal(size=(3, 3)) * motion
+    DB = T + rng.normal(size=(3, 3)) * motion
     return A0, DA, B0, DB
 
 
-def _ccd_trials(half_eps):
+def _ccd_trials(half_eps, ee=False):
     rng = np.random.default_rng(11)
     culled = culled_with_hit = 0
     hits_seen = 0
     for scale in (3e-4, 1e-3, 3e-3, 1e-2, 3e-2):
         for motion in (1e-7, 1e-6, 1e-5, 1e-4, 1e-3):
-            for _ in range(160):
-                A0, DA, B0, DB = _random_pair(rng, scale, motion)
-                hits = _ccd_hits(A0, A0 + DA, B0, B0 + DB)
-                hits_seen += bool(hits)
-                if _ccd_culled(A0, DA, B0, DB, half_eps):
-                    culled += 1
-                    culled_with_hit += bool(hits)
+            for common in (0.0, 1e-2):
+                for _ in range(80):
+                    A0, DA, B0, DB = _random_pair(rng, scale, motion, common)
+                    hits = _ccd_hits(A0, A0 + DA, B0, B0 + DB, ee)
+                    hits_seen += bool(hits)
+                    if _ccd_culled(A0, DA, B0, DB, half_eps, ee):
+                        culled += 1
+                        culled_with_hit += bool(hits)
     return culled, culled_with_hit, hits_seen
 
 
@@ -72,6 +89,25 @@ def test_ccd_pair_cull_never_drops_a_hit():
     assert bad == 0
 
 
+def test_ccd_pair_cull_with_edge_edge_never_drops_a_hit():
+    culled, bad, hits = _ccd_trials(0.5e-8, ee=True)
+    print(f"ccd cull (vt + ee): {culled} of 4000 pairs culled, {hits} pairs with a hit", flush=True)
+    assert culled > 1000 and hits > 50
+    assert bad == 0
+
+
+def test_ccd_cull_uses_the_relative_motion():
+    """A pair that translates as a piece by 1 cm, with 1e-7 m of motion of
+    its own, is culled: its cubic's leading coefficients vanish, whatever the
+    shared translation."""
+    rng = np.random.default_rng(5)
+    A0, DA, B0, DB = _random_pair(rng, 1e-3, 1e-7, 1e-2)
+    assert _ccd_culled(A0, DA, B0, DB)
+    assert not _ccd_hits(A0, A0 + DA, B0, B0 + DB, ee=True)
+    # the same pair with 1 cm of relative motion is kept
+    assert not _ccd_culled(A0, DA, B0, DB + np.array([1e-2, 0.0, 0.0]))
+
+
 def test_ccd_cull_check_catches_a_loose_bound():
     """At 50x the kernel's threshold the bound admits pairs whose cubic has
     roots the solver reports, and the check above flags them."""
@@ -80,8 +116,8 @@ def test_ccd_cull_check_catches_a_loose_bound():
 
 
 def _sphere_culled(A, B, dh, rel=1e-6):
-    ca, ra, _ = _tri_record(A)
-    cb, rb, _ = _tri_record(B)
+    ca, ra, _, _ = _tri_record(A)
+    cb, rb, _, _ = _tri_record(B)
     gap = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 - 1e-12) - ra - rb
     return gap > 0 and gap * gap > dh * (1.0 + rel)
 
@@ -105,8 +141,8 @@ def _sphere_trials(shrink):
             d2 = _min_dist2(A, B)
             near += d2 < dh
             # shrink < 1 makes the spheres too small: a planted mistake
-            ca, ra, _ = _tri_record(A)
-            cb, rb, _ = _tri_record(B)
+            ca, ra, _, _ = _tri_record(A)
+            cb, rb, _, _ = _tri_record(B)
             gap = np.sqrt(((ca - cb) ** 2).sum()) * (1.0 - 1e-12) - shrink * (ra + rb)
             if gap > 0 and gap * gap > dh * (1.0 + 1e-6):
                 culled += 1

@@ -809,19 +809,22 @@ struct Scene {
 };
 
 // Per-triangle bounds for the pair cull of k_ccd_pairs: a float centre c, the
-// largest distance r of a corner from c and the largest corner displacement
-// m, both rounded up. For a pair (A, B) every relative position in vt_ccd
-// (a corner of one triangle minus a corner of the other) is then at most
-// D = |c_A - c_B| + r_A + r_B long, and every relative motion at most
-// W = m_A + m_B.
+// largest distance r of a corner from c, a float reference displacement g
+// (the corners' mean displacement) and the largest distance s of a corner's
+// displacement from g, r and s rounded up. For a pair (A, B) every relative
+// position in vt_ccd (a corner of one triangle minus a corner of the other)
+// is then at most D = |c_A - c_B| + r_A + r_B long, and every relative motion
+// (the difference of two corners' displacements) at most
+// s_A + s_B + |g_A - g_B|: the cubic's leading coefficients depend on the
+// relative motion only, which is small where the cloth moves as a piece.
 struct CcdTri {
-  float c[3], r, m, pad[3];
+  float c[3], r, g[3], s;
 };
 
 __device__ __forceinline__ float round_up(double v) { return (float)(v * (1.0 + 1e-6)) + 1e-30f; }
 
-// Bounding sphere of each triangle at x (CcdTri with m = 0), for the pair
-// cull of the self-contact tests.
+// Bounding sphere of each triangle at x (CcdTri with g = 0, s = 0), for the
+// pair cull of the self-contact tests.
 __global__ void __launch_bounds__(256) k_tri_spheres(int t, const int* __restrict__ tri, const double* __restrict__ x,
                                                      CcdTri* __restrict__ out, const int* skip) {
   if (skip && *skip) return;
@@ -834,8 +837,8 @@ __global__ void __launch_bounds__(256) k_tri_spheres(int t, const int* __restric
   o.c[2] = (float)((a.z + b.z + c.z) / 3.0);
   const D3 cc = d3(o.c[0], o.c[1], o.c[2]);
   o.r = round_up(sqrt(fmax(len2(a - cc), fmax(len2(b - cc), len2(c - cc)))));
-  o.m = 0.0f;
-  o.pad[0] = o.pad[1] = o.pad[2] = 0.0f;
+  o.g[0] = o.g[1] = o.g[2] = 0.0f;
+  o.s = 0.0f;
   out[e] = o;
 }
 
@@ -1427,12 +1430,16 @@ __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __rest
               cz = (float)((a.z + b.z + c.z) / 3.0);
   const D3 cc = d3(cx, cy, cz);
   const double r2 = fmax(len2(a - cc), fmax(len2(b - cc), len2(c - cc)));
-  const double m2 = fmax(len2(ld(dl, tri[3 * e])), fmax(len2(ld(dl, tri[3 * e + 1])), len2(ld(dl, tri[3 * e + 2]))));
+  const D3 da = ld(dl, tri[3 * e]), db = ld(dl, tri[3 * e + 1]), dc = ld(dl, tri[3 * e + 2]);
+  const float gx = (float)((da.x + db.x + dc.x) / 3.0), gy = (float)((da.y + db.y + dc.y) / 3.0),
+              gz = (float)((da.z + db.z + dc.z) / 3.0);
+  const D3 g = d3(gx, gy, gz);
+  const double s2 = fmax(len2(da - g), fmax(len2(db - g), len2(dc - g)));
   CcdTri o;
   o.c[0] = cx; o.c[1] = cy; o.c[2] = cz;
   o.r = round_up(sqrt(r2));
-  o.m = round_up(sqrt(m2));
-  o.pad[0] = o.pad[1] = o.pad[2] = 0.0f;
+  o.g[0] = gx; o.g[1] = gy; o.g[2] = gz;
+  o.s = round_up(sqrt(s2));
   out[e] = o;
 }
 
@@ -1441,14 +1448,17 @@ __global__ void __launch_bounds__(256) k_ccd_tri_bounds(int t, const int* __rest
 // (ccd.rs:138-229) on the nine edge pairs of each triangle pair.
 //
 // Pair cull (exact): the cubic of a vertex-triangle test has coefficients
-// |ca| <= W^3, |cb| <= 3 D W^2, |cc| <= 3 D^2 W (D, W from CcdTri). An
-// edge-edge cubic has the same bounds with W = 2 max(m_A, m_B), since both
-// ends of one edge move (|db - da| <= 2 m_A); the cull then uses that W,
-// which also bounds the vertex-triangle tests (m_A + m_B <= it). When all
-// three are below half of the solver's 1e-8 threshold, cubic_roots falls
-// through quadratic_roots to the linear case and finds no root for any test
-// of the pair (the computed coefficients differ from the bounded ones by a
-// few ulps), so the pair cannot report a hit and is skipped.
+// |ca| <= W^3, |cb| <= 3 D W^2, |cc| <= 3 D^2 W, with D and the relative
+// motion bound W = s_A + s_B + |g_A - g_B| from CcdTri. An edge-edge cubic
+// has the same bounds with W = max(2 s_A, 2 s_B, s_A + s_B + |g_A - g_B|),
+// since it also holds the motion of one edge's end relative to its other end
+// (|db - da| <= 2 s_A); the cull then uses that W, which also bounds the
+// vertex-triangle tests. When all three are below half of the solver's 1e-8
+// threshold, cubic_roots falls through quadratic_roots to the linear case and
+// finds no root for any test of the pair (the computed coefficients differ
+// from the bounded ones by a few ulps, and the relative motions the tests
+// form from x and x + delta by a few ulps of the coordinates), so the pair
+// cannot report a hit and is skipped.
 //
 // k_ccd_cull applies the cull; block b writes the survivors of its share of
 // the pairs (pair_range) to its own region surv[b reg, (b + 1) reg) and their
@@ -1470,7 +1480,10 @@ __global__ void __launch_bounds__(256) k_ccd_cull(const int2* __restrict__ pairs
     const double gx = (double)A.c[0] - (double)B.c[0], gy = (double)A.c[1] - (double)B.c[1],
                  gz = (double)A.c[2] - (double)B.c[2];
     const double D = sqrt(gx * gx + gy * gy + gz * gz) * (1.0 + 1e-12) + (double)A.r + (double)B.r;
-    const double W = ee ? 2.0 * fmax((double)A.m, (double)B.m) : (double)A.m + (double)B.m;
+    const double hx = (double)A.g[0] - (double)B.g[0], hy = (double)A.g[1] - (double)B.g[1],
+                 hz = (double)A.g[2] - (double)B.g[2];
+    const double Wv = sqrt(hx * hx + hy * hy + hz * hz) * (1.0 + 1e-12) + (double)A.s + (double)B.s;
+    const double W = ee ? fmax(Wv, 2.0 * fmax((double)A.s, (double)B.s)) : Wv;
     constexpr double kHalfEps = 0.5e-8;
     return !(W * W * W < kHalfEps && 3.0 * D * W * W < kHalfEps && 3.0 * D * D * W < kHalfEps);
   }, surv + (size_t)blockIdx.x * reg);
