@@ -1693,7 +1693,10 @@ __global__ void __launch_bounds__(256) k_root_band(BandArgs<T> a, CommitArgs c, 
 // q0.y w, q2.x first front slot, q2.y slots per position, q2.z offset of
 // the inverse in Sroot). The front and the products are taken eight
 // entries at a time (scheduling barriers between the groups), which keeps
-// the LDS reads of a group, not of the whole row, in registers.
+// the LDS reads of a group, not of the whole row, in registers. Prefetching
+// a third position per thread (a 2295-wide front in one round of loads,
+// with two or three slots per position instead of four) measured slower at
+// 256x256: 12.4 -> 13.9 and 14.7 us per launch.
 template <int NL, class T>
 __global__ void __launch_bounds__(1024) k_root_dense(const TaskDev* __restrict__ tasks, BandArgs<T> a,
                                                      const T* __restrict__ Sroot_all, const T* __restrict__ F,
