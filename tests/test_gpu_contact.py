@@ -139,6 +139,11 @@ def test_ccd_step_matches_oracle():
     d = np.zeros_like(p0); d[1] = -0.05; moves.append(d)                       # everything through the ground
     d = np.zeros_like(p0); d[1, p0[1] > 0.04] = -0.03; moves.append(d)         # upper layer through the lower
     moves.append(rng.normal(size=p0.shape) * 0.01)
+    # the sheet as a piece, 22 cm sideways with 1e-7 m of motion of its own:
+    # the pair cull bounds the relative motion, so every self pair is culled
+    d = np.array([[0.2], [0.0], [0.1]]) + rng.normal(size=p0.shape) * 1e-7; moves.append(d)
+    # both layers 5 cm sideways, the upper one also down through the lower
+    d = np.zeros_like(p0); d[0] = 0.05; d[1, p0[1] > 0.04] = -0.03; moves.append(d)
     for case in ("self", "ground", "sphere", "all"):
         kw = dict(self_collision=case in ("self", "all"), ground=0.0 if case in ("ground", "all") else None,
                   sphere=((0.0, -0.25, 0.25), 0.26) if case in ("sphere", "all") else None)
