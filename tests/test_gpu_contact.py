@@ -76,8 +76,12 @@ def _folded_sheet(c=8, gap=0.02, seed=0):
 def test_bvh_pairs_match_all_pairs():
     from oracle import ref
     # 64 and 100 cells: 8192 and 20000 triangles, several tiles of the
-    # Morton sort (2048 keys per tile) and of the pair-offset scan
-    for c, margin, seed in ((12, 0.01, 0), (16, 0.05, 1), (10, 0.0, 2), (64, 0.004, 3), (100, 0.002, 4)):
+    # Morton sort (2048 keys per tile) and of the pair-offset scan. 24 cells
+    # at margin 0.1: about 95 pairs per triangle, so some leaves have more
+    # pairs than the single walk keeps (64) and are walked again, in the same
+    # detection as leaves whose kept pairs are copied.
+    for c, margin, seed in ((12, 0.01, 0), (16, 0.05, 1), (10, 0.0, 2), (64, 0.004, 3), (100, 0.002, 4),
+                            (24, 0.1, 6)):
         px, py, pz, idx, hg = _mesh(c, c, 1.0, 1.0)
         rng = np.random.default_rng(seed)
         pz = (rng.normal(size=px.size) * 0.05).astype(np.float32)

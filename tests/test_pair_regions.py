@@ -1,4 +1,5 @@
-"""The survivor regions of the pair culls (k_self_cull, k_ccd_cull).
+"""The survivor regions of the pair culls (k_self_cull, k_ccd_cull), and the
+broad phase's single walk (k_bvh_walk1, k_bvh_compact, k_bvh_more).
 
 Block b of a cull writes the surviving pairs of its share to its own region
 of ccd_region(cap) entries, with no bounds check in the kernel. This replays
@@ -61,3 +62,82 @@ def test_a_short_region_is_caught():
     rounds = block_rounds(cap, pair_blocks(cap))
     short = ccd_region(cap) - _BLOCK
     assert max(len(r) for r in rounds) * _BLOCK > short
+
+
+# ---- the broad phase's single walk (k_bvh_walk1, k_bvh_compact, k_bvh_more) ----
+
+_STAGE, _KEEP, _MORE = 4096, 16, 48  # kPairStage, kLeafStage, the host's wmore_leaf
+
+
+def _single_walk_list(cnt, c2=_MORE, more_size=None):
+    """Replays the index arithmetic of the three kernels for per-leaf pair
+    counts cnt (pair c of leaf k encoded as k * 4096 + c) and returns the
+    pair list they leave, asserting every index in bounds. more_size: the
+    entries of the second part (the host allocates t * c2)."""
+    t = len(cnt)
+    nch = (t + _BLOCK - 1) // _BLOCK
+    per = (nch + _X - 1) // _X
+    grid = (nch + 7) // 8 * 8
+    kept = np.full(nch * _STAGE, -1, np.int64)
+    more = np.full(t * c2 if more_size is None else more_size, -1, np.int64)
+    walk_more = False
+    for b in range(grid):  # k_bvh_walk1, leaf_chunk
+        ch = (b % _X) * per + b // _X
+        if ch >= nch:
+            continue
+        stage = np.full(_STAGE, -2, np.int64)
+        for tid in range(_BLOCK):
+            k = ch * _BLOCK + tid
+            if k >= t:
+                continue
+            for c in range(cnt[k]):
+                if c < _KEEP:
+                    assert tid * _KEEP + c < _STAGE
+                    stage[tid * _KEEP + c] = k * 4096 + c
+                elif c - _KEEP < c2:
+                    assert k * c2 + c - _KEEP < more.size
+                    more[k * c2 + c - _KEEP] = k * 4096 + c
+            walk_more |= cnt[k] > _KEEP + c2
+        assert (ch + 1) * _STAGE <= kept.size
+        kept[ch * _STAGE:(ch + 1) * _STAGE] = stage
+    off = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
+    pairs = np.full(int(np.sum(cnt)), -3, np.int64)
+    for b in range(nch):  # k_bvh_compact
+        k0, kn = b * _BLOCK, min(_BLOCK, t - b * _BLOCK)
+        so = list(off[k0:k0 + kn]) + [off[k0 + kn - 1] + cnt[k0 + kn - 1]]
+        for o in range(so[0], so[kn]):
+            lo, hi = 0, kn - 1
+            while lo < hi:
+                mid = (lo + hi + 1) >> 1
+                if so[mid] <= o:
+                    lo = mid
+                else:
+                    hi = mid - 1
+            if so[lo + 1] - so[lo] > _KEEP + c2:
+                continue
+            j, k = o - so[lo], k0 + lo
+            assert j < cnt[k]
+            pairs[o] = kept[k * _KEEP + j] if j < _KEEP else more[k * c2 + j - _KEEP]
+    if walk_more:  # k_bvh_more: the leaves with more pairs, walked again
+        for k in np.nonzero(np.asarray(cnt) > _KEEP + c2)[0]:
+            pairs[off[k]:off[k] + cnt[k]] = k * 4096 + np.arange(cnt[k])
+    return pairs, walk_more
+
+
+@pytest.mark.parametrize("t,mean", [(1, 3), (255, 25), (257, 25), (3000, 25), (2048, 60), (700, 0)])
+def test_single_walk_keeps_the_pair_list(t, mean):
+    rng = np.random.default_rng(t + mean)
+    cnt = rng.poisson(mean, size=t) if mean else np.zeros(t, np.int64)
+    cnt[::97] += 70  # a few leaves with more pairs than the walk keeps
+    pairs, _ = _single_walk_list(cnt)
+    want = np.concatenate([k * 4096 + np.arange(c) for k, c in enumerate(cnt)])
+    assert np.array_equal(pairs, want)
+
+
+def test_single_walk_check_catches_a_short_overflow_run():
+    """A second part one entry short of t * c2 (a planted mistake): the
+    replay's bounds check fails at the last leaf that fills its run."""
+    cnt = np.full(300, _KEEP + _MORE, np.int64)
+    _single_walk_list(cnt)
+    with pytest.raises(AssertionError):
+        _single_walk_list(cnt, more_size=300 * _MORE - 1)

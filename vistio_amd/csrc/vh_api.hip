@@ -51,6 +51,10 @@ struct IpcState {
   int* h_done = nullptr;   // pinned host copy of IpcDev::al_done
   unsigned int* h_np = nullptr;  // pinned host copies of BvhDev::npairs and ::ncont (same allocation)
   unsigned int *pcount = nullptr, *poff = nullptr;  // candidate pairs per leaf, their offsets
+  // the pairs k_bvh_walk1 keeps: kLeafStage per leaf (whole workgroup
+  // stages) and wmore_leaf per leaf after them
+  int2 *wkept = nullptr, *wmore = nullptr;
+  int wmore_leaf = 0;
   float* bpart = nullptr;                             // per-block centroid bounds of k_boxes
   unsigned int* sort_hist = nullptr;  // per-tile digit histograms of the Morton sort (vh_sort.hpp)
   unsigned int* sbuf = nullptr;       // tile sums of every scan (the scans run one after another)
@@ -949,11 +953,14 @@ void enqueue_pairs(vh_ctx* c, const double* x, float margin, const int* skip) {
   hipLaunchKernelGGL(vhc::k_node_bounds, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.st, I.coarse, nc, id_s, I.left,
                      I.range_lo, I.range_hi, I.node, leaf_of(c), I.orig, I.sorig, skip);
   const int walk_blocks = (nblk(t) + 7) / 8 * 8;  // see leaf_chunk
-  hipLaunchKernelGGL(vhc::k_bvh_count, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.pcount,
-                     skip);
+  hipLaunchKernelGGL(vhc::k_bvh_walk1, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.sorig,
+                     I.pcount, I.wkept, I.wmore, I.wmore_leaf, I.bd, skip);
   vhs::scan_exclusive(c->stream, I.pcount, I.poff, t, I.sbuf, I.ticket, skip);
-  hipLaunchKernelGGL(vhc::k_bvh_pairs, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.sorig, I.pcount,
-                     I.poff, I.pairs, I.cap, I.bd, skip);
+  // the kept pairs copied to the list; a leaf with more is walked again
+  hipLaunchKernelGGL(vhc::k_bvh_compact, dim3(nblk(t)), dim3(256), 0, c->stream, t, I.pcount, I.poff, I.wkept,
+                     I.wmore, I.wmore_leaf, I.pairs, I.cap, I.bd, skip);
+  hipLaunchKernelGGL(vhc::k_bvh_more, dim3(walk_blocks), dim3(256), 0, c->stream, t, I.node, leaf_of(c), I.sorig,
+                     I.pcount, I.poff, I.wmore_leaf, I.pairs, I.cap, I.bd, skip);
 }
 
 // CollisionPipeline::detect_ipc_contacts at positions x with scene sc.
@@ -2289,6 +2296,11 @@ vh_status vh_ipc_init(vh_ctx* ctx, const vh_ipc_params* params) {
   I.h_np = reinterpret_cast<unsigned int*>(I.h_done + 1);
   I.h_np[0] = I.h_np[1] = 0;
   UP(dalloc(ctx, &I.pcount, (size_t)tt)); UP(dalloc(ctx, &I.poff, (size_t)tt));
+  // k_bvh_walk1 keeps 16 + 48 pairs per leaf (a flat sheet has about 25,
+  // the drape's sheet where it wraps the sphere up to about 60)
+  I.wmore_leaf = 48;
+  UP(dalloc(ctx, &I.wkept, (size_t)nblk(tt) * vhc::kPairStage));
+  UP(dalloc(ctx, &I.wmore, (size_t)tt * I.wmore_leaf));
   UP(dalloc(ctx, &I.bpart, 6 * (size_t)nblk(tt)));
   UP(dalloc(ctx, &I.sort_hist, (size_t)vhs::sort_hist_words(tt)));
   UP(dalloc(ctx, &I.sbuf, (size_t)std::max(vhs::scan_scratch_words(tt), vhs::scan_scratch_words((long long)n + 1))));

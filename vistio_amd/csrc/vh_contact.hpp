@@ -322,6 +322,7 @@ struct BvhDev {
   unsigned int npairs;    // pairs found (may exceed the capacity)
   unsigned int ncont;     // self contacts of the last detection (may exceed the entry buffer)
   unsigned int crossings; // triangle pairs that cross (k_tri_crossings)
+  unsigned int walk_more; // a leaf has more pairs than k_bvh_walk1 keeps (k_bvh_more walks it again)
 };
 
 // bvh.rs:28-42: box of f32 positions, inflated by margin. Each block also
@@ -396,6 +397,7 @@ __global__ void k_bvh_reset(BvhDev* bd, const int* skip) {
   for (int a = 0; a < 3; ++a) { bd->scene[a] = f2o(FLT_MAX); bd->scene[3 + a] = f2o(-FLT_MAX); }
   bd->npairs = 0;
   bd->crossings = 0;
+  bd->walk_more = 0;
 }
 
 __device__ __forceinline__ unsigned int expand_bits(unsigned int v) {
@@ -615,10 +617,12 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
 }
 
 // Every leaf k collects the leaves j > k (Morton order) whose boxes overlap
-// its own: each unordered pair once. Two passes over the same walk: the
-// first counts the pairs of each leaf, an exclusive scan turns the counts
-// into offsets, the second writes each leaf's pairs at its offset (no
-// shared counter). A pair is stored as (a, b) with the caller's triangle
+// its own: each unordered pair once. One walk counts each leaf's pairs and
+// keeps them (k_bvh_walk1), an exclusive scan turns the counts into
+// offsets, and the kept pairs are copied there (k_bvh_compact); a leaf
+// with more pairs than the walk keeps is walked again and writes its pairs
+// at its offset (k_bvh_more). No shared counter, so the list is the same
+// every time. A pair is stored as (a, b) with the caller's triangle
 // index orig[a] < orig[b]: the reference tests vertices of its first
 // triangle against the second and edges of the first against the second
 // (self_collision.rs:199-276), and for parallel edges the closest points,
@@ -627,21 +631,19 @@ __global__ void __launch_bounds__(256) k_node_bounds(int t, const Box* __restric
 // when its box overlaps and its range reaches past k, and otherwise takes
 // its escape; every step moves forward in pre-order, so the walk ends at
 // kEnd.
-// Pairs of one workgroup of k_bvh_pairs staged in LDS before a coalesced
-// copy: the group's leaves are consecutive, so their pairs form one
-// contiguous run of the output. Pairs past the staging room go straight out.
-// 32 KB: 4096 pairs, 16 per leaf. On the flat 512x512 drape (about 25 pairs
-// per leaf) 16 KB left two thirds of the pairs to scattered 8-byte stores,
-// and the walk's node misses evicted their lines half written (265 MB written
-// for 103 MB of pairs); 32 KB (five workgroups per CU instead of eight) ran
-// the drape 2.4 % faster. 48 KB was slower again (emit walk 340 -> 357 us).
+// The LDS stage of a workgroup's walks, written out in whole lines: 32 KB,
+// 4096 pairs, 16 per leaf. (With a stage shared by the workgroup's leaves
+// and the pairs written at their final offsets, 16 KB left two thirds of
+// the flat 512x512 drape's pairs to scattered 8-byte stores whose lines the
+// walk's node misses evicted half written, 265 MB written for 103 MB of
+// pairs; 32 KB, five workgroups per CU instead of eight, ran the drape
+// 2.4 % faster; 48 KB was slower again, that walk 340 -> 357 us.)
 constexpr int kPairStage = 4096;
 
-template <bool kEmit>
+// The walk of leaf k; with kEmit, sink(c, pair) takes the leaf's c-th pair.
+template <bool kEmit, class Sink>
 __device__ __forceinline__ unsigned int bvh_walk(int k, int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
-                                                 const int* __restrict__ sorig, int2* __restrict__ pairs,
-                                                 unsigned int base, unsigned int cap, int2* stage,
-                                                 unsigned int stage_base) {
+                                                 const int* __restrict__ sorig, Sink&& sink) {
   const Rec me = leaf[k];
   const int ome = kEmit ? sorig[k] : 0;
   unsigned int cnt = 0;
@@ -659,12 +661,8 @@ __device__ __forceinline__ unsigned int bvh_walk(int k, int t, const Rec* __rest
     if (is_leaf) {
       if (j > k && ov) {
         if (kEmit) {
-          const unsigned int slot = base + cnt;
           const bool first = ome < oj;
-          const int2 pr = make_int2(first ? me.a : R.a, first ? R.a : me.a);
-          const unsigned int ls = slot - stage_base;
-          if (ls < (unsigned int)kPairStage) stage[ls] = pr;
-          else if (slot < cap) pairs[slot] = pr;
+          sink(cnt, make_int2(first ? me.a : R.a, first ? R.a : me.a));
         }
         ++cnt;
       }
@@ -691,39 +689,95 @@ __device__ __forceinline__ int leaf_chunk(int t) {
   return c < nch ? c : -1;
 }
 
-__global__ void __launch_bounds__(256) k_bvh_count(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
-                                                   unsigned int* __restrict__ count, const int* skip) {
-  if (skip && *skip) return;
-  const int ch = leaf_chunk(t);
-  if (ch < 0) return;
-  const int k = ch * blockDim.x + threadIdx.x;
-  if (k >= t) return;
-  count[k] = t < 2 ? 0u : bvh_walk<false>(k, t, node, leaf, nullptr, nullptr, 0, 0, nullptr, 0);
-}
-// offset: exclusive scan of count. The last leaf records the total.
-__global__ void __launch_bounds__(256) k_bvh_pairs(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
-                                                   const int* __restrict__ sorig,
-                                                   const unsigned int* __restrict__ count,
-                                                   const unsigned int* __restrict__ offset, int2* __restrict__ pairs,
-                                                   unsigned int cap, BvhDev* bd, const int* skip) {
+// One walk per leaf for counts and pairs alike. k_bvh_walk1 counts each
+// leaf's pairs (count) and keeps them: the first kLeafStage in the
+// workgroup's LDS stage, written out whole as the workgroup's run of kept
+// (leaf k's at kept[k kLeafStage ..]), the next c2 in leaf k's run of more
+// (more[k c2 ..]). A leaf with still more sets bd->walk_more. After the scan
+// of count, k_bvh_compact copies the kept runs to the pair list at the
+// leaves' offsets, and k_bvh_more walks the leaves with more pairs again,
+// writing theirs at their offsets. The list is the leaves' pairs in leaf
+// order, each leaf's in walk order. On the flat 512 x 512 drape (about 25
+// pairs per leaf) this replaces a count walk (about 250 us) by the copy.
+constexpr int kLeafStage = kPairStage / 256;  // 16 pairs per leaf in LDS
+
+__global__ void __launch_bounds__(256) k_bvh_walk1(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
+                                                   const int* __restrict__ sorig, unsigned int* __restrict__ count,
+                                                   int2* __restrict__ kept, int2* __restrict__ more, int c2,
+                                                   BvhDev* bd, const int* skip) {
   if (skip && *skip) return;  // uniform over the grid
   __shared__ int2 stage[kPairStage];
   const int ch = leaf_chunk(t);
   if (ch < 0) return;  // uniform over the block
-  const int k0 = ch * blockDim.x;
-  const int kl = min(k0 + (int)blockDim.x, t) - 1;  // the group's last leaf
-  const unsigned int gbase = offset[k0];
-  const unsigned int gend = offset[kl] + count[kl];
-  const int k = k0 + threadIdx.x;
+  const int k = ch * (int)blockDim.x + (int)threadIdx.x;
   if (k < t) {
-    const unsigned int base = offset[k];
-    if (k == t - 1) bd->npairs = base + count[k];
-    if (t >= 2 && count[k] != 0 && base < cap) bvh_walk<true>(k, t, node, leaf, sorig, pairs, base, cap, stage, gbase);
+    int2* mine = stage + threadIdx.x * kLeafStage;
+    int2* over = more + (size_t)k * c2;
+    const unsigned int n = t < 2 ? 0u : bvh_walk<true>(k, t, node, leaf, sorig, [&](unsigned int c, int2 pr) {
+      if (c < (unsigned int)kLeafStage) mine[c] = pr;
+      else if (c - kLeafStage < (unsigned int)c2) over[c - kLeafStage] = pr;
+    });
+    count[k] = n;
+    if (n > (unsigned int)(kLeafStage + c2)) bd->walk_more = 1u;
   }
   __syncthreads();
-  const unsigned int staged = min(gend - gbase, (unsigned int)kPairStage);
-  for (unsigned int q = threadIdx.x; q < staged; q += blockDim.x)
-    if (gbase + q < cap) pairs[gbase + q] = stage[q];
+  // the whole stage (entries past a leaf's count are never read)
+  int2* out = kept + (size_t)ch * kPairStage;
+  for (int q = threadIdx.x; q < kPairStage; q += blockDim.x) out[q] = stage[q];
+}
+
+// offset: exclusive scan of count. Block b copies the pairs of leaves
+// [256 b, 256 b + 256) to [offset[256 b], ...): each output slot finds its
+// leaf by a binary search over the block's offsets in LDS.
+__global__ void __launch_bounds__(256) k_bvh_compact(int t, const unsigned int* __restrict__ count,
+                                                     const unsigned int* __restrict__ offset,
+                                                     const int2* __restrict__ kept, const int2* __restrict__ more,
+                                                     int c2, int2* __restrict__ pairs, unsigned int cap, BvhDev* bd,
+                                                     const int* skip) {
+  if (skip && *skip) return;  // uniform over the grid
+  __shared__ unsigned int so[257];
+  const int k0 = (int)blockIdx.x * 256;
+  if (k0 >= t) return;
+  const int kn = min(256, t - k0);
+  if ((int)threadIdx.x < kn) so[threadIdx.x] = offset[k0 + threadIdx.x];
+  if (threadIdx.x == 0) {
+    so[kn] = offset[k0 + kn - 1] + count[k0 + kn - 1];
+    if (k0 + kn == t) bd->npairs = so[kn];
+  }
+  __syncthreads();
+  const unsigned int e = min(so[kn], cap);
+  for (unsigned int o = so[0] + threadIdx.x; o < e; o += blockDim.x) {
+    int lo = 0, hi = kn - 1;  // the last leaf whose run starts at or before o
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (so[mid] <= o) lo = mid;
+      else hi = mid - 1;
+    }
+    if (so[lo + 1] - so[lo] > (unsigned int)(kLeafStage + c2)) continue;  // k_bvh_more writes this leaf
+    const unsigned int j = o - so[lo];
+    const size_t k = (size_t)(k0 + lo);
+    pairs[o] = j < (unsigned int)kLeafStage ? kept[k * kLeafStage + j] : more[k * c2 + (j - kLeafStage)];
+  }
+}
+
+// The leaves with more pairs than k_bvh_walk1 keeps (kLeafStage + c2),
+// walked again; each writes its pairs at its offset.
+__global__ void __launch_bounds__(256) k_bvh_more(int t, const Rec* __restrict__ node, const Rec* __restrict__ leaf,
+                                                  const int* __restrict__ sorig, const unsigned int* __restrict__ count,
+                                                  const unsigned int* __restrict__ offset, int c2,
+                                                  int2* __restrict__ pairs, unsigned int cap, const BvhDev* bd,
+                                                  const int* skip) {
+  if ((skip && *skip) || !bd->walk_more) return;  // uniform over the grid
+  const int ch = leaf_chunk(t);
+  if (ch < 0) return;
+  const int k = ch * (int)blockDim.x + (int)threadIdx.x;
+  if (k >= t || count[k] <= (unsigned int)(kLeafStage + c2)) return;
+  const unsigned int base = offset[k];
+  if (base >= cap) return;
+  bvh_walk<true>(k, t, node, leaf, sorig, [&](unsigned int c, int2 pr) {
+    const unsigned int slot = base + c;
+    if (slot < cap) pairs[slot] = pr;
+  });
 }
 
 // Grid-stride loop over candidate pairs [0, np), one contiguous share per
